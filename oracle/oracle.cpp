@@ -1,0 +1,1569 @@
+// FP64 CPU oracle for the Blaze2D band-solve path.
+//
+// TEST INFRASTRUCTURE ONLY: the checker the HIP path is compared against.
+// Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+// load this library.  The product library (blaze2d_amd/csrc) never links it.
+//
+// A C++ restatement of the reference CPU path (Rust, crates/core and
+// crates/backend-cpu).  Each block below cites the reference lines it
+// follows; paths are relative to the reference repository root.
+//
+// Parity pins (what checks this restatement, see tests/test_oracle_*.py):
+//   FFT            numpy.fft.fft2 / ifft2, closed forms
+//   smoothing      web/public/data/epsilon-grids/*.json (reference export)
+//   Theta, mass    uniform-medium closed form, dense brute force vs numpy
+//   eigh, polar    numpy.linalg.eigh / svd
+//   Hungarian      brute force over permutations
+//   LOBPCG         diagonal operator closed form (_tests_eigensolver.rs)
+//   full path      web/public/data/benchmarks/series6-accuracy.json
+// Parity unpinned: bit-level faer/rustfft rounding, faer eigenvector gauge
+// inside degenerate clusters (only reachable through the full-path fixture).
+
+#include "oracle.h"
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <vector>
+
+namespace {
+
+using cd = std::complex<double>;
+using Field = std::vector<cd>;
+constexpr double PI = 3.14159265358979323846;
+
+// ---------------------------------------------------------------------------
+// Lattice / geometry (crates/core/src/lattice.rs:143-280, geometry.rs)
+// ---------------------------------------------------------------------------
+struct Lattice {
+    double a1[2], a2[2];
+    double det() const { return a1[0] * a2[1] - a1[1] * a2[0]; }
+    // lattice.rs:170-180
+    void reciprocal(double b1[2], double b2[2]) const {
+        double inv = 2.0 * PI / det();
+        b1[0] = a2[1] * inv; b1[1] = -a2[0] * inv;
+        b2[0] = -a1[1] * inv; b2[1] = a1[0] * inv;
+    }
+    // lattice.rs:182-187
+    void frac_to_cart(const double f[2], double c[2]) const {
+        c[0] = a1[0] * f[0] + a2[0] * f[1];
+        c[1] = a1[1] * f[0] + a2[1] * f[1];
+    }
+    double char_len() const { return std::sqrt(a1[0] * a1[0] + a1[1] * a1[1]); }
+};
+
+struct Atom { double pos[2]; double radius; double eps_inside; };
+
+struct Geometry {
+    Lattice lat;
+    double eps_bg;
+    std::vector<Atom> atoms;
+};
+
+// geometry.rs:103-119
+double wrap_unit(double v) { return v - std::floor(v); }
+double wrap_signed(double d) {
+    while (d >= 0.5) d -= 1.0;
+    while (d < -0.5) d += 1.0;
+    return d;
+}
+
+// geometry.rs:59-99 relative_permittivity_at_fractional
+double permittivity_at_fractional(const Geometry &g, const double frac_in[2]) {
+    double frac[2] = {wrap_unit(frac_in[0]), wrap_unit(frac_in[1])};
+    for (const Atom &a : g.atoms) {
+        double r = a.radius * g.lat.char_len();
+        double r_sq = r * r;
+        double dx = frac[0] - a.pos[0], dy = frac[1] - a.pos[1];
+        for (int n1 = -1; n1 <= 1; ++n1)
+            for (int n2 = -1; n2 <= 1; ++n2) {
+                double df[2] = {wrap_signed(dx) + n1, wrap_signed(dy) + n2};
+                double dc[2];
+                g.lat.frac_to_cart(df, dc);
+                if (dc[0] * dc[0] + dc[1] * dc[1] <= r_sq) return a.eps_inside;
+            }
+    }
+    return g.eps_bg;
+}
+
+struct Dielectric {
+    int nx = 0, ny = 0;
+    std::vector<double> eps, inv_eps;
+    std::vector<std::array<double, 4>> tensor;  // [xx, xy, yx, yy]; empty when unsmoothed
+    double b1[2], b2[2];
+};
+
+// analytic_geometry.rs: circle/pixel intersection ------------------------------
+struct PixelHit { double fill; bool has_normal; double normal[2]; };
+
+void circle_normal_at(const double c[2], const double p[2], double n[2]) {
+    double dx = p[0] - c[0], dy = p[1] - c[1];
+    double d = std::sqrt(dx * dx + dy * dy);
+    if (d < 1e-12) { n[0] = 1.0; n[1] = 0.0; }
+    else { n[0] = dx / d; n[1] = dy / d; }
+}
+
+// analytic_geometry.rs: circle_rectangle_intersection + integrate (16x16 midpoint)
+PixelHit circle_rect(const double cc[2], double radius, const double rc[2], const double rs[2]) {
+    PixelHit out{0.0, false, {0.0, 0.0}};
+    double hw = rs[0] / 2.0, hh = rs[1] / 2.0;
+    double rect_area = rs[0] * rs[1];
+    double x_min = rc[0] - hw, x_max = rc[0] + hw, y_min = rc[1] - hh, y_max = rc[1] + hh;
+    double corners[4][2] = {{x_min, y_min}, {x_max, y_min}, {x_min, y_max}, {x_max, y_max}};
+    double r2 = radius * radius;
+    int inside = 0;
+    double far2 = 0.0;
+    for (auto &c : corners) {
+        double ddx = c[0] - cc[0], ddy = c[1] - cc[1];
+        double d2 = ddx * ddx + ddy * ddy;
+        if (d2 <= r2) ++inside;
+        far2 = std::max(far2, d2);
+    }
+    if (inside == 4) { out.fill = 1.0; return out; }
+    double cx = std::min(std::max(cc[0], x_min), x_max);
+    double cy = std::min(std::max(cc[1], y_min), y_max);
+    double cdx = cx - cc[0], cdy = cy - cc[1];
+    if (cdx * cdx + cdy * cdy > r2) { out.fill = 0.0; return out; }
+    if (far2 <= r2) { out.fill = 1.0; return out; }
+    const int nsub = 16;
+    double sw = rs[0] / nsub, sh = rs[1] / nsub, sa = sw * sh;
+    double area = 0.0;
+    for (int iy = 0; iy < nsub; ++iy)
+        for (int ix = 0; ix < nsub; ++ix) {
+            double px = x_min + (ix + 0.5) * sw, py = y_min + (iy + 0.5) * sh;
+            double ddx = px - cc[0], ddy = py - cc[1];
+            if (ddx * ddx + ddy * ddy <= r2) area += sa;
+        }
+    double f = std::min(std::max(area / rect_area, 0.0), 1.0);
+    if (f < 1e-10) { out.fill = 0.0; return out; }
+    if (f > 1.0 - 1e-10) { out.fill = 1.0; return out; }
+    out.fill = f;
+    out.has_normal = true;
+    circle_normal_at(cc, rc, out.normal);  // compute_circle_interface_normal
+    return out;
+}
+
+// analytic_geometry.rs: compute_smoothed_dielectric
+void smoothed(double f, double ein, double eout, const double n[2], double &avg_eps,
+              double &avg_inv, std::array<double, 4> &t) {
+    avg_eps = f * ein + (1.0 - f) * eout;
+    avg_inv = f / ein + (1.0 - f) / eout;
+    if (f < 1e-10) { double inv = 1.0 / eout; avg_eps = eout; avg_inv = inv; t = {inv, 0, 0, inv}; return; }
+    if (f > 1.0 - 1e-10) { double inv = 1.0 / ein; avg_eps = ein; avg_inv = inv; t = {inv, 0, 0, inv}; return; }
+    double inv_t = 1.0 / avg_eps, delta = avg_inv - inv_t;
+    double pxx = n[0] * n[0], pxy = n[0] * n[1], pyy = n[1] * n[1];
+    t = {inv_t + delta * pyy, -delta * pxy, -delta * pxy, inv_t + delta * pxx};
+}
+
+// dielectric.rs:146-210 from_geometry and :473-574 build_smoothed_dielectric_analytic
+Dielectric build_dielectric(const Geometry &g, int nx, int ny, bool smoothing) {
+    Dielectric d;
+    d.nx = nx; d.ny = ny;
+    g.lat.reciprocal(d.b1, d.b2);
+    size_t n = (size_t)nx * ny;
+    d.eps.assign(n, g.eps_bg);
+    d.inv_eps.assign(n, 1.0 / g.eps_bg);
+    if (!smoothing) {
+        for (int iy = 0; iy < ny; ++iy)
+            for (int ix = 0; ix < nx; ++ix) {
+                double f[2] = {(double)ix / nx, (double)iy / ny};
+                double e = permittivity_at_fractional(g, f);
+                d.eps[iy * nx + ix] = e;
+                d.inv_eps[iy * nx + ix] = 1.0 / e;
+            }
+        return d;
+    }
+    d.tensor.assign(n, {0, 0, 0, 0});
+    double ps[2] = {std::sqrt(g.lat.a1[0] * g.lat.a1[0] + g.lat.a1[1] * g.lat.a1[1]) / nx,
+                    std::sqrt(g.lat.a2[0] * g.lat.a2[0] + g.lat.a2[1] * g.lat.a2[1]) / ny};
+    const double tol = 1e-6;  // SmoothingOptions::tolerance() (interface_tolerance 1e-6)
+    struct Circ { double c[2]; double r; double eps; };
+    std::vector<Circ> circles;
+    for (const Atom &a : g.atoms) {
+        double rc = a.radius * g.lat.char_len();
+        for (int di = -1; di <= 1; ++di)
+            for (int dj = -1; dj <= 1; ++dj) {
+                double f[2] = {a.pos[0] + di, a.pos[1] + dj};
+                Circ c;
+                g.lat.frac_to_cart(f, c.c);
+                c.r = rc; c.eps = a.eps_inside;
+                circles.push_back(c);
+            }
+    }
+    for (int iy = 0; iy < ny; ++iy)
+        for (int ix = 0; ix < nx; ++ix) {
+            size_t idx = (size_t)iy * nx + ix;
+            double fc[2] = {(double)ix / nx, (double)iy / ny}, pc[2];
+            g.lat.frac_to_cart(fc, pc);
+            bool have = false;
+            double best_f = 0, best_n[2] = {0, 0}, best_e = 0, total = 0;
+            for (const Circ &c : circles) {
+                PixelHit h = circle_rect(c.c, c.r, pc, ps);
+                if (h.fill > 1e-10) {
+                    total += h.fill;
+                    if (!have || h.fill > best_f) {
+                        double nrm[2];
+                        if (h.has_normal) { nrm[0] = h.normal[0]; nrm[1] = h.normal[1]; }
+                        else circle_normal_at(c.c, pc, nrm);
+                        best_f = h.fill; best_n[0] = nrm[0]; best_n[1] = nrm[1]; best_e = c.eps;
+                        have = true;
+                    }
+                }
+            }
+            total = std::min(std::max(total, 0.0), 1.0);
+            if (have) {
+                double f = std::min(best_f, total);
+                double ae, ai;
+                std::array<double, 4> t;
+                smoothed(f, best_e, g.eps_bg, best_n, ae, ai, t);
+                d.eps[idx] = ae;
+                d.inv_eps[idx] = ai;
+                bool interface = f > tol && f < 1.0 - tol;
+                d.tensor[idx] = interface ? t : std::array<double, 4>{ai, 0, 0, ai};
+            } else {
+                double inv = 1.0 / g.eps_bg;
+                d.eps[idx] = g.eps_bg;
+                d.inv_eps[idx] = inv;
+                d.tensor[idx] = {inv, 0, 0, inv};
+            }
+        }
+    return d;
+}
+
+// ---------------------------------------------------------------------------
+// FFT (backend-cpu/src/lib.rs:94-177): forward unnormalised e^{-2 pi i jk/n},
+// inverse e^{+...} scaled 1/(nx ny); rows (x contiguous) then columns.
+// Own mixed-radix recursive decimation in time (generic radix butterflies).
+// ---------------------------------------------------------------------------
+struct FFTPlan {
+    int n;
+    std::vector<int> fac;  // pairs (p, m)
+    std::vector<cd> tw;    // exp(-2 pi i k / n)
+};
+
+FFTPlan make_plan(int n) {
+    FFTPlan P;
+    P.n = n;
+    int m = n;
+    int p = 4;
+    while (m > 1) {
+        while (m % p) {
+            switch (p) {
+                case 4: p = 2; break;
+                case 2: p = 3; break;
+                default: p += 2; break;
+            }
+            if (p * p > m) p = m;
+        }
+        m /= p;
+        P.fac.push_back(p);
+        P.fac.push_back(m);
+    }
+    P.tw.resize(n);
+    for (int k = 0; k < n; ++k) {
+        double ang = -2.0 * PI * (double)k / (double)n;
+        P.tw[k] = cd(std::cos(ang), std::sin(ang));
+    }
+    return P;
+}
+
+void fft_work(cd *out, const cd *in, size_t fstride, size_t istride, const int *fac,
+              const FFTPlan &P, bool inv) {
+    int p = fac[0], m = fac[1];
+    if (m == 1) {
+        for (int k = 0; k < p; ++k) out[k] = in[k * fstride * istride];
+    } else {
+        for (int k = 0; k < p; ++k)
+            fft_work(out + k * m, in + k * fstride * istride, fstride * p, istride, fac + 2, P, inv);
+    }
+    cd scratch[64];
+    for (int u = 0; u < m; ++u) {
+        for (int q = 0; q < p; ++q) scratch[q] = out[q * m + u];
+        for (int q1 = 0; q1 < p; ++q1) {
+            int k = q1 * m + u;
+            cd acc = scratch[0];
+            size_t twidx = 0;
+            for (int q = 1; q < p; ++q) {
+                twidx += fstride * (size_t)k;
+                twidx %= (size_t)P.n;
+                cd w = P.tw[twidx];
+                if (inv) w = std::conj(w);
+                acc += scratch[q] * w;
+            }
+            out[k] = acc;
+        }
+    }
+}
+
+std::mutex plan_mu;
+std::map<int, std::shared_ptr<FFTPlan>> plan_cache;
+std::shared_ptr<FFTPlan> get_plan(int n) {
+    std::lock_guard<std::mutex> lk(plan_mu);
+    auto it = plan_cache.find(n);
+    if (it != plan_cache.end()) return it->second;
+    auto p = std::make_shared<FFTPlan>(make_plan(n));
+    plan_cache[n] = p;
+    return p;
+}
+
+void fft1d(const FFTPlan &P, const cd *in, size_t istride, cd *out, bool inv) {
+    fft_work(out, in, 1, istride, P.fac.data(), P, inv);
+}
+
+void fft2d(int nx, int ny, cd *data, bool inv) {
+    auto px = get_plan(nx), py = get_plan(ny);
+    std::vector<cd> tmp(std::max(nx, ny)), col(ny);
+    for (int iy = 0; iy < ny; ++iy) {
+        fft1d(*px, data + (size_t)iy * nx, 1, tmp.data(), inv);
+        std::copy(tmp.begin(), tmp.begin() + nx, data + (size_t)iy * nx);
+    }
+    for (int ix = 0; ix < nx; ++ix) {
+        fft1d(*py, data + ix, nx, tmp.data(), inv);
+        for (int iy = 0; iy < ny; ++iy) data[(size_t)iy * nx + ix] = tmp[iy];
+    }
+    if (inv) {
+        double s = 1.0 / ((double)nx * ny);
+        size_t n = (size_t)nx * ny;
+        for (size_t i = 0; i < n; ++i) data[i] *= s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Backend vector ops (backend-cpu/src/lib.rs:414-443)
+// ---------------------------------------------------------------------------
+inline cd dot(const Field &x, const Field &y) {  // sum conj(x) y in f64
+    double re = 0, im = 0;
+    for (size_t i = 0; i < x.size(); ++i) {
+        double a = x[i].real(), b = x[i].imag(), c = y[i].real(), d = y[i].imag();
+        re += a * c + b * d;
+        im += a * d - b * c;
+    }
+    return cd(re, im);
+}
+inline void axpy(cd alpha, const Field &x, Field &y) {
+    for (size_t i = 0; i < x.size(); ++i) y[i] += alpha * x[i];
+}
+inline void scale(cd alpha, Field &x) {
+    for (auto &v : x) v *= alpha;
+}
+
+// Gram G[i*nb + j] = sum_n conj(A_i[n]) B_j[n]   (faer X^H * Y, f64)
+void gram(const std::vector<const Field *> &A, const std::vector<const Field *> &B,
+          std::vector<cd> &G) {
+    size_t ra = A.size(), rb = B.size();
+    G.assign(ra * rb, cd(0, 0));
+    if (!ra || !rb) return;
+    size_t n = A[0]->size();
+    const size_t T = 32;
+    std::vector<double> are(T * ra), aim(T * ra), bre(T * rb), bim(T * rb);
+    std::vector<double> gre(ra * rb, 0.0), gim(ra * rb, 0.0);
+    for (size_t n0 = 0; n0 < n; n0 += T) {
+        size_t nt = std::min(T, n - n0);
+        for (size_t i = 0; i < ra; ++i)
+            for (size_t t = 0; t < nt; ++t) {
+                are[t * ra + i] = (*A[i])[n0 + t].real();
+                aim[t * ra + i] = (*A[i])[n0 + t].imag();
+            }
+        for (size_t j = 0; j < rb; ++j)
+            for (size_t t = 0; t < nt; ++t) {
+                bre[t * rb + j] = (*B[j])[n0 + t].real();
+                bim[t * rb + j] = (*B[j])[n0 + t].imag();
+            }
+        for (size_t t = 0; t < nt; ++t) {
+            const double *br = &bre[t * rb], *bi = &bim[t * rb];
+            for (size_t i = 0; i < ra; ++i) {
+                double ar = are[t * ra + i], ai = aim[t * ra + i];
+                double *gr = &gre[i * rb], *gi = &gim[i * rb];
+                for (size_t j = 0; j < rb; ++j) {
+                    gr[j] += ar * br[j] + ai * bi[j];
+                    gi[j] += ar * bi[j] - ai * br[j];
+                }
+            }
+        }
+    }
+    for (size_t k = 0; k < ra * rb; ++k) G[k] = cd(gre[k], gim[k]);
+}
+
+// out_j = sum_i A_i C(i, j), C column-major (r x q): C[i + j*ldc]
+void lincomb(const std::vector<const Field *> &A, const cd *C, size_t ldc, size_t q,
+             std::vector<Field> &out) {
+    size_t r = A.size();
+    size_t n = r ? A[0]->size() : 0;
+    out.assign(q, Field(n, cd(0, 0)));
+    for (size_t j = 0; j < q; ++j) {
+        double *o = reinterpret_cast<double *>(out[j].data());
+        for (size_t i = 0; i < r; ++i) {
+            double cr = C[i + j * ldc].real(), ci = C[i + j * ldc].imag();
+            const double *a = reinterpret_cast<const double *>(A[i]->data());
+            for (size_t k = 0; k < n; ++k) {
+                double ar = a[2 * k], ai = a[2 * k + 1];
+                o[2 * k] += ar * cr - ai * ci;
+                o[2 * k + 1] += ar * ci + ai * cr;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Dense Hermitian eigensolver: cyclic two-sided Jacobi in f64 (stands in for
+// faer self_adjoint_eigen, dense.rs:71-134; lower triangle read).
+// Returns ascending eigenvalues, eigenvectors column-major.
+// ---------------------------------------------------------------------------
+void jacobi_eigh(int n, const cd *a_colmajor, std::vector<double> &w, std::vector<cd> &V) {
+    std::vector<cd> A((size_t)n * n);
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) {
+            if (i > j) A[i + (size_t)j * n] = a_colmajor[i + (size_t)j * n];
+            else if (i < j) A[i + (size_t)j * n] = std::conj(a_colmajor[j + (size_t)i * n]);
+            else A[i + (size_t)i * n] = cd(a_colmajor[i + (size_t)i * n].real(), 0.0);
+        }
+    V.assign((size_t)n * n, cd(0, 0));
+    for (int i = 0; i < n; ++i) V[i + (size_t)i * n] = 1.0;
+    auto at = [&](int i, int j) -> cd & { return A[i + (size_t)j * n]; };
+    // Rotate every pair whose coupling is not negligible against its own
+    // diagonal (relative-accuracy stopping rule); stop after a quiet sweep.
+    for (int sweep = 0; sweep < 60; ++sweep) {
+        int rotations = 0;
+        for (int p = 0; p < n - 1; ++p)
+            for (int q = p + 1; q < n; ++q) {
+                cd h = at(p, q);
+                double habs = std::abs(h);
+                if (habs == 0.0) continue;
+                double app = at(p, p).real(), aqq = at(q, q).real();
+                if (habs <= 1e-17 * std::sqrt(std::fabs(app) * std::fabs(aqq))) { at(p, q) = 0; at(q, p) = 0; continue; }
+                ++rotations;
+                double tau = (aqq - app) / (2.0 * habs);
+                double t = (tau >= 0 ? -1.0 : 1.0) / (std::fabs(tau) + std::sqrt(1.0 + tau * tau));
+                double c = 1.0 / std::sqrt(1.0 + t * t), s = t * c;
+                cd eph = h / habs;             // e^{i phi}
+                cd sig = s * std::conj(eph);   // sigma = s e^{-i phi}
+                // columns: A[:,p] = c A[:,p] + sig A[:,q]; A[:,q] = -conj(sig) A[:,p] + c A[:,q]
+                for (int k = 0; k < n; ++k) {
+                    cd akp = at(k, p), akq = at(k, q);
+                    at(k, p) = c * akp + sig * akq;
+                    at(k, q) = -std::conj(sig) * akp + c * akq;
+                    cd vkp = V[k + (size_t)p * n], vkq = V[k + (size_t)q * n];
+                    V[k + (size_t)p * n] = c * vkp + sig * vkq;
+                    V[k + (size_t)q * n] = -std::conj(sig) * vkp + c * vkq;
+                }
+                // rows: A[p,:] = c A[p,:] + conj(sig) A[q,:]; A[q,:] = -sig A[p,:] + c A[q,:]
+                for (int k = 0; k < n; ++k) {
+                    cd apk = at(p, k), aqk = at(q, k);
+                    at(p, k) = c * apk + std::conj(sig) * aqk;
+                    at(q, k) = -sig * apk + c * aqk;
+                }
+                at(p, q) = 0; at(q, p) = 0;
+                at(p, p) = cd(at(p, p).real(), 0); at(q, q) = cd(at(q, q).real(), 0);
+            }
+        if (rotations == 0) break;
+    }
+    std::vector<int> idx(n);
+    for (int i = 0; i < n; ++i) idx[i] = i;
+    std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return at(x, x).real() < at(y, y).real(); });
+    w.resize(n);
+    std::vector<cd> Vs((size_t)n * n);
+    for (int j = 0; j < n; ++j) {
+        w[j] = at(idx[j], idx[j]).real();
+        for (int i = 0; i < n; ++i) Vs[i + (size_t)j * n] = V[i + (size_t)idx[j] * n];
+    }
+    V.swap(Vs);
+}
+
+// Polar factor U = W V^H of O = W S V^H (subspace_prediction.rs:491-531),
+// via the Hermitian eigendecomposition O^H O = V S^2 V^H:
+// U = O V S^{-1} V^H (unique for nonsingular O).  Returns sigma_min.
+double polar(int m, const std::vector<cd> &O /*col-major*/, std::vector<cd> &U) {
+    std::vector<cd> OhO((size_t)m * m);
+    for (int j = 0; j < m; ++j)
+        for (int i = 0; i < m; ++i) {
+            cd s = 0;
+            for (int k = 0; k < m; ++k) s += std::conj(O[k + (size_t)i * m]) * O[k + (size_t)j * m];
+            OhO[i + (size_t)j * m] = s;
+        }
+    std::vector<double> ev;
+    std::vector<cd> V;
+    jacobi_eigh(m, OhO.data(), ev, V);
+    double smin = INFINITY;
+    std::vector<double> sig(m);
+    for (int i = 0; i < m; ++i) { sig[i] = std::sqrt(std::max(ev[i], 0.0)); smin = std::min(smin, sig[i]); }
+    // W = O V S^{-1} (columns); complete null directions by Gram-Schmidt
+    std::vector<cd> Wm((size_t)m * m, cd(0, 0));
+    double smax = 0;
+    for (double s : sig) smax = std::max(smax, s);
+    for (int j = m - 1; j >= 0; --j) {
+        std::vector<cd> col(m, cd(0, 0));
+        if (sig[j] > 1e-13 * std::max(smax, 1e-300)) {
+            for (int i = 0; i < m; ++i) {
+                cd s = 0;
+                for (int k = 0; k < m; ++k) s += O[i + (size_t)k * m] * V[k + (size_t)j * m];
+                col[i] = s / sig[j];
+            }
+        } else {
+            for (int e = 0; e < m; ++e) {
+                std::fill(col.begin(), col.end(), cd(0, 0));
+                col[e] = 1.0;
+                for (int jj = j + 1; jj < m; ++jj) {
+                    cd c = 0;
+                    for (int i = 0; i < m; ++i) c += std::conj(Wm[i + (size_t)jj * m]) * col[i];
+                    for (int i = 0; i < m; ++i) col[i] -= c * Wm[i + (size_t)jj * m];
+                }
+                double nn = 0;
+                for (auto &v : col) nn += std::norm(v);
+                if (nn > 0.25) { nn = std::sqrt(nn); for (auto &v : col) v /= nn; break; }
+            }
+        }
+        for (int i = 0; i < m; ++i) Wm[i + (size_t)j * m] = col[i];
+    }
+    U.assign((size_t)m * m, cd(0, 0));
+    for (int j = 0; j < m; ++j)
+        for (int i = 0; i < m; ++i) {
+            cd s = 0;
+            for (int k = 0; k < m; ++k) s += Wm[i + (size_t)k * m] * std::conj(V[j + (size_t)k * m]);
+            U[i + (size_t)j * m] = s;
+        }
+    return smin;
+}
+
+// band_tracking.rs hungarian_assignment (max-weight, potentials method)
+std::vector<int> hungarian(const std::vector<std::vector<double>> &w) {
+    int n = (int)w.size();
+    if (n == 0) return {};
+    if (n == 1) return {0};
+    double mx = -INFINITY;
+    for (auto &r : w) for (double v : r) mx = std::max(mx, v);
+    std::vector<double> u(n + 1, 0.0), v(n + 1, 0.0);
+    std::vector<int> p(n + 1, 0), way(n + 1, 0);
+    for (int i = 1; i <= n; ++i) {
+        p[0] = i;
+        int j0 = 0;
+        std::vector<double> minv(n + 1, INFINITY);
+        std::vector<char> used(n + 1, 0);
+        while (true) {
+            used[j0] = 1;
+            int i0 = p[j0];
+            double delta = INFINITY;
+            int j1 = 0;
+            for (int j = 1; j <= n; ++j)
+                if (!used[j]) {
+                    double cur = (mx - w[i0 - 1][j - 1]) - u[i0] - v[j];
+                    if (cur < minv[j]) { minv[j] = cur; way[j] = j0; }
+                    if (minv[j] < delta) { delta = minv[j]; j1 = j; }
+                }
+            for (int j = 0; j <= n; ++j) {
+                if (used[j]) { u[p[j]] += delta; v[j] -= delta; }
+                else minv[j] -= delta;
+            }
+            j0 = j1;
+            if (p[j0] == 0) break;
+        }
+        while (true) {
+            int j1 = way[j0];
+            p[j0] = p[j1];
+            j0 = j1;
+            if (j0 == 0) break;
+        }
+    }
+    std::vector<int> perm(n, 0);
+    for (int j = 1; j <= n; ++j) perm[p[j] - 1] = j - 1;
+    return perm;
+}
+
+// ---------------------------------------------------------------------------
+// Operators (operators.rs:91-165, maxwell.rs)
+// ---------------------------------------------------------------------------
+struct Operator {
+    virtual ~Operator() {}
+    virtual void apply(const Field &in, Field &out) = 0;
+    virtual void apply_mass(const Field &in, Field &out) = 0;
+    virtual size_t len() const = 0;
+    virtual std::array<double, 2> bloch() const = 0;
+};
+
+struct Preconditioner {
+    virtual ~Preconditioner() {}
+    virtual void apply(Field &buf) = 0;
+};
+
+constexpr double KG_FLOOR = 1e-9;                 // operators.rs:62
+constexpr double TM_MASS_FRACTION = 1e-2;         // operators.rs:65
+
+struct KTables {
+    std::vector<double> kx, ky, ksq;
+    std::vector<char> clamped;
+};
+
+// maxwell.rs:1573-1690 build_k_plus_g_tables_with_reciprocal, clamp_gradient_components
+KTables build_k_tables(int nx, int ny, const double b1[2], const double b2[2], const double bloch[2]) {
+    KTables T;
+    size_t n = (size_t)nx * ny;
+    T.kx.resize(n); T.ky.resize(n); T.ksq.resize(n); T.clamped.assign(n, 0);
+    for (int iy = 0; iy < ny; ++iy)
+        for (int ix = 0; ix < nx; ++ix) {
+            size_t idx = (size_t)iy * nx + ix;
+            double n1 = ix <= nx / 2 ? ix : ix - nx;
+            double n2 = iy <= ny / 2 ? iy : iy - ny;
+            double gx = n1 * b1[0] + n2 * b2[0];
+            double gy = n1 * b1[1] + n2 * b2[1];
+            double rkx = gx + bloch[0], rky = gy + bloch[1];
+            double raw = rkx * rkx + rky * rky;
+            double ckx, cky;
+            if (!std::isfinite(rkx) || !std::isfinite(rky)) { ckx = std::sqrt(KG_FLOOR); cky = 0; }
+            else {
+                double ns = rkx * rkx + rky * rky;
+                if (ns >= KG_FLOOR) { ckx = rkx; cky = rky; }
+                else if (ns == 0.0) { ckx = std::sqrt(KG_FLOOR); cky = 0; }
+                else { double s = std::sqrt(KG_FLOOR / ns); ckx = rkx * s; cky = rky * s; }
+            }
+            if (raw <= KG_FLOOR) T.clamped[idx] = 1;
+            T.kx[idx] = ckx; T.ky[idx] = cky; T.ksq[idx] = ckx * ckx + cky * cky;
+        }
+    return T;
+}
+
+// fourier_diagonal.rs:146-189 SpectralStats::compute + adaptive_shift (0.5 s_min)
+double adaptive_shift(const std::vector<double> &ksq) {
+    double smin = INFINITY;
+    for (double v : ksq) if (v > 1e-6 && std::isfinite(v)) smin = std::min(smin, v);
+    if (!std::isfinite(smin)) smin = 1.0;
+    return 0.5 * smin;
+}
+
+struct ThetaOp : Operator {
+    const Dielectric &d;
+    int pol;  // 0 TM, 1 TE
+    double bl[2];
+    KTables T;
+    Field scratch, gx, gy;
+    ThetaOp(const Dielectric &diel, int polarization, const double bloch_[2]) : d(diel), pol(polarization) {
+        bl[0] = bloch_[0]; bl[1] = bloch_[1];
+        T = build_k_tables(d.nx, d.ny, d.b1, d.b2, bl);
+        size_t n = len();
+        scratch.resize(n); gx.resize(n); gy.resize(n);
+    }
+    size_t len() const override { return (size_t)d.nx * d.ny; }
+    std::array<double, 2> bloch() const override { return {bl[0], bl[1]}; }
+    bool is_gamma() const { return std::fabs(bl[0]) < 1e-12 && std::fabs(bl[1]) < 1e-12; }
+    // maxwell.rs:647-684 apply_te
+    void apply_te(const Field &in, Field &out) {
+        size_t n = len();
+        scratch = in;
+        fft2d(d.nx, d.ny, scratch.data(), false);
+        for (size_t i = 0; i < n; ++i) {  // compute_gradients_from_potential
+            gx[i] = scratch[i] * cd(0.0, T.kx[i]);
+            gy[i] = scratch[i] * cd(0.0, T.ky[i]);
+        }
+        fft2d(d.nx, d.ny, gx.data(), true);
+        fft2d(d.nx, d.ny, gy.data(), true);
+        if (!d.tensor.empty()) {  // apply_inv_eps (tensor)
+            for (size_t i = 0; i < n; ++i) {
+                cd ox = gx[i], oy = gy[i];
+                const auto &t = d.tensor[i];
+                gx[i] = ox * t[0] + oy * t[1];
+                gy[i] = ox * t[2] + oy * t[3];
+            }
+        } else {
+            for (size_t i = 0; i < n; ++i) { gx[i] *= d.inv_eps[i]; gy[i] *= d.inv_eps[i]; }
+        }
+        fft2d(d.nx, d.ny, gx.data(), false);
+        fft2d(d.nx, d.ny, gy.data(), false);
+        out.resize(n);
+        for (size_t i = 0; i < n; ++i) {  // assemble_divergence
+            cd div = cd(0.0, T.kx[i]) * gx[i] + cd(0.0, T.ky[i]) * gy[i];
+            out[i] = -div;
+        }
+        fft2d(d.nx, d.ny, out.data(), true);
+    }
+    // maxwell.rs:686-713 apply_tm
+    void apply_tm(const Field &in, Field &out) {
+        out = in;
+        fft2d(d.nx, d.ny, out.data(), false);
+        for (size_t i = 0; i < len(); ++i) out[i] *= T.ksq[i];
+        fft2d(d.nx, d.ny, out.data(), true);
+    }
+    void apply(const Field &in, Field &out) override { if (pol == 1) apply_te(in, out); else apply_tm(in, out); }
+    // maxwell.rs:1522-1530
+    void apply_mass(const Field &in, Field &out) override {
+        out = in;
+        if (pol == 0) for (size_t i = 0; i < len(); ++i) out[i] *= d.eps[i];
+    }
+};
+
+// fourier_diagonal.rs:310-342 + maxwell.rs:256-321,1692-1739
+struct FDPrecond : Preconditioner {
+    int nx, ny;
+    std::vector<double> inv_diag;
+    void apply(Field &buf) override {
+        fft2d(nx, ny, buf.data(), false);
+        for (size_t i = 0; i < buf.size(); ++i) buf[i] *= inv_diag[i];
+        fft2d(nx, ny, buf.data(), true);
+    }
+};
+
+std::unique_ptr<FDPrecond> build_fd(const ThetaOp &op) {
+    auto P = std::make_unique<FDPrecond>();
+    P->nx = op.d.nx; P->ny = op.d.ny;
+    double shift = adaptive_shift(op.T.ksq);
+    size_t n = op.len();
+    double eps_eff, mass_floor;
+    if (op.pol == 0) {
+        double s = 0; for (double v : op.d.eps) s += v;
+        eps_eff = n ? s / n : 1.0;                                  // arithmetic_mean
+        mass_floor = (!std::isfinite(eps_eff) || eps_eff <= 0) ? 0.0 : eps_eff * TM_MASS_FRACTION;
+    } else {
+        double s = 0; for (double v : op.d.inv_eps) s += v;
+        eps_eff = (n && s > 0) ? 1.0 / (s / n) : 1.0;              // harmonic_mean
+        mass_floor = 0.0;
+    }
+    P->inv_diag.resize(n);
+    for (size_t i = 0; i < n; ++i) {  // inverse_scale
+        double k = op.T.ksq[i];
+        double v;
+        if (!std::isfinite(k) || !std::isfinite(eps_eff) || eps_eff <= 0) v = 0.0;
+        else {
+            double safe_mass = (std::isfinite(mass_floor) && mass_floor > 0) ? mass_floor : 0.0;
+            double denom = std::max(k, 0.0) + safe_mass + shift * std::max(eps_eff, 1e-12);
+            v = eps_eff / denom;
+        }
+        if (op.is_gamma() && op.T.clamped[i]) v = 0.0;
+        P->inv_diag[i] = v;
+    }
+    return P;
+}
+
+// transverse_projection.rs:83-231
+struct TPPrecond : Preconditioner {
+    int nx, ny;
+    std::vector<double> kx, ky, inv_ksq, eps;
+    Field gx, gy;
+    void apply(Field &buf) override {
+        size_t n = buf.size();
+        gx.resize(n); gy.resize(n);
+        fft2d(nx, ny, buf.data(), false);
+        for (size_t i = 0; i < n; ++i) {
+            cd fx = cd(0.0, -kx[i]) * inv_ksq[i];
+            cd fy = cd(0.0, -ky[i]) * inv_ksq[i];
+            gx[i] = buf[i] * fx;
+            gy[i] = buf[i] * fy;
+        }
+        fft2d(nx, ny, gx.data(), true);
+        fft2d(nx, ny, gy.data(), true);
+        for (size_t i = 0; i < n; ++i) { gx[i] *= eps[i]; gy[i] *= eps[i]; }
+        fft2d(nx, ny, gx.data(), false);
+        fft2d(nx, ny, gy.data(), false);
+        for (size_t i = 0; i < n; ++i) {
+            cd div = cd(0.0, kx[i]) * gx[i] + cd(0.0, ky[i]) * gy[i];
+            buf[i] = -div * inv_ksq[i];
+        }
+        fft2d(nx, ny, buf.data(), true);
+    }
+};
+
+std::unique_ptr<TPPrecond> build_tp(const ThetaOp &op) {
+    auto P = std::make_unique<TPPrecond>();
+    P->nx = op.d.nx; P->ny = op.d.ny;
+    double shift = adaptive_shift(op.T.ksq);
+    size_t n = op.len();
+    P->kx = op.T.kx; P->ky = op.T.ky; P->eps = op.d.eps;
+    P->inv_ksq.resize(n);
+    for (size_t i = 0; i < n; ++i) {
+        if (op.T.clamped[i]) P->inv_ksq[i] = 0.0;
+        else {
+            double denom = op.T.ksq[i] + shift;
+            P->inv_ksq[i] = denom > 1e-15 ? 1.0 / denom : 0.0;
+        }
+    }
+    return P;
+}
+
+std::unique_ptr<Preconditioner> build_precond(const ThetaOp &op, int kind) {
+    if (kind < 0) kind = op.pol == 0 ? 1 : 2;  // diagnostics.rs:158-166 Auto
+    if (kind == 1) return build_fd(op);
+    if (kind == 2) return build_tp(op);
+    return nullptr;
+}
+
+// Diagonal test operator (crates/core/src/_tests_eigensolver.rs:17-119)
+struct DiagOp : Operator {
+    std::vector<double> diag;
+    size_t len() const override { return diag.size(); }
+    std::array<double, 2> bloch() const override { return {0.1, 0.1}; }
+    void apply(const Field &in, Field &out) override {
+        out.resize(in.size());
+        for (size_t i = 0; i < in.size(); ++i) out[i] = in[i] * diag[i];
+    }
+    void apply_mass(const Field &in, Field &out) override { out = in; }
+};
+
+// ---------------------------------------------------------------------------
+// SVQB (normalization.rs:281-544)
+// ---------------------------------------------------------------------------
+double b_norm(const Field &x, const Field &bx) { return std::sqrt(std::max(dot(x, bx).real(), 0.0)); }
+
+int svqb(std::vector<Field> &X, std::vector<Field> &BX) {
+    int p = (int)X.size();
+    if (p == 0) return 0;
+    for (int i = 0; i < p; ++i) {  // normalize_to_unit_b_norm_with_tol(1e-30)
+        double nrm = b_norm(X[i], BX[i]);
+        if (nrm > 1e-30) { cd s(1.0 / nrm, 0.0); scale(s, X[i]); scale(s, BX[i]); }
+    }
+    std::vector<const Field *> xa, ba;
+    for (int i = 0; i < p; ++i) { xa.push_back(&X[i]); ba.push_back(&BX[i]); }
+    std::vector<cd> G;  // row-major G(i,j)
+    gram(xa, ba, G);
+    // faer lower triangle of row-major gram: element (i,j), i>=j -> col-major[i + j p] = G(i,j)
+    std::vector<cd> Gc((size_t)p * p);
+    for (int i = 0; i < p; ++i) for (int j = 0; j < p; ++j) Gc[i + (size_t)j * p] = G[(size_t)i * p + j];
+    std::vector<double> ev;
+    std::vector<cd> V;
+    if (p == 1) { ev = {G[0].real()}; V = {cd(1, 0)}; }
+    else jacobi_eigh(p, Gc.data(), ev, V);
+    // descending order
+    std::vector<double> evd(p);
+    std::vector<cd> Vd((size_t)p * p);
+    for (int c = 0; c < p; ++c) {
+        evd[c] = ev[p - 1 - c];
+        for (int r = 0; r < p; ++r) Vd[r + (size_t)c * p] = V[r + (size_t)(p - 1 - c) * p];
+    }
+    if (p == 1) { evd = ev; Vd = V; }
+    double mx = 0.0;
+    for (double e : evd) mx = std::max(mx, e);
+    double thr = 1e-12 * mx;
+    std::vector<int> kept;
+    for (int i = 0; i < p; ++i) if (evd[i] >= thr && evd[i] > 0.0) kept.push_back(i);
+    int rank = (int)kept.size();
+    size_t n = X[0].size();
+    if (rank == 0) {
+        for (auto &x : X) std::fill(x.begin(), x.end(), cd(0, 0));
+        for (auto &x : BX) std::fill(x.begin(), x.end(), cd(0, 0));
+        return 0;
+    }
+    std::vector<cd> Tm((size_t)p * rank);
+    for (int c = 0; c < rank; ++c) {
+        int in = kept[c];
+        double sc = 1.0 / std::sqrt(std::max(evd[in], 1e-30));
+        for (int r = 0; r < p; ++r) Tm[r + (size_t)c * p] = Vd[r + (size_t)in * p] * sc;
+    }
+    std::vector<Field> Xn, BXn;
+    lincomb(xa, Tm.data(), p, rank, Xn);
+    lincomb(ba, Tm.data(), p, rank, BXn);
+    for (int c = 0; c < rank; ++c) { X[c] = Xn[c]; BX[c] = BXn[c]; }
+    for (int c = rank; c < p; ++c) {
+        std::fill(X[c].begin(), X[c].end(), cd(0, 0));
+        std::fill(BX[c].begin(), BX[c].end(), cd(0, 0));
+    }
+    (void)n;
+    return rank;
+}
+
+// ---------------------------------------------------------------------------
+// LOBPCG Eigensolver (eigensolver.rs:444-2437 + initialization.rs, deflation.rs)
+// ---------------------------------------------------------------------------
+struct Entry { Field x, bx, ax; };
+
+struct Deflation {  // deflation.rs:61-162,265-332
+    std::vector<Field> y, by;
+    std::vector<double> lam;
+    bool empty() const { return y.empty(); }
+    void project_single(Field &v, Field &bv) const {
+        for (size_t i = 0; i < y.size(); ++i) {
+            cd c = dot(y[i], bv);
+            axpy(-c, y[i], v);
+            axpy(-c, by[i], bv);
+        }
+    }
+    bool add_vector(const Field &v0, const Field &bv0, double eig) {
+        Field v = v0, bv = bv0;
+        project_single(v, bv);
+        double nrm = b_norm(v, bv);
+        if (nrm < 1e-10) return false;
+        cd s(1.0 / nrm, 0);
+        scale(s, v); scale(s, bv);
+        y.push_back(v); by.push_back(bv); lam.push_back(eig);
+        return true;
+    }
+    void project_block_no_mass(std::vector<Field> &V) const {
+        for (auto &v : V)
+            for (size_t i = 0; i < y.size(); ++i) {
+                cd c = std::conj(dot(v, by[i]));
+                axpy(-c, y[i], v);
+            }
+    }
+};
+
+// initialization.rs:113-125
+void seed_random(Field &v, double phase) {
+    uint64_t bits;
+    std::memcpy(&bits, &phase, 8);
+    uint64_t s = bits * 0x9E3779B97F4A7C15ULL;
+    if (s == 0) s = 0xDEADBEEFCAFEBABEULL;
+    for (auto &x : v) {
+        s ^= s << 13;
+        s ^= s >> 7;
+        s ^= s << 17;
+        double re = (double)(s >> 12) / (double)(1ULL << 52) * 2.0 - 1.0;
+        x = cd(re, 0.0);
+    }
+}
+
+struct SolveResult { std::vector<double> eigenvalues; int iterations; bool converged; };
+
+struct Eigensolver {
+    Operator &op;
+    Preconditioner *pc;
+    const std::vector<Field> *warm;
+    int n_bands, max_iter, block_size_cfg, exit_rule;
+    double tol;
+    std::vector<Entry> X;
+    std::vector<Field> W;
+    Deflation defl;
+    std::vector<double> lam, lam_prev;
+    std::vector<char> soft;
+
+    Eigensolver(Operator &o, Preconditioner *p, const std::vector<Field> *w, int nb, int mi, double t,
+                int bs, int er)
+        : op(o), pc(p), warm(w), n_bands(nb), max_iter(mi), block_size_cfg(bs), exit_rule(er), tol(t) {}
+
+    size_t soft_count() const { size_t c = 0; for (size_t i = 0; i < soft.size(); ++i) c += soft[i] ? 1 : 0; return c; }
+
+    // initialization.rs:314-367 build_entry_from_buffer
+    bool build_entry(Field v, double zero_tol, Entry &out) {
+        Field m;
+        op.apply_mass(v, m);
+        for (const Entry &e : X) {  // project_out (normalization.rs:139-152)
+            cd coeff = std::conj(dot(v, e.bx));
+            axpy(-coeff, e.x, v);
+            axpy(-coeff, e.bx, m);
+        }
+        double nrm = b_norm(v, m);
+        if (!(nrm > zero_tol)) return false;
+        cd s(1.0 / nrm, 0);
+        scale(s, v); scale(s, m);
+        out.x = std::move(v); out.bx = std::move(m);
+        op.apply(out.x, out.ax);
+        return true;
+    }
+
+    // eigensolver.rs:525-624 initialize
+    void initialize() {
+        auto bl = op.bloch();
+        size_t n = op.len();
+        if (std::fabs(bl[0]) < 1e-12 && std::fabs(bl[1]) < 1e-12) {
+            Field u0(n, cd(1.0, 0.0)), bu0;  // create_gamma_mode (initialization.rs:166-224)
+            op.apply_mass(u0, bu0);
+            double ns = dot(u0, bu0).real();
+            double nrm = ns > 0 ? std::sqrt(ns) : 1.0;
+            cd s(1.0 / nrm, 0);
+            scale(s, u0); scale(s, bu0);
+            if (b_norm(u0, bu0) > 1e-12) defl.add_vector(u0, bu0, 0.0);
+        }
+        int block = block_size_cfg == 0 ? n_bands + 2 : block_size_cfg;
+        block = std::max(block, std::max(n_bands, 1));
+        int max_attempts = block * 8;
+        // initialization.rs:248-309 initialize_block
+        if (warm) {
+            int taken = 0;
+            for (const Field &f : *warm) {
+                if (taken >= block) break;
+                ++taken;
+                Entry e;
+                if (build_entry(f, 1e-12, e)) X.push_back(std::move(e));
+                if ((int)X.size() >= block) break;
+            }
+        }
+        int attempts = 0;
+        double phase = 1.0;
+        while ((int)X.size() < block && attempts < max_attempts) {
+            Field v(n);
+            seed_random(v, phase);
+            phase += 1.0;
+            Entry e;
+            if (build_entry(v, 1e-12, e)) X.push_back(std::move(e));
+            else ++attempts;
+        }
+        if (!defl.empty()) {
+            for (Entry &e : X) {
+                defl.project_single(e.x, e.bx);
+                double nrm = b_norm(e.x, e.bx);
+                if (nrm > 1e-12) { cd s(1.0 / nrm, 0); scale(s, e.x); scale(s, e.bx); }
+            }
+            for (Entry &e : X) op.apply(e.x, e.ax);
+        }
+        soft.assign(X.size(), 0);
+        lam.clear();
+        for (Entry &e : X) {  // BlockEntry::rayleigh_quotient
+            double xax = dot(e.x, e.ax).real(), xbx = dot(e.x, e.bx).real();
+            lam.push_back(std::fabs(xbx) > 1e-15 ? xax / xbx : 0.0);
+        }
+    }
+
+    std::vector<double> rel_changes() const {  // eigensolver.rs:789-806
+        std::vector<double> out;
+        if (lam_prev.empty()) return std::vector<double>(lam.size(), INFINITY);
+        size_t k = std::min(lam.size(), lam_prev.size());
+        for (size_t i = 0; i < k; ++i) out.push_back(std::fabs(lam[i] - lam_prev[i]) / std::max(std::fabs(lam_prev[i]), 1e-10));
+        return out;
+    }
+
+    // eigensolver.rs:889-969 soft_lock_bands
+    size_t soft_lock(const std::vector<int> &want) {
+        if (want.empty()) return 0;
+        std::vector<char> lockset(std::max(lam.size(), soft.size()) + 1, 0);
+        for (int b : want) if (b < (int)lockset.size()) lockset[b] = 1;
+        std::vector<int> safe;
+        for (int b : want) {
+            if (b >= (int)lam.size() || b >= (int)soft.size()) continue;
+            if (soft[b]) continue;
+            double li = lam[b];
+            bool ok = true;
+            for (int j = 0; j < (int)lam.size(); ++j) {
+                if (j == b || j >= (int)soft.size()) continue;
+                if (soft[j]) continue;
+                double mx = std::max(std::max(std::fabs(li), std::fabs(lam[j])), 1e-10);
+                if (std::fabs(li - lam[j]) / mx < 1e-4 && !lockset[j]) { ok = false; break; }
+            }
+            if (ok) safe.push_back(b);
+        }
+        size_t newly = 0;
+        for (int b : safe) if (!soft[b]) { soft[b] = 1; ++newly; }
+        return newly;
+    }
+
+    // eigensolver.rs:978-1008 apply_soft_deflation (index i is the position in
+    // the given block, compared against soft[i] exactly as the reference does)
+    void soft_deflate(std::vector<Field> &R) const {
+        if (soft_count() == 0) return;
+        for (size_t i = 0; i < R.size(); ++i) {
+            if (i < soft.size() && soft[i]) continue;
+            for (size_t j = 0; j < soft.size(); ++j)
+                if (soft[j] && j < X.size()) {
+                    cd ov = dot(X[j].bx, R[i]);
+                    axpy(-ov, X[j].x, R[i]);
+                }
+        }
+    }
+
+    std::vector<double> all_eigenvalues() const {  // eigensolver.rs:2362-2375
+        std::vector<double> a = defl.lam;
+        a.insert(a.end(), lam.begin(), lam.end());
+        std::stable_sort(a.begin(), a.end());
+        return a;
+    }
+
+    // eigensolver.rs:2404-2437 all_eigenvectors
+    std::vector<Field> all_eigenvectors() const {
+        std::vector<std::pair<double, const Field *>> pr;
+        for (size_t i = 0; i < defl.y.size(); ++i) pr.push_back({defl.lam[i], &defl.y[i]});
+        for (size_t i = 0; i < lam.size() && i < X.size(); ++i) pr.push_back({lam[i], &X[i].x});
+        std::stable_sort(pr.begin(), pr.end(), [](auto &a, auto &b) { return a.first < b.first; });
+        std::vector<Field> out;
+        for (size_t i = 0; i < pr.size() && (int)i < n_bands; ++i) out.push_back(*pr[i].second);
+        return out;
+    }
+
+    SolveResult finish(bool conv, int iters) {
+        auto a = all_eigenvalues();
+        a.resize(std::min((size_t)n_bands, a.size()));
+        return {a, iters, conv};
+    }
+
+    // eigensolver.rs:1684-2192 solve
+    SolveResult solve() {
+        initialize();
+        int nb_req = n_bands;
+        int nb = std::min(nb_req, (int)X.size());
+        std::vector<double> conv_changes;
+        int iter_done = 0;
+        for (int iter = 0; iter < max_iter; ++iter) {
+            iter_done = iter;
+            size_t m = X.size();
+            if (m == 0) break;
+            // 1. residuals (:707-738)
+            std::vector<Field> R(m);
+            for (size_t i = 0; i < m; ++i) {
+                if (i < soft.size() && soft[i]) { R[i].assign(op.len(), cd(0, 0)); continue; }
+                R[i] = X[i].ax;
+                axpy(cd(-lam[i], 0.0), X[i].bx, R[i]);
+            }
+            // 2. deflation
+            if (!defl.empty()) defl.project_block_no_mass(R);
+            soft_deflate(R);
+            // 4. convergence (:1750-1801)
+            int n_conv = 0;
+            std::vector<double> ch;
+            if (iter >= 2) {
+                ch = rel_changes();
+                size_t ncheck = std::min(m, ch.size());
+                for (size_t i = 0; i < ncheck && (int)i < nb; ++i) if (ch[i] < tol) ++n_conv;
+            }
+            int n_hard = (int)defl.y.size();
+            int n_soft = (int)soft_count();
+            int total = exit_rule == 0 ? n_hard + n_soft + n_conv : n_soft;
+            if (total >= nb_req) return finish(true, iter + 1);
+            // 5. soft locking (:1809-1864)
+            if (iter >= 2) {
+                std::vector<int> want;
+                size_t ncheck = std::min(m, ch.size());
+                for (size_t i = 0; i < ncheck; ++i) if (ch[i] < tol) want.push_back((int)i);
+                if (!want.empty()) {
+                    size_t newly = soft_lock(want);
+                    if (newly > 0 && (int)soft_count() >= nb_req) return finish(true, iter + 1);
+                }
+            }
+            // 6. precondition active residuals
+            std::vector<Field> P;
+            for (size_t i = 0; i < m; ++i) if (!(i < soft.size() && soft[i])) P.push_back(R[i]);
+            if (pc) for (auto &p : P) pc->apply(p);
+            // 7. deflate P
+            if (!defl.empty()) defl.project_block_no_mass(P);
+            soft_deflate(P);
+            // 8. Z = [X, P_active, W_active] (:1032-1090)
+            size_t n_active = 0;
+            for (size_t i = 0; i < m && i < soft.size(); ++i) if (!soft[i]) ++n_active;
+            size_t psz = std::min(P.size(), n_active);
+            size_t wsz = W.empty() ? 0 : std::min(W.size(), n_active);
+            std::vector<Field> Z, BZ;
+            for (auto &e : X) { Z.push_back(e.x); BZ.push_back(e.bx); }
+            for (size_t i = 0; i < psz; ++i) Z.push_back(P[i]);
+            for (size_t i = 0; i < wsz; ++i) Z.push_back(W[i]);
+            for (size_t i = m; i < Z.size(); ++i) { Field b; op.apply_mass(Z[i], b); BZ.push_back(std::move(b)); }
+            // 9. SVQB
+            int rank = svqb(Z, BZ);
+            Z.resize(rank); BZ.resize(rank);
+            // 10. AQ, A_s = Q^H AQ (col-major)
+            std::vector<Field> AQ(rank);
+            for (int i = 0; i < rank; ++i) op.apply(Z[i], AQ[i]);
+            std::vector<const Field *> qa, aqa, bqa;
+            for (int i = 0; i < rank; ++i) { qa.push_back(&Z[i]); aqa.push_back(&AQ[i]); bqa.push_back(&BZ[i]); }
+            std::vector<cd> G;
+            gram(qa, aqa, G);  // row-major G(i,j) = q_i^H A q_j
+            std::vector<cd> As((size_t)rank * rank);
+            for (int i = 0; i < rank; ++i) for (int j = 0; j < rank; ++j) As[i + (size_t)j * rank] = G[(size_t)i * rank + j];
+            std::vector<double> theta;
+            std::vector<cd> Y;
+            if (rank > 0) jacobi_eigh(rank, As.data(), theta, Y);
+            // 12. Ritz update (:1341-1530)
+            lam_prev = lam;
+            int mm = std::min(n_bands, rank);
+            if (mm > 0) {
+                lam.assign(theta.begin(), theta.begin() + mm);
+                std::vector<Field> xn, bxn, axn;
+                lincomb(qa, Y.data(), rank, mm, xn);
+                lincomb(bqa, Y.data(), rank, mm, bxn);
+                lincomb(aqa, Y.data(), rank, mm, axn);
+                X.clear();
+                for (int j = 0; j < mm; ++j) X.push_back({std::move(xn[j]), std::move(bxn[j]), std::move(axn[j])});
+            }
+            // 13. W = Q Y[:, m:min(2m,r)] (:1548-1637)
+            int ws = mm, we = std::min(2 * mm, rank);
+            if (we - ws <= 0 || rank == 0) W.clear();
+            else lincomb(qa, Y.data() + (size_t)ws * rank, rank, we - ws, W);
+        }
+        int n_hard = (int)defl.y.size();
+        int n_soft = (int)soft_count();
+        bool conv = n_hard + n_soft >= nb_req;
+        return finish(conv, iter_done + 1);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Subspace prediction (subspace_prediction.rs:170-1110)
+// ---------------------------------------------------------------------------
+std::vector<cd> overlap(const std::vector<Field> &prev, const std::vector<Field> &curr,
+                        const std::vector<double> *eps) {
+    size_t m = std::min(prev.size(), curr.size());
+    std::vector<Field> pw, cw;
+    std::vector<const Field *> pa, ca;
+    if (eps) {
+        size_t n = prev[0].size();
+        std::vector<double> se(n);
+        for (size_t i = 0; i < n; ++i) se[i] = std::sqrt((*eps)[i]);
+        pw.resize(m); cw.resize(m);
+        for (size_t j = 0; j < m; ++j) {
+            pw[j].resize(n); cw[j].resize(n);
+            for (size_t i = 0; i < n; ++i) {
+                pw[j][i] = cd(prev[j][i].real() * se[i], prev[j][i].imag() * se[i]);
+                cw[j][i] = cd(curr[j][i].real() * se[i], curr[j][i].imag() * se[i]);
+            }
+            pa.push_back(&pw[j]); ca.push_back(&cw[j]);
+        }
+    } else {
+        for (size_t j = 0; j < m; ++j) { pa.push_back(&prev[j]); ca.push_back(&curr[j]); }
+    }
+    std::vector<cd> G;
+    gram(pa, ca, G);  // row-major (i, j)
+    std::vector<cd> O(m * m);
+    for (size_t i = 0; i < m; ++i) for (size_t j = 0; j < m; ++j) O[i + j * m] = G[i * m + j];
+    return O;
+}
+
+std::vector<Field> orthonormalize_fields(std::vector<Field> v, const std::vector<double> *eps) {
+    size_t m = v.size();
+    if (!m) return v;
+    size_t n = v[0].size();
+    for (size_t i = 0; i < m; ++i) {
+        double ns = 0;
+        if (eps) for (size_t k = 0; k < n; ++k) ns += (std::conj(v[i][k]) * v[i][k] * (*eps)[k]).real();
+        else for (size_t k = 0; k < n; ++k) ns += (std::conj(v[i][k]) * v[i][k]).real();
+        double nrm = std::sqrt(std::max(ns, 0.0));
+        if (nrm > 1e-14) { double s = 1.0 / nrm; for (auto &x : v[i]) x *= s; }
+        for (size_t j = i + 1; j < m; ++j) {
+            cd c = 0;
+            if (eps) for (size_t k = 0; k < n; ++k) c += std::conj(v[i][k]) * v[j][k] * (*eps)[k];
+            else for (size_t k = 0; k < n; ++k) c += std::conj(v[i][k]) * v[j][k];
+            for (size_t k = 0; k < n; ++k) v[j][k] -= c * v[i][k];
+        }
+    }
+    return v;
+}
+
+struct History {
+    int n_bands;
+    bool extrap = false;
+    bool has_prev = false, has_curr = false;
+    std::vector<Field> prev, curr;
+    double prev_k[2], curr_k[2];
+    void update(const double k[2], const std::vector<Field> &ev) {
+        if (has_curr) { prev = std::move(curr); prev_k[0] = curr_k[0]; prev_k[1] = curr_k[1]; has_prev = true; }
+        else has_prev = false;
+        size_t n = std::min(ev.size(), (size_t)n_bands);
+        curr.assign(ev.begin(), ev.begin() + n);
+        curr_k[0] = k[0]; curr_k[1] = k[1];
+        has_curr = true;
+    }
+    // returns empty when no prediction
+    std::vector<Field> predict(const double kn[2], const std::vector<double> *eps) {
+        if (!has_curr) return {};
+        if (!has_prev) return curr;
+        size_t m = std::min(prev.size(), curr.size());
+        std::vector<cd> O = overlap(prev, curr, eps), U;
+        double smin = polar((int)m, O, U);
+        bool degenerate = smin < 0.1;
+        double dkp[2] = {curr_k[0] - prev_k[0], curr_k[1] - prev_k[1]};
+        double dkn[2] = {kn[0] - curr_k[0], kn[1] - curr_k[1]};
+        bool corner = dkp[0] * dkn[0] + dkp[1] * dkn[1] < 0.0;
+        bool use_ex = extrap && !degenerate && !corner;
+        std::vector<Field> pred;
+        std::vector<const Field *> ca;
+        for (size_t j = 0; j < curr.size(); ++j) ca.push_back(&curr[j]);
+        std::vector<cd> R(m * m);  // rotation_adj(i, j) = conj(U(j, i)) * coeff
+        if (use_ex) {
+            double np = std::sqrt(dkp[0] * dkp[0] + dkp[1] * dkp[1]);
+            double nn = std::sqrt(dkn[0] * dkn[0] + dkn[1] * dkn[1]);
+            double alpha = np > 1e-12 ? nn / np : 1.0;
+            double ca_ = 1.0 + alpha;
+            for (size_t i = 0; i < m; ++i)
+                for (size_t j = 0; j < m; ++j) {
+                    cd u = U[j + i * m];
+                    R[i + j * m] = cd(u.real() * ca_, -u.imag() * ca_);
+                }
+            ca.resize(m);
+            lincomb(ca, R.data(), m, m, pred);
+            cd cp(-alpha, 0.0);
+            for (size_t j = 0; j < m; ++j)
+                for (size_t k = 0; k < pred[j].size(); ++k) pred[j][k] = pred[j][k] + cp * prev[j][k];
+        } else {
+            for (size_t i = 0; i < m; ++i)
+                for (size_t j = 0; j < m; ++j) {
+                    cd u = U[j + i * m];
+                    R[i + j * m] = cd(u.real(), -u.imag());
+                }
+            lincomb(ca, R.data(), m, m, pred);
+        }
+        return orthonormalize_fields(std::move(pred), eps);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Band tracking (band_tracking.rs:294-700)
+// ---------------------------------------------------------------------------
+std::vector<int> track_bands(const std::vector<Field> &prevv, const std::vector<Field> &currv,
+                             const std::vector<double> &pw, const std::vector<double> &cw,
+                             const std::vector<double> *eps) {
+    int n = (int)currv.size();
+    std::vector<int> ident(n);
+    for (int i = 0; i < n; ++i) ident[i] = i;
+    if (n == 0 || prevv.empty()) return ident;
+    std::vector<cd> O = overlap(prevv, currv, eps), U;
+    int m = (int)std::min(prevv.size(), currv.size());
+    polar(m, O, U);
+    // detect_degenerate_bands_from_rotation
+    std::vector<std::pair<int, int>> rblocks;
+    if (m >= 2) {
+        std::vector<char> amb(m, 0);
+        for (int i = 0; i < m; ++i) {
+            std::vector<double> row(m);
+            for (int j = 0; j < m; ++j) row[j] = std::abs(U[i + (size_t)j * m]);
+            std::sort(row.begin(), row.end(), std::greater<double>());
+            double mx = row[0], sec = row[1];
+            double ratio = mx > 1e-10 ? sec / mx : 0.0;
+            amb[i] = (mx < 0.7 || ratio > 0.5);
+        }
+        int bs = -1;
+        for (int i = 0; i < m; ++i) {
+            if (amb[i]) { if (bs < 0) bs = i; }
+            else if (bs >= 0) { if (i - bs > 1) rblocks.push_back({bs, i - bs}); bs = -1; }
+        }
+        if (bs >= 0 && m - bs > 1) rblocks.push_back({bs, m - bs});
+    }
+    // split_blocks_by_frequency
+    auto near = [](double a, double b, double at, double rt) {
+        double gap = std::fabs(a - b);
+        if (gap <= at) return true;
+        double sc = std::max(0.5 * (std::fabs(a) + std::fabs(b)), 1e-6);
+        return gap / sc <= rt;
+    };
+    std::vector<std::pair<int, int>> dblocks;
+    for (auto &b : rblocks) {
+        int start = b.first, size = b.second;
+        if (size < 2 || start >= (int)pw.size()) continue;
+        int end = std::min((int)pw.size(), start + size);
+        if (end - start < 2) continue;
+        int cs = start, cl = 1;
+        for (int idx = start + 1; idx < end; ++idx) {
+            if (near(pw[idx - 1], pw[idx], 5e-3, 0.015)) ++cl;
+            else { if (cl > 1) dblocks.push_back({cs, cl}); cs = idx; cl = 1; }
+        }
+        if (cl > 1) dblocks.push_back({cs, cl});
+    }
+    std::vector<std::vector<double>> W(m, std::vector<double>(m));
+    for (int i = 0; i < m; ++i) for (int j = 0; j < m; ++j) W[i][j] = std::abs(U[i + (size_t)j * m]);
+    std::vector<int> perm = hungarian(W);
+    auto refine = [&](int bstart, int bsize) {
+        int bend = bstart + bsize;
+        if (bend > (int)perm.size() || bend > (int)pw.size()) return;
+        std::vector<int> ci(bsize);
+        for (int k = 0; k < bsize; ++k) ci[k] = perm[bstart + k];
+        for (int c : ci) if (c >= (int)cw.size()) return;
+        std::vector<std::vector<double>> fw(bsize, std::vector<double>(bsize));
+        for (int a = 0; a < bsize; ++a)
+            for (int b = 0; b < bsize; ++b) fw[a][b] = -std::fabs(pw[bstart + a] - cw[ci[b]]);
+        std::vector<int> lp = hungarian(fw);
+        for (int li = 0; li < (int)lp.size(); ++li)
+            if (li < bsize && lp[li] < bsize) perm[bstart + li] = ci[lp[li]];
+    };
+    for (auto &b : dblocks) refine(b.first, b.second);
+    // detect_frequency_mismatch_blocks
+    std::vector<std::pair<int, int>> mblocks;
+    int bs = -1;
+    for (int i = 0; i < (int)perm.size(); ++i) {
+        int c = perm[i];
+        if (i >= (int)pw.size() || c >= (int)cw.size()) {
+            if (bs >= 0) { if (i - bs > 1) mblocks.push_back({bs, i - bs}); bs = -1; }
+            continue;
+        }
+        double mis = std::fabs(pw[i] - cw[c]);
+        double sc = std::max(std::max(std::fabs(pw[i]), std::fabs(cw[c])), 1e-3);
+        bool bad = mis > 5e-3 && mis / sc > 0.02;
+        if (bad) { if (bs < 0) bs = i; }
+        else if (bs >= 0) { if (i - bs > 1) mblocks.push_back({bs, i - bs}); bs = -1; }
+    }
+    if (bs >= 0 && (int)perm.size() - bs > 1) mblocks.push_back({bs, (int)perm.size() - bs});
+    for (auto &b : mblocks) refine(b.first, b.second);
+    if ((int)perm.size() < n) for (int i = (int)perm.size(); i < n; ++i) perm.push_back(i);
+    return perm;
+}
+
+// ---------------------------------------------------------------------------
+// bandstructure::run_core (drivers/bandstructure.rs:381-885), plain solve()
+// ---------------------------------------------------------------------------
+Geometry geometry_from_job(const OracleJob *j) {
+    Geometry g;
+    g.lat.a1[0] = j->a1[0]; g.lat.a1[1] = j->a1[1];
+    g.lat.a2[0] = j->a2[0]; g.lat.a2[1] = j->a2[1];
+    g.eps_bg = j->eps_bg;
+    for (int i = 0; i < j->n_atoms && i < ORACLE_MAX_ATOMS; ++i)
+        g.atoms.push_back({{j->atoms[i][0], j->atoms[i][1]}, j->atoms[i][2], j->atoms[i][3]});
+    return g;
+}
+
+int run_core(const OracleJob *job, double *omegas_out, int *iters_out, int *conv_out) {
+    Geometry g = geometry_from_job(job);
+    Dielectric d = build_dielectric(g, job->nx, job->ny, job->smoothing != 0);
+    int nb = job->n_bands;
+    const std::vector<double> *eps_track = job->pol == 0 ? &d.eps : nullptr;
+    History hist;
+    hist.n_bands = nb;
+    hist.extrap = job->extrapolation != 0;
+    std::vector<Field> prev_vecs;
+    std::vector<double> prev_om;
+    bool have_prev = false;
+    auto is_gamma_frac = [](const double *k) { return k[0] * k[0] + k[1] * k[1] < 1e-8; };
+    int nk = job->nk;
+    bool first_g = nk > 0 && is_gamma_frac(job->kpath);
+    bool last_g = nk > 0 && is_gamma_frac(job->kpath + 2 * (nk - 1));
+    bool reuse = job->reuse_gamma && first_g && last_g && nk > 1;
+    std::vector<double> first_gamma;
+    int first_gamma_iters = 0;
+    bool have_first_gamma = false;
+    int total_iters = 0;
+    for (int k = 0; k < nk; ++k) {
+        const double *kf = job->kpath + 2 * k;
+        bool isg = is_gamma_frac(kf);
+        if (reuse && k == nk - 1 && have_first_gamma) {
+            for (int b = 0; b < nb; ++b) omegas_out[(size_t)k * nb + b] = b < (int)first_gamma.size() ? first_gamma[b] : NAN;
+            iters_out[k] = first_gamma_iters;
+            conv_out[k] = 1;
+            continue;
+        }
+        double bloch[2] = {kf[0] * d.b1[0] + kf[1] * d.b2[0], kf[0] * d.b1[1] + kf[1] * d.b2[1]};
+        std::vector<Field> warm;
+        bool use_warm = false;
+        if (job->subspace_prediction) {
+            warm = hist.predict(kf, eps_track);
+            use_warm = !warm.empty();
+        } else if (have_prev) {
+            warm.assign(prev_vecs.begin(), prev_vecs.begin() + std::min(prev_vecs.size(), (size_t)nb));
+            use_warm = !warm.empty();
+        }
+        ThetaOp op(d, job->pol, bloch);
+        auto pc = build_precond(op, job->precond);
+        Eigensolver s(op, pc.get(), use_warm ? &warm : nullptr, nb, job->max_iter, job->tol, job->block_size,
+                      job->exit_rule);
+        SolveResult r = s.solve();
+        std::vector<Field> vecs = s.all_eigenvectors();
+        std::vector<double> om(r.eigenvalues.size());
+        for (size_t i = 0; i < om.size(); ++i) om[i] = r.eigenvalues[i] > 0 ? std::sqrt(r.eigenvalues[i]) : 0.0;
+        total_iters += r.iterations;
+        if (job->band_tracking && have_prev) {
+            std::vector<int> perm = track_bands(prev_vecs, vecs, prev_om, om, eps_track);
+            bool swaps = false;
+            for (size_t i = 0; i < perm.size(); ++i) if ((int)i != perm[i]) swaps = true;
+            if (swaps) {  // apply_permutation (band_tracking.rs:683-700)
+                size_t n = std::min(perm.size(), std::min(om.size(), vecs.size()));
+                auto om0 = om;
+                auto v0 = vecs;
+                for (size_t i = 0; i < n; ++i) {
+                    int src = perm[i];
+                    if (src < (int)om0.size()) om[i] = om0[src];
+                    if (src < (int)v0.size()) vecs[i] = v0[src];
+                }
+            }
+        }
+        prev_vecs = vecs;
+        prev_om = om;
+        have_prev = true;
+        if (job->subspace_prediction) hist.update(kf, vecs);
+        if (reuse && k == 0 && isg) { first_gamma = om; first_gamma_iters = r.iterations; have_first_gamma = true; }
+        for (int b = 0; b < nb; ++b) omegas_out[(size_t)k * nb + b] = b < (int)om.size() ? om[b] : NAN;
+        iters_out[k] = r.iterations;
+        conv_out[k] = r.converged ? 1 : 0;
+    }
+    return total_iters;
+}
+
+void to_fields(const double *in, size_t n, int count, std::vector<Field> &out) {
+    out.resize(count);
+    for (int c = 0; c < count; ++c) {
+        out[c].resize(n);
+        for (size_t i = 0; i < n; ++i) out[c][i] = cd(in[2 * (c * n + i)], in[2 * (c * n + i) + 1]);
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int oracle_bandstructure(const OracleJob *job, double *omegas, int *iters, int *converged) {
+    if (!job || job->nx <= 0 || job->ny <= 0 || job->n_bands <= 0) return -1;
+    return run_core(job, omegas, iters, converged);
+}
+
+int oracle_dielectric(const OracleJob *job, double *eps, double *inv_eps, double *tensor) {
+    Geometry g = geometry_from_job(job);
+    Dielectric d = build_dielectric(g, job->nx, job->ny, job->smoothing != 0);
+    size_t n = d.eps.size();
+    for (size_t i = 0; i < n; ++i) {
+        eps[i] = d.eps[i];
+        inv_eps[i] = d.inv_eps[i];
+        for (int t = 0; t < 4; ++t)
+            tensor[4 * i + t] = d.tensor.empty() ? (t == 0 || t == 3 ? d.inv_eps[i] : 0.0) : d.tensor[i][t];
+    }
+    return (int)n;
+}
+
+void oracle_fft2d(int nx, int ny, int count, double *data, int dir) {
+    for (int c = 0; c < count; ++c)
+        fft2d(nx, ny, reinterpret_cast<cd *>(data) + (size_t)c * nx * ny, dir > 0);
+}
+
+int oracle_operator_apply(const OracleJob *job, const double kfrac[2], int which, int count,
+                          const double *in, double *out) {
+    Geometry g = geometry_from_job(job);
+    Dielectric d = build_dielectric(g, job->nx, job->ny, job->smoothing != 0);
+    double bloch[2] = {kfrac[0] * d.b1[0] + kfrac[1] * d.b2[0], kfrac[0] * d.b1[1] + kfrac[1] * d.b2[1]};
+    ThetaOp op(d, job->pol, bloch);
+    auto pc = build_precond(op, job->precond);
+    size_t n = op.len();
+    std::vector<Field> v;
+    to_fields(in, n, count, v);
+    for (int c = 0; c < count; ++c) {
+        Field o;
+        if (which == 0) op.apply(v[c], o);
+        else if (which == 1) op.apply_mass(v[c], o);
+        else { o = v[c]; if (pc) pc->apply(o); }
+        for (size_t i = 0; i < n; ++i) { out[2 * (c * n + i)] = o[i].real(); out[2 * (c * n + i) + 1] = o[i].imag(); }
+    }
+    return 0;
+}
+
+void oracle_eigh(int n, const double *a, double *w, double *v) {
+    std::vector<double> ev;
+    std::vector<cd> V;
+    jacobi_eigh(n, reinterpret_cast<const cd *>(a), ev, V);
+    for (int i = 0; i < n; ++i) w[i] = ev[i];
+    std::memcpy(v, V.data(), sizeof(cd) * (size_t)n * n);
+}
+
+int oracle_svqb(int len, int p, double *x, double *bx) {
+    std::vector<Field> X, BX;
+    to_fields(x, len, p, X);
+    to_fields(bx, len, p, BX);
+    int r = svqb(X, BX);
+    for (int c = 0; c < p; ++c)
+        for (int i = 0; i < len; ++i) {
+            x[2 * ((size_t)c * len + i)] = X[c][i].real(); x[2 * ((size_t)c * len + i) + 1] = X[c][i].imag();
+            bx[2 * ((size_t)c * len + i)] = BX[c][i].real(); bx[2 * ((size_t)c * len + i) + 1] = BX[c][i].imag();
+        }
+    return r;
+}
+
+double oracle_polar(int m, const double *o, double *u) {
+    std::vector<cd> O(reinterpret_cast<const cd *>(o), reinterpret_cast<const cd *>(o) + (size_t)m * m), U;
+    double s = polar(m, O, U);
+    std::memcpy(u, U.data(), sizeof(cd) * (size_t)m * m);
+    return s;
+}
+
+void oracle_hungarian(int n, const double *w, int *perm) {
+    std::vector<std::vector<double>> W(n, std::vector<double>(n));
+    for (int i = 0; i < n; ++i) for (int j = 0; j < n; ++j) W[i][j] = w[i * n + j];
+    auto p = hungarian(W);
+    for (int i = 0; i < n; ++i) perm[i] = p[i];
+}
+
+int oracle_lobpcg_diagonal(int len, const double *diag, int n_bands, int max_iter, double tol,
+                           int use_precond, double *eigenvalues) {
+    DiagOp op;
+    op.diag.assign(diag, diag + len);
+    struct DiagPre : Preconditioner {
+        std::vector<double> inv;
+        void apply(Field &b) override { for (size_t i = 0; i < b.size(); ++i) b[i] *= inv[i]; }
+    } pre;
+    pre.inv.resize(len);
+    for (int i = 0; i < len; ++i) pre.inv[i] = 1.0 / diag[i];
+    Eigensolver s(op, use_precond ? &pre : nullptr, nullptr, n_bands, max_iter, tol, 0, 0);
+    SolveResult r = s.solve();
+    for (size_t i = 0; i < r.eigenvalues.size(); ++i) eigenvalues[i] = r.eigenvalues[i];
+    return r.converged ? r.iterations : -r.iterations;
+}
+
+}  // extern "C"
