@@ -1,0 +1,202 @@
+"""Band-structure jobs on the HIP backend.
+
+Mirrors ``BandStructureJob`` / ``RunOptions`` / ``run_with_options``
+(crates/core/src/drivers/bandstructure.rs:98-351), ``Geometry2D`` /
+``BasisAtom`` (crates/core/src/geometry.rs), ``Lattice2D``
+(crates/core/src/lattice.rs:143-180) and the k-path presets
+(crates/core/src/brillouin.rs:220-264, symmetry.rs:57-101).  Frequencies are
+returned as omega = sqrt(lambda) like ``BandStructureResult.bands``.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from . import _lib as L
+from .backend import HipBackend
+
+
+@dataclass
+class Lattice2D:
+    a1: tuple
+    a2: tuple
+
+    @staticmethod
+    def square(a: float = 1.0) -> "Lattice2D":
+        return Lattice2D((a, 0.0), (0.0, a))
+
+    @staticmethod
+    def hexagonal(a: float = 1.0) -> "Lattice2D":
+        return Lattice2D((a, 0.0), (0.5 * a, np.sqrt(3.0) * 0.5 * a))
+
+    def reciprocal(self):
+        det = self.a1[0] * self.a2[1] - self.a1[1] * self.a2[0]
+        inv = 2.0 * np.pi / det
+        return (self.a2[1] * inv, -self.a2[0] * inv), (-self.a1[1] * inv, self.a1[0] * inv)
+
+
+@dataclass
+class BasisAtom:
+    pos: tuple
+    radius: float
+    eps_inside: float = 1.0
+
+
+@dataclass
+class Geometry2D:
+    lattice: Lattice2D
+    eps_bg: float = 12.0
+    atoms: List[BasisAtom] = field(default_factory=list)
+
+    def to_c(self) -> L.Geometry:
+        g = L.Geometry()
+        g.a1[0], g.a1[1] = self.lattice.a1
+        g.a2[0], g.a2[1] = self.lattice.a2
+        g.eps_bg = self.eps_bg
+        if len(self.atoms) > L.MAX_ATOMS:
+            raise ValueError("at most 8 atoms")
+        g.n_atoms = len(self.atoms)
+        for i, a in enumerate(self.atoms):
+            g.atoms[i][0], g.atoms[i][1] = a.pos
+            g.atoms[i][2] = a.radius
+            g.atoms[i][3] = a.eps_inside
+        return g
+
+
+def k_path(lattice: str, segments_per_leg: int) -> np.ndarray:
+    """Square Gamma-X-M-Gamma or hexagonal Gamma-M-K-Gamma, linearly densified."""
+    if lattice == "square":
+        nodes = [(0.0, 0.0), (0.5, 0.0), (0.5, 0.5), (0.0, 0.0)]
+    elif lattice in ("hex", "hexagonal", "triangular"):
+        nodes = [(0.0, 0.0), (0.5, 0.0), (1.0 / 3.0, 1.0 / 3.0), (0.0, 0.0)]
+    else:
+        raise ValueError(lattice)
+    seg = max(int(segments_per_leg), 1)
+    path = [nodes[0]]
+    for a, b in zip(nodes[:-1], nodes[1:]):
+        for step in range(1, seg + 1):
+            t = step / seg
+            p = ((1.0 - t) * a[0] + t * b[0], (1.0 - t) * a[1] + t * b[1])
+            if path[-1] != p:
+                path.append(p)
+    return np.array(path, dtype=np.float64)
+
+
+@dataclass
+class RunOptions:
+    """bandstructure.rs:98-280 RunOptions (plus the solver exit rule)."""
+
+    precond: int = -1                  # -1 auto: TM fourier-diagonal, TE transverse projection
+    use_subspace_prediction: bool = True
+    use_extrapolation: bool = True
+    disable_band_tracking: bool = False
+    reuse_gamma: bool = True
+    exit_rule: int = 0                 # 0 Eigensolver::solve() accounting, 1 soft-locked bands only
+
+
+@dataclass
+class BandStructureJob:
+    geom: Geometry2D
+    nx: int
+    ny: int
+    pol: str
+    k_path: np.ndarray
+    n_bands: int = 8
+    tol: float = 1e-6
+    max_iter: int = 200
+    block_size: int = 0
+    smoothing: bool = True
+
+    def to_c(self, opts: RunOptions) -> L.Job:
+        j = L.Job()
+        j.geom = self.geom.to_c()
+        j.nx, j.ny = self.nx, self.ny
+        j.pol = 0 if self.pol.upper() == "TM" else 1
+        j.n_bands = self.n_bands
+        j.max_iter = self.max_iter
+        j.tol = self.tol
+        j.block_size = self.block_size
+        j.smoothing = int(self.smoothing)
+        j.precond = opts.precond
+        j.subspace_prediction = int(opts.use_subspace_prediction)
+        j.extrapolation = int(opts.use_extrapolation)
+        j.band_tracking = int(not opts.disable_band_tracking)
+        j.reuse_gamma = int(opts.reuse_gamma)
+        j.exit_rule = opts.exit_rule
+        self._kp = np.ascontiguousarray(self.k_path, dtype=np.float64)
+        if self._kp.ndim != 2 or self._kp.shape[1] != 2 or self._kp.shape[0] < 1:
+            raise ValueError("k_path must be an (nk, 2) array")
+        j.nk = self._kp.shape[0]
+        j.kpath = self._kp.ctypes.data_as(L.c_dp)
+        return j
+
+
+@dataclass
+class BandStructureResult:
+    k_path: np.ndarray
+    bands: np.ndarray           # [nk, n_bands] omega
+    iterations: np.ndarray      # [nk]
+    converged: np.ndarray       # [nk]
+
+
+@dataclass
+class BatchStats:
+    steps: int
+    iterations: int
+    solves: int
+    seconds: float
+    theta_ms: float
+    theta_launches: int
+
+
+def run_batch(backend: HipBackend, jobs: Sequence[BandStructureJob], options: Optional[RunOptions] = None,
+              slots: int = 64, precision: int = 0, profile: bool = False):
+    """Runs many band structures concurrently on one GPU (all on one grid)."""
+    opts = options or RunOptions()
+    jobs = list(jobs)
+    if not jobs:
+        return [], BatchStats(0, 0, 0, 0.0, 0.0, 0)
+    arr = (L.Job * len(jobs))()
+    for i, j in enumerate(jobs):
+        arr[i] = j.to_c(opts)
+    tot_om = sum(j.k_path.shape[0] * j.n_bands for j in jobs)
+    tot_k = sum(j.k_path.shape[0] for j in jobs)
+    om = np.zeros(tot_om)
+    it = np.zeros(tot_k, dtype=np.int32)
+    cv = np.zeros(tot_k, dtype=np.int32)
+    ro = L.RunOptions(slots, precision, int(profile))
+    st = L.RunStats()
+    L.check(L.lib().bh_bandstructure_run(backend.handle, arr, len(jobs), ctypes.byref(ro), om.ctypes.data_as(L.c_dp),
+                                         it.ctypes.data_as(L.c_ip), cv.ctypes.data_as(L.c_ip), ctypes.byref(st)),
+            "bh_bandstructure_run")
+    out = []
+    o = k = 0
+    for j in jobs:
+        nk = j.k_path.shape[0]
+        out.append(BandStructureResult(j.k_path, om[o:o + nk * j.n_bands].reshape(nk, j.n_bands).copy(),
+                                       it[k:k + nk].copy(), cv[k:k + nk].copy()))
+        o += nk * j.n_bands
+        k += nk
+    stats = BatchStats(st.steps, st.iterations, st.solves, st.seconds, st.theta_ms, st.theta_launches)
+    return out, stats
+
+
+def run_with_options(backend: HipBackend, job: BandStructureJob, options: Optional[RunOptions] = None):
+    """bandstructure::run_with_options for one job (bandstructure.rs:345-351)."""
+    res, _ = run_batch(backend, [job], options, slots=1)
+    return res[0]
+
+
+def dielectric(geom: Geometry2D, nx: int, ny: int, smoothing: bool = True):
+    """Dielectric2D::from_geometry (dielectric.rs:146-210): (eps, inv_eps, tensor[N,4])."""
+    g = geom.to_c()
+    n = nx * ny
+    eps = np.zeros(n)
+    inv = np.zeros(n)
+    ten = np.zeros((n, 4))
+    L.check(L.lib().bh_dielectric(ctypes.byref(g), nx, ny, int(smoothing), eps.ctypes.data_as(L.c_dp),
+                                  inv.ctypes.data_as(L.c_dp), ten.ctypes.data_as(L.c_dp)), "bh_dielectric")
+    return eps, inv, ten

@@ -1,0 +1,51 @@
+// Elementwise and reduction helpers for the band-solve engine and the
+// trait-level buffer API.
+#pragma once
+
+#include "common.h"
+
+namespace bh {
+
+// r_c = A x_{act[c]} - lambda_{act[c]} B x_{act[c]}   (eigensolver.rs:707-738)
+template <typename T>
+struct ResidualWork {
+    const cplx<T> *ax;
+    const cplx<T> *bx;
+    cplx<T> *out;
+    const int *act;       // band index of each active column
+    const double *lam;
+    long stride;          // column stride for ax, bx, out
+    int count, pad;
+};
+
+// lambda_c = Re(x^H A x) / Re(x^H B x)   (initialization.rs:52-57)
+template <typename T>
+struct RayleighWork {
+    const cplx<T> *x;
+    const cplx<T> *ax;
+    const cplx<T> *bx;
+    double *lam;
+    long stride;
+    int count, pad;
+};
+
+template <typename T>
+void launch_residual(const ResidualWork<T> *dwork, int nwork, int maxcount, long rows, hipStream_t st);
+template <typename T>
+void launch_rayleigh(const RayleighWork<T> *dwork, int nwork, int maxcount, long rows, hipStream_t st);
+// xorshift start vectors, phase = first_phase + column (initialization.rs:113-125)
+template <typename T>
+void launch_random(cplx<T> *out, int ncols, long rows, double first_phase, hipStream_t st);
+// out = value (complex), optional bout = value * w
+template <typename T>
+void launch_fill(cplx<T> *out, cplx<T> *bout, const T *w, double re, double im, long rows, hipStream_t st);
+
+// trait-level ops (backend-cpu/src/lib.rs:414-443)
+template <typename T>
+void launch_scale(cplx<T> *x, long n, double re, double im, hipStream_t st);
+template <typename T>
+void launch_axpy(const cplx<T> *x, cplx<T> *y, long n, double re, double im, hipStream_t st);
+template <typename T>
+void launch_dot(const cplx<T> *x, const cplx<T> *y, long n, double *out2, hipStream_t st);
+
+}  // namespace bh

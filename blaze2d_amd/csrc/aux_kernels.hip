@@ -1,0 +1,167 @@
+// Elementwise and reduction helpers (residuals, Rayleigh quotients, start
+// vectors, trait-level scale/axpy/dot).
+#include "aux_kernels.h"
+
+namespace bh {
+
+template <typename T>
+__global__ void residual_kernel(const ResidualWork<T> *__restrict__ work, long rows) {
+    const ResidualWork<T> &w = work[blockIdx.z];
+    const int c = blockIdx.y;
+    if (c >= w.count) return;
+    const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    const int b = w.act[c];
+    const double lam = w.lam[b];
+    cplx<T> a = w.ax[b * w.stride + r], m = w.bx[b * w.stride + r];
+    // backend axpy with alpha = (-lambda, 0) in storage precision
+    T al = (T)(-lam);
+    w.out[c * w.stride + r] = {a.x + al * m.x, a.y + al * m.y};
+}
+
+template <typename T>
+void launch_residual(const ResidualWork<T> *dwork, int nwork, int maxcount, long rows, hipStream_t st) {
+    if (nwork <= 0 || maxcount <= 0) return;
+    hipLaunchKernelGGL((residual_kernel<T>), dim3((rows + 255) / 256, maxcount, nwork), dim3(256), 0, st, dwork,
+                       rows);
+    BH_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) rayleigh_kernel(const RayleighWork<T> *__restrict__ work, long rows) {
+    const RayleighWork<T> &w = work[blockIdx.y];
+    const int c = blockIdx.x;
+    if (c >= w.count) return;
+    const cplx<T> *x = w.x + c * w.stride, *ax = w.ax + c * w.stride, *bx = w.bx + c * w.stride;
+    double num = 0, den = 0;
+    for (long r = threadIdx.x; r < rows; r += 256) {
+        cplx<T> xv = x[r], av = ax[r], bv = bx[r];
+        num += (double)xv.x * av.x + (double)xv.y * av.y;
+        den += (double)xv.x * bv.x + (double)xv.y * bv.y;
+    }
+    __shared__ double sn[256], sd[256];
+    sn[threadIdx.x] = num;
+    sd[threadIdx.x] = den;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            sn[threadIdx.x] += sn[threadIdx.x + s];
+            sd[threadIdx.x] += sd[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) w.lam[c] = fabs(sd[0]) > 1e-15 ? sn[0] / sd[0] : 0.0;
+}
+
+template <typename T>
+void launch_rayleigh(const RayleighWork<T> *dwork, int nwork, int maxcount, long rows, hipStream_t st) {
+    if (nwork <= 0 || maxcount <= 0) return;
+    hipLaunchKernelGGL((rayleigh_kernel<T>), dim3(maxcount, nwork), dim3(256), 0, st, dwork, rows);
+    BH_HIP_CHECK(hipGetLastError());
+}
+
+// state = bits(phase) * 0x9E3779B97F4A7C15 ; xorshift 13/7/17 ; (s >> 12) / 2^52 * 2 - 1
+template <typename T>
+__global__ void random_kernel(cplx<T> *out, int ncols, long rows, double first_phase) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncols) return;
+    double phase = first_phase + (double)c;
+    unsigned long long s = (unsigned long long)__double_as_longlong(phase) * 0x9E3779B97F4A7C15ULL;
+    if (s == 0) s = 0xDEADBEEFCAFEBABEULL;
+    cplx<T> *o = out + (long)c * rows;
+    for (long r = 0; r < rows; ++r) {
+        s ^= s << 13;
+        s ^= s >> 7;
+        s ^= s << 17;
+        double re = (double)(s >> 12) / 4503599627370496.0 * 2.0 - 1.0;
+        o[r] = {(T)re, (T)0};
+    }
+}
+
+template <typename T>
+void launch_random(cplx<T> *out, int ncols, long rows, double first_phase, hipStream_t st) {
+    if (ncols <= 0) return;
+    hipLaunchKernelGGL((random_kernel<T>), dim3((ncols + 63) / 64), dim3(64), 0, st, out, ncols, rows, first_phase);
+    BH_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+__global__ void fill_kernel(cplx<T> *out, cplx<T> *bout, const T *w, double re, double im, long rows) {
+    long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    cplx<T> v = {(T)re, (T)im};
+    out[r] = v;
+    if (bout) bout[r] = {v.x * w[r], v.y * w[r]};
+}
+
+template <typename T>
+void launch_fill(cplx<T> *out, cplx<T> *bout, const T *w, double re, double im, long rows, hipStream_t st) {
+    hipLaunchKernelGGL((fill_kernel<T>), dim3((rows + 255) / 256), dim3(256), 0, st, out, bout, w, re, im, rows);
+    BH_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+__global__ void scale_kernel(cplx<T> *x, long n, double re, double im) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    cplx<T> a = {(T)re, (T)im};
+    x[i] = cmul(x[i], a);
+}
+template <typename T>
+__global__ void axpy_kernel(const cplx<T> *x, cplx<T> *y, long n, double re, double im) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    cplx<T> a = {(T)re, (T)im};
+    y[i] = cadd(y[i], cmul(a, x[i]));
+}
+template <typename T>
+__global__ void __launch_bounds__(256) dot_kernel(const cplx<T> *x, const cplx<T> *y, long n, double *out) {
+    double re = 0, im = 0;
+    for (long i = threadIdx.x; i < n; i += 256) {
+        double a = x[i].x, b = x[i].y, c = y[i].x, d = y[i].y;
+        re += a * c + b * d;
+        im += a * d - b * c;
+    }
+    __shared__ double sr[256], si[256];
+    sr[threadIdx.x] = re;
+    si[threadIdx.x] = im;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            sr[threadIdx.x] += sr[threadIdx.x + s];
+            si[threadIdx.x] += si[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { out[0] = sr[0]; out[1] = si[0]; }
+}
+
+template <typename T>
+void launch_scale(cplx<T> *x, long n, double re, double im, hipStream_t st) {
+    hipLaunchKernelGGL((scale_kernel<T>), dim3((n + 255) / 256), dim3(256), 0, st, x, n, re, im);
+    BH_HIP_CHECK(hipGetLastError());
+}
+template <typename T>
+void launch_axpy(const cplx<T> *x, cplx<T> *y, long n, double re, double im, hipStream_t st) {
+    hipLaunchKernelGGL((axpy_kernel<T>), dim3((n + 255) / 256), dim3(256), 0, st, x, y, n, re, im);
+    BH_HIP_CHECK(hipGetLastError());
+}
+template <typename T>
+void launch_dot(const cplx<T> *x, const cplx<T> *y, long n, double *out2, hipStream_t st) {
+    hipLaunchKernelGGL((dot_kernel<T>), dim3(1), dim3(256), 0, st, x, y, n, out2);
+    BH_HIP_CHECK(hipGetLastError());
+}
+
+#define BH_INST(T)                                                                                         \
+    template void launch_residual<T>(const ResidualWork<T> *, int, int, long, hipStream_t);                 \
+    template void launch_rayleigh<T>(const RayleighWork<T> *, int, int, long, hipStream_t);                 \
+    template void launch_random<T>(cplx<T> *, int, long, double, hipStream_t);                              \
+    template void launch_fill<T>(cplx<T> *, cplx<T> *, const T *, double, double, long, hipStream_t);       \
+    template void launch_scale<T>(cplx<T> *, long, double, double, hipStream_t);                            \
+    template void launch_axpy<T>(const cplx<T> *, cplx<T> *, long, double, double, hipStream_t);            \
+    template void launch_dot<T>(const cplx<T> *, const cplx<T> *, long, double *, hipStream_t);
+BH_INST(double)
+BH_INST(float)
+#undef BH_INST
+
+}  // namespace bh

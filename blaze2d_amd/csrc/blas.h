@@ -1,0 +1,78 @@
+// Batched tall-skinny dense kernels (Gram, linear combination, residual) and
+// the batched small Hermitian eigensolver.
+#pragma once
+
+#include "common.h"
+
+namespace bh {
+
+// G(i, j) = sum_r conj(A_i[r]) w[r] B_j[r], partial sums per row chunk.
+template <typename T>
+struct GramWork {
+    const cplx<T> *A;
+    const cplx<T> *B;
+    const T *weight;        // optional real weight per row (eps for B-inner products)
+    const cplx<T> *const *a_ptrs;  // optional per-column pointers for A (else A + c*a_stride)
+    const cplx<T> *const *b_ptrs;  // optional per-column pointers for B
+    const int *dev_na;      // optional device-side counts
+    const int *dev_nb;
+    double *partial;        // [nchunks][ld][ld] complex (interleaved re, im)
+    long a_stride, b_stride;
+    int na, nb, ld, pad;
+};
+
+// out_j = sum_i A_i C(i, j)            (mode 0)
+// out_j = base_j - sum_i A_i C(i, j)   (mode 1; base may alias out)
+// optional out2_j = w * out_j          (mass output, TM)
+template <typename T>
+struct LincombWork {
+    const cplx<T> *A;
+    const cplx<T> *const *a_ptrs;  // optional per-column pointers for A
+    const int *dev_na;
+    const int *dev_nq;
+    const double *C;        // complex coefficients, C[i + j*ldc]
+    const cplx<T> *base;
+    cplx<T> *out;
+    cplx<T> *out2;
+    const T *w2;
+    long a_stride, base_stride, out_stride;
+    int na, nq, ldc, mode;
+};
+
+// Batched Hermitian eigenproblems on row-major partial Gram sums.
+struct EighWork {
+    const double *partial;  // [nchunks][ld][ld] complex
+    double *w;              // eigenvalues (ascending for RR, kept descending for SVQB)
+    double *V;              // RR: eigenvectors col-major (leading dim ldv); SVQB: T col-major n x rank
+    double *Vw;             // RR: copy of columns [cnt_x, cnt_x + cnt_w) (history directions W)
+    int *rank_out;          // SVQB: output rank; RR: n
+    int *cnt_x;             // RR: min(nsplit, n)
+    int *cnt_w;             // RR: min(2 nsplit, n) - cnt_x
+    const int *dev_n;
+    int n, nchunks, ld, mode;  // mode 0 = Rayleigh-Ritz, 1 = SVQB
+    int ldv, nsplit;
+};
+
+constexpr int GRAM_MAXC = 48;  // maximum columns per Gram operand
+constexpr int EIGH_MAXN = 64;
+
+template <typename T>
+void launch_gram(const GramWork<T> *dwork, int nwork, int nchunks, long rows, int maxa, int maxb,
+                 hipStream_t st);
+void launch_gram_reduce(const double *partial, double *out, int nwork, int nchunks, int ld, long part_stride,
+                        long out_stride, hipStream_t st);
+// Batched reduce with per-item partial/output pointers and an optional
+// uint8 mask (col-major ld x ld) zeroing selected coefficients.
+struct ReduceWork {
+    const double *partial;
+    double *out;
+    const unsigned char *mask;
+    int nchunks, pad;
+};
+void launch_reduce(const ReduceWork *dwork, int nwork, int ld, hipStream_t st);
+template <typename T>
+void launch_lincomb(const LincombWork<T> *dwork, int nwork, long rows, int maxq, hipStream_t st);
+void launch_eigh(const EighWork *dwork, int nwork, int maxn, hipStream_t st);
+size_t eigh_shared_bytes(int maxn);
+
+}  // namespace bh

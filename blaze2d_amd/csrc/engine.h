@@ -1,0 +1,61 @@
+// Batched band-structure engine: many LOBPCG instances ("slots") advance in
+// lockstep steps on one GPU; the host keeps the reference's per-instance
+// control logic, the GPU does every O(N) operation.
+#pragma once
+
+#include <array>
+#include <memory>
+#include <vector>
+
+#include "hostmath.h"
+
+namespace bh {
+
+struct JobSpec {
+    GeometrySpec geom;
+    int nx = 0, ny = 0;
+    int pol = 0;            // 0 TM, 1 TE
+    int n_bands = 8;
+    int max_iter = 200;
+    double tol = 1e-6;
+    int block_size = 0;     // 0 -> n_bands + 2
+    int smoothing = 1;
+    int precond = -1;       // -1 auto (TM FD, TE TP), 0 none, 1 FD, 2 TP
+    int prediction = 1;
+    int extrapolation = 1;
+    int tracking = 1;
+    int reuse_gamma = 1;
+    int exit_rule = 0;      // 0 solve() accounting, 1 soft-locked only
+    std::vector<std::array<double, 2>> kpath;
+};
+
+struct JobResult {
+    std::vector<double> omegas;  // nk * n_bands
+    std::vector<int> iters, converged;
+};
+
+struct RunStats {
+    long steps = 0;
+    long iterations = 0;          // total LOBPCG iterations over all solves
+    long solves = 0;              // k-point solves (excluding reused closing Gamma)
+    double seconds = 0;
+    double theta_ms = 0;          // accumulated operator-kernel time (events), if profiled
+    long theta_launches = 0;
+    long theta_columns = 0;
+};
+
+struct EngineOptions {
+    int slots = 64;         // concurrent band structures
+    int precision = 0;      // 0 fp64, 1 fp32 storage (reserved)
+    int profile = 0;        // time operator kernels with events
+};
+
+class EngineBase {
+  public:
+    virtual ~EngineBase() {}
+    virtual void run(const std::vector<JobSpec> &jobs, std::vector<JobResult> &out, RunStats &stats) = 0;
+};
+
+std::unique_ptr<EngineBase> make_engine(int nx, int ny, int nb_max, const EngineOptions &opt);
+
+}  // namespace bh

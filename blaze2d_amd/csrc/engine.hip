@@ -1,0 +1,1187 @@
+// Batched band-structure engine for one MI355X.
+//
+// A "slot" holds one band structure (bandstructure::run_core,
+// drivers/bandstructure.rs:381-885) while it walks its k-path.  All slots
+// advance together in steps; one step runs one LOBPCG iteration
+// (eigensolver.rs:1684-2192) for every iterating slot with batched kernels,
+// plus the k-point transitions (band tracking, subspace prediction, block
+// initialisation) of slots that just converged.  The host keeps the
+// reference's per-instance control (convergence by relative eigenvalue
+// change, soft locking with its degeneracy guard, the solve() exit rule)
+// and the O(nb^3) transition algebra on small Gram matrices; every O(N)
+// operation runs on the GPU.  One host synchronisation per step.
+//
+// Per-slot device layout (columns of N complex values, storage precision T):
+//   V  [C + 1] : Z = [X (m) | W (w) | P (a)];  column C holds the Gamma mode y0
+//   BV [C + 1] : B V (TM only; TE aliases V)
+//   AV [m0]    : A X
+//   Q, BQ, AQ [C] : SVQB output basis and its B / A images
+//   H  [2 nb]  : subspace history (previous / current converged eigenvectors)
+// W is written at columns [nb, nb + w) by the Ritz update.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "aux_kernels.h"
+#include "blas.h"
+#include "engine.h"
+#include "fft_ops.h"
+
+namespace bh {
+
+namespace {
+
+// Pinned staging arena mirrored on the device: descriptors for one step are
+// written on the host and uploaded with a single copy.
+struct Staging {
+    char *h = nullptr, *d = nullptr;
+    size_t cap = 0, off = 0;
+    void init(size_t bytes) {
+        cap = bytes;
+        BH_HIP_CHECK(hipHostMalloc((void **)&h, cap, hipHostMallocDefault));
+        BH_HIP_CHECK(hipMalloc((void **)&d, cap));
+    }
+    void release() {
+        if (h) (void)hipHostFree(h);
+        if (d) (void)hipFree(d);
+        h = d = nullptr;
+    }
+    template <typename X>
+    X *alloc(size_t n, X **dev) {
+        off = (off + 63) & ~size_t(63);
+        if (off + n * sizeof(X) > cap) throw std::runtime_error("staging arena exhausted");
+        X *hp = reinterpret_cast<X *>(h + off);
+        *dev = reinterpret_cast<X *>(d + off);
+        off += n * sizeof(X);
+        return hp;
+    }
+    void upload(hipStream_t st) {
+        if (off) BH_HIP_CHECK(hipMemcpyAsync(d, h, off, hipMemcpyHostToDevice, st));
+    }
+};
+
+template <typename T>
+T *dalloc(size_t n) {
+    T *p = nullptr;
+    if (n == 0) n = 1;
+    BH_HIP_CHECK(hipMalloc((void **)&p, n * sizeof(T)));
+    BH_HIP_CHECK(hipMemset(p, 0, n * sizeof(T)));
+    return p;
+}
+
+enum SlotState { FREE = 0, GRAM_PENDING = 1, ITER = 2 };
+
+struct Slot {
+    int state = FREE;
+    int job = -1;
+    int k = 0;
+    int nk = 0;
+    // job constants
+    int nb = 0, m0 = 0, pol = 0, precond = 0, exit_rule = 0, max_iter = 0;
+    double tol = 0;
+    bool prediction = true, extrapolation = true, tracking = true, reuse_gamma = true;
+    DielectricTables diel;
+    double y0val = 0;  // Gamma mode value 1 / ||1||_B
+    // per-solve state
+    int it = 0, m = 0, nW = 0;
+    std::vector<char> soft;
+    std::vector<double> lam, lam_prev;
+    bool lam_prev_valid = false;
+    bool gamma_defl = false;
+    OpParams params{};
+    // finished solve awaiting transition
+    std::vector<double> res_eigs;
+    int res_iters = 0;
+    bool res_conv = false;
+    std::vector<int> E_cols;  // column id in V (C == y0)
+    // K layout of the pending transition Gram
+    int kd = 0, nE = 0, nP = 0, nR = 0, rb_index = -1;
+    // tracking / history
+    bool have_prev = false;
+    std::vector<double> prev_om;
+    int hist_cur = -1;        // history buffer (0/1) holding the current eigenvectors
+    double cur_k[2] = {0, 0}, prev_k[2] = {0, 0};
+    bool hist_prev_valid = false;
+    // first Gamma reuse
+    bool have_first_gamma = false;
+    std::vector<double> first_gamma;
+    int first_gamma_iters = 0;
+    // planned transition coefficients (set in decide, consumed in plan)
+    bool t2_pending = false;
+    std::vector<zd> cx;       // kd x m0 coefficients over the physical transition basis
+    std::vector<int> eperm;   // permuted eigenvector columns for the history copy
+    int t2_kd = 0;
+    int t2_hist_new = -1;
+};
+
+bool is_gamma_frac(const std::array<double, 2> &k) { return k[0] * k[0] + k[1] * k[1] < 1e-8; }
+
+}  // namespace
+
+template <typename T>
+class Engine : public EngineBase {
+  public:
+    Engine(int nx_, int ny_, int nbmax_, const EngineOptions &opt) : nx(nx_), ny(ny_), nbmax(nbmax_), eopt(opt) {
+        if (!grid_supported(nx, ny)) throw std::runtime_error("unsupported grid size");
+        N = (long)nx * ny;
+        m0max = nbmax + 2;
+        C = std::max(2 * m0max, 3 * nbmax);
+        if (C > GRAM_MAXC) throw std::runtime_error("n_bands too large for the 48-column subspace kernels");
+        CV = C + 1;
+        S = std::max(1, opt.slots);
+        LD = GRAM_MAXC;
+        NRMAX = m0max + 2;
+        NCHMAX = (int)std::min<long>(64, std::max<long>(1, N / 32));
+        BH_HIP_CHECK(hipStreamCreate(&st));
+        V = dalloc<cplx<T>>((size_t)S * CV * N);
+        BV = dalloc<cplx<T>>((size_t)S * CV * N);
+        AV = dalloc<cplx<T>>((size_t)S * m0max * N);
+        Q = dalloc<cplx<T>>((size_t)S * C * N);
+        BQ = dalloc<cplx<T>>((size_t)S * C * N);
+        AQ = dalloc<cplx<T>>((size_t)S * C * N);
+        H = dalloc<cplx<T>>((size_t)S * 2 * nbmax * N);
+        RND = dalloc<cplx<T>>((size_t)NRMAX * N);
+        EPS = dalloc<T>((size_t)S * N);
+        TEN = dalloc<T>((size_t)S * 4 * N);
+        d_lam = dalloc<double>((size_t)S * LD);
+        d_svw = dalloc<double>((size_t)S * LD);
+        d_Y = dalloc<double>((size_t)S * LD * LD * 2);
+        d_Yw = dalloc<double>((size_t)S * LD * LD * 2);
+        d_T = dalloc<double>((size_t)S * LD * LD * 2);
+        d_coef = dalloc<double>((size_t)S * LD * LD * 2);
+        d_part = dalloc<double>((size_t)(2048 + 2 * S) * LD * LD * 2);
+        d_rank = dalloc<int>(S);
+        d_cnt = dalloc<int>(2 * (size_t)S);
+        d_gramrb = dalloc<double>((size_t)S * LD * LD * 2);
+        BH_HIP_CHECK(hipHostMalloc((void **)&h_lam, sizeof(double) * S * LD, hipHostMallocDefault));
+        BH_HIP_CHECK(hipHostMalloc((void **)&h_rank, sizeof(int) * S, hipHostMallocDefault));
+        BH_HIP_CHECK(hipHostMalloc((void **)&h_cnt, sizeof(int) * 2 * S, hipHostMallocDefault));
+        BH_HIP_CHECK(hipHostMalloc((void **)&h_gramrb, sizeof(double) * S * LD * LD * 2, hipHostMallocDefault));
+        stg.init((size_t)64 << 20);
+        // twiddle tables W_n^t = exp(-2 pi i t / n), computed in FP64 on the host
+        std::vector<cplx<T>> tx(nx), ty(ny);
+        for (int t = 0; t < nx; ++t) {
+            double a = -2.0 * M_PI * t / nx;
+            tx[t] = {(T)std::cos(a), (T)std::sin(a)};
+        }
+        for (int t = 0; t < ny; ++t) {
+            double a = -2.0 * M_PI * t / ny;
+            ty[t] = {(T)std::cos(a), (T)std::sin(a)};
+        }
+        twx = dalloc<cplx<T>>(nx);
+        twy = dalloc<cplx<T>>(ny);
+        BH_HIP_CHECK(hipMemcpy(twx, tx.data(), sizeof(cplx<T>) * nx, hipMemcpyHostToDevice));
+        BH_HIP_CHECK(hipMemcpy(twy, ty.data(), sizeof(cplx<T>) * ny, hipMemcpyHostToDevice));
+        launch_random<T>(RND, NRMAX, N, 1.0, st);
+        BH_HIP_CHECK(hipStreamSynchronize(st));
+        slots.resize(S);
+    }
+
+    ~Engine() override {
+        (void)hipStreamSynchronize(st);
+        for (void *p : std::initializer_list<void *>{V, BV, AV, Q, BQ, AQ, H, RND, EPS, TEN, d_lam, d_svw, d_Y, d_Yw,
+                                                     d_T, d_coef, d_part, d_rank, d_cnt, d_gramrb, twx, twy})
+            if (p) (void)hipFree(p);
+        (void)hipHostFree(h_lam);
+        (void)hipHostFree(h_rank);
+        (void)hipHostFree(h_cnt);
+        (void)hipHostFree(h_gramrb);
+        stg.release();
+        (void)hipStreamDestroy(st);
+    }
+
+    void run(const std::vector<JobSpec> &jobs, std::vector<JobResult> &out, RunStats &stats) override;
+
+  private:
+    int nx, ny;
+    long N;
+    int nbmax, m0max, C, CV, S, LD, NRMAX, NCHMAX;
+    EngineOptions eopt;
+    hipStream_t st{};
+    cplx<T> *V{}, *BV{}, *AV{}, *Q{}, *BQ{}, *AQ{}, *H{}, *RND{};
+    T *EPS{}, *TEN{};
+    double *d_lam{}, *d_svw{}, *d_Y{}, *d_Yw{}, *d_T{}, *d_coef{}, *d_part{}, *d_gramrb{};
+    int *d_rank{}, *d_cnt{};
+    double *h_lam{}, *h_gramrb{};
+    int *h_rank{}, *h_cnt{};
+    cplx<T> *twx{}, *twy{};
+    Staging stg;
+    std::vector<Slot> slots;
+    const std::vector<JobSpec> *jobs_{};
+    std::vector<JobResult> *out_{};
+    RunStats *stats_{};
+
+    // column addressing
+    cplx<T> *vcol(int s, int j) { return V + ((size_t)s * CV + j) * N; }
+    cplx<T> *bvcol(int s, int j) { return slots[s].pol == 1 ? vcol(s, j) : BV + ((size_t)s * CV + j) * N; }
+    cplx<T> *avcol(int s, int j) { return AV + ((size_t)s * m0max + j) * N; }
+    cplx<T> *qcol(int s, int j) { return Q + ((size_t)s * C + j) * N; }
+    cplx<T> *bqcol(int s, int j) { return slots[s].pol == 1 ? qcol(s, j) : BQ + ((size_t)s * C + j) * N; }
+    cplx<T> *aqcol(int s, int j) { return AQ + ((size_t)s * C + j) * N; }
+    cplx<T> *hcol(int s, int b, int j) { return H + ((size_t)s * 2 * nbmax + (size_t)b * nbmax + j) * N; }
+    cplx<T> *rcol(int j) { return RND + (size_t)j * N; }
+    T *eps(int s) { return EPS + (size_t)s * N; }
+    T *ten(int s) { return TEN + (size_t)s * 4 * N; }
+    double *coef(int s) { return d_coef + (size_t)s * LD * LD * 2; }
+
+    // --- per-step plans -----------------------------------------------------
+    struct GramItem { int slot; std::vector<const cplx<T> *> a, b; bool weighted; int na, nb; const int *dev_na = nullptr, *dev_nb = nullptr; };
+    struct StepPlan {
+        std::vector<int> fill_y0;            // new jobs: set the Gamma mode column
+        std::vector<int> table_upload;       // new jobs: upload eps / tensor tables
+        std::vector<int> t2;                 // transitions to apply (then iterate)
+        std::vector<int> iter;               // slots iterating this step
+        std::vector<int> t1;                 // transition Grams to compute
+    };
+
+    void start_job(int s, int job);
+    void schedule_t1(int s);
+    void transition_math(int s);
+    void absorb(int s);
+    void decide(int s);
+    void finish_solve(int s, bool conv, int iters);
+    void record_k(int s, int kidx, const std::vector<double> &om, int iters, bool conv);
+    void set_kpoint(int s);
+    void execute(StepPlan &plan);
+};
+
+template <typename T>
+void Engine<T>::record_k(int s, int kidx, const std::vector<double> &om, int iters, bool conv) {
+    Slot &sl = slots[s];
+    JobResult &r = (*out_)[sl.job];
+    for (int b = 0; b < sl.nb; ++b) r.omegas[(size_t)kidx * sl.nb + b] = b < (int)om.size() ? om[b] : NAN;
+    r.iters[kidx] = iters;
+    r.converged[kidx] = conv ? 1 : 0;
+}
+
+template <typename T>
+void Engine<T>::set_kpoint(int s) {
+    Slot &sl = slots[s];
+    const JobSpec &J = (*jobs_)[sl.job];
+    const auto &kf = J.kpath[sl.k];
+    OpParams &P = sl.params;
+    P.b1[0] = sl.diel.b1[0]; P.b1[1] = sl.diel.b1[1];
+    P.b2[0] = sl.diel.b2[0]; P.b2[1] = sl.diel.b2[1];
+    // lattice.rs:274-280 fractional_to_cartesian
+    P.bloch[0] = kf[0] * sl.diel.b1[0] + kf[1] * sl.diel.b2[0];
+    P.bloch[1] = kf[0] * sl.diel.b1[1] + kf[1] * sl.diel.b2[1];
+    P.pol = sl.pol;
+    P.gamma = (std::fabs(P.bloch[0]) < 1e-12 && std::fabs(P.bloch[1]) < 1e-12) ? 1 : 0;
+    P.precond = sl.precond;
+    P.shift = 0.5 * spectral_smin(nx, ny, P.b1, P.b2, P.bloch);  // SHIFT_SMIN_FRACTION
+    if (sl.pol == 0) {
+        P.eps_eff = sl.diel.eps_mean;
+        P.mass_floor = (std::isfinite(P.eps_eff) && P.eps_eff > 0) ? P.eps_eff * 1e-2 : 0.0;
+    } else {
+        P.eps_eff = sl.diel.eps_harm;
+        P.mass_floor = 0.0;
+    }
+    sl.gamma_defl = P.gamma != 0;
+}
+
+template <typename T>
+void Engine<T>::start_job(int s, int job) {
+    Slot &sl = slots[s];
+    const JobSpec &J = (*jobs_)[job];
+    sl = Slot();
+    sl.job = job;
+    sl.k = 0;
+    sl.nk = (int)J.kpath.size();
+    sl.nb = J.n_bands;
+    sl.m0 = std::max(J.block_size == 0 ? J.n_bands + 2 : J.block_size, std::max(J.n_bands, 1));
+    if (sl.nb > nbmax || sl.m0 > m0max) throw std::runtime_error("job n_bands exceeds engine capacity");
+    sl.pol = J.pol;
+    sl.precond = J.precond < 0 ? (J.pol == 0 ? 1 : 2) : J.precond;  // diagnostics.rs:158-166
+    sl.exit_rule = J.exit_rule;
+    sl.max_iter = J.max_iter;
+    sl.tol = J.tol;
+    sl.prediction = J.prediction != 0;
+    sl.extrapolation = J.extrapolation != 0;
+    sl.tracking = J.tracking != 0;
+    sl.reuse_gamma = J.reuse_gamma != 0;
+    sl.diel = build_dielectric(J.geom, nx, ny, J.smoothing != 0);
+    // create_gamma_mode (initialization.rs:166-224): u0 = 1, y0 = u0 / ||u0||_B
+    double nsq = sl.pol == 0 ? sl.diel.eps_sum : (double)N;
+    sl.y0val = 1.0 / (nsq > 0 ? std::sqrt(nsq) : 1.0);
+    set_kpoint(s);
+    sl.state = GRAM_PENDING;
+}
+
+// Transition Gram over K = [E (converged eigenvectors) | P (history current)
+// | R (start vectors) | y0], B-weighted.
+template <typename T>
+void Engine<T>::schedule_t1(int s) {
+    Slot &sl = slots[s];
+    sl.nE = (int)sl.E_cols.size();
+    sl.nP = (sl.hist_cur >= 0) ? sl.nb : 0;
+    sl.nR = sl.nE > 0 ? (sl.m0 - std::min(sl.nb, sl.m0) + 2) : sl.m0 + 2;
+    sl.nR = std::min(sl.nR, NRMAX);
+    sl.kd = sl.nE + sl.nP + sl.nR + 1;
+    if (sl.kd > LD) throw std::runtime_error("transition basis too wide");
+    sl.state = GRAM_PENDING;
+}
+
+template <typename T>
+void Engine<T>::finish_solve(int s, bool conv, int iters) {
+    Slot &sl = slots[s];
+    // collect_all_eigenvalues / all_eigenvectors (eigensolver.rs:2362-2437)
+    std::vector<std::pair<double, int>> pr;
+    if (sl.gamma_defl) pr.push_back({0.0, C});
+    for (int i = 0; i < (int)sl.lam.size() && i < sl.m; ++i) pr.push_back({sl.lam[i], i});
+    std::stable_sort(pr.begin(), pr.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
+    sl.res_eigs.clear();
+    sl.E_cols.clear();
+    for (int i = 0; i < (int)pr.size() && i < sl.nb; ++i) {
+        sl.res_eigs.push_back(pr[i].first);
+        sl.E_cols.push_back(pr[i].second);
+    }
+    sl.res_iters = iters;
+    sl.res_conv = conv;
+    stats_->iterations += iters;
+    stats_->solves += 1;
+    schedule_t1(s);
+}
+
+// Results of the previous step's Rayleigh-Ritz for an iterating slot.
+template <typename T>
+void Engine<T>::absorb(int s) {
+    Slot &sl = slots[s];
+    if (sl.it == 0) return;
+    int cx = h_cnt[2 * s], cw = h_cnt[2 * s + 1];
+    sl.lam_prev = sl.lam;
+    sl.lam_prev_valid = sl.it >= 2;
+    if (cx > 0) {
+        sl.lam.assign(h_lam + (size_t)s * LD, h_lam + (size_t)s * LD + cx);
+        sl.m = cx;
+    }
+    sl.nW = cw;
+}
+
+// Convergence and soft-locking decisions (eigensolver.rs:1750-1864).
+template <typename T>
+void Engine<T>::decide(int s) {
+    Slot &sl = slots[s];
+    const int it = sl.it;
+    if (it >= sl.max_iter) {
+        int nsoft = 0;
+        for (char c : sl.soft) nsoft += c ? 1 : 0;
+        finish_solve(s, (sl.gamma_defl ? 1 : 0) + nsoft >= sl.nb, sl.max_iter);
+        return;
+    }
+    const int nbc = std::min(sl.nb, sl.m0);
+    std::vector<double> ch;
+    int n_conv = 0;
+    if (it >= 2) {
+        size_t kk = std::min(sl.lam.size(), sl.lam_prev.size());
+        for (size_t i = 0; i < kk; ++i)
+            ch.push_back(std::fabs(sl.lam[i] - sl.lam_prev[i]) / std::max(std::fabs(sl.lam_prev[i]), 1e-10));
+        size_t ncheck = std::min((size_t)sl.m, ch.size());
+        for (size_t i = 0; i < ncheck && (int)i < nbc; ++i) n_conv += ch[i] < sl.tol ? 1 : 0;
+    }
+    int n_soft = 0;
+    for (char c : sl.soft) n_soft += c ? 1 : 0;
+    const int n_hard = sl.gamma_defl ? 1 : 0;
+    const int total = sl.exit_rule == 0 ? n_hard + n_soft + n_conv : n_soft;
+    if (total >= sl.nb) { finish_solve(s, true, it + 1); return; }
+    if (it >= 2) {
+        // soft_lock_bands with the 1e-4 degeneracy guard (eigensolver.rs:889-969)
+        std::vector<int> want;
+        size_t ncheck = std::min((size_t)sl.m, ch.size());
+        for (size_t i = 0; i < ncheck; ++i)
+            if (ch[i] < sl.tol) want.push_back((int)i);
+        if (!want.empty()) {
+            std::vector<char> lockset(sl.soft.size() + sl.lam.size() + 1, 0);
+            for (int b : want) lockset[b] = 1;
+            std::vector<int> safe;
+            for (int b : want) {
+                if (b >= (int)sl.lam.size() || b >= (int)sl.soft.size() || sl.soft[b]) continue;
+                double li = sl.lam[b];
+                bool ok = true;
+                for (int j = 0; j < (int)sl.lam.size(); ++j) {
+                    if (j == b || j >= (int)sl.soft.size() || sl.soft[j]) continue;
+                    double mx = std::max(std::max(std::fabs(li), std::fabs(sl.lam[j])), 1e-10);
+                    if (std::fabs(li - sl.lam[j]) / mx < 1e-4 && !lockset[j]) { ok = false; break; }
+                }
+                if (ok) safe.push_back(b);
+            }
+            int newly = 0;
+            for (int b : safe)
+                if (!sl.soft[b]) { sl.soft[b] = 1; ++newly; }
+            n_soft += newly;
+            if (newly > 0 && n_soft >= sl.nb) { finish_solve(s, true, it + 1); return; }
+        }
+    }
+    // continue iterating: the plan for this slot is built in execute()
+}
+
+// Band tracking, history update, subspace prediction and block initialisation
+// for the next k-point, all in coefficient space over the transition basis.
+template <typename T>
+void Engine<T>::transition_math(int s) {
+    Slot &sl = slots[s];
+    const JobSpec &J = (*jobs_)[sl.job];
+    const int kd = sl.kd, nE = sl.nE, nP = sl.nP, nR = sl.nR;
+    const int iE = 0, iP = nE, iR = nE + nP, iY = nE + nP + nR;
+    std::vector<zd> M((size_t)kd * kd);
+    const double *g = h_gramrb + (size_t)sl.rb_index * LD * LD * 2;
+    for (int j = 0; j < kd; ++j)
+        for (int i = 0; i < kd; ++i) M[i + (size_t)j * kd] = zd(g[2 * (i + (size_t)j * LD)], g[2 * (i + (size_t)j * LD) + 1]);
+    // Hermitian symmetrisation from the lower triangle
+    for (int j = 0; j < kd; ++j)
+        for (int i = 0; i < j; ++i) M[i + (size_t)j * kd] = std::conj(M[j + (size_t)i * kd]);
+    for (int i = 0; i < kd; ++i) M[i + (size_t)i * kd] = M[i + (size_t)i * kd].real();
+    CoefSpace cs{kd, &M};
+
+    std::vector<int> perm(nE);
+    std::iota(perm.begin(), perm.end(), 0);
+    bool next_solve = false;
+    int knext = sl.k + 1;
+    if (nE > 0) {
+        // finished solve at sl.k: omegas, tracking (bandstructure.rs:755-806)
+        std::vector<double> om(sl.res_eigs.size());
+        for (size_t i = 0; i < om.size(); ++i) om[i] = sl.res_eigs[i] > 0 ? std::sqrt(sl.res_eigs[i]) : 0.0;
+        if (sl.tracking && sl.have_prev && nP > 0) {
+            int m = std::min(nE, nP);
+            std::vector<zd> O((size_t)m * m);
+            for (int j = 0; j < m; ++j)
+                for (int i = 0; i < m; ++i) O[i + (size_t)j * m] = M[(iP + i) + (size_t)(iE + j) * kd];
+            std::vector<int> p = track_bands(m, O, sl.prev_om, om);
+            bool swaps = false;
+            for (int i = 0; i < (int)p.size(); ++i) swaps |= p[i] != i;
+            if (swaps) {
+                std::vector<double> om0 = om;
+                for (int i = 0; i < (int)p.size() && i < (int)om.size(); ++i) {
+                    om[i] = om0[p[i]];
+                    perm[i] = p[i];
+                }
+            }
+        }
+        sl.prev_om = om;
+        sl.have_prev = true;
+        record_k(s, sl.k, om, sl.res_iters, sl.res_conv);
+        if (sl.reuse_gamma && sl.k == 0 && is_gamma_frac(J.kpath[0])) {
+            sl.first_gamma = om;
+            sl.first_gamma_iters = sl.res_iters;
+            sl.have_first_gamma = true;
+        }
+        // closing Gamma reuse (bandstructure.rs:580-601)
+        bool reuse = sl.reuse_gamma && sl.nk > 1 && is_gamma_frac(J.kpath[0]) && is_gamma_frac(J.kpath[sl.nk - 1]);
+        if (knext == sl.nk - 1 && reuse && sl.have_first_gamma) {
+            record_k(s, knext, sl.first_gamma, sl.first_gamma_iters, true);
+            knext = sl.nk;
+        }
+        next_solve = knext < sl.nk;
+    } else {
+        knext = sl.k;  // first k-point of the job
+        next_solve = true;
+    }
+    if (!next_solve) {
+        sl.state = FREE;
+        sl.job = -1;
+        return;
+    }
+    // --- history update (subspace_prediction.rs:248-262) and prediction ---
+    std::vector<std::vector<zd>> warm;  // coefficient vectors over K (kd)
+    const bool had_cur = sl.hist_cur >= 0;
+    int new_hist = -1;
+    if (nE > 0) {
+        new_hist = had_cur ? 1 - sl.hist_cur : 0;
+        if (had_cur) { sl.prev_k[0] = sl.cur_k[0]; sl.prev_k[1] = sl.cur_k[1]; }
+        sl.hist_prev_valid = had_cur;
+        const auto &kc = J.kpath[sl.k];
+        sl.cur_k[0] = kc[0];
+        sl.cur_k[1] = kc[1];
+        const int nb = std::min(nE, sl.nb);
+        auto ecoef = [&](int i) {  // coefficient vector of the permuted eigenvector i
+            std::vector<zd> c(kd, 0.0);
+            c[iE + perm[i]] = 1.0;
+            return c;
+        };
+        if (!sl.prediction || !had_cur) {
+            for (int i = 0; i < nb; ++i) warm.push_back(ecoef(i));
+        } else {
+            // predict_with_rotation: O = prev^H B curr (curr = permuted E)
+            int m = std::min(nb, nP);
+            std::vector<zd> O((size_t)m * m), U;
+            for (int j = 0; j < m; ++j)
+                for (int i = 0; i < m; ++i) O[i + (size_t)j * m] = M[(iP + i) + (size_t)(iE + perm[j]) * kd];
+            double smin = polar(m, O, U);
+            const auto &kn = J.kpath[knext];
+            double dkp[2] = {sl.cur_k[0] - sl.prev_k[0], sl.cur_k[1] - sl.prev_k[1]};
+            double dkn[2] = {kn[0] - sl.cur_k[0], kn[1] - sl.cur_k[1]};
+            bool degenerate = smin < 0.1;                           // DEGENERACY_THRESHOLD
+            bool corner = dkp[0] * dkn[0] + dkp[1] * dkn[1] < 0.0;  // CORNER_DOT_THRESHOLD
+            bool extrap = sl.extrapolation && !degenerate && !corner;
+            double alpha = 0.0, ca = 1.0;
+            if (extrap) {
+                double np_ = std::sqrt(dkp[0] * dkp[0] + dkp[1] * dkp[1]);
+                double nn_ = std::sqrt(dkn[0] * dkn[0] + dkn[1] * dkn[1]);
+                alpha = np_ > 1e-12 ? nn_ / np_ : 1.0;
+                ca = 1.0 + alpha;
+            }
+            // pred_j = sum_i curr_i (ca conj(U(j,i))) - alpha prev_j
+            for (int j = 0; j < m; ++j) {
+                std::vector<zd> c(kd, 0.0);
+                for (int i = 0; i < m; ++i) {
+                    zd u = U[j + (size_t)i * m];
+                    c[iE + perm[i]] += zd(u.real() * ca, -u.imag() * ca);
+                }
+                if (extrap) c[iP + j] += zd(-alpha, 0.0);
+                warm.push_back(c);
+            }
+            // orthonormalize_fields: modified Gram-Schmidt in the B inner product
+            for (int i = 0; i < m; ++i) {
+                double nrm = std::sqrt(std::max(cs.inner(warm[i].data(), warm[i].data()).real(), 0.0));
+                if (nrm > 1e-14)
+                    for (auto &v : warm[i]) v /= nrm;
+                for (int j = i + 1; j < m; ++j) {
+                    zd c = cs.inner(warm[i].data(), warm[j].data());
+                    for (int q = 0; q < kd; ++q) warm[j][q] -= c * warm[i][q];
+                }
+            }
+        }
+    }
+    // --- initialize_block (initialization.rs:248-367) with Gamma deflation ---
+    sl.k = knext;
+    set_kpoint(s);
+    std::vector<std::vector<zd>> ent;
+    auto try_add = [&](std::vector<zd> v) {
+        for (auto &e : ent) {  // project_out: v -= <e, v>_B e
+            zd c = cs.inner(e.data(), v.data());
+            for (int q = 0; q < kd; ++q) v[q] -= c * e[q];
+        }
+        double nrm = std::sqrt(std::max(cs.inner(v.data(), v.data()).real(), 0.0));
+        if (!(nrm > 1e-12)) return false;
+        for (auto &x : v) x /= nrm;
+        ent.push_back(std::move(v));
+        return true;
+    };
+    for (size_t i = 0; i < warm.size() && (int)ent.size() < sl.m0; ++i) try_add(warm[i]);
+    int attempts = 0, r = 0;
+    while ((int)ent.size() < sl.m0 && attempts < sl.m0 * 8) {
+        if (r >= nR) throw std::runtime_error("ran out of start vectors during block initialisation");
+        std::vector<zd> v(kd, 0.0);
+        v[iR + r] = 1.0;
+        ++r;
+        if (!try_add(v)) ++attempts;
+    }
+    if (sl.gamma_defl) {
+        // deflation add_vector(y0) then project_single + renormalise each entry
+        std::vector<zd> y(kd, 0.0);
+        y[iY] = 1.0;
+        double yn = std::sqrt(std::max(cs.inner(y.data(), y.data()).real(), 0.0));
+        if (yn > 1e-12 && yn >= 1e-10)
+            for (auto &v : y) v /= yn;
+        for (auto &e : ent) {
+            zd c = cs.inner(y.data(), e.data());
+            for (int q = 0; q < kd; ++q) e[q] -= c * y[q];
+            double nrm = std::sqrt(std::max(cs.inner(e.data(), e.data()).real(), 0.0));
+            if (nrm > 1e-12)
+                for (auto &v : e) v /= nrm;
+        }
+    }
+    // physical basis for T2: [H_new (permuted E) | H_cur (P) | R | y0]
+    const int m0 = (int)ent.size();
+    sl.cx.assign((size_t)kd * std::max(m0, 1), 0.0);
+    for (int e = 0; e < m0; ++e)
+        for (int q = 0; q < kd; ++q) {
+            int pq = q;
+            if (q < nE) {
+                // E index q -> position i in the permuted order (perm[i] == q)
+                for (int i = 0; i < nE; ++i)
+                    if (perm[i] == q) { pq = i; break; }
+            }
+            sl.cx[pq + (size_t)e * kd] = ent[e][q];
+        }
+    sl.eperm.assign(nE, 0);
+    for (int i = 0; i < nE; ++i) sl.eperm[i] = sl.E_cols[perm[i]];
+    sl.t2_kd = kd;
+    sl.t2_hist_new = new_hist;
+    sl.t2_pending = true;
+    sl.it = 0;
+    sl.m = m0;
+    sl.nW = 0;
+    sl.soft.assign(m0, 0);
+    sl.lam.clear();
+    sl.lam_prev.clear();
+    sl.lam_prev_valid = false;
+    sl.state = ITER;
+}
+
+template <typename T>
+void Engine<T>::run(const std::vector<JobSpec> &jobs, std::vector<JobResult> &out, RunStats &stats) {
+    jobs_ = &jobs;
+    out_ = &out;
+    stats_ = &stats;
+    out.assign(jobs.size(), JobResult());
+    for (size_t j = 0; j < jobs.size(); ++j) {
+        const JobSpec &J = jobs[j];
+        if (J.nx != nx || J.ny != ny) throw std::runtime_error("job grid does not match engine grid");
+        out[j].omegas.assign(J.kpath.size() * J.n_bands, NAN);
+        out[j].iters.assign(J.kpath.size(), 0);
+        out[j].converged.assign(J.kpath.size(), 0);
+    }
+    for (auto &sl : slots) sl = Slot();
+    size_t next_job = 0;
+    auto t0 = std::chrono::steady_clock::now();
+    while (true) {
+        StepPlan plan;
+        // decisions from the previous step's readback
+        for (int s = 0; s < S; ++s) {
+            Slot &sl = slots[s];
+            if (sl.state == GRAM_PENDING && sl.rb_index >= 0) {
+                transition_math(s);
+                sl.rb_index = -1;
+                if (sl.state == ITER) plan.t2.push_back(s);
+            } else if (sl.state == ITER) {
+                absorb(s);
+                decide(s);
+            }
+        }
+        for (int s = 0; s < S; ++s) {
+            Slot &sl = slots[s];
+            if (sl.state == FREE && next_job < jobs.size()) {
+                start_job(s, (int)next_job++);
+                plan.table_upload.push_back(s);
+                plan.fill_y0.push_back(s);
+                sl.E_cols.clear();
+                schedule_t1(s);
+            }
+        }
+        bool any = false;
+        for (int s = 0; s < S; ++s) {
+            Slot &sl = slots[s];
+            if (sl.state == ITER) { plan.iter.push_back(s); any = true; }
+            else if (sl.state == GRAM_PENDING && sl.rb_index < 0) { plan.t1.push_back(s); any = true; }
+        }
+        if (!any) break;
+        execute(plan);
+        stats.steps++;
+    }
+    stats.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+template <typename T>
+void Engine<T>::execute(StepPlan &plan) {
+    stg.off = 0;
+    const bool need_b_eps = true;
+    (void)need_b_eps;
+    // OpParams for every slot (small), referenced by the operator work items
+    OpParams *dparams;
+    OpParams *hp = stg.alloc<OpParams>(S, &dparams);
+    for (int s = 0; s < S; ++s) hp[s] = slots[s].params;
+
+    // ---- new jobs: tables and the Gamma mode ----
+    struct TableCopy { T *dst; T *src; size_t n; };
+    std::vector<TableCopy> tcopies;
+    for (int s : plan.table_upload) {
+        Slot &sl = slots[s];
+        T *de, *dt;
+        T *he = stg.alloc<T>(N, &de);
+        T *ht = stg.alloc<T>(4 * N, &dt);
+        for (long i = 0; i < N; ++i) he[i] = (T)sl.diel.eps[i];
+        for (long i = 0; i < 4 * N; ++i) ht[i] = (T)sl.diel.tensor[i];
+        tcopies.push_back({eps(s), de, (size_t)N});
+        tcopies.push_back({ten(s), dt, (size_t)4 * N});
+    }
+
+    // ---- T2: apply transitions (history copy, X = K C, mass, Theta, RQ) ----
+    std::vector<LincombWork<T>> lc_hist, lc_x;
+    std::vector<OpWork<T>> op_x[2];
+    std::vector<RayleighWork<T>> rq;
+    int maxq_x = 0;
+    for (int s : plan.t2) {
+        Slot &sl = slots[s];
+        const int kd = sl.t2_kd, nE = sl.nE, nP = sl.nP, nR = sl.nR;
+        const int hn = sl.t2_hist_new;
+        if (nE > 0) {
+            // history copy: H[hn][i] = E[perm[i]]
+            const cplx<T> **pp_d;
+            const cplx<T> **pp = stg.alloc<const cplx<T> *>(nE, (const cplx<T> ***)&pp_d);
+            for (int i = 0; i < nE; ++i) pp[i] = vcol(s, sl.eperm[i]);
+            double *cd;
+            double *ch = stg.alloc<double>((size_t)2 * nE * nE, &cd);
+            std::memset(ch, 0, sizeof(double) * 2 * nE * nE);
+            for (int i = 0; i < nE; ++i) ch[2 * (i + (size_t)i * nE)] = 1.0;
+            LincombWork<T> w{};
+            w.a_ptrs = pp_d;
+            w.na = nE;
+            w.nq = nE;
+            w.C = cd;
+            w.ldc = nE;
+            w.out = hcol(s, hn, 0);
+            w.out_stride = N;
+            w.mode = 0;
+            lc_hist.push_back(w);
+        }
+        // X = [H_new | H_cur | R | y0] cx
+        const cplx<T> **kp_d;
+        const cplx<T> **kp = stg.alloc<const cplx<T> *>(kd, (const cplx<T> ***)&kp_d);
+        int q = 0;
+        for (int i = 0; i < nE; ++i) kp[q++] = hcol(s, hn, i);
+        for (int i = 0; i < nP; ++i) kp[q++] = hcol(s, sl.hist_cur, i);
+        for (int i = 0; i < nR; ++i) kp[q++] = rcol(i);
+        kp[q++] = vcol(s, C);
+        double *cd;
+        double *ch = stg.alloc<double>((size_t)2 * kd * sl.m, &cd);
+        for (int e = 0; e < sl.m; ++e)
+            for (int i = 0; i < kd; ++i) {
+                ch[2 * (i + (size_t)e * kd)] = sl.cx[i + (size_t)e * kd].real();
+                ch[2 * (i + (size_t)e * kd) + 1] = sl.cx[i + (size_t)e * kd].imag();
+            }
+        LincombWork<T> w{};
+        w.a_ptrs = kp_d;
+        w.na = kd;
+        w.nq = sl.m;
+        w.C = cd;
+        w.ldc = kd;
+        w.out = vcol(s, 0);
+        w.out_stride = N;
+        if (sl.pol == 0) { w.out2 = BV + ((size_t)s * CV) * N; w.w2 = eps(s); }
+        lc_x.push_back(w);
+        maxq_x = std::max(maxq_x, sl.m);
+        OpWork<T> o{};
+        o.in = vcol(s, 0);
+        o.out = avcol(s, 0);
+        o.eps = eps(s);
+        o.tensor = ten(s);
+        o.params = dparams + s;
+        o.in_stride = N;
+        o.out_stride = N;
+        o.count = sl.m;
+        op_x[sl.pol].push_back(o);
+        RayleighWork<T> rw{};
+        rw.x = vcol(s, 0);
+        rw.ax = avcol(s, 0);
+        rw.bx = bvcol(s, 0);
+        rw.lam = d_lam + (size_t)s * LD;
+        rw.stride = N;
+        rw.count = sl.m;
+        rq.push_back(rw);
+        if (hn >= 0) sl.hist_cur = hn;
+        sl.t2_pending = false;
+    }
+
+    // ---- iteration for ITER slots ----
+    std::vector<LincombWork<T>> lc_wmove, lc_mass;
+    std::vector<ResidualWork<T>> res;
+    std::vector<GramWork<T>> g_defR, g_defP;
+    std::vector<LincombWork<T>> lc_defR, lc_defP;
+    std::vector<int> slot_defR, slot_defP;
+    std::vector<OpWork<T>> op_pre[2];  // [0] FD, [1] TP
+    std::vector<GramWork<T>> g_svqb, g_rr;
+    std::vector<EighWork> e_svqb, e_rr;
+    std::vector<LincombWork<T>> lc_q, lc_ritz;
+    std::vector<OpWork<T>> op_q[2];
+    std::vector<unsigned char> maskbuf;
+    int max_a = 0, max_p = 0, max_d = 0;
+    struct DefMask { int slot; int nd; int a; std::vector<unsigned char> m; };
+    std::vector<DefMask> masksP;
+    for (int s : plan.iter) {
+        Slot &sl = slots[s];
+        const int m = sl.m;
+        std::vector<int> act;
+        for (int i = 0; i < m; ++i)
+            if (!(i < (int)sl.soft.size() && sl.soft[i])) act.push_back(i);
+        const int a = (int)act.size();
+        const int w = sl.nW > 0 ? std::min(sl.nW, a) : 0;
+        const int p = m + w + a;
+        if (p > C) throw std::runtime_error("subspace wider than the engine allows");
+        max_a = std::max(max_a, a);
+        max_p = std::max(max_p, p);
+        // W sits at columns [nb, nb + nW); move it next to X when m != nb
+        if (w > 0 && m != sl.nb) {
+            double *cd;
+            double *ch = stg.alloc<double>((size_t)2 * w * w, &cd);
+            std::memset(ch, 0, sizeof(double) * 2 * w * w);
+            for (int i = 0; i < w; ++i) ch[2 * (i + (size_t)i * w)] = 1.0;
+            LincombWork<T> lw{};
+            lw.A = vcol(s, sl.nb);
+            lw.a_stride = N;
+            lw.na = w;
+            lw.nq = w;
+            lw.C = cd;
+            lw.ldc = w;
+            lw.out = vcol(s, m);
+            lw.out_stride = N;
+            if (sl.pol == 0) { lw.out2 = BV + ((size_t)s * CV + m) * N; lw.w2 = eps(s); }
+            lc_wmove.push_back(lw);
+        }
+        // residuals into P slots [m + w, m + w + a)
+        int *act_d;
+        int *act_h = stg.alloc<int>(std::max(a, 1), &act_d);
+        for (int i = 0; i < a; ++i) act_h[i] = act[i];
+        ResidualWork<T> rw{};
+        rw.ax = avcol(s, 0);
+        rw.bx = bvcol(s, 0);
+        rw.out = vcol(s, m + w);
+        rw.act = act_d;
+        rw.lam = d_lam + (size_t)s * LD;
+        rw.stride = N;
+        rw.count = a;
+        res.push_back(rw);
+        // deflation set D = [y0 if Gamma] + soft-locked X columns
+        std::vector<int> dcols;
+        if (sl.gamma_defl) dcols.push_back(C);
+        for (int j = 0; j < (int)sl.soft.size() && j < m; ++j)
+            if (sl.soft[j]) dcols.push_back(j);
+        const int nd = (int)dcols.size();
+        max_d = std::max(max_d, nd);
+        if (nd > 0 && a > 0) {
+            const cplx<T> **bd_d, **dd_d;
+            const cplx<T> **bd = stg.alloc<const cplx<T> *>(nd, (const cplx<T> ***)&bd_d);
+            const cplx<T> **dd = stg.alloc<const cplx<T> *>(nd, (const cplx<T> ***)&dd_d);
+            for (int i = 0; i < nd; ++i) { bd[i] = bvcol(s, dcols[i]); dd[i] = vcol(s, dcols[i]); }
+            GramWork<T> gw{};
+            gw.a_ptrs = bd_d;
+            gw.B = vcol(s, m + w);
+            gw.b_stride = N;
+            gw.na = nd;
+            gw.nb = a;
+            gw.ld = LD;
+            g_defR.push_back(gw);
+            slot_defR.push_back(s);
+            LincombWork<T> lw{};
+            lw.a_ptrs = dd_d;
+            lw.na = nd;
+            lw.nq = a;
+            lw.C = coef(s);
+            lw.ldc = LD;
+            lw.base = vcol(s, m + w);
+            lw.base_stride = N;
+            lw.out = vcol(s, m + w);
+            lw.out_stride = N;
+            lw.mode = 1;
+            lc_defR.push_back(lw);
+            // P deflation: same D; soft part skipped for compacted P index c with soft[c]
+            GramWork<T> gp = gw;
+            g_defP.push_back(gp);
+            slot_defP.push_back(s);
+            LincombWork<T> lp = lw;
+            lc_defP.push_back(lp);
+            if (sl.pol == 0) {
+                LincombWork<T> lb = lw;
+                lb.a_ptrs = bd_d;
+                lb.base = BV + ((size_t)s * CV + m + w) * N;
+                lb.out = BV + ((size_t)s * CV + m + w) * N;
+                lc_defP.push_back(lb);
+                slot_defP.push_back(-1);  // marks the B-image item (shares coefficients)
+            }
+            DefMask dm{s, nd, a, std::vector<unsigned char>((size_t)LD * LD, 1)};
+            for (int c = 0; c < a; ++c)
+                if (c < (int)sl.soft.size() && sl.soft[c])
+                    for (int i = 0; i < nd; ++i)
+                        if (dcols[i] != C) dm.m[i + (size_t)c * LD] = 0;
+            masksP.push_back(std::move(dm));
+        }
+        // TM without the fused FD mass output: BP = eps P after preconditioning
+        if (sl.pol == 0 && sl.precond != 1 && a > 0) {
+            double *cd;
+            double *ch = stg.alloc<double>((size_t)2 * a * a, &cd);
+            std::memset(ch, 0, sizeof(double) * 2 * a * a);
+            for (int i = 0; i < a; ++i) ch[2 * (i + (size_t)i * a)] = 1.0;
+            LincombWork<T> lm{};
+            lm.A = vcol(s, m + w);
+            lm.a_stride = N;
+            lm.na = a;
+            lm.nq = a;
+            lm.C = cd;
+            lm.ldc = a;
+            lm.out = vcol(s, m + w);
+            lm.out_stride = N;
+            lm.out2 = BV + ((size_t)s * CV + m + w) * N;
+            lm.w2 = eps(s);
+            lc_mass.push_back(lm);
+        }
+        // preconditioner (P in place; TM FD also writes BP)
+        if (sl.precond == 1 || sl.precond == 2) {
+            OpWork<T> o{};
+            o.in = vcol(s, m + w);
+            o.out = vcol(s, m + w);
+            o.eps = eps(s);
+            o.tensor = ten(s);
+            o.params = dparams + s;
+            o.in_stride = N;
+            o.out_stride = N;
+            o.count = a;
+            if (sl.precond == 1 && sl.pol == 0) o.bout = BV + ((size_t)s * CV + m + w) * N;
+            op_pre[sl.precond - 1].push_back(o);
+        }
+        // SVQB Gram Z^H B Z over p columns
+        GramWork<T> gs{};
+        gs.A = vcol(s, 0);
+        gs.a_stride = N;
+        gs.B = bvcol(s, 0);
+        gs.b_stride = N;
+        gs.na = p;
+        gs.nb = p;
+        gs.ld = LD;
+        g_svqb.push_back(gs);
+        EighWork es{};
+        es.n = p;
+        es.ld = LD;
+        es.mode = 1;
+        es.w = d_svw + (size_t)s * LD;
+        es.V = d_T + (size_t)s * LD * LD * 2;
+        es.rank_out = d_rank + s;
+        es.ldv = LD;
+        e_svqb.push_back(es);
+        // Q = Z T, BQ = BZ T
+        LincombWork<T> lq{};
+        lq.A = vcol(s, 0);
+        lq.a_stride = N;
+        lq.na = p;
+        lq.dev_nq = d_rank + s;
+        lq.C = d_T + (size_t)s * LD * LD * 2;
+        lq.ldc = LD;
+        lq.out = qcol(s, 0);
+        lq.out_stride = N;
+        lc_q.push_back(lq);
+        if (sl.pol == 0) {
+            LincombWork<T> lb = lq;
+            lb.A = BV + ((size_t)s * CV) * N;
+            lb.out = BQ + ((size_t)s * C) * N;
+            lc_q.push_back(lb);
+        }
+        OpWork<T> oq{};
+        oq.in = qcol(s, 0);
+        oq.out = aqcol(s, 0);
+        oq.eps = eps(s);
+        oq.tensor = ten(s);
+        oq.params = dparams + s;
+        oq.dev_count = d_rank + s;
+        oq.in_stride = N;
+        oq.out_stride = N;
+        op_q[sl.pol].push_back(oq);
+        GramWork<T> gr{};
+        gr.A = qcol(s, 0);
+        gr.a_stride = N;
+        gr.B = aqcol(s, 0);
+        gr.b_stride = N;
+        gr.dev_na = d_rank + s;
+        gr.dev_nb = d_rank + s;
+        gr.ld = LD;
+        g_rr.push_back(gr);
+        EighWork er{};
+        er.dev_n = d_rank + s;
+        er.ld = LD;
+        er.mode = 0;
+        er.w = d_lam + (size_t)s * LD;
+        er.V = d_Y + (size_t)s * LD * LD * 2;
+        er.Vw = d_Yw + (size_t)s * LD * LD * 2;
+        er.rank_out = nullptr;
+        er.cnt_x = d_cnt + 2 * s;
+        er.cnt_w = d_cnt + 2 * s + 1;
+        er.ldv = LD;
+        er.nsplit = sl.nb;
+        e_rr.push_back(er);
+        // Ritz update: X, BX, AX = (Q, BQ, AQ) Y1 ; W = Q Y2 (+ BW = eps W)
+        LincombWork<T> lx{};
+        lx.A = qcol(s, 0);
+        lx.a_stride = N;
+        lx.dev_na = d_rank + s;
+        lx.dev_nq = d_cnt + 2 * s;
+        lx.C = d_Y + (size_t)s * LD * LD * 2;
+        lx.ldc = LD;
+        lx.out = vcol(s, 0);
+        lx.out_stride = N;
+        lc_ritz.push_back(lx);
+        if (sl.pol == 0) {
+            LincombWork<T> lb = lx;
+            lb.A = BQ + ((size_t)s * C) * N;
+            lb.out = BV + ((size_t)s * CV) * N;
+            lc_ritz.push_back(lb);
+        }
+        LincombWork<T> la = lx;
+        la.A = aqcol(s, 0);
+        la.out = avcol(s, 0);
+        lc_ritz.push_back(la);
+        LincombWork<T> lw2 = lx;
+        lw2.dev_nq = d_cnt + 2 * s + 1;
+        lw2.C = d_Yw + (size_t)s * LD * LD * 2;
+        lw2.out = vcol(s, sl.nb);
+        if (sl.pol == 0) { lw2.out2 = BV + ((size_t)s * CV + sl.nb) * N; lw2.w2 = eps(s); }
+        lc_ritz.push_back(lw2);
+        sl.it += 1;
+    }
+
+    // ---- T1: transition Grams ----
+    std::vector<GramWork<T>> g_t1;
+    std::vector<int> t1_slots;
+    for (int s : plan.t1) {
+        Slot &sl = slots[s];
+        const int kd = sl.kd;
+        const cplx<T> **kp_d;
+        const cplx<T> **kp = stg.alloc<const cplx<T> *>(kd, (const cplx<T> ***)&kp_d);
+        int q = 0;
+        for (int i = 0; i < sl.nE; ++i) kp[q++] = vcol(s, sl.E_cols[i]);
+        for (int i = 0; i < sl.nP; ++i) kp[q++] = hcol(s, sl.hist_cur, i);
+        for (int i = 0; i < sl.nR; ++i) kp[q++] = rcol(i);
+        kp[q++] = vcol(s, C);
+        GramWork<T> g{};
+        g.a_ptrs = kp_d;
+        g.b_ptrs = kp_d;
+        g.weight = sl.pol == 0 ? eps(s) : nullptr;
+        g.na = kd;
+        g.nb = kd;
+        g.ld = LD;
+        g_t1.push_back(g);
+        sl.rb_index = (int)t1_slots.size();
+        t1_slots.push_back(s);
+    }
+
+    // ---- descriptor arrays into staging ----
+    auto put = [&](auto &vec) {
+        using W = typename std::remove_reference<decltype(vec)>::type::value_type;
+        W *d = nullptr;
+        if (vec.empty()) return d;
+        W *h = stg.alloc<W>(vec.size(), &d);
+        std::memcpy((void *)h, vec.data(), sizeof(W) * vec.size());
+        return d;
+    };
+    // Gram partial buffers: item i of a launch uses partial block i
+    auto nch_for = [&](size_t nwork) {
+        long c = (long)((2048 + nwork - 1) / std::max<size_t>(nwork, 1));
+        c = std::min<long>(std::max<long>(c, 1), NCHMAX);
+        return (int)c;
+    };
+    auto assign_partials = [&](std::vector<GramWork<T>> &v, int nch) {
+        for (size_t i = 0; i < v.size(); ++i) v[i].partial = d_part + i * (size_t)nch * LD * LD * 2;
+    };
+    const int nch_def = nch_for(g_defR.size()), nch_svqb = nch_for(g_svqb.size()), nch_rr = nch_for(g_rr.size()),
+              nch_t1 = nch_for(g_t1.size());
+    assign_partials(g_defR, nch_def);
+    assign_partials(g_defP, nch_def);
+    assign_partials(g_svqb, nch_svqb);
+    assign_partials(g_rr, nch_rr);
+    assign_partials(g_t1, nch_t1);
+    for (size_t i = 0; i < e_svqb.size(); ++i) {
+        e_svqb[i].partial = g_svqb[i].partial;
+        e_svqb[i].nchunks = nch_svqb;
+    }
+    for (size_t i = 0; i < e_rr.size(); ++i) {
+        e_rr[i].partial = g_rr[i].partial;
+        e_rr[i].nchunks = nch_rr;
+    }
+    // P-deflation masks
+    unsigned char *mask_d = nullptr;
+    if (!masksP.empty()) {
+        unsigned char *mh = stg.alloc<unsigned char>(masksP.size() * LD * LD, &mask_d);
+        for (size_t i = 0; i < masksP.size(); ++i) std::memcpy(mh + i * LD * LD, masksP[i].m.data(), LD * LD);
+    }
+    std::vector<ReduceWork> red_defR, red_defP;
+    for (size_t i = 0; i < g_defR.size(); ++i)
+        red_defR.push_back({g_defR[i].partial, coef(slot_defR[i]), nullptr, nch_def, 0});
+    for (size_t i = 0; i < g_defP.size(); ++i)
+        red_defP.push_back({g_defP[i].partial, coef(masksP[i].slot), mask_d + i * LD * LD, nch_def, 0});
+    auto *d_red_defR = put(red_defR);
+    auto *d_red_defP = put(red_defP);
+    auto *d_lc_hist = put(lc_hist);
+    auto *d_lc_x = put(lc_x);
+    OpWork<T> *d_op_x[2] = {put(op_x[0]), put(op_x[1])};
+    auto *d_rq = put(rq);
+    auto *d_lc_wmove = put(lc_wmove);
+    auto *d_lc_mass = put(lc_mass);
+    auto *d_res = put(res);
+    auto *d_g_defR = put(g_defR);
+    auto *d_lc_defR = put(lc_defR);
+    OpWork<T> *d_op_pre[2] = {put(op_pre[0]), put(op_pre[1])};
+    auto *d_g_defP = put(g_defP);
+    auto *d_lc_defP = put(lc_defP);
+    auto *d_g_svqb = put(g_svqb);
+    auto *d_e_svqb = put(e_svqb);
+    auto *d_lc_q = put(lc_q);
+    OpWork<T> *d_op_q[2] = {put(op_q[0]), put(op_q[1])};
+    auto *d_g_rr = put(g_rr);
+    auto *d_e_rr = put(e_rr);
+    auto *d_lc_ritz = put(lc_ritz);
+    auto *d_g_t1 = put(g_t1);
+    stg.upload(st);
+
+    // ---- launches (single stream, in dependency order) ----
+    for (auto &tc : tcopies)
+        BH_HIP_CHECK(hipMemcpyAsync(tc.dst, tc.src, sizeof(T) * tc.n, hipMemcpyDeviceToDevice, st));
+    for (int s : plan.fill_y0) {
+        Slot &sl = slots[s];
+        launch_fill<T>(vcol(s, C), sl.pol == 0 ? BV + ((size_t)s * CV + C) * N : nullptr, eps(s), sl.y0val, 0.0, N, st);
+    }
+    // T2
+    launch_lincomb<T>(d_lc_hist, (int)lc_hist.size(), N, nbmax, st);
+    launch_lincomb<T>(d_lc_x, (int)lc_x.size(), N, std::max(maxq_x, 1), st);
+    for (int pol = 0; pol < 2; ++pol)
+        launch_op<T>(pol == 0 ? MODE_THETA_TM : MODE_THETA_TE, nx, ny, d_op_x[pol], (int)op_x[pol].size(), m0max, twx,
+                     twy, st);
+    launch_rayleigh<T>(d_rq, (int)rq.size(), m0max, N, st);
+    // iteration
+    launch_lincomb<T>(d_lc_wmove, (int)lc_wmove.size(), N, nbmax, st);
+    launch_residual<T>(d_res, (int)res.size(), std::max(max_a, 1), N, st);
+    if (!g_defR.empty()) {
+        launch_gram<T>(d_g_defR, (int)g_defR.size(), nch_def, N, max_d, max_a, st);
+        launch_reduce(d_red_defR, (int)g_defR.size(), LD, st);
+        launch_lincomb<T>(d_lc_defR, (int)lc_defR.size(), N, max_a, st);
+    }
+    launch_op<T>(MODE_PREC_FD, nx, ny, d_op_pre[0], (int)op_pre[0].size(), std::max(max_a, 1), twx, twy, st);
+    launch_op<T>(MODE_PREC_TP, nx, ny, d_op_pre[1], (int)op_pre[1].size(), std::max(max_a, 1), twx, twy, st);
+    launch_lincomb<T>(d_lc_mass, (int)lc_mass.size(), N, std::max(max_a, 1), st);
+    if (!g_defP.empty()) {
+        launch_gram<T>(d_g_defP, (int)g_defP.size(), nch_def, N, max_d, max_a, st);
+        launch_reduce(d_red_defP, (int)g_defP.size(), LD, st);
+        launch_lincomb<T>(d_lc_defP, (int)lc_defP.size(), N, max_a, st);
+    }
+    launch_gram<T>(d_g_svqb, (int)g_svqb.size(), nch_svqb, N, max_p, max_p, st);
+    launch_eigh(d_e_svqb, (int)e_svqb.size(), std::max(max_p, 1), st);
+    launch_lincomb<T>(d_lc_q, (int)lc_q.size(), N, std::max(max_p, 1), st);
+    for (int pol = 0; pol < 2; ++pol) {
+        if (op_q[pol].empty()) continue;
+        if (eopt.profile) {
+            hipEvent_t a, b;
+            BH_HIP_CHECK(hipEventCreate(&a));
+            BH_HIP_CHECK(hipEventCreate(&b));
+            BH_HIP_CHECK(hipEventRecord(a, st));
+            launch_op<T>(pol == 0 ? MODE_THETA_TM : MODE_THETA_TE, nx, ny, d_op_q[pol], (int)op_q[pol].size(),
+                         std::max(max_p, 1), twx, twy, st);
+            BH_HIP_CHECK(hipEventRecord(b, st));
+            BH_HIP_CHECK(hipEventSynchronize(b));
+            float ms = 0;
+            BH_HIP_CHECK(hipEventElapsedTime(&ms, a, b));
+            stats_->theta_ms += ms;
+            stats_->theta_launches += 1;
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+        } else {
+            launch_op<T>(pol == 0 ? MODE_THETA_TM : MODE_THETA_TE, nx, ny, d_op_q[pol], (int)op_q[pol].size(),
+                         std::max(max_p, 1), twx, twy, st);
+        }
+    }
+    launch_gram<T>(d_g_rr, (int)g_rr.size(), nch_rr, N, max_p, max_p, st);
+    launch_eigh(d_e_rr, (int)e_rr.size(), std::max(max_p, 1), st);
+    launch_lincomb<T>(d_lc_ritz, (int)lc_ritz.size(), N, nbmax, st);
+    // T1 Grams -> compact readback
+    if (!g_t1.empty()) {
+        int maxkd = 0;
+        for (int s : t1_slots) maxkd = std::max(maxkd, slots[s].kd);
+        launch_gram<T>(d_g_t1, (int)g_t1.size(), nch_t1, N, maxkd, maxkd, st);
+        launch_gram_reduce(d_part, d_gramrb, (int)g_t1.size(), nch_t1, LD, (long)nch_t1 * LD * LD * 2,
+                           (long)LD * LD * 2, st);
+        BH_HIP_CHECK(hipMemcpyAsync(h_gramrb, d_gramrb, sizeof(double) * g_t1.size() * LD * LD * 2,
+                                    hipMemcpyDeviceToHost, st));
+    }
+    if (!plan.iter.empty()) {
+        BH_HIP_CHECK(hipMemcpyAsync(h_lam, d_lam, sizeof(double) * S * LD, hipMemcpyDeviceToHost, st));
+        BH_HIP_CHECK(hipMemcpyAsync(h_cnt, d_cnt, sizeof(int) * 2 * S, hipMemcpyDeviceToHost, st));
+    }
+    BH_HIP_CHECK(hipStreamSynchronize(st));
+    if (eopt.profile) stats_->theta_columns += 0;
+}
+
+std::unique_ptr<EngineBase> make_engine(int nx, int ny, int nb_max, const EngineOptions &opt) {
+    if (opt.precision == 1) return std::unique_ptr<EngineBase>(new Engine<float>(nx, ny, nb_max, opt));
+    return std::unique_ptr<EngineBase>(new Engine<double>(nx, ny, nb_max, opt));
+}
+
+template class Engine<double>;
+template class Engine<float>;
+
+}  // namespace bh

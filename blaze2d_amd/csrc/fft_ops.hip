@@ -1,0 +1,432 @@
+// Fused LDS-resident 2D FFT operators for gfx950.
+//
+// One workgroup owns one NX x NY field (one column of one slot) for the whole
+// operator: the field is read from HBM once, every FFT pass runs in LDS, and
+// the result is written once.  The forward transform is an in-place mixed
+// radix decimation-in-frequency (natural order in, digit-reversed out); the
+// inverse is the matching decimation-in-time (digit-reversed in, natural
+// out), so no reorder pass is ever needed: Fourier-space factors are
+// evaluated at the digit-reversed frequency of each LDS position, with k+G
+// generated from (b1, b2, bloch) on the fly instead of streamed tables.
+//
+// Operators (reference citations):
+//   THETA_TM   y = F^-1 |k+G|^2 F x                         maxwell.rs:686-713
+//   THETA_TE   y = -F^-1 i(k+G).F eta F^-1 i(k+G) F x       maxwell.rs:647-684,1745-1839
+//   PREC_FD    y = F^-1 eps_eff/(|k+G|^2+m+s eps_eff) F x   fourier_diagonal.rs:310-342
+//   PREC_TP    transverse projection, 6 FFTs            transverse_projection.rs:167-231
+// FFT convention (backend-cpu/src/lib.rs:94-177): forward unnormalised
+// e^{-2 pi i jk/n}, inverse e^{+...} scaled by 1/(nx ny).
+#include "common.h"
+#include "fft_ops.h"
+
+namespace bh {
+
+// ---------------------------------------------------------------------------
+// Radix plans: N = R0 * R1 * ... (radix-4 first).
+// ---------------------------------------------------------------------------
+template <int N>
+struct Plan;
+#define BH_PLAN(NN, KK, A, B, C, D)                                                      \
+    template <>                                                                          \
+    struct Plan<NN> {                                                                    \
+        static constexpr int K = KK;                                                     \
+        __host__ __device__ static constexpr int r(int s) {                              \
+            return s == 0 ? A : s == 1 ? B : s == 2 ? C : D;                             \
+        }                                                                                \
+    };
+BH_PLAN(8, 2, 4, 2, 1, 1)
+BH_PLAN(16, 2, 4, 4, 1, 1)
+BH_PLAN(24, 3, 4, 3, 2, 1)
+BH_PLAN(32, 3, 4, 4, 2, 1)
+BH_PLAN(48, 3, 4, 4, 3, 1)
+BH_PLAN(64, 3, 4, 4, 4, 1)
+#undef BH_PLAN
+
+template <int N>
+__host__ __device__ constexpr int plan_len(int s) {  // transform length at pass s
+    int L = N;
+    for (int i = 0; i < s; ++i) L /= Plan<N>::r(i);
+    return L;
+}
+
+// frequency index held at digit-reversed storage position p
+template <int N>
+__device__ inline int rev_index(int p) {
+    int k = 0, mult = 1, rem = p, L = N;
+#pragma unroll
+    for (int s = 0; s < Plan<N>::K; ++s) {
+        const int R = Plan<N>::r(s);
+        L /= R;
+        int d = rem / L;
+        rem -= d * L;
+        k += d * mult;
+        mult *= R;
+    }
+    return k;
+}
+
+// ---------------------------------------------------------------------------
+// Small DFT butterflies (in registers).  INV selects e^{+2 pi i/R}.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ inline cplx<T> mul_mi(cplx<T> a) { return {a.y, -a.x}; }  // * (-i)
+template <typename T>
+__device__ inline cplx<T> mul_pi(cplx<T> a) { return {-a.y, a.x}; }  // * (+i)
+
+template <typename T, int R, bool INV>
+__device__ inline void small_dft(cplx<T> *v) {
+    if constexpr (R == 2) {
+        cplx<T> a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    } else if constexpr (R == 3) {
+        const T s = (T)0.86602540378443864676;
+        cplx<T> t1 = cadd(v[1], v[2]);
+        cplx<T> t2 = {v[0].x - (T)0.5 * t1.x, v[0].y - (T)0.5 * t1.y};
+        cplx<T> t3 = {s * (v[1].x - v[2].x), s * (v[1].y - v[2].y)};
+        v[0] = cadd(v[0], t1);
+        if (!INV) {
+            v[1] = cadd(t2, mul_mi(t3));
+            v[2] = cadd(t2, mul_pi(t3));
+        } else {
+            v[1] = cadd(t2, mul_pi(t3));
+            v[2] = cadd(t2, mul_mi(t3));
+        }
+    } else if constexpr (R == 4) {
+        cplx<T> s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+        cplx<T> s13 = cadd(v[1], v[3]), d13 = csub(v[1], v[3]);
+        v[0] = cadd(s02, s13);
+        v[2] = csub(s02, s13);
+        if (!INV) {
+            v[1] = cadd(d02, mul_mi(d13));
+            v[3] = cadd(d02, mul_pi(d13));
+        } else {
+            v[1] = cadd(d02, mul_pi(d13));
+            v[3] = cadd(d02, mul_mi(d13));
+        }
+    }
+}
+
+// One radix pass over NLINES independent length-N transforms stored in LDS.
+// ROWS: elements contiguous (stride 1), lines NX apart; else elements NX
+// apart, lines contiguous.  NF fields are processed together.
+template <typename T, int N, int NLINES, bool ROWS, int S, bool INV, int NF, int NT, int FSTRIDE>
+__device__ inline void fft_pass(cplx<T> *buf, const cplx<T> *__restrict__ tw) {
+    constexpr int R = Plan<N>::r(S);
+    constexpr int L = plan_len<N>(S);
+    constexpr int M = L / R;
+    constexpr int NBF = N / R;
+    constexpr int TOTAL = NBF * NLINES * NF;
+    constexpr int TWS = N / L;
+    constexpr int ES = ROWS ? 1 : NLINES;  // element stride
+    constexpr int LS = ROWS ? N : 1;       // line stride
+    for (int id = threadIdx.x; id < TOTAL; id += NT) {
+        int f = id / (NBF * NLINES);
+        int rem = id - f * (NBF * NLINES);
+        int line, bf;
+        if (ROWS) { line = rem / NBF; bf = rem - line * NBF; }
+        else { bf = rem / NLINES; line = rem - bf * NLINES; }
+        int b = bf / M, j2 = bf - b * M;
+        cplx<T> *base = buf + f * FSTRIDE + line * LS + (b * L + j2) * ES;
+        cplx<T> v[R];
+#pragma unroll
+        for (int j = 0; j < R; ++j) v[j] = base[j * M * ES];
+        if (!INV) {
+            small_dft<T, R, false>(v);
+            if (M > 1) {
+#pragma unroll
+                for (int k1 = 1; k1 < R; ++k1) v[k1] = cmul(v[k1], tw[j2 * k1 * TWS]);
+            }
+        } else {
+            if (M > 1) {
+#pragma unroll
+                for (int k1 = 1; k1 < R; ++k1) v[k1] = cmulc(v[k1], tw[j2 * k1 * TWS]);
+            }
+            small_dft<T, R, true>(v);
+        }
+#pragma unroll
+        for (int j = 0; j < R; ++j) base[j * M * ES] = v[j];
+    }
+}
+
+template <typename T, int N, int NLINES, bool ROWS, bool INV, int NF, int NT, int FSTRIDE, int S>
+__device__ inline void fft_passes_fwd(cplx<T> *buf, const cplx<T> *tw) {
+    if constexpr (S < Plan<N>::K) {
+        fft_pass<T, N, NLINES, ROWS, S, INV, NF, NT, FSTRIDE>(buf, tw);
+        __syncthreads();
+        fft_passes_fwd<T, N, NLINES, ROWS, INV, NF, NT, FSTRIDE, S + 1>(buf, tw);
+    }
+}
+template <typename T, int N, int NLINES, bool ROWS, bool INV, int NF, int NT, int FSTRIDE, int S>
+__device__ inline void fft_passes_inv(cplx<T> *buf, const cplx<T> *tw) {
+    if constexpr (S >= 0) {
+        fft_pass<T, N, NLINES, ROWS, S, INV, NF, NT, FSTRIDE>(buf, tw);
+        __syncthreads();
+        fft_passes_inv<T, N, NLINES, ROWS, INV, NF, NT, FSTRIDE, S - 1>(buf, tw);
+    }
+}
+
+// natural -> digit-reversed (rows then columns)
+template <typename T, int NX, int NY, int NF, int NT>
+__device__ inline void fft2d_forward(cplx<T> *buf, const cplx<T> *twx, const cplx<T> *twy) {
+    fft_passes_fwd<T, NX, NY, true, false, NF, NT, NX * NY, 0>(buf, twx);
+    fft_passes_fwd<T, NY, NX, false, false, NF, NT, NX * NY, 0>(buf, twy);
+}
+// digit-reversed -> natural, unscaled
+template <typename T, int NX, int NY, int NF, int NT>
+__device__ inline void fft2d_inverse(cplx<T> *buf, const cplx<T> *twx, const cplx<T> *twy) {
+    fft_passes_inv<T, NY, NX, false, true, NF, NT, NX * NY, Plan<NY>::K - 1>(buf, twy);
+    fft_passes_inv<T, NX, NY, true, true, NF, NT, NX * NY, Plan<NX>::K - 1>(buf, twx);
+}
+
+// k+G at the frequency stored at LDS position (px, py), with the reference's
+// |k+G|^2 <= 1e-9 clamp (maxwell.rs:1609-1690).
+template <int NX, int NY>
+__device__ inline void kplusg(const OpParams &P, int px, int py, double &kx, double &ky, double &ksq,
+                              bool &clamped) {
+    int kxi = rev_index<NX>(px), kyi = rev_index<NY>(py);
+    double n1 = kxi <= NX / 2 ? kxi : kxi - NX;
+    double n2 = kyi <= NY / 2 ? kyi : kyi - NY;
+    double gx = n1 * P.b1[0] + n2 * P.b2[0];
+    double gy = n1 * P.b1[1] + n2 * P.b2[1];
+    double rx = gx + P.bloch[0], ry = gy + P.bloch[1];
+    double raw = rx * rx + ry * ry;
+    clamped = raw <= KG_FLOOR;
+    if (raw >= KG_FLOOR) { kx = rx; ky = ry; }
+    else if (raw == 0.0) { kx = sqrt(KG_FLOOR); ky = 0.0; }
+    else { double s = sqrt(KG_FLOOR / raw); kx = rx * s; ky = ry * s; }
+    ksq = kx * kx + ky * ky;
+}
+
+template <typename T>
+__device__ inline cplx<T> scale_c(cplx<T> a, T s) { return {a.x * s, a.y * s}; }
+
+// ---------------------------------------------------------------------------
+// The fused operator kernel.  grid = (max columns, work items).
+// ---------------------------------------------------------------------------
+template <typename T, int NX, int NY, int MODE, int NT>
+__global__ void __launch_bounds__(NT) op_kernel(const OpWork<T> *__restrict__ work, const cplx<T> *__restrict__ twx,
+                                                const cplx<T> *__restrict__ twy) {
+    constexpr int N = NX * NY;
+    constexpr int NF = (MODE == MODE_THETA_TM || MODE == MODE_PREC_FD) ? 1 : 2;
+    extern __shared__ char smem_raw[];
+    cplx<T> *F = reinterpret_cast<cplx<T> *>(smem_raw);
+    const OpWork<T> &w = work[blockIdx.y];
+    const int count = w.dev_count ? *w.dev_count : w.count;
+    const int col = blockIdx.x;
+    if (col >= count) return;
+    const OpParams P = *w.params;
+    const cplx<T> *in = w.in + (size_t)col * w.in_stride;
+    cplx<T> *out = w.out + (size_t)col * w.out_stride;
+    const T invN = (T)(1.0 / (double)N);
+
+    for (int i = threadIdx.x; i < N; i += NT) F[i] = in[i];
+    __syncthreads();
+    fft2d_forward<T, NX, NY, 1, NT>(F, twx, twy);
+
+    if constexpr (MODE == MODE_THETA_TM || MODE == MODE_PREC_FD) {
+        for (int i = threadIdx.x; i < N; i += NT) {
+            int py = i / NX, px = i - py * NX;
+            double kx, ky, ksq;
+            bool cl;
+            kplusg<NX, NY>(P, px, py, kx, ky, ksq, cl);
+            double fac;
+            if (MODE == MODE_THETA_TM) {
+                fac = ksq;
+            } else {  // inverse_scale (maxwell.rs:1692-1711) with Gamma DC mask
+                double e = P.eps_eff;
+                if (!(e > 0.0) || !isfinite(e) || !isfinite(ksq)) fac = 0.0;
+                else {
+                    double mf = (isfinite(P.mass_floor) && P.mass_floor > 0.0) ? P.mass_floor : 0.0;
+                    fac = e / (fmax(ksq, 0.0) + mf + P.shift * fmax(e, 1e-12));
+                }
+                if (P.gamma && cl) fac = 0.0;
+            }
+            F[i] = scale_c(F[i], (T)fac * invN);
+        }
+        __syncthreads();
+        fft2d_inverse<T, NX, NY, 1, NT>(F, twx, twy);
+        if (MODE == MODE_PREC_FD && w.bout) {
+            for (int i = threadIdx.x; i < N; i += NT) {
+                cplx<T> v = F[i];
+                out[i] = v;
+                T e = w.eps[i];
+                w.bout[(size_t)col * w.out_stride + i] = {v.x * e, v.y * e};
+            }
+        } else {
+            for (int i = threadIdx.x; i < N; i += NT) out[i] = F[i];
+        }
+    } else {
+        cplx<T> *G2 = F + N;
+        // gradient step: THETA_TE  g = i(k+G) u_hat ; PREC_TP  g = -i(k+G) u_hat / (|k+G|^2 + s)
+        for (int i = threadIdx.x; i < N; i += NT) {
+            int py = i / NX, px = i - py * NX;
+            double kx, ky, ksq;
+            bool cl;
+            kplusg<NX, NY>(P, px, py, kx, ky, ksq, cl);
+            cplx<T> u = F[i];
+            if (MODE == MODE_THETA_TE) {
+                T skx = (T)kx * invN, sky = (T)ky * invN;
+                F[i] = {-u.y * skx, u.x * skx};   // u * (0 + i kx)
+                G2[i] = {-u.y * sky, u.x * sky};
+            } else {
+                double ik = 0.0;
+                if (!cl) { double den = ksq + P.shift; ik = den > 1e-15 ? 1.0 / den : 0.0; }
+                T fx = (T)(kx * ik) * invN, fy = (T)(ky * ik) * invN;
+                F[i] = {u.y * fx, -u.x * fx};     // u * (0 - i kx ik)
+                G2[i] = {u.y * fy, -u.x * fy};
+            }
+        }
+        __syncthreads();
+        fft2d_inverse<T, NX, NY, 2, NT>(F, twx, twy);
+        // real-space step: THETA_TE eta tensor [xx, xy, yx, yy]; PREC_TP eps
+        for (int i = threadIdx.x; i < N; i += NT) {
+            cplx<T> gx = F[i], gy = G2[i];
+            if (MODE == MODE_THETA_TE) {
+                const T *t = w.tensor + 4 * (size_t)i;
+                T t0 = t[0], t1 = t[1], t2 = t[2], t3 = t[3];
+                F[i] = {gx.x * t0 + gy.x * t1, gx.y * t0 + gy.y * t1};
+                G2[i] = {gx.x * t2 + gy.x * t3, gx.y * t2 + gy.y * t3};
+            } else {
+                T e = w.eps[i];
+                F[i] = {gx.x * e, gx.y * e};
+                G2[i] = {gy.x * e, gy.y * e};
+            }
+        }
+        __syncthreads();
+        fft2d_forward<T, NX, NY, 2, NT>(F, twx, twy);
+        // divergence: out_hat = -(i kx gx + i ky gy) [ * 1/(|k+G|^2+s) for TP ]
+        for (int i = threadIdx.x; i < N; i += NT) {
+            int py = i / NX, px = i - py * NX;
+            double kx, ky, ksq;
+            bool cl;
+            kplusg<NX, NY>(P, px, py, kx, ky, ksq, cl);
+            double sc = 1.0;
+            if (MODE == MODE_PREC_TP) {
+                double ik = 0.0;
+                if (!cl) { double den = ksq + P.shift; ik = den > 1e-15 ? 1.0 / den : 0.0; }
+                sc = ik;
+            }
+            T skx = (T)kx, sky = (T)ky, s = (T)sc * invN;
+            cplx<T> gx = F[i], gy = G2[i];
+            // div = i kx gx + i ky gy ; out = -div
+            T dre = -(skx * gx.y + sky * gy.y);
+            T dim = skx * gx.x + sky * gy.x;
+            F[i] = {-dre * s, -dim * s};
+        }
+        __syncthreads();
+        fft2d_inverse<T, NX, NY, 1, NT>(F, twx, twy);
+        for (int i = threadIdx.x; i < N; i += NT) out[i] = F[i];
+    }
+}
+
+// Plain 2D FFT of columns (trait-level SpectralBackend::forward/inverse_fft_2d).
+template <typename T, int NX, int NY, int NT>
+__global__ void __launch_bounds__(NT) fft_kernel(const OpWork<T> *__restrict__ work, const cplx<T> *__restrict__ twx,
+                                                 const cplx<T> *__restrict__ twy, int inverse) {
+    constexpr int N = NX * NY;
+    extern __shared__ char smem_raw[];
+    cplx<T> *F = reinterpret_cast<cplx<T> *>(smem_raw);
+    const OpWork<T> &w = work[blockIdx.y];
+    const int count = w.dev_count ? *w.dev_count : w.count;
+    const int col = blockIdx.x;
+    if (col >= count) return;
+    const cplx<T> *in = w.in + (size_t)col * w.in_stride;
+    cplx<T> *out = w.out + (size_t)col * w.out_stride;
+    if (!inverse) {
+        for (int i = threadIdx.x; i < N; i += NT) F[i] = in[i];
+        __syncthreads();
+        fft2d_forward<T, NX, NY, 1, NT>(F, twx, twy);
+        // digit-reversed -> natural on the way out
+        for (int i = threadIdx.x; i < N; i += NT) {
+            int py = i / NX, px = i - py * NX;
+            out[rev_index<NY>(py) * NX + rev_index<NX>(px)] = F[i];
+        }
+    } else {
+        const T invN = (T)(1.0 / (double)N);
+        for (int i = threadIdx.x; i < N; i += NT) {
+            int py = i / NX, px = i - py * NX;
+            F[i] = scale_c(in[rev_index<NY>(py) * NX + rev_index<NX>(px)], invN);
+        }
+        __syncthreads();
+        fft2d_inverse<T, NX, NY, 1, NT>(F, twx, twy);
+        for (int i = threadIdx.x; i < N; i += NT) out[i] = F[i];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host-side dispatch
+// ---------------------------------------------------------------------------
+template <typename T, int NX, int NY>
+static void launch_sized(int mode, const OpWork<T> *dwork, int nwork, int maxcols, const cplx<T> *twx,
+                         const cplx<T> *twy, hipStream_t st) {
+    constexpr int N = NX * NY;
+    constexpr int NT = N >= 4096 ? 1024 : (N >= 1024 ? 256 : 128);
+    dim3 grid(maxcols, nwork);
+    size_t sh1 = sizeof(cplx<T>) * N, sh2 = 2 * sh1;
+    // kernels using more than the default 64 KB of dynamic LDS must opt in
+    static const bool attrs_set = [&] {
+        BH_HIP_CHECK(hipFuncSetAttribute((const void *)op_kernel<T, NX, NY, MODE_THETA_TE, NT>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh2));
+        BH_HIP_CHECK(hipFuncSetAttribute((const void *)op_kernel<T, NX, NY, MODE_PREC_TP, NT>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh2));
+        BH_HIP_CHECK(hipFuncSetAttribute((const void *)op_kernel<T, NX, NY, MODE_THETA_TM, NT>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh1));
+        BH_HIP_CHECK(hipFuncSetAttribute((const void *)op_kernel<T, NX, NY, MODE_PREC_FD, NT>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh1));
+        BH_HIP_CHECK(hipFuncSetAttribute((const void *)fft_kernel<T, NX, NY, NT>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh1));
+        return true;
+    }();
+    (void)attrs_set;
+    switch (mode) {
+        case MODE_THETA_TM:
+            hipLaunchKernelGGL((op_kernel<T, NX, NY, MODE_THETA_TM, NT>), grid, dim3(NT), sh1, st, dwork, twx, twy);
+            break;
+        case MODE_THETA_TE:
+            hipLaunchKernelGGL((op_kernel<T, NX, NY, MODE_THETA_TE, NT>), grid, dim3(NT), sh2, st, dwork, twx, twy);
+            break;
+        case MODE_PREC_FD:
+            hipLaunchKernelGGL((op_kernel<T, NX, NY, MODE_PREC_FD, NT>), grid, dim3(NT), sh1, st, dwork, twx, twy);
+            break;
+        case MODE_PREC_TP:
+            hipLaunchKernelGGL((op_kernel<T, NX, NY, MODE_PREC_TP, NT>), grid, dim3(NT), sh2, st, dwork, twx, twy);
+            break;
+        case MODE_FFT_FWD:
+            hipLaunchKernelGGL((fft_kernel<T, NX, NY, NT>), grid, dim3(NT), sh1, st, dwork, twx, twy, 0);
+            break;
+        case MODE_FFT_INV:
+            hipLaunchKernelGGL((fft_kernel<T, NX, NY, NT>), grid, dim3(NT), sh1, st, dwork, twx, twy, 1);
+            break;
+        default:
+            throw std::runtime_error("unknown operator mode");
+    }
+    BH_HIP_CHECK(hipGetLastError());
+}
+
+bool grid_supported(int nx, int ny) {
+    return nx == ny && (nx == 8 || nx == 16 || nx == 24 || nx == 32 || nx == 48 || nx == 64);
+}
+
+template <typename T>
+void launch_op(int mode, int nx, int ny, const OpWork<T> *dwork, int nwork, int maxcols, const cplx<T> *twx,
+               const cplx<T> *twy, hipStream_t st) {
+    if (nwork <= 0 || maxcols <= 0) return;
+    if (nx != ny) throw std::runtime_error("only square grids are instantiated");
+    switch (nx) {
+        case 8: launch_sized<T, 8, 8>(mode, dwork, nwork, maxcols, twx, twy, st); break;
+        case 16: launch_sized<T, 16, 16>(mode, dwork, nwork, maxcols, twx, twy, st); break;
+        case 24: launch_sized<T, 24, 24>(mode, dwork, nwork, maxcols, twx, twy, st); break;
+        case 32: launch_sized<T, 32, 32>(mode, dwork, nwork, maxcols, twx, twy, st); break;
+        case 48: launch_sized<T, 48, 48>(mode, dwork, nwork, maxcols, twx, twy, st); break;
+        case 64: launch_sized<T, 64, 64>(mode, dwork, nwork, maxcols, twx, twy, st); break;
+        default: throw std::runtime_error("grid size " + std::to_string(nx) + " not instantiated");
+    }
+}
+
+template void launch_op<double>(int, int, int, const OpWork<double> *, int, int, const cplx<double> *,
+                                const cplx<double> *, hipStream_t);
+template void launch_op<float>(int, int, int, const OpWork<float> *, int, int, const cplx<float> *,
+                               const cplx<float> *, hipStream_t);
+
+}  // namespace bh

@@ -1,0 +1,156 @@
+/*
+ * blaze2d_hip: MI355X (gfx950) backend for the Blaze2D band-solve path.
+ *
+ * C ABI.  Plain pointers and sizes only; every call returns 0 on success
+ * and a negative status on failure, with the message in bh_last_error().
+ * Complex arrays are interleaved doubles (re, im).  Precision argument:
+ * 0 = FP64 storage, 1 = FP32 storage with FP64 accumulation.
+ *
+ * The entry points mirror the reference's boundary (Rust, relative to the
+ * reference repository root):
+ *   SpectralBackend / SpectralBuffer   crates/core/src/backend.rs:43-165
+ *   CpuBackend (what this replaces)    crates/backend-cpu/src/lib.rs:28-452
+ *   LinearOperator (Theta, mass)       crates/core/src/operators.rs:91-165
+ *   OperatorPreconditioner             crates/core/src/preconditioners.rs:102-113
+ *   Eigensolver::solve + run_core      crates/core/src/eigensolver.rs:1684-2192,
+ *                                      crates/core/src/drivers/bandstructure.rs:381-885
+ *   BulkDriver::benchmark job loop     crates/bulk-driver/src/driver.rs:1106-1257
+ */
+#ifndef BLAZE2D_HIP_H
+#define BLAZE2D_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BH_MAX_ATOMS 8
+
+typedef struct bh_context bh_context;
+typedef struct bh_buffer bh_buffer;
+typedef struct bh_theta bh_theta;
+
+/* Library identification. */
+const char *bh_version(void);
+/* Message of the last failing call on this thread ("" if none). */
+const char *bh_last_error(void);
+/* Number of visible HIP devices (0 when none). */
+int bh_device_count(void);
+
+/* Context on one device (one HIP stream).  Replaces CpuBackend::new
+ * (backend-cpu/src/lib.rs:40-90). */
+int bh_context_create(int device, bh_context **out);
+void bh_context_destroy(bh_context *ctx);
+int bh_synchronize(bh_context *ctx);
+
+/* ---- SpectralBackend level (backend.rs:73-165) ------------------------- */
+
+/* alloc_field: `count` zeroed fields of nx*ny complex values. */
+int bh_buffer_alloc(bh_context *ctx, int nx, int ny, int precision, int count, bh_buffer **out);
+void bh_buffer_free(bh_buffer *buf);
+/* Host (FP64 interleaved) <-> device (storage precision) for fields
+ * [first, first + count). */
+int bh_buffer_upload(bh_buffer *buf, int first, int count, const double *host);
+int bh_buffer_download(const bh_buffer *buf, int first, int count, double *host);
+/* forward_fft_2d (direction -1, unnormalised e^{-2 pi i jk/n}) or
+ * inverse_fft_2d (direction +1, scaled by 1/(nx ny)) of fields
+ * [first, first + count), in place (backend-cpu/src/lib.rs:94-275). */
+int bh_fft2d(bh_context *ctx, bh_buffer *buf, int first, int count, int direction);
+/* scale: x <- alpha x (backend-cpu/src/lib.rs:414-419). */
+int bh_scale(bh_context *ctx, bh_buffer *x, int ix, double alpha_re, double alpha_im);
+/* axpy: y <- alpha x + y (backend-cpu/src/lib.rs:421-426). */
+int bh_axpy(bh_context *ctx, double alpha_re, double alpha_im, const bh_buffer *x, int ix, bh_buffer *y, int iy);
+/* dot: sum conj(x) y accumulated in FP64 (backend-cpu/src/lib.rs:428-443). */
+int bh_dot(bh_context *ctx, const bh_buffer *x, int ix, const bh_buffer *y, int iy, double out[2]);
+/* gram_matrix: out[i*q + j] = <x_{xi0+i}, y_{yi0+j}>, row-major p x q
+ * (backend.rs:105-128). */
+int bh_gram(bh_context *ctx, const bh_buffer *x, int xi0, int p, const bh_buffer *y, int yi0, int q,
+            double *out);
+/* linear_combinations: out_j = sum_i q_{qi0+i} c[i + j*r], c col-major r x m
+ * (backend.rs:130-150). */
+int bh_lincomb(bh_context *ctx, const bh_buffer *q, int qi0, int r, const double *coeffs, int m, bh_buffer *out,
+               int oi0);
+
+/* ---- LinearOperator / OperatorPreconditioner level -------------------- */
+
+typedef struct {
+    int nx, ny;
+    int pol;                 /* 0 = TM, 1 = TE */
+    double b1[2], b2[2];     /* reciprocal lattice vectors (lattice.rs:170-180) */
+    double bloch[2];         /* Cartesian Bloch wavevector */
+    const double *eps;       /* N values: smoothed eps (TM mass, TP preconditioner) */
+    const double *tensor;    /* N*4 values [xx, xy, yx, yy] inverse-eps tensor (TE); may be NULL -> 1/eps */
+    int precond;             /* -1 auto (TM -> fourier-diagonal, TE -> transverse projection), 0 none, 1 FD, 2 TP */
+    int precision;
+} bh_theta_desc;
+
+/* ThetaOperator::new (maxwell.rs:126-176) plus the adaptive-shift
+ * preconditioner (maxwell.rs:218-379). */
+int bh_theta_create(bh_context *ctx, const bh_theta_desc *desc, bh_theta **out);
+void bh_theta_destroy(bh_theta *op);
+/* batch_apply: out_j = Theta in_j (maxwell.rs:647-713,1532-1542). */
+int bh_theta_apply(bh_theta *op, const bh_buffer *in, int ii0, bh_buffer *out, int oi0, int count);
+/* batch_apply_mass: out_j = B in_j (TM: eps, TE: identity) (maxwell.rs:1522-1530). */
+int bh_mass_apply(bh_theta *op, const bh_buffer *in, int ii0, bh_buffer *out, int oi0, int count);
+/* OperatorPreconditioner::batch_apply in place (fourier_diagonal.rs:310-342,
+ * transverse_projection.rs:167-231). */
+int bh_precond_apply(bh_theta *op, bh_buffer *buf, int first, int count);
+
+/* ---- Dielectric setup (dielectric.rs:146-210,473-574) ------------------ */
+
+typedef struct {
+    double a1[2], a2[2];     /* lattice vectors */
+    double eps_bg;
+    int n_atoms;
+    double atoms[BH_MAX_ATOMS][4]; /* fractional x, y, radius (units of |a1|), eps_inside */
+} bh_geometry;
+
+/* Smoothed eps, inverse eps and inverse-eps tensor on an nx x ny node grid. */
+int bh_dielectric(const bh_geometry *geom, int nx, int ny, int smoothing, double *eps, double *inv_eps,
+                  double *tensor);
+
+/* ---- Band structures (Eigensolver::solve driven by run_core) ----------- */
+
+typedef struct {
+    bh_geometry geom;
+    int nx, ny;
+    int pol;                 /* 0 TM, 1 TE */
+    int n_bands;
+    int max_iter;            /* default 200 */
+    double tol;              /* relative eigenvalue change (eigensolver.rs:345-400) */
+    int block_size;          /* 0 -> n_bands + 2 */
+    int smoothing;           /* 1 analytic smoothing (default) */
+    int precond;             /* -1 auto */
+    int subspace_prediction; /* 1 (default) */
+    int extrapolation;       /* 1 (default) */
+    int band_tracking;       /* 1 (default) */
+    int reuse_gamma;         /* 1 (default) */
+    int exit_rule;           /* 0: Eigensolver::solve() accounting; 1: soft-locked bands only */
+    int nk;
+    const double *kpath;     /* nk x 2 fractional k-points */
+} bh_job;
+
+typedef struct {
+    int slots;               /* concurrent band structures on the GPU (0 -> 64) */
+    int precision;           /* 0 FP64 */
+    int profile;             /* 1: time operator kernels with HIP events */
+} bh_run_options;
+
+typedef struct {
+    long steps;
+    long iterations;         /* LOBPCG iterations summed over solves */
+    long solves;             /* k-point solves (closing Gamma copies excluded) */
+    double seconds;          /* wall time of the solve loop */
+    double theta_ms;         /* summed Theta-kernel time (profile = 1) */
+    long theta_launches;
+} bh_run_stats;
+
+/* Runs every job to completion.  All jobs must share nx, ny.  Outputs per
+ * job j at offsets: omegas[(sum of nk*n_bands over jobs < j) ...] as
+ * omega (not omega / 2 pi), iters / converged per k-point. */
+int bh_bandstructure_run(bh_context *ctx, const bh_job *jobs, int njobs, const bh_run_options *opts,
+                         double *omegas, int *iters, int *converged, bh_run_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

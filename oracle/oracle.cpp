@@ -243,62 +243,93 @@ Dielectric build_dielectric(const Geometry &g, int nx, int ny, bool smoothing) {
 // ---------------------------------------------------------------------------
 struct FFTPlan {
     int n;
-    std::vector<int> fac;  // pairs (p, m)
-    std::vector<cd> tw;    // exp(-2 pi i k / n)
+    std::vector<int> radix;               // stage radices (4 first, then 2, 3, 5, ...)
+    std::vector<std::vector<cd>> tw;      // per stage: tw[k*(R-1) + (r-1)] = exp(-2 pi i k r / (Ns R))
 };
 
 FFTPlan make_plan(int n) {
     FFTPlan P;
     P.n = n;
     int m = n;
-    int p = 4;
-    while (m > 1) {
-        while (m % p) {
-            switch (p) {
-                case 4: p = 2; break;
-                case 2: p = 3; break;
-                default: p += 2; break;
+    while (m % 4 == 0) { P.radix.push_back(4); m /= 4; }
+    while (m % 2 == 0) { P.radix.push_back(2); m /= 2; }
+    for (int p = 3; m > 1; p += 2)
+        while (m % p == 0) { P.radix.push_back(p); m /= p; }
+    int Ns = 1;
+    for (int R : P.radix) {
+        std::vector<cd> t((size_t)Ns * (R - 1));
+        for (int k = 0; k < Ns; ++k)
+            for (int r = 1; r < R; ++r) {
+                double ang = -2.0 * PI * (double)k * r / ((double)Ns * R);
+                t[(size_t)k * (R - 1) + (r - 1)] = cd(std::cos(ang), std::sin(ang));
             }
-            if (p * p > m) p = m;
-        }
-        m /= p;
-        P.fac.push_back(p);
-        P.fac.push_back(m);
-    }
-    P.tw.resize(n);
-    for (int k = 0; k < n; ++k) {
-        double ang = -2.0 * PI * (double)k / (double)n;
-        P.tw[k] = cd(std::cos(ang), std::sin(ang));
+        P.tw.push_back(std::move(t));
+        Ns *= R;
     }
     return P;
 }
 
-void fft_work(cd *out, const cd *in, size_t fstride, size_t istride, const int *fac,
-              const FFTPlan &P, bool inv) {
-    int p = fac[0], m = fac[1];
-    if (m == 1) {
-        for (int k = 0; k < p; ++k) out[k] = in[k * fstride * istride];
+inline cd cmul_(cd a, cd b) {
+    return cd(a.real() * b.real() - a.imag() * b.imag(), a.real() * b.imag() + a.imag() * b.real());
+}
+
+// R-point DFT in place on v (forward e^{-2 pi i/R}, inverse e^{+})
+inline void small_dft(cd *v, int R, bool inv) {
+    if (R == 2) {
+        cd a = v[0], b = v[1];
+        v[0] = a + b; v[1] = a - b;
+    } else if (R == 4) {
+        cd s02 = v[0] + v[2], d02 = v[0] - v[2], s13 = v[1] + v[3], d13 = v[1] - v[3];
+        cd jd13 = inv ? cd(-d13.imag(), d13.real()) : cd(d13.imag(), -d13.real());  // (+/-i) d13
+        v[0] = s02 + s13; v[2] = s02 - s13;
+        v[1] = d02 + jd13; v[3] = d02 - jd13;
+    } else if (R == 3) {
+        const double s = 0.86602540378443864676;
+        cd t1 = v[1] + v[2], t2 = v[0] - 0.5 * t1;
+        cd d = v[1] - v[2];
+        cd t3 = inv ? cd(-s * d.imag(), s * d.real()) : cd(s * d.imag(), -s * d.real());
+        v[0] = v[0] + t1; v[1] = t2 + t3; v[2] = t2 - t3;
     } else {
-        for (int k = 0; k < p; ++k)
-            fft_work(out + k * m, in + k * fstride * istride, fstride * p, istride, fac + 2, P, inv);
-    }
-    cd scratch[64];
-    for (int u = 0; u < m; ++u) {
-        for (int q = 0; q < p; ++q) scratch[q] = out[q * m + u];
-        for (int q1 = 0; q1 < p; ++q1) {
-            int k = q1 * m + u;
-            cd acc = scratch[0];
-            size_t twidx = 0;
-            for (int q = 1; q < p; ++q) {
-                twidx += fstride * (size_t)k;
-                twidx %= (size_t)P.n;
-                cd w = P.tw[twidx];
-                if (inv) w = std::conj(w);
-                acc += scratch[q] * w;
+        cd tmp[64];
+        for (int k = 0; k < R; ++k) {
+            cd acc = 0;
+            for (int r = 0; r < R; ++r) {
+                double ang = (inv ? 2.0 : -2.0) * PI * (double)((k * r) % R) / R;
+                acc += cmul_(v[r], cd(std::cos(ang), std::sin(ang)));
             }
-            out[k] = acc;
+            tmp[k] = acc;
         }
+        for (int k = 0; k < R; ++k) v[k] = tmp[k];
     }
+}
+
+// Stockham autosort (natural order in and out); x and y are length-n buffers.
+// Returns the buffer holding the result.
+cd *stockham(const FFTPlan &P, cd *x, cd *y, bool inv) {
+    const int n = P.n;
+    int Ns = 1;
+    cd v[64];
+    for (size_t s = 0; s < P.radix.size(); ++s) {
+        const int R = P.radix[s];
+        const int stride = n / R;
+        const cd *tw = P.tw[s].data();
+        for (int j = 0; j < stride; ++j) {
+            const int k = j % Ns;
+            for (int r = 0; r < R; ++r) v[r] = x[j + r * stride];
+            if (k) {
+                for (int r = 1; r < R; ++r) {
+                    cd w = tw[(size_t)k * (R - 1) + (r - 1)];
+                    v[r] = cmul_(v[r], inv ? std::conj(w) : w);
+                }
+            }
+            small_dft(v, R, inv);
+            const int base = (j / Ns) * Ns * R + k;
+            for (int r = 0; r < R; ++r) y[base + r * Ns] = v[r];
+        }
+        std::swap(x, y);
+        Ns *= R;
+    }
+    return x;
 }
 
 std::mutex plan_mu;
@@ -312,20 +343,18 @@ std::shared_ptr<FFTPlan> get_plan(int n) {
     return p;
 }
 
-void fft1d(const FFTPlan &P, const cd *in, size_t istride, cd *out, bool inv) {
-    fft_work(out, in, 1, istride, P.fac.data(), P, inv);
-}
-
 void fft2d(int nx, int ny, cd *data, bool inv) {
     auto px = get_plan(nx), py = get_plan(ny);
-    std::vector<cd> tmp(std::max(nx, ny)), col(ny);
+    std::vector<cd> a(std::max(nx, ny)), b(std::max(nx, ny));
     for (int iy = 0; iy < ny; ++iy) {
-        fft1d(*px, data + (size_t)iy * nx, 1, tmp.data(), inv);
-        std::copy(tmp.begin(), tmp.begin() + nx, data + (size_t)iy * nx);
+        std::copy(data + (size_t)iy * nx, data + (size_t)iy * nx + nx, a.begin());
+        cd *r = stockham(*px, a.data(), b.data(), inv);
+        std::copy(r, r + nx, data + (size_t)iy * nx);
     }
     for (int ix = 0; ix < nx; ++ix) {
-        fft1d(*py, data + ix, nx, tmp.data(), inv);
-        for (int iy = 0; iy < ny; ++iy) data[(size_t)iy * nx + ix] = tmp[iy];
+        for (int iy = 0; iy < ny; ++iy) a[iy] = data[(size_t)iy * nx + ix];
+        cd *r = stockham(*py, a.data(), b.data(), inv);
+        for (int iy = 0; iy < ny; ++iy) data[(size_t)iy * nx + ix] = r[iy];
     }
     if (inv) {
         double s = 1.0 / ((double)nx * ny);
