@@ -73,8 +73,13 @@ class RunStats(ctypes.Structure):
         ("iterations", ctypes.c_long),
         ("solves", ctypes.c_long),
         ("seconds", ctypes.c_double),
-        ("theta_ms", ctypes.c_double),
-        ("theta_launches", ctypes.c_long),
+        ("theta_ms", ctypes.c_double * 2),
+        ("theta_launches", ctypes.c_long * 2),
+        ("theta_columns", ctypes.c_long * 2),
+        ("eigh_calls", ctypes.c_long * 2),
+        ("eigh_sweeps", ctypes.c_long * 2),
+        ("eigh_max_sweeps", ctypes.c_long * 2),
+        ("eigh_launch_max_sum", ctypes.c_long * 2),
     ]
 
 
@@ -82,7 +87,7 @@ EXPORTS = [
     "bh_version", "bh_last_error", "bh_device_count", "bh_context_create", "bh_context_destroy", "bh_synchronize",
     "bh_buffer_alloc", "bh_buffer_free", "bh_buffer_upload", "bh_buffer_download", "bh_fft2d", "bh_scale",
     "bh_axpy", "bh_dot", "bh_gram", "bh_lincomb", "bh_theta_create", "bh_theta_destroy", "bh_theta_apply",
-    "bh_mass_apply", "bh_precond_apply", "bh_dielectric", "bh_bandstructure_run",
+    "bh_mass_apply", "bh_precond_apply", "bh_dielectric", "bh_bandstructure_run", "bh_hermitian_eigh",
 ]
 
 _lib = None
@@ -114,6 +119,7 @@ def lib() -> ctypes.CDLL:
     L.bh_axpy.argtypes = [vp, ctypes.c_double, ctypes.c_double, vp, ctypes.c_int, vp, ctypes.c_int]
     L.bh_dot.argtypes = [vp, vp, ctypes.c_int, vp, ctypes.c_int, c_dp]
     L.bh_gram.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, vp, ctypes.c_int, ctypes.c_int, c_dp]
+    L.bh_hermitian_eigh.argtypes = [vp, ctypes.c_int, ctypes.c_int, c_dp, c_dp, c_dp]
     L.bh_lincomb.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, c_dp, ctypes.c_int, vp, ctypes.c_int]
     L.bh_theta_create.argtypes = [vp, ctypes.POINTER(ThetaDesc), ctypes.POINTER(vp)]
     L.bh_theta_destroy.argtypes = [vp]

@@ -70,6 +70,20 @@ class HipBackend:
                                 out.view(np.float64).ctypes.data_as(L.c_dp)), "gram_matrix")
         return out
 
+    def hermitian_eigh(self, mats: np.ndarray):
+        """Batched dense Hermitian eigendecomposition (ascending), FP64 on the device."""
+        a = np.asarray(mats, dtype=np.complex128)
+        if a.ndim == 2:
+            a = a[None]
+        cnt, n, _ = a.shape
+        flat = np.ascontiguousarray(np.transpose(a, (0, 2, 1)).reshape(cnt, -1))  # col-major per matrix
+        w = np.zeros((cnt, n))
+        v = np.zeros((cnt, n * n), dtype=np.complex128)
+        L.check(L.lib().bh_hermitian_eigh(self._ctx, n, cnt, flat.view(np.float64).ctypes.data_as(L.c_dp),
+                                          w.ctypes.data_as(L.c_dp), v.view(np.float64).ctypes.data_as(L.c_dp)),
+                "hermitian_eigh")
+        return w, np.transpose(v.reshape(cnt, n, n), (0, 2, 1))
+
     def linear_combinations(self, q: "Field", coeffs: np.ndarray, out: "Field", qi0: int = 0, oi0: int = 0):
         c = np.asfortranarray(coeffs, dtype=np.complex128)
         r, m = c.shape

@@ -148,8 +148,13 @@ class BatchStats:
     iterations: int
     solves: int
     seconds: float
-    theta_ms: float
-    theta_launches: int
+    theta_ms: tuple        # per polarisation (TM, TE), profile mode only
+    theta_launches: tuple
+    theta_columns: tuple
+    eigh_calls: tuple = (0, 0)      # (SVQB, Rayleigh-Ritz), profile mode only
+    eigh_sweeps: tuple = (0, 0)
+    eigh_max_sweeps: tuple = (0, 0)
+    eigh_launch_max_sum: tuple = (0, 0)
 
 
 def run_batch(backend: HipBackend, jobs: Sequence[BandStructureJob], options: Optional[RunOptions] = None,
@@ -158,7 +163,7 @@ def run_batch(backend: HipBackend, jobs: Sequence[BandStructureJob], options: Op
     opts = options or RunOptions()
     jobs = list(jobs)
     if not jobs:
-        return [], BatchStats(0, 0, 0, 0.0, 0.0, 0)
+        return [], BatchStats(0, 0, 0, 0.0, (0.0, 0.0), (0, 0), (0, 0))
     arr = (L.Job * len(jobs))()
     for i, j in enumerate(jobs):
         arr[i] = j.to_c(opts)
@@ -180,7 +185,9 @@ def run_batch(backend: HipBackend, jobs: Sequence[BandStructureJob], options: Op
                                        it[k:k + nk].copy(), cv[k:k + nk].copy()))
         o += nk * j.n_bands
         k += nk
-    stats = BatchStats(st.steps, st.iterations, st.solves, st.seconds, st.theta_ms, st.theta_launches)
+    stats = BatchStats(st.steps, st.iterations, st.solves, st.seconds, tuple(st.theta_ms), tuple(st.theta_launches),
+                       tuple(st.theta_columns), tuple(st.eigh_calls), tuple(st.eigh_sweeps),
+                       tuple(st.eigh_max_sweeps), tuple(st.eigh_launch_max_sum))
     return out, stats
 
 

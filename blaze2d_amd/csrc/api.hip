@@ -334,6 +334,49 @@ int bh_lincomb(bh_context *ctx, const bh_buffer *qb, int qi0, int r, const doubl
     BH_API_CATCH
 }
 
+int bh_hermitian_eigh(bh_context *ctx, int n, int count, const double *a, double *w, double *v) {
+    BH_API_TRY
+    if (!ctx || n <= 0 || n > EIGH_MAXN || count < 0 || !a || !w || !v) return fail("bh_hermitian_eigh: bad arguments");
+    if (count == 0) return 0;
+    const size_t nn = (size_t)n * n;
+    // device layout: [work items][partials (row-major G(i,j))][eigenvalues][eigenvectors]
+    size_t bytes = 256 + count * sizeof(EighWork) + count * (2 * nn + n + 2 * nn) * sizeof(double);
+    char *base = (char *)scratch(ctx, bytes);
+    EighWork *dw = (EighWork *)(base + 256);
+    double *dpart = (double *)(dw + count);
+    double *dwv = dpart + 2 * nn * count;
+    double *dv = dwv + (size_t)n * count;
+    std::vector<double> hp(2 * nn * count);
+    for (int c = 0; c < count; ++c)
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) {  // row-major G(i, j) = A(i, j)
+                hp[2 * (c * nn + (size_t)i * n + j)] = a[2 * (c * nn + i + (size_t)j * n)];
+                hp[2 * (c * nn + (size_t)i * n + j) + 1] = a[2 * (c * nn + i + (size_t)j * n) + 1];
+            }
+    std::vector<EighWork> hw(count);
+    for (int c = 0; c < count; ++c) {
+        EighWork e{};
+        e.partial = dpart + 2 * nn * c;
+        e.w = dwv + (size_t)n * c;
+        e.V = dv + 2 * nn * c;
+        e.n = n;
+        e.nchunks = 1;
+        e.ld = n;
+        e.mode = 0;
+        e.ldv = n;
+        e.nsplit = n;
+        hw[c] = e;
+    }
+    BH_HIP_CHECK(hipMemcpyAsync(dw, hw.data(), sizeof(EighWork) * count, hipMemcpyHostToDevice, ctx->st));
+    BH_HIP_CHECK(hipMemcpyAsync(dpart, hp.data(), sizeof(double) * hp.size(), hipMemcpyHostToDevice, ctx->st));
+    launch_eigh(dw, count, n, ctx->st);
+    BH_HIP_CHECK(hipMemcpyAsync(w, dwv, sizeof(double) * n * count, hipMemcpyDeviceToHost, ctx->st));
+    BH_HIP_CHECK(hipMemcpyAsync(v, dv, sizeof(double) * 2 * nn * count, hipMemcpyDeviceToHost, ctx->st));
+    BH_HIP_CHECK(hipStreamSynchronize(ctx->st));
+    return 0;
+    BH_API_CATCH
+}
+
 int bh_theta_create(bh_context *ctx, const bh_theta_desc *d, bh_theta **out) {
     BH_API_TRY
     if (!ctx || !d || !out || !d->eps) return fail("bh_theta_create: bad arguments");
@@ -535,8 +578,15 @@ int bh_bandstructure_run(bh_context *ctx, const bh_job *jobs, int njobs, const b
         stats->iterations = rs.iterations;
         stats->solves = rs.solves;
         stats->seconds = rs.seconds;
-        stats->theta_ms = rs.theta_ms;
-        stats->theta_launches = rs.theta_launches;
+        for (int p = 0; p < 2; ++p) {
+            stats->theta_ms[p] = rs.theta_ms[p];
+            stats->theta_launches[p] = rs.theta_launches[p];
+            stats->theta_columns[p] = rs.theta_columns[p];
+            stats->eigh_calls[p] = rs.eigh_calls[p];
+            stats->eigh_sweeps[p] = rs.eigh_sweeps[p];
+            stats->eigh_max_sweeps[p] = rs.eigh_max_sweeps[p];
+            stats->eigh_launch_max_sum[p] = rs.eigh_launch_max_sum[p];
+        }
     }
     return 0;
     BH_API_CATCH

@@ -18,7 +18,8 @@ struct GramWork {
     const int *dev_nb;
     double *partial;        // [nchunks][ld][ld] complex (interleaved re, im)
     long a_stride, b_stride;
-    int na, nb, ld, pad;
+    int na, nb, ld;
+    int lower_only;         // Hermitian products: only i >= j (16 x 16 tiles on or below the diagonal)
 };
 
 // out_j = sum_i A_i C(i, j)            (mode 0)
@@ -51,6 +52,7 @@ struct EighWork {
     const int *dev_n;
     int n, nchunks, ld, mode;  // mode 0 = Rayleigh-Ritz, 1 = SVQB
     int ldv, nsplit;
+    int *sweeps_out;        // optional: Jacobi sweeps used (diagnostics)
 };
 
 constexpr int GRAM_MAXC = 48;  // maximum columns per Gram operand

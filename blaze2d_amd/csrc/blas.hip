@@ -24,6 +24,7 @@ __global__ void __launch_bounds__(256) gram_kernel(const GramWork<T> *__restrict
     __shared__ cplx<T> As[GRAM_SR][GRAM_MAXC + 1];
     __shared__ cplx<T> Bs[GRAM_SR][GRAM_MAXC + 1];
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    const bool lower = w.lower_only != 0;
     double ar[3][3] = {}, ai[3][3] = {};
     const long r0 = (long)blockIdx.x * rows_per_chunk;
     const long r1 = r0 + rows_per_chunk < nrows ? r0 + rows_per_chunk : nrows;
@@ -63,6 +64,7 @@ __global__ void __launch_bounds__(256) gram_kernel(const GramWork<T> *__restrict
             for (int u = 0; u < 3; ++u)
 #pragma unroll
                 for (int v = 0; v < 3; ++v) {
+                    if (lower && v > u) continue;
                     ar[u][v] += axr[u] * bxr[v] + axi[u] * bxi[v];
                     ai[u][v] += axr[u] * bxi[v] - axi[u] * bxr[v];
                 }
@@ -279,7 +281,7 @@ __global__ void __launch_bounds__(EIGH_NT) eigh_kernel(const EighWork *__restric
         V[e] = make_double2(i == j ? 1.0 : 0.0, 0.0);
     }
     __syncthreads();
-    if (w.mode == 1) {
+    if (w.mode >= 1) {
         for (int i = tid; i < ne; i += EIGH_NT) {
             double nrm = i < n ? sqrt(fmax(A[i + i * ne].x, 0.0)) : 0.0;
             D[i] = nrm > 1e-30 ? 1.0 / nrm : 1.0;
@@ -293,8 +295,83 @@ __global__ void __launch_bounds__(EIGH_NT) eigh_kernel(const EighWork *__restric
         }
         __syncthreads();
     }
-    for (int sweep = 0; sweep < 60; ++sweep) {
-        if (tid == 0) nrot = 0;
+    if (w.mode == 2) {
+        // Cholesky QR of the scaled Gram G = L L^H, T = D L^{-H}: the same span
+        // as SVQB when no direction would be dropped.  Falls back to the SVQB
+        // eigendecomposition below when a pivot shows G is ill-conditioned.
+        double2 *L = V;  // V is free until the Jacobi path starts
+        __shared__ int chol_ok;
+        if (tid == 0) chol_ok = 1;
+        for (int e = tid; e < ne * ne; e += EIGH_NT) L[e] = A[e];
+        __syncthreads();
+        for (int j = 0; j < n; ++j) {
+            if (tid == 0) {
+                double d = L[j + j * ne].x;
+                for (int k = 0; k < j; ++k) {
+                    double2 l = L[j + k * ne];
+                    d -= l.x * l.x + l.y * l.y;
+                }
+                if (!(d > 1e-8)) chol_ok = 0;  // relative pivot: columns have unit B-norm
+                L[j + j * ne] = make_double2(d > 0.0 ? sqrt(d) : 1.0, 0.0);
+            }
+            __syncthreads();
+            if (!chol_ok) break;
+            const double ljj = L[j + j * ne].x;
+            for (int i = j + 1 + tid; i < n; i += EIGH_NT) {
+                double2 a = L[i + j * ne];
+                for (int k = 0; k < j; ++k) {  // a -= L[i,k] conj(L[j,k])
+                    double2 x = L[i + k * ne], y = L[j + k * ne];
+                    a.x -= x.x * y.x + x.y * y.y;
+                    a.y -= x.y * y.x - x.x * y.y;
+                }
+                L[i + j * ne] = make_double2(a.x / ljj, a.y / ljj);
+            }
+            __syncthreads();
+        }
+        if (chol_ok) {
+            // M = L^{-1} by forward substitution, one column per thread; T = D M^H
+            for (int c = tid; c < n; c += EIGH_NT) {
+                // M[i][c] for i >= c; kept in the A buffer's column c (A no longer needed)
+                for (int i = 0; i < n; ++i) A[i + c * ne] = make_double2(0.0, 0.0);
+                for (int i = c; i < n; ++i) {
+                    double2 acc = make_double2(i == c ? 1.0 : 0.0, 0.0);
+                    for (int k = c; k < i; ++k) {
+                        double2 l = L[i + k * ne], m = A[k + c * ne];
+                        acc.x -= l.x * m.x - l.y * m.y;
+                        acc.y -= l.x * m.y + l.y * m.x;
+                    }
+                    double lii = L[i + i * ne].x;
+                    A[i + c * ne] = make_double2(acc.x / lii, acc.y / lii);
+                }
+            }
+            __syncthreads();
+            for (int e = tid; e < n * n; e += EIGH_NT) {
+                int r = e % n, c = e / n;  // T[r][c] = D[r] conj(M[c][r]), zero below the diagonal
+                double2 m = r <= c ? A[c + r * ne] : make_double2(0.0, 0.0);
+                size_t o = 2 * ((size_t)r + (size_t)c * w.ldv);
+                w.V[o] = D[r] * m.x;
+                w.V[o + 1] = -D[r] * m.y;
+            }
+            if (tid == 0) {
+                if (w.rank_out) *w.rank_out = n;
+                if (w.sweeps_out) atomicAdd(w.sweeps_out + 1, 1);  // Cholesky path taken
+            }
+            return;
+        }
+        // fallback: restore V = I and run SVQB's eigendecomposition
+        for (int e = tid; e < ne * ne; e += EIGH_NT) {
+            int i = e % ne, j = e / ne;
+            V[e] = make_double2(i == j ? 1.0 : 0.0, 0.0);
+        }
+        __syncthreads();
+    }
+    // Each round applies np disjoint rotations.  The matrix is partitioned
+    // into np x np 2x2 blocks (row pair, column pair); a thread owning a block
+    // applies both the column rotation of its column pair and the row
+    // rotation of its row pair in registers, so a round needs two barriers.
+    __shared__ unsigned long long maxratio_bits;  // non-negative doubles order like their bit patterns
+    for (int sweep = 0; sweep < 40; ++sweep) {
+        if (tid == 0) { nrot = 0; maxratio_bits = 0ull; }
         __syncthreads();
         for (int r = 0; r < ne - 1; ++r) {
             if (tid < np) {
@@ -308,15 +385,17 @@ __global__ void __launch_bounds__(EIGH_NT) eigh_kernel(const EighWork *__restric
                 double c = 1.0;
                 int flag = 0;
                 double2 sg = make_double2(0.0, 0.0);
-                if (habs != 0.0 && habs > 1e-17 * sqrt(fabs(app) * fabs(aqq))) {
+                double scale = sqrt(fabs(app) * fabs(aqq));
+                if (habs != 0.0 && habs > 2.2e-16 * scale) {
                     flag = 1;
                     double tau = (aqq - app) / (2.0 * habs);
                     double t = (tau >= 0 ? -1.0 : 1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
                     c = 1.0 / sqrt(1.0 + t * t);
                     double s = t * c;
-                    // sigma = s * conj(h / |h|)
-                    sg = make_double2(s * h.x / habs, -s * h.y / habs);
+                    sg = make_double2(s * h.x / habs, -s * h.y / habs);  // sigma = s conj(h/|h|)
                     atomicAdd(&nrot, 1);
+                    double ratio = scale > 0.0 ? habs / scale : 1.0;
+                    atomicMax(&maxratio_bits, (unsigned long long)__double_as_longlong(ratio));
                 } else if (habs != 0.0) {
                     flag = 2;
                 }
@@ -327,18 +406,55 @@ __global__ void __launch_bounds__(EIGH_NT) eigh_kernel(const EighWork *__restric
                 rs[tid] = sg;
             }
             __syncthreads();
-            // columns: A[:,p] = c A[:,p] + sg A[:,q]; A[:,q] = -conj(sg) A[:,p] + c A[:,q]; same for V
-            for (int e = tid; e < np * ne; e += EIGH_NT) {
+            for (int b = tid; b < np * np; b += EIGH_NT) {
+                const int t1 = b / np, t2 = b - t1 * np;  // row pair, column pair
+                const int f1 = rf[t1], f2 = rf[t2];
+                if (f1 != 1 && f2 != 1) {
+                    if (t1 == t2 && f1 == 2) {
+                        int p = rp[t1], q = rq[t1];
+                        A[p + q * ne] = make_double2(0.0, 0.0);
+                        A[q + p * ne] = make_double2(0.0, 0.0);
+                    }
+                    continue;
+                }
+                const int p1 = rp[t1], q1 = rq[t1], p2 = rp[t2], q2 = rq[t2];
+                double2 a11 = A[p1 + p2 * ne], a12 = A[p1 + q2 * ne];
+                double2 a21 = A[q1 + p2 * ne], a22 = A[q1 + q2 * ne];
+                if (f2 == 1) {  // columns: [:,p] = c [:,p] + s [:,q]; [:,q] = -conj(s) [:,p] + c [:,q]
+                    const double c = rc[t2];
+                    const double2 s = rs[t2];
+                    double2 n11 = make_double2(c * a11.x + s.x * a12.x - s.y * a12.y, c * a11.y + s.x * a12.y + s.y * a12.x);
+                    double2 n12 = make_double2(-(s.x * a11.x + s.y * a11.y) + c * a12.x, -(s.x * a11.y - s.y * a11.x) + c * a12.y);
+                    double2 n21 = make_double2(c * a21.x + s.x * a22.x - s.y * a22.y, c * a21.y + s.x * a22.y + s.y * a22.x);
+                    double2 n22 = make_double2(-(s.x * a21.x + s.y * a21.y) + c * a22.x, -(s.x * a21.y - s.y * a21.x) + c * a22.y);
+                    a11 = n11; a12 = n12; a21 = n21; a22 = n22;
+                }
+                if (f1 == 1) {  // rows: [p,:] = c [p,:] + conj(s) [q,:]; [q,:] = -s [p,:] + c [q,:]
+                    const double c = rc[t1];
+                    const double2 s = rs[t1];
+                    double2 n11 = make_double2(c * a11.x + s.x * a21.x + s.y * a21.y, c * a11.y + s.x * a21.y - s.y * a21.x);
+                    double2 n21 = make_double2(-(s.x * a11.x - s.y * a11.y) + c * a21.x, -(s.x * a11.y + s.y * a11.x) + c * a21.y);
+                    double2 n12 = make_double2(c * a12.x + s.x * a22.x + s.y * a22.y, c * a12.y + s.x * a22.y - s.y * a22.x);
+                    double2 n22 = make_double2(-(s.x * a12.x - s.y * a12.y) + c * a22.x, -(s.x * a12.y + s.y * a12.x) + c * a22.y);
+                    a11 = n11; a12 = n12; a21 = n21; a22 = n22;
+                }
+                if (t1 == t2) {  // the rotated pair: exact zero off-diagonal, real diagonal
+                    a12 = make_double2(0.0, 0.0);
+                    a21 = make_double2(0.0, 0.0);
+                    a11.y = 0.0;
+                    a22.y = 0.0;
+                }
+                A[p1 + p2 * ne] = a11;
+                A[p1 + q2 * ne] = a12;
+                A[q1 + p2 * ne] = a21;
+                A[q1 + q2 * ne] = a22;
+            }
+            for (int e = tid; e < np * ne; e += EIGH_NT) {  // eigenvectors: column rotations only
                 int t = e / ne, k = e - t * ne;
                 if (rf[t] != 1) continue;
                 double c = rc[t];
-                int p = rp[t], q = rq[t];
                 double2 sg = rs[t];
-                double2 akp = A[k + p * ne], akq = A[k + q * ne];
-                A[k + p * ne] = make_double2(c * akp.x + sg.x * akq.x - sg.y * akq.y,
-                                             c * akp.y + sg.x * akq.y + sg.y * akq.x);
-                A[k + q * ne] = make_double2(-(sg.x * akp.x + sg.y * akp.y) + c * akq.x,
-                                             -(sg.x * akp.y - sg.y * akp.x) + c * akq.y);
+                int p = rp[t], q = rq[t];
                 double2 vkp = V[k + p * ne], vkq = V[k + q * ne];
                 V[k + p * ne] = make_double2(c * vkp.x + sg.x * vkq.x - sg.y * vkq.y,
                                              c * vkp.y + sg.x * vkq.y + sg.y * vkq.x);
@@ -346,30 +462,17 @@ __global__ void __launch_bounds__(EIGH_NT) eigh_kernel(const EighWork *__restric
                                              -(sg.x * vkp.y - sg.y * vkp.x) + c * vkq.y);
             }
             __syncthreads();
-            // rows: A[p,:] = c A[p,:] + conj(sg) A[q,:]; A[q,:] = -sg A[p,:] + c A[q,:]
-            for (int e = tid; e < np * ne; e += EIGH_NT) {
-                int t = e / ne, k = e - t * ne;
-                if (rf[t] != 1) continue;
-                double c = rc[t];
-                int p = rp[t], q = rq[t];
-                double2 sg = rs[t];
-                double2 apk = A[p + k * ne], aqk = A[q + k * ne];
-                A[p + k * ne] = make_double2(c * apk.x + sg.x * aqk.x + sg.y * aqk.y,
-                                             c * apk.y + sg.x * aqk.y - sg.y * aqk.x);
-                A[q + k * ne] = make_double2(-(sg.x * apk.x - sg.y * apk.y) + c * aqk.x,
-                                             -(sg.x * apk.y + sg.y * apk.x) + c * aqk.y);
-            }
-            __syncthreads();
-            if (tid < np && rf[tid] != 0) {
-                int p = rp[tid], q = rq[tid];
-                A[p + q * ne] = make_double2(0.0, 0.0);
-                A[q + p * ne] = make_double2(0.0, 0.0);
-                A[p + p * ne].y = 0.0;
-                A[q + q * ne].y = 0.0;
-            }
-            __syncthreads();
         }
-        if (nrot == 0) break;
+        // quadratic convergence: rotations of relative size < 1e-9 leave
+        // off-diagonal terms below 1e-18 relative, so no further sweep is needed
+        const bool done = nrot == 0 || __longlong_as_double((long long)maxratio_bits) < 1e-9;
+        if (done || sweep == 39) {
+            if (tid == 0 && w.sweeps_out) {
+                atomicAdd(w.sweeps_out, sweep + 1);
+                atomicMax(w.sweeps_out + 2, sweep + 1);  // maximum over the launch
+            }
+            break;
+        }
         __syncthreads();
     }
     // stable ascending ranks over the first n diagonal entries

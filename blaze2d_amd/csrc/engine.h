@@ -39,9 +39,13 @@ struct RunStats {
     long iterations = 0;          // total LOBPCG iterations over all solves
     long solves = 0;              // k-point solves (excluding reused closing Gamma)
     double seconds = 0;
-    double theta_ms = 0;          // accumulated operator-kernel time (events), if profiled
-    long theta_launches = 0;
-    long theta_columns = 0;
+    double theta_ms[2] = {0, 0};  // Theta-kernel time per polarisation [TM, TE] (events, profile mode)
+    long theta_launches[2] = {0, 0};
+    long theta_columns[2] = {0, 0};  // columns processed by the timed launches
+    long eigh_calls[2] = {0, 0};     // [SVQB, Rayleigh-Ritz] problems solved (profile mode)
+    long eigh_sweeps[2] = {0, 0};    // Jacobi sweeps summed over those problems
+    long eigh_max_sweeps[2] = {0, 0};      // largest sweep count seen in one problem
+    long eigh_launch_max_sum[2] = {0, 0};  // per launch maximum, summed over launches
 };
 
 struct EngineOptions {

@@ -154,6 +154,7 @@ class Engine : public EngineBase {
         d_part = dalloc<double>((size_t)(2048 + 2 * S) * LD * LD * 2);
         d_rank = dalloc<int>(S);
         d_cnt = dalloc<int>(2 * (size_t)S);
+        d_sweeps = dalloc<int>(6);
         d_gramrb = dalloc<double>((size_t)S * LD * LD * 2);
         BH_HIP_CHECK(hipHostMalloc((void **)&h_lam, sizeof(double) * S * LD, hipHostMallocDefault));
         BH_HIP_CHECK(hipHostMalloc((void **)&h_rank, sizeof(int) * S, hipHostMallocDefault));
@@ -182,7 +183,7 @@ class Engine : public EngineBase {
     ~Engine() override {
         (void)hipStreamSynchronize(st);
         for (void *p : std::initializer_list<void *>{V, BV, AV, Q, BQ, AQ, H, RND, EPS, TEN, d_lam, d_svw, d_Y, d_Yw,
-                                                     d_T, d_coef, d_part, d_rank, d_cnt, d_gramrb, twx, twy})
+                                                     d_T, d_coef, d_part, d_rank, d_cnt, d_sweeps, d_gramrb, twx, twy})
             if (p) (void)hipFree(p);
         (void)hipHostFree(h_lam);
         (void)hipHostFree(h_rank);
@@ -203,7 +204,7 @@ class Engine : public EngineBase {
     cplx<T> *V{}, *BV{}, *AV{}, *Q{}, *BQ{}, *AQ{}, *H{}, *RND{};
     T *EPS{}, *TEN{};
     double *d_lam{}, *d_svw{}, *d_Y{}, *d_Yw{}, *d_T{}, *d_coef{}, *d_part{}, *d_gramrb{};
-    int *d_rank{}, *d_cnt{};
+    int *d_rank{}, *d_cnt{}, *d_sweeps{};
     double *h_lam{}, *h_gramrb{};
     int *h_rank{}, *h_cnt{};
     cplx<T> *twx{}, *twy{};
@@ -918,15 +919,17 @@ void Engine<T>::execute(StepPlan &plan) {
         gs.na = p;
         gs.nb = p;
         gs.ld = LD;
+        gs.lower_only = 1;
         g_svqb.push_back(gs);
         EighWork es{};
         es.n = p;
         es.ld = LD;
-        es.mode = 1;
+        es.mode = 2;  // Cholesky QR, SVQB eigendecomposition when ill-conditioned
         es.w = d_svw + (size_t)s * LD;
         es.V = d_T + (size_t)s * LD * LD * 2;
         es.rank_out = d_rank + s;
         es.ldv = LD;
+        if (eopt.profile) es.sweeps_out = d_sweeps;
         e_svqb.push_back(es);
         // Q = Z T, BQ = BZ T
         LincombWork<T> lq{};
@@ -963,6 +966,7 @@ void Engine<T>::execute(StepPlan &plan) {
         gr.dev_na = d_rank + s;
         gr.dev_nb = d_rank + s;
         gr.ld = LD;
+        gr.lower_only = 1;
         g_rr.push_back(gr);
         EighWork er{};
         er.dev_n = d_rank + s;
@@ -976,6 +980,7 @@ void Engine<T>::execute(StepPlan &plan) {
         er.cnt_w = d_cnt + 2 * s + 1;
         er.ldv = LD;
         er.nsplit = sl.nb;
+        if (eopt.profile) er.sweeps_out = d_sweeps + 3;
         e_rr.push_back(er);
         // Ritz update: X, BX, AX = (Q, BQ, AQ) Y1 ; W = Q Y2 (+ BW = eps W)
         LincombWork<T> lx{};
@@ -1027,6 +1032,7 @@ void Engine<T>::execute(StepPlan &plan) {
         g.na = kd;
         g.nb = kd;
         g.ld = LD;
+        g.lower_only = 1;
         g_t1.push_back(g);
         sl.rb_index = (int)t1_slots.size();
         t1_slots.push_back(s);
@@ -1146,8 +1152,13 @@ void Engine<T>::execute(StepPlan &plan) {
             BH_HIP_CHECK(hipEventSynchronize(b));
             float ms = 0;
             BH_HIP_CHECK(hipEventElapsedTime(&ms, a, b));
-            stats_->theta_ms += ms;
-            stats_->theta_launches += 1;
+            stats_->theta_ms[pol] += ms;
+            stats_->theta_launches[pol] += 1;
+            // columns processed = SVQB ranks of the slots in this launch
+            std::vector<int> rk(S);
+            BH_HIP_CHECK(hipMemcpy(rk.data(), d_rank, sizeof(int) * S, hipMemcpyDeviceToHost));
+            for (int s2 : plan.iter)
+                if (slots[s2].pol == pol) stats_->theta_columns[pol] += rk[s2];
             (void)hipEventDestroy(a);
             (void)hipEventDestroy(b);
         } else {
@@ -1173,7 +1184,19 @@ void Engine<T>::execute(StepPlan &plan) {
         BH_HIP_CHECK(hipMemcpyAsync(h_cnt, d_cnt, sizeof(int) * 2 * S, hipMemcpyDeviceToHost, st));
     }
     BH_HIP_CHECK(hipStreamSynchronize(st));
-    if (eopt.profile) stats_->theta_columns += 0;
+    if (eopt.profile && !plan.iter.empty()) {
+        int sw[6];
+        BH_HIP_CHECK(hipMemcpy(sw, d_sweeps, sizeof(sw), hipMemcpyDeviceToHost));
+        BH_HIP_CHECK(hipMemset(d_sweeps, 0, sizeof(sw)));
+        stats_->eigh_sweeps[0] += sw[0];
+        stats_->eigh_sweeps[1] += sw[3];
+        stats_->eigh_max_sweeps[0] = std::max(stats_->eigh_max_sweeps[0], (long)sw[2]);
+        stats_->eigh_max_sweeps[1] = std::max(stats_->eigh_max_sweeps[1], (long)sw[5]);
+        stats_->eigh_launch_max_sum[0] += sw[2];
+        stats_->eigh_launch_max_sum[1] += sw[5];
+        stats_->eigh_calls[0] += (long)plan.iter.size();
+        stats_->eigh_calls[1] += (long)plan.iter.size();
+    }
 }
 
 std::unique_ptr<EngineBase> make_engine(int nx, int ny, int nb_max, const EngineOptions &opt) {

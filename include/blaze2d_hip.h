@@ -70,6 +70,12 @@ int bh_gram(bh_context *ctx, const bh_buffer *x, int xi0, int p, const bh_buffer
 int bh_lincomb(bh_context *ctx, const bh_buffer *q, int qi0, int r, const double *coeffs, int m, bh_buffer *out,
                int oi0);
 
+/* Dense Hermitian eigendecomposition of `count` independent n x n matrices
+ * (n <= 64) on the device, FP64: a is col-major n*n per matrix (lower
+ * triangle read), w receives ascending eigenvalues (n per matrix) and v the
+ * col-major eigenvectors.  dense::solve_hermitian_eigen (eigensolver/dense.rs:71-134). */
+int bh_hermitian_eigh(bh_context *ctx, int n, int count, const double *a, double *w, double *v);
+
 /* ---- LinearOperator / OperatorPreconditioner level -------------------- */
 
 typedef struct {
@@ -140,8 +146,13 @@ typedef struct {
     long iterations;         /* LOBPCG iterations summed over solves */
     long solves;             /* k-point solves (closing Gamma copies excluded) */
     double seconds;          /* wall time of the solve loop */
-    double theta_ms;         /* summed Theta-kernel time (profile = 1) */
-    long theta_launches;
+    double theta_ms[2];      /* summed Theta-kernel time [TM, TE] (profile = 1) */
+    long theta_launches[2];
+    long theta_columns[2];   /* columns those launches processed */
+    long eigh_calls[2];      /* [SVQB, Rayleigh-Ritz] dense problems (profile = 1) */
+    long eigh_sweeps[2];     /* Jacobi sweeps summed over them */
+    long eigh_max_sweeps[2]; /* largest sweep count of a single problem */
+    long eigh_launch_max_sum[2]; /* per-launch maximum sweep count, summed over launches */
 } bh_run_stats;
 
 /* Runs every job to completion.  All jobs must share nx, ny.  Outputs per

@@ -74,6 +74,22 @@ def test_gram_and_lincomb_match_numpy(be):
     assert np.abs(out.download() - want).max() <= RTOL_OP * np.abs(want).max()
 
 
+@pytest.mark.parametrize("n", [1, 2, 5, 14, 28, 36, 47, 48, 64])
+def test_hermitian_eigh_matches_numpy(be, n):
+    r = np.random.default_rng(100 + n)
+    m = r.standard_normal((3, n, n)) + 1j * r.standard_normal((3, n, n))
+    h = m + np.conj(np.transpose(m, (0, 2, 1)))
+    # a near-degenerate cluster and a wide spread, as in Rayleigh-Ritz matrices
+    h[1] = np.diag(np.linspace(0.1, 4e4, n)) + 1e-9 * h[1]
+    w, v = be.hermitian_eigh(h)
+    for c in range(3):
+        ref = np.linalg.eigvalsh(h[c])
+        sc = max(1.0, np.abs(ref).max())
+        assert np.abs(w[c] - ref).max() <= 1e-12 * sc
+        assert np.abs(h[c] @ v[c] - v[c] * w[c]).max() <= 1e-11 * sc
+        assert np.abs(v[c].conj().T @ v[c] - np.eye(n)).max() <= 1e-12
+
+
 CASES = [
     ("square", 24, "TM", [(0.5, 0.5, 0.3, 1.0)], 13.0, (0.2, 0.1)),
     ("square", 24, "TE", [(0.5, 0.5, 0.3, 1.0)], 13.0, (0.0, 0.0)),
@@ -167,3 +183,17 @@ def test_bandstructure_cfg2_hex_te_res64_short(be):
     om, it, cv, tot = O.bandstructure(O.Job(lattice="hex", eps_bg=13.0, atoms=[(0.5, 0.5, 0.3, 1.0)], nx=64, ny=64,
                                             pol="TE", n_bands=12, tol=1e-7, kpath=kp))
     _compare_bands(res, om, it, "cfg2 short")
+
+
+def test_res64_sweep_sample_all_bands_converged(be):
+    # headline workload (first sweep geometry, TM + TE) with the all-bands-converged
+    # exit: GPU vs the FP64 oracle run with the same rule, <= 1e-6 relative
+    from blaze2d_amd.sweep import res64_sweep
+    jobs = res64_sweep(1)
+    res, st = B.run_batch(be, jobs, B.RunOptions(exit_rule=1), slots=2)
+    for j, r in zip(jobs, res):
+        lat = "square" if abs(j.geom.lattice.a2[0]) < 1e-12 else "hex"
+        om, it, cv, tot = O.bandstructure(O.Job(
+            lattice=lat, eps_bg=j.geom.eps_bg, atoms=[(a.pos[0], a.pos[1], a.radius, a.eps_inside) for a in j.geom.atoms],
+            nx=64, ny=64, pol=j.pol, n_bands=12, tol=1e-7, kpath=j.k_path, exit_rule=1))
+        _compare_bands(r, om, it, f"sweep0 {j.pol}")
