@@ -62,8 +62,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--geoms", type=int, default=96, help="geometries in the sweep (x2 polarisations)")
-    ap.add_argument("--slots", type=int, default=0, help="concurrent band structures (0 = all)")
+    ap.add_argument("--geoms", type=int, default=256, help="geometries in the sweep (x2 polarisations)")
+    ap.add_argument("--slots", type=int, default=256, help="concurrent band structures (0 = all)")
+    ap.add_argument("--groups", type=int, default=2, help="slot groups pipelined on separate streams")
     ap.add_argument("--precision", type=int, default=0)
     ap.add_argument("--seed", type=int, default=0xB1A2E2D)
     ap.add_argument("--cpu-sample", type=int, default=4, help="band structures timed on the CPU oracle")
@@ -85,14 +86,14 @@ def main():
     ropts = RunOptions(exit_rule=args.exit_rule)
 
     jobs = res64_sweep(args.geoms, seed=args.seed, tol=1e-7)
-    slots = args.slots if args.slots > 0 else len(jobs)
+    slots = min(args.slots, len(jobs)) if args.slots > 0 else len(jobs)
     be = HipBackend(0)
     print(f"[bench] sweep: {args.geoms} geometries x 2 pols = {len(jobs)} band structures, "
           f"{solves_of(jobs)} solves per step, slots={slots}", file=sys.stderr, flush=True)
 
     for w in range(args.warmup):
         t = time.perf_counter()
-        res, st = run_batch(be, jobs, ropts, slots=slots, precision=args.precision)
+        res, st = run_batch(be, jobs, ropts, slots=slots, precision=args.precision, groups=args.groups)
         print(f"[bench] warmup {w}: {time.perf_counter() - t:.2f}s", file=sys.stderr, flush=True)
 
     t0 = time.perf_counter()
@@ -100,7 +101,7 @@ def main():
     tot_iters = 0
     last = None
     for k in range(args.steps):
-        res, st = run_batch(be, jobs, ropts, slots=slots, precision=args.precision)
+        res, st = run_batch(be, jobs, ropts, slots=slots, precision=args.precision, groups=args.groups)
         conv += sum(int(r.converged[:-1].sum()) for r in res)
         tot_iters += st.iterations
         last = res
@@ -129,6 +130,7 @@ def main():
             "converged_fraction": round(conv / max(expected, 1), 6),
             "lobpcg_iterations_per_solve": round(tot_iters / max(args.steps * solves_of(jobs), 1), 3),
             "slots": slots,
+            "groups": args.groups,
             "seed": args.seed,
             "exit_rule": "all bands soft-locked" if args.exit_rule == 1 else "reference solve() accounting",
         },
@@ -176,7 +178,8 @@ def main():
         # the reference's own exit accounting on the same sweep (one pass)
         if args.exit_rule != 0:
             t = time.perf_counter()
-            resf, stf = run_batch(be, jobs, RunOptions(exit_rule=0), slots=slots, precision=args.precision)
+            resf, stf = run_batch(be, jobs, RunOptions(exit_rule=0), slots=slots, precision=args.precision,
+                                  groups=args.groups)
             dtf = time.perf_counter() - t
             out["solve_exit_mode"] = {
                 "value": round(sum(int(r.converged[:-1].sum()) for r in resf) / dtf, 3),

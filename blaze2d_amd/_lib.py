@@ -64,7 +64,8 @@ class Job(ctypes.Structure):
 
 
 class RunOptions(ctypes.Structure):
-    _fields_ = [("slots", ctypes.c_int), ("precision", ctypes.c_int), ("profile", ctypes.c_int)]
+    _fields_ = [("slots", ctypes.c_int), ("precision", ctypes.c_int), ("profile", ctypes.c_int),
+                ("groups", ctypes.c_int)]
 
 
 class RunStats(ctypes.Structure):

@@ -158,8 +158,12 @@ class BatchStats:
 
 
 def run_batch(backend: HipBackend, jobs: Sequence[BandStructureJob], options: Optional[RunOptions] = None,
-              slots: int = 64, precision: int = 0, profile: bool = False):
-    """Runs many band structures concurrently on one GPU (all on one grid)."""
+              slots: int = 64, precision: int = 0, profile: bool = False, groups: int = 2):
+    """Runs many band structures concurrently on one GPU (all on one grid).
+
+    `slots` band structures are resident at once; they are split into `groups`
+    pipelined on separate HIP streams.
+    """
     opts = options or RunOptions()
     jobs = list(jobs)
     if not jobs:
@@ -172,7 +176,7 @@ def run_batch(backend: HipBackend, jobs: Sequence[BandStructureJob], options: Op
     om = np.zeros(tot_om)
     it = np.zeros(tot_k, dtype=np.int32)
     cv = np.zeros(tot_k, dtype=np.int32)
-    ro = L.RunOptions(slots, precision, int(profile))
+    ro = L.RunOptions(slots, precision, int(profile), groups)
     st = L.RunStats()
     L.check(L.lib().bh_bandstructure_run(backend.handle, arr, len(jobs), ctypes.byref(ro), om.ctypes.data_as(L.c_dp),
                                          it.ctypes.data_as(L.c_ip), cv.ctypes.data_as(L.c_ip), ctypes.byref(st)),

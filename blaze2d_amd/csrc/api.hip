@@ -557,6 +557,7 @@ int bh_bandstructure_run(bh_context *ctx, const bh_job *jobs, int njobs, const b
     eo.slots = std::min(eo.slots, njobs);
     eo.precision = opts ? opts->precision : 0;
     eo.profile = opts ? opts->profile : 0;
+    eo.groups = opts && opts->groups > 0 ? opts->groups : 2;
     BH_HIP_CHECK(hipSetDevice(ctx->device));
     auto eng = make_engine(js[0].nx, js[0].ny, nbmax, eo);
     std::vector<JobResult> res;

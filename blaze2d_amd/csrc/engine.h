@@ -51,7 +51,8 @@ struct RunStats {
 struct EngineOptions {
     int slots = 64;         // concurrent band structures
     int precision = 0;      // 0 fp64, 1 fp32 storage (reserved)
-    int profile = 0;        // time operator kernels with events
+    int profile = 0;        // time operator kernels with events (forces one group)
+    int groups = 2;         // slot groups pipelined on separate streams
 };
 
 class EngineBase {

@@ -62,6 +62,20 @@ struct Staging {
     }
 };
 
+// A group of slots with its own stream, staging arena, Gram partials and
+// readback buffers, so that groups can be planned on the host and run on the
+// GPU concurrently.
+struct Group {
+    hipStream_t st{};
+    hipEvent_t ev{};
+    Staging stg;
+    double *part = nullptr, *gramrb = nullptr, *h_gramrb = nullptr;
+    int *sweeps = nullptr;
+    int s0 = 0, s1 = 0;
+    bool inflight = false;
+    bool has_iter = false;
+};
+
 template <typename T>
 T *dalloc(size_t n) {
     T *p = nullptr;
@@ -151,16 +165,29 @@ class Engine : public EngineBase {
         d_Yw = dalloc<double>((size_t)S * LD * LD * 2);
         d_T = dalloc<double>((size_t)S * LD * LD * 2);
         d_coef = dalloc<double>((size_t)S * LD * LD * 2);
-        d_part = dalloc<double>((size_t)(2048 + 2 * S) * LD * LD * 2);
         d_rank = dalloc<int>(S);
         d_cnt = dalloc<int>(2 * (size_t)S);
-        d_sweeps = dalloc<int>(6);
-        d_gramrb = dalloc<double>((size_t)S * LD * LD * 2);
         BH_HIP_CHECK(hipHostMalloc((void **)&h_lam, sizeof(double) * S * LD, hipHostMallocDefault));
         BH_HIP_CHECK(hipHostMalloc((void **)&h_rank, sizeof(int) * S, hipHostMallocDefault));
         BH_HIP_CHECK(hipHostMalloc((void **)&h_cnt, sizeof(int) * 2 * S, hipHostMallocDefault));
-        BH_HIP_CHECK(hipHostMalloc((void **)&h_gramrb, sizeof(double) * S * LD * LD * 2, hipHostMallocDefault));
-        stg.init((size_t)64 << 20);
+        int ng = opt.profile ? 1 : std::max(1, std::min(opt.groups, S));
+        slots_per_group = (S + ng - 1) / ng;
+        ng = (S + slots_per_group - 1) / slots_per_group;
+        groups.resize(ng);
+        for (int g = 0; g < ng; ++g) {
+            Group &G = groups[g];
+            G.s0 = g * slots_per_group;
+            G.s1 = std::min(S, G.s0 + slots_per_group);
+            const int gs = G.s1 - G.s0;
+            BH_HIP_CHECK(hipStreamCreate(&G.st));
+            BH_HIP_CHECK(hipEventCreateWithFlags(&G.ev, hipEventDisableTiming));
+            // per slot and step: dielectric tables of a new job plus descriptors / coefficients
+            G.stg.init(((size_t)1 << 20) + (size_t)gs * (5 * (size_t)N * sizeof(T) + ((size_t)96 << 10)));
+            G.part = dalloc<double>((size_t)(2048 + 2 * gs) * LD * LD * 2);
+            G.gramrb = dalloc<double>((size_t)gs * LD * LD * 2);
+            BH_HIP_CHECK(hipHostMalloc((void **)&G.h_gramrb, sizeof(double) * gs * LD * LD * 2, hipHostMallocDefault));
+            G.sweeps = dalloc<int>(6);
+        }
         // twiddle tables W_n^t = exp(-2 pi i t / n), computed in FP64 on the host
         std::vector<cplx<T>> tx(nx), ty(ny);
         for (int t = 0; t < nx; ++t) {
@@ -182,14 +209,22 @@ class Engine : public EngineBase {
 
     ~Engine() override {
         (void)hipStreamSynchronize(st);
+        for (auto &G : groups) {
+            (void)hipStreamSynchronize(G.st);
+            (void)hipFree(G.part);
+            (void)hipFree(G.gramrb);
+            (void)hipFree(G.sweeps);
+            (void)hipHostFree(G.h_gramrb);
+            G.stg.release();
+            (void)hipEventDestroy(G.ev);
+            (void)hipStreamDestroy(G.st);
+        }
         for (void *p : std::initializer_list<void *>{V, BV, AV, Q, BQ, AQ, H, RND, EPS, TEN, d_lam, d_svw, d_Y, d_Yw,
-                                                     d_T, d_coef, d_part, d_rank, d_cnt, d_sweeps, d_gramrb, twx, twy})
+                                                     d_T, d_coef, d_rank, d_cnt, twx, twy})
             if (p) (void)hipFree(p);
         (void)hipHostFree(h_lam);
         (void)hipHostFree(h_rank);
         (void)hipHostFree(h_cnt);
-        (void)hipHostFree(h_gramrb);
-        stg.release();
         (void)hipStreamDestroy(st);
     }
 
@@ -203,12 +238,14 @@ class Engine : public EngineBase {
     hipStream_t st{};
     cplx<T> *V{}, *BV{}, *AV{}, *Q{}, *BQ{}, *AQ{}, *H{}, *RND{};
     T *EPS{}, *TEN{};
-    double *d_lam{}, *d_svw{}, *d_Y{}, *d_Yw{}, *d_T{}, *d_coef{}, *d_part{}, *d_gramrb{};
-    int *d_rank{}, *d_cnt{}, *d_sweeps{};
-    double *h_lam{}, *h_gramrb{};
+    double *d_lam{}, *d_svw{}, *d_Y{}, *d_Yw{}, *d_T{}, *d_coef{};
+    int *d_rank{}, *d_cnt{};
+    double *h_lam{};
     int *h_rank{}, *h_cnt{};
     cplx<T> *twx{}, *twy{};
-    Staging stg;
+    std::vector<Group> groups;
+    int slots_per_group = 1;
+    Group &group_of(int s) { return groups[s / slots_per_group]; }
     std::vector<Slot> slots;
     const std::vector<JobSpec> *jobs_{};
     std::vector<JobResult> *out_{};
@@ -245,7 +282,7 @@ class Engine : public EngineBase {
     void finish_solve(int s, bool conv, int iters);
     void record_k(int s, int kidx, const std::vector<double> &om, int iters, bool conv);
     void set_kpoint(int s);
-    void execute(StepPlan &plan);
+    void execute(StepPlan &plan, Group &G);
 };
 
 template <typename T>
@@ -426,7 +463,7 @@ void Engine<T>::transition_math(int s) {
     const int kd = sl.kd, nE = sl.nE, nP = sl.nP, nR = sl.nR;
     const int iE = 0, iP = nE, iR = nE + nP, iY = nE + nP + nR;
     std::vector<zd> M((size_t)kd * kd);
-    const double *g = h_gramrb + (size_t)sl.rb_index * LD * LD * 2;
+    const double *g = group_of(s).h_gramrb + (size_t)sl.rb_index * LD * LD * 2;
     for (int j = 0; j < kd; ++j)
         for (int i = 0; i < kd; ++i) M[i + (size_t)j * kd] = zd(g[2 * (i + (size_t)j * LD)], g[2 * (i + (size_t)j * LD) + 1]);
     // Hermitian symmetrisation from the lower triangle
@@ -627,45 +664,66 @@ void Engine<T>::run(const std::vector<JobSpec> &jobs, std::vector<JobResult> &ou
     for (auto &sl : slots) sl = Slot();
     size_t next_job = 0;
     auto t0 = std::chrono::steady_clock::now();
+    for (auto &G : groups) G.inflight = false;
+    // Groups alternate: while the GPU runs one group's step, the host plans
+    // the next; the groups' kernels overlap on the device.
     while (true) {
-        StepPlan plan;
-        // decisions from the previous step's readback
-        for (int s = 0; s < S; ++s) {
-            Slot &sl = slots[s];
-            if (sl.state == GRAM_PENDING && sl.rb_index >= 0) {
-                transition_math(s);
-                sl.rb_index = -1;
-                if (sl.state == ITER) plan.t2.push_back(s);
-            } else if (sl.state == ITER) {
-                absorb(s);
-                decide(s);
+        bool launched = false;
+        for (auto &G : groups) {
+            if (G.inflight) {
+                BH_HIP_CHECK(hipEventSynchronize(G.ev));
+                G.inflight = false;
             }
-        }
-        for (int s = 0; s < S; ++s) {
-            Slot &sl = slots[s];
-            if (sl.state == FREE && next_job < jobs.size()) {
-                start_job(s, (int)next_job++);
-                plan.table_upload.push_back(s);
-                plan.fill_y0.push_back(s);
-                sl.E_cols.clear();
-                schedule_t1(s);
+            StepPlan plan;
+            for (int s = G.s0; s < G.s1; ++s) {
+                Slot &sl = slots[s];
+                if (sl.state == GRAM_PENDING && sl.rb_index >= 0) {
+                    transition_math(s);
+                    sl.rb_index = -1;
+                    if (sl.state == ITER) plan.t2.push_back(s);
+                } else if (sl.state == ITER) {
+                    absorb(s);
+                    decide(s);
+                }
             }
+            for (int s = G.s0; s < G.s1; ++s) {
+                Slot &sl = slots[s];
+                if (sl.state == FREE && next_job < jobs.size()) {
+                    start_job(s, (int)next_job++);
+                    plan.table_upload.push_back(s);
+                    plan.fill_y0.push_back(s);
+                    sl.E_cols.clear();
+                    schedule_t1(s);
+                }
+            }
+            bool any = false;
+            for (int s = G.s0; s < G.s1; ++s) {
+                Slot &sl = slots[s];
+                if (sl.state == ITER) { plan.iter.push_back(s); any = true; }
+                else if (sl.state == GRAM_PENDING && sl.rb_index < 0) { plan.t1.push_back(s); any = true; }
+            }
+            if (!any) continue;
+            execute(plan, G);
+            stats.steps++;
+            launched = true;
         }
-        bool any = false;
-        for (int s = 0; s < S; ++s) {
-            Slot &sl = slots[s];
-            if (sl.state == ITER) { plan.iter.push_back(s); any = true; }
-            else if (sl.state == GRAM_PENDING && sl.rb_index < 0) { plan.t1.push_back(s); any = true; }
+        if (!launched) {
+            bool busy = false;
+            for (auto &G : groups) busy |= G.inflight;
+            if (!busy) break;
         }
-        if (!any) break;
-        execute(plan);
-        stats.steps++;
     }
+    for (auto &G : groups) BH_HIP_CHECK(hipStreamSynchronize(G.st));
     stats.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
 template <typename T>
-void Engine<T>::execute(StepPlan &plan) {
+void Engine<T>::execute(StepPlan &plan, Group &G) {
+    Staging &stg = G.stg;
+    const hipStream_t st = G.st;
+    double *const d_part = G.part;
+    double *const d_gramrb = G.gramrb;
+    int *const d_sweeps = G.sweeps;
     stg.off = 0;
     const bool need_b_eps = true;
     (void)need_b_eps;
@@ -1156,6 +1214,7 @@ void Engine<T>::execute(StepPlan &plan) {
             stats_->theta_launches[pol] += 1;
             // columns processed = SVQB ranks of the slots in this launch
             std::vector<int> rk(S);
+            BH_HIP_CHECK(hipStreamSynchronize(st));
             BH_HIP_CHECK(hipMemcpy(rk.data(), d_rank, sizeof(int) * S, hipMemcpyDeviceToHost));
             for (int s2 : plan.iter)
                 if (slots[s2].pol == pol) stats_->theta_columns[pol] += rk[s2];
@@ -1176,15 +1235,22 @@ void Engine<T>::execute(StepPlan &plan) {
         launch_gram<T>(d_g_t1, (int)g_t1.size(), nch_t1, N, maxkd, maxkd, st);
         launch_gram_reduce(d_part, d_gramrb, (int)g_t1.size(), nch_t1, LD, (long)nch_t1 * LD * LD * 2,
                            (long)LD * LD * 2, st);
-        BH_HIP_CHECK(hipMemcpyAsync(h_gramrb, d_gramrb, sizeof(double) * g_t1.size() * LD * LD * 2,
+        BH_HIP_CHECK(hipMemcpyAsync(G.h_gramrb, d_gramrb, sizeof(double) * g_t1.size() * LD * LD * 2,
                                     hipMemcpyDeviceToHost, st));
     }
     if (!plan.iter.empty()) {
-        BH_HIP_CHECK(hipMemcpyAsync(h_lam, d_lam, sizeof(double) * S * LD, hipMemcpyDeviceToHost, st));
-        BH_HIP_CHECK(hipMemcpyAsync(h_cnt, d_cnt, sizeof(int) * 2 * S, hipMemcpyDeviceToHost, st));
+        const int gs = G.s1 - G.s0;
+        BH_HIP_CHECK(hipMemcpyAsync(h_lam + (size_t)G.s0 * LD, d_lam + (size_t)G.s0 * LD, sizeof(double) * gs * LD,
+                                    hipMemcpyDeviceToHost, st));
+        BH_HIP_CHECK(hipMemcpyAsync(h_cnt + 2 * (size_t)G.s0, d_cnt + 2 * (size_t)G.s0, sizeof(int) * 2 * gs,
+                                    hipMemcpyDeviceToHost, st));
     }
+    BH_HIP_CHECK(hipEventRecord(G.ev, st));
+    G.inflight = true;
+    G.has_iter = !plan.iter.empty();
+    if (!eopt.profile) return;
     BH_HIP_CHECK(hipStreamSynchronize(st));
-    if (eopt.profile && !plan.iter.empty()) {
+    if (!plan.iter.empty()) {
         int sw[6];
         BH_HIP_CHECK(hipMemcpy(sw, d_sweeps, sizeof(sw), hipMemcpyDeviceToHost));
         BH_HIP_CHECK(hipMemset(d_sweeps, 0, sizeof(sw)));

@@ -138,7 +138,8 @@ typedef struct {
 typedef struct {
     int slots;               /* concurrent band structures on the GPU (0 -> 64) */
     int precision;           /* 0 FP64 */
-    int profile;             /* 1: time operator kernels with HIP events */
+    int profile;             /* 1: time operator kernels with HIP events (runs one slot group) */
+    int groups;              /* slot groups pipelined on separate HIP streams (0 -> 2) */
 } bh_run_options;
 
 typedef struct {
