@@ -226,7 +226,7 @@ template void launch_lincomb<float>(const LincombWork<float> *, int, long, int, 
 // into the Gram, eigenpairs in descending order, drop lambda < 1e-12 lambda_max
 // or <= 0, T = D Q_kept Lambda_kept^{-1/2}.
 // ---------------------------------------------------------------------------
-constexpr int EIGH_NT = 256;
+constexpr int EIGH_NT = 1024;
 
 size_t eigh_shared_bytes(int maxn) {
     size_t ne = maxn + (maxn & 1), np = ne / 2;
@@ -234,7 +234,7 @@ size_t eigh_shared_bytes(int maxn) {
     return sizeof(double) * (4 * ne * ne + ne + np + 2 * np) + sizeof(int) * 3 * np;
 }
 
-__global__ void __launch_bounds__(EIGH_NT) eigh_kernel(const EighWork *__restrict__ work) {
+__global__ void __launch_bounds__(1024) eigh_kernel(const EighWork *__restrict__ work) {
     const EighWork &w = work[blockIdx.x];
     const int n = w.dev_n ? *w.dev_n : w.n;
     const int tid = threadIdx.x;
@@ -370,6 +370,16 @@ __global__ void __launch_bounds__(EIGH_NT) eigh_kernel(const EighWork *__restric
     // applies both the column rotation of its column pair and the row
     // rotation of its row pair in registers, so a round needs two barriers.
     __shared__ unsigned long long maxratio_bits;  // non-negative doubles order like their bit patterns
+    // fixed per-thread work for every round: one 2x2 block (np*np <= 1024) and
+    // up to two eigenvector items (np*ne <= 2048)
+    const bool has_blk = tid < np * np;
+    const int bt1 = has_blk ? tid / np : 0, bt2 = has_blk ? tid - bt1 * np : 0;
+    const int nv = np * ne;
+    const int vt0 = tid < nv ? tid / ne : 0, vk0 = tid < nv ? tid - vt0 * ne : 0;
+    const int vt1 = tid + EIGH_NT < nv ? (tid + EIGH_NT) / ne : 0;
+    const int vk1 = tid + EIGH_NT < nv ? (tid + EIGH_NT) - vt1 * ne : 0;
+    // round-robin pairing for this thread's pair slot (tid < np), per round r:
+    // slot 0: (r, ne-1); slot t: ((r+t) mod (ne-1), (r-t) mod (ne-1))
     for (int sweep = 0; sweep < 40; ++sweep) {
         if (tid == 0) { nrot = 0; maxratio_bits = 0ull; }
         __syncthreads();
@@ -406,8 +416,9 @@ __global__ void __launch_bounds__(EIGH_NT) eigh_kernel(const EighWork *__restric
                 rs[tid] = sg;
             }
             __syncthreads();
-            for (int b = tid; b < np * np; b += EIGH_NT) {
-                const int t1 = b / np, t2 = b - t1 * np;  // row pair, column pair
+            do {
+                if (!has_blk) break;
+                const int t1 = bt1, t2 = bt2;  // row pair, column pair
                 const int f1 = rf[t1], f2 = rf[t2];
                 if (f1 != 1 && f2 != 1) {
                     if (t1 == t2 && f1 == 2) {
@@ -415,7 +426,7 @@ __global__ void __launch_bounds__(EIGH_NT) eigh_kernel(const EighWork *__restric
                         A[p + q * ne] = make_double2(0.0, 0.0);
                         A[q + p * ne] = make_double2(0.0, 0.0);
                     }
-                    continue;
+                    break;
                 }
                 const int p1 = rp[t1], q1 = rq[t1], p2 = rp[t2], q2 = rq[t2];
                 double2 a11 = A[p1 + p2 * ne], a12 = A[p1 + q2 * ne];
@@ -448,9 +459,9 @@ __global__ void __launch_bounds__(EIGH_NT) eigh_kernel(const EighWork *__restric
                 A[p1 + q2 * ne] = a12;
                 A[q1 + p2 * ne] = a21;
                 A[q1 + q2 * ne] = a22;
-            }
-            for (int e = tid; e < np * ne; e += EIGH_NT) {  // eigenvectors: column rotations only
-                int t = e / ne, k = e - t * ne;
+            } while (false);
+            for (int e = tid, it = 0; e < nv; e += EIGH_NT, ++it) {  // eigenvectors: column rotations only
+                const int t = it == 0 ? vt0 : vt1, k = it == 0 ? vk0 : vk1;
                 if (rf[t] != 1) continue;
                 double c = rc[t];
                 double2 sg = rs[t];
