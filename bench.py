@@ -105,7 +105,8 @@ def main():
         conv += sum(int(r.converged[:-1].sum()) for r in res)
         tot_iters += st.iterations
         last = res
-        print(f"[bench] step {k}: engine {st.seconds:.2f}s, {st.steps} engine steps", file=sys.stderr, flush=True)
+        print(f"[bench] step {k}: engine {st.seconds:.2f}s, {st.steps} engine steps, host plan "
+              f"{st.host_plan_seconds:.2f}s, host wait {st.host_wait_seconds:.2f}s", file=sys.stderr, flush=True)
     elapsed = time.perf_counter() - t0
     expected = args.steps * solves_of(jobs)
     value = conv / elapsed

@@ -81,6 +81,8 @@ class RunStats(ctypes.Structure):
         ("eigh_sweeps", ctypes.c_long * 2),
         ("eigh_max_sweeps", ctypes.c_long * 2),
         ("eigh_launch_max_sum", ctypes.c_long * 2),
+        ("host_wait_seconds", ctypes.c_double),
+        ("host_plan_seconds", ctypes.c_double),
     ]
 
 

@@ -155,6 +155,8 @@ class BatchStats:
     eigh_sweeps: tuple = (0, 0)
     eigh_max_sweeps: tuple = (0, 0)
     eigh_launch_max_sum: tuple = (0, 0)
+    host_wait_seconds: float = 0.0
+    host_plan_seconds: float = 0.0
 
 
 def run_batch(backend: HipBackend, jobs: Sequence[BandStructureJob], options: Optional[RunOptions] = None,
@@ -191,7 +193,8 @@ def run_batch(backend: HipBackend, jobs: Sequence[BandStructureJob], options: Op
         k += nk
     stats = BatchStats(st.steps, st.iterations, st.solves, st.seconds, tuple(st.theta_ms), tuple(st.theta_launches),
                        tuple(st.theta_columns), tuple(st.eigh_calls), tuple(st.eigh_sweeps),
-                       tuple(st.eigh_max_sweeps), tuple(st.eigh_launch_max_sum))
+                       tuple(st.eigh_max_sweeps), tuple(st.eigh_launch_max_sum), st.host_wait_seconds,
+                       st.host_plan_seconds)
     return out, stats
 
 

@@ -17,6 +17,10 @@ using namespace bh;
 struct bh_context {
     int device = 0;
     hipStream_t st{};
+    // band-solve slot groups run on these; created right after st so that each
+    // gets a hardware queue of its own (HIP shares queues once
+    // GPU_MAX_HW_QUEUES streams exist, and the null stream takes one as well)
+    hipStream_t gst[3]{};
     // twiddle tables per (n, precision)
     std::vector<std::pair<int, void *>> tw64, tw32;
     void *scratch = nullptr;
@@ -184,6 +188,7 @@ int bh_context_create(int device, bh_context **out) {
     auto *c = new bh_context();
     c->device = device;
     BH_HIP_CHECK(hipStreamCreate(&c->st));
+    for (auto &g : c->gst) BH_HIP_CHECK(hipStreamCreate(&g));
     *out = c;
     return 0;
     BH_API_CATCH
@@ -195,6 +200,10 @@ void bh_context_destroy(bh_context *ctx) {
     for (auto &p : ctx->tw64) (void)hipFree(p.second);
     for (auto &p : ctx->tw32) (void)hipFree(p.second);
     if (ctx->scratch) (void)hipFree(ctx->scratch);
+    for (auto &g : ctx->gst) {
+        (void)hipStreamSynchronize(g);
+        (void)hipStreamDestroy(g);
+    }
     (void)hipStreamDestroy(ctx->st);
     delete ctx;
 }
@@ -558,6 +567,10 @@ int bh_bandstructure_run(bh_context *ctx, const bh_job *jobs, int njobs, const b
     eo.precision = opts ? opts->precision : 0;
     eo.profile = opts ? opts->profile : 0;
     eo.groups = opts && opts->groups > 0 ? opts->groups : 2;
+    eo.setup_stream = ctx->st;
+    eo.group_streams[0] = ctx->gst[0];
+    eo.group_streams[1] = ctx->gst[1];
+    eo.group_streams[2] = ctx->gst[2];
     BH_HIP_CHECK(hipSetDevice(ctx->device));
     auto eng = make_engine(js[0].nx, js[0].ny, nbmax, eo);
     std::vector<JobResult> res;
@@ -588,6 +601,8 @@ int bh_bandstructure_run(bh_context *ctx, const bh_job *jobs, int njobs, const b
             stats->eigh_max_sweeps[p] = rs.eigh_max_sweeps[p];
             stats->eigh_launch_max_sum[p] = rs.eigh_launch_max_sum[p];
         }
+        stats->host_wait_seconds = rs.host_wait_seconds;
+        stats->host_plan_seconds = rs.host_plan_seconds;
     }
     return 0;
     BH_API_CATCH

@@ -7,6 +7,8 @@
 #include <memory>
 #include <vector>
 
+#include <hip/hip_runtime_api.h>
+
 #include "hostmath.h"
 
 namespace bh {
@@ -46,6 +48,8 @@ struct RunStats {
     long eigh_sweeps[2] = {0, 0};    // Jacobi sweeps summed over those problems
     long eigh_max_sweeps[2] = {0, 0};      // largest sweep count seen in one problem
     long eigh_launch_max_sum[2] = {0, 0};  // per launch maximum, summed over launches
+    double host_wait_seconds = 0;          // blocked on group completion events
+    double host_plan_seconds = 0;          // planning and enqueueing
 };
 
 struct EngineOptions {
@@ -53,6 +57,12 @@ struct EngineOptions {
     int precision = 0;      // 0 fp64, 1 fp32 storage (reserved)
     int profile = 0;        // time operator kernels with events (forces one group)
     int groups = 2;         // slot groups pipelined on separate streams
+    // Streams owned by the caller.  HIP gives each new stream its own hardware
+    // queue until GPU_MAX_HW_QUEUES is reached and then shares queues, which
+    // serialises the groups; a caller that creates the group streams early
+    // (before other streams and before any null-stream use) keeps them apart.
+    hipStream_t setup_stream = nullptr;
+    hipStream_t group_streams[3] = {nullptr, nullptr, nullptr};
 };
 
 class EngineBase {

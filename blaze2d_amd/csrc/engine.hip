@@ -74,14 +74,16 @@ struct Group {
     int s0 = 0, s1 = 0;
     bool inflight = false;
     bool has_iter = false;
+    bool own_st = false;
 };
 
+// zero-filled device allocation, cleared on the given stream
 template <typename T>
-T *dalloc(size_t n) {
+T *dalloc(size_t n, hipStream_t st) {
     T *p = nullptr;
     if (n == 0) n = 1;
     BH_HIP_CHECK(hipMalloc((void **)&p, n * sizeof(T)));
-    BH_HIP_CHECK(hipMemset(p, 0, n * sizeof(T)));
+    BH_HIP_CHECK(hipMemsetAsync(p, 0, n * sizeof(T), st));
     return p;
 }
 
@@ -148,25 +150,26 @@ class Engine : public EngineBase {
         LD = GRAM_MAXC;
         NRMAX = m0max + 2;
         NCHMAX = (int)std::min<long>(64, std::max<long>(1, N / 32));
-        BH_HIP_CHECK(hipStreamCreate(&st));
-        V = dalloc<cplx<T>>((size_t)S * CV * N);
-        BV = dalloc<cplx<T>>((size_t)S * CV * N);
-        AV = dalloc<cplx<T>>((size_t)S * m0max * N);
-        Q = dalloc<cplx<T>>((size_t)S * C * N);
-        BQ = dalloc<cplx<T>>((size_t)S * C * N);
-        AQ = dalloc<cplx<T>>((size_t)S * C * N);
-        H = dalloc<cplx<T>>((size_t)S * 2 * nbmax * N);
-        RND = dalloc<cplx<T>>((size_t)NRMAX * N);
-        EPS = dalloc<T>((size_t)S * N);
-        TEN = dalloc<T>((size_t)S * 4 * N);
-        d_lam = dalloc<double>((size_t)S * LD);
-        d_svw = dalloc<double>((size_t)S * LD);
-        d_Y = dalloc<double>((size_t)S * LD * LD * 2);
-        d_Yw = dalloc<double>((size_t)S * LD * LD * 2);
-        d_T = dalloc<double>((size_t)S * LD * LD * 2);
-        d_coef = dalloc<double>((size_t)S * LD * LD * 2);
-        d_rank = dalloc<int>(S);
-        d_cnt = dalloc<int>(2 * (size_t)S);
+        if (opt.setup_stream) st = opt.setup_stream;
+        else { BH_HIP_CHECK(hipStreamCreate(&st)); own_st = true; }
+        V = dalloc<cplx<T>>((size_t)S * CV * N, st);
+        BV = dalloc<cplx<T>>((size_t)S * CV * N, st);
+        AV = dalloc<cplx<T>>((size_t)S * m0max * N, st);
+        Q = dalloc<cplx<T>>((size_t)S * C * N, st);
+        BQ = dalloc<cplx<T>>((size_t)S * C * N, st);
+        AQ = dalloc<cplx<T>>((size_t)S * C * N, st);
+        H = dalloc<cplx<T>>((size_t)S * 2 * nbmax * N, st);
+        RND = dalloc<cplx<T>>((size_t)NRMAX * N, st);
+        EPS = dalloc<T>((size_t)S * N, st);
+        TEN = dalloc<T>((size_t)S * 4 * N, st);
+        d_lam = dalloc<double>((size_t)S * LD, st);
+        d_svw = dalloc<double>((size_t)S * LD, st);
+        d_Y = dalloc<double>((size_t)S * LD * LD * 2, st);
+        d_Yw = dalloc<double>((size_t)S * LD * LD * 2, st);
+        d_T = dalloc<double>((size_t)S * LD * LD * 2, st);
+        d_coef = dalloc<double>((size_t)S * LD * LD * 2, st);
+        d_rank = dalloc<int>(S, st);
+        d_cnt = dalloc<int>(2 * (size_t)S, st);
         BH_HIP_CHECK(hipHostMalloc((void **)&h_lam, sizeof(double) * S * LD, hipHostMallocDefault));
         BH_HIP_CHECK(hipHostMalloc((void **)&h_rank, sizeof(int) * S, hipHostMallocDefault));
         BH_HIP_CHECK(hipHostMalloc((void **)&h_cnt, sizeof(int) * 2 * S, hipHostMallocDefault));
@@ -179,14 +182,15 @@ class Engine : public EngineBase {
             G.s0 = g * slots_per_group;
             G.s1 = std::min(S, G.s0 + slots_per_group);
             const int gs = G.s1 - G.s0;
-            BH_HIP_CHECK(hipStreamCreate(&G.st));
+            if (g < 3 && opt.group_streams[g]) G.st = opt.group_streams[g];
+            else { BH_HIP_CHECK(hipStreamCreate(&G.st)); G.own_st = true; }
             BH_HIP_CHECK(hipEventCreateWithFlags(&G.ev, hipEventDisableTiming));
             // per slot and step: dielectric tables of a new job plus descriptors / coefficients
             G.stg.init(((size_t)1 << 20) + (size_t)gs * (5 * (size_t)N * sizeof(T) + ((size_t)96 << 10)));
-            G.part = dalloc<double>((size_t)(2048 + 2 * gs) * LD * LD * 2);
-            G.gramrb = dalloc<double>((size_t)gs * LD * LD * 2);
+            G.part = dalloc<double>((size_t)(2048 + 2 * gs) * LD * LD * 2, st);
+            G.gramrb = dalloc<double>((size_t)gs * LD * LD * 2, st);
             BH_HIP_CHECK(hipHostMalloc((void **)&G.h_gramrb, sizeof(double) * gs * LD * LD * 2, hipHostMallocDefault));
-            G.sweeps = dalloc<int>(6);
+            G.sweeps = dalloc<int>(6, st);
         }
         // twiddle tables W_n^t = exp(-2 pi i t / n), computed in FP64 on the host
         std::vector<cplx<T>> tx(nx), ty(ny);
@@ -198,10 +202,10 @@ class Engine : public EngineBase {
             double a = -2.0 * M_PI * t / ny;
             ty[t] = {(T)std::cos(a), (T)std::sin(a)};
         }
-        twx = dalloc<cplx<T>>(nx);
-        twy = dalloc<cplx<T>>(ny);
-        BH_HIP_CHECK(hipMemcpy(twx, tx.data(), sizeof(cplx<T>) * nx, hipMemcpyHostToDevice));
-        BH_HIP_CHECK(hipMemcpy(twy, ty.data(), sizeof(cplx<T>) * ny, hipMemcpyHostToDevice));
+        twx = dalloc<cplx<T>>(nx, st);
+        twy = dalloc<cplx<T>>(ny, st);
+        BH_HIP_CHECK(hipMemcpyAsync(twx, tx.data(), sizeof(cplx<T>) * nx, hipMemcpyHostToDevice, st));
+        BH_HIP_CHECK(hipMemcpyAsync(twy, ty.data(), sizeof(cplx<T>) * ny, hipMemcpyHostToDevice, st));
         launch_random<T>(RND, NRMAX, N, 1.0, st);
         BH_HIP_CHECK(hipStreamSynchronize(st));
         slots.resize(S);
@@ -217,7 +221,7 @@ class Engine : public EngineBase {
             (void)hipHostFree(G.h_gramrb);
             G.stg.release();
             (void)hipEventDestroy(G.ev);
-            (void)hipStreamDestroy(G.st);
+            if (G.own_st) (void)hipStreamDestroy(G.st);
         }
         for (void *p : std::initializer_list<void *>{V, BV, AV, Q, BQ, AQ, H, RND, EPS, TEN, d_lam, d_svw, d_Y, d_Yw,
                                                      d_T, d_coef, d_rank, d_cnt, twx, twy})
@@ -225,7 +229,7 @@ class Engine : public EngineBase {
         (void)hipHostFree(h_lam);
         (void)hipHostFree(h_rank);
         (void)hipHostFree(h_cnt);
-        (void)hipStreamDestroy(st);
+        if (own_st) (void)hipStreamDestroy(st);
     }
 
     void run(const std::vector<JobSpec> &jobs, std::vector<JobResult> &out, RunStats &stats) override;
@@ -236,6 +240,7 @@ class Engine : public EngineBase {
     int nbmax, m0max, C, CV, S, LD, NRMAX, NCHMAX;
     EngineOptions eopt;
     hipStream_t st{};
+    bool own_st = false;
     cplx<T> *V{}, *BV{}, *AV{}, *Q{}, *BQ{}, *AQ{}, *H{}, *RND{};
     T *EPS{}, *TEN{};
     double *d_lam{}, *d_svw{}, *d_Y{}, *d_Yw{}, *d_T{}, *d_coef{};
@@ -671,9 +676,12 @@ void Engine<T>::run(const std::vector<JobSpec> &jobs, std::vector<JobResult> &ou
         bool launched = false;
         for (auto &G : groups) {
             if (G.inflight) {
+                auto w0 = std::chrono::steady_clock::now();
                 BH_HIP_CHECK(hipEventSynchronize(G.ev));
+                stats.host_wait_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - w0).count();
                 G.inflight = false;
             }
+            auto p0 = std::chrono::steady_clock::now();
             StepPlan plan;
             for (int s = G.s0; s < G.s1; ++s) {
                 Slot &sl = slots[s];
@@ -702,8 +710,12 @@ void Engine<T>::run(const std::vector<JobSpec> &jobs, std::vector<JobResult> &ou
                 if (sl.state == ITER) { plan.iter.push_back(s); any = true; }
                 else if (sl.state == GRAM_PENDING && sl.rb_index < 0) { plan.t1.push_back(s); any = true; }
             }
-            if (!any) continue;
+            if (!any) {
+                stats.host_plan_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - p0).count();
+                continue;
+            }
             execute(plan, G);
+            stats.host_plan_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - p0).count();
             stats.steps++;
             launched = true;
         }
@@ -1214,8 +1226,8 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
             stats_->theta_launches[pol] += 1;
             // columns processed = SVQB ranks of the slots in this launch
             std::vector<int> rk(S);
+            BH_HIP_CHECK(hipMemcpyAsync(rk.data(), d_rank, sizeof(int) * S, hipMemcpyDeviceToHost, st));
             BH_HIP_CHECK(hipStreamSynchronize(st));
-            BH_HIP_CHECK(hipMemcpy(rk.data(), d_rank, sizeof(int) * S, hipMemcpyDeviceToHost));
             for (int s2 : plan.iter)
                 if (slots[s2].pol == pol) stats_->theta_columns[pol] += rk[s2];
             (void)hipEventDestroy(a);
@@ -1252,8 +1264,9 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
     BH_HIP_CHECK(hipStreamSynchronize(st));
     if (!plan.iter.empty()) {
         int sw[6];
-        BH_HIP_CHECK(hipMemcpy(sw, d_sweeps, sizeof(sw), hipMemcpyDeviceToHost));
-        BH_HIP_CHECK(hipMemset(d_sweeps, 0, sizeof(sw)));
+        BH_HIP_CHECK(hipMemcpyAsync(sw, d_sweeps, sizeof(sw), hipMemcpyDeviceToHost, st));
+        BH_HIP_CHECK(hipMemsetAsync(d_sweeps, 0, sizeof(sw), st));
+        BH_HIP_CHECK(hipStreamSynchronize(st));
         stats_->eigh_sweeps[0] += sw[0];
         stats_->eigh_sweeps[1] += sw[3];
         stats_->eigh_max_sweeps[0] = std::max(stats_->eigh_max_sweeps[0], (long)sw[2]);

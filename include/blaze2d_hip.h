@@ -154,6 +154,8 @@ typedef struct {
     long eigh_sweeps[2];     /* Jacobi sweeps summed over them */
     long eigh_max_sweeps[2]; /* largest sweep count of a single problem */
     long eigh_launch_max_sum[2]; /* per-launch maximum sweep count, summed over launches */
+    double host_wait_seconds;    /* host time blocked on group completion events */
+    double host_plan_seconds;    /* host time spent planning and enqueueing steps */
 } bh_run_stats;
 
 /* Runs every job to completion.  All jobs must share nx, ny.  Outputs per
