@@ -22,8 +22,10 @@ SOURCES = ["fft_ops.hip", "blas.hip", "aux_kernels.hip", "engine.hip", "api.hip"
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 
 # Packed FP32 instructions stay disabled in device code (-packed-fp32-ops).
+# -fcx-limited-range: host std::complex products and quotients use the plain
+# algebraic formulas instead of the NaN/Inf-recovering library calls.
 COMMON = [
-    "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+    "-O3", "-std=c++17", "-fPIC", "-fcx-limited-range", f"--offload-arch={ARCH}",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
 ]
@@ -33,7 +35,8 @@ def _compile(src: str) -> str:
     obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
     cmd = [HIPCC, *COMMON, "-c", os.path.join(CSRC, src), "-o", obj]
     if src.endswith(".cpp"):
-        cmd = [HIPCC, "-O3", "-std=c++17", "-fPIC", "-x", "c++", "-c", os.path.join(CSRC, src), "-o", obj]
+        cmd = [HIPCC, "-O3", "-std=c++17", "-fPIC", "-fcx-limited-range", "-x", "c++", "-c", os.path.join(CSRC, src),
+               "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stdout}\n{r.stderr}")

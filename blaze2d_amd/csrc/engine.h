@@ -57,6 +57,7 @@ struct EngineOptions {
     int precision = 0;      // 0 fp64, 1 fp32 storage (reserved)
     int profile = 0;        // time operator kernels with events (forces one group)
     int groups = 2;         // slot groups pipelined on separate streams
+    int host_threads = 0;   // host threads for per-slot planning (0: hardware threads, at most 16)
     // Streams owned by the caller.  HIP gives each new stream its own hardware
     // queue until GPU_MAX_HW_QUEUES is reached and then shares queues, which
     // serialises the groups; a caller that creates the group streams early

@@ -19,8 +19,15 @@
 //   H  [2 nb]  : subspace history (previous / current converged eigenvectors)
 // W is written at columns [nb, nb + w) by the Ritz update.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <condition_variable>
+#include <exception>
+#include <functional>
+#include <mutex>
+#include <thread>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <numeric>
 
@@ -60,6 +67,82 @@ struct Staging {
     void upload(hipStream_t st) {
         if (off) BH_HIP_CHECK(hipMemcpyAsync(d, h, off, hipMemcpyHostToDevice, st));
     }
+};
+
+// Fixed pool of host threads for per-slot work that is independent across
+// slots (k-point transitions, dielectric tables of new jobs).  The calling
+// thread takes part; the first exception thrown by any item is rethrown.
+class WorkerPool {
+  public:
+    explicit WorkerPool(int nthreads) {
+        for (int t = 0; t < nthreads; ++t) th_.emplace_back([this] { loop(); });
+    }
+    ~WorkerPool() {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto &t : th_) t.join();
+    }
+    void parallel_for(int n, const std::function<void(int)> &fn) {
+        if (n <= 0) return;
+        if (n == 1 || th_.empty()) {
+            for (int i = 0; i < n; ++i) fn(i);
+            return;
+        }
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            fn_ = &fn;
+            n_ = n;
+            next_.store(0);
+            active_ = (int)th_.size();
+            err_ = nullptr;
+            ++gen_;
+        }
+        cv_.notify_all();
+        drain();
+        std::unique_lock<std::mutex> lk(mu_);
+        done_cv_.wait(lk, [this] { return active_ == 0; });
+        fn_ = nullptr;
+        if (err_) std::rethrow_exception(err_);
+    }
+
+  private:
+    void drain() {
+        for (int i = next_.fetch_add(1); i < n_; i = next_.fetch_add(1)) {
+            try {
+                (*fn_)(i);
+            } catch (...) {
+                std::lock_guard<std::mutex> lk(mu_);
+                if (!err_) err_ = std::current_exception();
+            }
+        }
+    }
+    void loop() {
+        long seen = 0;
+        while (true) {
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+                if (stop_) return;
+                seen = gen_;
+            }
+            drain();
+            std::lock_guard<std::mutex> lk(mu_);
+            if (--active_ == 0) done_cv_.notify_all();
+        }
+    }
+    std::vector<std::thread> th_;
+    std::mutex mu_;
+    std::condition_variable cv_, done_cv_;
+    const std::function<void(int)> *fn_ = nullptr;
+    int n_ = 0;
+    std::atomic<int> next_{0};
+    int active_ = 0;
+    long gen_ = 0;
+    bool stop_ = false;
+    std::exception_ptr err_;
 };
 
 // A group of slots with its own stream, staging arena, Gram partials and
@@ -209,6 +292,9 @@ class Engine : public EngineBase {
         launch_random<T>(RND, NRMAX, N, 1.0, st);
         BH_HIP_CHECK(hipStreamSynchronize(st));
         slots.resize(S);
+        int hw = (int)std::thread::hardware_concurrency();
+        int nt = opt.host_threads > 0 ? opt.host_threads : std::min(std::max(hw, 1), 16);
+        pool.reset(new WorkerPool(std::max(nt - 1, 0)));
     }
 
     ~Engine() override {
@@ -241,6 +327,7 @@ class Engine : public EngineBase {
     EngineOptions eopt;
     hipStream_t st{};
     bool own_st = false;
+    std::unique_ptr<WorkerPool> pool;
     cplx<T> *V{}, *BV{}, *AV{}, *Q{}, *BQ{}, *AQ{}, *H{}, *RND{};
     T *EPS{}, *TEN{};
     double *d_lam{}, *d_svw{}, *d_Y{}, *d_Yw{}, *d_T{}, *d_coef{};
@@ -574,13 +661,18 @@ void Engine<T>::transition_math(int s) {
                 if (extrap) c[iP + j] += zd(-alpha, 0.0);
                 warm.push_back(c);
             }
-            // orthonormalize_fields: modified Gram-Schmidt in the B inner product
+            // orthonormalize_fields: modified Gram-Schmidt in the B inner
+            // product; with M w_i at hand, w_i^H M w_j = (M w_i)^H w_j
+            std::vector<zd> mw(kd);
             for (int i = 0; i < m; ++i) {
-                double nrm = std::sqrt(std::max(cs.inner(warm[i].data(), warm[i].data()).real(), 0.0));
-                if (nrm > 1e-14)
+                cs.apply(warm[i].data(), mw.data());
+                double nrm = std::sqrt(std::max(cdot(warm[i].data(), mw.data(), kd).real(), 0.0));
+                if (nrm > 1e-14) {
                     for (auto &v : warm[i]) v /= nrm;
+                    for (auto &v : mw) v /= nrm;
+                }
                 for (int j = i + 1; j < m; ++j) {
-                    zd c = cs.inner(warm[i].data(), warm[j].data());
+                    zd c = cdot(mw.data(), warm[j].data(), kd);
                     for (int q = 0; q < kd; ++q) warm[j][q] -= c * warm[i][q];
                 }
             }
@@ -589,16 +681,20 @@ void Engine<T>::transition_math(int s) {
     // --- initialize_block (initialization.rs:248-367) with Gamma deflation ---
     sl.k = knext;
     set_kpoint(s);
-    std::vector<std::vector<zd>> ent;
+    std::vector<std::vector<zd>> ent, ment;  // accepted entries and their images M e
     auto try_add = [&](std::vector<zd> v) {
-        for (auto &e : ent) {  // project_out: v -= <e, v>_B e
-            zd c = cs.inner(e.data(), v.data());
-            for (int q = 0; q < kd; ++q) v[q] -= c * e[q];
+        for (size_t k = 0; k < ent.size(); ++k) {  // project_out: v -= <e, v>_B e
+            zd c = cdot(ment[k].data(), v.data(), kd);
+            for (int q = 0; q < kd; ++q) v[q] -= c * ent[k][q];
         }
-        double nrm = std::sqrt(std::max(cs.inner(v.data(), v.data()).real(), 0.0));
+        std::vector<zd> mv(kd);
+        cs.apply(v.data(), mv.data());
+        double nrm = std::sqrt(std::max(cdot(v.data(), mv.data(), kd).real(), 0.0));
         if (!(nrm > 1e-12)) return false;
         for (auto &x : v) x /= nrm;
+        for (auto &x : mv) x /= nrm;
         ent.push_back(std::move(v));
+        ment.push_back(std::move(mv));
         return true;
     };
     for (size_t i = 0; i < warm.size() && (int)ent.size() < sl.m0; ++i) try_add(warm[i]);
@@ -612,15 +708,22 @@ void Engine<T>::transition_math(int s) {
     }
     if (sl.gamma_defl) {
         // deflation add_vector(y0) then project_single + renormalise each entry
-        std::vector<zd> y(kd, 0.0);
+        std::vector<zd> y(kd, 0.0), my(kd);
         y[iY] = 1.0;
-        double yn = std::sqrt(std::max(cs.inner(y.data(), y.data()).real(), 0.0));
-        if (yn > 1e-12 && yn >= 1e-10)
+        cs.apply(y.data(), my.data());
+        double yn = std::sqrt(std::max(cdot(y.data(), my.data(), kd).real(), 0.0));
+        if (yn > 1e-12 && yn >= 1e-10) {
             for (auto &v : y) v /= yn;
-        for (auto &e : ent) {
-            zd c = cs.inner(y.data(), e.data());
-            for (int q = 0; q < kd; ++q) e[q] -= c * y[q];
-            double nrm = std::sqrt(std::max(cs.inner(e.data(), e.data()).real(), 0.0));
+            for (auto &v : my) v /= yn;
+        }
+        for (size_t k = 0; k < ent.size(); ++k) {
+            auto &e = ent[k], &me = ment[k];
+            zd c = cdot(my.data(), e.data(), kd);
+            for (int q = 0; q < kd; ++q) {
+                e[q] -= c * y[q];
+                me[q] -= c * my[q];
+            }
+            double nrm = std::sqrt(std::max(cdot(e.data(), me.data(), kd).real(), 0.0));
             if (nrm > 1e-12)
                 for (auto &v : e) v /= nrm;
         }
@@ -683,10 +786,16 @@ void Engine<T>::run(const std::vector<JobSpec> &jobs, std::vector<JobResult> &ou
             }
             auto p0 = std::chrono::steady_clock::now();
             StepPlan plan;
+            // k-point transitions are independent across slots: run them on the pool
+            std::vector<int> trans;
+            for (int s = G.s0; s < G.s1; ++s)
+                if (slots[s].state == GRAM_PENDING && slots[s].rb_index >= 0) trans.push_back(s);
+            pool->parallel_for((int)trans.size(), [&](int i) { transition_math(trans[i]); });
+            std::vector<char> in_trans(G.s1 - G.s0, 0);
+            for (int s : trans) in_trans[s - G.s0] = 1;
             for (int s = G.s0; s < G.s1; ++s) {
                 Slot &sl = slots[s];
-                if (sl.state == GRAM_PENDING && sl.rb_index >= 0) {
-                    transition_math(s);
+                if (in_trans[s - G.s0]) {
                     sl.rb_index = -1;
                     if (sl.state == ITER) plan.t2.push_back(s);
                 } else if (sl.state == ITER) {
@@ -694,15 +803,16 @@ void Engine<T>::run(const std::vector<JobSpec> &jobs, std::vector<JobResult> &ou
                     decide(s);
                 }
             }
-            for (int s = G.s0; s < G.s1; ++s) {
-                Slot &sl = slots[s];
-                if (sl.state == FREE && next_job < jobs.size()) {
-                    start_job(s, (int)next_job++);
-                    plan.table_upload.push_back(s);
-                    plan.fill_y0.push_back(s);
-                    sl.E_cols.clear();
-                    schedule_t1(s);
-                }
+            std::vector<std::pair<int, int>> starts;  // (slot, job)
+            for (int s = G.s0; s < G.s1; ++s)
+                if (slots[s].state == FREE && next_job < jobs.size()) starts.push_back({s, (int)next_job++});
+            pool->parallel_for((int)starts.size(), [&](int i) { start_job(starts[i].first, starts[i].second); });
+            for (auto &sj : starts) {
+                const int s = sj.first;
+                plan.table_upload.push_back(s);
+                plan.fill_y0.push_back(s);
+                slots[s].E_cols.clear();
+                schedule_t1(s);
             }
             bool any = false;
             for (int s = G.s0; s < G.s1; ++s) {

@@ -426,16 +426,36 @@ std::vector<int> track_bands(int m, const std::vector<zd> &O, const std::vector<
     return perm;
 }
 
-zd CoefSpace::inner(const zd *a, const zd *b) const {
-    const std::vector<zd> &G = *M;
-    zd s = 0;
-    for (int j = 0; j < kd; ++j) {
-        if (b[j] == zd(0.0)) continue;
-        zd t = 0;
-        for (int i = 0; i < kd; ++i) t += std::conj(a[i]) * G[i + (size_t)j * kd];
-        s += t * b[j];
+zd cdot(const zd *a, const zd *b, int n) {
+    double re = 0.0, im = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const double ar = a[i].real(), ai = a[i].imag(), br = b[i].real(), bi = b[i].imag();
+        re += ar * br + ai * bi;
+        im += ar * bi - ai * br;
     }
-    return s;
+    return zd(re, im);
+}
+
+void CoefSpace::apply(const zd *v, zd *out) const {
+    const std::vector<zd> &G = *M;
+    std::vector<double> re(kd, 0.0), im(kd, 0.0);
+    for (int j = 0; j < kd; ++j) {
+        const double vr = v[j].real(), vi = v[j].imag();
+        if (vr == 0.0 && vi == 0.0) continue;
+        const zd *col = G.data() + (size_t)j * kd;
+        for (int i = 0; i < kd; ++i) {
+            const double gr = col[i].real(), gi = col[i].imag();
+            re[i] += gr * vr - gi * vi;
+            im[i] += gr * vi + gi * vr;
+        }
+    }
+    for (int i = 0; i < kd; ++i) out[i] = zd(re[i], im[i]);
+}
+
+zd CoefSpace::inner(const zd *a, const zd *b) const {
+    std::vector<zd> mb(kd);
+    apply(b, mb.data());
+    return cdot(a, mb.data(), kd);
 }
 
 }  // namespace bh

@@ -62,8 +62,12 @@ std::vector<int> track_bands(int m, const std::vector<zd> &overlap, const std::v
 // basis K whose (weighted) Gram matrix M = K^H B K is known (col-major kd x kd).
 struct CoefSpace {
     int kd;
-    const std::vector<zd> *M;
+    const std::vector<zd> *M;                  // Hermitian kd x kd, column-major
     zd inner(const zd *a, const zd *b) const;  // a^H M b
+    void apply(const zd *v, zd *out) const;    // out = M v
 };
+
+// a^H b over n entries
+zd cdot(const zd *a, const zd *b, int n);
 
 }  // namespace bh
