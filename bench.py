@@ -8,9 +8,11 @@ step = one batched pass over the whole sweep with its inputs resident on the
 device; the clock brackets exactly --steps passes after --warmup untimed ones.
 
 Also reported on the same line:
-  roofline      the dominant operator kernel (Theta apply) timed with HIP
-                events on the engine's own stream, algorithmic bytes per
-                column over its average launch time, against 8 TB/s HBM;
+  roofline      one kernel class (--timed-kernel: 2 = the dual Rayleigh-Ritz
+                Gram, 1 = the Theta operator) timed with HIP events on the
+                stream each launch runs on, during the timed steps themselves:
+                algorithmic bytes (and FLOPs) per launch over the average
+                launch time, against 8 TB/s HBM (and 78.6 TFLOP/s FP64);
   cpu_baseline  the FP64 C++ oracle (a restatement of the reference CPU path)
                 on a bounded sample of the same sweep, one band structure per
                 host thread;
@@ -34,18 +36,14 @@ sys.path.insert(0, ROOT)
 
 METRIC = "converged k-point solves/sec, res64 12 bands, ≤1e-6 rel freq err vs FP64"
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+TIMED_NAMES = {1: "op_kernel THETA_TM/TE (64x64 fused 2D FFT operator)",
+               2: "gram2_kernel (dual Rayleigh-Ritz Gram Z^H BZ, Z^H AZ)"}
 
 
 def solves_of(jobs):
     # a copied closing Gamma is not a solve
     return sum(j.k_path.shape[0] - 1 for j in jobs)
-
-
-def theta_bytes_per_column(n, pol, precision):
-    s = 8 if precision == 1 else 16
-    t = 4 * (4 if precision == 1 else 8)
-    # in + out (+ the 2x2 inverse-eps tensor for TE)
-    return n * (2 * s + (t if pol == "TE" else 0))
 
 
 def rel_err(gpu_bands, ref_bands):
@@ -70,6 +68,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=4, help="band structures timed on the CPU oracle")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip roofline / plain-GPU extra passes")
+    ap.add_argument("--timed-kernel", type=int, default=2, choices=(1, 2),
+                    help="kernel class timed for the roofline: 1 Theta operator, 2 dual Rayleigh-Ritz Gram")
     ap.add_argument("--exit-rule", type=int, default=1,
                     help="1: a solve ends when every band is soft-locked (all bands converged); "
                          "0: the reference solve() exit accounting")
@@ -100,10 +100,16 @@ def main():
     conv = 0
     tot_iters = 0
     last = None
+    timed = {"ms": 0.0, "launches": 0, "bytes": 0.0, "flops": 0.0}
     for k in range(args.steps):
-        res, st = run_batch(be, jobs, ropts, slots=slots, precision=args.precision, groups=args.groups)
+        res, st = run_batch(be, jobs, ropts, slots=slots, precision=args.precision, groups=args.groups,
+                            timed_kernel=args.timed_kernel)
         conv += sum(int(r.converged[:-1].sum()) for r in res)
         tot_iters += st.iterations
+        timed["ms"] += st.timed_ms
+        timed["launches"] += st.timed_launches
+        timed["bytes"] += st.timed_bytes
+        timed["flops"] += st.timed_flops
         last = res
         print(f"[bench] step {k}: engine {st.seconds:.2f}s, {st.steps} engine steps, host plan "
               f"{st.host_plan_seconds:.2f}s, host wait {st.host_wait_seconds:.2f}s", file=sys.stderr, flush=True)
@@ -137,38 +143,31 @@ def main():
         },
     }
 
+    if timed["launches"] > 0 and timed["ms"] > 0:
+        avg_s = timed["ms"] * 1e-3 / timed["launches"]
+        bytes_per = timed["bytes"] / timed["launches"]
+        flops_per = timed["flops"] / timed["launches"]
+        gbs = bytes_per / avg_s / 1e9
+        tfs = flops_per / avg_s / 1e12
+        intensity = flops_per / max(bytes_per, 1.0)
+        ridge = FP64_PEAK_TFLOPS * 1e12 / (HBM_PEAK_GBS * 1e9)
+        hbm = intensity < ridge
+        out["roofline"] = {
+            "bound": "hbm" if hbm else "mfma",
+            "achieved": round(gbs if hbm else tfs, 3),
+            "peak": HBM_PEAK_GBS if hbm else FP64_PEAK_TFLOPS,
+            "unit": "GB/s" if hbm else "TFLOP/s",
+            "frac": round(gbs / HBM_PEAK_GBS if hbm else tfs / FP64_PEAK_TFLOPS, 5),
+            "traffic": None,
+            "kernel": TIMED_NAMES[args.timed_kernel],
+            "avg_launch_ms": round(avg_s * 1e3, 4),
+            "launches": timed["launches"],
+            "algorithmic_bytes_per_launch": int(bytes_per),
+            "algorithmic_flops_per_launch": int(flops_per),
+            "achieved_tflops_fp64": round(tfs, 3),
+            "frac_fp64_peak": round(tfs / FP64_PEAK_TFLOPS, 5),
+        }
     if not args.no_extra:
-        # roofline of the dominant operator kernel: one profiled pass over a sub-batch
-        sub = jobs[: min(len(jobs), 64)]
-        res_p, stp = run_batch(be, sub, ropts, slots=len(sub), precision=args.precision, profile=True)
-        best = None
-        for pol_i, pol in enumerate(("TM", "TE")):
-            if stp.theta_launches[pol_i] == 0:
-                continue
-            avg_ms = stp.theta_ms[pol_i] / stp.theta_launches[pol_i]
-            cols_per_launch = stp.theta_columns[pol_i] / stp.theta_launches[pol_i]
-            bytes_per_launch = cols_per_launch * theta_bytes_per_column(64 * 64, pol, args.precision)
-            gbs = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-            entry = {"pol": pol, "avg_ms": avg_ms, "cols": cols_per_launch, "bytes": bytes_per_launch, "gbs": gbs,
-                     "total_ms": stp.theta_ms[pol_i]}
-            if best is None or entry["total_ms"] > best["total_ms"]:
-                best = entry
-        if best is not None:
-            out["roofline"] = {
-                "bound": "hbm",
-                "kernel": f"op_kernel THETA_{best['pol']} (64x64, fused 2D FFT operator)",
-                "achieved": round(best["gbs"], 2),
-                "peak": HBM_PEAK_GBS,
-                "unit": "GB/s",
-                "frac": round(best["gbs"] / HBM_PEAK_GBS, 5),
-                "traffic": None,
-                "avg_launch_ms": round(best["avg_ms"], 4),
-                "columns_per_launch": round(best["cols"], 1),
-                "algorithmic_bytes_per_launch": int(best["bytes"]),
-            }
-            out["eigh_sweeps_per_problem"] = {
-                "svqb": round(stp.eigh_sweeps[0] / max(stp.eigh_calls[0], 1), 2),
-                "rayleigh_ritz": round(stp.eigh_sweeps[1] / max(stp.eigh_calls[1], 1), 2)}
         # plain FP64 path, one band structure at a time (the batching baseline)
         pj = jobs[:2]
         t = time.perf_counter()

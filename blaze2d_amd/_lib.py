@@ -65,7 +65,7 @@ class Job(ctypes.Structure):
 
 class RunOptions(ctypes.Structure):
     _fields_ = [("slots", ctypes.c_int), ("precision", ctypes.c_int), ("profile", ctypes.c_int),
-                ("groups", ctypes.c_int)]
+                ("groups", ctypes.c_int), ("timed_kernel", ctypes.c_int)]
 
 
 class RunStats(ctypes.Structure):
@@ -74,13 +74,14 @@ class RunStats(ctypes.Structure):
         ("iterations", ctypes.c_long),
         ("solves", ctypes.c_long),
         ("seconds", ctypes.c_double),
-        ("theta_ms", ctypes.c_double * 2),
-        ("theta_launches", ctypes.c_long * 2),
-        ("theta_columns", ctypes.c_long * 2),
         ("eigh_calls", ctypes.c_long * 2),
         ("eigh_sweeps", ctypes.c_long * 2),
         ("eigh_max_sweeps", ctypes.c_long * 2),
         ("eigh_launch_max_sum", ctypes.c_long * 2),
+        ("timed_ms", ctypes.c_double),
+        ("timed_launches", ctypes.c_long),
+        ("timed_bytes", ctypes.c_double),
+        ("timed_flops", ctypes.c_double),
         ("host_wait_seconds", ctypes.c_double),
         ("host_plan_seconds", ctypes.c_double),
     ]

@@ -148,9 +148,10 @@ class BatchStats:
     iterations: int
     solves: int
     seconds: float
-    theta_ms: tuple        # per polarisation (TM, TE), profile mode only
-    theta_launches: tuple
-    theta_columns: tuple
+    timed_ms: float = 0.0           # launches of the timed kernel class (event-timed on their stream)
+    timed_launches: int = 0
+    timed_bytes: float = 0.0        # their algorithmic HBM bytes
+    timed_flops: float = 0.0        # their algorithmic FLOPs
     eigh_calls: tuple = (0, 0)      # (SVQB, Rayleigh-Ritz), profile mode only
     eigh_sweeps: tuple = (0, 0)
     eigh_max_sweeps: tuple = (0, 0)
@@ -160,16 +161,18 @@ class BatchStats:
 
 
 def run_batch(backend: HipBackend, jobs: Sequence[BandStructureJob], options: Optional[RunOptions] = None,
-              slots: int = 64, precision: int = 0, profile: bool = False, groups: int = 2):
+              slots: int = 64, precision: int = 0, profile: bool = False, groups: int = 2, timed_kernel: int = 0):
     """Runs many band structures concurrently on one GPU (all on one grid).
 
     `slots` band structures are resident at once; they are split into `groups`
-    pipelined on separate HIP streams.
+    pipelined on separate HIP streams.  `timed_kernel` (1 Theta operator,
+    2 dual Rayleigh-Ritz Gram) times every launch of that kernel class with HIP
+    events on its stream during the run.
     """
     opts = options or RunOptions()
     jobs = list(jobs)
     if not jobs:
-        return [], BatchStats(0, 0, 0, 0.0, (0.0, 0.0), (0, 0), (0, 0))
+        return [], BatchStats(0, 0, 0, 0.0)
     arr = (L.Job * len(jobs))()
     for i, j in enumerate(jobs):
         arr[i] = j.to_c(opts)
@@ -178,7 +181,7 @@ def run_batch(backend: HipBackend, jobs: Sequence[BandStructureJob], options: Op
     om = np.zeros(tot_om)
     it = np.zeros(tot_k, dtype=np.int32)
     cv = np.zeros(tot_k, dtype=np.int32)
-    ro = L.RunOptions(slots, precision, int(profile), groups)
+    ro = L.RunOptions(slots, precision, int(profile), groups, timed_kernel)
     st = L.RunStats()
     L.check(L.lib().bh_bandstructure_run(backend.handle, arr, len(jobs), ctypes.byref(ro), om.ctypes.data_as(L.c_dp),
                                          it.ctypes.data_as(L.c_ip), cv.ctypes.data_as(L.c_ip), ctypes.byref(st)),
@@ -191,8 +194,8 @@ def run_batch(backend: HipBackend, jobs: Sequence[BandStructureJob], options: Op
                                        it[k:k + nk].copy(), cv[k:k + nk].copy()))
         o += nk * j.n_bands
         k += nk
-    stats = BatchStats(st.steps, st.iterations, st.solves, st.seconds, tuple(st.theta_ms), tuple(st.theta_launches),
-                       tuple(st.theta_columns), tuple(st.eigh_calls), tuple(st.eigh_sweeps),
+    stats = BatchStats(st.steps, st.iterations, st.solves, st.seconds, st.timed_ms, st.timed_launches,
+                       st.timed_bytes, st.timed_flops, tuple(st.eigh_calls), tuple(st.eigh_sweeps),
                        tuple(st.eigh_max_sweeps), tuple(st.eigh_launch_max_sum), st.host_wait_seconds,
                        st.host_plan_seconds)
     return out, stats

@@ -567,6 +567,8 @@ int bh_bandstructure_run(bh_context *ctx, const bh_job *jobs, int njobs, const b
     eo.precision = opts ? opts->precision : 0;
     eo.profile = opts ? opts->profile : 0;
     eo.groups = opts && opts->groups > 0 ? opts->groups : 2;
+    eo.timed_kernel = opts ? opts->timed_kernel : 0;
+    if (eo.timed_kernel < 0 || eo.timed_kernel > 2) return fail("bh_bandstructure_run: timed_kernel must be 0, 1 or 2");
     eo.setup_stream = ctx->st;
     eo.group_streams[0] = ctx->gst[0];
     eo.group_streams[1] = ctx->gst[1];
@@ -593,14 +595,15 @@ int bh_bandstructure_run(bh_context *ctx, const bh_job *jobs, int njobs, const b
         stats->solves = rs.solves;
         stats->seconds = rs.seconds;
         for (int p = 0; p < 2; ++p) {
-            stats->theta_ms[p] = rs.theta_ms[p];
-            stats->theta_launches[p] = rs.theta_launches[p];
-            stats->theta_columns[p] = rs.theta_columns[p];
             stats->eigh_calls[p] = rs.eigh_calls[p];
             stats->eigh_sweeps[p] = rs.eigh_sweeps[p];
             stats->eigh_max_sweeps[p] = rs.eigh_max_sweeps[p];
             stats->eigh_launch_max_sum[p] = rs.eigh_launch_max_sum[p];
         }
+        stats->timed_ms = rs.timed_ms;
+        stats->timed_launches = rs.timed_launches;
+        stats->timed_bytes = rs.timed_bytes;
+        stats->timed_flops = rs.timed_flops;
         stats->host_wait_seconds = rs.host_wait_seconds;
         stats->host_plan_seconds = rs.host_plan_seconds;
     }

@@ -55,6 +55,31 @@ struct EighWork {
     int *sweeps_out;        // optional: Jacobi sweeps used (diagnostics)
 };
 
+// Both Rayleigh-Ritz Grams in one pass over the rows of the subspace Z:
+// G1 = Z^H (B Z) and G2 = Z^H (A Z), lower triangles, partial sums per row
+// chunk (row-major [chunk][ld][ld] complex, entry i*ld + j for i >= j).
+template <typename T>
+struct Gram2Work {
+    const cplx<T> *Z, *BZ, *AZ;  // column j at base + j*stride (BZ may equal Z)
+    double *partial1, *partial2;
+    long stride;
+    int n, ld;
+};
+
+// Rayleigh-Ritz on the dual Gram: B-orthonormalising transform T of Z from the
+// scaled G1 (Cholesky QR, or SVQB when a pivot shows G1 ill-conditioned),
+// projected operator M = T^H G2 T, its eigendecomposition M = Y diag(theta) Y^H,
+// and the coefficients C = T Y of the new X (first cnt_x Ritz vectors) and W
+// (next cnt_w) as combinations of the columns of Z.
+struct RRWork {
+    const double *partial1, *partial2;  // from the dual Gram
+    double *lam;            // ascending Ritz values (rank of them)
+    double *C;              // col-major n x (cnt_x + cnt_w), leading dim ldc, complex interleaved
+    int *cnt;               // [cnt_x, cnt_w, cnt_x + cnt_w]
+    int n, nchunks, ld, ldc, nsplit, pad;
+    int *sweeps_out;        // optional diagnostics (see rr_kernel)
+};
+
 constexpr int GRAM_MAXC = 48;  // maximum columns per Gram operand
 constexpr int EIGH_MAXN = 64;
 
@@ -75,6 +100,10 @@ void launch_reduce(const ReduceWork *dwork, int nwork, int ld, hipStream_t st);
 template <typename T>
 void launch_lincomb(const LincombWork<T> *dwork, int nwork, long rows, int maxq, hipStream_t st);
 void launch_eigh(const EighWork *dwork, int nwork, int maxn, hipStream_t st);
+// maxn: largest n in the launch (selects the 36- or 48-column kernel)
+template <typename T>
+void launch_gram2(const Gram2Work<T> *dwork, int nwork, int nchunks, long rows, int maxn, hipStream_t st);
+void launch_rr(const RRWork *dwork, int nwork, int maxn, hipStream_t st);
 size_t eigh_shared_bytes(int maxn);
 
 }  // namespace bh

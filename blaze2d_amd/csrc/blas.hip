@@ -234,6 +234,143 @@ size_t eigh_shared_bytes(int maxn) {
     return sizeof(double) * (4 * ne * ne + ne + np + 2 * np) + sizeof(int) * 3 * np;
 }
 
+// Scratch for one Jacobi round: the rotation of each of the np pairs.
+struct JacobiScratch {
+    double2 *rs;  // sigma = s conj(h / |h|)
+    double *rc;   // cosine
+    int *rp, *rq, *rf;  // pair (p < q); flag 0 identity, 1 rotate, 2 negligible (zero the pair)
+};
+
+// Parallel cyclic Jacobi for the Hermitian matrix A held in LDS (col-major,
+// leading dimension ld, order ne even, padding rows/columns zero), with the
+// rotations accumulated into V.  All EIGH_NT threads of the workgroup call it.
+//
+// Each round applies ne/2 disjoint rotations (round-robin pairing).  The matrix
+// is partitioned into np x np 2x2 blocks (row pair, column pair); a thread
+// owning a block applies both the column rotation of its column pair and the
+// row rotation of its row pair in registers, so a round needs two barriers.
+// A sweep ends the iteration when it needed no rotation or its largest
+// relative rotation |a_pq| / sqrt(|a_pp a_qq|) was below 1e-9 (quadratic
+// convergence leaves the off-diagonal below 1e-18 relative).  Returns the
+// number of sweeps (at most 40).
+__device__ int jacobi_lds(double2 *A, double2 *V, int ld, int ne, JacobiScratch sc, int *nrot,
+                          unsigned long long *maxratio_bits) {
+    const int tid = threadIdx.x;
+    const int np = ne / 2;
+    double2 *rs = sc.rs;
+    double *rc = sc.rc;
+    int *rp = sc.rp, *rq = sc.rq, *rf = sc.rf;
+    // fixed per-thread work for every round: one 2x2 block (np*np <= 1024) and
+    // up to two eigenvector items (np*ne <= 2048)
+    const bool has_blk = tid < np * np;
+    const int bt1 = has_blk ? tid / np : 0, bt2 = has_blk ? tid - bt1 * np : 0;
+    const int nv = np * ne;
+    const int vt0 = tid < nv ? tid / ne : 0, vk0 = tid < nv ? tid - vt0 * ne : 0;
+    const int vt1 = tid + EIGH_NT < nv ? (tid + EIGH_NT) / ne : 0;
+    const int vk1 = tid + EIGH_NT < nv ? (tid + EIGH_NT) - vt1 * ne : 0;
+    int sweep = 0;
+    for (; sweep < 40; ++sweep) {
+        if (tid == 0) { *nrot = 0; *maxratio_bits = 0ull; }
+        __syncthreads();
+        for (int r = 0; r < ne - 1; ++r) {
+            // round-robin pairing, pair slot t: slot 0 (r, ne-1); slot t ((r+t) mod (ne-1), (r-t) mod (ne-1))
+            if (tid < np) {
+                int p, q;
+                if (tid == 0) { p = r; q = ne - 1; }
+                else { p = (r + tid) % (ne - 1); q = (r - tid + ne - 1) % (ne - 1); }
+                if (p > q) { int t = p; p = q; q = t; }
+                double2 h = A[p + q * ld];
+                double habs = sqrt(h.x * h.x + h.y * h.y);
+                double app = A[p + p * ld].x, aqq = A[q + q * ld].x;
+                double c = 1.0;
+                int flag = 0;
+                double2 sg = make_double2(0.0, 0.0);
+                double scale = sqrt(fabs(app) * fabs(aqq));
+                if (habs != 0.0 && habs > 2.2e-16 * scale) {
+                    flag = 1;
+                    double tau = (aqq - app) / (2.0 * habs);
+                    double t = (tau >= 0 ? -1.0 : 1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+                    c = 1.0 / sqrt(1.0 + t * t);
+                    double s = t * c;
+                    sg = make_double2(s * h.x / habs, -s * h.y / habs);  // sigma = s conj(h/|h|)
+                    atomicAdd(nrot, 1);
+                    double ratio = scale > 0.0 ? habs / scale : 1.0;
+                    atomicMax(maxratio_bits, (unsigned long long)__double_as_longlong(ratio));
+                } else if (habs != 0.0) {
+                    flag = 2;
+                }
+                rf[tid] = flag;
+                rp[tid] = p;
+                rq[tid] = q;
+                rc[tid] = c;
+                rs[tid] = sg;
+            }
+            __syncthreads();
+            do {
+                if (!has_blk) break;
+                const int t1 = bt1, t2 = bt2;  // row pair, column pair
+                const int f1 = rf[t1], f2 = rf[t2];
+                if (f1 != 1 && f2 != 1) {
+                    if (t1 == t2 && f1 == 2) {
+                        int p = rp[t1], q = rq[t1];
+                        A[p + q * ld] = make_double2(0.0, 0.0);
+                        A[q + p * ld] = make_double2(0.0, 0.0);
+                    }
+                    break;
+                }
+                const int p1 = rp[t1], q1 = rq[t1], p2 = rp[t2], q2 = rq[t2];
+                double2 a11 = A[p1 + p2 * ld], a12 = A[p1 + q2 * ld];
+                double2 a21 = A[q1 + p2 * ld], a22 = A[q1 + q2 * ld];
+                if (f2 == 1) {  // columns: [:,p] = c [:,p] + s [:,q]; [:,q] = -conj(s) [:,p] + c [:,q]
+                    const double c = rc[t2];
+                    const double2 s = rs[t2];
+                    double2 n11 = make_double2(c * a11.x + s.x * a12.x - s.y * a12.y, c * a11.y + s.x * a12.y + s.y * a12.x);
+                    double2 n12 = make_double2(-(s.x * a11.x + s.y * a11.y) + c * a12.x, -(s.x * a11.y - s.y * a11.x) + c * a12.y);
+                    double2 n21 = make_double2(c * a21.x + s.x * a22.x - s.y * a22.y, c * a21.y + s.x * a22.y + s.y * a22.x);
+                    double2 n22 = make_double2(-(s.x * a21.x + s.y * a21.y) + c * a22.x, -(s.x * a21.y - s.y * a21.x) + c * a22.y);
+                    a11 = n11; a12 = n12; a21 = n21; a22 = n22;
+                }
+                if (f1 == 1) {  // rows: [p,:] = c [p,:] + conj(s) [q,:]; [q,:] = -s [p,:] + c [q,:]
+                    const double c = rc[t1];
+                    const double2 s = rs[t1];
+                    double2 n11 = make_double2(c * a11.x + s.x * a21.x + s.y * a21.y, c * a11.y + s.x * a21.y - s.y * a21.x);
+                    double2 n21 = make_double2(-(s.x * a11.x - s.y * a11.y) + c * a21.x, -(s.x * a11.y + s.y * a11.x) + c * a21.y);
+                    double2 n12 = make_double2(c * a12.x + s.x * a22.x + s.y * a22.y, c * a12.y + s.x * a22.y - s.y * a22.x);
+                    double2 n22 = make_double2(-(s.x * a12.x - s.y * a12.y) + c * a22.x, -(s.x * a12.y + s.y * a12.x) + c * a22.y);
+                    a11 = n11; a12 = n12; a21 = n21; a22 = n22;
+                }
+                if (t1 == t2) {  // the rotated pair: exact zero off-diagonal, real diagonal
+                    a12 = make_double2(0.0, 0.0);
+                    a21 = make_double2(0.0, 0.0);
+                    a11.y = 0.0;
+                    a22.y = 0.0;
+                }
+                A[p1 + p2 * ld] = a11;
+                A[p1 + q2 * ld] = a12;
+                A[q1 + p2 * ld] = a21;
+                A[q1 + q2 * ld] = a22;
+            } while (false);
+            for (int e = tid, it = 0; e < nv; e += EIGH_NT, ++it) {  // eigenvectors: column rotations only
+                const int t = it == 0 ? vt0 : vt1, k = it == 0 ? vk0 : vk1;
+                if (rf[t] != 1) continue;
+                double c = rc[t];
+                double2 sg = rs[t];
+                int p = rp[t], q = rq[t];
+                double2 vkp = V[k + p * ld], vkq = V[k + q * ld];
+                V[k + p * ld] = make_double2(c * vkp.x + sg.x * vkq.x - sg.y * vkq.y,
+                                             c * vkp.y + sg.x * vkq.y + sg.y * vkq.x);
+                V[k + q * ld] = make_double2(-(sg.x * vkp.x + sg.y * vkp.y) + c * vkq.x,
+                                             -(sg.x * vkp.y - sg.y * vkp.x) + c * vkq.y);
+            }
+            __syncthreads();
+        }
+        const bool done = *nrot == 0 || __longlong_as_double((long long)*maxratio_bits) < 1e-9;
+        __syncthreads();  // every thread has read the flags before thread 0 resets them
+        if (done) break;
+    }
+    return sweep < 40 ? sweep + 1 : 40;
+}
+
 __global__ void __launch_bounds__(1024) eigh_kernel(const EighWork *__restrict__ work) {
     const EighWork &w = work[blockIdx.x];
     const int n = w.dev_n ? *w.dev_n : w.n;
@@ -258,6 +395,7 @@ __global__ void __launch_bounds__(1024) eigh_kernel(const EighWork *__restrict__
     int *rq = rp + np;
     int *rf = rq + np;  // 0 identity, 1 rotate, 2 negligible (zero the pair)
     __shared__ int nrot;
+    __shared__ unsigned long long maxratio_bits;
 
     const int ld = w.ld;
     auto G = [&](int a, int b) -> double2 {
@@ -365,126 +503,10 @@ __global__ void __launch_bounds__(1024) eigh_kernel(const EighWork *__restrict__
         }
         __syncthreads();
     }
-    // Each round applies np disjoint rotations.  The matrix is partitioned
-    // into np x np 2x2 blocks (row pair, column pair); a thread owning a block
-    // applies both the column rotation of its column pair and the row
-    // rotation of its row pair in registers, so a round needs two barriers.
-    __shared__ unsigned long long maxratio_bits;  // non-negative doubles order like their bit patterns
-    // fixed per-thread work for every round: one 2x2 block (np*np <= 1024) and
-    // up to two eigenvector items (np*ne <= 2048)
-    const bool has_blk = tid < np * np;
-    const int bt1 = has_blk ? tid / np : 0, bt2 = has_blk ? tid - bt1 * np : 0;
-    const int nv = np * ne;
-    const int vt0 = tid < nv ? tid / ne : 0, vk0 = tid < nv ? tid - vt0 * ne : 0;
-    const int vt1 = tid + EIGH_NT < nv ? (tid + EIGH_NT) / ne : 0;
-    const int vk1 = tid + EIGH_NT < nv ? (tid + EIGH_NT) - vt1 * ne : 0;
-    // round-robin pairing for this thread's pair slot (tid < np), per round r:
-    // slot 0: (r, ne-1); slot t: ((r+t) mod (ne-1), (r-t) mod (ne-1))
-    for (int sweep = 0; sweep < 40; ++sweep) {
-        if (tid == 0) { nrot = 0; maxratio_bits = 0ull; }
-        __syncthreads();
-        for (int r = 0; r < ne - 1; ++r) {
-            if (tid < np) {
-                int p, q;
-                if (tid == 0) { p = r; q = ne - 1; }
-                else { p = (r + tid) % (ne - 1); q = (r - tid + ne - 1) % (ne - 1); }
-                if (p > q) { int t = p; p = q; q = t; }
-                double2 h = A[p + q * ne];
-                double habs = sqrt(h.x * h.x + h.y * h.y);
-                double app = A[p + p * ne].x, aqq = A[q + q * ne].x;
-                double c = 1.0;
-                int flag = 0;
-                double2 sg = make_double2(0.0, 0.0);
-                double scale = sqrt(fabs(app) * fabs(aqq));
-                if (habs != 0.0 && habs > 2.2e-16 * scale) {
-                    flag = 1;
-                    double tau = (aqq - app) / (2.0 * habs);
-                    double t = (tau >= 0 ? -1.0 : 1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-                    c = 1.0 / sqrt(1.0 + t * t);
-                    double s = t * c;
-                    sg = make_double2(s * h.x / habs, -s * h.y / habs);  // sigma = s conj(h/|h|)
-                    atomicAdd(&nrot, 1);
-                    double ratio = scale > 0.0 ? habs / scale : 1.0;
-                    atomicMax(&maxratio_bits, (unsigned long long)__double_as_longlong(ratio));
-                } else if (habs != 0.0) {
-                    flag = 2;
-                }
-                rf[tid] = flag;
-                rp[tid] = p;
-                rq[tid] = q;
-                rc[tid] = c;
-                rs[tid] = sg;
-            }
-            __syncthreads();
-            do {
-                if (!has_blk) break;
-                const int t1 = bt1, t2 = bt2;  // row pair, column pair
-                const int f1 = rf[t1], f2 = rf[t2];
-                if (f1 != 1 && f2 != 1) {
-                    if (t1 == t2 && f1 == 2) {
-                        int p = rp[t1], q = rq[t1];
-                        A[p + q * ne] = make_double2(0.0, 0.0);
-                        A[q + p * ne] = make_double2(0.0, 0.0);
-                    }
-                    break;
-                }
-                const int p1 = rp[t1], q1 = rq[t1], p2 = rp[t2], q2 = rq[t2];
-                double2 a11 = A[p1 + p2 * ne], a12 = A[p1 + q2 * ne];
-                double2 a21 = A[q1 + p2 * ne], a22 = A[q1 + q2 * ne];
-                if (f2 == 1) {  // columns: [:,p] = c [:,p] + s [:,q]; [:,q] = -conj(s) [:,p] + c [:,q]
-                    const double c = rc[t2];
-                    const double2 s = rs[t2];
-                    double2 n11 = make_double2(c * a11.x + s.x * a12.x - s.y * a12.y, c * a11.y + s.x * a12.y + s.y * a12.x);
-                    double2 n12 = make_double2(-(s.x * a11.x + s.y * a11.y) + c * a12.x, -(s.x * a11.y - s.y * a11.x) + c * a12.y);
-                    double2 n21 = make_double2(c * a21.x + s.x * a22.x - s.y * a22.y, c * a21.y + s.x * a22.y + s.y * a22.x);
-                    double2 n22 = make_double2(-(s.x * a21.x + s.y * a21.y) + c * a22.x, -(s.x * a21.y - s.y * a21.x) + c * a22.y);
-                    a11 = n11; a12 = n12; a21 = n21; a22 = n22;
-                }
-                if (f1 == 1) {  // rows: [p,:] = c [p,:] + conj(s) [q,:]; [q,:] = -s [p,:] + c [q,:]
-                    const double c = rc[t1];
-                    const double2 s = rs[t1];
-                    double2 n11 = make_double2(c * a11.x + s.x * a21.x + s.y * a21.y, c * a11.y + s.x * a21.y - s.y * a21.x);
-                    double2 n21 = make_double2(-(s.x * a11.x - s.y * a11.y) + c * a21.x, -(s.x * a11.y + s.y * a11.x) + c * a21.y);
-                    double2 n12 = make_double2(c * a12.x + s.x * a22.x + s.y * a22.y, c * a12.y + s.x * a22.y - s.y * a22.x);
-                    double2 n22 = make_double2(-(s.x * a12.x - s.y * a12.y) + c * a22.x, -(s.x * a12.y + s.y * a12.x) + c * a22.y);
-                    a11 = n11; a12 = n12; a21 = n21; a22 = n22;
-                }
-                if (t1 == t2) {  // the rotated pair: exact zero off-diagonal, real diagonal
-                    a12 = make_double2(0.0, 0.0);
-                    a21 = make_double2(0.0, 0.0);
-                    a11.y = 0.0;
-                    a22.y = 0.0;
-                }
-                A[p1 + p2 * ne] = a11;
-                A[p1 + q2 * ne] = a12;
-                A[q1 + p2 * ne] = a21;
-                A[q1 + q2 * ne] = a22;
-            } while (false);
-            for (int e = tid, it = 0; e < nv; e += EIGH_NT, ++it) {  // eigenvectors: column rotations only
-                const int t = it == 0 ? vt0 : vt1, k = it == 0 ? vk0 : vk1;
-                if (rf[t] != 1) continue;
-                double c = rc[t];
-                double2 sg = rs[t];
-                int p = rp[t], q = rq[t];
-                double2 vkp = V[k + p * ne], vkq = V[k + q * ne];
-                V[k + p * ne] = make_double2(c * vkp.x + sg.x * vkq.x - sg.y * vkq.y,
-                                             c * vkp.y + sg.x * vkq.y + sg.y * vkq.x);
-                V[k + q * ne] = make_double2(-(sg.x * vkp.x + sg.y * vkp.y) + c * vkq.x,
-                                             -(sg.x * vkp.y - sg.y * vkp.x) + c * vkq.y);
-            }
-            __syncthreads();
-        }
-        // quadratic convergence: rotations of relative size < 1e-9 leave
-        // off-diagonal terms below 1e-18 relative, so no further sweep is needed
-        const bool done = nrot == 0 || __longlong_as_double((long long)maxratio_bits) < 1e-9;
-        if (done || sweep == 39) {
-            if (tid == 0 && w.sweeps_out) {
-                atomicAdd(w.sweeps_out, sweep + 1);
-                atomicMax(w.sweeps_out + 2, sweep + 1);  // maximum over the launch
-            }
-            break;
-        }
-        __syncthreads();
+    const int sweeps = jacobi_lds(A, V, ne, ne, JacobiScratch{rs, rc, rp, rq, rf}, &nrot, &maxratio_bits);
+    if (tid == 0 && w.sweeps_out) {
+        atomicAdd(w.sweeps_out, sweeps);
+        atomicMax(w.sweeps_out + 2, sweeps);  // maximum over the launch
     }
     // stable ascending ranks over the first n diagonal entries
     __shared__ double wsh[EIGH_MAXN];
@@ -571,6 +593,454 @@ void launch_eigh(const EighWork *dwork, int nwork, int maxn, hipStream_t st) {
     }();
     (void)attr_set;
     hipLaunchKernelGGL(eigh_kernel, dim3(nwork), dim3(EIGH_NT), sh, st, dwork);
+    BH_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace bh
+
+namespace bh {
+
+// ---------------------------------------------------------------------------
+// Dual Gram for Rayleigh-Ritz: one pass over the rows of Z, BZ and AZ gives
+// G1 = Z^H BZ and G2 = Z^H AZ (lower triangles).  The n x n outputs are
+// tiled by 3 x 3 blocks on an NBK x NBK block grid; each thread owns one lower
+// block of both products (18 complex accumulators), so only the blocks that
+// intersect the n columns do work.  RG row groups split the staged rows and are
+// reduced through LDS at the end; grid = (row chunks, work items).
+// ---------------------------------------------------------------------------
+template <int NBK>
+struct Gram2Shape {
+    static constexpr int NLB = NBK * (NBK + 1) / 2;  // lower 3x3 blocks
+    static constexpr int RG = 256 / NLB;             // row groups per workgroup
+    static constexpr int COLS = 3 * NBK;
+    static constexpr int SR = 32;                    // rows staged per pass
+};
+
+template <int NBK>
+size_t gram2_shared_bytes() {
+    using S = Gram2Shape<NBK>;
+    size_t stage = 3ull * S::SR * S::COLS * sizeof(double2);
+    size_t red = (size_t)(S::RG > 1 ? S::RG - 1 : 0) * S::NLB * 36 * sizeof(double);
+    return stage > red ? stage : red;
+}
+
+template <typename T, int NBK>
+__global__ void __launch_bounds__(256) gram2_kernel(const Gram2Work<T> *__restrict__ work, long rows_per_chunk,
+                                                    long nrows) {
+    using S = Gram2Shape<NBK>;
+    const Gram2Work<T> &w = work[blockIdx.y];
+    const int n = w.n;
+    if (n <= 0) return;
+    extern __shared__ double2 g2s[];
+    double2 *Zs = g2s;                       // [SR][COLS]
+    double2 *Bs = Zs + S::SR * S::COLS;
+    double2 *As = Bs + S::SR * S::COLS;
+    const bool bz_alias = w.BZ == w.Z;
+    const int tid = threadIdx.x;
+    const int g = tid / S::NLB, lb = tid - g * S::NLB;
+    int bi = 0;
+    while ((bi + 1) * (bi + 2) / 2 <= lb) ++bi;
+    const int bj = lb - bi * (bi + 1) / 2;
+    const bool active = g < S::RG && 3 * bi < n;
+    double a1r[3][3] = {}, a1i[3][3] = {}, a2r[3][3] = {}, a2i[3][3] = {};
+    const long r0 = (long)blockIdx.x * rows_per_chunk;
+    const long r1 = r0 + rows_per_chunk < nrows ? r0 + rows_per_chunk : nrows;
+    for (long rb = r0; rb < r1; rb += S::SR) {
+        const int nr = (int)(r1 - rb < S::SR ? r1 - rb : S::SR);
+        for (int e = tid; e < S::SR * S::COLS; e += 256) {
+            const int c = e / S::SR, r = e - c * S::SR;
+            double2 z = make_double2(0.0, 0.0), b = z, a = z;
+            if (r < nr && c < n) {
+                const long o = (long)c * w.stride + rb + r;
+                cplx<T> zv = w.Z[o], av = w.AZ[o];
+                z = make_double2((double)zv.x, (double)zv.y);
+                a = make_double2((double)av.x, (double)av.y);
+                if (!bz_alias) {
+                    cplx<T> bv = w.BZ[o];
+                    b = make_double2((double)bv.x, (double)bv.y);
+                }
+            }
+            Zs[r * S::COLS + c] = z;
+            As[r * S::COLS + c] = a;
+            if (!bz_alias) Bs[r * S::COLS + c] = b;
+        }
+        __syncthreads();
+        if (active) {
+            const double2 *Bsrc = bz_alias ? Zs : Bs;
+            for (int r = g; r < nr; r += S::RG) {
+                double2 zi[3], bjv[3], ajv[3];
+#pragma unroll
+                for (int u = 0; u < 3; ++u) {
+                    zi[u] = Zs[r * S::COLS + 3 * bi + u];
+                    bjv[u] = Bsrc[r * S::COLS + 3 * bj + u];
+                    ajv[u] = As[r * S::COLS + 3 * bj + u];
+                }
+#pragma unroll
+                for (int u = 0; u < 3; ++u)
+#pragma unroll
+                    for (int v = 0; v < 3; ++v) {  // conj(z_i) * b_j
+                        a1r[u][v] += zi[u].x * bjv[v].x + zi[u].y * bjv[v].y;
+                        a1i[u][v] += zi[u].x * bjv[v].y - zi[u].y * bjv[v].x;
+                        a2r[u][v] += zi[u].x * ajv[v].x + zi[u].y * ajv[v].y;
+                        a2i[u][v] += zi[u].x * ajv[v].y - zi[u].y * ajv[v].x;
+                    }
+            }
+        }
+        __syncthreads();
+    }
+    // reduce the row groups through LDS (the staging area is free now)
+    double *red = reinterpret_cast<double *>(g2s);
+    if (S::RG > 1) {
+        if (active && g > 0) {
+            double *o = red + ((size_t)(g - 1) * S::NLB + lb) * 36;
+#pragma unroll
+            for (int u = 0; u < 3; ++u)
+#pragma unroll
+                for (int v = 0; v < 3; ++v) {
+                    o[4 * (3 * u + v)] = a1r[u][v];
+                    o[4 * (3 * u + v) + 1] = a1i[u][v];
+                    o[4 * (3 * u + v) + 2] = a2r[u][v];
+                    o[4 * (3 * u + v) + 3] = a2i[u][v];
+                }
+        }
+        __syncthreads();
+        if (active && g == 0) {
+            for (int gg = 1; gg < S::RG; ++gg) {
+                const double *o = red + ((size_t)(gg - 1) * S::NLB + lb) * 36;
+#pragma unroll
+                for (int u = 0; u < 3; ++u)
+#pragma unroll
+                    for (int v = 0; v < 3; ++v) {
+                        a1r[u][v] += o[4 * (3 * u + v)];
+                        a1i[u][v] += o[4 * (3 * u + v) + 1];
+                        a2r[u][v] += o[4 * (3 * u + v) + 2];
+                        a2i[u][v] += o[4 * (3 * u + v) + 3];
+                    }
+            }
+        }
+    }
+    if (!active || g != 0) return;
+    double *p1 = w.partial1 + 2 * (size_t)blockIdx.x * w.ld * w.ld;
+    double *p2 = w.partial2 + 2 * (size_t)blockIdx.x * w.ld * w.ld;
+#pragma unroll
+    for (int u = 0; u < 3; ++u)
+#pragma unroll
+        for (int v = 0; v < 3; ++v) {
+            const int i = 3 * bi + u, j = 3 * bj + v;
+            if (i < n && j < n) {
+                const size_t o = 2 * ((size_t)i * w.ld + j);
+                p1[o] = a1r[u][v];
+                p1[o + 1] = a1i[u][v];
+                p2[o] = a2r[u][v];
+                p2[o + 1] = a2i[u][v];
+            }
+        }
+}
+
+template <typename T, int NBK>
+void launch_gram2_sized(const Gram2Work<T> *dwork, int nwork, int nchunks, long rows, hipStream_t st) {
+    using S = Gram2Shape<NBK>;
+    const size_t sh = gram2_shared_bytes<NBK>();
+    static const bool attr_set = [sh] {
+        BH_HIP_CHECK(hipFuncSetAttribute((const void *)gram2_kernel<T, NBK>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+        return true;
+    }();
+    (void)attr_set;
+    long per = (rows + nchunks - 1) / nchunks;
+    per = (per + S::SR - 1) / S::SR * S::SR;
+    hipLaunchKernelGGL((gram2_kernel<T, NBK>), dim3(nchunks, nwork), dim3(256), sh, st, dwork, per, rows);
+    BH_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+void launch_gram2(const Gram2Work<T> *dwork, int nwork, int nchunks, long rows, int maxn, hipStream_t st) {
+    if (nwork <= 0 || maxn <= 0) return;
+    if (maxn <= 36) launch_gram2_sized<T, 12>(dwork, nwork, nchunks, rows, st);
+    else if (maxn <= GRAM_MAXC) launch_gram2_sized<T, 16>(dwork, nwork, nchunks, rows, st);
+    else throw std::runtime_error("dual Gram wider than 48 columns");
+}
+
+template void launch_gram2<double>(const Gram2Work<double> *, int, int, long, int, hipStream_t);
+template void launch_gram2<float>(const Gram2Work<float> *, int, int, long, int, hipStream_t);
+
+// ---------------------------------------------------------------------------
+// Rayleigh-Ritz on the dual Gram, one 1024-thread workgroup per instance, FP64.
+//  1. G1 (Hermitian, from the lower partial sums), scaled by D = diag(G1)^-1/2
+//     (normalization.rs:281-345 normalises each column to unit B-norm first).
+//  2. T: Cholesky G1s = L L^H with every pivot > 1e-8 -> T = D L^-H (upper
+//     triangular, rank n); else SVQB: G1s = Q diag(l) Q^H, keep l >= 1e-12 lmax
+//     and l > 0 in descending order, T = D Q_k diag(l_k)^-1/2
+//     (normalization.rs:355-531).  Both give B-orthonormal Z T spanning the
+//     kept directions.
+//  3. M = T^H G2 T (= Q^H A Q of eigensolver.rs:1232-1316), symmetrised.
+//  4. M = Y diag(theta) Y^H by Jacobi, ascending (dense.rs:71-134).
+//  5. C = T Y[:, 0:cx+cw], cx = min(rank, nsplit), cw = min(2 nsplit, rank) - cx:
+//     the new X = Z C[:, :cx] and W = Z C[:, cx:] (eigensolver.rs:1341-1637).
+// sweeps_out (optional): [0] SVQB Jacobi sweeps, [1] Cholesky count,
+// [2] max SVQB sweeps, [3] Rayleigh-Ritz sweeps, [5] max Rayleigh-Ritz sweeps.
+// ---------------------------------------------------------------------------
+size_t rr_shared_bytes(int maxn) {
+    size_t ne = maxn + (maxn & 1), np = ne / 2;
+    // A, V, T (complex ne x ne), D (ne), wsh (ne), rc (np), rs (complex np), rp/rq/rf, rank_of (int)
+    return sizeof(double) * (6 * ne * ne + 2 * ne + np + 2 * np) + sizeof(int) * (3 * np + ne);
+}
+
+__global__ void __launch_bounds__(1024) rr_kernel(const RRWork *__restrict__ work) {
+    const RRWork &w = work[blockIdx.x];
+    const int n = w.n;
+    const int tid = threadIdx.x;
+    if (n <= 0) {
+        if (tid == 0) { w.cnt[0] = 0; w.cnt[1] = 0; w.cnt[2] = 0; }
+        return;
+    }
+    const int ne = n + (n & 1);
+    const int np = ne / 2;
+    extern __shared__ double rsm[];
+    double2 *A = reinterpret_cast<double2 *>(rsm);  // col-major [i + j*ne]
+    double2 *V = A + ne * ne;
+    double2 *Tm = V + ne * ne;
+    double2 *rs = Tm + ne * ne;
+    double *D = reinterpret_cast<double *>(rs + np);
+    double *wsh = D + ne;
+    double *rc = wsh + ne;
+    int *rp = reinterpret_cast<int *>(rc + np);
+    int *rq = rp + np;
+    int *rf = rq + np;
+    int *rank_of = rf + np;
+    __shared__ int nrot, rank, chol_ok;
+    __shared__ unsigned long long maxratio_bits;
+    __shared__ double lmax;
+    const int ld = w.ld;
+    auto load = [&](const double *partial) {  // Hermitian n x n from lower partial sums, zero padding
+        for (int e = tid; e < ne * ne; e += EIGH_NT) {
+            const int i = e % ne, j = e / ne;
+            double2 v = make_double2(0.0, 0.0);
+            if (i < n && j < n) {
+                const int a = i >= j ? i : j, b = i >= j ? j : i;
+                double re = 0.0, im = 0.0;
+                for (int c = 0; c < w.nchunks; ++c) {
+                    const double *p = partial + 2 * ((size_t)c * ld * ld + (size_t)a * ld + b);
+                    re += p[0];
+                    im += p[1];
+                }
+                if (i == j) im = 0.0;
+                v = i >= j ? make_double2(re, im) : make_double2(re, -im);
+            }
+            A[e] = v;
+        }
+    };
+    // 1. scaled G1
+    load(w.partial1);
+    __syncthreads();
+    for (int i = tid; i < ne; i += EIGH_NT) {
+        double nrm = i < n ? sqrt(fmax(A[i + i * ne].x, 0.0)) : 0.0;
+        D[i] = nrm > 1e-30 ? 1.0 / nrm : 1.0;
+    }
+    __syncthreads();
+    for (int e = tid; e < ne * ne; e += EIGH_NT) {
+        const int i = e % ne, j = e / ne;
+        const double s = D[i] * D[j];
+        A[e].x *= s;
+        A[e].y *= s;
+        Tm[e] = make_double2(0.0, 0.0);
+    }
+    // 2. Cholesky into V (left-looking, one column per step)
+    double2 *L = V;
+    if (tid == 0) chol_ok = 1;
+    __syncthreads();
+    for (int e = tid; e < ne * ne; e += EIGH_NT) L[e] = A[e];
+    __syncthreads();
+    for (int j = 0; j < n; ++j) {
+        if (tid == 0) {
+            double d = L[j + j * ne].x;
+            for (int k = 0; k < j; ++k) {
+                const double2 l = L[j + k * ne];
+                d -= l.x * l.x + l.y * l.y;
+            }
+            if (!(d > 1e-8)) chol_ok = 0;  // relative pivot: columns have unit B-norm
+            L[j + j * ne] = make_double2(d > 0.0 ? sqrt(d) : 1.0, 0.0);
+        }
+        __syncthreads();
+        if (!chol_ok) break;
+        const double ljj = L[j + j * ne].x;
+        for (int i = j + 1 + tid; i < n; i += EIGH_NT) {
+            double2 a = L[i + j * ne];
+            for (int k = 0; k < j; ++k) {  // a -= L[i,k] conj(L[j,k])
+                const double2 x = L[i + k * ne], y = L[j + k * ne];
+                a.x -= x.x * y.x + x.y * y.y;
+                a.y -= x.y * y.x - x.x * y.y;
+            }
+            L[i + j * ne] = make_double2(a.x / ljj, a.y / ljj);
+        }
+        __syncthreads();
+    }
+    if (chol_ok) {
+        // M = L^-1 by forward substitution (one column per thread) into A, then T = D M^H
+        for (int c = tid; c < n; c += EIGH_NT) {
+            for (int i = 0; i < n; ++i) A[i + c * ne] = make_double2(0.0, 0.0);
+            for (int i = c; i < n; ++i) {
+                double2 acc = make_double2(i == c ? 1.0 : 0.0, 0.0);
+                for (int k = c; k < i; ++k) {
+                    const double2 l = L[i + k * ne], m = A[k + c * ne];
+                    acc.x -= l.x * m.x - l.y * m.y;
+                    acc.y -= l.x * m.y + l.y * m.x;
+                }
+                const double lii = L[i + i * ne].x;
+                A[i + c * ne] = make_double2(acc.x / lii, acc.y / lii);
+            }
+        }
+        __syncthreads();
+        for (int e = tid; e < n * n; e += EIGH_NT) {
+            const int r = e % n, c = e / n;  // T[r][c] = D[r] conj(M[c][r]), zero below the diagonal
+            const double2 m = r <= c ? A[c + r * ne] : make_double2(0.0, 0.0);
+            Tm[r + c * ne] = make_double2(D[r] * m.x, -D[r] * m.y);
+        }
+        if (tid == 0) {
+            rank = n;
+            if (w.sweeps_out) atomicAdd(w.sweeps_out + 1, 1);
+        }
+    } else {
+        // SVQB: eigendecomposition of the scaled G1 (A still holds it)
+        for (int e = tid; e < ne * ne; e += EIGH_NT) {
+            const int i = e % ne, j = e / ne;
+            V[e] = make_double2(i == j ? 1.0 : 0.0, 0.0);
+        }
+        __syncthreads();
+        const int sw = jacobi_lds(A, V, ne, ne, JacobiScratch{rs, rc, rp, rq, rf}, &nrot, &maxratio_bits);
+        for (int i = tid; i < n; i += EIGH_NT) wsh[i] = A[i + i * ne].x;
+        __syncthreads();
+        for (int i = tid; i < n; i += EIGH_NT) {
+            const double wi = wsh[i];
+            int rk = 0;
+            for (int j = 0; j < n; ++j) rk += (wsh[j] < wi || (wsh[j] == wi && j < i)) ? 1 : 0;
+            rank_of[i] = rk;
+        }
+        if (tid == 0) {
+            double m = 0.0;
+            for (int i = 0; i < n; ++i) m = fmax(m, wsh[i]);
+            lmax = m;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            const double thr = 1e-12 * lmax;
+            int r = 0;
+            for (int i = 0; i < n; ++i)
+                if (wsh[i] >= thr && wsh[i] > 0.0) ++r;
+            rank = r;
+            if (w.sweeps_out) {
+                atomicAdd(w.sweeps_out, sw);
+                atomicMax(w.sweeps_out + 2, sw);
+            }
+        }
+        __syncthreads();
+        // kept directions are the first `rank` of the descending order
+        for (int e = tid; e < n * n; e += EIGH_NT) {
+            const int row = e % n, i = e / n;
+            const int pos = n - 1 - rank_of[i];
+            if (pos >= rank) continue;
+            const double sc = D[row] / sqrt(fmax(wsh[i], 1e-30));
+            const double2 v = V[row + i * ne];
+            Tm[row + pos * ne] = make_double2(v.x * sc, v.y * sc);
+        }
+    }
+    __syncthreads();
+    const int r = rank;
+    const int re = r + (r & 1);
+    // 3. G2 into A, X = G2 T into V, M = T^H X into A (zero padded to re)
+    load(w.partial2);
+    __syncthreads();
+    for (int e = tid; e < n * r; e += EIGH_NT) {
+        const int i = e % n, c = e / n;
+        double xr = 0.0, xi = 0.0;
+        for (int k = 0; k < n; ++k) {
+            const double2 a = A[i + k * ne], t = Tm[k + c * ne];
+            xr += a.x * t.x - a.y * t.y;
+            xi += a.x * t.y + a.y * t.x;
+        }
+        V[i + c * ne] = make_double2(xr, xi);
+    }
+    __syncthreads();
+    for (int e = tid; e < ne * ne; e += EIGH_NT) {
+        const int a = e % ne, c = e / ne;
+        double mr = 0.0, mi = 0.0;
+        if (a < r && c < r) {
+            for (int k = 0; k < n; ++k) {  // conj(T[k][a]) X[k][c]
+                const double2 t = Tm[k + a * ne], x = V[k + c * ne];
+                mr += t.x * x.x + t.y * x.y;
+                mi += t.x * x.y - t.y * x.x;
+            }
+        }
+        A[e] = make_double2(mr, mi);
+    }
+    __syncthreads();
+    for (int e = tid; e < r * r; e += EIGH_NT) {  // Hermitian part, each pair by one thread
+        const int a = e % r, c = e / r;
+        if (a > c) continue;
+        if (a == c) {
+            A[a + a * ne].y = 0.0;
+            continue;
+        }
+        const double2 x = A[a + c * ne], y = A[c + a * ne];
+        const double2 h = make_double2(0.5 * (x.x + y.x), 0.5 * (x.y - y.y));
+        A[a + c * ne] = h;
+        A[c + a * ne] = make_double2(h.x, -h.y);
+    }
+    for (int e = tid; e < ne * ne; e += EIGH_NT) {
+        const int i = e % ne, j = e / ne;
+        V[e] = make_double2(i == j && i < re ? 1.0 : 0.0, 0.0);
+    }
+    __syncthreads();
+    // 4. M = Y diag(theta) Y^H
+    const int sw = jacobi_lds(A, V, ne, re, JacobiScratch{rs, rc, rp, rq, rf}, &nrot, &maxratio_bits);
+    if (tid == 0 && w.sweeps_out) {
+        atomicAdd(w.sweeps_out + 3, sw);
+        atomicMax(w.sweeps_out + 5, sw);
+    }
+    for (int i = tid; i < r; i += EIGH_NT) wsh[i] = A[i + i * ne].x;
+    __syncthreads();
+    for (int i = tid; i < r; i += EIGH_NT) {
+        const double wi = wsh[i];
+        int rk = 0;
+        for (int j = 0; j < r; ++j) rk += (wsh[j] < wi || (wsh[j] == wi && j < i)) ? 1 : 0;
+        rank_of[i] = rk;
+        w.lam[rk] = wi;
+    }
+    __syncthreads();
+    // 5. C = T Y for the first cx + cw Ritz vectors
+    const int cx = r < w.nsplit ? r : w.nsplit;
+    const int cw = (2 * w.nsplit < r ? 2 * w.nsplit : r) - cx;
+    for (int e = tid; e < n * r; e += EIGH_NT) {
+        const int k = e % n, i = e / n;
+        const int col = rank_of[i];
+        if (col >= cx + cw) continue;
+        double cr = 0.0, ci = 0.0;
+        for (int j = 0; j < r; ++j) {
+            const double2 t = Tm[k + j * ne], y = V[j + i * ne];
+            cr += t.x * y.x - t.y * y.y;
+            ci += t.x * y.y + t.y * y.x;
+        }
+        const size_t o = 2 * ((size_t)k + (size_t)col * w.ldc);
+        w.C[o] = cr;
+        w.C[o + 1] = ci;
+    }
+    if (tid == 0) {
+        w.cnt[0] = cx;
+        w.cnt[1] = cw;
+        w.cnt[2] = cx + cw;
+    }
+}
+
+void launch_rr(const RRWork *dwork, int nwork, int maxn, hipStream_t st) {
+    if (nwork <= 0) return;
+    if (maxn > GRAM_MAXC) throw std::runtime_error("Rayleigh-Ritz dimension above 48");
+    static const bool attr_set = [] {
+        BH_HIP_CHECK(hipFuncSetAttribute((const void *)rr_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         (int)rr_shared_bytes(GRAM_MAXC)));
+        return true;
+    }();
+    (void)attr_set;
+    hipLaunchKernelGGL(rr_kernel, dim3(nwork), dim3(EIGH_NT), rr_shared_bytes(std::max(maxn, 1)), st, dwork);
     BH_HIP_CHECK(hipGetLastError());
 }
 

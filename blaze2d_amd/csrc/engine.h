@@ -13,6 +13,8 @@
 
 namespace bh {
 
+enum TimedKernel { TIMED_NONE = 0, TIMED_THETA = 1, TIMED_GRAM2 = 2 };
+
 struct JobSpec {
     GeometrySpec geom;
     int nx = 0, ny = 0;
@@ -41,9 +43,12 @@ struct RunStats {
     long iterations = 0;          // total LOBPCG iterations over all solves
     long solves = 0;              // k-point solves (excluding reused closing Gamma)
     double seconds = 0;
-    double theta_ms[2] = {0, 0};  // Theta-kernel time per polarisation [TM, TE] (events, profile mode)
-    long theta_launches[2] = {0, 0};
-    long theta_columns[2] = {0, 0};  // columns processed by the timed launches
+    // launches of the kernel class EngineOptions::timed_kernel, timed with
+    // HIP events on the stream they run on, and their algorithmic work
+    double timed_ms = 0;
+    long timed_launches = 0;
+    double timed_bytes = 0;       // algorithmic HBM bytes summed over those launches
+    double timed_flops = 0;       // algorithmic FLOPs summed over those launches
     long eigh_calls[2] = {0, 0};     // [SVQB, Rayleigh-Ritz] problems solved (profile mode)
     long eigh_sweeps[2] = {0, 0};    // Jacobi sweeps summed over those problems
     long eigh_max_sweeps[2] = {0, 0};      // largest sweep count seen in one problem
@@ -58,6 +63,7 @@ struct EngineOptions {
     int profile = 0;        // time operator kernels with events (forces one group)
     int groups = 2;         // slot groups pipelined on separate streams
     int host_threads = 0;   // host threads for per-slot planning (0: hardware threads, at most 16)
+    int timed_kernel = 0;   // TIMED_* class whose launches are timed with events (0: none)
     // Streams owned by the caller.  HIP gives each new stream its own hardware
     // queue until GPU_MAX_HW_QUEUES is reached and then shares queues, which
     // serialises the groups; a caller that creates the group streams early

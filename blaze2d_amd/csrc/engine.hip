@@ -11,13 +11,13 @@
 // and the O(nb^3) transition algebra on small Gram matrices; every O(N)
 // operation runs on the GPU.  One host synchronisation per step.
 //
-// Per-slot device layout (columns of N complex values, storage precision T):
+// Per-slot device layout (columns of N complex values, storage precision T),
+// two copies b = 0, 1 of V, BV, AV; the slot's parity selects the current one
+// and the Ritz update writes the other:
 //   V  [C + 1] : Z = [X (m) | W (w) | P (a)];  column C holds the Gamma mode y0
-//   BV [C + 1] : B V (TM only; TE aliases V)
-//   AV [m0]    : A X
-//   Q, BQ, AQ [C] : SVQB output basis and its B / A images
+//   BV [C + 1] : B Z (TM only; TE aliases V)
+//   AV [C + 1] : A Z (A X and A W from the Ritz update, A P applied fresh)
 //   H  [2 nb]  : subspace history (previous / current converged eigenvectors)
-// W is written at columns [nb, nb + w) by the Ritz update.
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -158,6 +158,18 @@ struct Group {
     bool inflight = false;
     bool has_iter = false;
     bool own_st = false;
+    // event pairs around timed launches of the steps in flight
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> tev;
+    size_t tev_used = 0;
+    // adds the elapsed time of the completed pairs to *ms and recycles them
+    void harvest(double *ms) {
+        for (size_t i = 0; i < tev_used; ++i) {
+            float e = 0.f;
+            BH_HIP_CHECK(hipEventElapsedTime(&e, tev[i].first, tev[i].second));
+            *ms += e;
+        }
+        tev_used = 0;
+    }
 };
 
 // zero-filled device allocation, cleared on the given stream
@@ -185,6 +197,7 @@ struct Slot {
     double y0val = 0;  // Gamma mode value 1 / ||1||_B
     // per-solve state
     int it = 0, m = 0, nW = 0;
+    int par = 0;  // current copy of V / BV / AV
     std::vector<char> soft;
     std::vector<double> lam, lam_prev;
     bool lam_prev_valid = false;
@@ -235,27 +248,19 @@ class Engine : public EngineBase {
         NCHMAX = (int)std::min<long>(64, std::max<long>(1, N / 32));
         if (opt.setup_stream) st = opt.setup_stream;
         else { BH_HIP_CHECK(hipStreamCreate(&st)); own_st = true; }
-        V = dalloc<cplx<T>>((size_t)S * CV * N, st);
-        BV = dalloc<cplx<T>>((size_t)S * CV * N, st);
-        AV = dalloc<cplx<T>>((size_t)S * m0max * N, st);
-        Q = dalloc<cplx<T>>((size_t)S * C * N, st);
-        BQ = dalloc<cplx<T>>((size_t)S * C * N, st);
-        AQ = dalloc<cplx<T>>((size_t)S * C * N, st);
+        V = dalloc<cplx<T>>((size_t)S * 2 * CV * N, st);
+        BV = dalloc<cplx<T>>((size_t)S * 2 * CV * N, st);
+        AV = dalloc<cplx<T>>((size_t)S * 2 * CV * N, st);
         H = dalloc<cplx<T>>((size_t)S * 2 * nbmax * N, st);
         RND = dalloc<cplx<T>>((size_t)NRMAX * N, st);
         EPS = dalloc<T>((size_t)S * N, st);
         TEN = dalloc<T>((size_t)S * 4 * N, st);
         d_lam = dalloc<double>((size_t)S * LD, st);
-        d_svw = dalloc<double>((size_t)S * LD, st);
         d_Y = dalloc<double>((size_t)S * LD * LD * 2, st);
-        d_Yw = dalloc<double>((size_t)S * LD * LD * 2, st);
-        d_T = dalloc<double>((size_t)S * LD * LD * 2, st);
         d_coef = dalloc<double>((size_t)S * LD * LD * 2, st);
-        d_rank = dalloc<int>(S, st);
-        d_cnt = dalloc<int>(2 * (size_t)S, st);
+        d_cnt = dalloc<int>(3 * (size_t)S, st);
         BH_HIP_CHECK(hipHostMalloc((void **)&h_lam, sizeof(double) * S * LD, hipHostMallocDefault));
-        BH_HIP_CHECK(hipHostMalloc((void **)&h_rank, sizeof(int) * S, hipHostMallocDefault));
-        BH_HIP_CHECK(hipHostMalloc((void **)&h_cnt, sizeof(int) * 2 * S, hipHostMallocDefault));
+        BH_HIP_CHECK(hipHostMalloc((void **)&h_cnt, sizeof(int) * 3 * S, hipHostMallocDefault));
         int ng = opt.profile ? 1 : std::max(1, std::min(opt.groups, S));
         slots_per_group = (S + ng - 1) / ng;
         ng = (S + slots_per_group - 1) / slots_per_group;
@@ -270,7 +275,8 @@ class Engine : public EngineBase {
             BH_HIP_CHECK(hipEventCreateWithFlags(&G.ev, hipEventDisableTiming));
             // per slot and step: dielectric tables of a new job plus descriptors / coefficients
             G.stg.init(((size_t)1 << 20) + (size_t)gs * (5 * (size_t)N * sizeof(T) + ((size_t)96 << 10)));
-            G.part = dalloc<double>((size_t)(2048 + 2 * gs) * LD * LD * 2, st);
+            // partial sums: two products (dual Gram) of up to 2048 chunk blocks
+            G.part = dalloc<double>((size_t)2 * (2048 + 2 * gs) * LD * LD * 2, st);
             G.gramrb = dalloc<double>((size_t)gs * LD * LD * 2, st);
             BH_HIP_CHECK(hipHostMalloc((void **)&G.h_gramrb, sizeof(double) * gs * LD * LD * 2, hipHostMallocDefault));
             G.sweeps = dalloc<int>(6, st);
@@ -306,14 +312,16 @@ class Engine : public EngineBase {
             (void)hipFree(G.sweeps);
             (void)hipHostFree(G.h_gramrb);
             G.stg.release();
+            for (auto &e : G.tev) {
+                (void)hipEventDestroy(e.first);
+                (void)hipEventDestroy(e.second);
+            }
             (void)hipEventDestroy(G.ev);
             if (G.own_st) (void)hipStreamDestroy(G.st);
         }
-        for (void *p : std::initializer_list<void *>{V, BV, AV, Q, BQ, AQ, H, RND, EPS, TEN, d_lam, d_svw, d_Y, d_Yw,
-                                                     d_T, d_coef, d_rank, d_cnt, twx, twy})
+        for (void *p : std::initializer_list<void *>{V, BV, AV, H, RND, EPS, TEN, d_lam, d_Y, d_coef, d_cnt, twx, twy})
             if (p) (void)hipFree(p);
         (void)hipHostFree(h_lam);
-        (void)hipHostFree(h_rank);
         (void)hipHostFree(h_cnt);
         if (own_st) (void)hipStreamDestroy(st);
     }
@@ -328,12 +336,12 @@ class Engine : public EngineBase {
     hipStream_t st{};
     bool own_st = false;
     std::unique_ptr<WorkerPool> pool;
-    cplx<T> *V{}, *BV{}, *AV{}, *Q{}, *BQ{}, *AQ{}, *H{}, *RND{};
+    cplx<T> *V{}, *BV{}, *AV{}, *H{}, *RND{};
     T *EPS{}, *TEN{};
-    double *d_lam{}, *d_svw{}, *d_Y{}, *d_Yw{}, *d_T{}, *d_coef{};
-    int *d_rank{}, *d_cnt{};
+    double *d_lam{}, *d_Y{}, *d_coef{};
+    int *d_cnt{};
     double *h_lam{};
-    int *h_rank{}, *h_cnt{};
+    int *h_cnt{};
     cplx<T> *twx{}, *twy{};
     std::vector<Group> groups;
     int slots_per_group = 1;
@@ -344,17 +352,28 @@ class Engine : public EngineBase {
     RunStats *stats_{};
 
     // column addressing
-    cplx<T> *vcol(int s, int j) { return V + ((size_t)s * CV + j) * N; }
-    cplx<T> *bvcol(int s, int j) { return slots[s].pol == 1 ? vcol(s, j) : BV + ((size_t)s * CV + j) * N; }
-    cplx<T> *avcol(int s, int j) { return AV + ((size_t)s * m0max + j) * N; }
-    cplx<T> *qcol(int s, int j) { return Q + ((size_t)s * C + j) * N; }
-    cplx<T> *bqcol(int s, int j) { return slots[s].pol == 1 ? qcol(s, j) : BQ + ((size_t)s * C + j) * N; }
-    cplx<T> *aqcol(int s, int j) { return AQ + ((size_t)s * C + j) * N; }
+    // copy b of slot s; vcol/bvcol/avcol address the current copy
+    cplx<T> *vbuf(int s, int b, int j) { return V + (((size_t)s * 2 + b) * CV + j) * N; }
+    cplx<T> *bvbuf(int s, int b, int j) {
+        return slots[s].pol == 1 ? vbuf(s, b, j) : BV + (((size_t)s * 2 + b) * CV + j) * N;
+    }
+    cplx<T> *avbuf(int s, int b, int j) { return AV + (((size_t)s * 2 + b) * CV + j) * N; }
+    cplx<T> *vcol(int s, int j) { return vbuf(s, slots[s].par, j); }
+    cplx<T> *bvcol(int s, int j) { return bvbuf(s, slots[s].par, j); }
+    cplx<T> *avcol(int s, int j) { return avbuf(s, slots[s].par, j); }
     cplx<T> *hcol(int s, int b, int j) { return H + ((size_t)s * 2 * nbmax + (size_t)b * nbmax + j) * N; }
     cplx<T> *rcol(int j) { return RND + (size_t)j * N; }
     T *eps(int s) { return EPS + (size_t)s * N; }
     T *ten(int s) { return TEN + (size_t)s * 4 * N; }
     double *coef(int s) { return d_coef + (size_t)s * LD * LD * 2; }
+    // algorithmic work of one Theta column: field in + out (+ the 2x2 inverse-eps
+    // tensor for TE); FFT FLOPs 5 N log2 N per transform (2 for TM, 6 for TE)
+    double theta_bytes_per_col(int pol) const {
+        return (double)N * (2.0 * sizeof(cplx<T>) + (pol == 1 ? 4.0 * sizeof(T) : 0.0));
+    }
+    double theta_flops_per_col(int pol, double n_log) const {
+        return pol == 1 ? 30.0 * n_log + 40.0 * N : 10.0 * n_log + 2.0 * N;
+    }
 
     // --- per-step plans -----------------------------------------------------
     struct GramItem { int slot; std::vector<const cplx<T> *> a, b; bool weighted; int na, nb; const int *dev_na = nullptr, *dev_nb = nullptr; };
@@ -479,7 +498,7 @@ template <typename T>
 void Engine<T>::absorb(int s) {
     Slot &sl = slots[s];
     if (sl.it == 0) return;
-    int cx = h_cnt[2 * s], cw = h_cnt[2 * s + 1];
+    int cx = h_cnt[3 * s], cw = h_cnt[3 * s + 1];
     sl.lam_prev = sl.lam;
     sl.lam_prev_valid = sl.it >= 2;
     if (cx > 0) {
@@ -836,6 +855,7 @@ void Engine<T>::run(const std::vector<JobSpec> &jobs, std::vector<JobResult> &ou
         }
     }
     for (auto &G : groups) BH_HIP_CHECK(hipStreamSynchronize(G.st));
+    for (auto &G : groups) G.harvest(&stats.timed_ms);
     stats.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
@@ -843,6 +863,7 @@ template <typename T>
 void Engine<T>::execute(StepPlan &plan, Group &G) {
     Staging &stg = G.stg;
     const hipStream_t st = G.st;
+    G.harvest(&stats_->timed_ms);  // this group's previous step has completed
     double *const d_part = G.part;
     double *const d_gramrb = G.gramrb;
     int *const d_sweeps = G.sweeps;
@@ -920,7 +941,7 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
         w.ldc = kd;
         w.out = vcol(s, 0);
         w.out_stride = N;
-        if (sl.pol == 0) { w.out2 = BV + ((size_t)s * CV) * N; w.w2 = eps(s); }
+        if (sl.pol == 0) { w.out2 = bvcol(s, 0); w.w2 = eps(s); }
         lc_x.push_back(w);
         maxq_x = std::max(maxq_x, sl.m);
         OpWork<T> o{};
@@ -946,16 +967,18 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
     }
 
     // ---- iteration for ITER slots ----
-    std::vector<LincombWork<T>> lc_wmove, lc_mass;
+    std::vector<LincombWork<T>> lc_mass;
     std::vector<ResidualWork<T>> res;
     std::vector<GramWork<T>> g_defR, g_defP;
     std::vector<LincombWork<T>> lc_defR, lc_defP;
     std::vector<int> slot_defR, slot_defP;
     std::vector<OpWork<T>> op_pre[2];  // [0] FD, [1] TP
-    std::vector<GramWork<T>> g_svqb, g_rr;
-    std::vector<EighWork> e_svqb, e_rr;
-    std::vector<LincombWork<T>> lc_q, lc_ritz;
-    std::vector<OpWork<T>> op_q[2];
+    std::vector<OpWork<T>> op_p[2];    // Theta on the new directions [TM, TE]
+    long theta_cols[2] = {0, 0};
+    std::vector<Gram2Work<T>> g_dual;
+    std::vector<RRWork> e_rr2;
+    std::vector<LincombWork<T>> lc_ritz;
+    int max_ritz = 0;
     std::vector<unsigned char> maskbuf;
     int max_a = 0, max_p = 0, max_d = 0;
     struct DefMask { int slot; int nd; int a; std::vector<unsigned char> m; };
@@ -972,24 +995,8 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
         if (p > C) throw std::runtime_error("subspace wider than the engine allows");
         max_a = std::max(max_a, a);
         max_p = std::max(max_p, p);
-        // W sits at columns [nb, nb + nW); move it next to X when m != nb
-        if (w > 0 && m != sl.nb) {
-            double *cd;
-            double *ch = stg.alloc<double>((size_t)2 * w * w, &cd);
-            std::memset(ch, 0, sizeof(double) * 2 * w * w);
-            for (int i = 0; i < w; ++i) ch[2 * (i + (size_t)i * w)] = 1.0;
-            LincombWork<T> lw{};
-            lw.A = vcol(s, sl.nb);
-            lw.a_stride = N;
-            lw.na = w;
-            lw.nq = w;
-            lw.C = cd;
-            lw.ldc = w;
-            lw.out = vcol(s, m);
-            lw.out_stride = N;
-            if (sl.pol == 0) { lw.out2 = BV + ((size_t)s * CV + m) * N; lw.w2 = eps(s); }
-            lc_wmove.push_back(lw);
-        }
+        // X at [0, m), W at [m, m + nW) as the last Ritz update wrote them; the
+        // first w of W are used and residuals overwrite the rest
         // residuals into P slots [m + w, m + w + a)
         int *act_d;
         int *act_h = stg.alloc<int>(std::max(a, 1), &act_d);
@@ -1045,8 +1052,8 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
             if (sl.pol == 0) {
                 LincombWork<T> lb = lw;
                 lb.a_ptrs = bd_d;
-                lb.base = BV + ((size_t)s * CV + m + w) * N;
-                lb.out = BV + ((size_t)s * CV + m + w) * N;
+                lb.base = bvcol(s, m + w);
+                lb.out = bvcol(s, m + w);
                 lc_defP.push_back(lb);
                 slot_defP.push_back(-1);  // marks the B-image item (shares coefficients)
             }
@@ -1072,7 +1079,7 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
             lm.ldc = a;
             lm.out = vcol(s, m + w);
             lm.out_stride = N;
-            lm.out2 = BV + ((size_t)s * CV + m + w) * N;
+            lm.out2 = bvcol(s, m + w);
             lm.w2 = eps(s);
             lc_mass.push_back(lm);
         }
@@ -1087,108 +1094,64 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
             o.in_stride = N;
             o.out_stride = N;
             o.count = a;
-            if (sl.precond == 1 && sl.pol == 0) o.bout = BV + ((size_t)s * CV + m + w) * N;
+            if (sl.precond == 1 && sl.pol == 0) o.bout = bvcol(s, m + w);
             op_pre[sl.precond - 1].push_back(o);
         }
-        // SVQB Gram Z^H B Z over p columns
-        GramWork<T> gs{};
-        gs.A = vcol(s, 0);
-        gs.a_stride = N;
-        gs.B = bvcol(s, 0);
-        gs.b_stride = N;
-        gs.na = p;
-        gs.nb = p;
-        gs.ld = LD;
-        gs.lower_only = 1;
-        g_svqb.push_back(gs);
-        EighWork es{};
-        es.n = p;
-        es.ld = LD;
-        es.mode = 2;  // Cholesky QR, SVQB eigendecomposition when ill-conditioned
-        es.w = d_svw + (size_t)s * LD;
-        es.V = d_T + (size_t)s * LD * LD * 2;
-        es.rank_out = d_rank + s;
-        es.ldv = LD;
-        if (eopt.profile) es.sweeps_out = d_sweeps;
-        e_svqb.push_back(es);
-        // Q = Z T, BQ = BZ T
-        LincombWork<T> lq{};
-        lq.A = vcol(s, 0);
-        lq.a_stride = N;
-        lq.na = p;
-        lq.dev_nq = d_rank + s;
-        lq.C = d_T + (size_t)s * LD * LD * 2;
-        lq.ldc = LD;
-        lq.out = qcol(s, 0);
-        lq.out_stride = N;
-        lc_q.push_back(lq);
-        if (sl.pol == 0) {
-            LincombWork<T> lb = lq;
-            lb.A = BV + ((size_t)s * CV) * N;
-            lb.out = BQ + ((size_t)s * C) * N;
-            lc_q.push_back(lb);
+        // A P for the new directions; A X and A W come from the last Ritz update
+        if (a > 0) {
+            OpWork<T> o{};
+            o.in = vcol(s, m + w);
+            o.out = avcol(s, m + w);
+            o.eps = eps(s);
+            o.tensor = ten(s);
+            o.params = dparams + s;
+            o.in_stride = N;
+            o.out_stride = N;
+            o.count = a;
+            op_p[sl.pol].push_back(o);
+            theta_cols[sl.pol] += a;
         }
-        OpWork<T> oq{};
-        oq.in = qcol(s, 0);
-        oq.out = aqcol(s, 0);
-        oq.eps = eps(s);
-        oq.tensor = ten(s);
-        oq.params = dparams + s;
-        oq.dev_count = d_rank + s;
-        oq.in_stride = N;
-        oq.out_stride = N;
-        op_q[sl.pol].push_back(oq);
-        GramWork<T> gr{};
-        gr.A = qcol(s, 0);
-        gr.a_stride = N;
-        gr.B = aqcol(s, 0);
-        gr.b_stride = N;
-        gr.dev_na = d_rank + s;
-        gr.dev_nb = d_rank + s;
-        gr.ld = LD;
-        gr.lower_only = 1;
-        g_rr.push_back(gr);
-        EighWork er{};
-        er.dev_n = d_rank + s;
-        er.ld = LD;
-        er.mode = 0;
-        er.w = d_lam + (size_t)s * LD;
-        er.V = d_Y + (size_t)s * LD * LD * 2;
-        er.Vw = d_Yw + (size_t)s * LD * LD * 2;
-        er.rank_out = nullptr;
-        er.cnt_x = d_cnt + 2 * s;
-        er.cnt_w = d_cnt + 2 * s + 1;
-        er.ldv = LD;
-        er.nsplit = sl.nb;
-        if (eopt.profile) er.sweeps_out = d_sweeps + 3;
-        e_rr.push_back(er);
-        // Ritz update: X, BX, AX = (Q, BQ, AQ) Y1 ; W = Q Y2 (+ BW = eps W)
+        // both Rayleigh-Ritz Grams Z^H B Z and Z^H A Z in one pass
+        Gram2Work<T> g2{};
+        g2.Z = vcol(s, 0);
+        g2.BZ = bvcol(s, 0);
+        g2.AZ = avcol(s, 0);
+        g2.stride = N;
+        g2.n = p;
+        g2.ld = LD;
+        g_dual.push_back(g2);
+        RRWork rw2{};
+        rw2.lam = d_lam + (size_t)s * LD;
+        rw2.C = d_Y + (size_t)s * LD * LD * 2;
+        rw2.cnt = d_cnt + 3 * s;
+        rw2.n = p;
+        rw2.ld = LD;
+        rw2.ldc = LD;
+        rw2.nsplit = sl.nb;
+        if (eopt.profile) rw2.sweeps_out = d_sweeps;
+        e_rr2.push_back(rw2);
+        // Ritz update into the other copy: [X' | W'] = Z C, A[X' | W'] = (A Z) C,
+        // B[X' | W'] = eps [X' | W'] (TM)
+        const int nxt = 1 - sl.par;
         LincombWork<T> lx{};
-        lx.A = qcol(s, 0);
+        lx.A = vcol(s, 0);
         lx.a_stride = N;
-        lx.dev_na = d_rank + s;
-        lx.dev_nq = d_cnt + 2 * s;
+        lx.na = p;
+        lx.dev_nq = d_cnt + 3 * s + 2;
         lx.C = d_Y + (size_t)s * LD * LD * 2;
         lx.ldc = LD;
-        lx.out = vcol(s, 0);
+        lx.out = vbuf(s, nxt, 0);
         lx.out_stride = N;
+        if (sl.pol == 0) { lx.out2 = bvbuf(s, nxt, 0); lx.w2 = eps(s); }
         lc_ritz.push_back(lx);
-        if (sl.pol == 0) {
-            LincombWork<T> lb = lx;
-            lb.A = BQ + ((size_t)s * C) * N;
-            lb.out = BV + ((size_t)s * CV) * N;
-            lc_ritz.push_back(lb);
-        }
         LincombWork<T> la = lx;
-        la.A = aqcol(s, 0);
-        la.out = avcol(s, 0);
+        la.A = avcol(s, 0);
+        la.out = avbuf(s, nxt, 0);
+        la.out2 = nullptr;
+        la.w2 = nullptr;
         lc_ritz.push_back(la);
-        LincombWork<T> lw2 = lx;
-        lw2.dev_nq = d_cnt + 2 * s + 1;
-        lw2.C = d_Yw + (size_t)s * LD * LD * 2;
-        lw2.out = vcol(s, sl.nb);
-        if (sl.pol == 0) { lw2.out2 = BV + ((size_t)s * CV + sl.nb) * N; lw2.w2 = eps(s); }
-        lc_ritz.push_back(lw2);
+        max_ritz = std::max(max_ritz, std::min(p, 2 * sl.nb));
+        sl.par = nxt;
         sl.it += 1;
     }
 
@@ -1236,20 +1199,16 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
     auto assign_partials = [&](std::vector<GramWork<T>> &v, int nch) {
         for (size_t i = 0; i < v.size(); ++i) v[i].partial = d_part + i * (size_t)nch * LD * LD * 2;
     };
-    const int nch_def = nch_for(g_defR.size()), nch_svqb = nch_for(g_svqb.size()), nch_rr = nch_for(g_rr.size()),
-              nch_t1 = nch_for(g_t1.size());
+    const int nch_def = nch_for(g_defR.size()), nch_dual = nch_for(g_dual.size()), nch_t1 = nch_for(g_t1.size());
     assign_partials(g_defR, nch_def);
     assign_partials(g_defP, nch_def);
-    assign_partials(g_svqb, nch_svqb);
-    assign_partials(g_rr, nch_rr);
     assign_partials(g_t1, nch_t1);
-    for (size_t i = 0; i < e_svqb.size(); ++i) {
-        e_svqb[i].partial = g_svqb[i].partial;
-        e_svqb[i].nchunks = nch_svqb;
-    }
-    for (size_t i = 0; i < e_rr.size(); ++i) {
-        e_rr[i].partial = g_rr[i].partial;
-        e_rr[i].nchunks = nch_rr;
+    for (size_t i = 0; i < g_dual.size(); ++i) {  // two products per item
+        g_dual[i].partial1 = d_part + 2 * i * (size_t)nch_dual * LD * LD * 2;
+        g_dual[i].partial2 = g_dual[i].partial1 + (size_t)nch_dual * LD * LD * 2;
+        e_rr2[i].partial1 = g_dual[i].partial1;
+        e_rr2[i].partial2 = g_dual[i].partial2;
+        e_rr2[i].nchunks = nch_dual;
     }
     // P-deflation masks
     unsigned char *mask_d = nullptr;
@@ -1268,7 +1227,6 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
     auto *d_lc_x = put(lc_x);
     OpWork<T> *d_op_x[2] = {put(op_x[0]), put(op_x[1])};
     auto *d_rq = put(rq);
-    auto *d_lc_wmove = put(lc_wmove);
     auto *d_lc_mass = put(lc_mass);
     auto *d_res = put(res);
     auto *d_g_defR = put(g_defR);
@@ -1276,12 +1234,9 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
     OpWork<T> *d_op_pre[2] = {put(op_pre[0]), put(op_pre[1])};
     auto *d_g_defP = put(g_defP);
     auto *d_lc_defP = put(lc_defP);
-    auto *d_g_svqb = put(g_svqb);
-    auto *d_e_svqb = put(e_svqb);
-    auto *d_lc_q = put(lc_q);
-    OpWork<T> *d_op_q[2] = {put(op_q[0]), put(op_q[1])};
-    auto *d_g_rr = put(g_rr);
-    auto *d_e_rr = put(e_rr);
+    OpWork<T> *d_op_p[2] = {put(op_p[0]), put(op_p[1])};
+    auto *d_g_dual = put(g_dual);
+    auto *d_e_rr2 = put(e_rr2);
     auto *d_lc_ritz = put(lc_ritz);
     auto *d_g_t1 = put(g_t1);
     stg.upload(st);
@@ -1291,17 +1246,42 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
         BH_HIP_CHECK(hipMemcpyAsync(tc.dst, tc.src, sizeof(T) * tc.n, hipMemcpyDeviceToDevice, st));
     for (int s : plan.fill_y0) {
         Slot &sl = slots[s];
-        launch_fill<T>(vcol(s, C), sl.pol == 0 ? BV + ((size_t)s * CV + C) * N : nullptr, eps(s), sl.y0val, 0.0, N, st);
+        for (int b = 0; b < 2; ++b)  // the Gamma mode sits in column C of both copies
+            launch_fill<T>(vbuf(s, b, C), sl.pol == 0 ? bvbuf(s, b, C) : nullptr, eps(s), sl.y0val, 0.0, N, st);
     }
+    // launches of the timed kernel class are bracketed by events on this stream
+    auto timed = [&](int cls, double bytes, double flops, auto &&launch) {
+        if (eopt.timed_kernel != cls) { launch(); return; }
+        if (G.tev_used == G.tev.size()) {
+            hipEvent_t a, b;
+            BH_HIP_CHECK(hipEventCreate(&a));
+            BH_HIP_CHECK(hipEventCreate(&b));
+            G.tev.push_back({a, b});
+        }
+        auto &ev = G.tev[G.tev_used++];
+        BH_HIP_CHECK(hipEventRecord(ev.first, st));
+        launch();
+        BH_HIP_CHECK(hipEventRecord(ev.second, st));
+        stats_->timed_launches += 1;
+        stats_->timed_bytes += bytes;
+        stats_->timed_flops += flops;
+    };
+    const double n_log = (double)N * std::log2((double)N);
     // T2
     launch_lincomb<T>(d_lc_hist, (int)lc_hist.size(), N, nbmax, st);
     launch_lincomb<T>(d_lc_x, (int)lc_x.size(), N, std::max(maxq_x, 1), st);
-    for (int pol = 0; pol < 2; ++pol)
-        launch_op<T>(pol == 0 ? MODE_THETA_TM : MODE_THETA_TE, nx, ny, d_op_x[pol], (int)op_x[pol].size(), m0max, twx,
-                     twy, st);
+    for (int pol = 0; pol < 2; ++pol) {
+        if (op_x[pol].empty()) continue;
+        long cols = 0;
+        for (auto &o : op_x[pol]) cols += o.count;
+        timed(TIMED_THETA, (double)cols * theta_bytes_per_col(pol), (double)cols * theta_flops_per_col(pol, n_log),
+              [&] {
+                  launch_op<T>(pol == 0 ? MODE_THETA_TM : MODE_THETA_TE, nx, ny, d_op_x[pol], (int)op_x[pol].size(),
+                               m0max, twx, twy, st);
+              });
+    }
     launch_rayleigh<T>(d_rq, (int)rq.size(), m0max, N, st);
     // iteration
-    launch_lincomb<T>(d_lc_wmove, (int)lc_wmove.size(), N, nbmax, st);
     launch_residual<T>(d_res, (int)res.size(), std::max(max_a, 1), N, st);
     if (!g_defR.empty()) {
         launch_gram<T>(d_g_defR, (int)g_defR.size(), nch_def, N, max_d, max_a, st);
@@ -1316,40 +1296,25 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
         launch_reduce(d_red_defP, (int)g_defP.size(), LD, st);
         launch_lincomb<T>(d_lc_defP, (int)lc_defP.size(), N, max_a, st);
     }
-    launch_gram<T>(d_g_svqb, (int)g_svqb.size(), nch_svqb, N, max_p, max_p, st);
-    launch_eigh(d_e_svqb, (int)e_svqb.size(), std::max(max_p, 1), st);
-    launch_lincomb<T>(d_lc_q, (int)lc_q.size(), N, std::max(max_p, 1), st);
     for (int pol = 0; pol < 2; ++pol) {
-        if (op_q[pol].empty()) continue;
-        if (eopt.profile) {
-            hipEvent_t a, b;
-            BH_HIP_CHECK(hipEventCreate(&a));
-            BH_HIP_CHECK(hipEventCreate(&b));
-            BH_HIP_CHECK(hipEventRecord(a, st));
-            launch_op<T>(pol == 0 ? MODE_THETA_TM : MODE_THETA_TE, nx, ny, d_op_q[pol], (int)op_q[pol].size(),
-                         std::max(max_p, 1), twx, twy, st);
-            BH_HIP_CHECK(hipEventRecord(b, st));
-            BH_HIP_CHECK(hipEventSynchronize(b));
-            float ms = 0;
-            BH_HIP_CHECK(hipEventElapsedTime(&ms, a, b));
-            stats_->theta_ms[pol] += ms;
-            stats_->theta_launches[pol] += 1;
-            // columns processed = SVQB ranks of the slots in this launch
-            std::vector<int> rk(S);
-            BH_HIP_CHECK(hipMemcpyAsync(rk.data(), d_rank, sizeof(int) * S, hipMemcpyDeviceToHost, st));
-            BH_HIP_CHECK(hipStreamSynchronize(st));
-            for (int s2 : plan.iter)
-                if (slots[s2].pol == pol) stats_->theta_columns[pol] += rk[s2];
-            (void)hipEventDestroy(a);
-            (void)hipEventDestroy(b);
-        } else {
-            launch_op<T>(pol == 0 ? MODE_THETA_TM : MODE_THETA_TE, nx, ny, d_op_q[pol], (int)op_q[pol].size(),
-                         std::max(max_p, 1), twx, twy, st);
-        }
+        if (op_p[pol].empty()) continue;
+        timed(TIMED_THETA, (double)theta_cols[pol] * theta_bytes_per_col(pol),
+              (double)theta_cols[pol] * theta_flops_per_col(pol, n_log), [&] {
+                  launch_op<T>(pol == 0 ? MODE_THETA_TM : MODE_THETA_TE, nx, ny, d_op_p[pol], (int)op_p[pol].size(),
+                               std::max(max_a, 1), twx, twy, st);
+              });
     }
-    launch_gram<T>(d_g_rr, (int)g_rr.size(), nch_rr, N, max_p, max_p, st);
-    launch_eigh(d_e_rr, (int)e_rr.size(), std::max(max_p, 1), st);
-    launch_lincomb<T>(d_lc_ritz, (int)lc_ritz.size(), N, nbmax, st);
+    {
+        double bytes = 0, flops = 0;
+        for (auto &g2 : g_dual) {
+            bytes += (g2.BZ == g2.Z ? 2.0 : 3.0) * g2.n * (double)N * sizeof(cplx<T>);
+            flops += 2.0 * 8.0 * (double)N * g2.n * (g2.n + 1) / 2.0;
+        }
+        timed(TIMED_GRAM2, bytes, flops,
+              [&] { launch_gram2<T>(d_g_dual, (int)g_dual.size(), nch_dual, N, std::max(max_p, 1), st); });
+    }
+    launch_rr(d_e_rr2, (int)e_rr2.size(), std::max(max_p, 1), st);
+    launch_lincomb<T>(d_lc_ritz, (int)lc_ritz.size(), N, std::max(max_ritz, 1), st);
     // T1 Grams -> compact readback
     if (!g_t1.empty()) {
         int maxkd = 0;
@@ -1364,7 +1329,7 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
         const int gs = G.s1 - G.s0;
         BH_HIP_CHECK(hipMemcpyAsync(h_lam + (size_t)G.s0 * LD, d_lam + (size_t)G.s0 * LD, sizeof(double) * gs * LD,
                                     hipMemcpyDeviceToHost, st));
-        BH_HIP_CHECK(hipMemcpyAsync(h_cnt + 2 * (size_t)G.s0, d_cnt + 2 * (size_t)G.s0, sizeof(int) * 2 * gs,
+        BH_HIP_CHECK(hipMemcpyAsync(h_cnt + 3 * (size_t)G.s0, d_cnt + 3 * (size_t)G.s0, sizeof(int) * 3 * gs,
                                     hipMemcpyDeviceToHost, st));
     }
     BH_HIP_CHECK(hipEventRecord(G.ev, st));

@@ -140,6 +140,8 @@ typedef struct {
     int precision;           /* 0 FP64 */
     int profile;             /* 1: time operator kernels with HIP events (runs one slot group) */
     int groups;              /* slot groups pipelined on separate HIP streams (0 -> 2) */
+    int timed_kernel;        /* kernel class timed with HIP events on its stream:
+                                0 none, 1 Theta operator, 2 dual Rayleigh-Ritz Gram */
 } bh_run_options;
 
 typedef struct {
@@ -147,13 +149,14 @@ typedef struct {
     long iterations;         /* LOBPCG iterations summed over solves */
     long solves;             /* k-point solves (closing Gamma copies excluded) */
     double seconds;          /* wall time of the solve loop */
-    double theta_ms[2];      /* summed Theta-kernel time [TM, TE] (profile = 1) */
-    long theta_launches[2];
-    long theta_columns[2];   /* columns those launches processed */
     long eigh_calls[2];      /* [SVQB, Rayleigh-Ritz] dense problems (profile = 1) */
     long eigh_sweeps[2];     /* Jacobi sweeps summed over them */
     long eigh_max_sweeps[2]; /* largest sweep count of a single problem */
     long eigh_launch_max_sum[2]; /* per-launch maximum sweep count, summed over launches */
+    double timed_ms;             /* summed duration of the timed_kernel launches */
+    long timed_launches;
+    double timed_bytes;          /* their algorithmic HBM bytes */
+    double timed_flops;          /* their algorithmic FLOPs */
     double host_wait_seconds;    /* host time blocked on group completion events */
     double host_plan_seconds;    /* host time spent planning and enqueueing steps */
 } bh_run_stats;
