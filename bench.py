@@ -61,8 +61,8 @@ def main():
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--geoms", type=int, default=256, help="geometries in the sweep (x2 polarisations)")
-    ap.add_argument("--slots", type=int, default=256, help="concurrent band structures (0 = all)")
-    ap.add_argument("--groups", type=int, default=2, help="slot groups pipelined on separate streams")
+    ap.add_argument("--slots", type=int, default=512, help="concurrent band structures (0 = all)")
+    ap.add_argument("--groups", type=int, default=4, help="slot groups pipelined on separate streams")
     ap.add_argument("--precision", type=int, default=0)
     ap.add_argument("--seed", type=int, default=0xB1A2E2D)
     ap.add_argument("--cpu-sample", type=int, default=4, help="band structures timed on the CPU oracle")

@@ -161,7 +161,7 @@ class BatchStats:
 
 
 def run_batch(backend: HipBackend, jobs: Sequence[BandStructureJob], options: Optional[RunOptions] = None,
-              slots: int = 64, precision: int = 0, profile: bool = False, groups: int = 2, timed_kernel: int = 0):
+              slots: int = 64, precision: int = 0, profile: bool = False, groups: int = 4, timed_kernel: int = 0):
     """Runs many band structures concurrently on one GPU (all on one grid).
 
     `slots` band structures are resident at once; they are split into `groups`

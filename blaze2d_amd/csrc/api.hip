@@ -17,9 +17,10 @@ using namespace bh;
 struct bh_context {
     int device = 0;
     hipStream_t st{};
-    // band-solve slot groups run on these; created right after st so that each
-    // gets a hardware queue of its own (HIP shares queues once
-    // GPU_MAX_HW_QUEUES streams exist, and the null stream takes one as well)
+    // band-solve slot groups run on st and these; created right after st so
+    // that with HIP's default of 4 hardware queues per process each of four
+    // groups has a queue of its own (HIP shares queues once GPU_MAX_HW_QUEUES
+    // streams exist)
     hipStream_t gst[3]{};
     // twiddle tables per (n, precision)
     std::vector<std::pair<int, void *>> tw64, tw32;
@@ -566,13 +567,14 @@ int bh_bandstructure_run(bh_context *ctx, const bh_job *jobs, int njobs, const b
     eo.slots = std::min(eo.slots, njobs);
     eo.precision = opts ? opts->precision : 0;
     eo.profile = opts ? opts->profile : 0;
-    eo.groups = opts && opts->groups > 0 ? opts->groups : 2;
+    eo.groups = opts && opts->groups > 0 ? opts->groups : 4;
     eo.timed_kernel = opts ? opts->timed_kernel : 0;
     if (eo.timed_kernel < 0 || eo.timed_kernel > 2) return fail("bh_bandstructure_run: timed_kernel must be 0, 1 or 2");
     eo.setup_stream = ctx->st;
-    eo.group_streams[0] = ctx->gst[0];
-    eo.group_streams[1] = ctx->gst[1];
-    eo.group_streams[2] = ctx->gst[2];
+    eo.group_streams[0] = ctx->st;
+    eo.group_streams[1] = ctx->gst[0];
+    eo.group_streams[2] = ctx->gst[1];
+    eo.group_streams[3] = ctx->gst[2];
     BH_HIP_CHECK(hipSetDevice(ctx->device));
     auto eng = make_engine(js[0].nx, js[0].ny, nbmax, eo);
     std::vector<JobResult> res;

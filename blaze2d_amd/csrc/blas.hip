@@ -9,23 +9,25 @@
 namespace bh {
 
 // ---------------------------------------------------------------------------
-// Gram: grid = (nchunks, nwork); 256 threads as 16 x 16, each owning a 3 x 3
-// block of the (up to 48 x 48) output; rows staged through LDS 32 at a time.
+// Gram: grid = (nchunks, nwork); 256 threads as 16 x 16, each owning a BU x BU
+// block of the (up to 16 BU x 16 BU) output; rows staged through LDS 32 at a
+// time.  The launch picks BU from the widest operand (16, 32 or 48 columns).
 // ---------------------------------------------------------------------------
 constexpr int GRAM_SR = 32;
 
-template <typename T>
+template <typename T, int BU>
 __global__ void __launch_bounds__(256) gram_kernel(const GramWork<T> *__restrict__ work, long rows_per_chunk,
                                                    long nrows) {
+    constexpr int W = 16 * BU;
     const GramWork<T> &w = work[blockIdx.y];
     const int na = w.dev_na ? *w.dev_na : w.na;
     const int nb = w.dev_nb ? *w.dev_nb : w.nb;
-    if (na <= 0 || nb <= 0) return;
-    __shared__ cplx<T> As[GRAM_SR][GRAM_MAXC + 1];
-    __shared__ cplx<T> Bs[GRAM_SR][GRAM_MAXC + 1];
+    if (na <= 0 || nb <= 0 || na > W || nb > W) return;  // the launch sizes W from the widest item
+    __shared__ cplx<T> As[GRAM_SR][W + 1];
+    __shared__ cplx<T> Bs[GRAM_SR][W + 1];
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
     const bool lower = w.lower_only != 0;
-    double ar[3][3] = {}, ai[3][3] = {};
+    double ar[BU][BU] = {}, ai[BU][BU] = {};
     const long r0 = (long)blockIdx.x * rows_per_chunk;
     const long r1 = r0 + rows_per_chunk < nrows ? r0 + rows_per_chunk : nrows;
     for (long rb = r0; rb < r1; rb += GRAM_SR) {
@@ -48,9 +50,9 @@ __global__ void __launch_bounds__(256) gram_kernel(const GramWork<T> *__restrict
         }
         __syncthreads();
         for (int r = 0; r < nr; ++r) {
-            double axr[3], axi[3], bxr[3], bxi[3];
+            double axr[BU], axi[BU], bxr[BU], bxi[BU];
 #pragma unroll
-            for (int u = 0; u < 3; ++u) {
+            for (int u = 0; u < BU; ++u) {
                 int i = ty + 16 * u;
                 cplx<T> a = i < na ? As[r][i] : cplx<T>{(T)0, (T)0};
                 axr[u] = (double)a.x;
@@ -61,9 +63,9 @@ __global__ void __launch_bounds__(256) gram_kernel(const GramWork<T> *__restrict
                 bxi[u] = (double)b.y;
             }
 #pragma unroll
-            for (int u = 0; u < 3; ++u)
+            for (int u = 0; u < BU; ++u)
 #pragma unroll
-                for (int v = 0; v < 3; ++v) {
+                for (int v = 0; v < BU; ++v) {
                     if (lower && v > u) continue;
                     ar[u][v] += axr[u] * bxr[v] + axi[u] * bxi[v];
                     ai[u][v] += axr[u] * bxi[v] - axi[u] * bxr[v];
@@ -73,9 +75,9 @@ __global__ void __launch_bounds__(256) gram_kernel(const GramWork<T> *__restrict
     }
     double *out = w.partial + 2 * (size_t)blockIdx.x * w.ld * w.ld;
 #pragma unroll
-    for (int u = 0; u < 3; ++u)
+    for (int u = 0; u < BU; ++u)
 #pragma unroll
-        for (int v = 0; v < 3; ++v) {
+        for (int v = 0; v < BU; ++v) {
             int i = ty + 16 * u, j = tx + 16 * v;
             if (i < na && j < nb) {
                 out[2 * (i * w.ld + j)] = ar[u][v];
@@ -90,7 +92,13 @@ void launch_gram(const GramWork<T> *dwork, int nwork, int nchunks, long rows, in
     if (maxa > GRAM_MAXC || maxb > GRAM_MAXC) throw std::runtime_error("gram operand wider than 48 columns");
     long per = (rows + nchunks - 1) / nchunks;
     per = (per + GRAM_SR - 1) / GRAM_SR * GRAM_SR;
-    hipLaunchKernelGGL((gram_kernel<T>), dim3(nchunks, nwork), dim3(256), 0, st, dwork, per, rows);
+    const int wide = maxa > maxb ? maxa : maxb;
+    if (wide <= 16)
+        hipLaunchKernelGGL((gram_kernel<T, 1>), dim3(nchunks, nwork), dim3(256), 0, st, dwork, per, rows);
+    else if (wide <= 32)
+        hipLaunchKernelGGL((gram_kernel<T, 2>), dim3(nchunks, nwork), dim3(256), 0, st, dwork, per, rows);
+    else
+        hipLaunchKernelGGL((gram_kernel<T, 3>), dim3(nchunks, nwork), dim3(256), 0, st, dwork, per, rows);
     BH_HIP_CHECK(hipGetLastError());
 }
 
@@ -230,100 +238,115 @@ constexpr int EIGH_NT = 1024;
 
 size_t eigh_shared_bytes(int maxn) {
     size_t ne = maxn + (maxn & 1), np = ne / 2;
-    // A, V (complex), D (ne), rc (np), rs (complex np), rp/rq/rf (int np)
-    return sizeof(double) * (4 * ne * ne + ne + np + 2 * np) + sizeof(int) * 3 * np;
+    // A, V (complex ne x ne), two rounds of rotations (32 bytes each), D (ne)
+    return sizeof(double2) * 2 * ne * ne + 32 * 2 * np + sizeof(double) * ne;
 }
 
-// Scratch for one Jacobi round: the rotation of each of the np pairs.
-struct JacobiScratch {
-    double2 *rs;  // sigma = s conj(h / |h|)
-    double *rc;   // cosine
-    int *rp, *rq, *rf;  // pair (p < q); flag 0 identity, 1 rotate, 2 negligible (zero the pair)
+// One plane rotation of a Jacobi round, packed so that a reader needs two
+// 16-byte LDS loads: cosine c, sigma = s conj(h / |h|), and p | q << 8 | flag << 16
+// with flag 0 identity, 1 rotate, 2 negligible (zero the pair).
+struct alignas(16) JacobiRot {
+    double c;
+    int pk, pad;
+    double2 s;
 };
 
 // Parallel cyclic Jacobi for the Hermitian matrix A held in LDS (col-major,
 // leading dimension ld, order ne even, padding rows/columns zero), with the
-// rotations accumulated into V.  All EIGH_NT threads of the workgroup call it.
+// rotations accumulated into V.  All EIGH_NT threads of the workgroup call it;
+// rot holds 2 * ne / 2 entries (two consecutive rounds), red 64 doubles.
 //
 // Each round applies ne/2 disjoint rotations (round-robin pairing).  The matrix
 // is partitioned into np x np 2x2 blocks (row pair, column pair); a thread
 // owning a block applies both the column rotation of its column pair and the
-// row rotation of its row pair in registers, so a round needs two barriers.
-// A sweep ends the iteration when it needed no rotation or its largest
-// relative rotation |a_pq| / sqrt(|a_pp a_qq|) was below 1e-9 (quadratic
-// convergence leaves the off-diagonal below 1e-18 relative).  Returns the
-// number of sweeps (at most 40).
-__device__ int jacobi_lds(double2 *A, double2 *V, int ld, int ne, JacobiScratch sc, int *nrot,
-                          unsigned long long *maxratio_bits) {
+// row rotation of its row pair in registers.  The rotations of round r are
+// computed by the first np threads while the other threads apply round r-1's
+// rotations to V; a round needs two barriers.  Each rotating thread keeps the
+// largest relative rotation |a_pq|^2 / |a_pp a_qq| of the sweep in a register;
+// one wave reduction per sweep ends the iteration when it is below
+// stop_ratio^2 (quadratic convergence leaves the off-diagonal near stop_ratio^2
+// relative).  An odd leading dimension keeps the 16-byte LDS accesses free of
+// bank conflicts.  Returns the number of sweeps (at most 40).
+__device__ int jacobi_lds(double2 *A, double2 *V, int ld, int ne, JacobiRot *rot, double *red,
+                          double stop_ratio = 1e-9) {
     const int tid = threadIdx.x;
     const int np = ne / 2;
-    double2 *rs = sc.rs;
-    double *rc = sc.rc;
-    int *rp = sc.rp, *rq = sc.rq, *rf = sc.rf;
-    // fixed per-thread work for every round: one 2x2 block (np*np <= 1024) and
-    // up to two eigenvector items (np*ne <= 2048)
+    // fixed per-thread block for every round (np*np <= 1024)
     const bool has_blk = tid < np * np;
     const int bt1 = has_blk ? tid / np : 0, bt2 = has_blk ? tid - bt1 * np : 0;
-    const int nv = np * ne;
-    const int vt0 = tid < nv ? tid / ne : 0, vk0 = tid < nv ? tid - vt0 * ne : 0;
-    const int vt1 = tid + EIGH_NT < nv ? (tid + EIGH_NT) / ne : 0;
-    const int vk1 = tid + EIGH_NT < nv ? (tid + EIGH_NT) - vt1 * ne : 0;
+    const int nv = np * ne;  // eigenvector items (pair, row)
+    const double stop2 = stop_ratio * stop_ratio;
+    // V items of one round, rotations from parity slot `par`; threads [first, EIGH_NT) share them
+    auto rotate_v = [&](int par, int first) {
+        for (int e = tid - first; e >= 0 && e < nv; e += EIGH_NT - first) {
+            const int t = e / ne, k = e - t * ne;
+            const JacobiRot R = rot[par * np + t];
+            if (((R.pk >> 16) & 3) != 1) continue;
+            const int p = R.pk & 255, q = (R.pk >> 8) & 255;
+            const double c = R.c;
+            const double2 sg = R.s;
+            const double2 vkp = V[k + p * ld], vkq = V[k + q * ld];
+            V[k + p * ld] = make_double2(c * vkp.x + sg.x * vkq.x - sg.y * vkq.y, c * vkp.y + sg.x * vkq.y + sg.y * vkq.x);
+            V[k + q * ld] = make_double2(-(sg.x * vkp.x + sg.y * vkp.y) + c * vkq.x,
+                                         -(sg.x * vkp.y - sg.y * vkp.x) + c * vkq.y);
+        }
+    };
     int sweep = 0;
     for (; sweep < 40; ++sweep) {
-        if (tid == 0) { *nrot = 0; *maxratio_bits = 0ull; }
-        __syncthreads();
+        double my_max2 = 0.0;  // rotating threads: largest relative rotation^2 this sweep
         for (int r = 0; r < ne - 1; ++r) {
-            // round-robin pairing, pair slot t: slot 0 (r, ne-1); slot t ((r+t) mod (ne-1), (r-t) mod (ne-1))
+            const int par = r & 1;
             if (tid < np) {
+                // round-robin pairing, pair slot t: slot 0 (r, ne-1); slot t ((r+t) mod (ne-1), (r-t) mod (ne-1))
                 int p, q;
                 if (tid == 0) { p = r; q = ne - 1; }
                 else { p = (r + tid) % (ne - 1); q = (r - tid + ne - 1) % (ne - 1); }
                 if (p > q) { int t = p; p = q; q = t; }
-                double2 h = A[p + q * ld];
-                double habs = sqrt(h.x * h.x + h.y * h.y);
-                double app = A[p + p * ld].x, aqq = A[q + q * ld].x;
-                double c = 1.0;
+                const double2 h = A[p + q * ld];
+                const double hh = h.x * h.x + h.y * h.y;
+                const double app = A[p + p * ld].x, aqq = A[q + q * ld].x;
+                const double papq = fabs(app * aqq);
+                JacobiRot R;
+                R.c = 1.0;
+                R.s = make_double2(0.0, 0.0);
+                R.pad = 0;
                 int flag = 0;
-                double2 sg = make_double2(0.0, 0.0);
-                double scale = sqrt(fabs(app) * fabs(aqq));
-                if (habs != 0.0 && habs > 2.2e-16 * scale) {
+                if (hh != 0.0 && hh > 4.84e-32 * papq) {  // |h| > 2.2e-16 sqrt(|a_pp a_qq|)
                     flag = 1;
-                    double tau = (aqq - app) / (2.0 * habs);
-                    double t = (tau >= 0 ? -1.0 : 1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-                    c = 1.0 / sqrt(1.0 + t * t);
-                    double s = t * c;
-                    sg = make_double2(s * h.x / habs, -s * h.y / habs);  // sigma = s conj(h/|h|)
-                    atomicAdd(nrot, 1);
-                    double ratio = scale > 0.0 ? habs / scale : 1.0;
-                    atomicMax(maxratio_bits, (unsigned long long)__double_as_longlong(ratio));
-                } else if (habs != 0.0) {
+                    const double ri = rsqrt(hh);  // 1 / |h|
+                    const double tau = (aqq - app) * 0.5 * ri;
+                    const double t = (tau >= 0 ? -1.0 : 1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+                    const double c = rsqrt(1.0 + t * t);
+                    const double sr = t * c * ri;
+                    R.c = c;
+                    R.s = make_double2(sr * h.x, -sr * h.y);  // sigma = s conj(h/|h|)
+                    my_max2 = fmax(my_max2, papq > 0.0 ? hh / papq : 1.0);
+                } else if (hh != 0.0) {
                     flag = 2;
                 }
-                rf[tid] = flag;
-                rp[tid] = p;
-                rq[tid] = q;
-                rc[tid] = c;
-                rs[tid] = sg;
+                R.pk = p | (q << 8) | (flag << 16);
+                rot[par * np + tid] = R;
+            } else if (r > 0) {
+                rotate_v(1 - par, np);  // previous round's rotations
             }
             __syncthreads();
             do {
                 if (!has_blk) break;
-                const int t1 = bt1, t2 = bt2;  // row pair, column pair
-                const int f1 = rf[t1], f2 = rf[t2];
+                const JacobiRot R1 = rot[par * np + bt1], R2 = rot[par * np + bt2];  // row pair, column pair
+                const int f1 = (R1.pk >> 16) & 3, f2 = (R2.pk >> 16) & 3;
+                const int p1 = R1.pk & 255, q1 = (R1.pk >> 8) & 255, p2 = R2.pk & 255, q2 = (R2.pk >> 8) & 255;
                 if (f1 != 1 && f2 != 1) {
-                    if (t1 == t2 && f1 == 2) {
-                        int p = rp[t1], q = rq[t1];
-                        A[p + q * ld] = make_double2(0.0, 0.0);
-                        A[q + p * ld] = make_double2(0.0, 0.0);
+                    if (bt1 == bt2 && f1 == 2) {
+                        A[p1 + q1 * ld] = make_double2(0.0, 0.0);
+                        A[q1 + p1 * ld] = make_double2(0.0, 0.0);
                     }
                     break;
                 }
-                const int p1 = rp[t1], q1 = rq[t1], p2 = rp[t2], q2 = rq[t2];
                 double2 a11 = A[p1 + p2 * ld], a12 = A[p1 + q2 * ld];
                 double2 a21 = A[q1 + p2 * ld], a22 = A[q1 + q2 * ld];
                 if (f2 == 1) {  // columns: [:,p] = c [:,p] + s [:,q]; [:,q] = -conj(s) [:,p] + c [:,q]
-                    const double c = rc[t2];
-                    const double2 s = rs[t2];
+                    const double c = R2.c;
+                    const double2 s = R2.s;
                     double2 n11 = make_double2(c * a11.x + s.x * a12.x - s.y * a12.y, c * a11.y + s.x * a12.y + s.y * a12.x);
                     double2 n12 = make_double2(-(s.x * a11.x + s.y * a11.y) + c * a12.x, -(s.x * a11.y - s.y * a11.x) + c * a12.y);
                     double2 n21 = make_double2(c * a21.x + s.x * a22.x - s.y * a22.y, c * a21.y + s.x * a22.y + s.y * a22.x);
@@ -331,15 +354,15 @@ __device__ int jacobi_lds(double2 *A, double2 *V, int ld, int ne, JacobiScratch 
                     a11 = n11; a12 = n12; a21 = n21; a22 = n22;
                 }
                 if (f1 == 1) {  // rows: [p,:] = c [p,:] + conj(s) [q,:]; [q,:] = -s [p,:] + c [q,:]
-                    const double c = rc[t1];
-                    const double2 s = rs[t1];
+                    const double c = R1.c;
+                    const double2 s = R1.s;
                     double2 n11 = make_double2(c * a11.x + s.x * a21.x + s.y * a21.y, c * a11.y + s.x * a21.y - s.y * a21.x);
                     double2 n21 = make_double2(-(s.x * a11.x - s.y * a11.y) + c * a21.x, -(s.x * a11.y + s.y * a11.x) + c * a21.y);
                     double2 n12 = make_double2(c * a12.x + s.x * a22.x + s.y * a22.y, c * a12.y + s.x * a22.y - s.y * a22.x);
                     double2 n22 = make_double2(-(s.x * a12.x - s.y * a12.y) + c * a22.x, -(s.x * a12.y + s.y * a12.x) + c * a22.y);
                     a11 = n11; a12 = n12; a21 = n21; a22 = n22;
                 }
-                if (t1 == t2) {  // the rotated pair: exact zero off-diagonal, real diagonal
+                if (bt1 == bt2) {  // the rotated pair: exact zero off-diagonal, real diagonal
                     a12 = make_double2(0.0, 0.0);
                     a21 = make_double2(0.0, 0.0);
                     a11.y = 0.0;
@@ -350,22 +373,17 @@ __device__ int jacobi_lds(double2 *A, double2 *V, int ld, int ne, JacobiScratch 
                 A[q1 + p2 * ld] = a21;
                 A[q1 + q2 * ld] = a22;
             } while (false);
-            for (int e = tid, it = 0; e < nv; e += EIGH_NT, ++it) {  // eigenvectors: column rotations only
-                const int t = it == 0 ? vt0 : vt1, k = it == 0 ? vk0 : vk1;
-                if (rf[t] != 1) continue;
-                double c = rc[t];
-                double2 sg = rs[t];
-                int p = rp[t], q = rq[t];
-                double2 vkp = V[k + p * ld], vkq = V[k + q * ld];
-                V[k + p * ld] = make_double2(c * vkp.x + sg.x * vkq.x - sg.y * vkq.y,
-                                             c * vkp.y + sg.x * vkq.y + sg.y * vkq.x);
-                V[k + q * ld] = make_double2(-(sg.x * vkp.x + sg.y * vkp.y) + c * vkq.x,
-                                             -(sg.x * vkp.y - sg.y * vkp.x) + c * vkq.y);
-            }
             __syncthreads();
         }
-        const bool done = *nrot == 0 || __longlong_as_double((long long)*maxratio_bits) < 1e-9;
-        __syncthreads();  // every thread has read the flags before thread 0 resets them
+        rotate_v((ne - 2) & 1, 0);  // the sweep's last round
+        if (tid < 64) {  // the rotating threads are all in wave 0
+            double m = my_max2;
+            for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+            if (tid == 0) red[0] = m;
+        }
+        __syncthreads();
+        const bool done = red[0] < stop2;  // also true when the sweep needed no rotation
+        __syncthreads();  // every thread has read red[0] before the next sweep writes it
         if (done) break;
     }
     return sweep < 40 ? sweep + 1 : 40;
@@ -388,14 +406,9 @@ __global__ void __launch_bounds__(1024) eigh_kernel(const EighWork *__restrict__
     extern __shared__ double sm[];
     double2 *A = reinterpret_cast<double2 *>(sm);  // col-major [i + j*ne]
     double2 *V = A + ne * ne;
-    double2 *rs = V + ne * ne;                      // 16-byte aligned block first
-    double *D = reinterpret_cast<double *>(rs + np);
-    double *rc = D + ne;
-    int *rp = reinterpret_cast<int *>(rc + np);
-    int *rq = rp + np;
-    int *rf = rq + np;  // 0 identity, 1 rotate, 2 negligible (zero the pair)
-    __shared__ int nrot;
-    __shared__ unsigned long long maxratio_bits;
+    JacobiRot *rot = reinterpret_cast<JacobiRot *>(V + ne * ne);  // two rounds of rotations
+    double *D = reinterpret_cast<double *>(rot + 2 * np);
+    __shared__ double red[1];
 
     const int ld = w.ld;
     auto G = [&](int a, int b) -> double2 {
@@ -503,7 +516,7 @@ __global__ void __launch_bounds__(1024) eigh_kernel(const EighWork *__restrict__
         }
         __syncthreads();
     }
-    const int sweeps = jacobi_lds(A, V, ne, ne, JacobiScratch{rs, rc, rp, rq, rf}, &nrot, &maxratio_bits);
+    const int sweeps = jacobi_lds(A, V, ne, ne, rot, red);
     if (tid == 0 && w.sweeps_out) {
         atomicAdd(w.sweeps_out, sweeps);
         atomicMax(w.sweeps_out + 2, sweeps);  // maximum over the launch
@@ -781,10 +794,15 @@ template void launch_gram2<float>(const Gram2Work<float> *, int, int, long, int,
 // [2] max SVQB sweeps, [3] Rayleigh-Ritz sweeps, [5] max Rayleigh-Ritz sweeps.
 // ---------------------------------------------------------------------------
 size_t rr_shared_bytes(int maxn) {
-    size_t ne = maxn + (maxn & 1), np = ne / 2;
-    // A, V, T (complex ne x ne), D (ne), wsh (ne), rc (np), rs (complex np), rp/rq/rf, rank_of (int)
-    return sizeof(double) * (6 * ne * ne + 2 * ne + np + 2 * np) + sizeof(int) * (3 * np + ne);
+    size_t ne = maxn + (maxn & 1), np = ne / 2, lda = ne + 1;
+    // A, V, T (complex lda x ne), two rounds of rotations, D, wsh, piv (ne doubles), rank_of (ne ints)
+    return sizeof(double2) * 3 * lda * ne + 32 * 2 * np + sizeof(double) * 3 * ne + sizeof(int) * ne;
 }
+
+// Largest relative rotation of a Rayleigh-Ritz Jacobi sweep that ends the
+// iteration: the off-diagonal left behind is near its square (1e-12 relative),
+// far below what the eigenvalues (error ~ off^2 / gap) or the Ritz vectors need.
+constexpr double RR_STOP_RATIO = 1e-6;
 
 __global__ void __launch_bounds__(1024) rr_kernel(const RRWork *__restrict__ work) {
     const RRWork &w = work[blockIdx.x];
@@ -796,21 +814,19 @@ __global__ void __launch_bounds__(1024) rr_kernel(const RRWork *__restrict__ wor
     }
     const int ne = n + (n & 1);
     const int np = ne / 2;
+    const int lda = ne + 1;  // odd: conflict-free 16-byte LDS accesses
     extern __shared__ double rsm[];
-    double2 *A = reinterpret_cast<double2 *>(rsm);  // col-major [i + j*ne]
-    double2 *V = A + ne * ne;
-    double2 *Tm = V + ne * ne;
-    double2 *rs = Tm + ne * ne;
-    double *D = reinterpret_cast<double *>(rs + np);
+    double2 *A = reinterpret_cast<double2 *>(rsm);  // col-major [i + j*lda]
+    double2 *V = A + lda * ne;
+    double2 *Tm = V + lda * ne;
+    JacobiRot *rot = reinterpret_cast<JacobiRot *>(Tm + lda * ne);  // two rounds of rotations
+    double *D = reinterpret_cast<double *>(rot + 2 * np);
     double *wsh = D + ne;
-    double *rc = wsh + ne;
-    int *rp = reinterpret_cast<int *>(rc + np);
-    int *rq = rp + np;
-    int *rf = rq + np;
-    int *rank_of = rf + np;
-    __shared__ int nrot, rank, chol_ok;
-    __shared__ unsigned long long maxratio_bits;
+    double *piv = wsh + ne;
+    int *rank_of = reinterpret_cast<int *>(piv + ne);
+    __shared__ int rank, chol_ok;
     __shared__ double lmax;
+    __shared__ double red[1];
     const int ld = w.ld;
     auto load = [&](const double *partial) {  // Hermitian n x n from lower partial sums, zero padding
         for (int e = tid; e < ne * ne; e += EIGH_NT) {
@@ -827,74 +843,102 @@ __global__ void __launch_bounds__(1024) rr_kernel(const RRWork *__restrict__ wor
                 if (i == j) im = 0.0;
                 v = i >= j ? make_double2(re, im) : make_double2(re, -im);
             }
-            A[e] = v;
+            A[i + j * lda] = v;
         }
     };
     // 1. scaled G1
     load(w.partial1);
     __syncthreads();
     for (int i = tid; i < ne; i += EIGH_NT) {
-        double nrm = i < n ? sqrt(fmax(A[i + i * ne].x, 0.0)) : 0.0;
+        double nrm = i < n ? sqrt(fmax(A[i + i * lda].x, 0.0)) : 0.0;
         D[i] = nrm > 1e-30 ? 1.0 / nrm : 1.0;
     }
+    if (tid == 0) chol_ok = 1;
     __syncthreads();
     for (int e = tid; e < ne * ne; e += EIGH_NT) {
         const int i = e % ne, j = e / ne;
         const double s = D[i] * D[j];
-        A[e].x *= s;
-        A[e].y *= s;
-        Tm[e] = make_double2(0.0, 0.0);
+        const double2 a = make_double2(A[i + j * lda].x * s, A[i + j * lda].y * s);
+        A[i + j * lda] = a;
+        V[i + j * lda] = a;  // Cholesky works on a copy (L below the diagonal)
+        Tm[i + j * lda] = make_double2(0.0, 0.0);
     }
-    // 2. Cholesky into V (left-looking, one column per step)
+    __syncthreads();
+    // 2. right-looking Cholesky of the scaled G1 in L = V, one barrier per column:
+    // step j finalises column j (pivot piv[j] = sqrt(d_j), every reader
+    // recomputes d_j = L[j][j] - |L[j][j-1]|^2 from values no one writes in the
+    // step) and applies column j-1 to the trailing block.  The diagonal of L is
+    // piv; its LDS entries keep the partially updated values.
     double2 *L = V;
-    if (tid == 0) chol_ok = 1;
-    __syncthreads();
-    for (int e = tid; e < ne * ne; e += EIGH_NT) L[e] = A[e];
-    __syncthreads();
-    for (int j = 0; j < n; ++j) {
-        if (tid == 0) {
-            double d = L[j + j * ne].x;
-            for (int k = 0; k < j; ++k) {
-                const double2 l = L[j + k * ne];
-                d -= l.x * l.x + l.y * l.y;
-            }
-            if (!(d > 1e-8)) chol_ok = 0;  // relative pivot: columns have unit B-norm
-            L[j + j * ne] = make_double2(d > 0.0 ? sqrt(d) : 1.0, 0.0);
+    // lower-triangle entries (i, k), k <= i, owned by this thread (at most two for n <= 48)
+    int own_i[2] = {-1, -1}, own_k[2] = {0, 0};
+    {
+        const int ntri = n * (n + 1) / 2;
+        for (int u = 0; u < 2; ++u) {
+            const int e = tid + u * EIGH_NT;
+            if (e >= ntri) break;
+            int i = 0;
+            while ((i + 1) * (i + 2) / 2 <= e) ++i;
+            own_i[u] = i;
+            own_k[u] = e - i * (i + 1) / 2;
         }
-        __syncthreads();
-        if (!chol_ok) break;
-        const double ljj = L[j + j * ne].x;
-        for (int i = j + 1 + tid; i < n; i += EIGH_NT) {
-            double2 a = L[i + j * ne];
-            for (int k = 0; k < j; ++k) {  // a -= L[i,k] conj(L[j,k])
-                const double2 x = L[i + k * ne], y = L[j + k * ne];
+    }
+    for (int j = 0; j < n; ++j) {
+        for (int u = 0; u < 2; ++u) {
+            const int i = own_i[u], k = own_k[u];
+            if (i < 0 || k < j) continue;
+            double2 a = L[i + k * lda];
+            if (j > 0) {  // column j-1 (final) applied to the trailing entry
+                const double2 x = L[i + (j - 1) * lda], y = L[k + (j - 1) * lda];
                 a.x -= x.x * y.x + x.y * y.y;
                 a.y -= x.y * y.x - x.x * y.y;
             }
-            L[i + j * ne] = make_double2(a.x / ljj, a.y / ljj);
+            if (k == j) {
+                double d = L[j + j * lda].x;
+                if (j > 0) {
+                    const double2 y = L[j + (j - 1) * lda];
+                    d -= y.x * y.x + y.y * y.y;
+                }
+                const double sd = d > 0.0 ? sqrt(d) : 1.0;
+                if (i == j) {
+                    piv[j] = sd;
+                    if (!(d > 1e-8)) chol_ok = 0;  // relative pivot: columns have unit B-norm
+                } else {
+                    L[i + k * lda] = make_double2(a.x / sd, a.y / sd);
+                }
+            } else {
+                L[i + k * lda] = a;  // trailing entry, k > j
+            }
         }
         __syncthreads();
     }
     if (chol_ok) {
-        // M = L^-1 by forward substitution (one column per thread) into A, then T = D M^H
-        for (int c = tid; c < n; c += EIGH_NT) {
-            for (int i = 0; i < n; ++i) A[i + c * ne] = make_double2(0.0, 0.0);
-            for (int i = c; i < n; ++i) {
-                double2 acc = make_double2(i == c ? 1.0 : 0.0, 0.0);
-                for (int k = c; k < i; ++k) {
-                    const double2 l = L[i + k * ne], m = A[k + c * ne];
-                    acc.x -= l.x * m.x - l.y * m.y;
-                    acc.y -= l.x * m.y + l.y * m.x;
-                }
-                const double lii = L[i + i * ne].x;
-                A[i + c * ne] = make_double2(acc.x / lii, acc.y / lii);
-            }
+        // M = L^-1, right-looking: step i finalises row i of M (M[i][c] =
+        // B[i][c] / piv[i]) and removes it from the rows below; B starts as I
+        // in A.  T = D M^H is written as rows finish: T[c][i] = D[c] conj(M[i][c]).
+        for (int e = tid; e < ne * ne; e += EIGH_NT) {
+            const int i = e % ne, j = e / ne;
+            A[i + j * lda] = make_double2(i == j ? 1.0 : 0.0, 0.0);
         }
         __syncthreads();
-        for (int e = tid; e < n * n; e += EIGH_NT) {
-            const int r = e % n, c = e / n;  // T[r][c] = D[r] conj(M[c][r]), zero below the diagonal
-            const double2 m = r <= c ? A[c + r * ne] : make_double2(0.0, 0.0);
-            Tm[r + c * ne] = make_double2(D[r] * m.x, -D[r] * m.y);
+        for (int i = 0; i < n; ++i) {
+            const double inv = 1.0 / piv[i];
+            for (int u = 0; u < 2; ++u) {
+                const int k = own_i[u], c = own_k[u];  // rows k >= i, columns c <= i
+                if (k < i || c > i) continue;
+                const double2 b = A[i + c * lda];
+                const double2 m = make_double2(b.x * inv, b.y * inv);  // M[i][c]
+                if (k == i) {
+                    Tm[c + i * lda] = make_double2(D[c] * m.x, -D[c] * m.y);
+                } else {
+                    const double2 l = L[k + i * lda];
+                    double2 bk = A[k + c * lda];
+                    bk.x -= l.x * m.x - l.y * m.y;
+                    bk.y -= l.x * m.y + l.y * m.x;
+                    A[k + c * lda] = bk;
+                }
+            }
+            __syncthreads();
         }
         if (tid == 0) {
             rank = n;
@@ -904,11 +948,11 @@ __global__ void __launch_bounds__(1024) rr_kernel(const RRWork *__restrict__ wor
         // SVQB: eigendecomposition of the scaled G1 (A still holds it)
         for (int e = tid; e < ne * ne; e += EIGH_NT) {
             const int i = e % ne, j = e / ne;
-            V[e] = make_double2(i == j ? 1.0 : 0.0, 0.0);
+            V[i + j * lda] = make_double2(i == j ? 1.0 : 0.0, 0.0);
         }
         __syncthreads();
-        const int sw = jacobi_lds(A, V, ne, ne, JacobiScratch{rs, rc, rp, rq, rf}, &nrot, &maxratio_bits);
-        for (int i = tid; i < n; i += EIGH_NT) wsh[i] = A[i + i * ne].x;
+        const int sw = jacobi_lds(A, V, lda, ne, rot, red);
+        for (int i = tid; i < n; i += EIGH_NT) wsh[i] = A[i + i * lda].x;
         __syncthreads();
         for (int i = tid; i < n; i += EIGH_NT) {
             const double wi = wsh[i];
@@ -940,8 +984,8 @@ __global__ void __launch_bounds__(1024) rr_kernel(const RRWork *__restrict__ wor
             const int pos = n - 1 - rank_of[i];
             if (pos >= rank) continue;
             const double sc = D[row] / sqrt(fmax(wsh[i], 1e-30));
-            const double2 v = V[row + i * ne];
-            Tm[row + pos * ne] = make_double2(v.x * sc, v.y * sc);
+            const double2 v = V[row + i * lda];
+            Tm[row + pos * lda] = make_double2(v.x * sc, v.y * sc);
         }
     }
     __syncthreads();
@@ -950,15 +994,15 @@ __global__ void __launch_bounds__(1024) rr_kernel(const RRWork *__restrict__ wor
     // 3. G2 into A, X = G2 T into V, M = T^H X into A (zero padded to re)
     load(w.partial2);
     __syncthreads();
-    for (int e = tid; e < n * r; e += EIGH_NT) {
+        for (int e = tid; e < n * r; e += EIGH_NT) {
         const int i = e % n, c = e / n;
         double xr = 0.0, xi = 0.0;
         for (int k = 0; k < n; ++k) {
-            const double2 a = A[i + k * ne], t = Tm[k + c * ne];
+            const double2 a = A[i + k * lda], t = Tm[k + c * lda];
             xr += a.x * t.x - a.y * t.y;
             xi += a.x * t.y + a.y * t.x;
         }
-        V[i + c * ne] = make_double2(xr, xi);
+        V[i + c * lda] = make_double2(xr, xi);
     }
     __syncthreads();
     for (int e = tid; e < ne * ne; e += EIGH_NT) {
@@ -966,38 +1010,38 @@ __global__ void __launch_bounds__(1024) rr_kernel(const RRWork *__restrict__ wor
         double mr = 0.0, mi = 0.0;
         if (a < r && c < r) {
             for (int k = 0; k < n; ++k) {  // conj(T[k][a]) X[k][c]
-                const double2 t = Tm[k + a * ne], x = V[k + c * ne];
+                const double2 t = Tm[k + a * lda], x = V[k + c * lda];
                 mr += t.x * x.x + t.y * x.y;
                 mi += t.x * x.y - t.y * x.x;
             }
         }
-        A[e] = make_double2(mr, mi);
+        A[a + c * lda] = make_double2(mr, mi);
     }
     __syncthreads();
     for (int e = tid; e < r * r; e += EIGH_NT) {  // Hermitian part, each pair by one thread
         const int a = e % r, c = e / r;
         if (a > c) continue;
         if (a == c) {
-            A[a + a * ne].y = 0.0;
+            A[a + a * lda].y = 0.0;
             continue;
         }
-        const double2 x = A[a + c * ne], y = A[c + a * ne];
+        const double2 x = A[a + c * lda], y = A[c + a * lda];
         const double2 h = make_double2(0.5 * (x.x + y.x), 0.5 * (x.y - y.y));
-        A[a + c * ne] = h;
-        A[c + a * ne] = make_double2(h.x, -h.y);
+        A[a + c * lda] = h;
+        A[c + a * lda] = make_double2(h.x, -h.y);
     }
     for (int e = tid; e < ne * ne; e += EIGH_NT) {
         const int i = e % ne, j = e / ne;
-        V[e] = make_double2(i == j && i < re ? 1.0 : 0.0, 0.0);
+        V[i + j * lda] = make_double2(i == j && i < re ? 1.0 : 0.0, 0.0);
     }
     __syncthreads();
     // 4. M = Y diag(theta) Y^H
-    const int sw = jacobi_lds(A, V, ne, re, JacobiScratch{rs, rc, rp, rq, rf}, &nrot, &maxratio_bits);
+    const int sw = jacobi_lds(A, V, lda, re, rot, red, RR_STOP_RATIO);
     if (tid == 0 && w.sweeps_out) {
         atomicAdd(w.sweeps_out + 3, sw);
         atomicMax(w.sweeps_out + 5, sw);
     }
-    for (int i = tid; i < r; i += EIGH_NT) wsh[i] = A[i + i * ne].x;
+    for (int i = tid; i < r; i += EIGH_NT) wsh[i] = A[i + i * lda].x;
     __syncthreads();
     for (int i = tid; i < r; i += EIGH_NT) {
         const double wi = wsh[i];
@@ -1016,7 +1060,7 @@ __global__ void __launch_bounds__(1024) rr_kernel(const RRWork *__restrict__ wor
         if (col >= cx + cw) continue;
         double cr = 0.0, ci = 0.0;
         for (int j = 0; j < r; ++j) {
-            const double2 t = Tm[k + j * ne], y = V[j + i * ne];
+            const double2 t = Tm[k + j * lda], y = V[j + i * lda];
             cr += t.x * y.x - t.y * y.y;
             ci += t.x * y.y + t.y * y.x;
         }
@@ -1036,11 +1080,14 @@ void launch_rr(const RRWork *dwork, int nwork, int maxn, hipStream_t st) {
     if (maxn > GRAM_MAXC) throw std::runtime_error("Rayleigh-Ritz dimension above 48");
     static const bool attr_set = [] {
         BH_HIP_CHECK(hipFuncSetAttribute((const void *)rr_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                         (int)rr_shared_bytes(GRAM_MAXC)));
+                                         (int)std::max(rr_shared_bytes(GRAM_MAXC), (size_t)150 * 1024)));
         return true;
     }();
     (void)attr_set;
-    hipLaunchKernelGGL(rr_kernel, dim3(nwork), dim3(EIGH_NT), rr_shared_bytes(std::max(maxn, 1)), st, dwork);
+    // LDS request >= 150 KB: no other kernel's workgroups share the CU while a
+    // problem runs, so the latency-bound rounds are not slowed by co-resident waves
+    const size_t sh = std::max(rr_shared_bytes(std::max(maxn, 1)), (size_t)150 * 1024);
+    hipLaunchKernelGGL(rr_kernel, dim3(nwork), dim3(EIGH_NT), sh, st, dwork);
     BH_HIP_CHECK(hipGetLastError());
 }
 

@@ -61,7 +61,7 @@ struct EngineOptions {
     int slots = 64;         // concurrent band structures
     int precision = 0;      // 0 fp64, 1 fp32 storage (reserved)
     int profile = 0;        // time operator kernels with events (forces one group)
-    int groups = 2;         // slot groups pipelined on separate streams
+    int groups = 4;         // slot groups pipelined on separate streams
     int host_threads = 0;   // host threads for per-slot planning (0: hardware threads, at most 16)
     int timed_kernel = 0;   // TIMED_* class whose launches are timed with events (0: none)
     // Streams owned by the caller.  HIP gives each new stream its own hardware
@@ -69,7 +69,7 @@ struct EngineOptions {
     // serialises the groups; a caller that creates the group streams early
     // (before other streams and before any null-stream use) keeps them apart.
     hipStream_t setup_stream = nullptr;
-    hipStream_t group_streams[3] = {nullptr, nullptr, nullptr};
+    hipStream_t group_streams[4] = {nullptr, nullptr, nullptr, nullptr};
 };
 
 class EngineBase {

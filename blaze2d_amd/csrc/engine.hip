@@ -270,7 +270,7 @@ class Engine : public EngineBase {
             G.s0 = g * slots_per_group;
             G.s1 = std::min(S, G.s0 + slots_per_group);
             const int gs = G.s1 - G.s0;
-            if (g < 3 && opt.group_streams[g]) G.st = opt.group_streams[g];
+            if (g < 4 && opt.group_streams[g]) G.st = opt.group_streams[g];
             else { BH_HIP_CHECK(hipStreamCreate(&G.st)); G.own_st = true; }
             BH_HIP_CHECK(hipEventCreateWithFlags(&G.ev, hipEventDisableTiming));
             // per slot and step: dielectric tables of a new job plus descriptors / coefficients

@@ -139,7 +139,7 @@ typedef struct {
     int slots;               /* concurrent band structures on the GPU (0 -> 64) */
     int precision;           /* 0 FP64 */
     int profile;             /* 1: time operator kernels with HIP events (runs one slot group) */
-    int groups;              /* slot groups pipelined on separate HIP streams (0 -> 2) */
+    int groups;              /* slot groups pipelined on separate HIP streams (0 -> 4) */
     int timed_kernel;        /* kernel class timed with HIP events on its stream:
                                 0 none, 1 Theta operator, 2 dual Rayleigh-Ritz Gram */
 } bh_run_options;
