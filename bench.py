@@ -8,8 +8,9 @@ step = one batched pass over the whole sweep with its inputs resident on the
 device; the clock brackets exactly --steps passes after --warmup untimed ones.
 
 Also reported on the same line:
-  roofline      one kernel class (--timed-kernel: 2 = the dual Rayleigh-Ritz
-                Gram, 1 = the Theta operator) timed with HIP events on the
+  roofline      one kernel class (--timed-kernel: 3 = the Rayleigh-Ritz solve,
+                the dominant kernel; 2 = the dual Rayleigh-Ritz Gram; 1 = the
+                Theta operator) timed with HIP events on the
                 stream each launch runs on, during the timed steps themselves:
                 algorithmic bytes (and FLOPs) per launch over the average
                 launch time, against 8 TB/s HBM (and 78.6 TFLOP/s FP64);
@@ -37,8 +38,13 @@ sys.path.insert(0, ROOT)
 METRIC = "converged k-point solves/sec, res64 12 bands, ≤1e-6 rel freq err vs FP64"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+# HBM bytes per rr_kernel launch on the default workload (slots 512, groups 4),
+# FETCH_SIZE x2 + WRITE_SIZE x1 from two rocprofv3 --pmc passes scaled by a plain
+# copy (profiles/rr_kernel_traffic.md); reported only for that configuration.
+RR_TRAFFIC_BYTES = 18.68e6
 TIMED_NAMES = {1: "op_kernel THETA_TM/TE (64x64 fused 2D FFT operator)",
-               2: "gram2_kernel (dual Rayleigh-Ritz Gram Z^H BZ, Z^H AZ)"}
+               2: "gram2_kernel (dual Rayleigh-Ritz Gram Z^H BZ, Z^H AZ)",
+               3: "rr_kernel (Cholesky/SVQB + projection + Jacobi Rayleigh-Ritz, FP64)"}
 
 
 def solves_of(jobs):
@@ -68,12 +74,24 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=4, help="band structures timed on the CPU oracle")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip roofline / plain-GPU extra passes")
-    ap.add_argument("--timed-kernel", type=int, default=2, choices=(1, 2),
-                    help="kernel class timed for the roofline: 1 Theta operator, 2 dual Rayleigh-Ritz Gram")
+    ap.add_argument("--timed-kernel", type=int, default=3, choices=(1, 2, 3),
+                    help="kernel class timed for the roofline: 1 Theta operator, 2 dual Rayleigh-Ritz Gram, "
+                         "3 Rayleigh-Ritz solve (the dominant kernel)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 512 MiB device-to-device copy five times (scale for PMC byte counters)")
     ap.add_argument("--exit-rule", type=int, default=1,
                     help="1: a solve ends when every band is soft-locked (all bands converged); "
                          "0: the reference solve() exit accounting")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        src = torch.ones(64 << 20, dtype=torch.float64, device="cuda:0")  # 512 MiB
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": src.numel() * 8, "copy_bytes_written": src.numel() * 8, "copies": 5}))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
               "(band structures shard independently across GPUs, see DESIGN.md)", file=sys.stderr)
@@ -158,7 +176,8 @@ def main():
             "peak": HBM_PEAK_GBS if hbm else FP64_PEAK_TFLOPS,
             "unit": "GB/s" if hbm else "TFLOP/s",
             "frac": round(gbs / HBM_PEAK_GBS if hbm else tfs / FP64_PEAK_TFLOPS, 5),
-            "traffic": None,
+            "traffic": (RR_TRAFFIC_BYTES if (args.timed_kernel == 3 and args.slots == 512 and args.groups == 4
+                                              and args.geoms == 256 and args.precision == 0) else None),
             "kernel": TIMED_NAMES[args.timed_kernel],
             "avg_launch_ms": round(avg_s * 1e3, 4),
             "launches": timed["launches"],

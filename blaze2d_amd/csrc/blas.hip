@@ -809,7 +809,7 @@ __global__ void __launch_bounds__(1024) rr_kernel(const RRWork *__restrict__ wor
     const int n = w.n;
     const int tid = threadIdx.x;
     if (n <= 0) {
-        if (tid == 0) { w.cnt[0] = 0; w.cnt[1] = 0; w.cnt[2] = 0; }
+        if (tid == 0) { w.cnt[0] = 0; w.cnt[1] = 0; w.cnt[2] = 0; w.cnt[3] = 0; }
         return;
     }
     const int ne = n + (n & 1);
@@ -825,6 +825,7 @@ __global__ void __launch_bounds__(1024) rr_kernel(const RRWork *__restrict__ wor
     double *piv = wsh + ne;
     int *rank_of = reinterpret_cast<int *>(piv + ne);
     __shared__ int rank, chol_ok;
+    int rr_fallback_sweeps = 0;  // Jacobi sweeps of the SVQB fallback (thread 0's copy is reported)
     __shared__ double lmax;
     __shared__ double red[1];
     const int ld = w.ld;
@@ -951,7 +952,10 @@ __global__ void __launch_bounds__(1024) rr_kernel(const RRWork *__restrict__ wor
             V[i + j * lda] = make_double2(i == j ? 1.0 : 0.0, 0.0);
         }
         __syncthreads();
-        const int sw = jacobi_lds(A, V, lda, ne, rot, red);
+        // the eigenvalues only decide which directions are kept (1e-12 of the
+        // largest) and scale the kept ones: the Rayleigh-Ritz stop ratio suffices
+        const int sw = jacobi_lds(A, V, lda, ne, rot, red, RR_STOP_RATIO);
+        rr_fallback_sweeps = sw;
         for (int i = tid; i < n; i += EIGH_NT) wsh[i] = A[i + i * lda].x;
         __syncthreads();
         for (int i = tid; i < n; i += EIGH_NT) {
@@ -1072,6 +1076,7 @@ __global__ void __launch_bounds__(1024) rr_kernel(const RRWork *__restrict__ wor
         w.cnt[0] = cx;
         w.cnt[1] = cw;
         w.cnt[2] = cx + cw;
+        w.cnt[3] = sw + rr_fallback_sweeps;
     }
 }
 

@@ -198,6 +198,7 @@ struct Slot {
     // per-solve state
     int it = 0, m = 0, nW = 0;
     int par = 0;  // current copy of V / BV / AV
+    int last_p = 0;  // subspace width of the iteration in flight
     std::vector<char> soft;
     std::vector<double> lam, lam_prev;
     bool lam_prev_valid = false;
@@ -258,9 +259,9 @@ class Engine : public EngineBase {
         d_lam = dalloc<double>((size_t)S * LD, st);
         d_Y = dalloc<double>((size_t)S * LD * LD * 2, st);
         d_coef = dalloc<double>((size_t)S * LD * LD * 2, st);
-        d_cnt = dalloc<int>(3 * (size_t)S, st);
+        d_cnt = dalloc<int>(4 * (size_t)S, st);
         BH_HIP_CHECK(hipHostMalloc((void **)&h_lam, sizeof(double) * S * LD, hipHostMallocDefault));
-        BH_HIP_CHECK(hipHostMalloc((void **)&h_cnt, sizeof(int) * 3 * S, hipHostMallocDefault));
+        BH_HIP_CHECK(hipHostMalloc((void **)&h_cnt, sizeof(int) * 4 * S, hipHostMallocDefault));
         int ng = opt.profile ? 1 : std::max(1, std::min(opt.groups, S));
         slots_per_group = (S + ng - 1) / ng;
         ng = (S + slots_per_group - 1) / slots_per_group;
@@ -373,6 +374,14 @@ class Engine : public EngineBase {
     }
     double theta_flops_per_col(int pol, double n_log) const {
         return pol == 1 ? 30.0 * n_log + 40.0 * N : 10.0 * n_log + 2.0 * N;
+    }
+    // FLOPs of one Rayleigh-Ritz problem of order n (complex multiply-add = 8):
+    // Cholesky n^3/6, triangular inverse n^3/6, G2 T and T^H X n^3/2 each,
+    // cyclic Jacobi 4.5 n^3 per sweep (n^2/2 rotations of two rows and two
+    // columns of A and two columns of V), back-transform T Y n^2 k / 2
+    static double rr_flops(int n, int k, int sweeps) {
+        const double n3 = (double)n * n * n;
+        return 8.0 * (n3 / 3.0 + n3 + 4.5 * n3 * sweeps + 0.5 * (double)n * n * k);
     }
 
     // --- per-step plans -----------------------------------------------------
@@ -498,7 +507,8 @@ template <typename T>
 void Engine<T>::absorb(int s) {
     Slot &sl = slots[s];
     if (sl.it == 0) return;
-    int cx = h_cnt[3 * s], cw = h_cnt[3 * s + 1];
+    int cx = h_cnt[4 * s], cw = h_cnt[4 * s + 1];
+    if (eopt.timed_kernel == TIMED_RR) stats_->timed_flops += rr_flops(sl.last_p, cx + cw, h_cnt[4 * s + 3]);
     sl.lam_prev = sl.lam;
     sl.lam_prev_valid = sl.it >= 2;
     if (cx > 0) {
@@ -1123,13 +1133,14 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
         RRWork rw2{};
         rw2.lam = d_lam + (size_t)s * LD;
         rw2.C = d_Y + (size_t)s * LD * LD * 2;
-        rw2.cnt = d_cnt + 3 * s;
+        rw2.cnt = d_cnt + 4 * s;
         rw2.n = p;
         rw2.ld = LD;
         rw2.ldc = LD;
         rw2.nsplit = sl.nb;
         if (eopt.profile) rw2.sweeps_out = d_sweeps;
         e_rr2.push_back(rw2);
+        sl.last_p = p;
         // Ritz update into the other copy: [X' | W'] = Z C, A[X' | W'] = (A Z) C,
         // B[X' | W'] = eps [X' | W'] (TM)
         const int nxt = 1 - sl.par;
@@ -1137,7 +1148,7 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
         lx.A = vcol(s, 0);
         lx.a_stride = N;
         lx.na = p;
-        lx.dev_nq = d_cnt + 3 * s + 2;
+        lx.dev_nq = d_cnt + 4 * s + 2;
         lx.C = d_Y + (size_t)s * LD * LD * 2;
         lx.ldc = LD;
         lx.out = vbuf(s, nxt, 0);
@@ -1313,7 +1324,13 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
         timed(TIMED_GRAM2, bytes, flops,
               [&] { launch_gram2<T>(d_g_dual, (int)g_dual.size(), nch_dual, N, std::max(max_p, 1), st); });
     }
-    launch_rr(d_e_rr2, (int)e_rr2.size(), std::max(max_p, 1), st);
+    {
+        double bytes = 0;  // the two n x n Grams in, the n x 2nb coefficients out
+        for (auto &g2 : g_dual)
+            bytes += 2.0 * g2.n * g2.n * 16.0 + (double)g2.n * std::min(g2.n, 2 * nbmax) * 16.0;
+        // FLOPs are added when the sweep counts come back (absorb)
+        timed(TIMED_RR, bytes, 0.0, [&] { launch_rr(d_e_rr2, (int)e_rr2.size(), std::max(max_p, 1), st); });
+    }
     launch_lincomb<T>(d_lc_ritz, (int)lc_ritz.size(), N, std::max(max_ritz, 1), st);
     // T1 Grams -> compact readback
     if (!g_t1.empty()) {
@@ -1329,7 +1346,7 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
         const int gs = G.s1 - G.s0;
         BH_HIP_CHECK(hipMemcpyAsync(h_lam + (size_t)G.s0 * LD, d_lam + (size_t)G.s0 * LD, sizeof(double) * gs * LD,
                                     hipMemcpyDeviceToHost, st));
-        BH_HIP_CHECK(hipMemcpyAsync(h_cnt + 3 * (size_t)G.s0, d_cnt + 3 * (size_t)G.s0, sizeof(int) * 3 * gs,
+        BH_HIP_CHECK(hipMemcpyAsync(h_cnt + 4 * (size_t)G.s0, d_cnt + 4 * (size_t)G.s0, sizeof(int) * 4 * gs,
                                     hipMemcpyDeviceToHost, st));
     }
     BH_HIP_CHECK(hipEventRecord(G.ev, st));
