@@ -1203,7 +1203,7 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
     };
     // Gram partial buffers: item i of a launch uses partial block i
     auto nch_for = [&](size_t nwork) {
-        long c = (long)((2048 + nwork - 1) / std::max<size_t>(nwork, 1));
+        long c = (long)((1024 + nwork - 1) / std::max<size_t>(nwork, 1));
         c = std::min<long>(std::max<long>(c, 1), NCHMAX);
         return (int)c;
     };

@@ -141,7 +141,8 @@ typedef struct {
     int profile;             /* 1: time operator kernels with HIP events (runs one slot group) */
     int groups;              /* slot groups pipelined on separate HIP streams (0 -> 4) */
     int timed_kernel;        /* kernel class timed with HIP events on its stream:
-                                0 none, 1 Theta operator, 2 dual Rayleigh-Ritz Gram */
+                                0 none, 1 Theta operator, 2 dual Rayleigh-Ritz Gram,
+                                3 Rayleigh-Ritz solve (Cholesky/SVQB + projection + Jacobi) */
 } bh_run_options;
 
 typedef struct {
