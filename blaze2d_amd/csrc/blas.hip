@@ -253,11 +253,12 @@ struct alignas(16) JacobiRot {
 
 // Parallel cyclic Jacobi for the Hermitian matrix A held in LDS (col-major,
 // leading dimension ld, order ne even, padding rows/columns zero), with the
-// rotations accumulated into V.  All EIGH_NT threads of the workgroup call it;
-// rot holds 2 * ne / 2 entries (two consecutive rounds), red 64 doubles.
+// rotations accumulated into V.  All NT threads of the workgroup call it
+// (NT >= 64, ne / 2 <= 32); rot holds 2 * ne / 2 entries (two consecutive
+// rounds), red one double.
 //
 // Each round applies ne/2 disjoint rotations (round-robin pairing).  The matrix
-// is partitioned into np x np 2x2 blocks (row pair, column pair); a thread
+// is partitioned into np x np 2x2 blocks (row pair, column pair); the thread
 // owning a block applies both the column rotation of its column pair and the
 // row rotation of its row pair in registers.  The rotations of round r are
 // computed by the first np threads while the other threads apply round r-1's
@@ -267,28 +268,36 @@ struct alignas(16) JacobiRot {
 // stop_ratio^2 (quadratic convergence leaves the off-diagonal near stop_ratio^2
 // relative).  An odd leading dimension keeps the 16-byte LDS accesses free of
 // bank conflicts.  Returns the number of sweeps (at most 40).
+template <int NT>
 __device__ int jacobi_lds(double2 *A, double2 *V, int ld, int ne, JacobiRot *rot, double *red,
                           double stop_ratio = 1e-9) {
     const int tid = threadIdx.x;
     const int np = ne / 2;
-    // fixed per-thread block for every round (np*np <= 1024)
-    const bool has_blk = tid < np * np;
-    const int bt1 = has_blk ? tid / np : 0, bt2 = has_blk ? tid - bt1 * np : 0;
+    const int nblk = np * np;
     const int nv = np * ne;  // eigenvector items (pair, row)
     const double stop2 = stop_ratio * stop_ratio;
-    // V items of one round, rotations from parity slot `par`; threads [first, EIGH_NT) share them
+    // V items of one round, rotations from parity slot `par`; threads [first, NT) share them
     auto rotate_v = [&](int par, int first) {
-        for (int e = tid - first; e >= 0 && e < nv; e += EIGH_NT - first) {
-            const int t = e / ne, k = e - t * ne;
+        const int stride = NT - first;
+        int e = tid - first;
+        if (e < 0) return;
+        int t = e / ne, k = e - t * ne;
+        const int dt = stride / ne, dk = stride - dt * ne;
+        for (; e < nv; e += stride) {
             const JacobiRot R = rot[par * np + t];
-            if (((R.pk >> 16) & 3) != 1) continue;
-            const int p = R.pk & 255, q = (R.pk >> 8) & 255;
-            const double c = R.c;
-            const double2 sg = R.s;
-            const double2 vkp = V[k + p * ld], vkq = V[k + q * ld];
-            V[k + p * ld] = make_double2(c * vkp.x + sg.x * vkq.x - sg.y * vkq.y, c * vkp.y + sg.x * vkq.y + sg.y * vkq.x);
-            V[k + q * ld] = make_double2(-(sg.x * vkp.x + sg.y * vkp.y) + c * vkq.x,
-                                         -(sg.x * vkp.y - sg.y * vkp.x) + c * vkq.y);
+            if (((R.pk >> 16) & 3) == 1) {
+                const int p = R.pk & 255, q = (R.pk >> 8) & 255;
+                const double c = R.c;
+                const double2 sg = R.s;
+                const double2 vkp = V[k + p * ld], vkq = V[k + q * ld];
+                V[k + p * ld] = make_double2(c * vkp.x + sg.x * vkq.x - sg.y * vkq.y,
+                                             c * vkp.y + sg.x * vkq.y + sg.y * vkq.x);
+                V[k + q * ld] = make_double2(-(sg.x * vkp.x + sg.y * vkp.y) + c * vkq.x,
+                                             -(sg.x * vkp.y - sg.y * vkp.x) + c * vkq.y);
+            }
+            t += dt;
+            k += dk;
+            if (k >= ne) { k -= ne; ++t; }
         }
     };
     int sweep = 0;
@@ -330,8 +339,8 @@ __device__ int jacobi_lds(double2 *A, double2 *V, int ld, int ne, JacobiRot *rot
                 rotate_v(1 - par, np);  // previous round's rotations
             }
             __syncthreads();
-            do {
-                if (!has_blk) break;
+            for (int b = tid; b < nblk; b += NT) {
+                const int bt1 = b / np, bt2 = b - bt1 * np;
                 const JacobiRot R1 = rot[par * np + bt1], R2 = rot[par * np + bt2];  // row pair, column pair
                 const int f1 = (R1.pk >> 16) & 3, f2 = (R2.pk >> 16) & 3;
                 const int p1 = R1.pk & 255, q1 = (R1.pk >> 8) & 255, p2 = R2.pk & 255, q2 = (R2.pk >> 8) & 255;
@@ -340,7 +349,7 @@ __device__ int jacobi_lds(double2 *A, double2 *V, int ld, int ne, JacobiRot *rot
                         A[p1 + q1 * ld] = make_double2(0.0, 0.0);
                         A[q1 + p1 * ld] = make_double2(0.0, 0.0);
                     }
-                    break;
+                    continue;
                 }
                 double2 a11 = A[p1 + p2 * ld], a12 = A[p1 + q2 * ld];
                 double2 a21 = A[q1 + p2 * ld], a22 = A[q1 + q2 * ld];
@@ -372,7 +381,7 @@ __device__ int jacobi_lds(double2 *A, double2 *V, int ld, int ne, JacobiRot *rot
                 A[p1 + q2 * ld] = a12;
                 A[q1 + p2 * ld] = a21;
                 A[q1 + q2 * ld] = a22;
-            } while (false);
+            }
             __syncthreads();
         }
         rotate_v((ne - 2) & 1, 0);  // the sweep's last round
@@ -403,7 +412,9 @@ __global__ void __launch_bounds__(1024) eigh_kernel(const EighWork *__restrict__
     }
     const int ne = n + (n & 1);
     const int np = ne / 2;
-    extern __shared__ double sm[];
+    // 16-byte alignment: the array follows the static LDS variables, and the
+    // double2 accesses below need it for full-rate 16-byte LDS operations
+    extern __shared__ __attribute__((aligned(16))) double sm[];
     double2 *A = reinterpret_cast<double2 *>(sm);  // col-major [i + j*ne]
     double2 *V = A + ne * ne;
     JacobiRot *rot = reinterpret_cast<JacobiRot *>(V + ne * ne);  // two rounds of rotations
@@ -516,7 +527,7 @@ __global__ void __launch_bounds__(1024) eigh_kernel(const EighWork *__restrict__
         }
         __syncthreads();
     }
-    const int sweeps = jacobi_lds(A, V, ne, ne, rot, red);
+    const int sweeps = jacobi_lds<EIGH_NT>(A, V, ne, ne, rot, red);
     if (tid == 0 && w.sweeps_out) {
         atomicAdd(w.sweeps_out, sweeps);
         atomicMax(w.sweeps_out + 2, sweeps);  // maximum over the launch
@@ -644,7 +655,7 @@ __global__ void __launch_bounds__(256) gram2_kernel(const Gram2Work<T> *__restri
     const Gram2Work<T> &w = work[blockIdx.y];
     const int n = w.n;
     if (n <= 0) return;
-    extern __shared__ double2 g2s[];
+    extern __shared__ __attribute__((aligned(16))) double2 g2s[];
     double2 *Zs = g2s;                       // [SR][COLS]
     double2 *Bs = Zs + S::SR * S::COLS;
     double2 *As = Bs + S::SR * S::COLS;
@@ -804,7 +815,15 @@ size_t rr_shared_bytes(int maxn) {
 // far below what the eigenvalues (error ~ off^2 / gap) or the Ritz vectors need.
 constexpr double RR_STOP_RATIO = 1e-6;
 
-__global__ void __launch_bounds__(1024) rr_kernel(const RRWork *__restrict__ work) {
+// Threads per Rayleigh-Ritz problem (fewer threads measured slower per problem:
+// 0.25 / 0.39 / 0.48 / 0.67 / 1.14 ms for 1024 / 512 / 256 / 128 / 64 threads,
+// 128 problems of order 36, 4 sweeps), and the LDS each workgroup requests so
+// that no other kernel's workgroups share its CU.
+constexpr int RR_NT = 1024;
+constexpr size_t RR_LDS_RESERVE = (size_t)150 * 1024;
+
+template <int NT>
+__global__ void __launch_bounds__(NT) rr_kernel(const RRWork *__restrict__ work) {
     const RRWork &w = work[blockIdx.x];
     const int n = w.n;
     const int tid = threadIdx.x;
@@ -815,7 +834,9 @@ __global__ void __launch_bounds__(1024) rr_kernel(const RRWork *__restrict__ wor
     const int ne = n + (n & 1);
     const int np = ne / 2;
     const int lda = ne + 1;  // odd: conflict-free 16-byte LDS accesses
-    extern __shared__ double rsm[];
+    // 16-byte alignment: the array follows the static LDS variables, and the
+    // double2 accesses below need it for full-rate 16-byte LDS operations
+    extern __shared__ __attribute__((aligned(16))) double rsm[];
     double2 *A = reinterpret_cast<double2 *>(rsm);  // col-major [i + j*lda]
     double2 *V = A + lda * ne;
     double2 *Tm = V + lda * ne;
@@ -830,7 +851,7 @@ __global__ void __launch_bounds__(1024) rr_kernel(const RRWork *__restrict__ wor
     __shared__ double red[1];
     const int ld = w.ld;
     auto load = [&](const double *partial) {  // Hermitian n x n from lower partial sums, zero padding
-        for (int e = tid; e < ne * ne; e += EIGH_NT) {
+        for (int e = tid; e < ne * ne; e += NT) {
             const int i = e % ne, j = e / ne;
             double2 v = make_double2(0.0, 0.0);
             if (i < n && j < n) {
@@ -850,13 +871,13 @@ __global__ void __launch_bounds__(1024) rr_kernel(const RRWork *__restrict__ wor
     // 1. scaled G1
     load(w.partial1);
     __syncthreads();
-    for (int i = tid; i < ne; i += EIGH_NT) {
+    for (int i = tid; i < ne; i += NT) {
         double nrm = i < n ? sqrt(fmax(A[i + i * lda].x, 0.0)) : 0.0;
         D[i] = nrm > 1e-30 ? 1.0 / nrm : 1.0;
     }
     if (tid == 0) chol_ok = 1;
     __syncthreads();
-    for (int e = tid; e < ne * ne; e += EIGH_NT) {
+    for (int e = tid; e < ne * ne; e += NT) {
         const int i = e % ne, j = e / ne;
         const double s = D[i] * D[j];
         const double2 a = make_double2(A[i + j * lda].x * s, A[i + j * lda].y * s);
@@ -871,21 +892,28 @@ __global__ void __launch_bounds__(1024) rr_kernel(const RRWork *__restrict__ wor
     // step) and applies column j-1 to the trailing block.  The diagonal of L is
     // piv; its LDS entries keep the partially updated values.
     double2 *L = V;
-    // lower-triangle entries (i, k), k <= i, owned by this thread (at most two for n <= 48)
-    int own_i[2] = {-1, -1}, own_k[2] = {0, 0};
+    // lower-triangle entries (i, k), k <= i, owned by this thread: positions
+    // tid, tid + NT, ... of the row-order triangle (n(n+1)/2 <= 1176 for n <= 48)
+    constexpr int MAXE = (1176 + NT - 1) / NT;
+    int own_i[MAXE], own_k[MAXE];
     {
         const int ntri = n * (n + 1) / 2;
-        for (int u = 0; u < 2; ++u) {
-            const int e = tid + u * EIGH_NT;
-            if (e >= ntri) break;
-            int i = 0;
-            while ((i + 1) * (i + 2) / 2 <= e) ++i;
+#pragma unroll
+        for (int u = 0; u < MAXE; ++u) {
+            const int e = tid + u * NT;
+            int i = -1, k = 0;
+            if (e < ntri) {
+                i = 0;
+                while ((i + 1) * (i + 2) / 2 <= e) ++i;
+                k = e - i * (i + 1) / 2;
+            }
             own_i[u] = i;
-            own_k[u] = e - i * (i + 1) / 2;
+            own_k[u] = k;
         }
     }
     for (int j = 0; j < n; ++j) {
-        for (int u = 0; u < 2; ++u) {
+#pragma unroll
+        for (int u = 0; u < MAXE; ++u) {
             const int i = own_i[u], k = own_k[u];
             if (i < 0 || k < j) continue;
             double2 a = L[i + k * lda];
@@ -917,14 +945,15 @@ __global__ void __launch_bounds__(1024) rr_kernel(const RRWork *__restrict__ wor
         // M = L^-1, right-looking: step i finalises row i of M (M[i][c] =
         // B[i][c] / piv[i]) and removes it from the rows below; B starts as I
         // in A.  T = D M^H is written as rows finish: T[c][i] = D[c] conj(M[i][c]).
-        for (int e = tid; e < ne * ne; e += EIGH_NT) {
+        for (int e = tid; e < ne * ne; e += NT) {
             const int i = e % ne, j = e / ne;
             A[i + j * lda] = make_double2(i == j ? 1.0 : 0.0, 0.0);
         }
         __syncthreads();
         for (int i = 0; i < n; ++i) {
             const double inv = 1.0 / piv[i];
-            for (int u = 0; u < 2; ++u) {
+#pragma unroll
+            for (int u = 0; u < MAXE; ++u) {
                 const int k = own_i[u], c = own_k[u];  // rows k >= i, columns c <= i
                 if (k < i || c > i) continue;
                 const double2 b = A[i + c * lda];
@@ -947,18 +976,18 @@ __global__ void __launch_bounds__(1024) rr_kernel(const RRWork *__restrict__ wor
         }
     } else {
         // SVQB: eigendecomposition of the scaled G1 (A still holds it)
-        for (int e = tid; e < ne * ne; e += EIGH_NT) {
+        for (int e = tid; e < ne * ne; e += NT) {
             const int i = e % ne, j = e / ne;
             V[i + j * lda] = make_double2(i == j ? 1.0 : 0.0, 0.0);
         }
         __syncthreads();
         // the eigenvalues only decide which directions are kept (1e-12 of the
         // largest) and scale the kept ones: the Rayleigh-Ritz stop ratio suffices
-        const int sw = jacobi_lds(A, V, lda, ne, rot, red, RR_STOP_RATIO);
+        const int sw = jacobi_lds<NT>(A, V, lda, ne, rot, red, RR_STOP_RATIO);
         rr_fallback_sweeps = sw;
-        for (int i = tid; i < n; i += EIGH_NT) wsh[i] = A[i + i * lda].x;
+        for (int i = tid; i < n; i += NT) wsh[i] = A[i + i * lda].x;
         __syncthreads();
-        for (int i = tid; i < n; i += EIGH_NT) {
+        for (int i = tid; i < n; i += NT) {
             const double wi = wsh[i];
             int rk = 0;
             for (int j = 0; j < n; ++j) rk += (wsh[j] < wi || (wsh[j] == wi && j < i)) ? 1 : 0;
@@ -983,7 +1012,7 @@ __global__ void __launch_bounds__(1024) rr_kernel(const RRWork *__restrict__ wor
         }
         __syncthreads();
         // kept directions are the first `rank` of the descending order
-        for (int e = tid; e < n * n; e += EIGH_NT) {
+        for (int e = tid; e < n * n; e += NT) {
             const int row = e % n, i = e / n;
             const int pos = n - 1 - rank_of[i];
             if (pos >= rank) continue;
@@ -998,7 +1027,7 @@ __global__ void __launch_bounds__(1024) rr_kernel(const RRWork *__restrict__ wor
     // 3. G2 into A, X = G2 T into V, M = T^H X into A (zero padded to re)
     load(w.partial2);
     __syncthreads();
-        for (int e = tid; e < n * r; e += EIGH_NT) {
+        for (int e = tid; e < n * r; e += NT) {
         const int i = e % n, c = e / n;
         double xr = 0.0, xi = 0.0;
         for (int k = 0; k < n; ++k) {
@@ -1009,7 +1038,7 @@ __global__ void __launch_bounds__(1024) rr_kernel(const RRWork *__restrict__ wor
         V[i + c * lda] = make_double2(xr, xi);
     }
     __syncthreads();
-    for (int e = tid; e < ne * ne; e += EIGH_NT) {
+    for (int e = tid; e < ne * ne; e += NT) {
         const int a = e % ne, c = e / ne;
         double mr = 0.0, mi = 0.0;
         if (a < r && c < r) {
@@ -1022,7 +1051,7 @@ __global__ void __launch_bounds__(1024) rr_kernel(const RRWork *__restrict__ wor
         A[a + c * lda] = make_double2(mr, mi);
     }
     __syncthreads();
-    for (int e = tid; e < r * r; e += EIGH_NT) {  // Hermitian part, each pair by one thread
+    for (int e = tid; e < r * r; e += NT) {  // Hermitian part, each pair by one thread
         const int a = e % r, c = e / r;
         if (a > c) continue;
         if (a == c) {
@@ -1034,20 +1063,20 @@ __global__ void __launch_bounds__(1024) rr_kernel(const RRWork *__restrict__ wor
         A[a + c * lda] = h;
         A[c + a * lda] = make_double2(h.x, -h.y);
     }
-    for (int e = tid; e < ne * ne; e += EIGH_NT) {
+    for (int e = tid; e < ne * ne; e += NT) {
         const int i = e % ne, j = e / ne;
         V[i + j * lda] = make_double2(i == j && i < re ? 1.0 : 0.0, 0.0);
     }
     __syncthreads();
     // 4. M = Y diag(theta) Y^H
-    const int sw = jacobi_lds(A, V, lda, re, rot, red, RR_STOP_RATIO);
+    const int sw = jacobi_lds<NT>(A, V, lda, re, rot, red, RR_STOP_RATIO);
     if (tid == 0 && w.sweeps_out) {
         atomicAdd(w.sweeps_out + 3, sw);
         atomicMax(w.sweeps_out + 5, sw);
     }
-    for (int i = tid; i < r; i += EIGH_NT) wsh[i] = A[i + i * lda].x;
+    for (int i = tid; i < r; i += NT) wsh[i] = A[i + i * lda].x;
     __syncthreads();
-    for (int i = tid; i < r; i += EIGH_NT) {
+    for (int i = tid; i < r; i += NT) {
         const double wi = wsh[i];
         int rk = 0;
         for (int j = 0; j < r; ++j) rk += (wsh[j] < wi || (wsh[j] == wi && j < i)) ? 1 : 0;
@@ -1058,7 +1087,7 @@ __global__ void __launch_bounds__(1024) rr_kernel(const RRWork *__restrict__ wor
     // 5. C = T Y for the first cx + cw Ritz vectors
     const int cx = r < w.nsplit ? r : w.nsplit;
     const int cw = (2 * w.nsplit < r ? 2 * w.nsplit : r) - cx;
-    for (int e = tid; e < n * r; e += EIGH_NT) {
+    for (int e = tid; e < n * r; e += NT) {
         const int k = e % n, i = e / n;
         const int col = rank_of[i];
         if (col >= cx + cw) continue;
@@ -1084,15 +1113,16 @@ void launch_rr(const RRWork *dwork, int nwork, int maxn, hipStream_t st) {
     if (nwork <= 0) return;
     if (maxn > GRAM_MAXC) throw std::runtime_error("Rayleigh-Ritz dimension above 48");
     static const bool attr_set = [] {
-        BH_HIP_CHECK(hipFuncSetAttribute((const void *)rr_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                         (int)std::max(rr_shared_bytes(GRAM_MAXC), (size_t)150 * 1024)));
+        BH_HIP_CHECK(hipFuncSetAttribute((const void *)rr_kernel<RR_NT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         (int)std::max(rr_shared_bytes(GRAM_MAXC), RR_LDS_RESERVE)));
         return true;
     }();
     (void)attr_set;
-    // LDS request >= 150 KB: no other kernel's workgroups share the CU while a
-    // problem runs, so the latency-bound rounds are not slowed by co-resident waves
-    const size_t sh = std::max(rr_shared_bytes(std::max(maxn, 1)), (size_t)150 * 1024);
-    hipLaunchKernelGGL(rr_kernel, dim3(nwork), dim3(EIGH_NT), sh, st, dwork);
+    // The LDS request leaves no room for other kernels' workgroups on a CU
+    // that runs problems, so the latency-bound rounds are not slowed by
+    // co-resident waves.
+    const size_t sh = std::max(rr_shared_bytes(std::max(maxn, 1)), RR_LDS_RESERVE);
+    hipLaunchKernelGGL(rr_kernel<RR_NT>, dim3(nwork), dim3(RR_NT), sh, st, dwork);
     BH_HIP_CHECK(hipGetLastError());
 }
 

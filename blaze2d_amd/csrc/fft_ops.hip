@@ -209,7 +209,7 @@ __global__ void __launch_bounds__(NT) op_kernel(const OpWork<T> *__restrict__ wo
                                                 const cplx<T> *__restrict__ twy) {
     constexpr int N = NX * NY;
     constexpr int NF = (MODE == MODE_THETA_TM || MODE == MODE_PREC_FD) ? 1 : 2;
-    extern __shared__ char smem_raw[];
+    extern __shared__ __attribute__((aligned(16))) char smem_raw[];
     cplx<T> *F = reinterpret_cast<cplx<T> *>(smem_raw);
     const OpWork<T> &w = work[blockIdx.y];
     const int count = w.dev_count ? *w.dev_count : w.count;
@@ -325,7 +325,7 @@ template <typename T, int NX, int NY, int NT>
 __global__ void __launch_bounds__(NT) fft_kernel(const OpWork<T> *__restrict__ work, const cplx<T> *__restrict__ twx,
                                                  const cplx<T> *__restrict__ twy, int inverse) {
     constexpr int N = NX * NY;
-    extern __shared__ char smem_raw[];
+    extern __shared__ __attribute__((aligned(16))) char smem_raw[];
     cplx<T> *F = reinterpret_cast<cplx<T> *>(smem_raw);
     const OpWork<T> &w = work[blockIdx.y];
     const int count = w.dev_count ? *w.dev_count : w.count;
