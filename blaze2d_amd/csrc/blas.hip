@@ -151,12 +151,12 @@ void launch_reduce(const ReduceWork *dwork, int nwork, int ld, hipStream_t st) {
 }
 
 // ---------------------------------------------------------------------------
-// Linear combination: grid = (row blocks, nwork, output groups of QG).
+// Linear combination: grid = (row blocks, nwork, output groups of QG); one row
+// per thread, QG outputs in registers.
 // ---------------------------------------------------------------------------
-constexpr int LC_QG = 16;
 constexpr int LC_NT = 256;
 
-template <typename T>
+template <typename T, int LC_QG>
 __global__ void __launch_bounds__(LC_NT) lincomb_kernel(const LincombWork<T> *__restrict__ work, long nrows) {
     const LincombWork<T> &w = work[blockIdx.y];
     const int na = w.dev_na ? *w.dev_na : w.na;
@@ -164,7 +164,7 @@ __global__ void __launch_bounds__(LC_NT) lincomb_kernel(const LincombWork<T> *__
     const int q0 = blockIdx.z * LC_QG;
     if (q0 >= nq) return;
     const int qn = nq - q0 < LC_QG ? nq - q0 : LC_QG;
-    __shared__ double Cs[2 * GRAM_MAXC * LC_QG];
+    __shared__ double2 Cs[GRAM_MAXC * LC_QG];  // [input i][output j], contiguous in j
     for (int e = threadIdx.x; e < na * LC_QG; e += LC_NT) {
         int i = e % na, j = e / na;
         double cr = 0, ci = 0;
@@ -172,8 +172,7 @@ __global__ void __launch_bounds__(LC_NT) lincomb_kernel(const LincombWork<T> *__
             cr = w.C[2 * (i + (size_t)(q0 + j) * w.ldc)];
             ci = w.C[2 * (i + (size_t)(q0 + j) * w.ldc) + 1];
         }
-        Cs[2 * (j * GRAM_MAXC + i)] = cr;
-        Cs[2 * (j * GRAM_MAXC + i) + 1] = ci;
+        Cs[i * LC_QG + j] = make_double2(cr, ci);
     }
     __syncthreads();
     const long row = (long)blockIdx.x * LC_NT + threadIdx.x;
@@ -187,14 +186,14 @@ __global__ void __launch_bounds__(LC_NT) lincomb_kernel(const LincombWork<T> *__
         double xr = (double)a.x, xi = (double)a.y;
 #pragma unroll
         for (int j = 0; j < LC_QG; ++j) {
-            double cr = Cs[2 * (j * GRAM_MAXC + i)], ci = Cs[2 * (j * GRAM_MAXC + i) + 1];
-            accr[j] += xr * cr - xi * ci;
-            acci[j] += xr * ci + xi * cr;
+            const double2 c = Cs[i * LC_QG + j];
+            accr[j] += xr * c.x - xi * c.y;
+            acci[j] += xr * c.y + xi * c.x;
         }
     }
 #pragma unroll
     for (int j = 0; j < LC_QG; ++j) {
-        if (j >= qn) break;
+        if (j >= qn) continue;  // not break: keeps the loop unrolled and accr/acci in registers
         const long oc = q0 + j;
         double vr = accr[j], vi = acci[j];
         if (w.mode == 1) {
@@ -214,8 +213,11 @@ __global__ void __launch_bounds__(LC_NT) lincomb_kernel(const LincombWork<T> *__
 template <typename T>
 void launch_lincomb(const LincombWork<T> *dwork, int nwork, long rows, int maxq, hipStream_t st) {
     if (nwork <= 0 || maxq <= 0) return;
-    dim3 grid((unsigned)((rows + LC_NT - 1) / LC_NT), nwork, (maxq + LC_QG - 1) / LC_QG);
-    hipLaunchKernelGGL((lincomb_kernel<T>), grid, dim3(LC_NT), 0, st, dwork, rows);
+    const unsigned rb = (unsigned)((rows + LC_NT - 1) / LC_NT);
+    // 16 outputs per thread: a 32-output variant (one read of each input for
+    // up to 32 outputs) measured slower, its register use leaves too few
+    // waves to hide the column loads
+    hipLaunchKernelGGL((lincomb_kernel<T, 16>), dim3(rb, nwork, (maxq + 15) / 16), dim3(LC_NT), 0, st, dwork, rows);
     BH_HIP_CHECK(hipGetLastError());
 }
 
