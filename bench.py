@@ -2,7 +2,8 @@
 """Throughput of the batched band-solve path on one MI355X.
 
 Metric: converged k-point solves per second on the res-64 sweep (the seeded
-synthetic geometry generator fixed at 64 x 64 and 12 bands, TM and TE, 31-point
+synthetic geometry generator fixed at 64 x 64 and 12 bands, 2048 geometries as
+in the batched-sweep configuration, TM and TE -> 4096 band structures, 31-point
 closed paths -> 30 solves per band structure, LOBPCG tolerance 1e-7).  One
 step = one batched pass over the whole sweep with its inputs resident on the
 device; the clock brackets exactly --steps passes after --warmup untimed ones.
@@ -42,9 +43,11 @@ FP64_PEAK_TFLOPS = 78.6
 # FETCH_SIZE x2 + WRITE_SIZE x1 from two rocprofv3 --pmc passes scaled by a plain
 # copy (profiles/rr_kernel_traffic.md); reported only for that configuration.
 RR_TRAFFIC_BYTES = 18.68e6
+RR_TRAFFIC_CONFIG = (512, 4, 256)  # slots, groups, geometries the counters were read on
 TIMED_NAMES = {1: "op_kernel THETA_TM/TE (64x64 fused 2D FFT operator)",
                2: "gram2_kernel (dual Rayleigh-Ritz Gram Z^H BZ, Z^H AZ)",
-               3: "rr_kernel (Cholesky/SVQB + projection + Jacobi Rayleigh-Ritz, FP64)"}
+               3: "rr_kernel (Cholesky/SVQB + projection + Jacobi Rayleigh-Ritz, FP64)",
+               4: "lincomb_kernel (Ritz update [X'|W'] = Z C, A Z C)"}
 
 
 def solves_of(jobs):
@@ -66,17 +69,17 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--geoms", type=int, default=256, help="geometries in the sweep (x2 polarisations)")
-    ap.add_argument("--slots", type=int, default=512, help="concurrent band structures (0 = all)")
+    ap.add_argument("--geoms", type=int, default=2048, help="geometries in the sweep (x2 polarisations)")
+    ap.add_argument("--slots", type=int, default=1024, help="concurrent band structures (0 = all)")
     ap.add_argument("--groups", type=int, default=4, help="slot groups pipelined on separate streams")
     ap.add_argument("--precision", type=int, default=0)
     ap.add_argument("--seed", type=int, default=0xB1A2E2D)
     ap.add_argument("--cpu-sample", type=int, default=4, help="band structures timed on the CPU oracle")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip roofline / plain-GPU extra passes")
-    ap.add_argument("--timed-kernel", type=int, default=3, choices=(1, 2, 3),
+    ap.add_argument("--timed-kernel", type=int, default=3, choices=(1, 2, 3, 4),
                     help="kernel class timed for the roofline: 1 Theta operator, 2 dual Rayleigh-Ritz Gram, "
-                         "3 Rayleigh-Ritz solve (the dominant kernel)")
+                         "3 Rayleigh-Ritz solve (the dominant kernel), 4 Ritz-update linear combinations")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 512 MiB device-to-device copy five times (scale for PMC byte counters)")
     ap.add_argument("--exit-rule", type=int, default=1,
@@ -176,8 +179,10 @@ def main():
             "peak": HBM_PEAK_GBS if hbm else FP64_PEAK_TFLOPS,
             "unit": "GB/s" if hbm else "TFLOP/s",
             "frac": round(gbs / HBM_PEAK_GBS if hbm else tfs / FP64_PEAK_TFLOPS, 5),
-            "traffic": (RR_TRAFFIC_BYTES if (args.timed_kernel == 3 and args.slots == 512 and args.groups == 4
-                                              and args.geoms == 256 and args.precision == 0) else None),
+            "traffic": (RR_TRAFFIC_BYTES if (args.timed_kernel == 3 and args.slots == RR_TRAFFIC_CONFIG[0]
+                                              and args.groups == RR_TRAFFIC_CONFIG[1]
+                                              and args.geoms == RR_TRAFFIC_CONFIG[2] and args.precision == 0)
+                        else None),
             "kernel": TIMED_NAMES[args.timed_kernel],
             "avg_launch_ms": round(avg_s * 1e3, 4),
             "launches": timed["launches"],

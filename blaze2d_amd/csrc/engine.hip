@@ -1331,7 +1331,18 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
         // FLOPs are added when the sweep counts come back (absorb)
         timed(TIMED_RR, bytes, 0.0, [&] { launch_rr(d_e_rr2, (int)e_rr2.size(), std::max(max_p, 1), st); });
     }
-    launch_lincomb<T>(d_lc_ritz, (int)lc_ritz.size(), N, std::max(max_ritz, 1), st);
+    {
+        // algorithmic: each input column read once, each output written once
+        // (the output count min(p, 2nb) is the device-side count's bound)
+        double bytes = 0, flops = 0;
+        for (auto &l : lc_ritz) {
+            const double nq = std::min(l.na, 2 * nbmax);
+            bytes += (double)N * sizeof(cplx<T>) * (l.na + nq * (l.out2 ? 2.0 : 1.0));
+            flops += 8.0 * N * l.na * nq;
+        }
+        timed(TIMED_RITZ, bytes, flops,
+              [&] { launch_lincomb<T>(d_lc_ritz, (int)lc_ritz.size(), N, std::max(max_ritz, 1), st); });
+    }
     // T1 Grams -> compact readback
     if (!g_t1.empty()) {
         int maxkd = 0;

@@ -142,7 +142,8 @@ typedef struct {
     int groups;              /* slot groups pipelined on separate HIP streams (0 -> 4) */
     int timed_kernel;        /* kernel class timed with HIP events on its stream:
                                 0 none, 1 Theta operator, 2 dual Rayleigh-Ritz Gram,
-                                3 Rayleigh-Ritz solve (Cholesky/SVQB + projection + Jacobi) */
+                                3 Rayleigh-Ritz solve (Cholesky/SVQB + projection + Jacobi),
+                                4 Ritz-update linear combinations */
 } bh_run_options;
 
 typedef struct {
