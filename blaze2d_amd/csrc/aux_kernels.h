@@ -10,7 +10,8 @@ namespace bh {
 template <typename T>
 struct ResidualWork {
     const cplx<T> *ax;
-    const cplx<T> *bx;
+    const cplx<T> *bx;    // B x, or x itself when bw is set
+    const T *bw;          // optional: B x = bw * x (TM mass: bw = eps)
     cplx<T> *out;
     const int *act;       // band index of each active column
     const double *lam;
@@ -23,7 +24,8 @@ template <typename T>
 struct RayleighWork {
     const cplx<T> *x;
     const cplx<T> *ax;
-    const cplx<T> *bx;
+    const cplx<T> *bx;    // B x, or x itself when bw is set
+    const T *bw;          // optional: B x = bw * x
     double *lam;
     long stride;
     int count, pad;

@@ -14,6 +14,10 @@ __global__ void residual_kernel(const ResidualWork<T> *__restrict__ work, long r
     const int b = w.act[c];
     const double lam = w.lam[b];
     cplx<T> a = w.ax[b * w.stride + r], m = w.bx[b * w.stride + r];
+    if (w.bw) {
+        const T s = w.bw[r];
+        m = {m.x * s, m.y * s};
+    }
     // backend axpy with alpha = (-lambda, 0) in storage precision
     T al = (T)(-lam);
     w.out[c * w.stride + r] = {a.x + al * m.x, a.y + al * m.y};
@@ -36,6 +40,10 @@ __global__ void __launch_bounds__(256) rayleigh_kernel(const RayleighWork<T> *__
     double num = 0, den = 0;
     for (long r = threadIdx.x; r < rows; r += 256) {
         cplx<T> xv = x[r], av = ax[r], bv = bx[r];
+        if (w.bw) {
+            const T s = w.bw[r];
+            bv = {bv.x * s, bv.y * s};
+        }
         num += (double)xv.x * av.x + (double)xv.y * av.y;
         den += (double)xv.x * bv.x + (double)xv.y * bv.y;
     }

@@ -61,6 +61,7 @@ struct EighWork {
 template <typename T>
 struct Gram2Work {
     const cplx<T> *Z, *BZ, *AZ;  // column j at base + j*stride (BZ may equal Z)
+    const T *bweight;            // if set, B Z = bweight * Z and BZ is not read (TM mass)
     double *partial1, *partial2;
     long stride;
     int n, ld;

@@ -661,7 +661,8 @@ __global__ void __launch_bounds__(256) gram2_kernel(const Gram2Work<T> *__restri
     double2 *Zs = g2s;                       // [SR][COLS]
     double2 *Bs = Zs + S::SR * S::COLS;
     double2 *As = Bs + S::SR * S::COLS;
-    const bool bz_alias = w.BZ == w.Z;
+    const bool weighted = w.bweight != nullptr;
+    const bool bz_alias = !weighted && w.BZ == w.Z;
     const int tid = threadIdx.x;
     const int g = tid / S::NLB, lb = tid - g * S::NLB;
     int bi = 0;
@@ -681,7 +682,10 @@ __global__ void __launch_bounds__(256) gram2_kernel(const Gram2Work<T> *__restri
                 cplx<T> zv = w.Z[o], av = w.AZ[o];
                 z = make_double2((double)zv.x, (double)zv.y);
                 a = make_double2((double)av.x, (double)av.y);
-                if (!bz_alias) {
+                if (weighted) {  // B z in storage precision, as a stored mass image would hold it
+                    const T s = w.bweight[rb + r];
+                    b = make_double2((double)(zv.x * s), (double)(zv.y * s));
+                } else if (!bz_alias) {
                     cplx<T> bv = w.BZ[o];
                     b = make_double2((double)bv.x, (double)bv.y);
                 }

@@ -12,11 +12,12 @@
 // operation runs on the GPU.  One host synchronisation per step.
 //
 // Per-slot device layout (columns of N complex values, storage precision T),
-// two copies b = 0, 1 of V, BV, AV; the slot's parity selects the current one
+// two copies b = 0, 1 of V and AV; the slot's parity selects the current one
 // and the Ritz update writes the other:
 //   V  [C + 1] : Z = [X (m) | W (w) | P (a)];  column C holds the Gamma mode y0
-//   BV [C + 1] : B Z (TM only; TE aliases V)
 //   AV [C + 1] : A Z (A X and A W from the Ritz update, A P applied fresh)
+// B Z is never stored: TE has B = I, and TM's B = eps is applied where a
+// kernel reads B Z (residuals, Rayleigh quotients, Grams).
 //   H  [2 nb]  : subspace history (previous / current converged eigenvectors)
 #include <algorithm>
 #include <atomic>
@@ -197,7 +198,7 @@ struct Slot {
     double y0val = 0;  // Gamma mode value 1 / ||1||_B
     // per-solve state
     int it = 0, m = 0, nW = 0;
-    int par = 0;  // current copy of V / BV / AV
+    int par = 0;  // current copy of V / AV
     int last_p = 0;  // subspace width of the iteration in flight
     std::vector<char> soft;
     std::vector<double> lam, lam_prev;
@@ -250,7 +251,6 @@ class Engine : public EngineBase {
         if (opt.setup_stream) st = opt.setup_stream;
         else { BH_HIP_CHECK(hipStreamCreate(&st)); own_st = true; }
         V = dalloc<cplx<T>>((size_t)S * 2 * CV * N, st);
-        BV = dalloc<cplx<T>>((size_t)S * 2 * CV * N, st);
         AV = dalloc<cplx<T>>((size_t)S * 2 * CV * N, st);
         H = dalloc<cplx<T>>((size_t)S * 2 * nbmax * N, st);
         RND = dalloc<cplx<T>>((size_t)NRMAX * N, st);
@@ -320,7 +320,7 @@ class Engine : public EngineBase {
             (void)hipEventDestroy(G.ev);
             if (G.own_st) (void)hipStreamDestroy(G.st);
         }
-        for (void *p : std::initializer_list<void *>{V, BV, AV, H, RND, EPS, TEN, d_lam, d_Y, d_coef, d_cnt, twx, twy})
+        for (void *p : std::initializer_list<void *>{V, AV, H, RND, EPS, TEN, d_lam, d_Y, d_coef, d_cnt, twx, twy})
             if (p) (void)hipFree(p);
         (void)hipHostFree(h_lam);
         (void)hipHostFree(h_cnt);
@@ -337,7 +337,7 @@ class Engine : public EngineBase {
     hipStream_t st{};
     bool own_st = false;
     std::unique_ptr<WorkerPool> pool;
-    cplx<T> *V{}, *BV{}, *AV{}, *H{}, *RND{};
+    cplx<T> *V{}, *AV{}, *H{}, *RND{};
     T *EPS{}, *TEN{};
     double *d_lam{}, *d_Y{}, *d_coef{};
     int *d_cnt{};
@@ -353,15 +353,13 @@ class Engine : public EngineBase {
     RunStats *stats_{};
 
     // column addressing
-    // copy b of slot s; vcol/bvcol/avcol address the current copy
+    // copy b of slot s; vcol/avcol address the current copy
     cplx<T> *vbuf(int s, int b, int j) { return V + (((size_t)s * 2 + b) * CV + j) * N; }
-    cplx<T> *bvbuf(int s, int b, int j) {
-        return slots[s].pol == 1 ? vbuf(s, b, j) : BV + (((size_t)s * 2 + b) * CV + j) * N;
-    }
     cplx<T> *avbuf(int s, int b, int j) { return AV + (((size_t)s * 2 + b) * CV + j) * N; }
     cplx<T> *vcol(int s, int j) { return vbuf(s, slots[s].par, j); }
-    cplx<T> *bvcol(int s, int j) { return bvbuf(s, slots[s].par, j); }
     cplx<T> *avcol(int s, int j) { return avbuf(s, slots[s].par, j); }
+    // B weight of slot s: eps for TM, none (identity) for TE
+    const T *bweight(int s) { return slots[s].pol == 0 ? eps(s) : nullptr; }
     cplx<T> *hcol(int s, int b, int j) { return H + ((size_t)s * 2 * nbmax + (size_t)b * nbmax + j) * N; }
     cplx<T> *rcol(int j) { return RND + (size_t)j * N; }
     T *eps(int s) { return EPS + (size_t)s * N; }
@@ -951,7 +949,6 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
         w.ldc = kd;
         w.out = vcol(s, 0);
         w.out_stride = N;
-        if (sl.pol == 0) { w.out2 = bvcol(s, 0); w.w2 = eps(s); }
         lc_x.push_back(w);
         maxq_x = std::max(maxq_x, sl.m);
         OpWork<T> o{};
@@ -967,7 +964,8 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
         RayleighWork<T> rw{};
         rw.x = vcol(s, 0);
         rw.ax = avcol(s, 0);
-        rw.bx = bvcol(s, 0);
+        rw.bx = vcol(s, 0);
+        rw.bw = bweight(s);
         rw.lam = d_lam + (size_t)s * LD;
         rw.stride = N;
         rw.count = sl.m;
@@ -977,7 +975,6 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
     }
 
     // ---- iteration for ITER slots ----
-    std::vector<LincombWork<T>> lc_mass;
     std::vector<ResidualWork<T>> res;
     std::vector<GramWork<T>> g_defR, g_defP;
     std::vector<LincombWork<T>> lc_defR, lc_defP;
@@ -1013,7 +1010,8 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
         for (int i = 0; i < a; ++i) act_h[i] = act[i];
         ResidualWork<T> rw{};
         rw.ax = avcol(s, 0);
-        rw.bx = bvcol(s, 0);
+        rw.bx = vcol(s, 0);
+        rw.bw = bweight(s);
         rw.out = vcol(s, m + w);
         rw.act = act_d;
         rw.lam = d_lam + (size_t)s * LD;
@@ -1028,12 +1026,13 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
         const int nd = (int)dcols.size();
         max_d = std::max(max_d, nd);
         if (nd > 0 && a > 0) {
-            const cplx<T> **bd_d, **dd_d;
-            const cplx<T> **bd = stg.alloc<const cplx<T> *>(nd, (const cplx<T> ***)&bd_d);
+            const cplx<T> **dd_d;
             const cplx<T> **dd = stg.alloc<const cplx<T> *>(nd, (const cplx<T> ***)&dd_d);
-            for (int i = 0; i < nd; ++i) { bd[i] = bvcol(s, dcols[i]); dd[i] = vcol(s, dcols[i]); }
+            for (int i = 0; i < nd; ++i) dd[i] = vcol(s, dcols[i]);
+            // coefficients <B d_i, r_c> = sum conj(d_i) w r_c with w the B weight
             GramWork<T> gw{};
-            gw.a_ptrs = bd_d;
+            gw.a_ptrs = dd_d;
+            gw.weight = bweight(s);
             gw.B = vcol(s, m + w);
             gw.b_stride = N;
             gw.na = nd;
@@ -1059,14 +1058,6 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
             slot_defP.push_back(s);
             LincombWork<T> lp = lw;
             lc_defP.push_back(lp);
-            if (sl.pol == 0) {
-                LincombWork<T> lb = lw;
-                lb.a_ptrs = bd_d;
-                lb.base = bvcol(s, m + w);
-                lb.out = bvcol(s, m + w);
-                lc_defP.push_back(lb);
-                slot_defP.push_back(-1);  // marks the B-image item (shares coefficients)
-            }
             DefMask dm{s, nd, a, std::vector<unsigned char>((size_t)LD * LD, 1)};
             for (int c = 0; c < a; ++c)
                 if (c < (int)sl.soft.size() && sl.soft[c])
@@ -1074,26 +1065,7 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
                         if (dcols[i] != C) dm.m[i + (size_t)c * LD] = 0;
             masksP.push_back(std::move(dm));
         }
-        // TM without the fused FD mass output: BP = eps P after preconditioning
-        if (sl.pol == 0 && sl.precond != 1 && a > 0) {
-            double *cd;
-            double *ch = stg.alloc<double>((size_t)2 * a * a, &cd);
-            std::memset(ch, 0, sizeof(double) * 2 * a * a);
-            for (int i = 0; i < a; ++i) ch[2 * (i + (size_t)i * a)] = 1.0;
-            LincombWork<T> lm{};
-            lm.A = vcol(s, m + w);
-            lm.a_stride = N;
-            lm.na = a;
-            lm.nq = a;
-            lm.C = cd;
-            lm.ldc = a;
-            lm.out = vcol(s, m + w);
-            lm.out_stride = N;
-            lm.out2 = bvcol(s, m + w);
-            lm.w2 = eps(s);
-            lc_mass.push_back(lm);
-        }
-        // preconditioner (P in place; TM FD also writes BP)
+        // preconditioner (P in place)
         if (sl.precond == 1 || sl.precond == 2) {
             OpWork<T> o{};
             o.in = vcol(s, m + w);
@@ -1104,7 +1076,6 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
             o.in_stride = N;
             o.out_stride = N;
             o.count = a;
-            if (sl.precond == 1 && sl.pol == 0) o.bout = bvcol(s, m + w);
             op_pre[sl.precond - 1].push_back(o);
         }
         // A P for the new directions; A X and A W come from the last Ritz update
@@ -1124,7 +1095,8 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
         // both Rayleigh-Ritz Grams Z^H B Z and Z^H A Z in one pass
         Gram2Work<T> g2{};
         g2.Z = vcol(s, 0);
-        g2.BZ = bvcol(s, 0);
+        g2.BZ = vcol(s, 0);
+        g2.bweight = bweight(s);
         g2.AZ = avcol(s, 0);
         g2.stride = N;
         g2.n = p;
@@ -1153,7 +1125,6 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
         lx.ldc = LD;
         lx.out = vbuf(s, nxt, 0);
         lx.out_stride = N;
-        if (sl.pol == 0) { lx.out2 = bvbuf(s, nxt, 0); lx.w2 = eps(s); }
         lc_ritz.push_back(lx);
         LincombWork<T> la = lx;
         la.A = avcol(s, 0);
@@ -1238,7 +1209,6 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
     auto *d_lc_x = put(lc_x);
     OpWork<T> *d_op_x[2] = {put(op_x[0]), put(op_x[1])};
     auto *d_rq = put(rq);
-    auto *d_lc_mass = put(lc_mass);
     auto *d_res = put(res);
     auto *d_g_defR = put(g_defR);
     auto *d_lc_defR = put(lc_defR);
@@ -1258,7 +1228,7 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
     for (int s : plan.fill_y0) {
         Slot &sl = slots[s];
         for (int b = 0; b < 2; ++b)  // the Gamma mode sits in column C of both copies
-            launch_fill<T>(vbuf(s, b, C), sl.pol == 0 ? bvbuf(s, b, C) : nullptr, eps(s), sl.y0val, 0.0, N, st);
+            launch_fill<T>(vbuf(s, b, C), nullptr, eps(s), sl.y0val, 0.0, N, st);
     }
     // launches of the timed kernel class are bracketed by events on this stream
     auto timed = [&](int cls, double bytes, double flops, auto &&launch) {
@@ -1301,7 +1271,6 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
     }
     launch_op<T>(MODE_PREC_FD, nx, ny, d_op_pre[0], (int)op_pre[0].size(), std::max(max_a, 1), twx, twy, st);
     launch_op<T>(MODE_PREC_TP, nx, ny, d_op_pre[1], (int)op_pre[1].size(), std::max(max_a, 1), twx, twy, st);
-    launch_lincomb<T>(d_lc_mass, (int)lc_mass.size(), N, std::max(max_a, 1), st);
     if (!g_defP.empty()) {
         launch_gram<T>(d_g_defP, (int)g_defP.size(), nch_def, N, max_d, max_a, st);
         launch_reduce(d_red_defP, (int)g_defP.size(), LD, st);
@@ -1318,7 +1287,7 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
     {
         double bytes = 0, flops = 0;
         for (auto &g2 : g_dual) {
-            bytes += (g2.BZ == g2.Z ? 2.0 : 3.0) * g2.n * (double)N * sizeof(cplx<T>);
+            bytes += 2.0 * g2.n * (double)N * sizeof(cplx<T>) + (g2.bweight ? (double)N * sizeof(T) : 0.0);
             flops += 2.0 * 8.0 * (double)N * g2.n * (g2.n + 1) / 2.0;
         }
         timed(TIMED_GRAM2, bytes, flops,
