@@ -47,7 +47,8 @@ RR_TRAFFIC_CONFIG = (512, 4, 256)  # slots, groups, geometries the counters were
 TIMED_NAMES = {1: "op_kernel THETA_TM/TE (64x64 fused 2D FFT operator)",
                2: "gram2_kernel (dual Rayleigh-Ritz Gram Z^H BZ, Z^H AZ)",
                3: "rr_kernel (Cholesky/SVQB + projection + Jacobi Rayleigh-Ritz, FP64)",
-               4: "lincomb_kernel (Ritz update [X'|W'] = Z C, A Z C)"}
+               4: "lincomb_kernel (Ritz update [X'|W'] = Z C, A Z C)",
+               5: "deflate_kernel (residual / direction deflation against locked bands)"}
 
 
 def solves_of(jobs):
@@ -77,7 +78,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=4, help="band structures timed on the CPU oracle")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip roofline / plain-GPU extra passes")
-    ap.add_argument("--timed-kernel", type=int, default=3, choices=(1, 2, 3, 4),
+    ap.add_argument("--timed-kernel", type=int, default=3, choices=(1, 2, 3, 4, 5),
                     help="kernel class timed for the roofline: 1 Theta operator, 2 dual Rayleigh-Ritz Gram, "
                          "3 Rayleigh-Ritz solve (the dominant kernel), 4 Ritz-update linear combinations")
     ap.add_argument("--calibrate-copy", action="store_true",

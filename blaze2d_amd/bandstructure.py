@@ -166,8 +166,9 @@ def run_batch(backend: HipBackend, jobs: Sequence[BandStructureJob], options: Op
 
     `slots` band structures are resident at once; they are split into `groups`
     pipelined on separate HIP streams.  `timed_kernel` (1 Theta operator,
-    2 dual Rayleigh-Ritz Gram) times every launch of that kernel class with HIP
-    events on its stream during the run.
+    2 dual Rayleigh-Ritz Gram, 3 Rayleigh-Ritz solve, 4 Ritz update, 5
+    deflation) times every launch of that kernel class with HIP events on its
+    stream during the run.
     """
     opts = options or RunOptions()
     jobs = list(jobs)

@@ -569,7 +569,7 @@ int bh_bandstructure_run(bh_context *ctx, const bh_job *jobs, int njobs, const b
     eo.profile = opts ? opts->profile : 0;
     eo.groups = opts && opts->groups > 0 ? opts->groups : 4;
     eo.timed_kernel = opts ? opts->timed_kernel : 0;
-    if (eo.timed_kernel < 0 || eo.timed_kernel > 4) return fail("bh_bandstructure_run: timed_kernel must be 0..4");
+    if (eo.timed_kernel < 0 || eo.timed_kernel > 5) return fail("bh_bandstructure_run: timed_kernel must be 0..5");
     eo.setup_stream = ctx->st;
     eo.group_streams[0] = ctx->st;
     eo.group_streams[1] = ctx->gst[0];

@@ -60,8 +60,8 @@ struct EighWork {
 // chunk (row-major [chunk][ld][ld] complex, entry i*ld + j for i >= j).
 template <typename T>
 struct Gram2Work {
-    const cplx<T> *Z, *BZ, *AZ;  // column j at base + j*stride (BZ may equal Z)
-    const T *bweight;            // if set, B Z = bweight * Z and BZ is not read (TM mass)
+    const cplx<T> *Z, *AZ;  // column j at base + j*stride
+    const T *bweight;       // B Z = bweight * Z (TM mass); null: B = I
     double *partial1, *partial2;
     long stride;
     int n, ld;
@@ -98,6 +98,24 @@ struct ReduceWork {
     int nchunks, pad;
 };
 void launch_reduce(const ReduceWork *dwork, int nwork, int ld, hipStream_t st);
+
+// Block deflation of a columns against nd vectors (deflation.rs:318-332,
+// eigensolver.rs:859-865,978-1008):
+//   r_c <- r_c - sum_i d_i <B d_i, r_c>,  <B d, r> = sum conj(d) w r,
+// every coefficient taken from the undeflated r_c; a pair (i, c) whose
+// mask[i + c*ld] is 0 is skipped.  One launch replaces Gram + reduce + lincomb.
+template <typename T>
+struct DeflateWork {
+    const cplx<T> *const *d_ptrs;  // nd vectors
+    const T *weight;               // optional B weight per row (TM: eps)
+    cplx<T> *R;                    // a columns, stride apart, updated in place
+    const unsigned char *mask;     // optional, col-major ld x ld
+    long stride;
+    int nd, a, ld, pad;
+};
+constexpr int DEFL_MAXD = 24;  // most deflation vectors per item
+template <typename T>
+void launch_deflate(const DeflateWork<T> *dwork, int nwork, long rows, int maxa, int maxd, hipStream_t st);
 template <typename T>
 void launch_lincomb(const LincombWork<T> *dwork, int nwork, long rows, int maxq, hipStream_t st);
 void launch_eigh(const EighWork *dwork, int nwork, int maxn, hipStream_t st);

@@ -151,6 +151,168 @@ void launch_reduce(const ReduceWork *dwork, int nwork, int ld, hipStream_t st) {
 }
 
 // ---------------------------------------------------------------------------
+// Deflation: one workgroup per (item, group of DEFL_CPW columns).  The columns
+// stay in registers (DEFL_RPT rows per thread) between the coefficient pass
+// and the update, so each is read and written once; the nd vectors are read
+// twice per column group, the second time from L2.  The vectors are taken
+// DEFL_DB at a time with independent loads, accumulators and wave
+// reductions, so their memory and shuffle latencies overlap.  Workgroups are
+// placed XCD-aware: linear block b runs on XCD b % 8, and all column groups
+// of one item get the same b % 8, so the item's vectors are fetched into one
+// L2.
+// ---------------------------------------------------------------------------
+constexpr int DEFL_NT = 512;
+constexpr int DEFL_RPT = 8;  // rows per thread: N <= 4096
+constexpr int DEFL_CPW = 1;  // columns per workgroup
+constexpr int DEFL_DB = 8;   // vectors per batch
+constexpr int NXCD = 8;
+
+template <typename T>
+__global__ void __launch_bounds__(DEFL_NT) deflate_kernel(const DeflateWork<T> *__restrict__ work, int nwork,
+                                                          int ngrp, long nrows) {
+    constexpr int NW = DEFL_NT / 64;
+    const int b = blockIdx.x;
+    const int q = b / NXCD;
+    const int item = (q / ngrp) * NXCD + b % NXCD;
+    const int c0 = (q % ngrp) * DEFL_CPW;
+    if (item >= nwork) return;
+    const DeflateWork<T> &w = work[item];
+    const int nd = w.nd;
+    if (c0 >= w.a || nd <= 0) return;
+    const int nc = w.a - c0 < DEFL_CPW ? w.a - c0 : DEFL_CPW;
+    __shared__ double red[NW][DEFL_MAXD][DEFL_CPW][2];
+    __shared__ double coef[DEFL_MAXD][DEFL_CPW][2];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const bool full = nrows == (long)DEFL_NT * DEFL_RPT;
+    double ur[DEFL_CPW][DEFL_RPT], ui[DEFL_CPW][DEFL_RPT], ws[DEFL_RPT];
+#pragma unroll
+    for (int k = 0; k < DEFL_RPT; ++k) {
+        const long i = tid + (long)k * DEFL_NT;
+        ws[k] = (w.weight && i < nrows) ? (double)w.weight[i] : 1.0;
+    }
+#pragma unroll
+    for (int c = 0; c < DEFL_CPW; ++c) {
+        const cplx<T> *r = w.R + (size_t)(c0 + c) * w.stride;
+#pragma unroll
+        for (int k = 0; k < DEFL_RPT; ++k) {
+            const long i = tid + (long)k * DEFL_NT;
+            const cplx<T> v = (c < nc && i < nrows) ? r[i] : cplx<T>{(T)0, (T)0};
+            ur[c][k] = (double)v.x;
+            ui[c][k] = (double)v.y;
+        }
+    }
+    // coefficients <B d, r_c> = sum conj(w d) r_c
+    for (int d0 = 0; d0 < nd; d0 += DEFL_DB) {
+        double sr[DEFL_DB][DEFL_CPW], si[DEFL_DB][DEFL_CPW];
+#pragma unroll
+        for (int j = 0; j < DEFL_DB; ++j) {
+#pragma unroll
+            for (int c = 0; c < DEFL_CPW; ++c) sr[j][c] = si[j][c] = 0.0;
+            if (d0 + j < nd) {
+                const cplx<T> *dv = w.d_ptrs[d0 + j];
+#pragma unroll
+                for (int k = 0; k < DEFL_RPT; ++k) {
+                    const long i = tid + (long)k * DEFL_NT;
+                    if (full || i < nrows) {
+                        const cplx<T> x = dv[i];
+                        const double xr = ws[k] * (double)x.x, xi = ws[k] * (double)x.y;
+#pragma unroll
+                        for (int c = 0; c < DEFL_CPW; ++c) {
+                            sr[j][c] += xr * ur[c][k] + xi * ui[c][k];
+                            si[j][c] += xr * ui[c][k] - xi * ur[c][k];
+                        }
+                    }
+                }
+            }
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+            for (int j = 0; j < DEFL_DB; ++j)
+#pragma unroll
+                for (int c = 0; c < DEFL_CPW; ++c) {
+                    sr[j][c] += __shfl_xor(sr[j][c], off);
+                    si[j][c] += __shfl_xor(si[j][c], off);
+                }
+        }
+        if (lane == 0) {
+#pragma unroll
+            for (int j = 0; j < DEFL_DB; ++j)
+                if (d0 + j < nd)
+#pragma unroll
+                    for (int c = 0; c < DEFL_CPW; ++c) {
+                        red[wv][d0 + j][c][0] = sr[j][c];
+                        red[wv][d0 + j][c][1] = si[j][c];
+                    }
+        }
+    }
+    __syncthreads();
+    if (tid < nd * DEFL_CPW) {
+        const int d = tid / DEFL_CPW, c = tid - d * DEFL_CPW;
+        double cr = 0.0, ci = 0.0;
+#pragma unroll
+        for (int v = 0; v < NW; ++v) {
+            cr += red[v][d][c][0];
+            ci += red[v][d][c][1];
+        }
+        if (c >= nc || (w.mask && !w.mask[d + (size_t)(c0 + c) * w.ld])) cr = ci = 0.0;
+        coef[d][c][0] = cr;
+        coef[d][c][1] = ci;
+    }
+    __syncthreads();
+    // r_c -= sum_d d coef(d, c)
+    for (int d0 = 0; d0 < nd; d0 += DEFL_DB) {
+#pragma unroll
+        for (int j = 0; j < DEFL_DB; ++j) {
+            if (d0 + j < nd) {
+                const cplx<T> *dv = w.d_ptrs[d0 + j];
+                double cr[DEFL_CPW], ci[DEFL_CPW];
+#pragma unroll
+                for (int c = 0; c < DEFL_CPW; ++c) {
+                    cr[c] = coef[d0 + j][c][0];
+                    ci[c] = coef[d0 + j][c][1];
+                }
+#pragma unroll
+                for (int k = 0; k < DEFL_RPT; ++k) {
+                    const long i = tid + (long)k * DEFL_NT;
+                    if (full || i < nrows) {
+                        const cplx<T> x = dv[i];
+#pragma unroll
+                        for (int c = 0; c < DEFL_CPW; ++c) {
+                            ur[c][k] -= (double)x.x * cr[c] - (double)x.y * ci[c];
+                            ui[c][k] -= (double)x.x * ci[c] + (double)x.y * cr[c];
+                        }
+                    }
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < DEFL_CPW; ++c) {
+        if (c >= nc) break;
+        cplx<T> *r = w.R + (size_t)(c0 + c) * w.stride;
+#pragma unroll
+        for (int k = 0; k < DEFL_RPT; ++k) {
+            const long i = tid + (long)k * DEFL_NT;
+            if (i < nrows) r[i] = cplx<T>{(T)ur[c][k], (T)ui[c][k]};
+        }
+    }
+}
+
+template <typename T>
+void launch_deflate(const DeflateWork<T> *dwork, int nwork, long rows, int maxa, int maxd, hipStream_t st) {
+    if (nwork <= 0 || maxa <= 0) return;
+    if (rows > (long)DEFL_NT * DEFL_RPT) throw std::runtime_error("deflation column longer than 4096 rows");
+    if (maxd > DEFL_MAXD) throw std::runtime_error("more deflation vectors than the kernel holds");
+    const long groups = (nwork + NXCD - 1) / NXCD;
+    const int ngrp = (maxa + DEFL_CPW - 1) / DEFL_CPW;
+    hipLaunchKernelGGL((deflate_kernel<T>), dim3((unsigned)(groups * ngrp * NXCD)), dim3(DEFL_NT), 0, st, dwork,
+                       nwork, ngrp, rows);
+    BH_HIP_CHECK(hipGetLastError());
+}
+template void launch_deflate<double>(const DeflateWork<double> *, int, long, int, int, hipStream_t);
+template void launch_deflate<float>(const DeflateWork<float> *, int, long, int, int, hipStream_t);
+
+// ---------------------------------------------------------------------------
 // Linear combination: grid = (row blocks, nwork, output groups of QG); one row
 // per thread, QG outputs in registers.
 // ---------------------------------------------------------------------------
@@ -627,8 +789,8 @@ void launch_eigh(const EighWork *dwork, int nwork, int maxn, hipStream_t st) {
 namespace bh {
 
 // ---------------------------------------------------------------------------
-// Dual Gram for Rayleigh-Ritz: one pass over the rows of Z, BZ and AZ gives
-// G1 = Z^H BZ and G2 = Z^H AZ (lower triangles).  The n x n outputs are
+// Dual Gram for Rayleigh-Ritz: one pass over the rows of Z and AZ gives
+// G1 = Z^H BZ and G2 = Z^H AZ (lower triangles), B Z formed on the fly.  The n x n outputs are
 // tiled by 3 x 3 blocks on an NBK x NBK block grid; each thread owns one lower
 // block of both products (18 complex accumulators), so only the blocks that
 // intersect the n columns do work.  RG row groups split the staged rows and are
@@ -645,13 +807,16 @@ struct Gram2Shape {
 template <int NBK>
 size_t gram2_shared_bytes() {
     using S = Gram2Shape<NBK>;
-    size_t stage = 3ull * S::SR * S::COLS * sizeof(double2);
-    size_t red = (size_t)(S::RG > 1 ? S::RG - 1 : 0) * S::NLB * 36 * sizeof(double);
+    size_t stage = 2ull * S::SR * S::COLS * sizeof(double2) + S::SR * sizeof(double);
+    size_t red = (size_t)S::NLB * 36 * sizeof(double);
     return stage > red ? stage : red;
 }
 
+// Z^H B Z and Z^H A Z (lower 3x3 blocks) over one row chunk.  B Z is formed
+// from the staged Z and the row weight, so only Z and AZ pass through LDS
+// (36 KB per workgroup at 36 columns: four workgroups per CU).
 template <typename T, int NBK>
-__global__ void __launch_bounds__(256) gram2_kernel(const Gram2Work<T> *__restrict__ work, long rows_per_chunk,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) gram2_kernel(const Gram2Work<T> *__restrict__ work, long rows_per_chunk,
                                                     long nrows) {
     using S = Gram2Shape<NBK>;
     const Gram2Work<T> &w = work[blockIdx.y];
@@ -659,10 +824,9 @@ __global__ void __launch_bounds__(256) gram2_kernel(const Gram2Work<T> *__restri
     if (n <= 0) return;
     extern __shared__ __attribute__((aligned(16))) double2 g2s[];
     double2 *Zs = g2s;                       // [SR][COLS]
-    double2 *Bs = Zs + S::SR * S::COLS;
-    double2 *As = Bs + S::SR * S::COLS;
+    double2 *As = Zs + S::SR * S::COLS;
+    double *Ws = reinterpret_cast<double *>(As + S::SR * S::COLS);  // [SR]
     const bool weighted = w.bweight != nullptr;
-    const bool bz_alias = !weighted && w.BZ == w.Z;
     const int tid = threadIdx.x;
     const int g = tid / S::NLB, lb = tid - g * S::NLB;
     int bi = 0;
@@ -676,33 +840,28 @@ __global__ void __launch_bounds__(256) gram2_kernel(const Gram2Work<T> *__restri
         const int nr = (int)(r1 - rb < S::SR ? r1 - rb : S::SR);
         for (int e = tid; e < S::SR * S::COLS; e += 256) {
             const int c = e / S::SR, r = e - c * S::SR;
-            double2 z = make_double2(0.0, 0.0), b = z, a = z;
+            double2 z = make_double2(0.0, 0.0), a = z;
             if (r < nr && c < n) {
                 const long o = (long)c * w.stride + rb + r;
                 cplx<T> zv = w.Z[o], av = w.AZ[o];
                 z = make_double2((double)zv.x, (double)zv.y);
                 a = make_double2((double)av.x, (double)av.y);
-                if (weighted) {  // B z in storage precision, as a stored mass image would hold it
-                    const T s = w.bweight[rb + r];
-                    b = make_double2((double)(zv.x * s), (double)(zv.y * s));
-                } else if (!bz_alias) {
-                    cplx<T> bv = w.BZ[o];
-                    b = make_double2((double)bv.x, (double)bv.y);
-                }
             }
             Zs[r * S::COLS + c] = z;
             As[r * S::COLS + c] = a;
-            if (!bz_alias) Bs[r * S::COLS + c] = b;
         }
+        if (weighted && tid < S::SR) Ws[tid] = tid < nr ? (double)w.bweight[rb + tid] : 0.0;
         __syncthreads();
         if (active) {
-            const double2 *Bsrc = bz_alias ? Zs : Bs;
             for (int r = g; r < nr; r += S::RG) {
                 double2 zi[3], bjv[3], ajv[3];
+                const T s = weighted ? (T)Ws[r] : (T)1;
 #pragma unroll
                 for (int u = 0; u < 3; ++u) {
                     zi[u] = Zs[r * S::COLS + 3 * bi + u];
-                    bjv[u] = Bsrc[r * S::COLS + 3 * bj + u];
+                    const double2 zj = Zs[r * S::COLS + 3 * bj + u];
+                    // B z in storage precision, as a stored mass image would hold it
+                    bjv[u] = weighted ? make_double2((double)((T)zj.x * s), (double)((T)zj.y * s)) : zj;
                     ajv[u] = As[r * S::COLS + 3 * bj + u];
                 }
 #pragma unroll
@@ -718,11 +877,12 @@ __global__ void __launch_bounds__(256) gram2_kernel(const Gram2Work<T> *__restri
         }
         __syncthreads();
     }
-    // reduce the row groups through LDS (the staging area is free now)
+    // reduce the row groups into group 0 through LDS, one group at a time
+    // (the staging area is free now)
     double *red = reinterpret_cast<double *>(g2s);
-    if (S::RG > 1) {
-        if (active && g > 0) {
-            double *o = red + ((size_t)(g - 1) * S::NLB + lb) * 36;
+    for (int gg = 1; gg < S::RG; ++gg) {
+        if (active && g == gg) {
+            double *o = red + (size_t)lb * 36;
 #pragma unroll
             for (int u = 0; u < 3; ++u)
 #pragma unroll
@@ -735,19 +895,18 @@ __global__ void __launch_bounds__(256) gram2_kernel(const Gram2Work<T> *__restri
         }
         __syncthreads();
         if (active && g == 0) {
-            for (int gg = 1; gg < S::RG; ++gg) {
-                const double *o = red + ((size_t)(gg - 1) * S::NLB + lb) * 36;
+            const double *o = red + (size_t)lb * 36;
 #pragma unroll
-                for (int u = 0; u < 3; ++u)
+            for (int u = 0; u < 3; ++u)
 #pragma unroll
-                    for (int v = 0; v < 3; ++v) {
-                        a1r[u][v] += o[4 * (3 * u + v)];
-                        a1i[u][v] += o[4 * (3 * u + v) + 1];
-                        a2r[u][v] += o[4 * (3 * u + v) + 2];
-                        a2i[u][v] += o[4 * (3 * u + v) + 3];
-                    }
-            }
+                for (int v = 0; v < 3; ++v) {
+                    a1r[u][v] += o[4 * (3 * u + v)];
+                    a1i[u][v] += o[4 * (3 * u + v) + 1];
+                    a2r[u][v] += o[4 * (3 * u + v) + 2];
+                    a2i[u][v] += o[4 * (3 * u + v) + 3];
+                }
         }
+        __syncthreads();
     }
     if (!active || g != 0) return;
     double *p1 = w.partial1 + 2 * (size_t)blockIdx.x * w.ld * w.ld;

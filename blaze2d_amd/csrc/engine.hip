@@ -976,9 +976,7 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
 
     // ---- iteration for ITER slots ----
     std::vector<ResidualWork<T>> res;
-    std::vector<GramWork<T>> g_defR, g_defP;
-    std::vector<LincombWork<T>> lc_defR, lc_defP;
-    std::vector<int> slot_defR, slot_defP;
+    std::vector<DeflateWork<T>> defR, defP;
     std::vector<OpWork<T>> op_pre[2];  // [0] FD, [1] TP
     std::vector<OpWork<T>> op_p[2];    // Theta on the new directions [TM, TE]
     long theta_cols[2] = {0, 0};
@@ -1029,35 +1027,18 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
             const cplx<T> **dd_d;
             const cplx<T> **dd = stg.alloc<const cplx<T> *>(nd, (const cplx<T> ***)&dd_d);
             for (int i = 0; i < nd; ++i) dd[i] = vcol(s, dcols[i]);
-            // coefficients <B d_i, r_c> = sum conj(d_i) w r_c with w the B weight
-            GramWork<T> gw{};
-            gw.a_ptrs = dd_d;
-            gw.weight = bweight(s);
-            gw.B = vcol(s, m + w);
-            gw.b_stride = N;
-            gw.na = nd;
-            gw.nb = a;
-            gw.ld = LD;
-            g_defR.push_back(gw);
-            slot_defR.push_back(s);
-            LincombWork<T> lw{};
-            lw.a_ptrs = dd_d;
-            lw.na = nd;
-            lw.nq = a;
-            lw.C = coef(s);
-            lw.ldc = LD;
-            lw.base = vcol(s, m + w);
-            lw.base_stride = N;
-            lw.out = vcol(s, m + w);
-            lw.out_stride = N;
-            lw.mode = 1;
-            lc_defR.push_back(lw);
+            // r_c -= sum_i d_i <B d_i, r_c>, <B d, r> = sum conj(d) w r with w the B weight
+            DeflateWork<T> dw{};
+            dw.d_ptrs = dd_d;
+            dw.weight = bweight(s);
+            dw.R = vcol(s, m + w);
+            dw.stride = N;
+            dw.nd = nd;
+            dw.a = a;
+            dw.ld = LD;
+            defR.push_back(dw);
             // P deflation: same D; soft part skipped for compacted P index c with soft[c]
-            GramWork<T> gp = gw;
-            g_defP.push_back(gp);
-            slot_defP.push_back(s);
-            LincombWork<T> lp = lw;
-            lc_defP.push_back(lp);
+            defP.push_back(dw);
             DefMask dm{s, nd, a, std::vector<unsigned char>((size_t)LD * LD, 1)};
             for (int c = 0; c < a; ++c)
                 if (c < (int)sl.soft.size() && sl.soft[c])
@@ -1095,7 +1076,6 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
         // both Rayleigh-Ritz Grams Z^H B Z and Z^H A Z in one pass
         Gram2Work<T> g2{};
         g2.Z = vcol(s, 0);
-        g2.BZ = vcol(s, 0);
         g2.bweight = bweight(s);
         g2.AZ = avcol(s, 0);
         g2.stride = N;
@@ -1181,9 +1161,7 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
     auto assign_partials = [&](std::vector<GramWork<T>> &v, int nch) {
         for (size_t i = 0; i < v.size(); ++i) v[i].partial = d_part + i * (size_t)nch * LD * LD * 2;
     };
-    const int nch_def = nch_for(g_defR.size()), nch_dual = nch_for(g_dual.size()), nch_t1 = nch_for(g_t1.size());
-    assign_partials(g_defR, nch_def);
-    assign_partials(g_defP, nch_def);
+    const int nch_dual = nch_for(g_dual.size()), nch_t1 = nch_for(g_t1.size());
     assign_partials(g_t1, nch_t1);
     for (size_t i = 0; i < g_dual.size(); ++i) {  // two products per item
         g_dual[i].partial1 = d_part + 2 * i * (size_t)nch_dual * LD * LD * 2;
@@ -1198,23 +1176,15 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
         unsigned char *mh = stg.alloc<unsigned char>(masksP.size() * LD * LD, &mask_d);
         for (size_t i = 0; i < masksP.size(); ++i) std::memcpy(mh + i * LD * LD, masksP[i].m.data(), LD * LD);
     }
-    std::vector<ReduceWork> red_defR, red_defP;
-    for (size_t i = 0; i < g_defR.size(); ++i)
-        red_defR.push_back({g_defR[i].partial, coef(slot_defR[i]), nullptr, nch_def, 0});
-    for (size_t i = 0; i < g_defP.size(); ++i)
-        red_defP.push_back({g_defP[i].partial, coef(masksP[i].slot), mask_d + i * LD * LD, nch_def, 0});
-    auto *d_red_defR = put(red_defR);
-    auto *d_red_defP = put(red_defP);
+    for (size_t i = 0; i < defP.size(); ++i) defP[i].mask = mask_d + i * LD * LD;
     auto *d_lc_hist = put(lc_hist);
     auto *d_lc_x = put(lc_x);
     OpWork<T> *d_op_x[2] = {put(op_x[0]), put(op_x[1])};
     auto *d_rq = put(rq);
     auto *d_res = put(res);
-    auto *d_g_defR = put(g_defR);
-    auto *d_lc_defR = put(lc_defR);
+    auto *d_defR = put(defR);
     OpWork<T> *d_op_pre[2] = {put(op_pre[0]), put(op_pre[1])};
-    auto *d_g_defP = put(g_defP);
-    auto *d_lc_defP = put(lc_defP);
+    auto *d_defP = put(defP);
     OpWork<T> *d_op_p[2] = {put(op_p[0]), put(op_p[1])};
     auto *d_g_dual = put(g_dual);
     auto *d_e_rr2 = put(e_rr2);
@@ -1264,18 +1234,25 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
     launch_rayleigh<T>(d_rq, (int)rq.size(), m0max, N, st);
     // iteration
     launch_residual<T>(d_res, (int)res.size(), std::max(max_a, 1), N, st);
-    if (!g_defR.empty()) {
-        launch_gram<T>(d_g_defR, (int)g_defR.size(), nch_def, N, max_d, max_a, st);
-        launch_reduce(d_red_defR, (int)g_defR.size(), LD, st);
-        launch_lincomb<T>(d_lc_defR, (int)lc_defR.size(), N, max_a, st);
-    }
+    // algorithmic: each column read and written once, each vector and weight read once
+    auto defl_bytes = [&](const std::vector<DeflateWork<T>> &v) {
+        double b = 0;
+        for (auto &d : v) b += (double)N * ((2.0 * d.a + d.nd) * sizeof(cplx<T>) + (d.weight ? sizeof(T) : 0));
+        return b;
+    };
+    auto defl_flops = [&](const std::vector<DeflateWork<T>> &v) {
+        double f = 0;
+        for (auto &d : v) f += 2.0 * 8.0 * (double)N * d.a * d.nd;
+        return f;
+    };
+    if (!defR.empty())
+        timed(TIMED_DEFL, defl_bytes(defR), defl_flops(defR),
+              [&] { launch_deflate<T>(d_defR, (int)defR.size(), N, max_a, max_d, st); });
     launch_op<T>(MODE_PREC_FD, nx, ny, d_op_pre[0], (int)op_pre[0].size(), std::max(max_a, 1), twx, twy, st);
     launch_op<T>(MODE_PREC_TP, nx, ny, d_op_pre[1], (int)op_pre[1].size(), std::max(max_a, 1), twx, twy, st);
-    if (!g_defP.empty()) {
-        launch_gram<T>(d_g_defP, (int)g_defP.size(), nch_def, N, max_d, max_a, st);
-        launch_reduce(d_red_defP, (int)g_defP.size(), LD, st);
-        launch_lincomb<T>(d_lc_defP, (int)lc_defP.size(), N, max_a, st);
-    }
+    if (!defP.empty())
+        timed(TIMED_DEFL, defl_bytes(defP), defl_flops(defP),
+              [&] { launch_deflate<T>(d_defP, (int)defP.size(), N, max_a, max_d, st); });
     for (int pol = 0; pol < 2; ++pol) {
         if (op_p[pol].empty()) continue;
         timed(TIMED_THETA, (double)theta_cols[pol] * theta_bytes_per_col(pol),

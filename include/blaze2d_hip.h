@@ -143,7 +143,8 @@ typedef struct {
     int timed_kernel;        /* kernel class timed with HIP events on its stream:
                                 0 none, 1 Theta operator, 2 dual Rayleigh-Ritz Gram,
                                 3 Rayleigh-Ritz solve (Cholesky/SVQB + projection + Jacobi),
-                                4 Ritz-update linear combinations */
+                                4 Ritz-update linear combinations,
+                                5 residual and direction deflation */
 } bh_run_options;
 
 typedef struct {
