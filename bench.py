@@ -10,8 +10,9 @@ device; the clock brackets exactly --steps passes after --warmup untimed ones.
 
 Also reported on the same line:
   roofline      one kernel class (--timed-kernel: 3 = the Rayleigh-Ritz solve,
-                the dominant kernel; 2 = the dual Rayleigh-Ritz Gram; 1 = the
-                Theta operator) timed with HIP events on the
+                the dominant kernel; 1 = the Theta operator; 2 = the dual
+                Rayleigh-Ritz Gram; 4 = the Ritz update; 5 = the deflation)
+                timed with HIP events on the
                 stream each launch runs on, during the timed steps themselves:
                 algorithmic bytes (and FLOPs) per launch over the average
                 launch time, against 8 TB/s HBM (and 78.6 TFLOP/s FP64);
@@ -39,11 +40,12 @@ sys.path.insert(0, ROOT)
 METRIC = "converged k-point solves/sec, res64 12 bands, ≤1e-6 rel freq err vs FP64"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
-# HBM bytes per rr_kernel launch on the default workload (slots 512, groups 4),
-# FETCH_SIZE x2 + WRITE_SIZE x1 from two rocprofv3 --pmc passes scaled by a plain
-# copy (profiles/rr_kernel_traffic.md); reported only for that configuration.
-RR_TRAFFIC_BYTES = 18.68e6
-RR_TRAFFIC_CONFIG = (512, 4, 256)  # slots, groups, geometries the counters were read on
+# HBM bytes per rr_kernel launch on the default workload (2048 geometries,
+# slots 1024, groups 4), FETCH_SIZE x2 + WRITE_SIZE x1 from two rocprofv3 --pmc
+# passes scaled by a plain copy (profiles/rr_kernel_traffic.md); reported only
+# for that configuration.
+RR_TRAFFIC_BYTES = 14.48e6
+RR_TRAFFIC_CONFIG = (1024, 4, 2048)  # slots, groups, geometries the counters were read on
 TIMED_NAMES = {1: "op_kernel THETA_TM/TE (64x64 fused 2D FFT operator)",
                2: "gram2_kernel (dual Rayleigh-Ritz Gram Z^H BZ, Z^H AZ)",
                3: "rr_kernel (Cholesky/SVQB + projection + Jacobi Rayleigh-Ritz, FP64)",
