@@ -376,10 +376,15 @@ template <typename T>
 void launch_lincomb(const LincombWork<T> *dwork, int nwork, long rows, int maxq, hipStream_t st) {
     if (nwork <= 0 || maxq <= 0) return;
     const unsigned rb = (unsigned)((rows + LC_NT - 1) / LC_NT);
-    // 16 outputs per thread: a 32-output variant (one read of each input for
-    // up to 32 outputs) measured slower, its register use leaves too few
-    // waves to hide the column loads
-    hipLaunchKernelGGL((lincomb_kernel<T, 16>), dim3(rb, nwork, (maxq + 15) / 16), dim3(LC_NT), 0, st, dwork, rows);
+    // Outputs per thread: 24 when every item has at most 24 (the Ritz update's
+    // [X'|W'] = Z C with 2nb = 24 columns reads each input once), else 16.  A
+    // 32-output variant measured slower: its register use leaves too few waves
+    // to hide the column loads.
+    if (maxq > 16 && maxq <= 24)
+        hipLaunchKernelGGL((lincomb_kernel<T, 24>), dim3(rb, nwork, 1), dim3(LC_NT), 0, st, dwork, rows);
+    else
+        hipLaunchKernelGGL((lincomb_kernel<T, 16>), dim3(rb, nwork, (maxq + 15) / 16), dim3(LC_NT), 0, st, dwork,
+                           rows);
     BH_HIP_CHECK(hipGetLastError());
 }
 
