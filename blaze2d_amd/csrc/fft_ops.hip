@@ -142,10 +142,22 @@ __device__ inline void small_dft(cplx<T> *v) {
     }
 }
 
+// LDS position of the field element with flat index i (row-major, rows RW
+// long).  For 64-wide rows the low three bits of the column are XORed with the
+// next three: a radix-8 pass over rows whose eight lanes read elements 8 apart
+// (128 bytes, one bank cycle) then hits eight different bank groups instead of
+// one, and the accesses that are contiguous across lanes stay within the same
+// 128-byte block.  It is a bijection on every 64-element row.
+template <int RW>
+__device__ inline int lds_pos(int i) {
+    if constexpr (RW == 64) return i ^ ((i >> 3) & 7);
+    else return i;
+}
+
 // One radix pass over NLINES independent length-N transforms stored in LDS.
 // ROWS: elements contiguous (stride 1), lines NX apart; else elements NX
 // apart, lines contiguous.  NF fields are processed together.
-template <typename T, int N, int NLINES, bool ROWS, int S, bool INV, int NF, int NT, int FSTRIDE>
+template <typename T, int N, int NLINES, bool ROWS, int S, bool INV, int NF, int NT, int FSTRIDE, int RW>
 __device__ inline void fft_pass(cplx<T> *buf, const cplx<T> *__restrict__ tw) {
     constexpr int R = Plan<N>::r(S);
     constexpr int L = plan_len<N>(S);
@@ -163,10 +175,10 @@ __device__ inline void fft_pass(cplx<T> *buf, const cplx<T> *__restrict__ tw) {
         if (ROWS) { line = rem / NBF; bf = rem - line * NBF; }
         else { bf = rem / NLINES; line = rem - bf * NLINES; }
         int b = bf / M, j2 = bf - b * M;
-        cplx<T> *base = buf + f * FSTRIDE + line * LS + (b * L + j2) * ES;
+        const int e0 = f * FSTRIDE + line * LS + (b * L + j2) * ES;
         cplx<T> v[R];
 #pragma unroll
-        for (int j = 0; j < R; ++j) v[j] = base[j * M * ES];
+        for (int j = 0; j < R; ++j) v[j] = buf[lds_pos<RW>(e0 + j * M * ES)];
         if (!INV) {
             small_dft<T, R, false>(v);
             if (M > 1) {
@@ -181,38 +193,38 @@ __device__ inline void fft_pass(cplx<T> *buf, const cplx<T> *__restrict__ tw) {
             small_dft<T, R, true>(v);
         }
 #pragma unroll
-        for (int j = 0; j < R; ++j) base[j * M * ES] = v[j];
+        for (int j = 0; j < R; ++j) buf[lds_pos<RW>(e0 + j * M * ES)] = v[j];
     }
 }
 
-template <typename T, int N, int NLINES, bool ROWS, bool INV, int NF, int NT, int FSTRIDE, int S>
+template <typename T, int N, int NLINES, bool ROWS, bool INV, int NF, int NT, int FSTRIDE, int RW, int S>
 __device__ inline void fft_passes_fwd(cplx<T> *buf, const cplx<T> *tw) {
     if constexpr (S < Plan<N>::K) {
-        fft_pass<T, N, NLINES, ROWS, S, INV, NF, NT, FSTRIDE>(buf, tw);
+        fft_pass<T, N, NLINES, ROWS, S, INV, NF, NT, FSTRIDE, RW>(buf, tw);
         __syncthreads();
-        fft_passes_fwd<T, N, NLINES, ROWS, INV, NF, NT, FSTRIDE, S + 1>(buf, tw);
+        fft_passes_fwd<T, N, NLINES, ROWS, INV, NF, NT, FSTRIDE, RW, S + 1>(buf, tw);
     }
 }
-template <typename T, int N, int NLINES, bool ROWS, bool INV, int NF, int NT, int FSTRIDE, int S>
+template <typename T, int N, int NLINES, bool ROWS, bool INV, int NF, int NT, int FSTRIDE, int RW, int S>
 __device__ inline void fft_passes_inv(cplx<T> *buf, const cplx<T> *tw) {
     if constexpr (S >= 0) {
-        fft_pass<T, N, NLINES, ROWS, S, INV, NF, NT, FSTRIDE>(buf, tw);
+        fft_pass<T, N, NLINES, ROWS, S, INV, NF, NT, FSTRIDE, RW>(buf, tw);
         __syncthreads();
-        fft_passes_inv<T, N, NLINES, ROWS, INV, NF, NT, FSTRIDE, S - 1>(buf, tw);
+        fft_passes_inv<T, N, NLINES, ROWS, INV, NF, NT, FSTRIDE, RW, S - 1>(buf, tw);
     }
 }
 
 // natural -> digit-reversed (rows then columns)
 template <typename T, int NX, int NY, int NF, int NT>
 __device__ inline void fft2d_forward(cplx<T> *buf, const cplx<T> *twx, const cplx<T> *twy) {
-    fft_passes_fwd<T, NX, NY, true, false, NF, NT, NX * NY, 0>(buf, twx);
-    fft_passes_fwd<T, NY, NX, false, false, NF, NT, NX * NY, 0>(buf, twy);
+    fft_passes_fwd<T, NX, NY, true, false, NF, NT, NX * NY, NX, 0>(buf, twx);
+    fft_passes_fwd<T, NY, NX, false, false, NF, NT, NX * NY, NX, 0>(buf, twy);
 }
 // digit-reversed -> natural, unscaled
 template <typename T, int NX, int NY, int NF, int NT>
 __device__ inline void fft2d_inverse(cplx<T> *buf, const cplx<T> *twx, const cplx<T> *twy) {
-    fft_passes_inv<T, NY, NX, false, true, NF, NT, NX * NY, Plan<NY>::K - 1>(buf, twy);
-    fft_passes_inv<T, NX, NY, true, true, NF, NT, NX * NY, Plan<NX>::K - 1>(buf, twx);
+    fft_passes_inv<T, NY, NX, false, true, NF, NT, NX * NY, NX, Plan<NY>::K - 1>(buf, twy);
+    fft_passes_inv<T, NX, NY, true, true, NF, NT, NX * NY, NX, Plan<NX>::K - 1>(buf, twx);
 }
 
 // k+G at the frequency stored at LDS position (px, py), with the reference's
@@ -256,7 +268,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) op
     cplx<T> *out = w.out + (size_t)col * w.out_stride;
     const T invN = (T)(1.0 / (double)N);
 
-    for (int i = threadIdx.x; i < N; i += NT) F[i] = in[i];
+    for (int i = threadIdx.x; i < N; i += NT) F[lds_pos<NX>(i)] = in[i];
     __syncthreads();
     fft2d_forward<T, NX, NY, 1, NT>(F, twx, twy);
 
@@ -279,20 +291,20 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) op
                 }
                 if (P.gamma && cl) fac = 0.0;
             }
-            F[i] = scale_c(F[i], (T)fac * invN);
+            F[lds_pos<NX>(i)] = scale_c(F[lds_pos<NX>(i)], (T)fac * invN);
         }
         __syncthreads();
         fft2d_inverse<T, NX, NY, 1, NT>(F, twx, twy);
         if (MODE == MODE_PREC_FD && w.bout) {
 #pragma unroll 2
             for (int i = threadIdx.x; i < N; i += NT) {
-                cplx<T> v = F[i];
+                cplx<T> v = F[lds_pos<NX>(i)];
                 out[i] = v;
                 T e = w.eps[i];
                 w.bout[(size_t)col * w.out_stride + i] = {v.x * e, v.y * e};
             }
         } else {
-            for (int i = threadIdx.x; i < N; i += NT) out[i] = F[i];
+            for (int i = threadIdx.x; i < N; i += NT) out[i] = F[lds_pos<NX>(i)];
         }
     } else {
         cplx<T> *G2 = F + N;
@@ -303,17 +315,17 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) op
             double kx, ky, ksq;
             bool cl;
             kplusg<NX, NY>(P, px, py, kx, ky, ksq, cl);
-            cplx<T> u = F[i];
+            cplx<T> u = F[lds_pos<NX>(i)];
             if (MODE == MODE_THETA_TE) {
                 T skx = (T)kx * invN, sky = (T)ky * invN;
-                F[i] = {-u.y * skx, u.x * skx};   // u * (0 + i kx)
-                G2[i] = {-u.y * sky, u.x * sky};
+                F[lds_pos<NX>(i)] = {-u.y * skx, u.x * skx};   // u * (0 + i kx)
+                G2[lds_pos<NX>(i)] = {-u.y * sky, u.x * sky};
             } else {
                 double ik = 0.0;
                 if (!cl) { double den = ksq + P.shift; ik = den > 1e-15 ? 1.0 / den : 0.0; }
                 T fx = (T)(kx * ik) * invN, fy = (T)(ky * ik) * invN;
-                F[i] = {u.y * fx, -u.x * fx};     // u * (0 - i kx ik)
-                G2[i] = {u.y * fy, -u.x * fy};
+                F[lds_pos<NX>(i)] = {u.y * fx, -u.x * fx};     // u * (0 - i kx ik)
+                G2[lds_pos<NX>(i)] = {u.y * fy, -u.x * fy};
             }
         }
         __syncthreads();
@@ -321,16 +333,16 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) op
         // real-space step: THETA_TE eta tensor [xx, xy, yx, yy]; PREC_TP eps
 #pragma unroll 2
         for (int i = threadIdx.x; i < N; i += NT) {
-            cplx<T> gx = F[i], gy = G2[i];
+            cplx<T> gx = F[lds_pos<NX>(i)], gy = G2[lds_pos<NX>(i)];
             if (MODE == MODE_THETA_TE) {
                 const T *t = w.tensor + 4 * (size_t)i;
                 T t0 = t[0], t1 = t[1], t2 = t[2], t3 = t[3];
-                F[i] = {gx.x * t0 + gy.x * t1, gx.y * t0 + gy.y * t1};
-                G2[i] = {gx.x * t2 + gy.x * t3, gx.y * t2 + gy.y * t3};
+                F[lds_pos<NX>(i)] = {gx.x * t0 + gy.x * t1, gx.y * t0 + gy.y * t1};
+                G2[lds_pos<NX>(i)] = {gx.x * t2 + gy.x * t3, gx.y * t2 + gy.y * t3};
             } else {
                 T e = w.eps[i];
-                F[i] = {gx.x * e, gx.y * e};
-                G2[i] = {gy.x * e, gy.y * e};
+                F[lds_pos<NX>(i)] = {gx.x * e, gx.y * e};
+                G2[lds_pos<NX>(i)] = {gy.x * e, gy.y * e};
             }
         }
         __syncthreads();
@@ -349,15 +361,15 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) op
                 sc = ik;
             }
             T skx = (T)kx, sky = (T)ky, s = (T)sc * invN;
-            cplx<T> gx = F[i], gy = G2[i];
+            cplx<T> gx = F[lds_pos<NX>(i)], gy = G2[lds_pos<NX>(i)];
             // div = i kx gx + i ky gy ; out = -div
             T dre = -(skx * gx.y + sky * gy.y);
             T dim = skx * gx.x + sky * gy.x;
-            F[i] = {-dre * s, -dim * s};
+            F[lds_pos<NX>(i)] = {-dre * s, -dim * s};
         }
         __syncthreads();
         fft2d_inverse<T, NX, NY, 1, NT>(F, twx, twy);
-        for (int i = threadIdx.x; i < N; i += NT) out[i] = F[i];
+        for (int i = threadIdx.x; i < N; i += NT) out[i] = F[lds_pos<NX>(i)];
     }
 }
 
@@ -375,25 +387,25 @@ __global__ void __launch_bounds__(NT) fft_kernel(const OpWork<T> *__restrict__ w
     const cplx<T> *in = w.in + (size_t)col * w.in_stride;
     cplx<T> *out = w.out + (size_t)col * w.out_stride;
     if (!inverse) {
-        for (int i = threadIdx.x; i < N; i += NT) F[i] = in[i];
+        for (int i = threadIdx.x; i < N; i += NT) F[lds_pos<NX>(i)] = in[i];
         __syncthreads();
         fft2d_forward<T, NX, NY, 1, NT>(F, twx, twy);
         // digit-reversed -> natural on the way out
 #pragma unroll 2
         for (int i = threadIdx.x; i < N; i += NT) {
             int py = i / NX, px = i - py * NX;
-            out[rev_index<NY>(py) * NX + rev_index<NX>(px)] = F[i];
+            out[rev_index<NY>(py) * NX + rev_index<NX>(px)] = F[lds_pos<NX>(i)];
         }
     } else {
         const T invN = (T)(1.0 / (double)N);
 #pragma unroll 2
         for (int i = threadIdx.x; i < N; i += NT) {
             int py = i / NX, px = i - py * NX;
-            F[i] = scale_c(in[rev_index<NY>(py) * NX + rev_index<NX>(px)], invN);
+            F[lds_pos<NX>(i)] = scale_c(in[rev_index<NY>(py) * NX + rev_index<NX>(px)], invN);
         }
         __syncthreads();
         fft2d_inverse<T, NX, NY, 1, NT>(F, twx, twy);
-        for (int i = threadIdx.x; i < N; i += NT) out[i] = F[i];
+        for (int i = threadIdx.x; i < N; i += NT) out[i] = F[lds_pos<NX>(i)];
     }
 }
 
