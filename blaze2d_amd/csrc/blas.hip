@@ -164,7 +164,7 @@ void launch_reduce(const ReduceWork *dwork, int nwork, int ld, hipStream_t st) {
 constexpr int DEFL_NT = 512;
 constexpr int DEFL_RPT = 8;  // rows per thread: N <= 4096
 constexpr int DEFL_CPW = 1;  // columns per workgroup
-constexpr int DEFL_DB = 8;   // vectors per batch
+constexpr int DEFL_DB = 4;   // vectors per batch
 constexpr int NXCD = 8;
 
 template <typename T>
@@ -183,12 +183,11 @@ __global__ void __launch_bounds__(DEFL_NT) deflate_kernel(const DeflateWork<T> *
     __shared__ double red[NW][DEFL_MAXD][DEFL_CPW][2];
     __shared__ double coef[DEFL_MAXD][DEFL_CPW][2];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const bool full = nrows == (long)DEFL_NT * DEFL_RPT;
     double ur[DEFL_CPW][DEFL_RPT], ui[DEFL_CPW][DEFL_RPT], ws[DEFL_RPT];
 #pragma unroll
     for (int k = 0; k < DEFL_RPT; ++k) {
         const long i = tid + (long)k * DEFL_NT;
-        ws[k] = (w.weight && i < nrows) ? (double)w.weight[i] : 1.0;
+        ws[k] = i < nrows ? (w.weight ? (double)w.weight[i] : 1.0) : 0.0;  // 0: row past the end
     }
 #pragma unroll
     for (int c = 0; c < DEFL_CPW; ++c) {
@@ -201,27 +200,26 @@ __global__ void __launch_bounds__(DEFL_NT) deflate_kernel(const DeflateWork<T> *
             ui[c][k] = (double)v.y;
         }
     }
-    // coefficients <B d, r_c> = sum conj(w d) r_c
+    // coefficients <B d, r_c> = sum conj(w d) r_c.  The loads carry no
+    // branches (a batch past nd re-reads the last vector and its sums are
+    // dropped; rows past the end read the last row with weight 0), so the
+    // compiler can issue them all before the arithmetic.
     for (int d0 = 0; d0 < nd; d0 += DEFL_DB) {
         double sr[DEFL_DB][DEFL_CPW], si[DEFL_DB][DEFL_CPW];
 #pragma unroll
         for (int j = 0; j < DEFL_DB; ++j) {
 #pragma unroll
             for (int c = 0; c < DEFL_CPW; ++c) sr[j][c] = si[j][c] = 0.0;
-            if (d0 + j < nd) {
-                const cplx<T> *dv = w.d_ptrs[d0 + j];
+            const cplx<T> *dv = w.d_ptrs[d0 + j < nd ? d0 + j : nd - 1];
 #pragma unroll
-                for (int k = 0; k < DEFL_RPT; ++k) {
-                    const long i = tid + (long)k * DEFL_NT;
-                    if (full || i < nrows) {
-                        const cplx<T> x = dv[i];
-                        const double xr = ws[k] * (double)x.x, xi = ws[k] * (double)x.y;
+            for (int k = 0; k < DEFL_RPT; ++k) {
+                const long i = tid + (long)k * DEFL_NT;
+                const cplx<T> x = dv[i < nrows ? i : nrows - 1];
+                const double xr = ws[k] * (double)x.x, xi = ws[k] * (double)x.y;
 #pragma unroll
-                        for (int c = 0; c < DEFL_CPW; ++c) {
-                            sr[j][c] += xr * ur[c][k] + xi * ui[c][k];
-                            si[j][c] += xr * ui[c][k] - xi * ur[c][k];
-                        }
-                    }
+                for (int c = 0; c < DEFL_CPW; ++c) {
+                    sr[j][c] += xr * ur[c][k] + xi * ui[c][k];
+                    si[j][c] += xr * ui[c][k] - xi * ur[c][k];
                 }
             }
         }
@@ -259,29 +257,27 @@ __global__ void __launch_bounds__(DEFL_NT) deflate_kernel(const DeflateWork<T> *
         coef[d][c][1] = ci;
     }
     __syncthreads();
-    // r_c -= sum_d d coef(d, c)
+    // r_c -= sum_d d coef(d, c), branch-free as above (coefficient 0 past nd)
     for (int d0 = 0; d0 < nd; d0 += DEFL_DB) {
 #pragma unroll
         for (int j = 0; j < DEFL_DB; ++j) {
-            if (d0 + j < nd) {
-                const cplx<T> *dv = w.d_ptrs[d0 + j];
-                double cr[DEFL_CPW], ci[DEFL_CPW];
+            const bool in = d0 + j < nd;
+            const int dj = in ? d0 + j : nd - 1;
+            const cplx<T> *dv = w.d_ptrs[dj];
+            double cr[DEFL_CPW], ci[DEFL_CPW];
+#pragma unroll
+            for (int c = 0; c < DEFL_CPW; ++c) {
+                cr[c] = in ? coef[dj][c][0] : 0.0;
+                ci[c] = in ? coef[dj][c][1] : 0.0;
+            }
+#pragma unroll
+            for (int k = 0; k < DEFL_RPT; ++k) {
+                const long i = tid + (long)k * DEFL_NT;
+                const cplx<T> x = dv[i < nrows ? i : nrows - 1];
 #pragma unroll
                 for (int c = 0; c < DEFL_CPW; ++c) {
-                    cr[c] = coef[d0 + j][c][0];
-                    ci[c] = coef[d0 + j][c][1];
-                }
-#pragma unroll
-                for (int k = 0; k < DEFL_RPT; ++k) {
-                    const long i = tid + (long)k * DEFL_NT;
-                    if (full || i < nrows) {
-                        const cplx<T> x = dv[i];
-#pragma unroll
-                        for (int c = 0; c < DEFL_CPW; ++c) {
-                            ur[c][k] -= (double)x.x * cr[c] - (double)x.y * ci[c];
-                            ui[c][k] -= (double)x.x * ci[c] + (double)x.y * cr[c];
-                        }
-                    }
+                    ur[c][k] -= (double)x.x * cr[c] - (double)x.y * ci[c];
+                    ui[c][k] -= (double)x.x * ci[c] + (double)x.y * cr[c];
                 }
             }
         }

@@ -267,6 +267,12 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) op
     const cplx<T> *in = w.in + (size_t)col * w.in_stride;
     cplx<T> *out = w.out + (size_t)col * w.out_stride;
     const T invN = (T)(1.0 / (double)N);
+    // twiddle tables staged behind the fields
+    cplx<T> *tlx = F + NF * N, *tly = tlx + NX;
+    for (int i = threadIdx.x; i < NX; i += NT) tlx[i] = twx[i];
+    for (int i = threadIdx.x; i < NY; i += NT) tly[i] = twy[i];
+    twx = tlx;
+    twy = tly;
 
     for (int i = threadIdx.x; i < N; i += NT) F[lds_pos<NX>(i)] = in[i];
     __syncthreads();
@@ -386,6 +392,11 @@ __global__ void __launch_bounds__(NT) fft_kernel(const OpWork<T> *__restrict__ w
     if (col >= count) return;
     const cplx<T> *in = w.in + (size_t)col * w.in_stride;
     cplx<T> *out = w.out + (size_t)col * w.out_stride;
+    cplx<T> *tlx = F + N, *tly = tlx + NX;  // twiddle tables staged behind the field
+    for (int i = threadIdx.x; i < NX; i += NT) tlx[i] = twx[i];
+    for (int i = threadIdx.x; i < NY; i += NT) tly[i] = twy[i];
+    twx = tlx;
+    twy = tly;
     if (!inverse) {
         for (int i = threadIdx.x; i < N; i += NT) F[lds_pos<NX>(i)] = in[i];
         __syncthreads();
@@ -424,7 +435,8 @@ static void launch_sized(int mode, const OpWork<T> *dwork, int nwork, int maxcol
     constexpr int N = NX * NY;
     constexpr int NT1 = op_threads(N, 1), NT = op_threads(N, 2);
     dim3 grid(maxcols, nwork);
-    size_t sh1 = sizeof(cplx<T>) * N, sh2 = 2 * sh1;
+    // one or two fields, then the two twiddle tables
+    const size_t sh1 = sizeof(cplx<T>) * (N + NX + NY), sh2 = sizeof(cplx<T>) * (2 * N + NX + NY);
     // kernels using more than the default 64 KB of dynamic LDS must opt in
     static const bool attrs_set = [&] {
         BH_HIP_CHECK(hipFuncSetAttribute((const void *)op_kernel<T, NX, NY, MODE_THETA_TE, NT>,
