@@ -154,11 +154,23 @@ __device__ inline int lds_pos(int i) {
     else return i;
 }
 
+// Where a pass takes its input or puts its output: the LDS field, or (for
+// the first forward and the last inverse pass of a one-field operator) the
+// field in HBM, which saves one LDS round trip and one barrier at each end.
+struct PassIO {
+    const void *gin = nullptr;   // global input (natural order)
+    void *gout = nullptr;        // global output (natural order)
+    const void *geps = nullptr;  // optional: gbout = geps * output
+    void *gbout = nullptr;
+};
+
 // One radix pass over NLINES independent length-N transforms stored in LDS.
 // ROWS: elements contiguous (stride 1), lines NX apart; else elements NX
-// apart, lines contiguous.  NF fields are processed together.
-template <typename T, int N, int NLINES, bool ROWS, int S, bool INV, int NF, int NT, int FSTRIDE, int RW>
-__device__ inline void fft_pass(cplx<T> *buf, const cplx<T> *__restrict__ tw) {
+// apart, lines contiguous.  NF fields are processed together.  GIN / GOUT:
+// read the input from / write the output to HBM (io) instead of LDS.
+template <typename T, int N, int NLINES, bool ROWS, int S, bool INV, int NF, int NT, int FSTRIDE, int RW,
+          bool GIN = false, bool GOUT = false>
+__device__ inline void fft_pass(cplx<T> *buf, const cplx<T> *__restrict__ tw, const PassIO &io = PassIO()) {
     constexpr int R = Plan<N>::r(S);
     constexpr int L = plan_len<N>(S);
     constexpr int M = L / R;
@@ -178,7 +190,10 @@ __device__ inline void fft_pass(cplx<T> *buf, const cplx<T> *__restrict__ tw) {
         const int e0 = f * FSTRIDE + line * LS + (b * L + j2) * ES;
         cplx<T> v[R];
 #pragma unroll
-        for (int j = 0; j < R; ++j) v[j] = buf[lds_pos<RW>(e0 + j * M * ES)];
+        for (int j = 0; j < R; ++j) {
+            if constexpr (GIN) v[j] = static_cast<const cplx<T> *>(io.gin)[e0 + j * M * ES];
+            else v[j] = buf[lds_pos<RW>(e0 + j * M * ES)];
+        }
         if (!INV) {
             small_dft<T, R, false>(v);
             if (M > 1) {
@@ -193,7 +208,18 @@ __device__ inline void fft_pass(cplx<T> *buf, const cplx<T> *__restrict__ tw) {
             small_dft<T, R, true>(v);
         }
 #pragma unroll
-        for (int j = 0; j < R; ++j) buf[lds_pos<RW>(e0 + j * M * ES)] = v[j];
+        for (int j = 0; j < R; ++j) {
+            if constexpr (GOUT) {
+                const int e = e0 + j * M * ES;
+                static_cast<cplx<T> *>(io.gout)[e] = v[j];
+                if (io.gbout) {
+                    const T w = static_cast<const T *>(io.geps)[e];
+                    static_cast<cplx<T> *>(io.gbout)[e] = {v[j].x * w, v[j].y * w};
+                }
+            } else {
+                buf[lds_pos<RW>(e0 + j * M * ES)] = v[j];
+            }
+        }
     }
 }
 
@@ -205,12 +231,13 @@ __device__ inline void fft_passes_fwd(cplx<T> *buf, const cplx<T> *tw) {
         fft_passes_fwd<T, N, NLINES, ROWS, INV, NF, NT, FSTRIDE, RW, S + 1>(buf, tw);
     }
 }
-template <typename T, int N, int NLINES, bool ROWS, bool INV, int NF, int NT, int FSTRIDE, int RW, int S>
+template <typename T, int N, int NLINES, bool ROWS, bool INV, int NF, int NT, int FSTRIDE, int RW, int S,
+          int SEND = 0>
 __device__ inline void fft_passes_inv(cplx<T> *buf, const cplx<T> *tw) {
-    if constexpr (S >= 0) {
+    if constexpr (S >= SEND) {
         fft_pass<T, N, NLINES, ROWS, S, INV, NF, NT, FSTRIDE, RW>(buf, tw);
         __syncthreads();
-        fft_passes_inv<T, N, NLINES, ROWS, INV, NF, NT, FSTRIDE, RW, S - 1>(buf, tw);
+        fft_passes_inv<T, N, NLINES, ROWS, INV, NF, NT, FSTRIDE, RW, S - 1, SEND>(buf, tw);
     }
 }
 
@@ -225,6 +252,26 @@ template <typename T, int NX, int NY, int NF, int NT>
 __device__ inline void fft2d_inverse(cplx<T> *buf, const cplx<T> *twx, const cplx<T> *twy) {
     fft_passes_inv<T, NY, NX, false, true, NF, NT, NX * NY, NX, Plan<NY>::K - 1>(buf, twy);
     fft_passes_inv<T, NX, NY, true, true, NF, NT, NX * NY, NX, Plan<NX>::K - 1>(buf, twx);
+}
+// one field, natural order in HBM -> digit-reversed in LDS (the first row
+// pass reads HBM; twiddles must be visible in LDS before the call)
+template <typename T, int NX, int NY, int NT>
+__device__ inline void fft2d_forward_from(cplx<T> *buf, const cplx<T> *gin, const cplx<T> *twx,
+                                          const cplx<T> *twy) {
+    PassIO io;
+    io.gin = gin;
+    fft_pass<T, NX, NY, true, 0, false, 1, NT, NX * NY, NX, true, false>(buf, twx, io);
+    __syncthreads();
+    fft_passes_fwd<T, NX, NY, true, false, 1, NT, NX * NY, NX, 1>(buf, twx);
+    fft_passes_fwd<T, NY, NX, false, false, 1, NT, NX * NY, NX, 0>(buf, twy);
+}
+// one field, digit-reversed in LDS -> natural order in HBM (the last row pass
+// writes gout, and gbout = geps * gout when gbout is set), unscaled
+template <typename T, int NX, int NY, int NT>
+__device__ inline void fft2d_inverse_to(cplx<T> *buf, const PassIO &io, const cplx<T> *twx, const cplx<T> *twy) {
+    fft_passes_inv<T, NY, NX, false, true, 1, NT, NX * NY, NX, Plan<NY>::K - 1>(buf, twy);
+    fft_passes_inv<T, NX, NY, true, true, 1, NT, NX * NY, NX, Plan<NX>::K - 1, 1>(buf, twx);
+    fft_pass<T, NX, NY, true, 0, true, 1, NT, NX * NY, NX, false, true>(buf, twx, io);
 }
 
 // k+G at the frequency stored at LDS position (px, py), with the reference's
@@ -273,10 +320,8 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) op
     for (int i = threadIdx.x; i < NY; i += NT) tly[i] = twy[i];
     twx = tlx;
     twy = tly;
-
-    for (int i = threadIdx.x; i < N; i += NT) F[lds_pos<NX>(i)] = in[i];
     __syncthreads();
-    fft2d_forward<T, NX, NY, 1, NT>(F, twx, twy);
+    fft2d_forward_from<T, NX, NY, NT>(F, in, twx, twy);
 
     if constexpr (MODE == MODE_THETA_TM || MODE == MODE_PREC_FD) {
 #pragma unroll 2
@@ -300,18 +345,13 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) op
             F[lds_pos<NX>(i)] = scale_c(F[lds_pos<NX>(i)], (T)fac * invN);
         }
         __syncthreads();
-        fft2d_inverse<T, NX, NY, 1, NT>(F, twx, twy);
+        PassIO io;
+        io.gout = out;
         if (MODE == MODE_PREC_FD && w.bout) {
-#pragma unroll 2
-            for (int i = threadIdx.x; i < N; i += NT) {
-                cplx<T> v = F[lds_pos<NX>(i)];
-                out[i] = v;
-                T e = w.eps[i];
-                w.bout[(size_t)col * w.out_stride + i] = {v.x * e, v.y * e};
-            }
-        } else {
-            for (int i = threadIdx.x; i < N; i += NT) out[i] = F[lds_pos<NX>(i)];
+            io.geps = w.eps;
+            io.gbout = w.bout + (size_t)col * w.out_stride;
         }
+        fft2d_inverse_to<T, NX, NY, NT>(F, io, twx, twy);
     } else {
         cplx<T> *G2 = F + N;
         // gradient step: THETA_TE  g = i(k+G) u_hat ; PREC_TP  g = -i(k+G) u_hat / (|k+G|^2 + s)
@@ -374,8 +414,9 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) op
             F[lds_pos<NX>(i)] = {-dre * s, -dim * s};
         }
         __syncthreads();
-        fft2d_inverse<T, NX, NY, 1, NT>(F, twx, twy);
-        for (int i = threadIdx.x; i < N; i += NT) out[i] = F[lds_pos<NX>(i)];
+        PassIO io;
+        io.gout = out;
+        fft2d_inverse_to<T, NX, NY, NT>(F, io, twx, twy);
     }
 }
 
