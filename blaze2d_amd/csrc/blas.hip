@@ -981,12 +981,16 @@ size_t rr_shared_bytes(int maxn) {
 // far below what the eigenvalues (error ~ off^2 / gap) or the Ritz vectors need.
 constexpr double RR_STOP_RATIO = 1e-6;
 
-// Threads per Rayleigh-Ritz problem (fewer threads measured slower per problem:
-// 0.25 / 0.39 / 0.48 / 0.67 / 1.14 ms for 1024 / 512 / 256 / 128 / 64 threads,
-// 128 problems of order 36, 4 sweeps), and the LDS each workgroup requests so
-// that no other kernel's workgroups share its CU.
-constexpr int RR_NT = 1024;
-constexpr size_t RR_LDS_RESERVE = (size_t)150 * 1024;
+// Threads per Rayleigh-Ritz problem and the LDS each workgroup requests.  One
+// problem alone on a CU runs fastest with 1024 threads (0.25 / 0.39 / 0.48 ms
+// for 1024 / 512 / 256 threads, 128 problems of order 36, 4 sweeps), but two
+// 512-thread problems per CU use the CU better: on the res-64 sweep 512
+// threads with 75 KB each (two problems per CU, no other kernel's workgroups
+// beside them) ran 2% faster overall than 1024 threads with 150 KB (one per
+// CU); without the reservation, co-resident streaming kernels slow the
+// latency-bound Jacobi rounds down and the sweep runs 2-3% slower.
+constexpr int RR_NT = 512;
+constexpr size_t RR_LDS_RESERVE = (size_t)75 * 1024;
 
 template <int NT>
 __global__ void __launch_bounds__(NT) rr_kernel(const RRWork *__restrict__ work) {
