@@ -467,14 +467,22 @@ __device__ int jacobi_lds(double2 *A, double2 *V, int ld, int ne, JacobiRot *rot
     };
     int sweep = 0;
     for (; sweep < 40; ++sweep) {
-        double my_max2 = 0.0;  // rotating threads: largest relative rotation^2 this sweep
+        // rotating threads: 1 once a rotation of this sweep had relative size
+        // |a_pq|^2 / |a_pp a_qq| >= stop2 (a_pp a_qq = 0 counts as large)
+        double my_max2 = 0.0;
         for (int r = 0; r < ne - 1; ++r) {
             const int par = r & 1;
             if (tid < np) {
                 // round-robin pairing, pair slot t: slot 0 (r, ne-1); slot t ((r+t) mod (ne-1), (r-t) mod (ne-1))
+                // (both sums lie in [0, 2(ne-1)), so one conditional subtraction reduces them)
                 int p, q;
                 if (tid == 0) { p = r; q = ne - 1; }
-                else { p = (r + tid) % (ne - 1); q = (r - tid + ne - 1) % (ne - 1); }
+                else {
+                    p = r + tid;
+                    q = r - tid + ne - 1;
+                    if (p >= ne - 1) p -= ne - 1;
+                    if (q >= ne - 1) q -= ne - 1;
+                }
                 if (p > q) { int t = p; p = q; q = t; }
                 const double2 h = A[p + q * ld];
                 const double hh = h.x * h.x + h.y * h.y;
@@ -487,14 +495,18 @@ __device__ int jacobi_lds(double2 *A, double2 *V, int ld, int ne, JacobiRot *rot
                 int flag = 0;
                 if (hh != 0.0 && hh > 4.84e-32 * papq) {  // |h| > 2.2e-16 sqrt(|a_pp a_qq|)
                     flag = 1;
+                    // t = sgn / u with u = |tau| + sqrt(1 + tau^2) >= 1, so
+                    // c = 1 / sqrt(1 + t^2) = u w and |s| = |t| c = w with
+                    // w = 1 / sqrt(u^2 + 1): no division, c^2 + |s|^2 = 1 to rounding
                     const double ri = rsqrt(hh);  // 1 / |h|
                     const double tau = (aqq - app) * 0.5 * ri;
-                    const double t = (tau >= 0 ? -1.0 : 1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-                    const double c = rsqrt(1.0 + t * t);
-                    const double sr = t * c * ri;
-                    R.c = c;
+                    const double at = fmin(fabs(tau), 1e100);  // u^2 stays finite
+                    const double u = at + sqrt(1.0 + at * at);
+                    const double wv = rsqrt(u * u + 1.0);
+                    const double sr = (tau >= 0 ? -wv : wv) * ri;
+                    R.c = u * wv;
                     R.s = make_double2(sr * h.x, -sr * h.y);  // sigma = s conj(h/|h|)
-                    my_max2 = fmax(my_max2, papq > 0.0 ? hh / papq : 1.0);
+                    if (hh >= stop2 * papq) my_max2 = 1.0;
                 } else if (hh != 0.0) {
                     flag = 2;
                 }
