@@ -8,6 +8,23 @@
 
 namespace bh {
 
+// Complex multiply-accumulate as four FMAs (written out: the plain expression
+// acc += a*b + c*d compiles to a multiply, an FMA and an add).
+// (re, im) += a * b
+__device__ inline void cmac(double &re, double &im, double ar, double ai, double br, double bi) {
+    re = fma(ar, br, re);
+    re = fma(-ai, bi, re);
+    im = fma(ar, bi, im);
+    im = fma(ai, br, im);
+}
+// (re, im) += conj(a) * b
+__device__ inline void cmacc(double &re, double &im, double ar, double ai, double br, double bi) {
+    re = fma(ar, br, re);
+    re = fma(ai, bi, re);
+    im = fma(ar, bi, im);
+    im = fma(-ai, br, im);
+}
+
 // ---------------------------------------------------------------------------
 // Gram: grid = (nchunks, nwork); 256 threads as 16 x 16, each owning a BU x BU
 // block of the (up to 16 BU x 16 BU) output; rows staged through LDS 32 at a
@@ -67,8 +84,7 @@ __global__ void __launch_bounds__(256) gram_kernel(const GramWork<T> *__restrict
 #pragma unroll
                 for (int v = 0; v < BU; ++v) {
                     if (lower && v > u) continue;
-                    ar[u][v] += axr[u] * bxr[v] + axi[u] * bxi[v];
-                    ai[u][v] += axr[u] * bxi[v] - axi[u] * bxr[v];
+                    cmacc(ar[u][v], ai[u][v], axr[u], axi[u], bxr[v], bxi[v]);
                 }
         }
         __syncthreads();
@@ -218,8 +234,7 @@ __global__ void __launch_bounds__(DEFL_NT) deflate_kernel(const DeflateWork<T> *
                 const double xr = ws[k] * (double)x.x, xi = ws[k] * (double)x.y;
 #pragma unroll
                 for (int c = 0; c < DEFL_CPW; ++c) {
-                    sr[j][c] += xr * ur[c][k] + xi * ui[c][k];
-                    si[j][c] += xr * ui[c][k] - xi * ur[c][k];
+                    cmacc(sr[j][c], si[j][c], xr, xi, ur[c][k], ui[c][k]);
                 }
             }
         }
@@ -276,8 +291,7 @@ __global__ void __launch_bounds__(DEFL_NT) deflate_kernel(const DeflateWork<T> *
                 const cplx<T> x = dv[i < nrows ? i : nrows - 1];
 #pragma unroll
                 for (int c = 0; c < DEFL_CPW; ++c) {
-                    ur[c][k] -= (double)x.x * cr[c] - (double)x.y * ci[c];
-                    ui[c][k] -= (double)x.x * ci[c] + (double)x.y * cr[c];
+                    cmac(ur[c][k], ui[c][k], -(double)x.x, -(double)x.y, cr[c], ci[c]);
                 }
             }
         }
@@ -345,8 +359,7 @@ __global__ void __launch_bounds__(LC_NT) lincomb_kernel(const LincombWork<T> *__
 #pragma unroll
         for (int j = 0; j < LC_QG; ++j) {
             const double2 c = Cs[i * LC_QG + j];
-            accr[j] += xr * c.x - xi * c.y;
-            acci[j] += xr * c.y + xi * c.x;
+            cmac(accr[j], acci[j], xr, xi, c.x, c.y);
         }
     }
 #pragma unroll
@@ -863,28 +876,26 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) g
             Zs[r * S::COLS + c] = z;
             As[r * S::COLS + c] = a;
         }
-        if (weighted && tid < S::SR) Ws[tid] = tid < nr ? (double)w.bweight[rb + tid] : 0.0;
+        if (tid < S::SR) Ws[tid] = tid < nr ? (weighted ? (double)w.bweight[rb + tid] : 1.0) : 0.0;
         __syncthreads();
         if (active) {
             for (int r = g; r < nr; r += S::RG) {
                 double2 zi[3], bjv[3], ajv[3];
-                const T s = weighted ? (T)Ws[r] : (T)1;
+                const T s = (T)Ws[r];  // 1 when B = I
 #pragma unroll
                 for (int u = 0; u < 3; ++u) {
                     zi[u] = Zs[r * S::COLS + 3 * bi + u];
                     const double2 zj = Zs[r * S::COLS + 3 * bj + u];
                     // B z in storage precision, as a stored mass image would hold it
-                    bjv[u] = weighted ? make_double2((double)((T)zj.x * s), (double)((T)zj.y * s)) : zj;
+                    bjv[u] = make_double2((double)((T)zj.x * s), (double)((T)zj.y * s));
                     ajv[u] = As[r * S::COLS + 3 * bj + u];
                 }
 #pragma unroll
                 for (int u = 0; u < 3; ++u)
 #pragma unroll
-                    for (int v = 0; v < 3; ++v) {  // conj(z_i) * b_j
-                        a1r[u][v] += zi[u].x * bjv[v].x + zi[u].y * bjv[v].y;
-                        a1i[u][v] += zi[u].x * bjv[v].y - zi[u].y * bjv[v].x;
-                        a2r[u][v] += zi[u].x * ajv[v].x + zi[u].y * ajv[v].y;
-                        a2i[u][v] += zi[u].x * ajv[v].y - zi[u].y * ajv[v].x;
+                    for (int v = 0; v < 3; ++v) {  // conj(z_i) * b_j, conj(z_i) * a_j
+                        cmacc(a1r[u][v], a1i[u][v], zi[u].x, zi[u].y, bjv[v].x, bjv[v].y);
+                        cmacc(a2r[u][v], a2i[u][v], zi[u].x, zi[u].y, ajv[v].x, ajv[v].y);
                     }
             }
         }
