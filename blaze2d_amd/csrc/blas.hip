@@ -1110,10 +1110,9 @@ __global__ void __launch_bounds__(NT) rr_kernel(const RRWork *__restrict__ work)
             const int i = own_i[u], k = own_k[u];
             if (i < 0 || k < j) continue;
             double2 a = L[i + k * lda];
-            if (j > 0) {  // column j-1 (final) applied to the trailing entry
+            if (j > 0) {  // column j-1 (final) applied to the trailing entry: a -= x conj(y)
                 const double2 x = L[i + (j - 1) * lda], y = L[k + (j - 1) * lda];
-                a.x -= x.x * y.x + x.y * y.y;
-                a.y -= x.y * y.x - x.x * y.y;
+                cmacc(a.x, a.y, -y.x, -y.y, x.x, x.y);
             }
             if (k == j) {
                 double d = L[j + j * lda].x;
@@ -1156,8 +1155,7 @@ __global__ void __launch_bounds__(NT) rr_kernel(const RRWork *__restrict__ work)
                 } else {
                     const double2 l = L[k + i * lda];
                     double2 bk = A[k + c * lda];
-                    bk.x -= l.x * m.x - l.y * m.y;
-                    bk.y -= l.x * m.y + l.y * m.x;
+                    cmac(bk.x, bk.y, -l.x, -l.y, m.x, m.y);  // bk -= l m
                     A[k + c * lda] = bk;
                 }
             }
@@ -1225,8 +1223,7 @@ __global__ void __launch_bounds__(NT) rr_kernel(const RRWork *__restrict__ work)
         double xr = 0.0, xi = 0.0;
         for (int k = 0; k < n; ++k) {
             const double2 a = A[i + k * lda], t = Tm[k + c * lda];
-            xr += a.x * t.x - a.y * t.y;
-            xi += a.x * t.y + a.y * t.x;
+            cmac(xr, xi, a.x, a.y, t.x, t.y);
         }
         V[i + c * lda] = make_double2(xr, xi);
     }
@@ -1237,8 +1234,7 @@ __global__ void __launch_bounds__(NT) rr_kernel(const RRWork *__restrict__ work)
         if (a < r && c < r) {
             for (int k = 0; k < n; ++k) {  // conj(T[k][a]) X[k][c]
                 const double2 t = Tm[k + a * lda], x = V[k + c * lda];
-                mr += t.x * x.x + t.y * x.y;
-                mi += t.x * x.y - t.y * x.x;
+                cmacc(mr, mi, t.x, t.y, x.x, x.y);
             }
         }
         A[a + c * lda] = make_double2(mr, mi);
@@ -1287,8 +1283,7 @@ __global__ void __launch_bounds__(NT) rr_kernel(const RRWork *__restrict__ work)
         double cr = 0.0, ci = 0.0;
         for (int j = 0; j < r; ++j) {
             const double2 t = Tm[k + j * lda], y = V[j + i * lda];
-            cr += t.x * y.x - t.y * y.y;
-            ci += t.x * y.y + t.y * y.x;
+            cmac(cr, ci, t.x, t.y, y.x, y.y);
         }
         const size_t o = 2 * ((size_t)k + (size_t)col * w.ldc);
         w.C[o] = cr;
