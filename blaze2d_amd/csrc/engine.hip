@@ -29,6 +29,7 @@
 #include <thread>
 #include <cmath>
 #include <cstdio>
+#include <cstdint>
 #include <cstring>
 #include <numeric>
 
@@ -73,6 +74,15 @@ struct Staging {
 // Fixed pool of host threads for per-slot work that is independent across
 // slots (k-point transitions, dielectric tables of new jobs).  The calling
 // thread takes part; the first exception thrown by any item is rethrown.
+//
+// Batches are small (tens of items of ~10 us), so the pool is built for
+// latency.  One parallel_for at a time.  A generation number and the next item index
+// share one 64-bit word (gen << 32 | index), so a worker can claim an item
+// (compare-and-swap) only for the generation it read; a worker that wakes late
+// for a finished generation fails the claim and never touches a newer
+// generation's items.  The caller works through items too and returns once
+// every item has completed (not once every worker has woken up).  Workers
+// spin briefly between batches, then sleep on a condition variable.
 class WorkerPool {
   public:
     explicit WorkerPool(int nthreads) {
@@ -81,7 +91,7 @@ class WorkerPool {
     ~WorkerPool() {
         {
             std::lock_guard<std::mutex> lk(mu_);
-            stop_ = true;
+            stop_.store(true);
         }
         cv_.notify_all();
         for (auto &t : th_) t.join();
@@ -92,57 +102,77 @@ class WorkerPool {
             for (int i = 0; i < n; ++i) fn(i);
             return;
         }
-        {
+        const uint64_t g = (word_.load(std::memory_order_relaxed) >> 32) + 1;
+        Batch &b = batch_[g & 1];
+        b.fn.store(&fn, std::memory_order_relaxed);
+        b.n.store(n, std::memory_order_relaxed);
+        b.done.store(0, std::memory_order_relaxed);
+        err_ = nullptr;
+        word_.store(g << 32, std::memory_order_release);  // publish generation g, index 0
+        if (sleepers_.load(std::memory_order_acquire) > 0) {
             std::lock_guard<std::mutex> lk(mu_);
-            fn_ = &fn;
-            n_ = n;
-            next_.store(0);
-            active_ = (int)th_.size();
-            err_ = nullptr;
-            ++gen_;
+            cv_.notify_all();
         }
-        cv_.notify_all();
-        drain();
-        std::unique_lock<std::mutex> lk(mu_);
-        done_cv_.wait(lk, [this] { return active_ == 0; });
-        fn_ = nullptr;
+        work(g);
+        while (b.done.load(std::memory_order_acquire) < n) std::this_thread::yield();
         if (err_) std::rethrow_exception(err_);
     }
 
   private:
-    void drain() {
-        for (int i = next_.fetch_add(1); i < n_; i = next_.fetch_add(1)) {
+    // fn and n are atomic only because a worker that read an old generation
+    // may still compare against n while the slot is refilled; its claim then
+    // fails, so it never calls the new fn
+    struct Batch {
+        std::atomic<const std::function<void(int)> *> fn{nullptr};
+        std::atomic<int> n{0};
+        std::atomic<int> done{0};
+    };
+    // claims and runs items of generation g until none is left
+    void work(uint64_t g) {
+        Batch &b = batch_[g & 1];
+        uint64_t w = word_.load(std::memory_order_acquire);
+        while ((w >> 32) == g) {
+            const int i = (int)(w & 0xffffffffu);
+            if (i >= b.n.load(std::memory_order_relaxed)) return;
+            if (!word_.compare_exchange_weak(w, w + 1, std::memory_order_acq_rel, std::memory_order_acquire))
+                continue;  // w reloaded
             try {
-                (*fn_)(i);
+                (*b.fn.load(std::memory_order_relaxed))(i);
             } catch (...) {
                 std::lock_guard<std::mutex> lk(mu_);
                 if (!err_) err_ = std::current_exception();
             }
+            b.done.fetch_add(1, std::memory_order_acq_rel);
+            w = word_.load(std::memory_order_acquire);
         }
     }
     void loop() {
-        long seen = 0;
+        uint64_t seen = 0;
         while (true) {
-            {
-                std::unique_lock<std::mutex> lk(mu_);
-                cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
-                if (stop_) return;
-                seen = gen_;
+            uint64_t g = word_.load(std::memory_order_acquire) >> 32;
+            for (int spin = 0; g == seen && spin < 20000 && !stop_.load(std::memory_order_relaxed); ++spin) {
+                std::this_thread::yield();
+                g = word_.load(std::memory_order_acquire) >> 32;
             }
-            drain();
-            std::lock_guard<std::mutex> lk(mu_);
-            if (--active_ == 0) done_cv_.notify_all();
+            if (g == seen) {
+                std::unique_lock<std::mutex> lk(mu_);
+                sleepers_.fetch_add(1, std::memory_order_acq_rel);
+                cv_.wait(lk, [&] { return stop_.load() || (word_.load(std::memory_order_acquire) >> 32) != seen; });
+                sleepers_.fetch_sub(1, std::memory_order_acq_rel);
+                if (stop_.load()) return;
+                continue;
+            }
+            seen = g;
+            work(g);
         }
     }
     std::vector<std::thread> th_;
     std::mutex mu_;
-    std::condition_variable cv_, done_cv_;
-    const std::function<void(int)> *fn_ = nullptr;
-    int n_ = 0;
-    std::atomic<int> next_{0};
-    int active_ = 0;
-    long gen_ = 0;
-    bool stop_ = false;
+    std::condition_variable cv_;
+    std::atomic<uint64_t> word_{0};
+    Batch batch_[2];
+    std::atomic<int> sleepers_{0};
+    std::atomic<bool> stop_{false};
     std::exception_ptr err_;
 };
 
