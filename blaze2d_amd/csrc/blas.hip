@@ -588,6 +588,205 @@ __device__ int jacobi_lds(double2 *A, double2 *V, int ld, int ne, JacobiRot *rot
     return sweep < 40 ? sweep + 1 : 40;
 }
 
+// ---------------------------------------------------------------------------
+// Parallel cyclic Jacobi with one barrier per round, for the Rayleigh-Ritz
+// problem.  The matrix is double-buffered (A0, A1): round r's 2x2 blocks are
+// read from the current buffer and written rotated to the other, so the
+// threads computing round r+1's rotations (one per pair, in wave 0) can rebuild
+// the three entries each needs (a_pp, a_qq, a_pq after round r) from the
+// current buffer and round r's rotations while the other waves apply round r
+// to the blocks and to V.  Same pairing, rotations, stop rule and results as
+// jacobi_lds, with one barrier per round instead of two.  final_buf returns
+// which buffer holds the diagonalised matrix.
+// ---------------------------------------------------------------------------
+// pair slot t of round r: (r, ne-1) for t = 0, else ((r+t) mod (ne-1),
+// (r-t) mod (ne-1)); both sums lie in [0, 2(ne-1)); ordered p < q
+__device__ inline void rr_pair(int r, int t, int ne, int &p, int &q) {
+    if (t == 0) {
+        p = r;
+        q = ne - 1;
+    } else {
+        p = r + t;
+        q = r - t + ne - 1;
+        if (p >= ne - 1) p -= ne - 1;
+        if (q >= ne - 1) q -= ne - 1;
+    }
+    if (p > q) {
+        const int x = p;
+        p = q;
+        q = x;
+    }
+}
+// pair slot of index i in round r
+__device__ inline int rr_slot(int r, int i, int ne) {
+    if (i == ne - 1 || i == r) return 0;
+    int d = i - r;
+    if (d < 0) d += ne - 1;
+    return d <= ne / 2 - 1 ? d : ne - 1 - d;
+}
+// the rotation that annihilates a_pq (as jacobi_lds); big = 1 when its
+// relative size |a_pq|^2 / |a_pp a_qq| reaches stop2
+__device__ inline JacobiRot jacobi_rotation(int p, int q, double app, double aqq, double2 h, double stop2,
+                                            double &big) {
+    JacobiRot R;
+    R.c = 1.0;
+    R.s = make_double2(0.0, 0.0);
+    R.pad = 0;
+    const double hh = h.x * h.x + h.y * h.y;
+    const double papq = fabs(app * aqq);
+    int flag = 0;
+    if (hh != 0.0 && hh > 4.84e-32 * papq) {  // |h| > 2.2e-16 sqrt(|a_pp a_qq|)
+        flag = 1;
+        const double ri = rsqrt(hh);  // 1 / |h|
+        const double tau = (aqq - app) * 0.5 * ri;
+        const double at = fmin(fabs(tau), 1e100);
+        const double u = at + sqrt(1.0 + at * at);
+        const double wv = rsqrt(u * u + 1.0);
+        const double sr = (tau >= 0 ? -wv : wv) * ri;
+        R.c = u * wv;
+        R.s = make_double2(sr * h.x, -sr * h.y);  // sigma = s conj(h/|h|)
+        if (hh >= stop2 * papq) big = 1.0;
+    } else if (hh != 0.0) {
+        flag = 2;
+    }
+    R.pk = p | (q << 8) | (flag << 16);
+    return R;
+}
+// a 2x2 block (rows of pair R1, columns of pair R2) after the round's
+// rotations, as jacobi_lds applies them; diag: R1 and R2 are the same pair
+__device__ inline void jacobi_block(double2 &a11, double2 &a12, double2 &a21, double2 &a22, const JacobiRot &R1,
+                                    const JacobiRot &R2, bool diag) {
+    const int f1 = (R1.pk >> 16) & 3, f2 = (R2.pk >> 16) & 3;
+    if (f1 != 1 && f2 != 1) {
+        if (diag && f1 == 2) {
+            a12 = make_double2(0.0, 0.0);
+            a21 = make_double2(0.0, 0.0);
+        }
+        return;
+    }
+    if (f2 == 1) {  // columns: [:,p] = c [:,p] + s [:,q]; [:,q] = -conj(s) [:,p] + c [:,q]
+        const double c = R2.c;
+        const double2 s = R2.s;
+        double2 n11 = make_double2(c * a11.x + s.x * a12.x - s.y * a12.y, c * a11.y + s.x * a12.y + s.y * a12.x);
+        double2 n12 = make_double2(-(s.x * a11.x + s.y * a11.y) + c * a12.x, -(s.x * a11.y - s.y * a11.x) + c * a12.y);
+        double2 n21 = make_double2(c * a21.x + s.x * a22.x - s.y * a22.y, c * a21.y + s.x * a22.y + s.y * a22.x);
+        double2 n22 = make_double2(-(s.x * a21.x + s.y * a21.y) + c * a22.x, -(s.x * a21.y - s.y * a21.x) + c * a22.y);
+        a11 = n11; a12 = n12; a21 = n21; a22 = n22;
+    }
+    if (f1 == 1) {  // rows: [p,:] = c [p,:] + conj(s) [q,:]; [q,:] = -s [p,:] + c [q,:]
+        const double c = R1.c;
+        const double2 s = R1.s;
+        double2 n11 = make_double2(c * a11.x + s.x * a21.x + s.y * a21.y, c * a11.y + s.x * a21.y - s.y * a21.x);
+        double2 n21 = make_double2(-(s.x * a11.x - s.y * a11.y) + c * a21.x, -(s.x * a11.y + s.y * a11.x) + c * a21.y);
+        double2 n12 = make_double2(c * a12.x + s.x * a22.x + s.y * a22.y, c * a12.y + s.x * a22.y - s.y * a22.x);
+        double2 n22 = make_double2(-(s.x * a12.x - s.y * a12.y) + c * a22.x, -(s.x * a12.y + s.y * a12.x) + c * a22.y);
+        a11 = n11; a12 = n12; a21 = n21; a22 = n22;
+    }
+    if (diag) {  // the rotated pair: exact zero off-diagonal, real diagonal
+        a12 = make_double2(0.0, 0.0);
+        a21 = make_double2(0.0, 0.0);
+        a11.y = 0.0;
+        a22.y = 0.0;
+    }
+}
+// Rotating threads: tid < ne/2 (wave 0); block and V threads: tid >= 64.
+// Needs NT >= 128 and ne / 2 <= 32; rot holds 2 * ne / 2 entries, red one double.
+template <int NT>
+__device__ int jacobi_lds_db(double2 *A0, double2 *A1, double2 *V, int ld, int ne, JacobiRot *rot, double *red,
+                             double stop_ratio, int &final_buf) {
+    const int tid = threadIdx.x;
+    const int np = ne / 2;
+    const int nblk = np * np;
+    const int nwork = nblk + np * ne;  // block items, then V items (pair, row)
+    const double stop2 = stop_ratio * stop_ratio;
+    double big_cur = 0.0, big_next = 0.0;  // rotating threads: this sweep's and the next sweep's
+    if (tid < np) {  // round 0 of the first sweep, from A0
+        int p, q;
+        rr_pair(0, tid, ne, p, q);
+        rot[tid] = jacobi_rotation(p, q, A0[p + p * ld].x, A0[q + q * ld].x, A0[p + q * ld], stop2, big_cur);
+    }
+    __syncthreads();
+    int cur = 0, par = 0;
+    int sweep = 0;
+    for (; sweep < 40; ++sweep) {
+        for (int r = 0; r < ne - 1; ++r) {
+            const double2 *Ac = cur ? A1 : A0;
+            double2 *An = cur ? A0 : A1;
+            const JacobiRot *Rr = rot + par * np;
+            if (tid >= 64) {
+                for (int it = tid - 64; it < nwork; it += NT - 64) {
+                    if (it < nblk) {
+                        const int bt1 = it / np, bt2 = it - bt1 * np;
+                        const JacobiRot R1 = Rr[bt1], R2 = Rr[bt2];  // row pair, column pair
+                        const int p1 = R1.pk & 255, q1 = (R1.pk >> 8) & 255;
+                        const int p2 = R2.pk & 255, q2 = (R2.pk >> 8) & 255;
+                        double2 a11 = Ac[p1 + p2 * ld], a12 = Ac[p1 + q2 * ld];
+                        double2 a21 = Ac[q1 + p2 * ld], a22 = Ac[q1 + q2 * ld];
+                        jacobi_block(a11, a12, a21, a22, R1, R2, bt1 == bt2);
+                        An[p1 + p2 * ld] = a11;
+                        An[p1 + q2 * ld] = a12;
+                        An[q1 + p2 * ld] = a21;
+                        An[q1 + q2 * ld] = a22;
+                    } else {
+                        const int e = it - nblk;
+                        const int t = e / ne, k = e - t * ne;
+                        const JacobiRot R = Rr[t];
+                        if (((R.pk >> 16) & 3) == 1) {
+                            const int p = R.pk & 255, q = (R.pk >> 8) & 255;
+                            const double c = R.c;
+                            const double2 sg = R.s;
+                            const double2 vkp = V[k + p * ld], vkq = V[k + q * ld];
+                            V[k + p * ld] = make_double2(c * vkp.x + sg.x * vkq.x - sg.y * vkq.y,
+                                                         c * vkp.y + sg.x * vkq.y + sg.y * vkq.x);
+                            V[k + q * ld] = make_double2(-(sg.x * vkp.x + sg.y * vkp.y) + c * vkq.x,
+                                                         -(sg.x * vkp.y - sg.y * vkp.x) + c * vkq.y);
+                        }
+                    }
+                }
+            } else if (tid < np) {
+                // round r+1 (round 0 of the next sweep after the last round):
+                // a_pp, a_qq from the diagonal blocks of p's and q's round-r
+                // pairs, a_pq from their off-diagonal block
+                int rn = r + 1;
+                const bool wraps = rn == ne - 1;
+                if (wraps) rn = 0;
+                int p, q;
+                rr_pair(rn, tid, ne, p, q);
+                const int sP = rr_slot(r, p, ne), sQ = rr_slot(r, q, ne);  // equal only when ne = 2
+                const JacobiRot RP = Rr[sP], RQ = Rr[sQ];
+                const int pP = RP.pk & 255, qP = (RP.pk >> 8) & 255;
+                const int pQ = RQ.pk & 255, qQ = (RQ.pk >> 8) & 255;
+                double2 b11 = Ac[pP + pP * ld], b12 = Ac[pP + qP * ld], b21 = Ac[qP + pP * ld], b22 = Ac[qP + qP * ld];
+                jacobi_block(b11, b12, b21, b22, RP, RP, true);
+                const double app = p == pP ? b11.x : b22.x;
+                double2 d11 = Ac[pQ + pQ * ld], d12 = Ac[pQ + qQ * ld], d21 = Ac[qQ + pQ * ld], d22 = Ac[qQ + qQ * ld];
+                jacobi_block(d11, d12, d21, d22, RQ, RQ, true);
+                const double aqq = q == pQ ? d11.x : d22.x;
+                double2 c11 = Ac[pP + pQ * ld], c12 = Ac[pP + qQ * ld], c21 = Ac[qP + pQ * ld], c22 = Ac[qP + qQ * ld];
+                jacobi_block(c11, c12, c21, c22, RP, RQ, sP == sQ);
+                const double2 h = p == pP ? (q == pQ ? c11 : c12) : (q == pQ ? c21 : c22);
+                rot[(1 - par) * np + tid] = jacobi_rotation(p, q, app, aqq, h, stop2, wraps ? big_next : big_cur);
+            }
+            __syncthreads();
+            cur ^= 1;
+            par ^= 1;
+        }
+        if (tid < 64) {  // the rotating threads are all in wave 0
+            double m = big_cur;
+            for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+            if (tid == 0) red[0] = m;
+        }
+        __syncthreads();
+        const bool done = red[0] < stop2;  // also true when the sweep needed no rotation
+        __syncthreads();  // every thread has read red[0] before the next sweep writes it
+        big_cur = big_next;
+        big_next = 0.0;
+        if (done) break;
+    }
+    final_buf = cur;
+    return sweep < 40 ? sweep + 1 : 40;
+}
+
 __global__ void __launch_bounds__(1024) eigh_kernel(const EighWork *__restrict__ work) {
     const EighWork &w = work[blockIdx.x];
     const int n = w.dev_n ? *w.dev_n : w.n;
@@ -1256,14 +1455,29 @@ __global__ void __launch_bounds__(NT) rr_kernel(const RRWork *__restrict__ work)
         const int i = e % ne, j = e / ne;
         V[i + j * lda] = make_double2(i == j && i < re ? 1.0 : 0.0, 0.0);
     }
+    // T waits in this slot's coefficient array (the output, written only at
+    // the end) while its LDS holds the Jacobi double buffer
+    for (int e = tid; e < n * r; e += NT) {
+        const int k = e % n, c = e / n;
+        const double2 t = Tm[k + c * lda];
+        w.C[2 * ((size_t)k + (size_t)c * w.ldc)] = t.x;
+        w.C[2 * ((size_t)k + (size_t)c * w.ldc) + 1] = t.y;
+    }
     __syncthreads();
     // 4. M = Y diag(theta) Y^H
-    const int sw = jacobi_lds<NT>(A, V, lda, re, rot, red, RR_STOP_RATIO);
+    int fb = 0;
+    const int sw = jacobi_lds_db<NT>(A, Tm, V, lda, re, rot, red, RR_STOP_RATIO, fb);
+    const double2 *Af = fb ? Tm : A;  // the diagonalised matrix
+    double2 *Tl = fb ? A : Tm;        // free again: T comes back here
     if (tid == 0 && w.sweeps_out) {
         atomicAdd(w.sweeps_out + 3, sw);
         atomicMax(w.sweeps_out + 5, sw);
     }
-    for (int i = tid; i < r; i += NT) wsh[i] = A[i + i * lda].x;
+    for (int i = tid; i < r; i += NT) wsh[i] = Af[i + i * lda].x;
+    for (int e = tid; e < n * r; e += NT) {
+        const int k = e % n, c = e / n;
+        Tl[k + c * lda] = make_double2(w.C[2 * ((size_t)k + (size_t)c * w.ldc)], w.C[2 * ((size_t)k + (size_t)c * w.ldc) + 1]);
+    }
     __syncthreads();
     for (int i = tid; i < r; i += NT) {
         const double wi = wsh[i];
@@ -1282,7 +1496,7 @@ __global__ void __launch_bounds__(NT) rr_kernel(const RRWork *__restrict__ work)
         if (col >= cx + cw) continue;
         double cr = 0.0, ci = 0.0;
         for (int j = 0; j < r; ++j) {
-            const double2 t = Tm[k + j * lda], y = V[j + i * lda];
+            const double2 t = Tl[k + j * lda], y = V[j + i * lda];
             cmac(cr, ci, t.x, t.y, y.x, y.y);
         }
         const size_t o = 2 * ((size_t)k + (size_t)col * w.ldc);
