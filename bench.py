@@ -195,6 +195,20 @@ def main():
             "frac_fp64_peak": round(tfs / FP64_PEAK_TFLOPS, 5),
         }
     if not args.no_extra:
+        # dielectric tables of the whole sweep on the GPU (one launch, event-timed
+        # kernel) against the host path on a sample, one thread
+        from blaze2d_amd.bandstructure import dielectric, dielectric_batch
+        geoms = [j.geom for j in jobs[::2]]
+        dielectric_batch(be, geoms[:8], 64, 64, True)  # first-launch cost out of the way
+        _, _, _, kms = dielectric_batch(be, geoms, 64, 64, True)
+        t = time.perf_counter()
+        for g in geoms[:32]:
+            dielectric(g, 64, 64, True)
+        host_s = (time.perf_counter() - t) / 32
+        out["dielectric_gpu"] = {
+            "value": round(len(geoms) * 4096 / (kms * 1e-3), 1), "unit": "smoothed nodes/s",
+            "kernel_ms": round(kms, 4), "geometries": len(geoms), "grid": "64x64",
+            "host_one_thread_nodes_per_s": round(4096 / host_s, 1)}
         # plain FP64 path, one band structure at a time (the batching baseline)
         pj = jobs[:2]
         t = time.perf_counter()

@@ -91,7 +91,8 @@ EXPORTS = [
     "bh_version", "bh_last_error", "bh_device_count", "bh_context_create", "bh_context_destroy", "bh_synchronize",
     "bh_buffer_alloc", "bh_buffer_free", "bh_buffer_upload", "bh_buffer_download", "bh_fft2d", "bh_scale",
     "bh_axpy", "bh_dot", "bh_gram", "bh_lincomb", "bh_theta_create", "bh_theta_destroy", "bh_theta_apply",
-    "bh_mass_apply", "bh_precond_apply", "bh_dielectric", "bh_bandstructure_run", "bh_hermitian_eigh",
+    "bh_mass_apply", "bh_precond_apply", "bh_dielectric", "bh_dielectric_batch", "bh_bandstructure_run",
+    "bh_hermitian_eigh",
 ]
 
 _lib = None
@@ -132,6 +133,8 @@ def lib() -> ctypes.CDLL:
     L.bh_mass_apply.argtypes = [vp, vp, ctypes.c_int, vp, ctypes.c_int, ctypes.c_int]
     L.bh_precond_apply.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int]
     L.bh_dielectric.argtypes = [ctypes.POINTER(Geometry), ctypes.c_int, ctypes.c_int, ctypes.c_int, c_dp, c_dp, c_dp]
+    L.bh_dielectric_batch.argtypes = [ctypes.c_void_p, ctypes.POINTER(Geometry), ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_int, ctypes.c_int, c_dp, c_dp, c_dp, c_dp]
     L.bh_bandstructure_run.argtypes = [vp, ctypes.POINTER(Job), ctypes.c_int, ctypes.POINTER(RunOptions), c_dp, c_ip,
                                        c_ip, ctypes.POINTER(RunStats)]
     _lib = L

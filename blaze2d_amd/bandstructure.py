@@ -208,6 +208,26 @@ def run_with_options(backend: HipBackend, job: BandStructureJob, options: Option
     return res[0]
 
 
+def dielectric_batch(backend: HipBackend, geoms: Sequence[Geometry2D], nx: int, ny: int, smoothing: bool = True):
+    """Dielectric2D::from_geometry for many geometries at once, on the GPU.
+
+    Returns (eps[G, N], inv_eps[G, N], tensor[G, N, 4], kernel milliseconds).
+    """
+    ng = len(geoms)
+    if ng == 0:
+        raise ValueError("no geometries")
+    arr = (L.Geometry * ng)(*[g.to_c() for g in geoms])
+    n = nx * ny
+    eps = np.zeros((ng, n))
+    inv = np.zeros((ng, n))
+    ten = np.zeros((ng, n, 4))
+    ms = ctypes.c_double(0.0)
+    L.check(L.lib().bh_dielectric_batch(backend.handle, arr, ng, nx, ny, int(smoothing), eps.ctypes.data_as(L.c_dp),
+                                        inv.ctypes.data_as(L.c_dp), ten.ctypes.data_as(L.c_dp), ctypes.byref(ms)),
+            "bh_dielectric_batch")
+    return eps, inv, ten, ms.value
+
+
 def dielectric(geom: Geometry2D, nx: int, ny: int, smoothing: bool = True):
     """Dielectric2D::from_geometry (dielectric.rs:146-210): (eps, inv_eps, tensor[N,4])."""
     g = geom.to_c()

@@ -8,6 +8,7 @@
 
 #include "aux_kernels.h"
 #include "blas.h"
+#include "dielectric.h"
 #include "engine.h"
 #include "fft_ops.h"
 #include "hostmath.h"
@@ -526,6 +527,64 @@ int bh_dielectric(const bh_geometry *geom, int nx, int ny, int smoothing, double
     if (eps) std::memcpy(eps, d.eps.data(), n * 8);
     if (inv_eps) std::memcpy(inv_eps, d.inv_eps.data(), n * 8);
     if (tensor) std::memcpy(tensor, d.tensor.data(), 4 * n * 8);
+    return 0;
+    BH_API_CATCH
+}
+
+int bh_dielectric_batch(bh_context *ctx, const bh_geometry *geoms, int ngeom, int nx, int ny, int smoothing,
+                        double *eps, double *inv_eps, double *tensor, double *gpu_ms) {
+    BH_API_TRY
+    if (!ctx || !geoms || ngeom <= 0 || nx <= 0 || ny <= 0) return fail("bh_dielectric_batch: bad arguments");
+    if ((long)nx * ny > (1L << 26)) return fail("bh_dielectric_batch: grid too large");
+    std::vector<DielGeom> hg(ngeom);
+    for (int k = 0; k < ngeom; ++k) {
+        const bh_geometry &g = geoms[k];
+        if (g.n_atoms < 0 || g.n_atoms > BH_MAX_ATOMS) return fail("bh_dielectric_batch: n_atoms out of range");
+        DielGeom &d = hg[k];
+        d = DielGeom{};
+        for (int c = 0; c < 2; ++c) {
+            d.a1[c] = g.a1[c];
+            d.a2[c] = g.a2[c];
+        }
+        d.eps_bg = g.eps_bg;
+        d.n_atoms = g.n_atoms;
+        for (int a = 0; a < g.n_atoms; ++a)
+            for (int c = 0; c < 4; ++c) d.atoms[a][c] = g.atoms[a][c];
+    }
+    BH_HIP_CHECK(hipSetDevice(ctx->device));
+    const size_t n = (size_t)nx * ny * ngeom;
+    double *d_eps = nullptr, *d_inv = nullptr, *d_ten = nullptr;
+    void *d_geo = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto release = [&] {
+        if (d_eps) (void)hipFree(d_eps);
+        if (d_inv) (void)hipFree(d_inv);
+        if (d_ten) (void)hipFree(d_ten);
+        if (d_geo) (void)hipFree(d_geo);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    };
+    try {
+        if (eps) BH_HIP_CHECK(hipMalloc(&d_eps, n * sizeof(double)));
+        if (inv_eps) BH_HIP_CHECK(hipMalloc(&d_inv, n * sizeof(double)));
+        if (tensor) BH_HIP_CHECK(hipMalloc(&d_ten, 4 * n * sizeof(double)));
+        BH_HIP_CHECK(hipMalloc(&d_geo, dielectric_geom_bytes(ngeom)));
+        BH_HIP_CHECK(hipEventCreate(&e0));
+        BH_HIP_CHECK(hipEventCreate(&e1));
+        launch_dielectric(hg.data(), ngeom, nx, ny, smoothing != 0, d_eps, d_inv, d_ten, d_geo, ctx->st, e0, e1);
+        if (gpu_ms) {
+            float ms = 0.0f;
+            BH_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            *gpu_ms = ms;
+        }
+        if (eps) BH_HIP_CHECK(hipMemcpy(eps, d_eps, n * sizeof(double), hipMemcpyDeviceToHost));
+        if (inv_eps) BH_HIP_CHECK(hipMemcpy(inv_eps, d_inv, n * sizeof(double), hipMemcpyDeviceToHost));
+        if (tensor) BH_HIP_CHECK(hipMemcpy(tensor, d_ten, 4 * n * sizeof(double), hipMemcpyDeviceToHost));
+    } catch (...) {
+        release();
+        throw;
+    }
+    release();
     return 0;
     BH_API_CATCH
 }

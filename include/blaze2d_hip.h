@@ -113,6 +113,14 @@ typedef struct {
 /* Smoothed eps, inverse eps and inverse-eps tensor on an nx x ny node grid. */
 int bh_dielectric(const bh_geometry *geom, int nx, int ny, int smoothing, double *eps, double *inv_eps,
                   double *tensor);
+/* The same tables for ngeom geometries on one grid, computed on the GPU (one
+ * thread per node; replaces Dielectric2D::from_geometry, dielectric.rs:146-210,
+ * and the analytic smoothing of analytic_geometry.rs:183-541, for whole
+ * sweeps and large grids).  Host outputs laid out [geometry][node] (tensor
+ * [geometry][node][4]); any may be NULL.  gpu_ms (may be NULL) receives the
+ * kernel's event-timed duration. */
+int bh_dielectric_batch(bh_context *ctx, const bh_geometry *geoms, int ngeom, int nx, int ny, int smoothing,
+                        double *eps, double *inv_eps, double *tensor, double *gpu_ms);
 
 /* ---- Band structures (Eigensolver::solve driven by run_core) ----------- */
 
