@@ -696,8 +696,8 @@ __device__ int jacobi_lds_db(double2 *A0, double2 *A1, double2 *V, int ld, int n
                              double stop_ratio, int &final_buf) {
     const int tid = threadIdx.x;
     const int np = ne / 2;
-    const int nblk = np * np;
-    const int nwork = nblk + np * ne;  // block items, then V items (pair, row)
+    const int ntri = np * (np + 1) / 2;  // block pairs bt1 >= bt2: the lower block triangle
+    const int nwork = ntri + np * ne;    // block items, then V items (pair, row)
     const double stop2 = stop_ratio * stop_ratio;
     double big_cur = 0.0, big_next = 0.0;  // rotating threads: this sweep's and the next sweep's
     if (tid < np) {  // round 0 of the first sweep, from A0
@@ -715,8 +715,13 @@ __device__ int jacobi_lds_db(double2 *A0, double2 *A1, double2 *V, int ld, int n
             const JacobiRot *Rr = rot + par * np;
             if (tid >= 64) {
                 for (int it = tid - 64; it < nwork; it += NT - 64) {
-                    if (it < nblk) {
-                        const int bt1 = it / np, bt2 = it - bt1 * np;
+                    if (it < ntri) {
+                        // lower block (bt1 >= bt2); the matching upper block is
+                        // its conjugate transpose, written by the same thread
+                        int bt1 = (int)((sqrt(8.0 * it + 1.0) - 1.0) * 0.5);
+                        if ((bt1 + 1) * (bt1 + 2) / 2 <= it) ++bt1;
+                        if (bt1 * (bt1 + 1) / 2 > it) --bt1;
+                        const int bt2 = it - bt1 * (bt1 + 1) / 2;
                         const JacobiRot R1 = Rr[bt1], R2 = Rr[bt2];  // row pair, column pair
                         const int p1 = R1.pk & 255, q1 = (R1.pk >> 8) & 255;
                         const int p2 = R2.pk & 255, q2 = (R2.pk >> 8) & 255;
@@ -727,8 +732,14 @@ __device__ int jacobi_lds_db(double2 *A0, double2 *A1, double2 *V, int ld, int n
                         An[p1 + q2 * ld] = a12;
                         An[q1 + p2 * ld] = a21;
                         An[q1 + q2 * ld] = a22;
+                        if (bt1 != bt2) {
+                            An[p2 + p1 * ld] = make_double2(a11.x, -a11.y);
+                            An[q2 + p1 * ld] = make_double2(a12.x, -a12.y);
+                            An[p2 + q1 * ld] = make_double2(a21.x, -a21.y);
+                            An[q2 + q1 * ld] = make_double2(a22.x, -a22.y);
+                        }
                     } else {
-                        const int e = it - nblk;
+                        const int e = it - ntri;
                         const int t = e / ne, k = e - t * ne;
                         const JacobiRot R = Rr[t];
                         if (((R.pk >> 16) & 3) == 1) {
