@@ -75,7 +75,8 @@ def main():
     ap.add_argument("--geoms", type=int, default=2048, help="geometries in the sweep (x2 polarisations)")
     ap.add_argument("--slots", type=int, default=1024, help="concurrent band structures (0 = all)")
     ap.add_argument("--groups", type=int, default=4, help="slot groups pipelined on separate streams")
-    ap.add_argument("--precision", type=int, default=0)
+    ap.add_argument("--precision", type=int, default=0, choices=(0, 1, 2),
+                    help="0 FP64; 1 FP32 fields; 2 FP64 fields with FP32 preconditioners")
     ap.add_argument("--seed", type=int, default=0xB1A2E2D)
     ap.add_argument("--cpu-sample", type=int, default=4, help="band structures timed on the CPU oracle")
     ap.add_argument("--no-cpu", action="store_true")
@@ -151,7 +152,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == 0 else "f32+f64",
+        "dtype": {0: "f64", 1: "f32 fields, f64 accumulation", 2: "f64, f32 preconditioners"}[args.precision],
         "data": "synthetic (seeded geometry sweep, no external data)",
         "config": {
             "workload": f"res64 sweep: {args.geoms} seeded geometries (square/hex, eps_bg~U[2,16], 1-2 inclusions) "
@@ -226,6 +227,17 @@ def main():
                 "value": round(sum(int(r.converged[:-1].sum()) for r in resf) / dtf, 3),
                 "unit": "k-point solves/s",
                 "lobpcg_iterations_per_solve": round(stf.iterations / max(solves_of(jobs), 1), 3)}
+        # FP64 fields with the FD / TP preconditioners in FP32 (precision 2), one pass
+        if args.precision == 0:
+            t = time.perf_counter()
+            resm, stm = run_batch(be, jobs, ropts, slots=slots, precision=2, groups=args.groups)
+            dtm = time.perf_counter() - t
+            out["fp32_preconditioners"] = {
+                "value": round(sum(int(r.converged[:-1].sum()) for r in resm) / dtm, 3),
+                "unit": "k-point solves/s",
+                "lobpcg_iterations_per_solve": round(stm.iterations / max(solves_of(jobs), 1), 3),
+                "max_rel_freq_diff_vs_fp64_run": rel_err(np.concatenate([r.bands for r in resm]),
+                                                         np.concatenate([r.bands for r in last]))}
 
     if not args.no_cpu and args.cpu_sample > 0:
         from oracle import oracle as O

@@ -625,6 +625,7 @@ int bh_bandstructure_run(bh_context *ctx, const bh_job *jobs, int njobs, const b
     eo.slots = opts && opts->slots > 0 ? opts->slots : 64;
     eo.slots = std::min(eo.slots, njobs);
     eo.precision = opts ? opts->precision : 0;
+    if (eo.precision < 0 || eo.precision > 2) return fail("bh_bandstructure_run: precision must be 0, 1 or 2");
     eo.profile = opts ? opts->profile : 0;
     eo.groups = opts && opts->groups > 0 ? opts->groups : 4;
     eo.timed_kernel = opts ? opts->timed_kernel : 0;

@@ -59,7 +59,7 @@ struct RunStats {
 
 struct EngineOptions {
     int slots = 64;         // concurrent band structures
-    int precision = 0;      // 0 fp64, 1 fp32 storage (reserved)
+    int precision = 0;      // 0 fp64, 1 fp32 storage, 2 fp64 with FP32 preconditioners
     int profile = 0;        // time operator kernels with events (forces one group)
     int groups = 4;         // slot groups pipelined on separate streams
     int host_threads = 0;   // host threads for per-slot planning (0: hardware threads, at most 16)

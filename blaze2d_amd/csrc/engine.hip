@@ -1278,8 +1278,11 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
     if (!defR.empty())
         timed(TIMED_DEFL, defl_bytes(defR), defl_flops(defR),
               [&] { launch_deflate<T>(d_defR, (int)defR.size(), N, max_a, max_d, st); });
-    launch_op<T>(MODE_PREC_FD, nx, ny, d_op_pre[0], (int)op_pre[0].size(), std::max(max_a, 1), twx, twy, st);
-    launch_op<T>(MODE_PREC_TP, nx, ny, d_op_pre[1], (int)op_pre[1].size(), std::max(max_a, 1), twx, twy, st);
+    const bool prec_fp32 = eopt.precision == 2;
+    launch_op<T>(MODE_PREC_FD, nx, ny, d_op_pre[0], (int)op_pre[0].size(), std::max(max_a, 1), twx, twy, st,
+                 prec_fp32);
+    launch_op<T>(MODE_PREC_TP, nx, ny, d_op_pre[1], (int)op_pre[1].size(), std::max(max_a, 1), twx, twy, st,
+                 prec_fp32);
     if (!defP.empty())
         timed(TIMED_DEFL, defl_bytes(defP), defl_flops(defP),
               [&] { launch_deflate<T>(d_defP, (int)defP.size(), N, max_a, max_d, st); });

@@ -32,8 +32,10 @@ struct OpWork {
 
 bool grid_supported(int nx, int ny);
 
+// prec_fp32: the FD and TP preconditioners of an FP64 launch compute in FP32
+// (FP64 columns in and out)
 template <typename T>
 void launch_op(int mode, int nx, int ny, const OpWork<T> *dwork, int nwork, int maxcols, const cplx<T> *twx,
-               const cplx<T> *twy, hipStream_t st);
+               const cplx<T> *twy, hipStream_t st, bool prec_fp32 = false);
 
 }  // namespace bh

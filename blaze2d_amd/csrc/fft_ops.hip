@@ -19,6 +19,8 @@
 #include "common.h"
 #include "fft_ops.h"
 
+#include <type_traits>
+
 namespace bh {
 
 // Threads per workgroup of the 64 x 64 operator kernels, for one field (TM,
@@ -169,7 +171,7 @@ struct PassIO {
 // apart, lines contiguous.  NF fields are processed together.  GIN / GOUT:
 // read the input from / write the output to HBM (io) instead of LDS.
 template <typename T, int N, int NLINES, bool ROWS, int S, bool INV, int NF, int NT, int FSTRIDE, int RW,
-          bool GIN = false, bool GOUT = false>
+          bool GIN = false, bool GOUT = false, typename TS = T>
 __device__ inline void fft_pass(cplx<T> *buf, const cplx<T> *__restrict__ tw, const PassIO &io = PassIO()) {
     constexpr int R = Plan<N>::r(S);
     constexpr int L = plan_len<N>(S);
@@ -191,7 +193,10 @@ __device__ inline void fft_pass(cplx<T> *buf, const cplx<T> *__restrict__ tw, co
         cplx<T> v[R];
 #pragma unroll
         for (int j = 0; j < R; ++j) {
-            if constexpr (GIN) v[j] = static_cast<const cplx<T> *>(io.gin)[e0 + j * M * ES];
+            if constexpr (GIN) {  // HBM in storage precision TS
+                const cplx<TS> g = static_cast<const cplx<TS> *>(io.gin)[e0 + j * M * ES];
+                v[j] = {(T)g.x, (T)g.y};
+            }
             else v[j] = buf[lds_pos<RW>(e0 + j * M * ES)];
         }
         if (!INV) {
@@ -211,10 +216,11 @@ __device__ inline void fft_pass(cplx<T> *buf, const cplx<T> *__restrict__ tw, co
         for (int j = 0; j < R; ++j) {
             if constexpr (GOUT) {
                 const int e = e0 + j * M * ES;
-                static_cast<cplx<T> *>(io.gout)[e] = v[j];
+                const cplx<TS> o = {(TS)v[j].x, (TS)v[j].y};
+                static_cast<cplx<TS> *>(io.gout)[e] = o;
                 if (io.gbout) {
-                    const T w = static_cast<const T *>(io.geps)[e];
-                    static_cast<cplx<T> *>(io.gbout)[e] = {v[j].x * w, v[j].y * w};
+                    const TS w = static_cast<const TS *>(io.geps)[e];
+                    static_cast<cplx<TS> *>(io.gbout)[e] = {o.x * w, o.y * w};
                 }
             } else {
                 buf[lds_pos<RW>(e0 + j * M * ES)] = v[j];
@@ -255,23 +261,23 @@ __device__ inline void fft2d_inverse(cplx<T> *buf, const cplx<T> *twx, const cpl
 }
 // one field, natural order in HBM -> digit-reversed in LDS (the first row
 // pass reads HBM; twiddles must be visible in LDS before the call)
-template <typename T, int NX, int NY, int NT>
-__device__ inline void fft2d_forward_from(cplx<T> *buf, const cplx<T> *gin, const cplx<T> *twx,
+template <typename T, int NX, int NY, int NT, typename TS = T>
+__device__ inline void fft2d_forward_from(cplx<T> *buf, const cplx<TS> *gin, const cplx<T> *twx,
                                           const cplx<T> *twy) {
     PassIO io;
     io.gin = gin;
-    fft_pass<T, NX, NY, true, 0, false, 1, NT, NX * NY, NX, true, false>(buf, twx, io);
+    fft_pass<T, NX, NY, true, 0, false, 1, NT, NX * NY, NX, true, false, TS>(buf, twx, io);
     __syncthreads();
     fft_passes_fwd<T, NX, NY, true, false, 1, NT, NX * NY, NX, 1>(buf, twx);
     fft_passes_fwd<T, NY, NX, false, false, 1, NT, NX * NY, NX, 0>(buf, twy);
 }
 // one field, digit-reversed in LDS -> natural order in HBM (the last row pass
 // writes gout, and gbout = geps * gout when gbout is set), unscaled
-template <typename T, int NX, int NY, int NT>
+template <typename T, int NX, int NY, int NT, typename TS = T>
 __device__ inline void fft2d_inverse_to(cplx<T> *buf, const PassIO &io, const cplx<T> *twx, const cplx<T> *twy) {
     fft_passes_inv<T, NY, NX, false, true, 1, NT, NX * NY, NX, Plan<NY>::K - 1>(buf, twy);
     fft_passes_inv<T, NX, NY, true, true, 1, NT, NX * NY, NX, Plan<NX>::K - 1, 1>(buf, twx);
-    fft_pass<T, NX, NY, true, 0, true, 1, NT, NX * NY, NX, false, true>(buf, twx, io);
+    fft_pass<T, NX, NY, true, 0, true, 1, NT, NX * NY, NX, false, true, TS>(buf, twx, io);
 }
 
 // k+G at the frequency stored at LDS position (px, py), with the reference's
@@ -297,31 +303,33 @@ template <typename T>
 __device__ inline cplx<T> scale_c(cplx<T> a, T s) { return {a.x * s, a.y * s}; }
 
 // ---------------------------------------------------------------------------
-// The fused operator kernel.  grid = (max columns, work items).
+// The fused operator kernel.  grid = (max columns, work items).  TS is the
+// storage precision of the columns and tables, T the precision of the FFTs
+// and pointwise steps (T = float with TS = double: a preconditioner applied in
+// FP32 to FP64 columns).
 // ---------------------------------------------------------------------------
-template <typename T, int NX, int NY, int MODE, int NT>
-__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) op_kernel(const OpWork<T> *__restrict__ work, const cplx<T> *__restrict__ twx,
-                                                const cplx<T> *__restrict__ twy) {
+template <typename TS, int NX, int NY, int MODE, int NT, typename T = TS>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) op_kernel(const OpWork<TS> *__restrict__ work, const cplx<TS> *__restrict__ gtwx,
+                                                const cplx<TS> *__restrict__ gtwy) {
     constexpr int N = NX * NY;
     constexpr int NF = (MODE == MODE_THETA_TM || MODE == MODE_PREC_FD) ? 1 : 2;
     extern __shared__ __attribute__((aligned(16))) char smem_raw[];
     cplx<T> *F = reinterpret_cast<cplx<T> *>(smem_raw);
-    const OpWork<T> &w = work[blockIdx.y];
+    const OpWork<TS> &w = work[blockIdx.y];
     const int count = w.dev_count ? *w.dev_count : w.count;
     const int col = blockIdx.x;
     if (col >= count) return;
     const OpParams P = *w.params;
-    const cplx<T> *in = w.in + (size_t)col * w.in_stride;
-    cplx<T> *out = w.out + (size_t)col * w.out_stride;
+    const cplx<TS> *in = w.in + (size_t)col * w.in_stride;
+    cplx<TS> *out = w.out + (size_t)col * w.out_stride;
     const T invN = (T)(1.0 / (double)N);
     // twiddle tables staged behind the fields
     cplx<T> *tlx = F + NF * N, *tly = tlx + NX;
-    for (int i = threadIdx.x; i < NX; i += NT) tlx[i] = twx[i];
-    for (int i = threadIdx.x; i < NY; i += NT) tly[i] = twy[i];
-    twx = tlx;
-    twy = tly;
+    for (int i = threadIdx.x; i < NX; i += NT) tlx[i] = {(T)gtwx[i].x, (T)gtwx[i].y};
+    for (int i = threadIdx.x; i < NY; i += NT) tly[i] = {(T)gtwy[i].x, (T)gtwy[i].y};
+    const cplx<T> *twx = tlx, *twy = tly;
     __syncthreads();
-    fft2d_forward_from<T, NX, NY, NT>(F, in, twx, twy);
+    fft2d_forward_from<T, NX, NY, NT, TS>(F, in, twx, twy);
 
     if constexpr (MODE == MODE_THETA_TM || MODE == MODE_PREC_FD) {
 #pragma unroll 2
@@ -351,7 +359,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) op
             io.geps = w.eps;
             io.gbout = w.bout + (size_t)col * w.out_stride;
         }
-        fft2d_inverse_to<T, NX, NY, NT>(F, io, twx, twy);
+        fft2d_inverse_to<T, NX, NY, NT, TS>(F, io, twx, twy);
     } else {
         cplx<T> *G2 = F + N;
         // gradient step: THETA_TE  g = i(k+G) u_hat ; PREC_TP  g = -i(k+G) u_hat / (|k+G|^2 + s)
@@ -381,12 +389,12 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) op
         for (int i = threadIdx.x; i < N; i += NT) {
             cplx<T> gx = F[lds_pos<NX>(i)], gy = G2[lds_pos<NX>(i)];
             if (MODE == MODE_THETA_TE) {
-                const T *t = w.tensor + 4 * (size_t)i;
-                T t0 = t[0], t1 = t[1], t2 = t[2], t3 = t[3];
+                const TS *t = w.tensor + 4 * (size_t)i;
+                T t0 = (T)t[0], t1 = (T)t[1], t2 = (T)t[2], t3 = (T)t[3];
                 F[lds_pos<NX>(i)] = {gx.x * t0 + gy.x * t1, gx.y * t0 + gy.y * t1};
                 G2[lds_pos<NX>(i)] = {gx.x * t2 + gy.x * t3, gx.y * t2 + gy.y * t3};
             } else {
-                T e = w.eps[i];
+                T e = (T)w.eps[i];
                 F[lds_pos<NX>(i)] = {gx.x * e, gx.y * e};
                 G2[lds_pos<NX>(i)] = {gy.x * e, gy.y * e};
             }
@@ -416,7 +424,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) op
         __syncthreads();
         PassIO io;
         io.gout = out;
-        fft2d_inverse_to<T, NX, NY, NT>(F, io, twx, twy);
+        fft2d_inverse_to<T, NX, NY, NT, TS>(F, io, twx, twy);
     }
 }
 
@@ -470,9 +478,39 @@ constexpr int op_threads(int N, int NF) {
     return N >= 1024 ? 256 : 128;
 }
 
+// FP32 preconditioners on FP64 columns: FP32 fields need half the LDS, so the
+// two-field TP kernel fits two 512-thread workgroups per CU
+template <int NX, int NY, int MODE>
+static void launch_prec_fp32(const OpWork<double> *dwork, int nwork, int maxcols, const cplx<double> *twx,
+                             const cplx<double> *twy, hipStream_t st) {
+    constexpr int N = NX * NY;
+    constexpr int NF = MODE == MODE_PREC_FD ? 1 : 2;
+    constexpr int NT = N >= 4096 ? 512 : (N >= 1024 ? 256 : 128);
+    const size_t sh = sizeof(cplx<float>) * (NF * N + NX + NY);
+    static const bool attr_set = [&] {
+        BH_HIP_CHECK(hipFuncSetAttribute((const void *)op_kernel<double, NX, NY, MODE, NT, float>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+        return true;
+    }();
+    (void)attr_set;
+    hipLaunchKernelGGL((op_kernel<double, NX, NY, MODE, NT, float>), dim3(maxcols, nwork), dim3(NT), sh, st, dwork,
+                       twx, twy);
+    BH_HIP_CHECK(hipGetLastError());
+}
+
 template <typename T, int NX, int NY>
 static void launch_sized(int mode, const OpWork<T> *dwork, int nwork, int maxcols, const cplx<T> *twx,
-                         const cplx<T> *twy, hipStream_t st) {
+                         const cplx<T> *twy, hipStream_t st, bool prec_fp32) {
+    if constexpr (std::is_same<T, double>::value) {
+        if (prec_fp32 && mode == MODE_PREC_FD) {
+            launch_prec_fp32<NX, NY, MODE_PREC_FD>(dwork, nwork, maxcols, twx, twy, st);
+            return;
+        }
+        if (prec_fp32 && mode == MODE_PREC_TP) {
+            launch_prec_fp32<NX, NY, MODE_PREC_TP>(dwork, nwork, maxcols, twx, twy, st);
+            return;
+        }
+    }
     constexpr int N = NX * NY;
     constexpr int NT1 = op_threads(N, 1), NT = op_threads(N, 2);
     dim3 grid(maxcols, nwork);
@@ -524,23 +562,23 @@ bool grid_supported(int nx, int ny) {
 
 template <typename T>
 void launch_op(int mode, int nx, int ny, const OpWork<T> *dwork, int nwork, int maxcols, const cplx<T> *twx,
-               const cplx<T> *twy, hipStream_t st) {
+               const cplx<T> *twy, hipStream_t st, bool prec_fp32) {
     if (nwork <= 0 || maxcols <= 0) return;
     if (nx != ny) throw std::runtime_error("only square grids are instantiated");
     switch (nx) {
-        case 8: launch_sized<T, 8, 8>(mode, dwork, nwork, maxcols, twx, twy, st); break;
-        case 16: launch_sized<T, 16, 16>(mode, dwork, nwork, maxcols, twx, twy, st); break;
-        case 24: launch_sized<T, 24, 24>(mode, dwork, nwork, maxcols, twx, twy, st); break;
-        case 32: launch_sized<T, 32, 32>(mode, dwork, nwork, maxcols, twx, twy, st); break;
-        case 48: launch_sized<T, 48, 48>(mode, dwork, nwork, maxcols, twx, twy, st); break;
-        case 64: launch_sized<T, 64, 64>(mode, dwork, nwork, maxcols, twx, twy, st); break;
+        case 8: launch_sized<T, 8, 8>(mode, dwork, nwork, maxcols, twx, twy, st, prec_fp32); break;
+        case 16: launch_sized<T, 16, 16>(mode, dwork, nwork, maxcols, twx, twy, st, prec_fp32); break;
+        case 24: launch_sized<T, 24, 24>(mode, dwork, nwork, maxcols, twx, twy, st, prec_fp32); break;
+        case 32: launch_sized<T, 32, 32>(mode, dwork, nwork, maxcols, twx, twy, st, prec_fp32); break;
+        case 48: launch_sized<T, 48, 48>(mode, dwork, nwork, maxcols, twx, twy, st, prec_fp32); break;
+        case 64: launch_sized<T, 64, 64>(mode, dwork, nwork, maxcols, twx, twy, st, prec_fp32); break;
         default: throw std::runtime_error("grid size " + std::to_string(nx) + " not instantiated");
     }
 }
 
 template void launch_op<double>(int, int, int, const OpWork<double> *, int, int, const cplx<double> *,
-                                const cplx<double> *, hipStream_t);
+                                const cplx<double> *, hipStream_t, bool);
 template void launch_op<float>(int, int, int, const OpWork<float> *, int, int, const cplx<float> *,
-                               const cplx<float> *, hipStream_t);
+                               const cplx<float> *, hipStream_t, bool);
 
 }  // namespace bh

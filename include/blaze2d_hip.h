@@ -145,7 +145,8 @@ typedef struct {
 
 typedef struct {
     int slots;               /* concurrent band structures on the GPU (0 -> 64) */
-    int precision;           /* 0 FP64 */
+    int precision;           /* 0 FP64; 1 FP32 fields (FP64 accumulation and dense algebra);
+                                2 FP64 fields with the preconditioners (FD, TP) computed in FP32 */
     int profile;             /* 1: time operator kernels with HIP events (runs one slot group) */
     int groups;              /* slot groups pipelined on separate HIP streams (0 -> 4) */
     int timed_kernel;        /* kernel class timed with HIP events on its stream:

@@ -197,3 +197,20 @@ def test_res64_sweep_sample_all_bands_converged(be):
             lattice=lat, eps_bg=j.geom.eps_bg, atoms=[(a.pos[0], a.pos[1], a.radius, a.eps_inside) for a in j.geom.atoms],
             nx=64, ny=64, pol=j.pol, n_bands=12, tol=1e-7, kpath=j.k_path, exit_rule=1))
         _compare_bands(r, om, it, f"sweep0 {j.pol}")
+
+
+@pytest.mark.parametrize("precision", [2])
+def test_fp32_preconditioner_sweep_sample(be, precision):
+    # precision 2: FP64 fields with the FD / TP preconditioners computed in
+    # FP32; the preconditioner only shapes the search directions, so the
+    # converged frequencies must still meet the FP64 bound (sweep geometries
+    # 0-2, TM + TE, all-bands-converged exit)
+    from blaze2d_amd.sweep import res64_sweep
+    jobs = res64_sweep(3)
+    res, st = B.run_batch(be, jobs, B.RunOptions(exit_rule=1), slots=6, precision=precision)
+    for j, r in zip(jobs, res):
+        lat = "square" if abs(j.geom.lattice.a2[0]) < 1e-12 else "hex"
+        om, it, cv, tot = O.bandstructure(O.Job(
+            lattice=lat, eps_bg=j.geom.eps_bg, atoms=[(a.pos[0], a.pos[1], a.radius, a.eps_inside) for a in j.geom.atoms],
+            nx=64, ny=64, pol=j.pol, n_bands=12, tol=1e-7, kpath=j.k_path, exit_rule=1))
+        _compare_bands(r, om, it, f"fp32-prec {j.pol}")
