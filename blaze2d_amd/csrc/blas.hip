@@ -690,7 +690,7 @@ __device__ inline void jacobi_block(double2 &a11, double2 &a12, double2 &a21, do
     }
 }
 // Rotating threads: tid < ne/2 (wave 0); block and V threads: tid >= 64.
-// Needs NT >= 128 and ne / 2 <= 32; rot holds 2 * ne / 2 entries, red one double.
+// Needs NT >= 128 and ne <= 48; rot holds 2 * ne / 2 entries, red one double.
 template <int NT>
 __device__ int jacobi_lds_db(double2 *A0, double2 *A1, double2 *V, int ld, int ne, JacobiRot *rot, double *red,
                              double stop_ratio, int &final_buf) {
@@ -699,6 +699,27 @@ __device__ int jacobi_lds_db(double2 *A0, double2 *A1, double2 *V, int ld, int n
     const int ntri = np * (np + 1) / 2;  // block pairs bt1 >= bt2: the lower block triangle
     const int nwork = ntri + np * ne;    // block items, then V items (pair, row)
     const double stop2 = stop_ratio * stop_ratio;
+    // A worker's items are the same in every round: decode them once.  Item
+    // it = tid - 64 + j (NT - 64); a block item packs bt1 | bt2 << 8, a V item
+    // t | k << 8 | 1 << 16, and -1 marks no item.  ne <= 48 gives at most
+    // 300 + 1152 items.
+    constexpr int JW = (300 + 1152 + (NT - 64) - 1) / (NT - 64);
+    int item[JW];
+#pragma unroll
+    for (int j = 0; j < JW; ++j) {
+        const int it = tid - 64 + j * (NT - 64);
+        item[j] = -1;
+        if (tid < 64 || it >= nwork) continue;
+        if (it < ntri) {
+            int bt1 = (int)((sqrt(8.0 * it + 1.0) - 1.0) * 0.5);
+            if ((bt1 + 1) * (bt1 + 2) / 2 <= it) ++bt1;
+            if (bt1 * (bt1 + 1) / 2 > it) --bt1;
+            item[j] = bt1 | (it - bt1 * (bt1 + 1) / 2) << 8;
+        } else {
+            const int e = it - ntri;
+            item[j] = e / ne | (e % ne) << 8 | 1 << 16;
+        }
+    }
     double big_cur = 0.0, big_next = 0.0;  // rotating threads: this sweep's and the next sweep's
     if (tid < np) {  // round 0 of the first sweep, from A0
         int p, q;
@@ -714,14 +735,14 @@ __device__ int jacobi_lds_db(double2 *A0, double2 *A1, double2 *V, int ld, int n
             double2 *An = cur ? A0 : A1;
             const JacobiRot *Rr = rot + par * np;
             if (tid >= 64) {
-                for (int it = tid - 64; it < nwork; it += NT - 64) {
-                    if (it < ntri) {
+#pragma unroll
+                for (int j = 0; j < JW; ++j) {
+                    const int d = item[j];
+                    if (d < 0) break;
+                    if (!(d >> 16)) {
                         // lower block (bt1 >= bt2); the matching upper block is
                         // its conjugate transpose, written by the same thread
-                        int bt1 = (int)((sqrt(8.0 * it + 1.0) - 1.0) * 0.5);
-                        if ((bt1 + 1) * (bt1 + 2) / 2 <= it) ++bt1;
-                        if (bt1 * (bt1 + 1) / 2 > it) --bt1;
-                        const int bt2 = it - bt1 * (bt1 + 1) / 2;
+                        const int bt1 = d & 255, bt2 = (d >> 8) & 255;
                         const JacobiRot R1 = Rr[bt1], R2 = Rr[bt2];  // row pair, column pair
                         const int p1 = R1.pk & 255, q1 = (R1.pk >> 8) & 255;
                         const int p2 = R2.pk & 255, q2 = (R2.pk >> 8) & 255;
@@ -739,8 +760,7 @@ __device__ int jacobi_lds_db(double2 *A0, double2 *A1, double2 *V, int ld, int n
                             An[q2 + q1 * ld] = make_double2(a22.x, -a22.y);
                         }
                     } else {
-                        const int e = it - ntri;
-                        const int t = e / ne, k = e - t * ne;
+                        const int t = d & 255, k = (d >> 8) & 255;
                         const JacobiRot R = Rr[t];
                         if (((R.pk >> 16) & 3) == 1) {
                             const int p = R.pk & 255, q = (R.pk >> 8) & 255;

@@ -623,6 +623,11 @@ void Engine<T>::transition_math(int s) {
 
     std::vector<int> perm(nE);
     std::iota(perm.begin(), perm.end(), 0);
+    // the tracking overlap's polar factor, kept for the prediction below
+    std::vector<zd> track_u;
+    double track_smin = 0.0;
+    bool track_full = false;
+    int track_m = 0;
     bool next_solve = false;
     int knext = sl.k + 1;
     if (nE > 0) {
@@ -634,7 +639,8 @@ void Engine<T>::transition_math(int s) {
             std::vector<zd> O((size_t)m * m);
             for (int j = 0; j < m; ++j)
                 for (int i = 0; i < m; ++i) O[i + (size_t)j * m] = M[(iP + i) + (size_t)(iE + j) * kd];
-            std::vector<int> p = track_bands(m, O, sl.prev_om, om);
+            std::vector<int> p = track_bands(m, O, sl.prev_om, om, &track_u, &track_smin, &track_full);
+            track_m = m;
             bool swaps = false;
             for (int i = 0; i < (int)p.size(); ++i) swaps |= p[i] != i;
             if (swaps) {
@@ -694,7 +700,18 @@ void Engine<T>::transition_math(int s) {
             std::vector<zd> O((size_t)m * m), U;
             for (int j = 0; j < m; ++j)
                 for (int i = 0; i < m; ++i) O[i + (size_t)j * m] = M[(iP + i) + (size_t)(iE + perm[j]) * kd];
-            double smin = polar(m, O, U);
+            // O here is the tracking overlap with its columns permuted by perm
+            // (O P); when that overlap had full rank its polar factor is
+            // unique, and the factor of O P is U P: reuse it
+            double smin;
+            if (track_full && track_m == m) {
+                U.resize((size_t)m * m);
+                for (int j = 0; j < m; ++j)
+                    for (int i = 0; i < m; ++i) U[i + (size_t)j * m] = track_u[i + (size_t)perm[j] * m];
+                smin = track_smin;
+            } else {
+                smin = polar(m, O, U);
+            }
             const auto &kn = J.kpath[knext];
             double dkp[2] = {sl.cur_k[0] - sl.prev_k[0], sl.cur_k[1] - sl.prev_k[1]};
             double dkn[2] = {kn[0] - sl.cur_k[0], kn[1] - sl.cur_k[1]};

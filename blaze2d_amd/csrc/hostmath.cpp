@@ -190,7 +190,8 @@ double spectral_smin(int nx, int ny, const double b1[2], const double b2[2], con
     return std::isfinite(smin) ? smin : 1.0;
 }
 
-void jacobi_eigh(int n, const std::vector<zd> &a, std::vector<double> &w, std::vector<zd> &V) {
+void eigh(int n, const std::vector<zd> &a, std::vector<double> &w, std::vector<zd> &V) {
+    // A from the lower triangle, col-major
     std::vector<zd> A((size_t)n * n);
     for (int j = 0; j < n; ++j)
         for (int i = 0; i < n; ++i) {
@@ -198,55 +199,125 @@ void jacobi_eigh(int n, const std::vector<zd> &a, std::vector<double> &w, std::v
             else if (i < j) A[i + (size_t)j * n] = std::conj(a[j + (size_t)i * n]);
             else A[i + (size_t)i * n] = a[i + (size_t)i * n].real();
         }
-    V.assign((size_t)n * n, 0.0);
-    for (int i = 0; i < n; ++i) V[i + (size_t)i * n] = 1.0;
     auto at = [&](int i, int j) -> zd & { return A[i + (size_t)j * n]; };
-    for (int sweep = 0; sweep < 60; ++sweep) {
-        int rot = 0;
-        for (int p = 0; p < n - 1; ++p)
-            for (int q = p + 1; q < n; ++q) {
-                zd h = at(p, q);
-                double ha = std::abs(h);
-                if (ha == 0.0) continue;
-                double app = at(p, p).real(), aqq = at(q, q).real();
-                if (ha <= 1e-17 * std::sqrt(std::fabs(app) * std::fabs(aqq))) { at(p, q) = at(q, p) = 0.0; continue; }
-                ++rot;
-                double tau = (aqq - app) / (2 * ha);
-                double t = (tau >= 0 ? -1.0 : 1.0) / (std::fabs(tau) + std::sqrt(1 + tau * tau));
-                double c = 1 / std::sqrt(1 + t * t), s = t * c;
-                zd sg = s * std::conj(h / ha);
-                for (int k = 0; k < n; ++k) {
-                    zd kp = at(k, p), kq = at(k, q);
-                    at(k, p) = c * kp + sg * kq;
-                    at(k, q) = -std::conj(sg) * kp + c * kq;
-                    zd vp = V[k + (size_t)p * n], vq = V[k + (size_t)q * n];
-                    V[k + (size_t)p * n] = c * vp + sg * vq;
-                    V[k + (size_t)q * n] = -std::conj(sg) * vp + c * vq;
-                }
-                for (int k = 0; k < n; ++k) {
-                    zd pk = at(p, k), qk = at(q, k);
-                    at(p, k) = c * pk + std::conj(sg) * qk;
-                    at(q, k) = -sg * pk + c * qk;
-                }
-                at(p, q) = at(q, p) = 0.0;
-                at(p, p) = at(p, p).real();
-                at(q, q) = at(q, q).real();
-            }
-        if (!rot) break;
+    // 1. Householder reduction to tridiagonal form, A = Q T Q^H.  Step k maps
+    // x = A[k+1:, k] to alpha e_1 with the Hermitian reflector H = I - 2 u u^H,
+    // u = (x - alpha e_1) / |x - alpha e_1|, alpha = -|x| x_0 / |x_0|; the
+    // trailing block B becomes H B H = B - u q^H - q u^H with p = 2 B u,
+    // q = p - (u^H p) u, and Q collects Q H_k.
+    std::vector<zd> Q((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) Q[i + (size_t)i * n] = 1.0;
+    std::vector<zd> u(n), p(n);
+    for (int k = 0; k + 2 < n; ++k) {
+        const int r0 = k + 1;
+        double tail = 0.0;
+        for (int i = r0 + 1; i < n; ++i) tail += std::norm(at(i, k));
+        if (tail == 0.0) continue;  // already reduced; the off-diagonal stays complex
+        const zd x0 = at(r0, k);
+        const double ax0 = std::abs(x0), xn = std::sqrt(ax0 * ax0 + tail);
+        const zd ph = ax0 > 0.0 ? x0 / ax0 : zd(1.0);
+        const zd alpha = -xn * ph;
+        // |x - alpha e_1|^2 = 2 |x| (|x| + |x_0|)
+        const double un = 1.0 / std::sqrt(2.0 * xn * (xn + ax0));
+        u[r0] = (x0 - alpha) * un;
+        for (int i = r0 + 1; i < n; ++i) u[i] = at(i, k) * un;
+        for (int i = r0; i < n; ++i) {
+            zd s = 0.0;
+            for (int j = r0; j < n; ++j) s += at(i, j) * u[j];
+            p[i] = 2.0 * s;
+        }
+        double c = 0.0;  // u^H p, real
+        for (int i = r0; i < n; ++i) c += (std::conj(u[i]) * p[i]).real();
+        for (int i = r0; i < n; ++i) p[i] -= c * u[i];
+        for (int j = r0; j < n; ++j) {
+            const zd uj = std::conj(u[j]), pj = std::conj(p[j]);
+            for (int i = r0; i < n; ++i) at(i, j) -= u[i] * pj + p[i] * uj;
+        }
+        at(r0, k) = alpha;
+        at(k, r0) = std::conj(alpha);
+        for (int i = r0 + 1; i < n; ++i) at(i, k) = at(k, i) = 0.0;
+        for (int i = 0; i < n; ++i) {  // Q[:, r0:] -= 2 (Q[:, r0:] u) u^H
+            zd s = 0.0;
+            for (int j = r0; j < n; ++j) s += Q[i + (size_t)j * n] * u[j];
+            s *= 2.0;
+            for (int j = r0; j < n; ++j) Q[i + (size_t)j * n] -= s * std::conj(u[j]);
+        }
     }
+    // 2. T = D T' D^H with T' real symmetric: ph_0 = 1, ph_{k+1} = ph_k e_k / |e_k|
+    std::vector<double> d(n), e(n, 0.0);
+    std::vector<zd> phase(n, 1.0);
+    for (int k = 0; k < n; ++k) d[k] = at(k, k).real();
+    for (int k = 0; k + 1 < n; ++k) {
+        const zd ek = at(k + 1, k);
+        const double ae = std::abs(ek);
+        e[k] = ae;
+        phase[k + 1] = ae > 0.0 ? phase[k] * (ek / ae) : phase[k];
+    }
+    // 3. implicit-shift QL on T' (Wilkinson shift from the leading 2 x 2),
+    // rotations accumulated in Z
+    std::vector<double> Z((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) Z[i + (size_t)i * n] = 1.0;
+    const double eps = std::numeric_limits<double>::epsilon();
+    for (int l = 0; l < n; ++l) {
+        for (int iter = 0; iter < 60; ++iter) {
+            int m = l;
+            for (; m + 1 < n; ++m)
+                if (std::fabs(e[m]) <= eps * (std::fabs(d[m]) + std::fabs(d[m + 1]))) break;
+            if (m == l) break;
+            double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+            double r = std::hypot(g, 1.0);
+            g = d[m] - d[l] + e[l] / (g + (g >= 0.0 ? r : -r));
+            double s = 1.0, c = 1.0, shift = 0.0;
+            bool underflow = false;
+            for (int i = m - 1; i >= l; --i) {
+                const double f = s * e[i], b = c * e[i];
+                r = std::hypot(f, g);
+                e[i + 1] = r;
+                if (r == 0.0) {  // split: restart on the smaller problem
+                    d[i + 1] -= shift;
+                    e[m] = 0.0;
+                    underflow = true;
+                    break;
+                }
+                s = f / r;
+                c = g / r;
+                g = d[i + 1] - shift;
+                r = (d[i] - g) * s + 2.0 * c * b;
+                shift = s * r;
+                d[i + 1] = g + shift;
+                g = c * r - b;
+                double *zi = Z.data() + (size_t)i * n, *zi1 = zi + n;
+                for (int k = 0; k < n; ++k) {
+                    const double zk1 = zi1[k], zk = zi[k];
+                    zi1[k] = s * zk + c * zk1;
+                    zi[k] = c * zk - s * zk1;
+                }
+            }
+            if (underflow) continue;
+            d[l] -= shift;
+            e[l] = g;
+            e[m] = 0.0;
+        }
+    }
+    // 4. ascending order; V = Q diag(phase) Z
     std::vector<int> idx(n);
     for (int i = 0; i < n; ++i) idx[i] = i;
-    std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return at(x, x).real() < at(y, y).real(); });
+    std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return d[x] < d[y]; });
     w.resize(n);
-    std::vector<zd> Vs((size_t)n * n);
+    V.assign((size_t)n * n, 0.0);
     for (int j = 0; j < n; ++j) {
-        w[j] = at(idx[j], idx[j]).real();
-        for (int i = 0; i < n; ++i) Vs[i + (size_t)j * n] = V[i + (size_t)idx[j] * n];
+        w[j] = d[idx[j]];
+        const double *zj = Z.data() + (size_t)idx[j] * n;
+        for (int k = 0; k < n; ++k) {
+            const zd f = phase[k] * zj[k];
+            if (f == 0.0) continue;
+            const zd *qk = Q.data() + (size_t)k * n;
+            for (int i = 0; i < n; ++i) V[i + (size_t)j * n] += qk[i] * f;
+        }
     }
-    V.swap(Vs);
 }
 
-double polar(int m, const std::vector<zd> &O, std::vector<zd> &U) {
+double polar(int m, const std::vector<zd> &O, std::vector<zd> &U, bool *full_rank) {
     std::vector<zd> OhO((size_t)m * m);
     for (int j = 0; j < m; ++j)
         for (int i = 0; i < m; ++i) {
@@ -256,7 +327,7 @@ double polar(int m, const std::vector<zd> &O, std::vector<zd> &U) {
         }
     std::vector<double> ev;
     std::vector<zd> V;
-    jacobi_eigh(m, OhO, ev, V);
+    eigh(m, OhO, ev, V);
     std::vector<double> sig(m);
     double smin = std::numeric_limits<double>::infinity(), smax = 0;
     for (int i = 0; i < m; ++i) {
@@ -264,6 +335,7 @@ double polar(int m, const std::vector<zd> &O, std::vector<zd> &U) {
         smin = std::min(smin, sig[i]);
         smax = std::max(smax, sig[i]);
     }
+    if (full_rank) *full_rank = true;
     std::vector<zd> W((size_t)m * m, 0.0);
     for (int j = m - 1; j >= 0; --j) {
         std::vector<zd> col(m, 0.0);
@@ -274,6 +346,7 @@ double polar(int m, const std::vector<zd> &O, std::vector<zd> &U) {
                 col[i] = s / sig[j];
             }
         } else {
+            if (full_rank) *full_rank = false;
             for (int e = 0; e < m; ++e) {
                 std::fill(col.begin(), col.end(), zd(0.0));
                 col[e] = 1.0;
@@ -346,9 +419,14 @@ std::vector<int> hungarian(const std::vector<std::vector<double>> &w) {
 }
 
 std::vector<int> track_bands(int m, const std::vector<zd> &O, const std::vector<double> &pw,
-                             const std::vector<double> &cw) {
+                             const std::vector<double> &cw, std::vector<zd> *polar_u, double *polar_smin,
+                             bool *polar_full) {
     std::vector<zd> U;
-    polar(m, O, U);
+    bool full = false;
+    const double smin = polar(m, O, U, &full);
+    if (polar_u) *polar_u = U;
+    if (polar_smin) *polar_smin = smin;
+    if (polar_full) *polar_full = full;
     auto mag = [&](int i, int j) { return std::abs(U[i + (size_t)j * m]); };
     // detect_degenerate_bands_from_rotation
     std::vector<std::pair<int, int>> rb;

@@ -45,18 +45,25 @@ void reciprocal(const GeometrySpec &g, double b1[2], double b2[2]);
 // fourier_diagonal.rs:146-189: s_min over clamped |k+G|^2 > 1e-6
 double spectral_smin(int nx, int ny, const double b1[2], const double b2[2], const double bloch[2]);
 
-// Hermitian eigendecomposition (cyclic Jacobi), ascending, col-major vectors.
-void jacobi_eigh(int n, const std::vector<zd> &a, std::vector<double> &w, std::vector<zd> &v);
+// Hermitian eigendecomposition from the lower triangle of a (Householder
+// tridiagonalisation, implicit-shift QL), ascending, col-major vectors.
+void eigh(int n, const std::vector<zd> &a, std::vector<double> &w, std::vector<zd> &v);
 
 // Polar factor U of an m x m col-major matrix (subspace_prediction.rs:491-531).
-double polar(int m, const std::vector<zd> &o, std::vector<zd> &u);
+// Returns the smallest singular value; full_rank (optional) is set when every
+// singular value passed the 1e-13 relative cut, i.e. U = O (O^H O)^-1/2 is the
+// unique polar factor.
+double polar(int m, const std::vector<zd> &o, std::vector<zd> &u, bool *full_rank = nullptr);
 
 // band_tracking.rs hungarian_assignment
 std::vector<int> hungarian(const std::vector<std::vector<double>> &w);
 
 // band_tracking.rs track_bands_with_frequencies given the overlap O = prev^H B curr.
+// polar_u / polar_smin / polar_full (optional) receive the polar() results of
+// the overlap.
 std::vector<int> track_bands(int m, const std::vector<zd> &overlap, const std::vector<double> &prev_om,
-                             const std::vector<double> &curr_om);
+                             const std::vector<double> &curr_om, std::vector<zd> *polar_u = nullptr,
+                             double *polar_smin = nullptr, bool *polar_full = nullptr);
 
 // Coefficient-space vector algebra: vectors are columns of coefficients over a
 // basis K whose (weighted) Gram matrix M = K^H B K is known (col-major kd x kd).
