@@ -1047,24 +1047,27 @@ namespace bh {
 // ---------------------------------------------------------------------------
 // Dual Gram for Rayleigh-Ritz: one pass over the rows of Z and AZ gives
 // G1 = Z^H BZ and G2 = Z^H AZ (lower triangles), B Z formed on the fly.  The n x n outputs are
-// tiled by 3 x 3 blocks on an NBK x NBK block grid; each thread owns one lower
-// block of both products (18 complex accumulators), so only the blocks that
-// intersect the n columns do work.  RG row groups split the staged rows and are
-// reduced through LDS at the end; grid = (row chunks, work items).
+// tiled by 3 x 3 blocks; each thread owns one lower block of both products (18
+// complex accumulators).  Only the nlb = nbk (nbk + 1) / 2 blocks that
+// intersect the n columns (nbk = ceil(n / 3)) are assigned, and the 256
+// threads form 256 / nlb row groups that split the staged rows, so a narrow
+// problem keeps as many threads busy as a wide one.  The row groups are summed
+// by a tree reduction through LDS at the end; grid = (row chunks, work items).
 // ---------------------------------------------------------------------------
 template <int NBK>
 struct Gram2Shape {
-    static constexpr int NLB = NBK * (NBK + 1) / 2;  // lower 3x3 blocks
-    static constexpr int RG = 256 / NLB;             // row groups per workgroup
+    static constexpr int NLB = NBK * (NBK + 1) / 2;  // lower 3x3 blocks at the widest
     static constexpr int COLS = 3 * NBK;
-    static constexpr int SR = 32;                    // rows staged per pass
+    static constexpr int SR = 32;                    // rows staged per pass at the widest
+    static constexpr int MAX_SR = SR * COLS / 3;     // rows staged per pass at the narrowest
 };
 
 template <int NBK>
 size_t gram2_shared_bytes() {
     using S = Gram2Shape<NBK>;
-    size_t stage = 2ull * S::SR * S::COLS * sizeof(double2) + S::SR * sizeof(double);
-    size_t red = (size_t)S::NLB * 36 * sizeof(double);
+    size_t stage = 2ull * S::SR * S::COLS * sizeof(double2) + S::MAX_SR * sizeof(double);
+    // a reduction step parks at most half the row groups: (256 / nlb) / 2 * nlb <= 128 blocks
+    size_t red = (size_t)128 * 36 * sizeof(double);
     return stage > red ? stage : red;
 }
 
@@ -1081,21 +1084,27 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) g
     extern __shared__ __attribute__((aligned(16))) double2 g2s[];
     double2 *Zs = g2s;                       // [SR][COLS]
     double2 *As = Zs + S::SR * S::COLS;
-    double *Ws = reinterpret_cast<double *>(As + S::SR * S::COLS);  // [SR]
+    double *Ws = reinterpret_cast<double *>(As + S::SR * S::COLS);  // [MAX_SR]
     const bool weighted = w.bweight != nullptr;
     const int tid = threadIdx.x;
-    const int g = tid / S::NLB, lb = tid - g * S::NLB;
+    const int nbk = (n + 2) / 3;                 // block rows that meet the n columns
+    const int nlb = nbk * (nbk + 1) / 2;         // their lower blocks
+    const int rg = 256 / nlb;                    // row groups
+    const int ncs = 3 * nbk;                     // staged columns (zero beyond n)
+    const int sr = S::SR * S::COLS / ncs;        // rows staged per pass: narrow problems stage more
+    const int dc = 256 / sr, dr = 256 - dc * sr;
+    const int g = tid / nlb, lb = tid - g * nlb;
     int bi = 0;
     while ((bi + 1) * (bi + 2) / 2 <= lb) ++bi;
     const int bj = lb - bi * (bi + 1) / 2;
-    const bool active = g < S::RG && 3 * bi < n;
+    const bool active = g < rg;
     double a1r[3][3] = {}, a1i[3][3] = {}, a2r[3][3] = {}, a2i[3][3] = {};
     const long r0 = (long)blockIdx.x * rows_per_chunk;
     const long r1 = r0 + rows_per_chunk < nrows ? r0 + rows_per_chunk : nrows;
-    for (long rb = r0; rb < r1; rb += S::SR) {
-        const int nr = (int)(r1 - rb < S::SR ? r1 - rb : S::SR);
-        for (int e = tid; e < S::SR * S::COLS; e += 256) {
-            const int c = e / S::SR, r = e - c * S::SR;
+    for (long rb = r0; rb < r1; rb += sr) {
+        const int nr = (int)(r1 - rb < sr ? r1 - rb : sr);
+        // element e = c * sr + r walks by 256 = dc * sr + dr without dividing
+        for (int c = tid / sr, r = tid - (tid / sr) * sr; c < ncs;) {
             double2 z = make_double2(0.0, 0.0), a = z;
             if (r < nr && c < n) {
                 const long o = (long)c * w.stride + rb + r;
@@ -1103,22 +1112,28 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) g
                 z = make_double2((double)zv.x, (double)zv.y);
                 a = make_double2((double)av.x, (double)av.y);
             }
-            Zs[r * S::COLS + c] = z;
-            As[r * S::COLS + c] = a;
+            Zs[r * ncs + c] = z;
+            As[r * ncs + c] = a;
+            c += dc;
+            r += dr;
+            if (r >= sr) {
+                r -= sr;
+                ++c;
+            }
         }
-        if (tid < S::SR) Ws[tid] = tid < nr ? (weighted ? (double)w.bweight[rb + tid] : 1.0) : 0.0;
+        for (int r = tid; r < sr; r += 256) Ws[r] = r < nr ? (weighted ? (double)w.bweight[rb + r] : 1.0) : 0.0;
         __syncthreads();
         if (active) {
-            for (int r = g; r < nr; r += S::RG) {
+            for (int r = g; r < nr; r += rg) {
                 double2 zi[3], bjv[3], ajv[3];
                 const T s = (T)Ws[r];  // 1 when B = I
 #pragma unroll
                 for (int u = 0; u < 3; ++u) {
-                    zi[u] = Zs[r * S::COLS + 3 * bi + u];
-                    const double2 zj = Zs[r * S::COLS + 3 * bj + u];
+                    zi[u] = Zs[r * ncs + 3 * bi + u];
+                    const double2 zj = Zs[r * ncs + 3 * bj + u];
                     // B z in storage precision, as a stored mass image would hold it
                     bjv[u] = make_double2((double)((T)zj.x * s), (double)((T)zj.y * s));
-                    ajv[u] = As[r * S::COLS + 3 * bj + u];
+                    ajv[u] = As[r * ncs + 3 * bj + u];
                 }
 #pragma unroll
                 for (int u = 0; u < 3; ++u)
@@ -1131,12 +1146,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) g
         }
         __syncthreads();
     }
-    // reduce the row groups into group 0 through LDS, one group at a time
-    // (the staging area is free now)
+    // tree reduction of the row groups into group 0 through LDS (the staging
+    // area is free now): groups [h, cnt) park their sums, groups [0, cnt - h)
+    // add them, cnt = h
     double *red = reinterpret_cast<double *>(g2s);
-    for (int gg = 1; gg < S::RG; ++gg) {
-        if (active && g == gg) {
-            double *o = red + (size_t)lb * 36;
+    for (int cnt = rg; cnt > 1;) {
+        const int h = (cnt + 1) / 2;
+        if (active && g >= h && g < cnt) {
+            double *o = red + (size_t)((g - h) * nlb + lb) * 36;
 #pragma unroll
             for (int u = 0; u < 3; ++u)
 #pragma unroll
@@ -1148,8 +1165,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) g
                 }
         }
         __syncthreads();
-        if (active && g == 0) {
-            const double *o = red + (size_t)lb * 36;
+        if (active && g < cnt - h) {
+            const double *o = red + (size_t)(g * nlb + lb) * 36;
 #pragma unroll
             for (int u = 0; u < 3; ++u)
 #pragma unroll
@@ -1161,6 +1178,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) g
                 }
         }
         __syncthreads();
+        cnt = h;
     }
     if (!active || g != 0) return;
     double *p1 = w.partial1 + 2 * (size_t)blockIdx.x * w.ld * w.ld;
