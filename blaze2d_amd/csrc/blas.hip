@@ -324,9 +324,47 @@ template void launch_deflate<float>(const DeflateWork<float> *, int, long, int, 
 
 // ---------------------------------------------------------------------------
 // Linear combination: grid = (row blocks, nwork, output groups of QG); one row
-// per thread, QG outputs in registers.
+// per thread, the group's outputs in registers.  A workgroup accumulates only
+// as many outputs as its item has (QA <= QG, chosen per item): the Ritz update
+// of a narrow subspace writes 14 columns, and the other ten would be
+// multiply-adds of zeros.
 // ---------------------------------------------------------------------------
 constexpr int LC_NT = 256;
+
+template <typename T, int QG, int QA>
+__device__ __forceinline__ void lincomb_rows(const LincombWork<T> &w, const double2 *Cs, int na, int qn, int q0,
+                                             long row) {
+    double accr[QA], acci[QA];
+#pragma unroll
+    for (int j = 0; j < QA; ++j) { accr[j] = 0; acci[j] = 0; }
+    for (int i = 0; i < na; ++i) {
+        const cplx<T> *col = w.a_ptrs ? w.a_ptrs[i] : w.A + (long)i * w.a_stride;
+        cplx<T> a = col[row];
+        double xr = (double)a.x, xi = (double)a.y;
+#pragma unroll
+        for (int j = 0; j < QA; ++j) {
+            const double2 c = Cs[i * QG + j];
+            cmac(accr[j], acci[j], xr, xi, c.x, c.y);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < QA; ++j) {
+        if (j >= qn) continue;  // not break: keeps the loop unrolled and accr/acci in registers
+        const long oc = q0 + j;
+        double vr = accr[j], vi = acci[j];
+        if (w.mode == 1) {
+            cplx<T> b = w.base[oc * w.base_stride + row];
+            vr = (double)b.x - vr;
+            vi = (double)b.y - vi;
+        }
+        cplx<T> o = {(T)vr, (T)vi};
+        w.out[oc * w.out_stride + row] = o;
+        if (w.out2) {
+            T s = w.w2[row];
+            w.out2[oc * w.out_stride + row] = {o.x * s, o.y * s};
+        }
+    }
+}
 
 template <typename T, int LC_QG>
 __global__ void __launch_bounds__(LC_NT) lincomb_kernel(const LincombWork<T> *__restrict__ work, long nrows) {
@@ -349,36 +387,13 @@ __global__ void __launch_bounds__(LC_NT) lincomb_kernel(const LincombWork<T> *__
     __syncthreads();
     const long row = (long)blockIdx.x * LC_NT + threadIdx.x;
     if (row >= nrows) return;
-    double accr[LC_QG], acci[LC_QG];
-#pragma unroll
-    for (int j = 0; j < LC_QG; ++j) { accr[j] = 0; acci[j] = 0; }
-    for (int i = 0; i < na; ++i) {
-        const cplx<T> *col = w.a_ptrs ? w.a_ptrs[i] : w.A + (long)i * w.a_stride;
-        cplx<T> a = col[row];
-        double xr = (double)a.x, xi = (double)a.y;
-#pragma unroll
-        for (int j = 0; j < LC_QG; ++j) {
-            const double2 c = Cs[i * LC_QG + j];
-            cmac(accr[j], acci[j], xr, xi, c.x, c.y);
-        }
-    }
-#pragma unroll
-    for (int j = 0; j < LC_QG; ++j) {
-        if (j >= qn) continue;  // not break: keeps the loop unrolled and accr/acci in registers
-        const long oc = q0 + j;
-        double vr = accr[j], vi = acci[j];
-        if (w.mode == 1) {
-            cplx<T> b = w.base[oc * w.base_stride + row];
-            vr = (double)b.x - vr;
-            vi = (double)b.y - vi;
-        }
-        cplx<T> o = {(T)vr, (T)vi};
-        w.out[oc * w.out_stride + row] = o;
-        if (w.out2) {
-            T s = w.w2[row];
-            w.out2[oc * w.out_stride + row] = {o.x * s, o.y * s};
-        }
-    }
+    // qn is the same for the whole workgroup
+    if (qn <= 8) lincomb_rows<T, LC_QG, 8>(w, Cs, na, qn, q0, row);
+    else if (qn <= 12) lincomb_rows<T, LC_QG, 12>(w, Cs, na, qn, q0, row);
+    else if (LC_QG > 16 && qn <= 14) lincomb_rows<T, LC_QG, (LC_QG > 16 ? 14 : LC_QG)>(w, Cs, na, qn, q0, row);
+    else if (qn <= 16) lincomb_rows<T, LC_QG, 16>(w, Cs, na, qn, q0, row);
+    else if (LC_QG > 16 && qn <= 20) lincomb_rows<T, LC_QG, (LC_QG > 16 ? 20 : LC_QG)>(w, Cs, na, qn, q0, row);
+    else lincomb_rows<T, LC_QG, LC_QG>(w, Cs, na, qn, q0, row);
 }
 
 template <typename T>
