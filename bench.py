@@ -210,6 +210,21 @@ def main():
             "value": round(len(geoms) * 4096 / (kms * 1e-3), 1), "unit": "smoothed nodes/s",
             "kernel_ms": round(kms, 4), "geometries": len(geoms), "grid": "64x64",
             "host_one_thread_nodes_per_s": round(4096 / host_s, 1)}
+        # the reference's one-time per-job operator diagnostics (kappa estimates),
+        # batched over the whole sweep; with_diagnostics adds their wall time to
+        # one step of the headline, as the reference's session totals include them
+        from blaze2d_amd.bandstructure import operator_diagnostics
+        operator_diagnostics(be, jobs[:8])
+        t = time.perf_counter()
+        diags, dms = operator_diagnostics(be, jobs)
+        dwall = time.perf_counter() - t
+        step_s = elapsed / max(args.steps, 1)
+        out["operator_diagnostics_gpu"] = {
+            "value": round(len(jobs) / dwall, 1), "unit": "jobs/s", "jobs": len(jobs),
+            "device_ms": round(dms, 2), "wall_ms": round(1000 * dwall, 2),
+            "power_iterations": 15,
+            "headline_with_diagnostics": round(value * step_s / (step_s + dwall), 3),
+            "max_self_adjoint_err": max(d.self_adjoint_err for d in diags)}
         # plain FP64 path, one band structure at a time (the batching baseline)
         pj = jobs[:2]
         t = time.perf_counter()

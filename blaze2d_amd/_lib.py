@@ -87,12 +87,25 @@ class RunStats(ctypes.Structure):
     ]
 
 
+class Diagnostics(ctypes.Structure):
+    _fields_ = [
+        ("k_index", ctypes.c_int),
+        ("precond", ctypes.c_int),
+        ("lambda_max", ctypes.c_double),
+        ("lambda_min", ctypes.c_double),
+        ("kappa", ctypes.c_double),
+        ("pm_lambda_max", ctypes.c_double),
+        ("pm_kappa", ctypes.c_double),
+        ("self_adjoint_err", ctypes.c_double),
+    ]
+
+
 EXPORTS = [
     "bh_version", "bh_last_error", "bh_device_count", "bh_context_create", "bh_context_destroy", "bh_synchronize",
     "bh_buffer_alloc", "bh_buffer_free", "bh_buffer_upload", "bh_buffer_download", "bh_fft2d", "bh_scale",
     "bh_axpy", "bh_dot", "bh_gram", "bh_lincomb", "bh_theta_create", "bh_theta_destroy", "bh_theta_apply",
     "bh_mass_apply", "bh_precond_apply", "bh_dielectric", "bh_dielectric_batch", "bh_bandstructure_run",
-    "bh_hermitian_eigh",
+    "bh_hermitian_eigh", "bh_operator_diagnostics",
 ]
 
 _lib = None
@@ -137,6 +150,8 @@ def lib() -> ctypes.CDLL:
                                       ctypes.c_int, ctypes.c_int, c_dp, c_dp, c_dp, c_dp]
     L.bh_bandstructure_run.argtypes = [vp, ctypes.POINTER(Job), ctypes.c_int, ctypes.POINTER(RunOptions), c_dp, c_ip,
                                        c_ip, ctypes.POINTER(RunStats)]
+    L.bh_operator_diagnostics.argtypes = [vp, ctypes.POINTER(Job), ctypes.c_int, ctypes.c_int,
+                                          ctypes.POINTER(Diagnostics), c_dp]
     _lib = L
     return L
 

@@ -228,6 +228,55 @@ def dielectric_batch(backend: HipBackend, geoms: Sequence[Geometry2D], nx: int, 
     return eps, inv, ten, ms.value
 
 
+@dataclass
+class OperatorDiagnostics:
+    """One job's one-time operator diagnostics (bandstructure.rs:437-498)."""
+    k_index: int               # first non-Gamma k-point, -1 when the path has none
+    precond: int               # 0 none, 1 Fourier-diagonal, 2 transverse projection
+    lambda_max: float
+    lambda_min: float          # smallest clamped |k+G|^2
+    kappa: float               # kappa(A)
+    pm_lambda_max: float
+    pm_kappa: float            # kappa(M^-1 A)
+    self_adjoint_err: float
+
+    @property
+    def self_adjoint_status(self) -> str:
+        """The reference's label for the self-adjointness error (bandstructure.rs:471-477)."""
+        e = self.self_adjoint_err
+        return "exact" if e < 1e-12 else ("good" if e < 1e-6 else "WARN")
+
+    @property
+    def reduction(self) -> float:
+        """kappa(A) / kappa(M^-1 A), the factor the reference logs."""
+        return self.kappa / self.pm_kappa if self.pm_kappa else float("nan")
+
+
+def operator_diagnostics(backend: HipBackend, jobs: Sequence[BandStructureJob], options: Optional[RunOptions] = None,
+                         power_iterations: int = 15):
+    """The reference's one-time per-job operator diagnostics, batched on the GPU.
+
+    For each job, at its first non-Gamma k-point: the Theta operator's
+    self-adjointness error and the power-iteration estimates of kappa(A) and
+    kappa(M^-1 A) (maxwell.rs:382-582, run by run_core at
+    bandstructure.rs:437-498).  Returns ([OperatorDiagnostics], device ms).
+    """
+    opts = options or RunOptions()
+    jobs = list(jobs)
+    if not jobs:
+        return [], 0.0
+    arr = (L.Job * len(jobs))()
+    for i, j in enumerate(jobs):
+        arr[i] = j.to_c(opts)
+    out = (L.Diagnostics * len(jobs))()
+    ms = ctypes.c_double(0.0)
+    L.check(L.lib().bh_operator_diagnostics(backend.handle, arr, len(jobs), power_iterations, out, ctypes.byref(ms)),
+            "bh_operator_diagnostics")
+    res = [OperatorDiagnostics(d.k_index, d.precond, d.lambda_max, d.lambda_min, d.kappa, d.pm_lambda_max,
+                               d.pm_kappa, d.self_adjoint_err) for d in out]
+    return res, ms.value
+
+
 def dielectric(geom: Geometry2D, nx: int, ny: int, smoothing: bool = True):
     """Dielectric2D::from_geometry (dielectric.rs:146-210): (eps, inv_eps, tensor[N,4])."""
     g = geom.to_c()

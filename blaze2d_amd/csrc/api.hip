@@ -8,6 +8,7 @@
 
 #include "aux_kernels.h"
 #include "blas.h"
+#include "diagnostics.h"
 #include "dielectric.h"
 #include "engine.h"
 #include "fft_ops.h"
@@ -580,6 +581,243 @@ int bh_dielectric_batch(bh_context *ctx, const bh_geometry *geoms, int ngeom, in
         if (eps) BH_HIP_CHECK(hipMemcpy(eps, d_eps, n * sizeof(double), hipMemcpyDeviceToHost));
         if (inv_eps) BH_HIP_CHECK(hipMemcpy(inv_eps, d_inv, n * sizeof(double), hipMemcpyDeviceToHost));
         if (tensor) BH_HIP_CHECK(hipMemcpy(tensor, d_ten, 4 * n * sizeof(double), hipMemcpyDeviceToHost));
+    } catch (...) {
+        release();
+        throw;
+    }
+    release();
+    return 0;
+    BH_API_CATCH
+}
+
+DielGeom to_diel_geom(const bh_geometry &g) {
+    DielGeom d{};
+    for (int c = 0; c < 2; ++c) {
+        d.a1[c] = g.a1[c];
+        d.a2[c] = g.a2[c];
+    }
+    d.eps_bg = g.eps_bg;
+    d.n_atoms = g.n_atoms;
+    for (int a = 0; a < g.n_atoms; ++a)
+        for (int c = 0; c < 4; ++c) d.atoms[a][c] = g.atoms[a][c];
+    return d;
+}
+
+int bh_operator_diagnostics(bh_context *ctx, const bh_job *jobs, int njobs, int power_iterations,
+                            bh_diagnostics *out, double *gpu_ms) {
+    BH_API_TRY
+    if (!ctx || !jobs || njobs <= 0 || !out) return fail("bh_operator_diagnostics: bad arguments");
+    if (power_iterations < 1 || power_iterations > 10000)
+        return fail("bh_operator_diagnostics: power_iterations out of range");
+    const int nx = jobs[0].nx, ny = jobs[0].ny;
+    for (int j = 0; j < njobs; ++j) {
+        if (jobs[j].nx != nx || jobs[j].ny != ny) return fail("bh_operator_diagnostics: jobs must share the grid");
+        if (jobs[j].geom.n_atoms < 0 || jobs[j].geom.n_atoms > BH_MAX_ATOMS)
+            return fail("bh_operator_diagnostics: n_atoms out of range");
+        if (jobs[j].pol != 0 && jobs[j].pol != 1) return fail("bh_operator_diagnostics: pol must be 0 or 1");
+        if (jobs[j].nk < 0 || (jobs[j].nk > 0 && !jobs[j].kpath)) return fail("bh_operator_diagnostics: bad k-path");
+    }
+    if (!grid_supported(nx, ny)) return fail("bh_operator_diagnostics: grid size not instantiated");
+    BH_HIP_CHECK(hipSetDevice(ctx->device));
+    const long n = (long)nx * ny;
+    // jobs with a non-Gamma k-point, grouped by (polarisation, preconditioner,
+    // smoothing) so that each launch has one operator mode
+    struct Act {
+        int job, k, pol, prec, smooth;
+    };
+    std::vector<Act> act;
+    for (int j = 0; j < njobs; ++j) {
+        const bh_job &J = jobs[j];
+        out[j] = bh_diagnostics{};
+        out[j].k_index = -1;
+        for (int k = 0; k < J.nk; ++k) {
+            const double kx = J.kpath[2 * k], ky = J.kpath[2 * k + 1];
+            if (kx * kx + ky * ky >= 1e-8) {
+                out[j].k_index = k;
+                break;
+            }
+        }
+        if (out[j].k_index < 0) continue;
+        const int prec = J.precond < 0 ? (J.pol == 0 ? 1 : 2) : J.precond;
+        if (prec < 0 || prec > 2) return fail("bh_operator_diagnostics: unknown preconditioner");
+        out[j].precond = prec;
+        act.push_back({j, out[j].k_index, J.pol, prec, J.smoothing ? 1 : 0});
+    }
+    std::stable_sort(act.begin(), act.end(), [](const Act &a, const Act &b) {
+        if (a.pol != b.pol) return a.pol < b.pol;
+        if (a.prec != b.prec) return a.prec < b.prec;
+        return a.smooth < b.smooth;
+    });
+    // jobs per chunk: four fields, eps and tensor per job, about 1 GiB at most
+    const long per_job = n * (4 * 16 + 5 * 8);
+    const int chunk = (int)std::max<long>(1, std::min<long>(4096, (1L << 30) / per_job));
+    const cplx<double> *twx = twiddles<double>(ctx, nx), *twy = twiddles<double>(ctx, ny);
+    const hipStream_t st = ctx->st;
+    double *d_f = nullptr, *d_eps = nullptr, *d_ten = nullptr, *d_res = nullptr;
+    void *d_geo = nullptr, *d_meta = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto release = [&] {
+        for (void *p : {(void *)d_f, (void *)d_eps, (void *)d_ten, (void *)d_res, d_geo, d_meta})
+            if (p) (void)hipFree(p);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    };
+    try {
+        const int cap = (int)std::min<size_t>(act.size(), (size_t)chunk);
+        if (cap > 0) {
+            BH_HIP_CHECK(hipMalloc((void **)&d_f, (size_t)cap * 4 * n * 16));
+            BH_HIP_CHECK(hipMalloc((void **)&d_eps, (size_t)cap * n * 8));
+            BH_HIP_CHECK(hipMalloc((void **)&d_ten, (size_t)cap * n * 32));
+            BH_HIP_CHECK(hipMalloc((void **)&d_res, (size_t)cap * 3 * 8));
+            BH_HIP_CHECK(hipMalloc(&d_geo, dielectric_geom_bytes(cap)));
+            // params, three OpWork lists (self-adjoint; power step from column 0 or 1),
+            // the preconditioner lists and four DiagItem lists
+            BH_HIP_CHECK(hipMalloc(&d_meta, (size_t)cap * (sizeof(OpParams) + 5 * sizeof(OpWork<double>) +
+                                                           4 * sizeof(DiagItem))));
+        }
+        BH_HIP_CHECK(hipEventCreate(&e0));
+        BH_HIP_CHECK(hipEventCreate(&e1));
+        double total_ms = 0.0;
+        for (size_t c0 = 0; c0 < act.size();) {
+            // a chunk: same polarisation and preconditioner, at most cap jobs
+            size_t c1 = c0 + 1;
+            while (c1 < act.size() && c1 - c0 < (size_t)cap && act[c1].pol == act[c0].pol &&
+                   act[c1].prec == act[c0].prec)
+                ++c1;
+            const int m = (int)(c1 - c0), pol = act[c0].pol, prec = act[c0].prec;
+            // host parameters
+            std::vector<OpParams> hp(m);
+            std::vector<double> lam_min(m);
+            for (int i = 0; i < m; ++i) {
+                const bh_job &J = jobs[act[c0 + i].job];
+                const GeometrySpec g = to_geom(J.geom);
+                double b1[2], b2[2];
+                reciprocal(g, b1, b2);
+                const double kx = J.kpath[2 * act[c0 + i].k], ky = J.kpath[2 * act[c0 + i].k + 1];
+                OpParams &P = hp[i];
+                P = OpParams{};
+                for (int c = 0; c < 2; ++c) {
+                    P.b1[c] = b1[c];
+                    P.b2[c] = b2[c];
+                    P.bloch[c] = kx * b1[c] + ky * b2[c];
+                }
+                P.pol = pol;
+                P.gamma = (std::fabs(P.bloch[0]) < 1e-12 && std::fabs(P.bloch[1]) < 1e-12) ? 1 : 0;
+                P.precond = prec;
+                P.shift = 0.5 * spectral_smin(nx, ny, P.b1, P.b2, P.bloch);
+                lam_min[i] = kpg_sq_min(nx, ny, P.b1, P.b2, P.bloch);
+            }
+            char *meta = (char *)d_meta;
+            OpParams *d_par = (OpParams *)meta;
+            OpWork<double> *w_sa = (OpWork<double> *)(d_par + cap);
+            OpWork<double> *w_p0 = w_sa + cap, *w_p1 = w_p0 + cap;  // Theta from column 0 / 1
+            OpWork<double> *w_m0 = w_p1 + cap, *w_m1 = w_m0 + cap;  // preconditioner on column 1 / 0
+            DiagItem *i_sa = (DiagItem *)(w_m1 + cap);
+            DiagItem *i_n0 = i_sa + cap, *i_s0 = i_n0 + cap, *i_s1 = i_s0 + cap;
+            std::vector<OpWork<double>> hw(5 * (size_t)m);
+            std::vector<DiagItem> hi(4 * (size_t)m);
+            auto col = [&](int i, int c) { return (cplx<double> *)d_f + ((size_t)i * 4 + c) * n; };
+            for (int i = 0; i < m; ++i) {
+                OpWork<double> w{};
+                w.eps = d_eps + (size_t)i * n;
+                w.tensor = d_ten + (size_t)i * n * 4;
+                w.params = d_par + i;
+                w.in_stride = n;
+                w.out_stride = n;
+                w.count = 1;
+                OpWork<double> sa = w;  // x, y in columns 0, 2 -> Ax, Ay in 1, 3
+                sa.in = col(i, 0);
+                sa.out = col(i, 1);
+                sa.in_stride = sa.out_stride = 2 * n;
+                sa.count = 2;
+                OpWork<double> p0 = w, p1 = w, m0 = w, m1 = w;
+                p0.in = col(i, 0);
+                p0.out = col(i, 1);
+                p1.in = col(i, 1);
+                p1.out = col(i, 0);
+                m0.in = m0.out = col(i, 1);
+                m1.in = m1.out = col(i, 0);
+                hw[i] = sa;
+                hw[m + i] = p0;
+                hw[2 * m + i] = p1;
+                hw[3 * m + i] = m0;
+                hw[4 * m + i] = m1;
+                double *r = d_res + 3 * i;  // lambda, pm lambda, self-adjoint error
+                hi[i] = DiagItem{col(i, 0), col(i, 1), col(i, 2), col(i, 3), r + 2};
+                hi[m + i] = DiagItem{col(i, 0), nullptr, nullptr, nullptr, nullptr};
+                hi[2 * m + i] = DiagItem{col(i, 0), col(i, 1), nullptr, nullptr, r};
+                hi[3 * m + i] = DiagItem{col(i, 1), col(i, 0), nullptr, nullptr, r};
+            }
+            BH_HIP_CHECK(hipMemcpyAsync(d_par, hp.data(), sizeof(OpParams) * m, hipMemcpyHostToDevice, st));
+            BH_HIP_CHECK(hipMemcpyAsync(w_sa, hw.data(), sizeof(OpWork<double>) * m, hipMemcpyHostToDevice, st));
+            BH_HIP_CHECK(hipMemcpyAsync(w_p0, hw.data() + m, sizeof(OpWork<double>) * m, hipMemcpyHostToDevice, st));
+            BH_HIP_CHECK(hipMemcpyAsync(w_p1, hw.data() + 2 * m, sizeof(OpWork<double>) * m, hipMemcpyHostToDevice, st));
+            BH_HIP_CHECK(hipMemcpyAsync(w_m0, hw.data() + 3 * m, sizeof(OpWork<double>) * m, hipMemcpyHostToDevice, st));
+            BH_HIP_CHECK(hipMemcpyAsync(w_m1, hw.data() + 4 * m, sizeof(OpWork<double>) * m, hipMemcpyHostToDevice, st));
+            for (int l = 0; l < 4; ++l)  // the four lists sit cap entries apart
+                BH_HIP_CHECK(hipMemcpyAsync(i_sa + (size_t)l * cap, hi.data() + (size_t)l * m, sizeof(DiagItem) * m,
+                                            hipMemcpyHostToDevice, st));
+            std::vector<DielGeom> hg(m);
+            for (int i = 0; i < m; ++i) hg[i] = to_diel_geom(jobs[act[c0 + i].job].geom);
+            BH_HIP_CHECK(hipEventRecord(e0, st));
+            // dielectric tables, one launch per run of equal smoothing
+            for (int a = 0; a < m;) {
+                int b = a + 1;
+                while (b < m && act[c0 + b].smooth == act[c0 + a].smooth) ++b;
+                launch_dielectric(hg.data() + a, b - a, nx, ny, act[c0 + a].smooth != 0, d_eps + (size_t)a * n,
+                                  nullptr, d_ten + (size_t)a * n * 4, d_geo, st);
+                a = b;
+            }
+            launch_diag_params(d_par, d_eps, m, n, st);
+            const int mode_a = pol == 0 ? MODE_THETA_TM : MODE_THETA_TE;
+            const int mode_m = prec == 1 ? MODE_PREC_FD : MODE_PREC_TP;
+            // self-adjointness: Ax, Ay
+            launch_diag_init(col(0, 0), col(0, 2), 4 * n, m, n, st);
+            launch_op<double>(mode_a, nx, ny, w_sa, m, 2, twx, twy, st);
+            launch_diag_reduce(i_sa, m, 1, n, st);
+            // power iterations on Theta, then on M^-1 Theta, each from the normalised x
+            std::vector<DiagItem> h2(2 * (size_t)m);  // lives until the stream is synchronised
+            for (int phase = 0; phase < (prec ? 2 : 1); ++phase) {
+                if (phase == 1) launch_diag_init(col(0, 0), col(0, 2), 4 * n, m, n, st);
+                launch_diag_reduce(i_n0, m, 2, n, st);
+                for (int t = 0; t < power_iterations; ++t) {
+                    const bool odd = t & 1;
+                    launch_op<double>(mode_a, nx, ny, odd ? w_p1 : w_p0, m, 1, twx, twy, st);
+                    if (phase == 1) launch_op<double>(mode_m, nx, ny, odd ? w_m1 : w_m0, m, 1, twx, twy, st);
+                    // lambda into slot 0 (Theta) or 1 (M^-1 Theta) of each job's results
+                    launch_diag_reduce(odd ? i_s1 : i_s0, m, 0, n, st);
+                }
+                if (phase == 0 && prec) {
+                    // keep Theta's lambda: the preconditioned pass writes slot 1
+                    for (int i = 0; i < m; ++i) {
+                        h2[i] = hi[2 * m + i];
+                        h2[m + i] = hi[3 * m + i];
+                        h2[i].out = h2[m + i].out = d_res + 3 * i + 1;
+                    }
+                    BH_HIP_CHECK(hipMemcpyAsync(i_s0, h2.data(), sizeof(DiagItem) * m, hipMemcpyHostToDevice, st));
+                    BH_HIP_CHECK(
+                        hipMemcpyAsync(i_s1, h2.data() + m, sizeof(DiagItem) * m, hipMemcpyHostToDevice, st));
+                }
+            }
+            BH_HIP_CHECK(hipEventRecord(e1, st));
+            std::vector<double> res(3 * (size_t)m);
+            BH_HIP_CHECK(hipMemcpyAsync(res.data(), d_res, sizeof(double) * 3 * m, hipMemcpyDeviceToHost, st));
+            BH_HIP_CHECK(hipStreamSynchronize(st));
+            float ms = 0.0f;
+            BH_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            total_ms += ms;
+            for (int i = 0; i < m; ++i) {
+                bh_diagnostics &o = out[act[c0 + i].job];
+                o.lambda_max = res[3 * i];
+                o.lambda_min = lam_min[i];
+                o.kappa = lam_min[i] > 1e-15 ? o.lambda_max / lam_min[i] : INFINITY;
+                o.pm_lambda_max = prec ? res[3 * i + 1] : 0.0;
+                o.pm_kappa = o.pm_lambda_max;
+                o.self_adjoint_err = res[3 * i + 2];
+            }
+            c0 = c1;
+        }
+        if (gpu_ms) *gpu_ms = total_ms;
     } catch (...) {
         release();
         throw;

@@ -190,6 +190,32 @@ double spectral_smin(int nx, int ny, const double b1[2], const double b2[2], con
     return std::isfinite(smin) ? smin : 1.0;
 }
 
+double kpg_sq_min(int nx, int ny, const double b1[2], const double b2[2], const double bloch[2]) {
+    const double floor_sq = 1e-9;
+    double m = std::numeric_limits<double>::infinity();
+    for (int iy = 0; iy < ny; ++iy)
+        for (int ix = 0; ix < nx; ++ix) {
+            const double n1 = ix <= nx / 2 ? ix : ix - nx, n2 = iy <= ny / 2 ? iy : iy - ny;
+            double kx = n1 * b1[0] + n2 * b2[0] + bloch[0], ky = n1 * b1[1] + n2 * b2[1] + bloch[1];
+            if (!std::isfinite(kx) || !std::isfinite(ky)) {
+                kx = std::sqrt(floor_sq);
+                ky = 0.0;
+            } else {
+                const double sq = kx * kx + ky * ky;
+                if (sq == 0.0) {
+                    kx = std::sqrt(floor_sq);
+                    ky = 0.0;
+                } else if (sq < floor_sq) {
+                    const double sc = std::sqrt(floor_sq / sq);
+                    kx *= sc;
+                    ky *= sc;
+                }
+            }
+            m = std::min(m, kx * kx + ky * ky);
+        }
+    return std::isfinite(m) ? m : 0.0;
+}
+
 void eigh(int n, const std::vector<zd> &a, std::vector<double> &w, std::vector<zd> &V) {
     // A from the lower triangle, col-major
     std::vector<zd> A((size_t)n * n);

@@ -45,6 +45,10 @@ void reciprocal(const GeometrySpec &g, double b1[2], double b2[2]);
 // fourier_diagonal.rs:146-189: s_min over clamped |k+G|^2 > 1e-6
 double spectral_smin(int nx, int ny, const double b1[2], const double b2[2], const double bloch[2]);
 
+// maxwell.rs:1609-1690: the smallest |k+G|^2 after clamping each k+G to
+// length >= sqrt(1e-9) (ThetaOperator::k_plus_g_sq_min)
+double kpg_sq_min(int nx, int ny, const double b1[2], const double b2[2], const double bloch[2]);
+
 // Hermitian eigendecomposition from the lower triangle of a (Householder
 // tridiagonalisation, implicit-shift QL), ascending, col-major vectors.
 void eigh(int n, const std::vector<zd> &a, std::vector<double> &w, std::vector<zd> &v);

@@ -173,6 +173,30 @@ typedef struct {
     double host_plan_seconds;    /* host time spent planning and enqueueing steps */
 } bh_run_stats;
 
+/* One-time operator diagnostics of a job (drivers/bandstructure.rs:437-498):
+ * at the job's first non-Gamma k-point (|k_frac|^2 >= 1e-8), the Theta
+ * operator's self-adjointness error and power-iteration condition estimates
+ * of Theta and of the preconditioned operator (maxwell.rs:382-582). */
+typedef struct {
+    int k_index;             /* k-point used; -1 when the path has only Gamma (then all zero) */
+    int precond;             /* preconditioner of the pm estimate: 0 none, 1 FD, 2 TP */
+    double lambda_max;       /* Rayleigh quotient of the last power iteration on Theta */
+    double lambda_min;       /* smallest clamped |k+G|^2 (k_plus_g_sq_min) */
+    double kappa;            /* lambda_max / lambda_min, +inf when lambda_min <= 1e-15 */
+    double pm_lambda_max;    /* the same on M^-1 Theta; 0 when precond = 0 */
+    double pm_kappa;         /* pm_lambda_max / 1 (the reference's lambda_min estimate) */
+    double self_adjoint_err; /* |<Ax, y> - <x, Ay>| / (|Ax| |y|) */
+} bh_diagnostics;
+
+/* The diagnostics of njobs jobs (sharing nx, ny) on the GPU: dielectric
+ * tables, start vectors and power_iterations steps (the reference uses 15)
+ * batched over the jobs.  gpu_ms (may be NULL) receives the event-timed device
+ * time.  Replaces ThetaOperator::{check_self_adjointness,
+ * estimate_condition_number, estimate_preconditioned_condition_number}
+ * (maxwell.rs:382-582) as run once per job by run_core. */
+int bh_operator_diagnostics(bh_context *ctx, const bh_job *jobs, int njobs, int power_iterations,
+                            bh_diagnostics *out, double *gpu_ms);
+
 /* Runs every job to completion.  All jobs must share nx, ny.  Outputs per
  * job j at offsets: omegas[(sum of nk*n_bands over jobs < j) ...] as
  * omega (not omega / 2 pi), iters / converged per k-point. */

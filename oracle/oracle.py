@@ -201,6 +201,87 @@ def operator_apply(job: Job, kfrac, which: str, x: np.ndarray) -> np.ndarray:
     return out
 
 
+def kpg_sq(job: Job, kfrac) -> np.ndarray:
+    """Clamped |k+G|^2 on the grid, row-major iy * nx + ix
+    (maxwell.rs:1573-1690: FFT index order, |k+G| clamped to >= sqrt(1e-9))."""
+    a1, a2 = job.lattice_vectors()
+    det = a1[0] * a2[1] - a1[1] * a2[0]
+    b1 = (2 * np.pi * a2[1] / det, -2 * np.pi * a2[0] / det)
+    b2 = (-2 * np.pi * a1[1] / det, 2 * np.pi * a1[0] / det)
+    n1 = np.array([i if i <= job.nx // 2 else i - job.nx for i in range(job.nx)], dtype=float)
+    n2 = np.array([i if i <= job.ny // 2 else i - job.ny for i in range(job.ny)], dtype=float)
+    N2, N1 = np.meshgrid(n2, n1, indexing="ij")
+    bx = kfrac[0] * b1[0] + kfrac[1] * b2[0]
+    by = kfrac[0] * b1[1] + kfrac[1] * b2[1]
+    kx = N1 * b1[0] + N2 * b2[0] + bx
+    ky = N1 * b1[1] + N2 * b2[1] + by
+    sq = kx * kx + ky * ky
+    small = sq < 1e-9
+    sc = np.sqrt(1e-9 / np.where(sq > 0, sq, 1.0))
+    kx = np.where(small, np.where(sq > 0, kx * sc, np.sqrt(1e-9)), kx)
+    ky = np.where(small, np.where(sq > 0, ky * sc, 0.0), ky)
+    return (kx * kx + ky * ky).reshape(-1)
+
+
+def diagnostic_start_vectors(n: int):
+    """The x and y start vectors of maxwell.rs:386-391 and :529-538."""
+    i = np.arange(n, dtype=np.float64)
+    x = np.sin(i * 0.618033988749895) + 1j * np.cos(i * 0.414213562373095)
+    y = np.cos(i * np.sqrt(2.0)) + 1j * np.sin(i * np.sqrt(3.0))
+    return x, y
+
+
+def operator_diagnostics(job: Job, power_iterations: int = 15):
+    """run_core's one-time diagnostics (bandstructure.rs:437-498) over the
+    oracle's operators: ThetaOperator::check_self_adjointness,
+    estimate_condition_number and estimate_preconditioned_condition_number
+    (maxwell.rs:382-582) at the first non-Gamma k-point (|k|^2 >= 1e-8,
+    bandstructure.rs:72-81).  Returns a dict, or None when the path is all Gamma.
+    Pinned by the closed form of the TM power iteration (Theta_TM is diagonal
+    in Fourier space) in tests/test_oracle.py."""
+    kp = np.atleast_2d(job.kpath if job.kpath is not None else np.zeros((1, 2)))
+    idx = next((i for i, k in enumerate(kp) if k[0] * k[0] + k[1] * k[1] >= 1e-8), None)
+    if idx is None:
+        return None
+    k = kp[idx]
+    n = job.nx * job.ny
+    x, y = diagnostic_start_vectors(n)
+
+    def A(v):
+        return operator_apply(job, k, "A", v[None])[0]
+
+    def P(v):
+        return operator_apply(job, k, "P", v[None])[0]
+
+    ax, ay = A(x), A(y)
+    scale = np.linalg.norm(ax) * np.linalg.norm(y)
+    sa = abs(np.vdot(ax, y) - np.vdot(x, ay)) / scale if scale > 1e-15 else 0.0
+
+    def power(pre):
+        v = x / np.linalg.norm(x)
+        lam = 0.0
+        for _ in range(power_iterations):
+            av = A(v)
+            if pre:
+                av = P(av)
+            lam = float(np.real(np.vdot(v, av)))
+            nr = np.linalg.norm(av)
+            if nr > 1e-15:
+                av = av / nr
+            v = av
+        return lam
+
+    prec = job.precond if job.precond >= 0 else (1 if job.pol.upper() == "TM" else 2)
+    lam_max = power(False)
+    lam_min = float(kpg_sq(job, k).min())
+    pm = power(True) if prec else 0.0
+    return {
+        "k_index": idx, "precond": prec, "lambda_max": lam_max, "lambda_min": lam_min,
+        "kappa": lam_max / lam_min if lam_min > 1e-15 else float("inf"),
+        "pm_lambda_max": pm, "pm_kappa": pm, "self_adjoint_err": float(sa),
+    }
+
+
 def eigh(a: np.ndarray):
     n = a.shape[0]
     ac = np.asfortranarray(a, dtype=np.complex128)

@@ -249,3 +249,47 @@ def test_series6_golden_frequencies(pol):
     rel = np.abs(f - ref)[m] / ref[m]
     assert rel.max() < 1e-9, rel.max()
     assert np.abs(f[~m]).max() < 1e-9
+
+
+# --- one-time operator diagnostics (bandstructure.rs:437-498) ---------------
+def _tm_power_closed_form(job, kfrac, iters=15):
+    """Theta_TM = F^-1 diag(|k+G|^2) F, so the power iteration from x has the
+    Rayleigh quotient sum w d^(2t-1) / sum w d^(2t-2) at step t, w = |F x|^2."""
+    x, _ = O.diagnostic_start_vectors(job.nx * job.ny)
+    w = np.abs(np.fft.fft2(x.reshape(job.ny, job.nx))).reshape(-1) ** 2
+    d = O.kpg_sq(job, kfrac)
+    r = d / d.max()
+    return d.max() * np.sum(w * r ** (2 * iters - 1)) / np.sum(w * r ** (2 * iters - 2))
+
+
+@pytest.mark.parametrize("lattice", ["square", "hex"])
+def test_diagnostics_tm_power_iteration_closed_form(lattice):
+    kp = np.array([[0.0, 0.0], [0.1, 0.05], [0.2, 0.1]])
+    job = O.Job(lattice=lattice, eps_bg=13.0, atoms=[(0.5, 0.5, 0.3, 1.0)], nx=24, ny=24, pol="TM", kpath=kp)
+    d = O.operator_diagnostics(job)
+    assert d["k_index"] == 1  # Gamma is skipped
+    want = _tm_power_closed_form(job, kp[1])
+    assert abs(d["lambda_max"] - want) < 1e-10 * want
+    assert d["lambda_min"] == O.kpg_sq(job, kp[1]).min()
+    assert d["kappa"] == d["lambda_max"] / d["lambda_min"]
+    assert d["self_adjoint_err"] < 1e-13  # Theta is Hermitian: "exact"
+
+
+def test_diagnostics_kpg_clamp_and_gamma_only():
+    job = O.Job(lattice="square", nx=12, ny=12, pol="TE", kpath=np.zeros((3, 2)))
+    assert O.operator_diagnostics(job) is None  # no non-Gamma k-point
+    # at Gamma the G = 0 entry is clamped to 1e-9, every other |G|^2 >= (2 pi)^2
+    d = O.kpg_sq(job, [0.0, 0.0])
+    assert np.isclose(d.min(), 1e-9) and np.sort(d)[1] >= (2 * np.pi) ** 2 * (1 - 1e-12)
+
+
+def test_diagnostics_te_preconditioning_reduces_kappa():
+    kp = np.array([[0.25, 0.0]])
+    job = O.Job(lattice="hex", eps_bg=12.0, atoms=[(0.5, 0.5, 0.35, 1.0)], nx=24, ny=24, pol="TE", kpath=kp)
+    d = O.operator_diagnostics(job)
+    assert d["k_index"] == 0 and d["precond"] == 2
+    assert d["self_adjoint_err"] < 1e-13
+    # the transverse projection -i(k+G)/(|k+G|^2+s) . eps . -i(k+G)/(|k+G|^2+s)
+    # is negative definite, so the dominant eigenvalue of M^-1 A (what the
+    # power iteration returns) is negative; its size is what preconditioning cuts
+    assert d["pm_kappa"] < 0 and abs(d["pm_kappa"]) < d["kappa"] / 100
