@@ -41,11 +41,11 @@ METRIC = "converged k-point solves/sec, res64 12 bands, ≤1e-6 rel freq err vs 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 # HBM bytes per rr_kernel launch on the default workload (2048 geometries,
-# slots 1024, groups 4), FETCH_SIZE x2 + WRITE_SIZE x1 from two rocprofv3 --pmc
+# slots 2048, groups 4), FETCH_SIZE x2 + WRITE_SIZE x1 from two rocprofv3 --pmc
 # passes scaled by a plain copy (profiles/rr_kernel_traffic.md); reported only
 # for that configuration.
-RR_TRAFFIC_BYTES = 15.48e6
-RR_TRAFFIC_CONFIG = (1024, 4, 2048)  # slots, groups, geometries the counters were read on
+RR_TRAFFIC_BYTES = 18.12e6
+RR_TRAFFIC_CONFIG = (2048, 4, 2048)  # slots, groups, geometries the counters were read on
 TIMED_NAMES = {1: "op_kernel THETA_TM/TE (64x64 fused 2D FFT operator)",
                2: "gram2_kernel (dual Rayleigh-Ritz Gram Z^H BZ, Z^H AZ)",
                3: "rr_kernel (Cholesky/SVQB + projection + Jacobi Rayleigh-Ritz, FP64)",
@@ -73,7 +73,7 @@ def main():
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--geoms", type=int, default=2048, help="geometries in the sweep (x2 polarisations)")
-    ap.add_argument("--slots", type=int, default=1024, help="concurrent band structures (0 = all)")
+    ap.add_argument("--slots", type=int, default=2048, help="concurrent band structures (0 = all)")
     ap.add_argument("--groups", type=int, default=4, help="slot groups pipelined on separate streams")
     ap.add_argument("--precision", type=int, default=0, choices=(0, 1, 2),
                     help="0 FP64; 1 FP32 fields; 2 FP64 fields with FP32 preconditioners")
