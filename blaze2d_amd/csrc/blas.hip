@@ -177,10 +177,16 @@ void launch_reduce(const ReduceWork *dwork, int nwork, int ld, hipStream_t st) {
 // of one item get the same b % 8, so the item's vectors are fetched into one
 // L2.
 // ---------------------------------------------------------------------------
-constexpr int DEFL_NT = 512;
-constexpr int DEFL_RPT = 8;  // rows per thread: N <= 4096
-constexpr int DEFL_CPW = 1;  // columns per workgroup
-constexpr int DEFL_DB = 4;   // vectors per batch
+// Shape: 1024 threads x 4 rows, two columns per workgroup, vectors two at a
+// time (102 VGPRs, 16 waves per CU).  On the res-64 sweep at 2048 slots it
+// ran the whole sweep 2% faster than 512 x 8 with one column and batches of
+// four (206 VGPRs, 8 waves per CU): the second column halves the vectors'
+// L2 reads.  512 x 8 with batches of two made the deflation launches shortest
+// but the sweep 3% slower.
+constexpr int DEFL_NT = 1024;
+constexpr int DEFL_RPT = 4;  // rows per thread: N <= 4096
+constexpr int DEFL_CPW = 2;  // columns per workgroup
+constexpr int DEFL_DB = 2;   // vectors per batch
 constexpr int NXCD = 8;
 
 template <typename T>
