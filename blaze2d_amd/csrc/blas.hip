@@ -209,7 +209,7 @@ __global__ void __launch_bounds__(DEFL_NT) deflate_kernel(const DeflateWork<T> *
 #pragma unroll
     for (int k = 0; k < DEFL_RPT; ++k) {
         const long i = tid + (long)k * DEFL_NT;
-        ws[k] = i < nrows ? (w.weight ? (double)w.weight[i] : 1.0) : 0.0;  // 0: row past the end
+        ws[k] = i < nrows ? (w.weight ? (double)gmem(w.weight)[i] : 1.0) : 0.0;  // 0: row past the end
     }
 #pragma unroll
     for (int c = 0; c < DEFL_CPW; ++c) {
@@ -217,7 +217,7 @@ __global__ void __launch_bounds__(DEFL_NT) deflate_kernel(const DeflateWork<T> *
 #pragma unroll
         for (int k = 0; k < DEFL_RPT; ++k) {
             const long i = tid + (long)k * DEFL_NT;
-            const cplx<T> v = (c < nc && i < nrows) ? r[i] : cplx<T>{(T)0, (T)0};
+            const cplx<T> v = (c < nc && i < nrows) ? gload(r, i) : cplx<T>{(T)0, (T)0};
             ur[c][k] = (double)v.x;
             ui[c][k] = (double)v.y;
         }
@@ -236,7 +236,7 @@ __global__ void __launch_bounds__(DEFL_NT) deflate_kernel(const DeflateWork<T> *
 #pragma unroll
             for (int k = 0; k < DEFL_RPT; ++k) {
                 const long i = tid + (long)k * DEFL_NT;
-                const cplx<T> x = dv[i < nrows ? i : nrows - 1];
+                const cplx<T> x = gload(dv, i < nrows ? i : nrows - 1);
                 const double xr = ws[k] * (double)x.x, xi = ws[k] * (double)x.y;
 #pragma unroll
                 for (int c = 0; c < DEFL_CPW; ++c) {
@@ -273,7 +273,7 @@ __global__ void __launch_bounds__(DEFL_NT) deflate_kernel(const DeflateWork<T> *
             cr += red[v][d][c][0];
             ci += red[v][d][c][1];
         }
-        if (c >= nc || (w.mask && !w.mask[d + (size_t)(c0 + c) * w.ld])) cr = ci = 0.0;
+        if (c >= nc || (w.mask && !gmem(w.mask)[d + (size_t)(c0 + c) * w.ld])) cr = ci = 0.0;
         coef[d][c][0] = cr;
         coef[d][c][1] = ci;
     }
@@ -294,7 +294,7 @@ __global__ void __launch_bounds__(DEFL_NT) deflate_kernel(const DeflateWork<T> *
 #pragma unroll
             for (int k = 0; k < DEFL_RPT; ++k) {
                 const long i = tid + (long)k * DEFL_NT;
-                const cplx<T> x = dv[i < nrows ? i : nrows - 1];
+                const cplx<T> x = gload(dv, i < nrows ? i : nrows - 1);
 #pragma unroll
                 for (int c = 0; c < DEFL_CPW; ++c) {
                     cmac(ur[c][k], ui[c][k], -(double)x.x, -(double)x.y, cr[c], ci[c]);
@@ -309,7 +309,7 @@ __global__ void __launch_bounds__(DEFL_NT) deflate_kernel(const DeflateWork<T> *
 #pragma unroll
         for (int k = 0; k < DEFL_RPT; ++k) {
             const long i = tid + (long)k * DEFL_NT;
-            if (i < nrows) r[i] = cplx<T>{(T)ur[c][k], (T)ui[c][k]};
+            if (i < nrows) gstore(r, i, cplx<T>{(T)ur[c][k], (T)ui[c][k]});
         }
     }
 }
@@ -345,7 +345,7 @@ __device__ __forceinline__ void lincomb_rows(const LincombWork<T> &w, const doub
     for (int j = 0; j < QA; ++j) { accr[j] = 0; acci[j] = 0; }
     for (int i = 0; i < na; ++i) {
         const cplx<T> *col = w.a_ptrs ? w.a_ptrs[i] : w.A + (long)i * w.a_stride;
-        cplx<T> a = col[row];
+        cplx<T> a = gload(col, row);
         double xr = (double)a.x, xi = (double)a.y;
 #pragma unroll
         for (int j = 0; j < QA; ++j) {
@@ -359,15 +359,15 @@ __device__ __forceinline__ void lincomb_rows(const LincombWork<T> &w, const doub
         const long oc = q0 + j;
         double vr = accr[j], vi = acci[j];
         if (w.mode == 1) {
-            cplx<T> b = w.base[oc * w.base_stride + row];
+            cplx<T> b = gload(w.base, oc * w.base_stride + row);
             vr = (double)b.x - vr;
             vi = (double)b.y - vi;
         }
         cplx<T> o = {(T)vr, (T)vi};
-        w.out[oc * w.out_stride + row] = o;
+        gstore(w.out, oc * w.out_stride + row, o);
         if (w.out2) {
-            T s = w.w2[row];
-            w.out2[oc * w.out_stride + row] = {o.x * s, o.y * s};
+            T s = gmem(w.w2)[row];
+            gstore(w.out2, oc * w.out_stride + row, cplx<T>{o.x * s, o.y * s});
         }
     }
 }
@@ -385,8 +385,8 @@ __global__ void __launch_bounds__(LC_NT) lincomb_kernel(const LincombWork<T> *__
         int i = e % na, j = e / na;
         double cr = 0, ci = 0;
         if (j < qn) {
-            cr = w.C[2 * (i + (size_t)(q0 + j) * w.ldc)];
-            ci = w.C[2 * (i + (size_t)(q0 + j) * w.ldc) + 1];
+            cr = gmem(w.C)[2 * (i + (size_t)(q0 + j) * w.ldc)];
+            ci = gmem(w.C)[2 * (i + (size_t)(q0 + j) * w.ldc) + 1];
         }
         Cs[i * LC_QG + j] = make_double2(cr, ci);
     }
@@ -1129,7 +1129,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) g
             double2 z = make_double2(0.0, 0.0), a = z;
             if (r < nr && c < n) {
                 const long o = (long)c * w.stride + rb + r;
-                cplx<T> zv = w.Z[o], av = w.AZ[o];
+                cplx<T> zv = gload(w.Z, o), av = gload(w.AZ, o);
                 z = make_double2((double)zv.x, (double)zv.y);
                 a = make_double2((double)av.x, (double)av.y);
             }
@@ -1142,7 +1142,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) g
                 ++c;
             }
         }
-        for (int r = tid; r < sr; r += 256) Ws[r] = r < nr ? (weighted ? (double)w.bweight[rb + r] : 1.0) : 0.0;
+        for (int r = tid; r < sr; r += 256) Ws[r] = r < nr ? (weighted ? (double)gmem(w.bweight)[rb + r] : 1.0) : 0.0;
         __syncthreads();
         if (active) {
             for (int r = g; r < nr; r += rg) {
@@ -1211,10 +1211,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) g
             const int i = 3 * bi + u, j = 3 * bj + v;
             if (i < n && j < n) {
                 const size_t o = 2 * ((size_t)i * w.ld + j);
-                p1[o] = a1r[u][v];
-                p1[o + 1] = a1i[u][v];
-                p2[o] = a2r[u][v];
-                p2[o + 1] = a2i[u][v];
+                gmem(p1)[o] = a1r[u][v];
+                gmem(p1)[o + 1] = a1i[u][v];
+                gmem(p2)[o] = a2r[u][v];
+                gmem(p2)[o + 1] = a2i[u][v];
             }
         }
 }

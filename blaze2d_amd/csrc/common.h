@@ -16,6 +16,27 @@ struct cplx {
     T x, y;
 };
 
+// The global-memory view of a device pointer taken from a work descriptor.
+// Through it, loads and stores become global_* instructions; through the
+// generic pointer they are flat_* ones, which also count against the LDS wait
+// counter, so every wait for an LDS read would wait for them too.
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T *gmem(T *p) {
+    return (__attribute__((address_space(1))) T *)p;
+}
+// element i of a complex array through the global view (both parts in one access)
+template <typename T>
+__device__ __forceinline__ cplx<T> gload(const cplx<T> *p, long i) {
+    const __attribute__((address_space(1))) cplx<T> *g = gmem(p) + i;
+    return cplx<T>{g->x, g->y};
+}
+template <typename T>
+__device__ __forceinline__ void gstore(cplx<T> *p, long i, cplx<T> v) {
+    __attribute__((address_space(1))) cplx<T> *g = gmem(p) + i;
+    g->x = v.x;
+    g->y = v.y;
+}
+
 template <typename T>
 __host__ __device__ inline cplx<T> cmul(cplx<T> a, cplx<T> b) {
     return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};

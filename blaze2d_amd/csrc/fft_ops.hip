@@ -194,7 +194,7 @@ __device__ inline void fft_pass(cplx<T> *buf, const cplx<T> *__restrict__ tw, co
 #pragma unroll
         for (int j = 0; j < R; ++j) {
             if constexpr (GIN) {  // HBM in storage precision TS
-                const cplx<TS> g = static_cast<const cplx<TS> *>(io.gin)[e0 + j * M * ES];
+                const cplx<TS> g = gload(static_cast<const cplx<TS> *>(io.gin), e0 + j * M * ES);
                 v[j] = {(T)g.x, (T)g.y};
             }
             else v[j] = buf[lds_pos<RW>(e0 + j * M * ES)];
@@ -217,10 +217,10 @@ __device__ inline void fft_pass(cplx<T> *buf, const cplx<T> *__restrict__ tw, co
             if constexpr (GOUT) {
                 const int e = e0 + j * M * ES;
                 const cplx<TS> o = {(TS)v[j].x, (TS)v[j].y};
-                static_cast<cplx<TS> *>(io.gout)[e] = o;
+                gstore(static_cast<cplx<TS> *>(io.gout), e, o);
                 if (io.gbout) {
-                    const TS w = static_cast<const TS *>(io.geps)[e];
-                    static_cast<cplx<TS> *>(io.gbout)[e] = {o.x * w, o.y * w};
+                    const TS w = gmem(static_cast<const TS *>(io.geps))[e];
+                    gstore(static_cast<cplx<TS> *>(io.gbout), e, cplx<TS>{o.x * w, o.y * w});
                 }
             } else {
                 buf[lds_pos<RW>(e0 + j * M * ES)] = v[j];
@@ -389,12 +389,12 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) op
         for (int i = threadIdx.x; i < N; i += NT) {
             cplx<T> gx = F[lds_pos<NX>(i)], gy = G2[lds_pos<NX>(i)];
             if (MODE == MODE_THETA_TE) {
-                const TS *t = w.tensor + 4 * (size_t)i;
+                const auto *t = gmem(w.tensor + 4 * (size_t)i);
                 T t0 = (T)t[0], t1 = (T)t[1], t2 = (T)t[2], t3 = (T)t[3];
                 F[lds_pos<NX>(i)] = {gx.x * t0 + gy.x * t1, gx.y * t0 + gy.y * t1};
                 G2[lds_pos<NX>(i)] = {gx.x * t2 + gy.x * t3, gx.y * t2 + gy.y * t3};
             } else {
-                T e = (T)w.eps[i];
+                T e = (T)gmem(w.eps)[i];
                 F[lds_pos<NX>(i)] = {gx.x * e, gx.y * e};
                 G2[lds_pos<NX>(i)] = {gy.x * e, gy.y * e};
             }
