@@ -52,14 +52,14 @@ __global__ void __launch_bounds__(256) gram_kernel(const GramWork<T> *__restrict
         for (int e = threadIdx.x; e < GRAM_SR * na; e += 256) {
             int c = e / GRAM_SR, r = e - c * GRAM_SR;
             const cplx<T> *col = w.a_ptrs ? w.a_ptrs[c] : w.A + (long)c * w.a_stride;
-            As[r][c] = r < nr ? col[rb + r] : cplx<T>{(T)0, (T)0};
+            As[r][c] = r < nr ? gload(col, rb + r) : cplx<T>{(T)0, (T)0};
         }
         for (int e = threadIdx.x; e < GRAM_SR * nb; e += 256) {
             int c = e / GRAM_SR, r = e - c * GRAM_SR;
             const cplx<T> *col = w.b_ptrs ? w.b_ptrs[c] : w.B + (long)c * w.b_stride;
-            cplx<T> v = r < nr ? col[rb + r] : cplx<T>{(T)0, (T)0};
+            cplx<T> v = r < nr ? gload(col, rb + r) : cplx<T>{(T)0, (T)0};
             if (w.weight && r < nr) {
-                T s = w.weight[rb + r];
+                T s = gmem(w.weight)[rb + r];
                 v.x *= s;
                 v.y *= s;
             }
@@ -96,8 +96,8 @@ __global__ void __launch_bounds__(256) gram_kernel(const GramWork<T> *__restrict
         for (int v = 0; v < BU; ++v) {
             int i = ty + 16 * u, j = tx + 16 * v;
             if (i < na && j < nb) {
-                out[2 * (i * w.ld + j)] = ar[u][v];
-                out[2 * (i * w.ld + j) + 1] = ai[u][v];
+                gmem(out)[2 * (i * w.ld + j)] = ar[u][v];
+                gmem(out)[2 * (i * w.ld + j) + 1] = ai[u][v];
             }
         }
 }
@@ -1320,7 +1320,7 @@ __global__ void __launch_bounds__(NT) rr_kernel(const RRWork *__restrict__ work)
                 const int a = i >= j ? i : j, b = i >= j ? j : i;
                 double re = 0.0, im = 0.0;
                 for (int c = 0; c < w.nchunks; ++c) {
-                    const double *p = partial + 2 * ((size_t)c * ld * ld + (size_t)a * ld + b);
+                    const auto *p = gmem(partial + 2 * ((size_t)c * ld * ld + (size_t)a * ld + b));
                     re += p[0];
                     im += p[1];
                 }
@@ -1530,8 +1530,8 @@ __global__ void __launch_bounds__(NT) rr_kernel(const RRWork *__restrict__ work)
     for (int e = tid; e < n * r; e += NT) {
         const int k = e % n, c = e / n;
         const double2 t = Tm[k + c * lda];
-        w.C[2 * ((size_t)k + (size_t)c * w.ldc)] = t.x;
-        w.C[2 * ((size_t)k + (size_t)c * w.ldc) + 1] = t.y;
+        gmem(w.C)[2 * ((size_t)k + (size_t)c * w.ldc)] = t.x;
+        gmem(w.C)[2 * ((size_t)k + (size_t)c * w.ldc) + 1] = t.y;
     }
     __syncthreads();
     // 4. M = Y diag(theta) Y^H
@@ -1546,7 +1546,8 @@ __global__ void __launch_bounds__(NT) rr_kernel(const RRWork *__restrict__ work)
     for (int i = tid; i < r; i += NT) wsh[i] = Af[i + i * lda].x;
     for (int e = tid; e < n * r; e += NT) {
         const int k = e % n, c = e / n;
-        Tl[k + c * lda] = make_double2(w.C[2 * ((size_t)k + (size_t)c * w.ldc)], w.C[2 * ((size_t)k + (size_t)c * w.ldc) + 1]);
+        Tl[k + c * lda] = make_double2(gmem(w.C)[2 * ((size_t)k + (size_t)c * w.ldc)],
+                                       gmem(w.C)[2 * ((size_t)k + (size_t)c * w.ldc) + 1]);
     }
     __syncthreads();
     for (int i = tid; i < r; i += NT) {
@@ -1570,8 +1571,8 @@ __global__ void __launch_bounds__(NT) rr_kernel(const RRWork *__restrict__ work)
             cmac(cr, ci, t.x, t.y, y.x, y.y);
         }
         const size_t o = 2 * ((size_t)k + (size_t)col * w.ldc);
-        w.C[o] = cr;
-        w.C[o + 1] = ci;
+        gmem(w.C)[o] = cr;
+        gmem(w.C)[o + 1] = ci;
     }
     if (tid == 0) {
         w.cnt[0] = cx;
