@@ -308,10 +308,12 @@ def _expand_legacy(cfg: BulkConfig) -> List[ExpandedJob]:
         spec = r.get(key)
         return None if spec is None else range_values(float(spec["min"]), float(spec["max"]), float(spec["step"]))
 
-    eps_vals = rng("eps_bg") or [_geom_eps_bg(cfg.geometry)]
-    res_vals = [int(round(v)) for v in rng("resolution")] if r.get("resolution") else [cfg.grid_nx]
-    pol_vals = [_pol(p, "TM") for p in r["polarization"]] if r.get("polarization") else [cfg.polarization]
-    lat_vals = [_lattice_type_name(t) for t in r["lattice_type"]] if r.get("lattice_type") else [None]
+    # an axis given but empty (a range with min > max, an empty list) yields no
+    # jobs, as in the reference; only an absent axis takes the base value
+    eps_vals = rng("eps_bg") if "eps_bg" in r else [_geom_eps_bg(cfg.geometry)]
+    res_vals = [int(round(v)) for v in rng("resolution")] if "resolution" in r else [cfg.grid_nx]
+    pol_vals = [_pol(p, "TM") for p in r["polarization"]] if "polarization" in r else [cfg.polarization]
+    lat_vals = [_lattice_type_name(t) for t in r["lattice_type"]] if "lattice_type" in r else [None]
     atom_ranges = r.get("atoms", [])
     per_atom = []
     for i, base in enumerate(_geom_atoms(cfg.geometry)):

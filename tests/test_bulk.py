@@ -62,6 +62,12 @@ def test_expand_multiple_ranges_legacy():
     assert [(j.params.eps_bg, j.params.polarization) for j in jobs[:3]] == [(10.0, "TM"), (10.0, "TE"), (11.0, "TM")]
 
 
+def test_expand_empty_legacy_axis_gives_no_jobs():
+    # an empty range (min > max) or list is an axis of length zero, not an absent axis
+    assert bulk.expand_jobs(_cfg("[ranges]\neps_bg = {min = 12.0, max = 10.0, step = 1.0}\n")) == []
+    assert bulk.expand_jobs(_cfg("[ranges]\npolarization = []\n")) == []
+
+
 def test_expand_ordered_sweeps():
     jobs = bulk.expand_jobs(_cfg('[[sweeps]]\nparameter = "atom0.radius"\nmin = 0.2\nmax = 0.3\nstep = 0.1\n'
                                  '[[sweeps]]\nparameter = "eps_bg"\nmin = 10.0\nmax = 12.0\nstep = 1.0\n'))
