@@ -21,7 +21,10 @@ Also reported on the same line:
                 host thread;
   accuracy      max relative frequency error of the GPU results against the
                 oracle on that sample;
-  plain_gpu     the same FP64 solver run one band structure at a time.
+  plain_gpu     the same FP64 solver run one band structure at a time;
+  single_solve_hex_te  one hex TE 64x64 band structure (12 bands, 61
+                k-points) through the FP64 path and the FP32-preconditioner
+                path, with the per-band frequency difference between them.
 """
 from __future__ import annotations
 
@@ -270,6 +273,33 @@ def main():
         dtp = time.perf_counter() - t
         out["plain_gpu"] = {"value": round(solves_of(pj) / dtp, 3), "unit": "k-point solves/s",
                             "sample": "first 2 band structures of the sweep, slots=1, FP64"}
+        # one band structure on its own: hex lattice, eps_bg 13 with an eps 1 hole
+        # of r = 0.3, TE, 64x64, 12 bands, Gamma-M-K-Gamma at 61 k-points, tol 1e-7;
+        # the FP64 path against the FP32-preconditioner path, per band
+        from blaze2d_amd.bandstructure import (BandStructureJob, BasisAtom, Geometry2D, Lattice2D,
+                                               k_path as make_k_path)
+        hj = BandStructureJob(Geometry2D(Lattice2D.hexagonal(), 13.0, [BasisAtom((0.5, 0.5), 0.3, 1.0)]), 64, 64,
+                              "TE", make_k_path("hex", 20), n_bands=12, tol=1e-7)
+        single = {}
+        for prec in (0, 2):
+            run_batch(be, [hj], ropts, slots=1, precision=prec)  # warm-up
+            t = time.perf_counter()
+            (hr,), hst = run_batch(be, [hj], ropts, slots=1, precision=prec)
+            single[prec] = (hr, time.perf_counter() - t, hst.iterations)
+        b64, b32 = np.sort(single[0][0].bands, axis=1), np.sort(single[2][0].bands, axis=1)
+        pos = b64 > 1e-8
+        per_band = [float(np.max(np.abs(b32[:, i] - b64[:, i])[pos[:, i]] / b64[:, i][pos[:, i]]))
+                    if pos[:, i].any() else 0.0 for i in range(b64.shape[1])]
+        out["single_solve_hex_te"] = {
+            "workload": "hex eps_bg 13, eps 1 hole r 0.3, TE, 64x64, 12 bands, Gamma-M-K-Gamma 61 k-points, tol 1e-7, "
+                        "one slot",
+            "exit_rule": out["config"]["exit_rule"],
+            "fp64": {"seconds": round(single[0][1], 4), "k_points_per_s": round(61 / single[0][1], 2),
+                     "lobpcg_iterations": single[0][2], "converged": int(single[0][0].converged.sum())},
+            "fp32_preconditioners": {"seconds": round(single[2][1], 4), "k_points_per_s": round(61 / single[2][1], 2),
+                                     "lobpcg_iterations": single[2][2],
+                                     "converged": int(single[2][0].converged.sum())},
+            "max_rel_freq_diff_per_band": [float(f"{e:.3e}") for e in per_band]}
         # the reference's own exit accounting on the same sweep (one pass)
         if args.exit_rule != 0:
             t = time.perf_counter()

@@ -174,15 +174,24 @@ def test_bandstructure_batch_mixed(be):
         _compare_bands(r, om, it, f"batch {s[0]} {s[3]}")
 
 
-def test_bandstructure_cfg2_hex_te_res64_short(be):
-    # hex eps_bg=13, hole r=0.3, TE, 64x64, 12 bands, tol 1e-7 (short 13-point path)
-    kp = B.k_path("hex", 4)
+def test_bandstructure_cfg2_hex_te_res64_61k(be):
+    # hex eps_bg=13, hole r=0.3, TE, 64x64, 12 bands, tol 1e-7, Gamma-M-K-Gamma at
+    # 61 k-points.  FP64 under the reference's solve() exit accounting follows
+    # the oracle's iterations exactly.  That accounting can end a solve before
+    # every band has converged (band 0 next to the closing Gamma here), so a
+    # path whose iterations differ, FP32 preconditioners (precision 2), is
+    # compared under the all-bands-converged exit, with the oracle run the
+    # same way.
+    kp = B.k_path("hex", 20)
+    assert kp.shape == (61, 2)
     job = B.BandStructureJob(B.Geometry2D(B.Lattice2D.hexagonal(), 13.0, [B.BasisAtom((0.5, 0.5), 0.3, 1.0)]),
                              64, 64, "TE", kp, n_bands=12, tol=1e-7)
-    res = B.run_with_options(be, job)
-    om, it, cv, tot = O.bandstructure(O.Job(lattice="hex", eps_bg=13.0, atoms=[(0.5, 0.5, 0.3, 1.0)], nx=64, ny=64,
-                                            pol="TE", n_bands=12, tol=1e-7, kpath=kp))
-    _compare_bands(res, om, it, "cfg2 short")
+    for precision, rule in ((0, 0), (2, 1)):
+        om, it, cv, tot = O.bandstructure(O.Job(lattice="hex", eps_bg=13.0, atoms=[(0.5, 0.5, 0.3, 1.0)], nx=64,
+                                                ny=64, pol="TE", n_bands=12, tol=1e-7, kpath=kp, exit_rule=rule))
+        assert cv.all()
+        (res,), _ = B.run_batch(be, [job], B.RunOptions(exit_rule=rule), slots=1, precision=precision)
+        _compare_bands(res, om, it, f"hex TE 61k precision {precision} exit rule {rule}")
 
 
 def test_res64_sweep_sample_all_bands_converged(be):
