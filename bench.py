@@ -99,7 +99,8 @@ def main():
     ap.add_argument("--precision", type=int, default=0, choices=(0, 1, 2),
                     help="0 FP64; 1 FP32 fields; 2 FP64 fields with FP32 preconditioners")
     ap.add_argument("--seed", type=int, default=0xB1A2E2D)
-    ap.add_argument("--cpu-sample", type=int, default=4, help="band structures timed on the CPU oracle")
+    ap.add_argument("--cpu-sample", type=int, default=16,
+                    help="band structures timed on the CPU oracle, one per thread (16 threads at most)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip roofline / plain-GPU extra passes")
     ap.add_argument("--timed-kernel", type=int, default=3, choices=(1, 2, 3, 4, 5),
