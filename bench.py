@@ -122,8 +122,8 @@ def main():
         print(json.dumps({"copy_bytes_read": src.numel() * 8, "copy_bytes_written": src.numel() * 8, "copies": 5}))
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
-              "(band structures shard independently across GPUs, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (band structures are "
+              "independent, so a multi-GPU run would give each rank its share of the sweep)", file=sys.stderr)
         sys.exit(2)
 
     from blaze2d_amd import HipBackend

@@ -1,6 +1,6 @@
 // FP64 CPU oracle for the Blaze2D band-solve path.
 //
-// TEST INFRASTRUCTURE ONLY: the checker the HIP path is compared against.
+// TEST INFRASTRUCTURE ONLY: the HIP path's results are compared against it.
 // Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
 // load this library.  The product library (blaze2d_amd/csrc) never links it.
 //

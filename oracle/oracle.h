@@ -1,8 +1,8 @@
 /*
  * FP64 CPU oracle for the Blaze2D band-solve path.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP
- * implementation is compared against.  Only tests/, the smoke() entry and
+ * TEST INFRASTRUCTURE ONLY.  The HIP implementation's results are
+ * compared against this library's.  Only tests/, the smoke() entry and
  * bench.py's cpu_baseline leg may load it; the product library never links
  * or calls it.
  *

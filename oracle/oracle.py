@@ -1,7 +1,7 @@
 """ctypes wrapper for the FP64 CPU oracle (liboracle.so).
 
-TEST INFRASTRUCTURE ONLY.  The oracle is the checker for the HIP band-solve
-path; only tests/, ``__graft_entry__.smoke()`` and ``bench.py``'s
+TEST INFRASTRUCTURE ONLY.  The HIP band-solve path's results are compared
+against the oracle; only tests/, ``__graft_entry__.smoke()`` and ``bench.py``'s
 cpu_baseline leg import this module.  The product package ``blaze2d_amd``
 never imports it.
 """
