@@ -84,6 +84,9 @@ class RunStats(ctypes.Structure):
         ("timed_flops", ctypes.c_double),
         ("host_wait_seconds", ctypes.c_double),
         ("host_plan_seconds", ctypes.c_double),
+        ("rr_problems", ctypes.c_long),
+        ("svqb_fallbacks", ctypes.c_long),
+        ("rank_drops", ctypes.c_long),
     ]
 
 
@@ -105,7 +108,7 @@ EXPORTS = [
     "bh_buffer_alloc", "bh_buffer_free", "bh_buffer_upload", "bh_buffer_download", "bh_fft2d", "bh_scale",
     "bh_axpy", "bh_dot", "bh_gram", "bh_lincomb", "bh_theta_create", "bh_theta_destroy", "bh_theta_apply",
     "bh_mass_apply", "bh_precond_apply", "bh_dielectric", "bh_dielectric_batch", "bh_bandstructure_run",
-    "bh_hermitian_eigh", "bh_operator_diagnostics",
+    "bh_hermitian_eigh", "bh_operator_diagnostics", "bh_rayleigh_ritz",
 ]
 
 _lib = None
@@ -138,6 +141,7 @@ def lib() -> ctypes.CDLL:
     L.bh_dot.argtypes = [vp, vp, ctypes.c_int, vp, ctypes.c_int, c_dp]
     L.bh_gram.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, vp, ctypes.c_int, ctypes.c_int, c_dp]
     L.bh_hermitian_eigh.argtypes = [vp, ctypes.c_int, ctypes.c_int, c_dp, c_dp, c_dp]
+    L.bh_rayleigh_ritz.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, c_dp, c_dp, c_dp, c_dp, c_ip]
     L.bh_lincomb.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, c_dp, ctypes.c_int, vp, ctypes.c_int]
     L.bh_theta_create.argtypes = [vp, ctypes.POINTER(ThetaDesc), ctypes.POINTER(vp)]
     L.bh_theta_destroy.argtypes = [vp]

@@ -84,6 +84,29 @@ class HipBackend:
                 "hermitian_eigh")
         return w, np.transpose(v.reshape(cnt, n, n), (0, 2, 1))
 
+    def rayleigh_ritz(self, g1: np.ndarray, g2: np.ndarray, nsplit: int):
+        """Batched Rayleigh-Ritz from the Grams G1 = Z^H B Z, G2 = Z^H A Z ([count,] n, n).
+
+        Returns (lam[count, n], coef[count, n, n], info[count, 4]) as
+        bh_rayleigh_ritz: ascending Ritz values (rank of them), coefficients of
+        the new X and W Ritz vectors over Z's columns, and {rank, SVQB path
+        taken, cnt_x, cnt_w}."""
+        a1 = np.asarray(g1, dtype=np.complex128)
+        a2 = np.asarray(g2, dtype=np.complex128)
+        if a1.ndim == 2:
+            a1, a2 = a1[None], a2[None]
+        cnt, n, _ = a1.shape
+        f1 = np.ascontiguousarray(np.transpose(a1, (0, 2, 1)).reshape(cnt, -1))
+        f2 = np.ascontiguousarray(np.transpose(a2, (0, 2, 1)).reshape(cnt, -1))
+        lam = np.zeros((cnt, n))
+        coef = np.zeros((cnt, n * n), dtype=np.complex128)
+        info = np.zeros((cnt, 4), dtype=np.int32)
+        L.check(L.lib().bh_rayleigh_ritz(self._ctx, n, cnt, nsplit, f1.view(np.float64).ctypes.data_as(L.c_dp),
+                                         f2.view(np.float64).ctypes.data_as(L.c_dp), lam.ctypes.data_as(L.c_dp),
+                                         coef.view(np.float64).ctypes.data_as(L.c_dp),
+                                         info.ctypes.data_as(L.c_ip)), "rayleigh_ritz")
+        return lam, np.transpose(coef.reshape(cnt, n, n), (0, 2, 1)), info
+
     def linear_combinations(self, q: "Field", coeffs: np.ndarray, out: "Field", qi0: int = 0, oi0: int = 0):
         c = np.asfortranarray(coeffs, dtype=np.complex128)
         r, m = c.shape

@@ -158,6 +158,9 @@ class BatchStats:
     eigh_launch_max_sum: tuple = (0, 0)
     host_wait_seconds: float = 0.0
     host_plan_seconds: float = 0.0
+    rr_problems: int = 0            # Rayleigh-Ritz problems solved
+    svqb_fallbacks: int = 0         # of them, B-orthonormalised on the SVQB path
+    rank_drops: int = 0             # of them, with a direction dropped
 
 
 def run_batch(backend: HipBackend, jobs: Sequence[BandStructureJob], options: Optional[RunOptions] = None,
@@ -198,7 +201,7 @@ def run_batch(backend: HipBackend, jobs: Sequence[BandStructureJob], options: Op
     stats = BatchStats(st.steps, st.iterations, st.solves, st.seconds, st.timed_ms, st.timed_launches,
                        st.timed_bytes, st.timed_flops, tuple(st.eigh_calls), tuple(st.eigh_sweeps),
                        tuple(st.eigh_max_sweeps), tuple(st.eigh_launch_max_sum), st.host_wait_seconds,
-                       st.host_plan_seconds)
+                       st.host_plan_seconds, st.rr_problems, st.svqb_fallbacks, st.rank_drops)
     return out, stats
 
 

@@ -60,7 +60,10 @@ def load_config(source: str) -> Dict[str, Any]:
 
 
 def range_values(vmin: float, vmax: float, step: float) -> List[float]:
-    """RangeSpec::values: accumulate from min while v <= max + step * 1e-9."""
+    """RangeSpec::values: accumulate from min while v <= max + step * 1e-9.
+    A step <= 0 never reaches max and is rejected."""
+    if not step > 0:
+        raise ValueError(f"range step must be positive (got {step})")
     out = []
     v = vmin
     while v <= vmax + step * 1e-9:
@@ -179,6 +182,52 @@ class BulkConfig:
         self.output_mode = str(out.get("mode", "full")).lower()
         self.output_dir = str(out.get("directory", "./bulk_output"))
         self.output_prefix = str(out.get("prefix", "job"))
+        self.validate()
+
+    def validate(self) -> None:
+        """BulkConfig::validate (config.rs:1215-1300) with SweepSpec::validate
+        (:338-376) and validate_parameter_path (:528-577); raises ValueError."""
+        if self.sweeps:
+            seen = set()
+            for sw in self.sweeps:
+                name = sw.get("parameter")
+                if name in seen:
+                    raise ValueError(f"duplicate sweep parameter '{name}'")
+                seen.add(name)
+                has_range = all(sw.get(k) is not None for k in ("min", "max", "step"))
+                has_values = sw.get("values") is not None
+                if not has_range and not has_values:
+                    raise ValueError(f"sweep '{name}' must specify either (min, max, step) or values")
+                if has_range and has_values:
+                    raise ValueError(f"sweep '{name}' cannot have both range and values")
+                if has_range:
+                    if float(sw["step"]) <= 0.0:
+                        raise ValueError(f"sweep '{name}' step must be positive")
+                    if float(sw["min"]) > float(sw["max"]):
+                        raise ValueError(f"sweep '{name}' min ({sw['min']}) > max ({sw['max']})")
+                if name not in ("eps_bg", "resolution", "polarization", "lattice_type") and \
+                        not _ATOM_PATH.match(str(name)):
+                    raise ValueError(f"invalid sweep parameter path '{name}'")
+            idx = [int(m.group(1)) for m in (_ATOM_PATH.match(str(sw.get("parameter"))) for sw in self.sweeps) if m]
+            if idx and max(idx) > 0 and not _geom_atoms(self.effective_geometry()):
+                raise ValueError(f"sweep references atom{max(idx)} but no atoms defined in geometry")
+        if self.solver_type != "maxwell":
+            return
+        if self.effective_geometry() is None:
+            raise ValueError("Maxwell solver requires geometry (use [geometry] or [defaults.geometry])")
+        if not self.sweeps:
+            e = self.ranges.get("eps_bg")
+            if e is not None:
+                if float(e["step"]) <= 0.0:
+                    raise ValueError("eps_bg step must be positive")
+                if float(e["min"]) > float(e["max"]):
+                    raise ValueError("eps_bg min > max")
+            r = self.ranges.get("resolution")
+            if r is not None:
+                if float(r["step"]) < 1.0:
+                    raise ValueError("resolution step must be >= 1")
+                if float(r["min"]) < 4.0:
+                    raise ValueError("resolution min must be >= 4")
 
     # effective_geometry / effective_polarization / effective_resolution (config.rs:1167-1183)
     def effective_geometry(self) -> Optional[Dict[str, Any]]:

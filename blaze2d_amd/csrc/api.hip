@@ -389,9 +389,80 @@ int bh_hermitian_eigh(bh_context *ctx, int n, int count, const double *a, double
     BH_API_CATCH
 }
 
+int bh_rayleigh_ritz(bh_context *ctx, int n, int count, int nsplit, const double *g1, const double *g2, double *lam,
+                     double *coef, int *info) {
+    BH_API_TRY
+    if (!ctx || n <= 0 || n > GRAM_MAXC || count < 0 || nsplit <= 0 || !g1 || !g2 || !lam || !coef || !info)
+        return fail("bh_rayleigh_ritz: bad arguments");
+    if (count == 0) return 0;
+    BH_HIP_CHECK(hipSetDevice(ctx->device));
+    const size_t nn = (size_t)n * n;
+    // device layout: [work items][G1 partials][G2 partials][lam][C][counts], each
+    // partial row-major G(i, j) with one chunk
+    const size_t wbytes = ((count * sizeof(RRWork) + 255) / 256) * 256;
+    const size_t bytes = wbytes + sizeof(double) * count * (2 * nn * 3 + n) + sizeof(int) * 8 * count;
+    char *base = (char *)scratch(ctx, bytes);
+    RRWork *dw = (RRWork *)base;
+    double *dp1 = (double *)(base + wbytes);
+    double *dp2 = dp1 + 2 * nn * count;
+    double *dc = dp2 + 2 * nn * count;
+    double *dl = dc + 2 * nn * count;
+    int *dcnt = (int *)(dl + (size_t)n * count);
+    std::vector<double> h1(2 * nn * count), h2(2 * nn * count);
+    for (int c = 0; c < count; ++c)
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) {
+                const size_t src = 2 * (c * nn + i + (size_t)j * n), dst = 2 * (c * nn + (size_t)i * n + j);
+                h1[dst] = g1[src];
+                h1[dst + 1] = g1[src + 1];
+                h2[dst] = g2[src];
+                h2[dst + 1] = g2[src + 1];
+            }
+    std::vector<RRWork> hw(count);
+    for (int c = 0; c < count; ++c) {
+        RRWork r{};
+        r.partial1 = dp1 + 2 * nn * c;
+        r.partial2 = dp2 + 2 * nn * c;
+        r.lam = dl + (size_t)n * c;
+        r.C = dc + 2 * nn * c;
+        r.cnt = dcnt + 8 * c;
+        r.n = n;
+        r.nchunks = 1;
+        r.ld = n;
+        r.ldc = n;
+        r.nsplit = nsplit;
+        hw[c] = r;
+    }
+    BH_HIP_CHECK(hipMemsetAsync(dl, 0, sizeof(double) * n * count, ctx->st));
+    BH_HIP_CHECK(hipMemcpyAsync(dw, hw.data(), sizeof(RRWork) * count, hipMemcpyHostToDevice, ctx->st));
+    BH_HIP_CHECK(hipMemcpyAsync(dp1, h1.data(), sizeof(double) * h1.size(), hipMemcpyHostToDevice, ctx->st));
+    BH_HIP_CHECK(hipMemcpyAsync(dp2, h2.data(), sizeof(double) * h2.size(), hipMemcpyHostToDevice, ctx->st));
+    launch_rr(dw, count, n, ctx->st);
+    std::vector<int> hc(8 * (size_t)count);
+    BH_HIP_CHECK(hipMemcpyAsync(lam, dl, sizeof(double) * n * count, hipMemcpyDeviceToHost, ctx->st));
+    BH_HIP_CHECK(hipMemcpyAsync(coef, dc, sizeof(double) * 2 * nn * count, hipMemcpyDeviceToHost, ctx->st));
+    BH_HIP_CHECK(hipMemcpyAsync(hc.data(), dcnt, sizeof(int) * hc.size(), hipMemcpyDeviceToHost, ctx->st));
+    BH_HIP_CHECK(hipStreamSynchronize(ctx->st));
+    for (int c = 0; c < count; ++c) {
+        const int *k = hc.data() + 8 * c;
+        info[4 * c] = k[5];
+        info[4 * c + 1] = k[4];
+        info[4 * c + 2] = k[0];
+        info[4 * c + 3] = k[1];
+        // columns past the Ritz coefficients held the B-orthonormalising
+        // transform while the kernel ran
+        for (size_t e = 2 * ((size_t)(k[0] + k[1]) * n); e < 2 * nn; ++e) coef[2 * c * nn + e] = 0.0;
+    }
+    return 0;
+    BH_API_CATCH
+}
+
 int bh_theta_create(bh_context *ctx, const bh_theta_desc *d, bh_theta **out) {
     BH_API_TRY
     if (!ctx || !d || !out || !d->eps) return fail("bh_theta_create: bad arguments");
+    if (d->pol != 0 && d->pol != 1) return fail("bh_theta_create: pol must be 0 (TM) or 1 (TE)");
+    if (d->precond < -1 || d->precond > 2) return fail("bh_theta_create: precond must be -1..2");
+    if (d->precision != 0 && d->precision != 1) return fail("bh_theta_create: precision must be 0 or 1");
     if (!grid_supported(d->nx, d->ny)) return fail("bh_theta_create: grid size not instantiated");
     auto *op = new bh_theta();
     op->ctx = ctx;
@@ -474,6 +545,9 @@ int bh_theta_apply(bh_theta *op, const bh_buffer *in, int ii0, bh_buffer *out, i
 int bh_mass_apply(bh_theta *op, const bh_buffer *in, int ii0, bh_buffer *out, int oi0, int count) {
     BH_API_TRY
     if (!op || !check_idx(in, ii0, count) || !check_idx(out, oi0, count)) return fail("bh_mass_apply: bad range");
+    if (in->nx != op->nx || in->ny != op->ny || out->nx != op->nx || out->ny != op->ny ||
+        in->precision != op->precision || out->precision != op->precision)
+        return fail("bh_mass_apply: buffer shape or precision mismatch");
     const long n = (long)in->n();
     // identity coefficients with a pointwise eps image (TM) via the lincomb kernel
     for (int c = 0; c < count; ++c) {
@@ -523,6 +597,7 @@ int bh_dielectric(const bh_geometry *geom, int nx, int ny, int smoothing, double
                   double *tensor) {
     BH_API_TRY
     if (!geom || nx <= 0 || ny <= 0) return fail("bh_dielectric: bad arguments");
+    if (geom->n_atoms < 0 || geom->n_atoms > BH_MAX_ATOMS) return fail("bh_dielectric: n_atoms out of range");
     DielectricTables d = build_dielectric(to_geom(*geom), nx, ny, smoothing != 0);
     size_t n = (size_t)nx * ny;
     if (eps) std::memcpy(eps, d.eps.data(), n * 8);
@@ -839,6 +914,12 @@ int bh_bandstructure_run(bh_context *ctx, const bh_job *jobs, int njobs, const b
         if (b.nx != jobs[0].nx || b.ny != jobs[0].ny) return fail("bh_bandstructure_run: jobs must share the grid");
         if (b.nk <= 0 || !b.kpath) return fail("bh_bandstructure_run: empty k-path");
         if (b.n_bands <= 0) return fail("bh_bandstructure_run: n_bands must be positive");
+        if (b.pol != 0 && b.pol != 1) return fail("bh_bandstructure_run: pol must be 0 (TM) or 1 (TE)");
+        if (b.precond < -1 || b.precond > 2) return fail("bh_bandstructure_run: precond must be -1..2");
+        if (b.exit_rule != 0 && b.exit_rule != 1) return fail("bh_bandstructure_run: exit_rule must be 0 or 1");
+        if (b.geom.n_atoms < 0 || b.geom.n_atoms > BH_MAX_ATOMS)
+            return fail("bh_bandstructure_run: n_atoms out of range");
+        if (b.block_size < 0) return fail("bh_bandstructure_run: block_size must be >= 0");
         JobSpec &s = js[j];
         s.geom = to_geom(b.geom);
         s.nx = b.nx;
@@ -906,6 +987,9 @@ int bh_bandstructure_run(bh_context *ctx, const bh_job *jobs, int njobs, const b
         stats->timed_flops = rs.timed_flops;
         stats->host_wait_seconds = rs.host_wait_seconds;
         stats->host_plan_seconds = rs.host_plan_seconds;
+        stats->rr_problems = rs.rr_problems;
+        stats->svqb_fallbacks = rs.svqb_fallbacks;
+        stats->rank_drops = rs.rank_drops;
     }
     return 0;
     BH_API_CATCH

@@ -76,7 +76,7 @@ struct RRWork {
     const double *partial1, *partial2;  // from the dual Gram
     double *lam;            // ascending Ritz values (rank of them)
     double *C;              // col-major n x (cnt_x + cnt_w), leading dim ldc, complex interleaved
-    int *cnt;               // [cnt_x, cnt_w, cnt_x + cnt_w, Jacobi sweeps run]
+    int *cnt;               // [cnt_x, cnt_w, cnt_x + cnt_w, Jacobi sweeps run, SVQB path taken (0/1), rank]
     int n, nchunks, ld, ldc, nsplit, pad;
     int *sweeps_out;        // optional diagnostics (see rr_kernel)
 };

@@ -898,76 +898,6 @@ __global__ void __launch_bounds__(1024) eigh_kernel(const EighWork *__restrict__
         }
         __syncthreads();
     }
-    if (w.mode == 2) {
-        // Cholesky QR of the scaled Gram G = L L^H, T = D L^{-H}: the same span
-        // as SVQB when no direction would be dropped.  Falls back to the SVQB
-        // eigendecomposition below when a pivot shows G is ill-conditioned.
-        double2 *L = V;  // V is free until the Jacobi path starts
-        __shared__ int chol_ok;
-        if (tid == 0) chol_ok = 1;
-        for (int e = tid; e < ne * ne; e += EIGH_NT) L[e] = A[e];
-        __syncthreads();
-        for (int j = 0; j < n; ++j) {
-            if (tid == 0) {
-                double d = L[j + j * ne].x;
-                for (int k = 0; k < j; ++k) {
-                    double2 l = L[j + k * ne];
-                    d -= l.x * l.x + l.y * l.y;
-                }
-                if (!(d > 1e-8)) chol_ok = 0;  // relative pivot: columns have unit B-norm
-                L[j + j * ne] = make_double2(d > 0.0 ? sqrt(d) : 1.0, 0.0);
-            }
-            __syncthreads();
-            if (!chol_ok) break;
-            const double ljj = L[j + j * ne].x;
-            for (int i = j + 1 + tid; i < n; i += EIGH_NT) {
-                double2 a = L[i + j * ne];
-                for (int k = 0; k < j; ++k) {  // a -= L[i,k] conj(L[j,k])
-                    double2 x = L[i + k * ne], y = L[j + k * ne];
-                    a.x -= x.x * y.x + x.y * y.y;
-                    a.y -= x.y * y.x - x.x * y.y;
-                }
-                L[i + j * ne] = make_double2(a.x / ljj, a.y / ljj);
-            }
-            __syncthreads();
-        }
-        if (chol_ok) {
-            // M = L^{-1} by forward substitution, one column per thread; T = D M^H
-            for (int c = tid; c < n; c += EIGH_NT) {
-                // M[i][c] for i >= c; kept in the A buffer's column c (A no longer needed)
-                for (int i = 0; i < n; ++i) A[i + c * ne] = make_double2(0.0, 0.0);
-                for (int i = c; i < n; ++i) {
-                    double2 acc = make_double2(i == c ? 1.0 : 0.0, 0.0);
-                    for (int k = c; k < i; ++k) {
-                        double2 l = L[i + k * ne], m = A[k + c * ne];
-                        acc.x -= l.x * m.x - l.y * m.y;
-                        acc.y -= l.x * m.y + l.y * m.x;
-                    }
-                    double lii = L[i + i * ne].x;
-                    A[i + c * ne] = make_double2(acc.x / lii, acc.y / lii);
-                }
-            }
-            __syncthreads();
-            for (int e = tid; e < n * n; e += EIGH_NT) {
-                int r = e % n, c = e / n;  // T[r][c] = D[r] conj(M[c][r]), zero below the diagonal
-                double2 m = r <= c ? A[c + r * ne] : make_double2(0.0, 0.0);
-                size_t o = 2 * ((size_t)r + (size_t)c * w.ldv);
-                w.V[o] = D[r] * m.x;
-                w.V[o + 1] = -D[r] * m.y;
-            }
-            if (tid == 0) {
-                if (w.rank_out) *w.rank_out = n;
-                if (w.sweeps_out) atomicAdd(w.sweeps_out + 1, 1);  // Cholesky path taken
-            }
-            return;
-        }
-        // fallback: restore V = I and run SVQB's eigendecomposition
-        for (int e = tid; e < ne * ne; e += EIGH_NT) {
-            int i = e % ne, j = e / ne;
-            V[e] = make_double2(i == j ? 1.0 : 0.0, 0.0);
-        }
-        __syncthreads();
-    }
     const int sweeps = jacobi_lds<EIGH_NT>(A, V, ne, ne, rot, red);
     if (tid == 0 && w.sweeps_out) {
         atomicAdd(w.sweeps_out, sweeps);
@@ -1250,7 +1180,8 @@ template void launch_gram2<float>(const Gram2Work<float> *, int, int, long, int,
 // Rayleigh-Ritz on the dual Gram, one 1024-thread workgroup per instance, FP64.
 //  1. G1 (Hermitian, from the lower partial sums), scaled by D = diag(G1)^-1/2
 //     (normalization.rs:281-345 normalises each column to unit B-norm first).
-//  2. T: Cholesky G1s = L L^H with every pivot > 1e-8 -> T = D L^-H (upper
+//  2. T: Cholesky G1s = L L^H with every pivot > 1e-8 and ||L^-1||_F^2 n 1e-12
+//     <= 1 (which proves SVQB would keep every direction) -> T = D L^-H (upper
 //     triangular, rank n); else SVQB: G1s = Q diag(l) Q^H, keep l >= 1e-12 lmax
 //     and l > 0 in descending order, T = D Q_k diag(l_k)^-1/2
 //     (normalization.rs:355-531).  Both give B-orthonormal Z T spanning the
@@ -1290,7 +1221,7 @@ __global__ void __launch_bounds__(NT) rr_kernel(const RRWork *__restrict__ work)
     const int n = w.n;
     const int tid = threadIdx.x;
     if (n <= 0) {
-        if (tid == 0) { w.cnt[0] = 0; w.cnt[1] = 0; w.cnt[2] = 0; w.cnt[3] = 0; }
+        if (tid == 0) { w.cnt[0] = 0; w.cnt[1] = 0; w.cnt[2] = 0; w.cnt[3] = 0; w.cnt[4] = 0; w.cnt[5] = 0; }
         return;
     }
     const int ne = n + (n & 1);
@@ -1308,6 +1239,8 @@ __global__ void __launch_bounds__(NT) rr_kernel(const RRWork *__restrict__ work)
     double *piv = wsh + ne;
     int *rank_of = reinterpret_cast<int *>(piv + ne);
     __shared__ int rank, chol_ok;
+    __shared__ double msq;
+    int svqb_used = 0;
     int rr_fallback_sweeps = 0;  // Jacobi sweeps of the SVQB fallback (thread 0's copy is reported)
     __shared__ double lmax;
     __shared__ double red[1];
@@ -1410,7 +1343,9 @@ __global__ void __launch_bounds__(NT) rr_kernel(const RRWork *__restrict__ work)
             const int i = e % ne, j = e / ne;
             A[i + j * lda] = make_double2(i == j ? 1.0 : 0.0, 0.0);
         }
+        if (tid == 0) msq = 0.0;
         __syncthreads();
+        double msq_own = 0.0;  // this thread's share of ||M||_F^2
         for (int i = 0; i < n; ++i) {
             const double inv = 1.0 / piv[i];
 #pragma unroll
@@ -1421,6 +1356,7 @@ __global__ void __launch_bounds__(NT) rr_kernel(const RRWork *__restrict__ work)
                 const double2 m = make_double2(b.x * inv, b.y * inv);  // M[i][c]
                 if (k == i) {
                     Tm[c + i * lda] = make_double2(D[c] * m.x, -D[c] * m.y);
+                    msq_own += m.x * m.x + m.y * m.y;
                 } else {
                     const double2 l = L[k + i * lda];
                     double2 bk = A[k + c * lda];
@@ -1430,12 +1366,35 @@ __global__ void __launch_bounds__(NT) rr_kernel(const RRWork *__restrict__ work)
             }
             __syncthreads();
         }
+        // SVQB keeps every direction iff lambda_min >= 1e-12 lambda_max of the
+        // scaled G1.  lambda_max <= trace = n (unit diagonal) and lambda_min =
+        // 1 / ||L^-1||_2^2 >= 1 / ||M||_F^2, so ||M||_F^2 n 1e-12 <= 1 proves
+        // that SVQB would drop nothing and Z T spans what SVQB's basis spans.
+        // Otherwise (a nearly dependent block whose Cholesky pivots still
+        // passed) the SVQB path decides, on G1 reloaded from the partial sums.
+        for (int o = 32; o > 0; o >>= 1) msq_own += __shfl_xor(msq_own, o);
+        if ((tid & 63) == 0) atomicAdd(&msq, msq_own);
+        __syncthreads();
         if (tid == 0) {
-            rank = n;
-            if (w.sweeps_out) atomicAdd(w.sweeps_out + 1, 1);
+            if (msq * (double)n * 1e-12 > 1.0) chol_ok = 0;
+            else rank = n;
+            if (chol_ok && w.sweeps_out) atomicAdd(w.sweeps_out + 1, 1);
         }
-    } else {
-        // SVQB: eigendecomposition of the scaled G1 (A still holds it)
+        __syncthreads();
+        if (!chol_ok) {
+            load(w.partial1);
+            __syncthreads();
+            for (int e = tid; e < ne * ne; e += NT) {
+                const int i = e % ne, j = e / ne;
+                const double s = D[i] * D[j];
+                A[i + j * lda] = make_double2(A[i + j * lda].x * s, A[i + j * lda].y * s);
+            }
+            __syncthreads();
+        }
+    }
+    if (!chol_ok) {
+        // SVQB: eigendecomposition of the scaled G1 (A holds it)
+        svqb_used = 1;
         for (int e = tid; e < ne * ne; e += NT) {
             const int i = e % ne, j = e / ne;
             V[i + j * lda] = make_double2(i == j ? 1.0 : 0.0, 0.0);
@@ -1579,6 +1538,8 @@ __global__ void __launch_bounds__(NT) rr_kernel(const RRWork *__restrict__ work)
         w.cnt[1] = cw;
         w.cnt[2] = cx + cw;
         w.cnt[3] = sw + rr_fallback_sweeps;
+        w.cnt[4] = svqb_used;
+        w.cnt[5] = r;
     }
 }
 

@@ -55,6 +55,9 @@ struct RunStats {
     long eigh_launch_max_sum[2] = {0, 0};  // per launch maximum, summed over launches
     double host_wait_seconds = 0;          // blocked on group completion events
     double host_plan_seconds = 0;          // planning and enqueueing
+    long rr_problems = 0;                  // Rayleigh-Ritz problems solved
+    long svqb_fallbacks = 0;               // of them, solved on the SVQB path (Cholesky QR not provably equivalent)
+    long rank_drops = 0;                   // of them, with a direction dropped (rank < subspace width)
 };
 
 struct EngineOptions {

@@ -215,6 +215,9 @@ T *dalloc(size_t n, hipStream_t st) {
 
 enum SlotState { FREE = 0, GRAM_PENDING = 1, ITER = 2 };
 
+// ints per slot in the Rayleigh-Ritz counter array (RRWork::cnt)
+constexpr int CNT_STRIDE = 8;
+
 struct Slot {
     int state = FREE;
     int job = -1;
@@ -289,9 +292,9 @@ class Engine : public EngineBase {
         d_lam = dalloc<double>((size_t)S * LD, st);
         d_Y = dalloc<double>((size_t)S * LD * LD * 2, st);
         d_coef = dalloc<double>((size_t)S * LD * LD * 2, st);
-        d_cnt = dalloc<int>(4 * (size_t)S, st);
+        d_cnt = dalloc<int>(CNT_STRIDE * (size_t)S, st);
         BH_HIP_CHECK(hipHostMalloc((void **)&h_lam, sizeof(double) * S * LD, hipHostMallocDefault));
-        BH_HIP_CHECK(hipHostMalloc((void **)&h_cnt, sizeof(int) * 4 * S, hipHostMallocDefault));
+        BH_HIP_CHECK(hipHostMalloc((void **)&h_cnt, sizeof(int) * CNT_STRIDE * S, hipHostMallocDefault));
         int ng = opt.profile ? 1 : std::max(1, std::min(opt.groups, S));
         slots_per_group = (S + ng - 1) / ng;
         ng = (S + slots_per_group - 1) / slots_per_group;
@@ -535,8 +538,12 @@ template <typename T>
 void Engine<T>::absorb(int s) {
     Slot &sl = slots[s];
     if (sl.it == 0) return;
-    int cx = h_cnt[4 * s], cw = h_cnt[4 * s + 1];
-    if (eopt.timed_kernel == TIMED_RR) stats_->timed_flops += rr_flops(sl.last_p, cx + cw, h_cnt[4 * s + 3]);
+    const int *cnt = h_cnt + CNT_STRIDE * (size_t)s;
+    int cx = cnt[0], cw = cnt[1];
+    if (eopt.timed_kernel == TIMED_RR) stats_->timed_flops += rr_flops(sl.last_p, cx + cw, cnt[3]);
+    stats_->rr_problems += 1;
+    stats_->svqb_fallbacks += cnt[4];
+    if (cnt[5] < sl.last_p) stats_->rank_drops += 1;
     sl.lam_prev = sl.lam;
     sl.lam_prev_valid = sl.it >= 2;
     if (cx > 0) {
@@ -1132,7 +1139,7 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
         RRWork rw2{};
         rw2.lam = d_lam + (size_t)s * LD;
         rw2.C = d_Y + (size_t)s * LD * LD * 2;
-        rw2.cnt = d_cnt + 4 * s;
+        rw2.cnt = d_cnt + CNT_STRIDE * s;
         rw2.n = p;
         rw2.ld = LD;
         rw2.ldc = LD;
@@ -1147,7 +1154,7 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
         lx.A = vcol(s, 0);
         lx.a_stride = N;
         lx.na = p;
-        lx.dev_nq = d_cnt + 4 * s + 2;
+        lx.dev_nq = d_cnt + CNT_STRIDE * s + 2;
         lx.C = d_Y + (size_t)s * LD * LD * 2;
         lx.ldc = LD;
         lx.out = vbuf(s, nxt, 0);
@@ -1353,7 +1360,7 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
         const int gs = G.s1 - G.s0;
         BH_HIP_CHECK(hipMemcpyAsync(h_lam + (size_t)G.s0 * LD, d_lam + (size_t)G.s0 * LD, sizeof(double) * gs * LD,
                                     hipMemcpyDeviceToHost, st));
-        BH_HIP_CHECK(hipMemcpyAsync(h_cnt + 4 * (size_t)G.s0, d_cnt + 4 * (size_t)G.s0, sizeof(int) * 4 * gs,
+        BH_HIP_CHECK(hipMemcpyAsync(h_cnt + CNT_STRIDE * (size_t)G.s0, d_cnt + CNT_STRIDE * (size_t)G.s0, sizeof(int) * CNT_STRIDE * gs,
                                     hipMemcpyDeviceToHost, st));
     }
     BH_HIP_CHECK(hipEventRecord(G.ev, st));

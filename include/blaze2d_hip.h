@@ -76,6 +76,22 @@ int bh_lincomb(bh_context *ctx, const bh_buffer *q, int qi0, int r, const double
  * col-major eigenvectors.  dense::solve_hermitian_eigen (eigensolver/dense.rs:71-134). */
 int bh_hermitian_eigh(bh_context *ctx, int n, int count, const double *a, double *w, double *v);
 
+/* Rayleigh-Ritz of `count` subspaces Z of width n (n <= 48) from their Grams,
+ * FP64 on the device.  g1 = Z^H B Z and g2 = Z^H A Z, col-major n x n complex
+ * per problem (lower triangles read).  The block is B-orthonormalised as
+ * svqb_orthonormalize does (normalization.rs:281-544: unit B-norm columns,
+ * directions below 1e-12 of the largest Gram eigenvalue dropped; Cholesky QR
+ * is taken only when it provably drops nothing), the operator projected
+ * (eigensolver.rs:1232-1316) and diagonalised in ascending order
+ * (dense.rs:71-134).  Outputs per problem: lam[n] (rank ascending Ritz values,
+ * then zeros); coef[n x n] col-major complex whose first cnt_x + cnt_w columns
+ * are the coefficients over Z's columns of the new X (cnt_x = min(rank,
+ * nsplit)) and W (cnt_w = min(2 nsplit, rank) - cnt_x) Ritz vectors
+ * (eigensolver.rs:1341-1637), the rest zero; info[4] = {rank, SVQB path taken
+ * (0 or 1), cnt_x, cnt_w}. */
+int bh_rayleigh_ritz(bh_context *ctx, int n, int count, int nsplit, const double *g1, const double *g2, double *lam,
+                     double *coef, int *info);
+
 /* ---- LinearOperator / OperatorPreconditioner level -------------------- */
 
 typedef struct {
@@ -171,6 +187,9 @@ typedef struct {
     double timed_flops;          /* their algorithmic FLOPs */
     double host_wait_seconds;    /* host time blocked on group completion events */
     double host_plan_seconds;    /* host time spent planning and enqueueing steps */
+    long rr_problems;            /* Rayleigh-Ritz problems solved */
+    long svqb_fallbacks;         /* of them, B-orthonormalised on the SVQB path */
+    long rank_drops;             /* of them, with a direction dropped (rank below the subspace width) */
 } bh_run_stats;
 
 /* One-time operator diagnostics of a job (drivers/bandstructure.rs:437-498):

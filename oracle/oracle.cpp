@@ -940,7 +940,22 @@ void seed_random(Field &v, double phase) {
     }
 }
 
-struct SolveResult { std::vector<double> eigenvalues; int iterations; bool converged; };
+// rel: relative residual B-norms ||r_i||_B / max(|lambda_i|, 1e-10) of the
+// first n_bands block entries at the last "log" iteration the solve ran
+// (eigensolver.rs:744-779, taken at :1729-1737); rel_iter: that iteration, 1-based
+struct SolveResult {
+    std::vector<double> eigenvalues;
+    int iterations;
+    bool converged;
+    std::vector<double> rel;
+    int rel_iter;
+};
+
+// eigensolver.rs:251-259 should_log_iteration (iter 0-based)
+bool should_log_iteration(int iter) {
+    const int n = iter + 1;
+    return (n >= 1 && n <= 5) || n == 10 || n == 20 || n == 50 || n == 100 || (n > 100 && n % 50 == 0);
+}
 
 struct Eigensolver {
     Operator &op;
@@ -953,6 +968,8 @@ struct Eigensolver {
     Deflation defl;
     std::vector<double> lam, lam_prev;
     std::vector<char> soft;
+    std::vector<double> last_rel;
+    int last_rel_iter = 0;
 
     Eigensolver(Operator &o, Preconditioner *p, const std::vector<Field> *w, int nb, int mi, double t,
                 int bs, int er)
@@ -1098,7 +1115,9 @@ struct Eigensolver {
     SolveResult finish(bool conv, int iters) {
         auto a = all_eigenvalues();
         a.resize(std::min((size_t)n_bands, a.size()));
-        return {a, iters, conv};
+        std::vector<double> rel = last_rel;
+        rel.resize(std::min((size_t)n_bands, rel.size()));
+        return {a, iters, conv, rel, last_rel_iter};
     }
 
     // eigensolver.rs:1684-2192 solve
@@ -1122,6 +1141,17 @@ struct Eigensolver {
             // 2. deflation
             if (!defl.empty()) defl.project_block_no_mass(R);
             soft_deflate(R);
+            // 3. residual B-norms on log iterations (:744-779, :1729-1737)
+            if (should_log_iteration(iter)) {
+                last_rel.assign(m, 0.0);
+                for (size_t i = 0; i < m; ++i) {
+                    Field br;
+                    op.apply_mass(R[i], br);
+                    const double nb2 = std::max(dot(R[i], br).real(), 0.0);
+                    last_rel[i] = std::sqrt(nb2) / std::max(std::fabs(lam[i]), 1e-10);
+                }
+                last_rel_iter = iter + 1;
+            }
             // 4. convergence (:1750-1801)
             int n_conv = 0;
             std::vector<double> ch;
@@ -1411,7 +1441,8 @@ Geometry geometry_from_job(const OracleJob *j) {
     return g;
 }
 
-int run_core(const OracleJob *job, double *omegas_out, int *iters_out, int *conv_out) {
+int run_core(const OracleJob *job, double *omegas_out, int *iters_out, int *conv_out, double *rel_out,
+             int *rel_iter_out) {
     Geometry g = geometry_from_job(job);
     Dielectric d = build_dielectric(g, job->nx, job->ny, job->smoothing != 0);
     int nb = job->n_bands;
@@ -1427,8 +1458,8 @@ int run_core(const OracleJob *job, double *omegas_out, int *iters_out, int *conv
     bool first_g = nk > 0 && is_gamma_frac(job->kpath);
     bool last_g = nk > 0 && is_gamma_frac(job->kpath + 2 * (nk - 1));
     bool reuse = job->reuse_gamma && first_g && last_g && nk > 1;
-    std::vector<double> first_gamma;
-    int first_gamma_iters = 0;
+    std::vector<double> first_gamma, first_gamma_rel;
+    int first_gamma_iters = 0, first_gamma_rel_iter = 0;
     bool have_first_gamma = false;
     int total_iters = 0;
     for (int k = 0; k < nk; ++k) {
@@ -1438,6 +1469,10 @@ int run_core(const OracleJob *job, double *omegas_out, int *iters_out, int *conv
             for (int b = 0; b < nb; ++b) omegas_out[(size_t)k * nb + b] = b < (int)first_gamma.size() ? first_gamma[b] : NAN;
             iters_out[k] = first_gamma_iters;
             conv_out[k] = 1;
+            if (rel_out)
+                for (int b = 0; b < nb; ++b)
+                    rel_out[(size_t)k * nb + b] = b < (int)first_gamma_rel.size() ? first_gamma_rel[b] : NAN;
+            if (rel_iter_out) rel_iter_out[k] = first_gamma_rel_iter;
             continue;
         }
         double bloch[2] = {kf[0] * d.b1[0] + kf[1] * d.b2[0], kf[0] * d.b1[1] + kf[1] * d.b2[1]};
@@ -1478,7 +1513,16 @@ int run_core(const OracleJob *job, double *omegas_out, int *iters_out, int *conv
         prev_om = om;
         have_prev = true;
         if (job->subspace_prediction) hist.update(kf, vecs);
-        if (reuse && k == 0 && isg) { first_gamma = om; first_gamma_iters = r.iterations; have_first_gamma = true; }
+        if (reuse && k == 0 && isg) {
+            first_gamma = om;
+            first_gamma_iters = r.iterations;
+            first_gamma_rel = r.rel;
+            first_gamma_rel_iter = r.rel_iter;
+            have_first_gamma = true;
+        }
+        if (rel_out)
+            for (int b = 0; b < nb; ++b) rel_out[(size_t)k * nb + b] = b < (int)r.rel.size() ? r.rel[b] : NAN;
+        if (rel_iter_out) rel_iter_out[k] = r.rel_iter;
         for (int b = 0; b < nb; ++b) omegas_out[(size_t)k * nb + b] = b < (int)om.size() ? om[b] : NAN;
         iters_out[k] = r.iterations;
         conv_out[k] = r.converged ? 1 : 0;
@@ -1503,7 +1547,13 @@ extern "C" {
 
 int oracle_bandstructure(const OracleJob *job, double *omegas, int *iters, int *converged) {
     if (!job || job->nx <= 0 || job->ny <= 0 || job->n_bands <= 0) return -1;
-    return run_core(job, omegas, iters, converged);
+    return run_core(job, omegas, iters, converged, nullptr, nullptr);
+}
+
+int oracle_bandstructure_residuals(const OracleJob *job, double *omegas, int *iters, int *converged,
+                                   double *rel_residuals, int *rel_residual_iters) {
+    if (!job || job->nx <= 0 || job->ny <= 0 || job->n_bands <= 0) return -1;
+    return run_core(job, omegas, iters, converged, rel_residuals, rel_residual_iters);
 }
 
 int oracle_dielectric(const OracleJob *job, double *eps, double *inv_eps, double *tensor) {

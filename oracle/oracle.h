@@ -47,6 +47,12 @@ typedef struct {
  * omegas: nk*n_bands (omega, not omega/2pi); iters, converged: nk each.
  * Returns total LOBPCG iterations, or -1 on error. */
 int oracle_bandstructure(const OracleJob *job, double *omegas, int *iters, int *converged);
+/* The same, also returning per k-point the relative residual B-norms
+ * ||r_i||_B / max(|lambda_i|, 1e-10) of the first n_bands block entries (in
+ * the solver's block order) at the last log iteration of the solve
+ * (eigensolver.rs:251-259,744-779,1729-1737) and that iteration (1-based). */
+int oracle_bandstructure_residuals(const OracleJob *job, double *omegas, int *iters, int *converged,
+                                   double *rel_residuals, int *rel_residual_iters);
 
 /* Dielectric sampling / analytic smoothing (dielectric.rs).  Outputs N values
  * (eps, inv_eps) and N*4 tensor entries [xx, xy, yx, yy]. */

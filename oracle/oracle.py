@@ -65,6 +65,8 @@ def lib() -> ctypes.CDLL:
         ip = ctypes.POINTER(ctypes.c_int)
         L.oracle_bandstructure.argtypes = [ctypes.POINTER(_OracleJob), dp, ip, ip]
         L.oracle_bandstructure.restype = ctypes.c_int
+        L.oracle_bandstructure_residuals.argtypes = [ctypes.POINTER(_OracleJob), dp, ip, ip, dp, ip]
+        L.oracle_bandstructure_residuals.restype = ctypes.c_int
         L.oracle_dielectric.argtypes = [ctypes.POINTER(_OracleJob), dp, dp, dp]
         L.oracle_dielectric.restype = ctypes.c_int
         L.oracle_fft2d.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, ctypes.c_int]
@@ -166,6 +168,23 @@ def bandstructure(job: Job):
     if tot < 0:
         raise RuntimeError("oracle_bandstructure failed")
     return om, it, cv, tot
+
+
+def bandstructure_residuals(job: Job):
+    """bandstructure() plus, per k-point, the relative residual B-norms of the
+    first n_bands block entries at the solve's last log iteration and that
+    iteration (1-based): (omegas, iterations, converged, total, rel[nk, nb], rel_iter[nk])."""
+    cj, kp = job.to_c()
+    nk = cj.nk
+    om = np.zeros((nk, job.n_bands))
+    it = np.zeros(nk, dtype=np.int32)
+    cv = np.zeros(nk, dtype=np.int32)
+    rel = np.zeros((nk, job.n_bands))
+    rit = np.zeros(nk, dtype=np.int32)
+    tot = lib().oracle_bandstructure_residuals(ctypes.byref(cj), _dp(om), _ip(it), _ip(cv), _dp(rel), _ip(rit))
+    if tot < 0:
+        raise RuntimeError("oracle_bandstructure_residuals failed")
+    return om, it, cv, tot, rel, rit
 
 
 def dielectric(job: Job):

@@ -63,9 +63,36 @@ def test_expand_multiple_ranges_legacy():
 
 
 def test_expand_empty_legacy_axis_gives_no_jobs():
-    # an empty range (min > max) or list is an axis of length zero, not an absent axis
-    assert bulk.expand_jobs(_cfg("[ranges]\neps_bg = {min = 12.0, max = 10.0, step = 1.0}\n")) == []
+    # an empty list is an axis of length zero, not an absent axis
     assert bulk.expand_jobs(_cfg("[ranges]\npolarization = []\n")) == []
+
+
+@pytest.mark.parametrize("extra,msg", [
+    # BulkConfig::validate (config.rs:1215-1300): legacy ranges
+    ("[ranges]\neps_bg = {min = 12.0, max = 10.0, step = 1.0}\n", "eps_bg min > max"),
+    ("[ranges]\neps_bg = {min = 10.0, max = 12.0, step = 0.0}\n", "eps_bg step"),
+    ("[ranges]\neps_bg = {min = 10.0, max = 12.0, step = -1.0}\n", "eps_bg step"),
+    ("[ranges]\nresolution = {min = 2.0, max = 32.0, step = 8.0}\n", "resolution min"),
+    ("[ranges]\nresolution = {min = 16.0, max = 32.0, step = 0.5}\n", "resolution step"),
+    # SweepSpec::validate (config.rs:338-376)
+    ('[[sweeps]]\nparameter = "eps_bg"\nmin = 10.0\nmax = 12.0\nstep = 0.0\n', "step must be positive"),
+    ('[[sweeps]]\nparameter = "eps_bg"\nmin = 12.0\nmax = 10.0\nstep = 1.0\n', "min"),
+    ('[[sweeps]]\nparameter = "eps_bg"\n', "either"),
+    ('[[sweeps]]\nparameter = "eps_bg"\nmin = 1.0\nmax = 2.0\nstep = 1.0\nvalues = [1.0]\n', "both"),
+    ('[[sweeps]]\nparameter = "eps_bg"\nvalues = [1.0]\n[[sweeps]]\nparameter = "eps_bg"\nvalues = [2.0]\n',
+     "duplicate"),
+    ('[[sweeps]]\nparameter = "atomX.radius"\nvalues = [0.2]\n', "invalid sweep parameter"),
+    ('[[sweeps]]\nparameter = "atom0.colour"\nvalues = [0.2]\n', "invalid sweep parameter"),
+])
+def test_invalid_configs_are_rejected(extra, msg):
+    with pytest.raises(ValueError, match=msg):
+        _cfg(extra)
+
+
+def test_range_values_rejects_non_positive_step():
+    for step in (0.0, -0.5):
+        with pytest.raises(ValueError):
+            bulk.range_values(0.0, 1.0, step)
 
 
 def test_expand_ordered_sweeps():

@@ -144,6 +144,47 @@ def test_preconditioners_formulas():
         assert np.allclose(got, want, atol=1e-12)
 
 
+@pytest.mark.parametrize("lattice", ["square", "hex"])
+@pytest.mark.parametrize("pol", ["TM", "TE"])
+@pytest.mark.parametrize("kfrac", [(0.13, 0.27), (0.0, 0.0)])
+def test_preconditioners_uniform_plane_wave_closed_form(lattice, pol, kfrac):
+    # In a uniform medium eps(r) = e every plane wave exp(i G.r) is an
+    # eigenvector of both preconditioners, with closed-form eigenvalues
+    # (q = |k+G|, s = 0.5 * the smallest |k+G|^2 above 1e-6):
+    #   TM Fourier diagonal:     e / (q^2 + 0.01 e + s e)
+    #   TE transverse projection: -q^2 e / (q^2 + s)^2   (six FFTs and a real-space
+    #   eps multiply in between, so this checks the gradient / divergence signs)
+    # and 0 for the clamped G = 0 mode at Gamma.
+    nx, ny, e = 12, 10, 7.5
+    job = O.Job(lattice=lattice, eps_bg=e, atoms=[], nx=nx, ny=ny, pol=pol)
+    a1, a2 = job.lattice_vectors()
+    det = a1[0] * a2[1] - a1[1] * a2[0]
+    b1 = np.array([2 * np.pi * a2[1] / det, -2 * np.pi * a2[0] / det])
+    b2 = np.array([-2 * np.pi * a1[1] / det, 2 * np.pi * a1[0] / det])
+    kc = kfrac[0] * b1 + kfrac[1] * b2
+
+    def qsq(n1, n2):
+        m1 = n1 if n1 <= nx // 2 else n1 - nx
+        m2 = n2 if n2 <= ny // 2 else n2 - ny
+        v = kc + m1 * b1 + m2 * b2
+        return float(v @ v)
+
+    allq = np.array([qsq(i, j) for j in range(ny) for i in range(nx)])
+    s = 0.5 * allq[allq > 1e-6].min()
+    iy, ix = np.mgrid[0:ny, 0:nx]
+    for n1, n2 in [(0, 0), (1, 0), (0, 1), (3, 7), (11, 9), (6, 5)]:
+        x = np.exp(2j * np.pi * (n1 * ix / nx + n2 * iy / ny)).reshape(-1)
+        q2 = qsq(n1, n2)
+        if q2 <= 1e-9:
+            want = 0.0
+        elif pol == "TM":
+            want = e / (q2 + 0.01 * e + s * e)
+        else:
+            want = -q2 * e / (q2 + s) ** 2
+        got = O.operator_apply(job, kfrac, "P", x)[0]
+        assert np.abs(got - want * x).max() <= 1e-12 * max(abs(want), 1e-3), (n1, n2, want)
+
+
 # --- dense linear algebra ---------------------------------------------------
 @pytest.mark.parametrize("n", [1, 2, 3, 12, 36, 48])
 def test_eigh_matches_numpy(n):
