@@ -109,6 +109,7 @@ EXPORTS = [
     "bh_axpy", "bh_dot", "bh_gram", "bh_lincomb", "bh_theta_create", "bh_theta_destroy", "bh_theta_apply",
     "bh_mass_apply", "bh_precond_apply", "bh_dielectric", "bh_dielectric_batch", "bh_bandstructure_run",
     "bh_hermitian_eigh", "bh_operator_diagnostics", "bh_rayleigh_ritz",
+    "bh_bandstructure_run_ex",
 ]
 
 _lib = None
@@ -154,6 +155,8 @@ def lib() -> ctypes.CDLL:
                                       ctypes.c_int, ctypes.c_int, c_dp, c_dp, c_dp, c_dp]
     L.bh_bandstructure_run.argtypes = [vp, ctypes.POINTER(Job), ctypes.c_int, ctypes.POINTER(RunOptions), c_dp, c_ip,
                                        c_ip, ctypes.POINTER(RunStats)]
+    L.bh_bandstructure_run_ex.argtypes = [vp, ctypes.POINTER(Job), ctypes.c_int, ctypes.POINTER(RunOptions), c_dp,
+                                          c_ip, c_ip, c_dp, c_ip, ctypes.POINTER(RunStats)]
     L.bh_operator_diagnostics.argtypes = [vp, ctypes.POINTER(Job), ctypes.c_int, ctypes.c_int,
                                           ctypes.POINTER(Diagnostics), c_dp]
     _lib = L

@@ -140,6 +140,11 @@ class BandStructureResult:
     bands: np.ndarray           # [nk, n_bands] omega
     iterations: np.ndarray      # [nk]
     converged: np.ndarray       # [nk]
+    # relative residual B-norms ||r||_B / |lambda| of the first n_bands block
+    # entries at each solve's last log iteration (eigensolver.rs:744-779), and
+    # that iteration (1-based)
+    relative_residuals: Optional[np.ndarray] = None   # [nk, n_bands]
+    residual_iteration: Optional[np.ndarray] = None   # [nk]
 
 
 @dataclass
@@ -185,17 +190,23 @@ def run_batch(backend: HipBackend, jobs: Sequence[BandStructureJob], options: Op
     om = np.zeros(tot_om)
     it = np.zeros(tot_k, dtype=np.int32)
     cv = np.zeros(tot_k, dtype=np.int32)
+    rel = np.zeros(tot_om)
+    rit = np.zeros(tot_k, dtype=np.int32)
     ro = L.RunOptions(slots, precision, int(profile), groups, timed_kernel)
     st = L.RunStats()
-    L.check(L.lib().bh_bandstructure_run(backend.handle, arr, len(jobs), ctypes.byref(ro), om.ctypes.data_as(L.c_dp),
-                                         it.ctypes.data_as(L.c_ip), cv.ctypes.data_as(L.c_ip), ctypes.byref(st)),
-            "bh_bandstructure_run")
+    L.check(L.lib().bh_bandstructure_run_ex(backend.handle, arr, len(jobs), ctypes.byref(ro),
+                                            om.ctypes.data_as(L.c_dp), it.ctypes.data_as(L.c_ip),
+                                            cv.ctypes.data_as(L.c_ip), rel.ctypes.data_as(L.c_dp),
+                                            rit.ctypes.data_as(L.c_ip), ctypes.byref(st)),
+            "bh_bandstructure_run_ex")
     out = []
     o = k = 0
     for j in jobs:
         nk = j.k_path.shape[0]
         out.append(BandStructureResult(j.k_path, om[o:o + nk * j.n_bands].reshape(nk, j.n_bands).copy(),
-                                       it[k:k + nk].copy(), cv[k:k + nk].copy()))
+                                       it[k:k + nk].copy(), cv[k:k + nk].copy(),
+                                       rel[o:o + nk * j.n_bands].reshape(nk, j.n_bands).copy(),
+                                       rit[k:k + nk].copy()))
         o += nk * j.n_bands
         k += nk
     stats = BatchStats(st.steps, st.iterations, st.solves, st.seconds, st.timed_ms, st.timed_launches,

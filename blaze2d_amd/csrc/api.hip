@@ -904,6 +904,12 @@ int bh_operator_diagnostics(bh_context *ctx, const bh_job *jobs, int njobs, int 
 
 int bh_bandstructure_run(bh_context *ctx, const bh_job *jobs, int njobs, const bh_run_options *opts, double *omegas,
                          int *iters, int *converged, bh_run_stats *stats) {
+    return bh_bandstructure_run_ex(ctx, jobs, njobs, opts, omegas, iters, converged, nullptr, nullptr, stats);
+}
+
+int bh_bandstructure_run_ex(bh_context *ctx, const bh_job *jobs, int njobs, const bh_run_options *opts,
+                            double *omegas, int *iters, int *converged, double *rel_residuals,
+                            int *rel_residual_iters, bh_run_stats *stats) {
     BH_API_TRY
     if (!ctx || (!jobs && njobs > 0) || njobs < 0) return fail("bh_bandstructure_run: bad arguments");
     if (njobs == 0) return 0;
@@ -963,9 +969,12 @@ int bh_bandstructure_run(bh_context *ctx, const bh_job *jobs, int njobs, const b
     for (int j = 0; j < njobs; ++j) {
         const JobResult &r = res[j];
         if (omegas) std::memcpy(omegas + off, r.omegas.data(), sizeof(double) * r.omegas.size());
+        if (rel_residuals)
+            std::memcpy(rel_residuals + off, r.rel_residuals.data(), sizeof(double) * r.rel_residuals.size());
         for (size_t k = 0; k < r.iters.size(); ++k) {
             if (iters) iters[koff + k] = r.iters[k];
             if (converged) converged[koff + k] = r.converged[k];
+            if (rel_residual_iters) rel_residual_iters[koff + k] = r.rel_iters[k];
         }
         off += r.omegas.size();
         koff += r.iters.size();

@@ -36,6 +36,10 @@ struct JobSpec {
 struct JobResult {
     std::vector<double> omegas;  // nk * n_bands
     std::vector<int> iters, converged;
+    // relative residual B-norms of the first n_bands block entries at each
+    // solve's last log iteration (nk * n_bands) and that iteration, 1-based (nk)
+    std::vector<double> rel_residuals;
+    std::vector<int> rel_iters;
 };
 
 struct RunStats {

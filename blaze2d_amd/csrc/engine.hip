@@ -184,6 +184,7 @@ struct Group {
     hipEvent_t ev{};
     Staging stg;
     double *part = nullptr, *gramrb = nullptr, *h_gramrb = nullptr;
+    double *h_rel = nullptr;  // readback of the group's residual norms
     int *sweeps = nullptr;
     int s0 = 0, s1 = 0;
     bool inflight = false;
@@ -234,6 +235,7 @@ struct Slot {
     int par = 0;  // current copy of V / AV
     int last_p = 0;  // subspace width of the iteration in flight
     std::vector<char> soft;
+    std::vector<char> soft_iter;  // soft-locked set when the current iteration's residuals are formed
     std::vector<double> lam, lam_prev;
     bool lam_prev_valid = false;
     bool gamma_defl = false;
@@ -255,6 +257,18 @@ struct Slot {
     bool have_first_gamma = false;
     std::vector<double> first_gamma;
     int first_gamma_iters = 0;
+    std::vector<double> first_gamma_rel;
+    int first_gamma_rel_iter = 0;
+    // relative residual B-norms: the latest log iteration's values (block
+    // order) and the norms of an iteration in flight (read back next step)
+    std::vector<double> rel_last;
+    int rel_last_iter = 0;
+    bool rel_pending = false;     // an iteration with norms is in flight
+    bool rel_final = false;       // norms of the exit iteration are computed in the transition step
+    int rel_pending_iter = 0;     // its 1-based iteration
+    std::vector<int> rel_act;     // its active (non-soft-locked) bands
+    std::vector<char> rel_soft;   // soft-locked bands at the exit iteration (rel_final)
+    int rel_m = 0;                // block size of that iteration
     // planned transition coefficients (set in decide, consumed in plan)
     bool t2_pending = false;
     std::vector<zd> cx;       // kd x m0 coefficients over the physical transition basis
@@ -264,6 +278,13 @@ struct Slot {
 };
 
 bool is_gamma_frac(const std::array<double, 2> &k) { return k[0] * k[0] + k[1] * k[1] < 1e-8; }
+
+// eigensolver.rs:251-259 should_log_iteration (iteration index 0-based): the
+// iterations whose residual B-norms the reference computes
+bool should_log_iteration(int iter) {
+    const int n = iter + 1;
+    return (n >= 1 && n <= 5) || n == 10 || n == 20 || n == 50 || n == 100 || (n > 100 && n % 50 == 0);
+}
 
 }  // namespace
 
@@ -293,6 +314,7 @@ class Engine : public EngineBase {
         d_Y = dalloc<double>((size_t)S * LD * LD * 2, st);
         d_coef = dalloc<double>((size_t)S * LD * LD * 2, st);
         d_cnt = dalloc<int>(CNT_STRIDE * (size_t)S, st);
+        d_rel = dalloc<double>((size_t)S * LD, st);
         BH_HIP_CHECK(hipHostMalloc((void **)&h_lam, sizeof(double) * S * LD, hipHostMallocDefault));
         BH_HIP_CHECK(hipHostMalloc((void **)&h_cnt, sizeof(int) * CNT_STRIDE * S, hipHostMallocDefault));
         int ng = opt.profile ? 1 : std::max(1, std::min(opt.groups, S));
@@ -313,6 +335,7 @@ class Engine : public EngineBase {
             G.part = dalloc<double>((size_t)2 * (2048 + 2 * gs) * LD * LD * 2, st);
             G.gramrb = dalloc<double>((size_t)gs * LD * LD * 2, st);
             BH_HIP_CHECK(hipHostMalloc((void **)&G.h_gramrb, sizeof(double) * gs * LD * LD * 2, hipHostMallocDefault));
+            BH_HIP_CHECK(hipHostMalloc((void **)&G.h_rel, sizeof(double) * gs * LD, hipHostMallocDefault));
             G.sweeps = dalloc<int>(6, st);
         }
         // twiddle tables W_n^t = exp(-2 pi i t / n), computed in FP64 on the host
@@ -345,6 +368,7 @@ class Engine : public EngineBase {
             (void)hipFree(G.gramrb);
             (void)hipFree(G.sweeps);
             (void)hipHostFree(G.h_gramrb);
+            (void)hipHostFree(G.h_rel);
             G.stg.release();
             for (auto &e : G.tev) {
                 (void)hipEventDestroy(e.first);
@@ -353,7 +377,7 @@ class Engine : public EngineBase {
             (void)hipEventDestroy(G.ev);
             if (G.own_st) (void)hipStreamDestroy(G.st);
         }
-        for (void *p : std::initializer_list<void *>{V, AV, H, RND, EPS, TEN, d_lam, d_Y, d_coef, d_cnt, twx, twy})
+        for (void *p : std::initializer_list<void *>{V, AV, H, RND, EPS, TEN, d_lam, d_Y, d_coef, d_cnt, d_rel, twx, twy})
             if (p) (void)hipFree(p);
         (void)hipHostFree(h_lam);
         (void)hipHostFree(h_cnt);
@@ -374,6 +398,7 @@ class Engine : public EngineBase {
     T *EPS{}, *TEN{};
     double *d_lam{}, *d_Y{}, *d_coef{};
     int *d_cnt{};
+    double *d_rel{};  // relative residual B-norms, LD per slot
     double *h_lam{};
     int *h_cnt{};
     cplx<T> *twx{}, *twy{};
@@ -431,18 +456,39 @@ class Engine : public EngineBase {
     void absorb(int s);
     void decide(int s);
     void finish_solve(int s, bool conv, int iters);
-    void record_k(int s, int kidx, const std::vector<double> &om, int iters, bool conv);
+    void record_k(int s, int kidx, const std::vector<double> &om, int iters, bool conv,
+                  const std::vector<double> &rel, int rel_iter);
+    void take_norms(int s);
     void set_kpoint(int s);
     void execute(StepPlan &plan, Group &G);
 };
 
 template <typename T>
-void Engine<T>::record_k(int s, int kidx, const std::vector<double> &om, int iters, bool conv) {
+void Engine<T>::record_k(int s, int kidx, const std::vector<double> &om, int iters, bool conv,
+                         const std::vector<double> &rel, int rel_iter) {
     Slot &sl = slots[s];
     JobResult &r = (*out_)[sl.job];
-    for (int b = 0; b < sl.nb; ++b) r.omegas[(size_t)kidx * sl.nb + b] = b < (int)om.size() ? om[b] : NAN;
+    for (int b = 0; b < sl.nb; ++b) {
+        r.omegas[(size_t)kidx * sl.nb + b] = b < (int)om.size() ? om[b] : NAN;
+        r.rel_residuals[(size_t)kidx * sl.nb + b] = b < (int)rel.size() ? rel[b] : NAN;
+    }
     r.iters[kidx] = iters;
     r.converged[kidx] = conv ? 1 : 0;
+    r.rel_iters[kidx] = rel_iter;
+}
+
+// The norms of the iteration in flight, from this group's readback.
+template <typename T>
+void Engine<T>::take_norms(int s) {
+    Slot &sl = slots[s];
+    if (!sl.rel_pending) return;
+    const double *h = group_of(s).h_rel + (size_t)(s - group_of(s).s0) * LD;
+    sl.rel_last.assign(std::min(sl.rel_m, sl.nb), 0.0);  // soft-locked bands: zero residual
+    for (int b : sl.rel_act)
+        if (b < (int)sl.rel_last.size()) sl.rel_last[b] = h[b];
+    sl.rel_last_iter = sl.rel_pending_iter;
+    sl.rel_pending = false;
+    sl.rel_final = false;
 }
 
 template <typename T>
@@ -538,6 +584,7 @@ template <typename T>
 void Engine<T>::absorb(int s) {
     Slot &sl = slots[s];
     if (sl.it == 0) return;
+    take_norms(s);
     const int *cnt = h_cnt + CNT_STRIDE * (size_t)s;
     int cx = cnt[0], cw = cnt[1];
     if (eopt.timed_kernel == TIMED_RR) stats_->timed_flops += rr_flops(sl.last_p, cx + cw, cnt[3]);
@@ -558,6 +605,19 @@ template <typename T>
 void Engine<T>::decide(int s) {
     Slot &sl = slots[s];
     const int it = sl.it;
+    // an exit at a log iteration: the reference computes that iteration's
+    // residual norms before its exit test, from the soft-locked set before any
+    // new locks; they are computed in the transition step
+    auto exit_norms = [&](const std::vector<char> &soft0) {
+        if (!should_log_iteration(it)) return;
+        sl.rel_final = sl.rel_pending = true;
+        sl.rel_pending_iter = it + 1;
+        sl.rel_m = sl.m;
+        sl.rel_soft = soft0;
+        sl.rel_act.clear();
+        for (int i = 0; i < sl.m; ++i)
+            if (!(i < (int)soft0.size() && soft0[i])) sl.rel_act.push_back(i);
+    };
     if (it >= sl.max_iter) {
         int nsoft = 0;
         for (char c : sl.soft) nsoft += c ? 1 : 0;
@@ -578,7 +638,15 @@ void Engine<T>::decide(int s) {
     for (char c : sl.soft) n_soft += c ? 1 : 0;
     const int n_hard = sl.gamma_defl ? 1 : 0;
     const int total = sl.exit_rule == 0 ? n_hard + n_soft + n_conv : n_soft;
-    if (total >= sl.nb) { finish_solve(s, true, it + 1); return; }
+    if (total >= sl.nb) {
+        exit_norms(sl.soft);
+        finish_solve(s, true, it + 1);
+        return;
+    }
+    const std::vector<char> soft_before = sl.soft;
+    // the reference forms this iteration's residuals (and deflates them)
+    // before it soft-locks new bands; only P drops the newly locked ones
+    sl.soft_iter = soft_before;
     if (it >= 2) {
         // soft_lock_bands with the 1e-4 degeneracy guard (eigensolver.rs:889-969)
         std::vector<int> want;
@@ -604,7 +672,11 @@ void Engine<T>::decide(int s) {
             for (int b : safe)
                 if (!sl.soft[b]) { sl.soft[b] = 1; ++newly; }
             n_soft += newly;
-            if (newly > 0 && n_soft >= sl.nb) { finish_solve(s, true, it + 1); return; }
+            if (newly > 0 && n_soft >= sl.nb) {
+                exit_norms(soft_before);
+                finish_solve(s, true, it + 1);
+                return;
+            }
         }
     }
     // continue iterating: the plan for this slot is built in execute()
@@ -660,16 +732,20 @@ void Engine<T>::transition_math(int s) {
         }
         sl.prev_om = om;
         sl.have_prev = true;
-        record_k(s, sl.k, om, sl.res_iters, sl.res_conv);
+        take_norms(s);
+        record_k(s, sl.k, om, sl.res_iters, sl.res_conv, sl.rel_last, sl.rel_last_iter);
         if (sl.reuse_gamma && sl.k == 0 && is_gamma_frac(J.kpath[0])) {
             sl.first_gamma = om;
             sl.first_gamma_iters = sl.res_iters;
+            sl.first_gamma_rel = sl.rel_last;
+            sl.first_gamma_rel_iter = sl.rel_last_iter;
             sl.have_first_gamma = true;
         }
         // closing Gamma reuse (bandstructure.rs:580-601)
         bool reuse = sl.reuse_gamma && sl.nk > 1 && is_gamma_frac(J.kpath[0]) && is_gamma_frac(J.kpath[sl.nk - 1]);
         if (knext == sl.nk - 1 && reuse && sl.have_first_gamma) {
-            record_k(s, knext, sl.first_gamma, sl.first_gamma_iters, true);
+            record_k(s, knext, sl.first_gamma, sl.first_gamma_iters, true, sl.first_gamma_rel,
+                     sl.first_gamma_rel_iter);
             knext = sl.nk;
         }
         next_solve = knext < sl.nk;
@@ -829,8 +905,12 @@ void Engine<T>::transition_math(int s) {
     sl.t2_pending = true;
     sl.it = 0;
     sl.m = m0;
+    sl.rel_last.clear();
+    sl.rel_last_iter = 0;
+    sl.rel_pending = sl.rel_final = false;
     sl.nW = 0;
     sl.soft.assign(m0, 0);
+    sl.soft_iter.assign(m0, 0);
     sl.lam.clear();
     sl.lam_prev.clear();
     sl.lam_prev_valid = false;
@@ -849,6 +929,8 @@ void Engine<T>::run(const std::vector<JobSpec> &jobs, std::vector<JobResult> &ou
         out[j].omegas.assign(J.kpath.size() * J.n_bands, NAN);
         out[j].iters.assign(J.kpath.size(), 0);
         out[j].converged.assign(J.kpath.size(), 0);
+        out[j].rel_residuals.assign(J.kpath.size() * J.n_bands, NAN);
+        out[j].rel_iters.assign(J.kpath.size(), 0);
     }
     for (auto &sl : slots) sl = Slot();
     size_t next_job = 0;
@@ -1030,6 +1112,7 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
 
     // ---- iteration for ITER slots ----
     std::vector<ResidualWork<T>> res;
+    std::vector<NormWork<T>> norms;  // residual B-norms on log iterations
     std::vector<DeflateWork<T>> defR, defP;
     std::vector<OpWork<T>> op_pre[2];  // [0] FD, [1] TP
     std::vector<OpWork<T>> op_p[2];    // Theta on the new directions [TM, TE]
@@ -1039,7 +1122,7 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
     std::vector<LincombWork<T>> lc_ritz;
     int max_ritz = 0;
     std::vector<unsigned char> maskbuf;
-    int max_a = 0, max_p = 0, max_d = 0;
+    int max_a = 0, max_p = 0, max_d = 0, max_r = 0;
     struct DefMask { int slot; int nd; int a; std::vector<unsigned char> m; };
     std::vector<DefMask> masksP;
     for (int s : plan.iter) {
@@ -1049,17 +1132,26 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
         for (int i = 0; i < m; ++i)
             if (!(i < (int)sl.soft.size() && sl.soft[i])) act.push_back(i);
         const int a = (int)act.size();
+        // residual columns: the active bands, then those soft-locked in this
+        // iteration's decision (their residuals are formed, deflated and
+        // normed as the reference's are, but not taken into P)
+        std::vector<int> rcols = act;
+        for (int i = 0; i < m; ++i)
+            if (i < (int)sl.soft.size() && sl.soft[i] && !(i < (int)sl.soft_iter.size() && sl.soft_iter[i]))
+                rcols.push_back(i);
+        const int ar = (int)rcols.size();
         const int w = sl.nW > 0 ? std::min(sl.nW, a) : 0;
         const int p = m + w + a;
-        if (p > C) throw std::runtime_error("subspace wider than the engine allows");
+        if (p > C || m + w + ar > C) throw std::runtime_error("subspace wider than the engine allows");
         max_a = std::max(max_a, a);
+        max_r = std::max(max_r, ar);
         max_p = std::max(max_p, p);
         // X at [0, m), W at [m, m + nW) as the last Ritz update wrote them; the
         // first w of W are used and residuals overwrite the rest
-        // residuals into P slots [m + w, m + w + a)
+        // residuals into P slots [m + w, m + w + ar)
         int *act_d;
-        int *act_h = stg.alloc<int>(std::max(a, 1), &act_d);
-        for (int i = 0; i < a; ++i) act_h[i] = act[i];
+        int *act_h = stg.alloc<int>(std::max(ar, 1), &act_d);
+        for (int i = 0; i < ar; ++i) act_h[i] = rcols[i];
         ResidualWork<T> rw{};
         rw.ax = avcol(s, 0);
         rw.bx = vcol(s, 0);
@@ -1068,9 +1160,32 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
         rw.act = act_d;
         rw.lam = d_lam + (size_t)s * LD;
         rw.stride = N;
-        rw.count = a;
+        rw.count = ar;
         res.push_back(rw);
-        // deflation set D = [y0 if Gamma] + soft-locked X columns
+        // residual deflation against D_old = [y0 if Gamma] + the bands soft-locked
+        // before this iteration (eigensolver.rs:1715-1722)
+        std::vector<int> rdcols;
+        if (sl.gamma_defl) rdcols.push_back(C);
+        for (int j = 0; j < (int)sl.soft_iter.size() && j < m; ++j)
+            if (sl.soft_iter[j]) rdcols.push_back(j);
+        const int nrd = (int)rdcols.size();
+        max_d = std::max(max_d, nrd);
+        if (nrd > 0 && ar > 0) {
+            const cplx<T> **dd_d;
+            const cplx<T> **dd = stg.alloc<const cplx<T> *>(nrd, (const cplx<T> ***)&dd_d);
+            for (int i = 0; i < nrd; ++i) dd[i] = vcol(s, rdcols[i]);
+            // r_c -= sum_i d_i <B d_i, r_c>, <B d, r> = sum conj(d) w r with w the B weight
+            DeflateWork<T> dw{};
+            dw.d_ptrs = dd_d;
+            dw.weight = bweight(s);
+            dw.R = vcol(s, m + w);
+            dw.stride = N;
+            dw.nd = nrd;
+            dw.a = ar;
+            dw.ld = LD;
+            defR.push_back(dw);
+        }
+        // direction deflation against D_new = [y0 if Gamma] + every soft-locked band
         std::vector<int> dcols;
         if (sl.gamma_defl) dcols.push_back(C);
         for (int j = 0; j < (int)sl.soft.size() && j < m; ++j)
@@ -1081,7 +1196,6 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
             const cplx<T> **dd_d;
             const cplx<T> **dd = stg.alloc<const cplx<T> *>(nd, (const cplx<T> ***)&dd_d);
             for (int i = 0; i < nd; ++i) dd[i] = vcol(s, dcols[i]);
-            // r_c -= sum_i d_i <B d_i, r_c>, <B d, r> = sum conj(d) w r with w the B weight
             DeflateWork<T> dw{};
             dw.d_ptrs = dd_d;
             dw.weight = bweight(s);
@@ -1090,8 +1204,7 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
             dw.nd = nd;
             dw.a = a;
             dw.ld = LD;
-            defR.push_back(dw);
-            // P deflation: same D; soft part skipped for compacted P index c with soft[c]
+            // soft part skipped for compacted P index c with soft[c]
             defP.push_back(dw);
             DefMask dm{s, nd, a, std::vector<unsigned char>((size_t)LD * LD, 1)};
             for (int c = 0; c < a; ++c)
@@ -1099,6 +1212,21 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
                     for (int i = 0; i < nd; ++i)
                         if (dcols[i] != C) dm.m[i + (size_t)c * LD] = 0;
             masksP.push_back(std::move(dm));
+        }
+        if (should_log_iteration(sl.it)) {  // eigensolver.rs:1729-1737
+            NormWork<T> nw{};
+            nw.r = vcol(s, m + w);
+            nw.bw = bweight(s);
+            nw.act = act_d;
+            nw.lam = d_lam + (size_t)s * LD;
+            nw.rel = d_rel + (size_t)s * LD;
+            nw.stride = N;
+            nw.count = ar;
+            norms.push_back(nw);
+            sl.rel_pending = true;
+            sl.rel_pending_iter = sl.it + 1;
+            sl.rel_act = rcols;
+            sl.rel_m = m;
         }
         // preconditioner (P in place)
         if (sl.precond == 1 || sl.precond == 2) {
@@ -1176,6 +1304,55 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
     std::vector<int> t1_slots;
     for (int s : plan.t1) {
         Slot &sl = slots[s];
+        if (sl.rel_final) {
+            // the exit iteration's residuals R = A X - B X lambda of the
+            // non-soft-locked bands into the free columns [m, m + a), deflated
+            // against y0 and the soft-locked bands, then their B-norms
+            const int m = sl.rel_m, a = (int)sl.rel_act.size();
+            int *act_d;
+            int *act_h = stg.alloc<int>(std::max(a, 1), &act_d);
+            for (int i = 0; i < a; ++i) act_h[i] = sl.rel_act[i];
+            ResidualWork<T> rw{};
+            rw.ax = avcol(s, 0);
+            rw.bx = vcol(s, 0);
+            rw.bw = bweight(s);
+            rw.out = vcol(s, m);
+            rw.act = act_d;
+            rw.lam = d_lam + (size_t)s * LD;
+            rw.stride = N;
+            rw.count = a;
+            res.push_back(rw);
+            max_r = std::max(max_r, a);
+            std::vector<int> dcols;
+            if (sl.gamma_defl) dcols.push_back(C);
+            for (int j = 0; j < (int)sl.rel_soft.size() && j < m; ++j)
+                if (sl.rel_soft[j]) dcols.push_back(j);
+            const int nd = (int)dcols.size();
+            if (nd > 0 && a > 0) {
+                const cplx<T> **dd_d;
+                const cplx<T> **dd = stg.alloc<const cplx<T> *>(nd, (const cplx<T> ***)&dd_d);
+                for (int i = 0; i < nd; ++i) dd[i] = vcol(s, dcols[i]);
+                DeflateWork<T> dw{};
+                dw.d_ptrs = dd_d;
+                dw.weight = bweight(s);
+                dw.R = vcol(s, m);
+                dw.stride = N;
+                dw.nd = nd;
+                dw.a = a;
+                dw.ld = LD;
+                defR.push_back(dw);
+                max_d = std::max(max_d, nd);
+            }
+            NormWork<T> nw{};
+            nw.r = vcol(s, m);
+            nw.bw = bweight(s);
+            nw.act = act_d;
+            nw.lam = d_lam + (size_t)s * LD;
+            nw.rel = d_rel + (size_t)s * LD;
+            nw.stride = N;
+            nw.count = a;
+            norms.push_back(nw);
+        }
         const int kd = sl.kd;
         const cplx<T> **kp_d;
         const cplx<T> **kp = stg.alloc<const cplx<T> *>(kd, (const cplx<T> ***)&kp_d);
@@ -1237,6 +1414,7 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
     auto *d_rq = put(rq);
     auto *d_res = put(res);
     auto *d_defR = put(defR);
+    auto *d_norms = put(norms);
     OpWork<T> *d_op_pre[2] = {put(op_pre[0]), put(op_pre[1])};
     auto *d_defP = put(defP);
     OpWork<T> *d_op_p[2] = {put(op_p[0]), put(op_p[1])};
@@ -1287,7 +1465,7 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
     }
     launch_rayleigh<T>(d_rq, (int)rq.size(), m0max, N, st);
     // iteration
-    launch_residual<T>(d_res, (int)res.size(), std::max(max_a, 1), N, st);
+    launch_residual<T>(d_res, (int)res.size(), std::max(max_r, 1), N, st);
     // algorithmic: each column read and written once, each vector and weight read once
     auto defl_bytes = [&](const std::vector<DeflateWork<T>> &v) {
         double b = 0;
@@ -1301,7 +1479,8 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
     };
     if (!defR.empty())
         timed(TIMED_DEFL, defl_bytes(defR), defl_flops(defR),
-              [&] { launch_deflate<T>(d_defR, (int)defR.size(), N, max_a, max_d, st); });
+              [&] { launch_deflate<T>(d_defR, (int)defR.size(), N, max_r, max_d, st); });
+    launch_bnorm<T>(d_norms, (int)norms.size(), std::max(max_r, 1), N, st);
     const bool prec_fp32 = eopt.precision == 2;
     launch_op<T>(MODE_PREC_FD, nx, ny, d_op_pre[0], (int)op_pre[0].size(), std::max(max_a, 1), twx, twy, st,
                  prec_fp32);
@@ -1354,6 +1533,11 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
         launch_gram_reduce(d_part, d_gramrb, (int)g_t1.size(), nch_t1, LD, (long)nch_t1 * LD * LD * 2,
                            (long)LD * LD * 2, st);
         BH_HIP_CHECK(hipMemcpyAsync(G.h_gramrb, d_gramrb, sizeof(double) * g_t1.size() * LD * LD * 2,
+                                    hipMemcpyDeviceToHost, st));
+    }
+    if (!norms.empty()) {
+        const int gs = G.s1 - G.s0;
+        BH_HIP_CHECK(hipMemcpyAsync(G.h_rel, d_rel + (size_t)G.s0 * LD, sizeof(double) * gs * LD,
                                     hipMemcpyDeviceToHost, st));
     }
     if (!plan.iter.empty()) {

@@ -222,6 +222,17 @@ int bh_operator_diagnostics(bh_context *ctx, const bh_job *jobs, int njobs, int 
 int bh_bandstructure_run(bh_context *ctx, const bh_job *jobs, int njobs, const bh_run_options *opts,
                          double *omegas, int *iters, int *converged, bh_run_stats *stats);
 
+/* bh_bandstructure_run plus the residual diagnostics the reference's solve()
+ * computes on its log iterations (1-5, 10, 20, 50, 100, then every 50;
+ * eigensolver.rs:251-259,1729-1737): per k-point, the relative residual
+ * B-norms ||r_i||_B / max(|lambda_i|, 1e-10) of the first n_bands block
+ * entries (eigensolver.rs:744-779; block order, soft-locked bands 0) at the
+ * solve's last log iteration, at the same offsets as omegas, and that
+ * iteration (1-based) at the same offsets as iters.  Either may be NULL. */
+int bh_bandstructure_run_ex(bh_context *ctx, const bh_job *jobs, int njobs, const bh_run_options *opts,
+                            double *omegas, int *iters, int *converged, double *rel_residuals,
+                            int *rel_residual_iters, bh_run_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif
