@@ -7,6 +7,8 @@
 namespace bh {
 
 // r_c = A x_{act[c]} - lambda_{act[c]} B x_{act[c]}   (eigensolver.rs:707-738)
+// With rel set, also the relative residual B-norm of each column
+// (eigensolver.rs:744-779): rel[act[c]] = sqrt(Re r_c^H B r_c) / max(|lambda|, 1e-10)
 template <typename T>
 struct ResidualWork {
     const cplx<T> *ax;
@@ -15,6 +17,7 @@ struct ResidualWork {
     cplx<T> *out;
     const int *act;       // band index of each active column
     const double *lam;
+    double *rel;          // optional: residual norms, indexed by band
     long stride;          // column stride for ax, bx, out
     int count, pad;
 };
@@ -31,23 +34,8 @@ struct RayleighWork {
     int count, pad;
 };
 
-// Relative residual B-norms (eigensolver.rs:744-779):
-//   rel[act[c]] = sqrt(max(Re r_c^H B r_c, 0)) / max(|lambda_{act[c]}|, 1e-10)
-template <typename T>
-struct NormWork {
-    const cplx<T> *r;     // count columns, stride apart
-    const T *bw;          // optional B weight (TM: eps); null: B = I
-    const int *act;       // band index of each column
-    const double *lam;
-    double *rel;          // indexed by band
-    long stride;
-    int count, pad;
-};
-
 template <typename T>
 void launch_residual(const ResidualWork<T> *dwork, int nwork, int maxcount, long rows, hipStream_t st);
-template <typename T>
-void launch_bnorm(const NormWork<T> *dwork, int nwork, int maxcount, long rows, hipStream_t st);
 template <typename T>
 void launch_rayleigh(const RayleighWork<T> *dwork, int nwork, int maxcount, long rows, hipStream_t st);
 // xorshift start vectors, phase = first_phase + column (initialization.rs:113-125)

@@ -4,64 +4,44 @@
 
 namespace bh {
 
+// One workgroup per (column, work item), so that the column's B-norm is
+// reduced inside the workgroup (no atomics; independent of launch order).
 template <typename T>
-__global__ void residual_kernel(const ResidualWork<T> *__restrict__ work, long rows) {
-    const ResidualWork<T> &w = work[blockIdx.z];
-    const int c = blockIdx.y;
+__global__ void __launch_bounds__(256) residual_kernel(const ResidualWork<T> *__restrict__ work, long rows) {
+    const ResidualWork<T> &w = work[blockIdx.y];
+    const int c = blockIdx.x;
     if (c >= w.count) return;
-    const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= rows) return;
     const int b = w.act[c];
     const double lam = w.lam[b];
-    cplx<T> a = w.ax[b * w.stride + r], m = w.bx[b * w.stride + r];
-    if (w.bw) {
-        const T s = w.bw[r];
-        m = {m.x * s, m.y * s};
-    }
     // backend axpy with alpha = (-lambda, 0) in storage precision
-    T al = (T)(-lam);
-    w.out[c * w.stride + r] = {a.x + al * m.x, a.y + al * m.y};
+    const T al = (T)(-lam);
+    const cplx<T> *ax = w.ax + (size_t)b * w.stride, *bx = w.bx + (size_t)b * w.stride;
+    cplx<T> *out = w.out + (size_t)c * w.stride;
+    double acc = 0.0;
+#pragma unroll 4
+    for (long r = threadIdx.x; r < rows; r += 256) {
+        const cplx<T> a = gload(ax, r);
+        cplx<T> m = gload(bx, r);
+        const T s = w.bw ? gmem(w.bw)[r] : (T)1;
+        if (w.bw) m = {m.x * s, m.y * s};
+        const cplx<T> o = {a.x + al * m.x, a.y + al * m.y};
+        gstore(out, r, o);
+        // Re(conj(r) (B r)), B r in storage precision as a mass apply forms it
+        acc += (double)o.x * (double)(o.x * s) + (double)o.y * (double)(o.y * s);
+    }
+    if (!w.rel) return;
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+    __shared__ double part[4];
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        w.rel[b] = sqrt(fmax(part[0] + part[1] + part[2] + part[3], 0.0)) / fmax(fabs(lam), 1e-10);
 }
 
 template <typename T>
 void launch_residual(const ResidualWork<T> *dwork, int nwork, int maxcount, long rows, hipStream_t st) {
     if (nwork <= 0 || maxcount <= 0) return;
-    hipLaunchKernelGGL((residual_kernel<T>), dim3((rows + 255) / 256, maxcount, nwork), dim3(256), 0, st, dwork,
-                       rows);
-    BH_HIP_CHECK(hipGetLastError());
-}
-
-// one workgroup per (column, work item); the column's sum is reduced in the
-// workgroup, so the result does not depend on launch order
-template <typename T>
-__global__ void __launch_bounds__(256) bnorm_kernel(const NormWork<T> *__restrict__ work, long rows) {
-    const NormWork<T> &w = work[blockIdx.y];
-    const int c = blockIdx.x;
-    if (c >= w.count) return;
-    const cplx<T> *r = w.r + c * w.stride;
-    double acc = 0.0;
-    for (long i = threadIdx.x; i < rows; i += 256) {
-        const cplx<T> v = gload(r, i);
-        const double s = w.bw ? (double)gmem(w.bw)[i] : 1.0;
-        // Re(conj(r) (B r)) with B r formed in storage precision, as a mass apply would
-        const double bx = (double)((T)(v.x * (T)s)), by = (double)((T)(v.y * (T)s));
-        acc += (double)v.x * bx + (double)v.y * by;
-    }
-    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
-    __shared__ double part[4];
-    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const double sq = part[0] + part[1] + part[2] + part[3];
-        const int b = w.act[c];
-        w.rel[b] = sqrt(fmax(sq, 0.0)) / fmax(fabs(w.lam[b]), 1e-10);
-    }
-}
-
-template <typename T>
-void launch_bnorm(const NormWork<T> *dwork, int nwork, int maxcount, long rows, hipStream_t st) {
-    if (nwork <= 0 || maxcount <= 0) return;
-    hipLaunchKernelGGL((bnorm_kernel<T>), dim3(maxcount, nwork), dim3(256), 0, st, dwork, rows);
+    hipLaunchKernelGGL((residual_kernel<T>), dim3(maxcount, nwork), dim3(256), 0, st, dwork, rows);
     BH_HIP_CHECK(hipGetLastError());
 }
 
@@ -197,7 +177,6 @@ void launch_dot(const cplx<T> *x, const cplx<T> *y, long n, double *out2, hipStr
 #define BH_INST(T)                                                                                         \
     template void launch_residual<T>(const ResidualWork<T> *, int, int, long, hipStream_t);                 \
     template void launch_rayleigh<T>(const RayleighWork<T> *, int, int, long, hipStream_t);                 \
-    template void launch_bnorm<T>(const NormWork<T> *, int, int, long, hipStream_t);                        \
     template void launch_random<T>(cplx<T> *, int, long, double, hipStream_t);                              \
     template void launch_fill<T>(cplx<T> *, cplx<T> *, const T *, double, double, long, hipStream_t);       \
     template void launch_scale<T>(cplx<T> *, long, double, double, hipStream_t);                            \

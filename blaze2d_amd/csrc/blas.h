@@ -110,6 +110,11 @@ struct DeflateWork {
     const T *weight;               // optional B weight per row (TM: eps)
     cplx<T> *R;                    // a columns, stride apart, updated in place
     const unsigned char *mask;     // optional, col-major ld x ld
+    // optional relative residual B-norms of the deflated columns
+    // (eigensolver.rs:744-779): rel[act[c]] = ||r_c||_B / max(|lam[act[c]]|, 1e-10)
+    const int *act;
+    const double *lam;
+    double *rel;
     long stride;
     int nd, a, ld, pad;
 };

@@ -312,6 +312,31 @@ __global__ void __launch_bounds__(DEFL_NT) deflate_kernel(const DeflateWork<T> *
             if (i < nrows) gstore(r, i, cplx<T>{(T)ur[c][k], (T)ui[c][k]});
         }
     }
+    if (!w.rel) return;
+    // B-norms of the stored (storage-precision) deflated columns
+    double nsq[DEFL_CPW];
+#pragma unroll
+    for (int c = 0; c < DEFL_CPW; ++c) {
+        nsq[c] = 0.0;
+#pragma unroll
+        for (int k = 0; k < DEFL_RPT; ++k) {
+            const T xr = (T)ur[c][k], xi = (T)ui[c][k], sw = (T)ws[k];
+            nsq[c] += (double)xr * (double)(xr * sw) + (double)xi * (double)(xi * sw);
+        }
+        for (int off = 32; off > 0; off >>= 1) nsq[c] += __shfl_xor(nsq[c], off);
+    }
+    __syncthreads();  // red is reused
+    if (lane == 0)
+#pragma unroll
+        for (int c = 0; c < DEFL_CPW; ++c) red[wv][0][c][0] = nsq[c];
+    __syncthreads();
+    if (tid < nc) {
+        double t = 0.0;
+#pragma unroll
+        for (int v = 0; v < NW; ++v) t += red[v][0][tid][0];
+        const int band = w.act[c0 + tid];
+        w.rel[band] = sqrt(fmax(t, 0.0)) / fmax(fabs(w.lam[band]), 1e-10);
+    }
 }
 
 template <typename T>

@@ -1112,7 +1112,7 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
 
     // ---- iteration for ITER slots ----
     std::vector<ResidualWork<T>> res;
-    std::vector<NormWork<T>> norms;  // residual B-norms on log iterations
+    bool any_norms = false;  // residual B-norms on log iterations, read back after the step
     std::vector<DeflateWork<T>> defR, defP;
     std::vector<OpWork<T>> op_pre[2];  // [0] FD, [1] TP
     std::vector<OpWork<T>> op_p[2];    // Theta on the new directions [TM, TE]
@@ -1161,7 +1161,9 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
         rw.lam = d_lam + (size_t)s * LD;
         rw.stride = N;
         rw.count = ar;
-        res.push_back(rw);
+        // the reference's residual B-norms on log iterations (eigensolver.rs:1729-1737),
+        // taken by the last kernel that writes the residuals
+        const bool log_it = should_log_iteration(sl.it);
         // residual deflation against D_old = [y0 if Gamma] + the bands soft-locked
         // before this iteration (eigensolver.rs:1715-1722)
         std::vector<int> rdcols;
@@ -1183,8 +1185,16 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
             dw.nd = nrd;
             dw.a = ar;
             dw.ld = LD;
+            if (log_it) {
+                dw.act = act_d;
+                dw.lam = d_lam + (size_t)s * LD;
+                dw.rel = d_rel + (size_t)s * LD;
+            }
             defR.push_back(dw);
+        } else if (log_it) {
+            rw.rel = d_rel + (size_t)s * LD;
         }
+        res.push_back(rw);
         // direction deflation against D_new = [y0 if Gamma] + every soft-locked band
         std::vector<int> dcols;
         if (sl.gamma_defl) dcols.push_back(C);
@@ -1213,16 +1223,8 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
                         if (dcols[i] != C) dm.m[i + (size_t)c * LD] = 0;
             masksP.push_back(std::move(dm));
         }
-        if (should_log_iteration(sl.it)) {  // eigensolver.rs:1729-1737
-            NormWork<T> nw{};
-            nw.r = vcol(s, m + w);
-            nw.bw = bweight(s);
-            nw.act = act_d;
-            nw.lam = d_lam + (size_t)s * LD;
-            nw.rel = d_rel + (size_t)s * LD;
-            nw.stride = N;
-            nw.count = ar;
-            norms.push_back(nw);
+        if (log_it) {
+            any_norms = true;
             sl.rel_pending = true;
             sl.rel_pending_iter = sl.it + 1;
             sl.rel_act = rcols;
@@ -1321,8 +1323,8 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
             rw.lam = d_lam + (size_t)s * LD;
             rw.stride = N;
             rw.count = a;
-            res.push_back(rw);
             max_r = std::max(max_r, a);
+            any_norms = true;
             std::vector<int> dcols;
             if (sl.gamma_defl) dcols.push_back(C);
             for (int j = 0; j < (int)sl.rel_soft.size() && j < m; ++j)
@@ -1340,18 +1342,15 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
                 dw.nd = nd;
                 dw.a = a;
                 dw.ld = LD;
+                dw.act = act_d;
+                dw.lam = d_lam + (size_t)s * LD;
+                dw.rel = d_rel + (size_t)s * LD;
                 defR.push_back(dw);
                 max_d = std::max(max_d, nd);
+            } else {
+                rw.rel = d_rel + (size_t)s * LD;
             }
-            NormWork<T> nw{};
-            nw.r = vcol(s, m);
-            nw.bw = bweight(s);
-            nw.act = act_d;
-            nw.lam = d_lam + (size_t)s * LD;
-            nw.rel = d_rel + (size_t)s * LD;
-            nw.stride = N;
-            nw.count = a;
-            norms.push_back(nw);
+            res.push_back(rw);
         }
         const int kd = sl.kd;
         const cplx<T> **kp_d;
@@ -1414,7 +1413,6 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
     auto *d_rq = put(rq);
     auto *d_res = put(res);
     auto *d_defR = put(defR);
-    auto *d_norms = put(norms);
     OpWork<T> *d_op_pre[2] = {put(op_pre[0]), put(op_pre[1])};
     auto *d_defP = put(defP);
     OpWork<T> *d_op_p[2] = {put(op_p[0]), put(op_p[1])};
@@ -1480,7 +1478,6 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
     if (!defR.empty())
         timed(TIMED_DEFL, defl_bytes(defR), defl_flops(defR),
               [&] { launch_deflate<T>(d_defR, (int)defR.size(), N, max_r, max_d, st); });
-    launch_bnorm<T>(d_norms, (int)norms.size(), std::max(max_r, 1), N, st);
     const bool prec_fp32 = eopt.precision == 2;
     launch_op<T>(MODE_PREC_FD, nx, ny, d_op_pre[0], (int)op_pre[0].size(), std::max(max_a, 1), twx, twy, st,
                  prec_fp32);
@@ -1535,7 +1532,7 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
         BH_HIP_CHECK(hipMemcpyAsync(G.h_gramrb, d_gramrb, sizeof(double) * g_t1.size() * LD * LD * 2,
                                     hipMemcpyDeviceToHost, st));
     }
-    if (!norms.empty()) {
+    if (any_norms) {
         const int gs = G.s1 - G.s0;
         BH_HIP_CHECK(hipMemcpyAsync(G.h_rel, d_rel + (size_t)G.s0 * LD, sizeof(double) * gs * LD,
                                     hipMemcpyDeviceToHost, st));
