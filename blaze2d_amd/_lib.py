@@ -109,7 +109,7 @@ EXPORTS = [
     "bh_axpy", "bh_dot", "bh_gram", "bh_lincomb", "bh_theta_create", "bh_theta_destroy", "bh_theta_apply",
     "bh_mass_apply", "bh_precond_apply", "bh_dielectric", "bh_dielectric_batch", "bh_bandstructure_run",
     "bh_hermitian_eigh", "bh_operator_diagnostics", "bh_rayleigh_ritz",
-    "bh_bandstructure_run_ex",
+    "bh_bandstructure_run_ex", "bh_gram_noalloc",
 ]
 
 _lib = None
@@ -141,6 +141,7 @@ def lib() -> ctypes.CDLL:
     L.bh_axpy.argtypes = [vp, ctypes.c_double, ctypes.c_double, vp, ctypes.c_int, vp, ctypes.c_int]
     L.bh_dot.argtypes = [vp, vp, ctypes.c_int, vp, ctypes.c_int, c_dp]
     L.bh_gram.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, vp, ctypes.c_int, ctypes.c_int, c_dp]
+    L.bh_gram_noalloc.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, vp, ctypes.c_int, ctypes.c_int, c_dp]
     L.bh_hermitian_eigh.argtypes = [vp, ctypes.c_int, ctypes.c_int, c_dp, c_dp, c_dp]
     L.bh_rayleigh_ritz.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, c_dp, c_dp, c_dp, c_dp, c_ip]
     L.bh_lincomb.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, c_dp, ctypes.c_int, vp, ctypes.c_int]

@@ -28,7 +28,29 @@ struct bh_context {
     std::vector<std::pair<int, void *>> tw64, tw32;
     void *scratch = nullptr;
     size_t scratch_bytes = 0;
+    // device address of the file-scope allocation-free scratch array on this
+    // context's device (set once at context creation)
+    char *noalloc_scratch = nullptr;
 };
+
+// Scratch of bh_gram_noalloc, the Gram entry point that allocates nothing:
+// the work descriptor, the per-chunk partial sums (at most 64 chunks of a
+// 48 x 48 complex block), their reduction and the packed p x q result.  One
+// copy per device, held until the process ends; every call on a device shares
+// it, so calls that overlap on one device race.
+constexpr int NOALLOC_CHUNKS = 64;
+constexpr size_t NOALLOC_BLOCK = sizeof(double) * 2 * GRAM_MAXC * GRAM_MAXC;
+constexpr size_t NOALLOC_SCRATCH_BYTES = 256 + (NOALLOC_CHUNKS + 2) * NOALLOC_BLOCK;  // 2.4 MB
+static __device__ __align__(256) char scratch[NOALLOC_SCRATCH_BYTES];
+
+// out[i q + j] = red[i + j ld] (row-major p x q complex from the col-major reduction)
+__global__ void gram_pack_kernel(const double *__restrict__ red, int ld, int p, int q, double *__restrict__ out) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= p * q) return;
+    const int i = e / q, j = e - i * q;
+    out[2 * e] = red[2 * (i + (size_t)j * ld)];
+    out[2 * e + 1] = red[2 * (i + (size_t)j * ld) + 1];
+}
 
 struct bh_buffer {
     bh_context *ctx;
@@ -77,7 +99,7 @@ const cplx<T> *twiddles(bh_context *ctx, int n) {
     return (const cplx<T> *)d;
 }
 
-void *scratch(bh_context *ctx, size_t bytes) {
+void *ctx_scratch(bh_context *ctx, size_t bytes) {
     if (bytes > ctx->scratch_bytes) {
         if (ctx->scratch) BH_HIP_CHECK(hipFree(ctx->scratch));
         BH_HIP_CHECK(hipMalloc(&ctx->scratch, bytes));
@@ -99,7 +121,7 @@ void run_op(bh_context *ctx, int mode, int nx, int ny, const void *in, void *out
     w.in_stride = (long)nx * ny;
     w.out_stride = (long)nx * ny;
     w.count = count;
-    OpWork<T> *dw = (OpWork<T> *)scratch(ctx, sizeof(OpWork<T>));
+    OpWork<T> *dw = (OpWork<T> *)ctx_scratch(ctx, sizeof(OpWork<T>));
     BH_HIP_CHECK(hipMemcpyAsync(dw, &w, sizeof(w), hipMemcpyHostToDevice, ctx->st));
     launch_op<T>(mode, nx, ny, dw, 1, count, twiddles<T>(ctx, nx), twiddles<T>(ctx, ny), ctx->st);
     BH_HIP_CHECK(hipStreamSynchronize(ctx->st));
@@ -111,7 +133,7 @@ void run_gram(bh_context *ctx, const bh_buffer *x, int xi0, int p, const bh_buff
     const long n = (long)x->n();
     const int nch = (int)std::min<long>(64, std::max<long>(1, n / 64));
     const int ld = GRAM_MAXC;
-    char *base = (char *)scratch(ctx, sizeof(GramWork<T>) + 256 + sizeof(double) * 2 * ld * ld * (nch + 1));
+    char *base = (char *)ctx_scratch(ctx, sizeof(GramWork<T>) + 256 + sizeof(double) * 2 * ld * ld * (nch + 1));
     GramWork<T> w{};
     w.A = (const cplx<T> *)x->col(xi0);
     w.B = (const cplx<T> *)y->col(yi0);
@@ -140,7 +162,7 @@ template <typename T>
 void run_lincomb(bh_context *ctx, const bh_buffer *qb, int qi0, int r, const double *coeffs, int m, bh_buffer *out,
                  int oi0) {
     const long n = (long)qb->n();
-    char *base = (char *)scratch(ctx, 256 + sizeof(double) * 2 * (size_t)r * m);
+    char *base = (char *)ctx_scratch(ctx, 256 + sizeof(double) * 2 * (size_t)r * m);
     LincombWork<T> w{};
     w.A = (const cplx<T> *)qb->col(qi0);
     w.a_stride = n;
@@ -192,6 +214,17 @@ int bh_context_create(int device, bh_context **out) {
     c->device = device;
     BH_HIP_CHECK(hipStreamCreate(&c->st));
     for (auto &g : c->gst) BH_HIP_CHECK(hipStreamCreate(&g));
+    // the allocation-free entry points use the device address of `scratch` on
+    // this device (the host-side name is only a stand-in)
+    void *sp = nullptr;
+    const hipError_t se = hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch));
+    if (se != hipSuccess) {
+        for (auto &g : c->gst) (void)hipStreamDestroy(g);
+        (void)hipStreamDestroy(c->st);
+        delete c;
+        return fail(std::string("bh_context_create: scratch symbol address: ") + hipGetErrorString(se));
+    }
+    c->noalloc_scratch = (char *)sp;
     *out = c;
     return 0;
     BH_API_CATCH
@@ -308,7 +341,7 @@ int bh_dot(bh_context *ctx, const bh_buffer *x, int ix, const bh_buffer *y, int 
     BH_API_TRY
     if (!check_idx(x, ix, 1) || !check_idx(y, iy, 1) || x->n() != y->n() || x->precision != y->precision || !out)
         return fail("bh_dot: bad arguments");
-    double *d = (double *)scratch(ctx, 16);
+    double *d = (double *)ctx_scratch(ctx, 16);
     if (x->precision == 0)
         launch_dot<double>((const cplx<double> *)x->col(ix), (const cplx<double> *)y->col(iy), (long)x->n(), d, ctx->st);
     else
@@ -327,6 +360,51 @@ int bh_gram(bh_context *ctx, const bh_buffer *x, int xi0, int p, const bh_buffer
     if (p == 0 || q == 0) return 0;
     if (x->precision == 0) run_gram<double>(ctx, x, xi0, p, y, yi0, q, out);
     else run_gram<float>(ctx, x, xi0, p, y, yi0, q, out);
+    return 0;
+    BH_API_CATCH
+}
+
+int bh_gram_noalloc(bh_context *ctx, const bh_buffer *x, int xi0, int p, const bh_buffer *y, int yi0, int q,
+                    double *out) {
+    BH_API_TRY
+    if (!ctx || !check_idx(x, xi0, p) || !check_idx(y, yi0, q) || x->n() != y->n() || x->precision != y->precision ||
+        !out)
+        return fail("bh_gram_noalloc: bad arguments");
+    if (p > GRAM_MAXC || q > GRAM_MAXC) return fail("bh_gram_noalloc: at most 48 columns per operand");
+    if (!ctx->noalloc_scratch) return fail("bh_gram_noalloc: context has no scratch address");
+    if (p == 0 || q == 0) return 0;
+    const long n = (long)x->n();
+    const int nch = (int)std::min<long>(NOALLOC_CHUNKS, std::max<long>(1, n / 64));  // as bh_gram
+    const int ld = GRAM_MAXC;
+    char *base = ctx->noalloc_scratch;
+    double *part = (double *)(base + 256);
+    double *red = (double *)((char *)part + (size_t)nch * NOALLOC_BLOCK);
+    double *packed = (double *)((char *)red + NOALLOC_BLOCK);
+    if ((size_t)((char *)packed - base) + sizeof(double) * 2 * p * q > NOALLOC_SCRATCH_BYTES)
+        return fail("bh_gram_noalloc: problem exceeds the 2.4 MB scratch");
+    // every partial entry read below (i < p, j < q of each chunk) is written by
+    // the Gram kernel, so the shared scratch needs no clearing between calls
+    auto go = [&](auto tag) {
+        using T = decltype(tag);
+        GramWork<T> w{};
+        w.A = (const cplx<T> *)x->col(xi0);
+        w.B = (const cplx<T> *)y->col(yi0);
+        w.a_stride = n;
+        w.b_stride = n;
+        w.na = p;
+        w.nb = q;
+        w.ld = ld;
+        w.partial = part;
+        BH_HIP_CHECK(hipMemcpyAsync(base, &w, sizeof(w), hipMemcpyHostToDevice, ctx->st));
+        launch_gram<T>((const GramWork<T> *)base, 1, nch, n, p, q, ctx->st);
+    };
+    if (x->precision == 0) go(double());
+    else go(float());
+    launch_gram_reduce(part, red, 1, nch, ld, 0, 0, ctx->st);
+    hipLaunchKernelGGL(gram_pack_kernel, dim3((p * q + 255) / 256), dim3(256), 0, ctx->st, red, ld, p, q, packed);
+    BH_HIP_CHECK(hipGetLastError());
+    BH_HIP_CHECK(hipMemcpyAsync(out, packed, sizeof(double) * 2 * p * q, hipMemcpyDeviceToHost, ctx->st));
+    BH_HIP_CHECK(hipStreamSynchronize(ctx->st));
     return 0;
     BH_API_CATCH
 }
@@ -353,7 +431,7 @@ int bh_hermitian_eigh(bh_context *ctx, int n, int count, const double *a, double
     const size_t nn = (size_t)n * n;
     // device layout: [work items][partials (row-major G(i,j))][eigenvalues][eigenvectors]
     size_t bytes = 256 + count * sizeof(EighWork) + count * (2 * nn + n + 2 * nn) * sizeof(double);
-    char *base = (char *)scratch(ctx, bytes);
+    char *base = (char *)ctx_scratch(ctx, bytes);
     EighWork *dw = (EighWork *)(base + 256);
     double *dpart = (double *)(dw + count);
     double *dwv = dpart + 2 * nn * count;
@@ -401,7 +479,7 @@ int bh_rayleigh_ritz(bh_context *ctx, int n, int count, int nsplit, const double
     // partial row-major G(i, j) with one chunk
     const size_t wbytes = ((count * sizeof(RRWork) + 255) / 256) * 256;
     const size_t bytes = wbytes + sizeof(double) * count * (2 * nn * 3 + n) + sizeof(int) * 8 * count;
-    char *base = (char *)scratch(ctx, bytes);
+    char *base = (char *)ctx_scratch(ctx, bytes);
     RRWork *dw = (RRWork *)base;
     double *dp1 = (double *)(base + wbytes);
     double *dp2 = dp1 + 2 * nn * count;
@@ -557,7 +635,7 @@ int bh_mass_apply(bh_theta *op, const bh_buffer *in, int ii0, bh_buffer *out, in
                                         op->ctx->st));
             continue;
         }
-        char *base = (char *)scratch(op->ctx, 512);
+        char *base = (char *)ctx_scratch(op->ctx, 512);
         auto go = [&](auto tag) {
             using T = decltype(tag);
             LincombWork<T> w{};

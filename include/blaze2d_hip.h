@@ -65,6 +65,15 @@ int bh_dot(bh_context *ctx, const bh_buffer *x, int ix, const bh_buffer *y, int 
  * (backend.rs:105-128). */
 int bh_gram(bh_context *ctx, const bh_buffer *x, int xi0, int p, const bh_buffer *y, int yi0, int q,
             double *out);
+/* bh_gram with no allocation of any kind: no counterpart in the reference;
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, limits (p, q <= 48) and result as bh_gram, bit for bit.  Its
+ * partial sums live in a fixed 2.4 MB device array per GPU, set up by
+ * bh_context_create and held until the process ends; calls that overlap on
+ * one GPU share that array and race. */
+int bh_gram_noalloc(bh_context *ctx, const bh_buffer *x, int xi0, int p, const bh_buffer *y, int yi0, int q,
+                    double *out);
 /* linear_combinations: out_j = sum_i q_{qi0+i} c[i + j*r], c col-major r x m
  * (backend.rs:130-150). */
 int bh_lincomb(bh_context *ctx, const bh_buffer *q, int qi0, int r, const double *coeffs, int m, bh_buffer *out,
