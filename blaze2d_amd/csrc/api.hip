@@ -26,6 +26,8 @@ struct bh_context {
     hipStream_t gst[3]{};
     // twiddle tables per (n, precision)
     std::vector<std::pair<int, void *>> tw64, tw32;
+    // digit-reversal tables per line length (multi-pass operators)
+    std::vector<std::pair<int, int *>> rev;
     void *scratch = nullptr;
     size_t scratch_bytes = 0;
     // device address of the file-scope allocation-free scratch array on this
@@ -108,6 +110,17 @@ void *ctx_scratch(bh_context *ctx, size_t bytes) {
     return ctx->scratch;
 }
 
+const int *revtab(bh_context *ctx, int n) {
+    for (auto &p : ctx->rev)
+        if (p.first == n) return p.second;
+    const std::vector<int> h = digit_reversal(n);
+    int *d = nullptr;
+    BH_HIP_CHECK(hipMalloc((void **)&d, sizeof(int) * n));
+    BH_HIP_CHECK(hipMemcpy(d, h.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    ctx->rev.push_back({n, d});
+    return d;
+}
+
 template <typename T>
 void run_op(bh_context *ctx, int mode, int nx, int ny, const void *in, void *out, int count, const OpParams *dparams,
             const void *eps, const void *ten, void *bout) {
@@ -121,9 +134,17 @@ void run_op(bh_context *ctx, int mode, int nx, int ny, const void *in, void *out
     w.in_stride = (long)nx * ny;
     w.out_stride = (long)nx * ny;
     w.count = count;
-    OpWork<T> *dw = (OpWork<T> *)ctx_scratch(ctx, sizeof(OpWork<T>));
+    // multi-pass TE / TP: a second field per column behind the descriptor
+    const bool two = grid_multipass(nx, ny) && (mode == MODE_THETA_TE || mode == MODE_PREC_TP);
+    char *base = (char *)ctx_scratch(ctx, 256 + (two ? (size_t)count * nx * ny * sizeof(cplx<T>) : 0));
+    OpWork<T> *dw = (OpWork<T> *)base;
+    if (two) {
+        w.tmp = (cplx<T> *)(base + 256);
+        w.tmp_stride = (long)nx * ny;
+    }
     BH_HIP_CHECK(hipMemcpyAsync(dw, &w, sizeof(w), hipMemcpyHostToDevice, ctx->st));
-    launch_op<T>(mode, nx, ny, dw, 1, count, twiddles<T>(ctx, nx), twiddles<T>(ctx, ny), ctx->st);
+    launch_op<T>(mode, nx, ny, dw, 1, count, twiddles<T>(ctx, nx), twiddles<T>(ctx, ny), ctx->st, false,
+                 revtab(ctx, nx), revtab(ctx, ny));
     BH_HIP_CHECK(hipStreamSynchronize(ctx->st));
 }
 
@@ -235,6 +256,7 @@ void bh_context_destroy(bh_context *ctx) {
     (void)hipStreamSynchronize(ctx->st);
     for (auto &p : ctx->tw64) (void)hipFree(p.second);
     for (auto &p : ctx->tw32) (void)hipFree(p.second);
+    for (auto &p : ctx->rev) (void)hipFree(p.second);
     if (ctx->scratch) (void)hipFree(ctx->scratch);
     for (auto &g : ctx->gst) {
         (void)hipStreamSynchronize(g);
@@ -802,7 +824,7 @@ int bh_operator_diagnostics(bh_context *ctx, const bh_job *jobs, int njobs, int 
         return a.smooth < b.smooth;
     });
     // jobs per chunk: four fields, eps and tensor per job, about 1 GiB at most
-    const long per_job = n * (4 * 16 + 5 * 8);
+    const long per_job = n * (6 * 16 + 5 * 8);
     const int chunk = (int)std::max<long>(1, std::min<long>(4096, (1L << 30) / per_job));
     const cplx<double> *twx = twiddles<double>(ctx, nx), *twy = twiddles<double>(ctx, ny);
     const hipStream_t st = ctx->st;
@@ -818,7 +840,8 @@ int bh_operator_diagnostics(bh_context *ctx, const bh_job *jobs, int njobs, int 
     try {
         const int cap = (int)std::min<size_t>(act.size(), (size_t)chunk);
         if (cap > 0) {
-            BH_HIP_CHECK(hipMalloc((void **)&d_f, (size_t)cap * 4 * n * 16));
+            // four fields per job, then two second-field columns for multi-pass TE / TP
+            BH_HIP_CHECK(hipMalloc((void **)&d_f, (size_t)cap * 6 * n * 16));
             BH_HIP_CHECK(hipMalloc((void **)&d_eps, (size_t)cap * n * 8));
             BH_HIP_CHECK(hipMalloc((void **)&d_ten, (size_t)cap * n * 32));
             BH_HIP_CHECK(hipMalloc((void **)&d_res, (size_t)cap * 3 * 8));
@@ -869,12 +892,14 @@ int bh_operator_diagnostics(bh_context *ctx, const bh_job *jobs, int njobs, int 
             DiagItem *i_n0 = i_sa + cap, *i_s0 = i_n0 + cap, *i_s1 = i_s0 + cap;
             std::vector<OpWork<double>> hw(5 * (size_t)m);
             std::vector<DiagItem> hi(4 * (size_t)m);
-            auto col = [&](int i, int c) { return (cplx<double> *)d_f + ((size_t)i * 4 + c) * n; };
+            auto col = [&](int i, int c) { return (cplx<double> *)d_f + ((size_t)i * 6 + c) * n; };
             for (int i = 0; i < m; ++i) {
                 OpWork<double> w{};
                 w.eps = d_eps + (size_t)i * n;
                 w.tensor = d_ten + (size_t)i * n * 4;
                 w.params = d_par + i;
+                w.tmp = col(i, 4);
+                w.tmp_stride = n;
                 w.in_stride = n;
                 w.out_stride = n;
                 w.count = 1;
@@ -925,18 +950,20 @@ int bh_operator_diagnostics(bh_context *ctx, const bh_job *jobs, int njobs, int 
             const int mode_a = pol == 0 ? MODE_THETA_TM : MODE_THETA_TE;
             const int mode_m = prec == 1 ? MODE_PREC_FD : MODE_PREC_TP;
             // self-adjointness: Ax, Ay
-            launch_diag_init(col(0, 0), col(0, 2), 4 * n, m, n, st);
-            launch_op<double>(mode_a, nx, ny, w_sa, m, 2, twx, twy, st);
+            launch_diag_init(col(0, 0), col(0, 2), 6 * n, m, n, st);
+            const int *rvx = revtab(ctx, nx), *rvy = revtab(ctx, ny);
+            launch_op<double>(mode_a, nx, ny, w_sa, m, 2, twx, twy, st, false, rvx, rvy);
             launch_diag_reduce(i_sa, m, 1, n, st);
             // power iterations on Theta, then on M^-1 Theta, each from the normalised x
             std::vector<DiagItem> h2(2 * (size_t)m);  // lives until the stream is synchronised
             for (int phase = 0; phase < (prec ? 2 : 1); ++phase) {
-                if (phase == 1) launch_diag_init(col(0, 0), col(0, 2), 4 * n, m, n, st);
+                if (phase == 1) launch_diag_init(col(0, 0), col(0, 2), 6 * n, m, n, st);
                 launch_diag_reduce(i_n0, m, 2, n, st);
                 for (int t = 0; t < power_iterations; ++t) {
                     const bool odd = t & 1;
-                    launch_op<double>(mode_a, nx, ny, odd ? w_p1 : w_p0, m, 1, twx, twy, st);
-                    if (phase == 1) launch_op<double>(mode_m, nx, ny, odd ? w_m1 : w_m0, m, 1, twx, twy, st);
+                    launch_op<double>(mode_a, nx, ny, odd ? w_p1 : w_p0, m, 1, twx, twy, st, false, rvx, rvy);
+                    if (phase == 1)
+                        launch_op<double>(mode_m, nx, ny, odd ? w_m1 : w_m0, m, 1, twx, twy, st, false, rvx, rvy);
                     // lambda into slot 0 (Theta) or 1 (M^-1 Theta) of each job's results
                     launch_diag_reduce(odd ? i_s1 : i_s0, m, 0, n, st);
                 }

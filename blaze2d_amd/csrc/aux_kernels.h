@@ -34,8 +34,23 @@ struct RayleighWork {
     int count, pad;
 };
 
+// Relative residual B-norms of columns already formed (the row-chunked
+// deflation path): rel[act[c]] = sqrt(Re r_c^H B r_c) / max(|lambda|, 1e-10)
+template <typename T>
+struct NormWork {
+    const cplx<T> *r;     // count columns, stride apart
+    const T *bw;          // optional B weight (TM: eps); null: B = I
+    const int *act;       // band index of each column
+    const double *lam;
+    double *rel;          // indexed by band
+    long stride;
+    int count, pad;
+};
+
 template <typename T>
 void launch_residual(const ResidualWork<T> *dwork, int nwork, int maxcount, long rows, hipStream_t st);
+template <typename T>
+void launch_bnorm(const NormWork<T> *dwork, int nwork, int maxcount, long rows, hipStream_t st);
 template <typename T>
 void launch_rayleigh(const RayleighWork<T> *dwork, int nwork, int maxcount, long rows, hipStream_t st);
 // xorshift start vectors, phase = first_phase + column (initialization.rs:113-125)

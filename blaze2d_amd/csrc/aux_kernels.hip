@@ -45,6 +45,37 @@ void launch_residual(const ResidualWork<T> *dwork, int nwork, int maxcount, long
     BH_HIP_CHECK(hipGetLastError());
 }
 
+// one workgroup per (column, work item), the column's sum reduced in the workgroup
+template <typename T>
+__global__ void __launch_bounds__(256) bnorm_kernel(const NormWork<T> *__restrict__ work, long rows) {
+    const NormWork<T> &w = work[blockIdx.y];
+    const int c = blockIdx.x;
+    if (c >= w.count) return;
+    const cplx<T> *r = w.r + (size_t)c * w.stride;
+    double acc = 0.0;
+#pragma unroll 4
+    for (long i = threadIdx.x; i < rows; i += 256) {
+        const cplx<T> v = gload(r, i);
+        const T s = w.bw ? gmem(w.bw)[i] : (T)1;
+        acc += (double)v.x * (double)(v.x * s) + (double)v.y * (double)(v.y * s);
+    }
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    __shared__ double part[4];
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int b = w.act[c];
+        w.rel[b] = sqrt(fmax(part[0] + part[1] + part[2] + part[3], 0.0)) / fmax(fabs(w.lam[b]), 1e-10);
+    }
+}
+
+template <typename T>
+void launch_bnorm(const NormWork<T> *dwork, int nwork, int maxcount, long rows, hipStream_t st) {
+    if (nwork <= 0 || maxcount <= 0) return;
+    hipLaunchKernelGGL((bnorm_kernel<T>), dim3(maxcount, nwork), dim3(256), 0, st, dwork, rows);
+    BH_HIP_CHECK(hipGetLastError());
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) rayleigh_kernel(const RayleighWork<T> *__restrict__ work, long rows) {
     const RayleighWork<T> &w = work[blockIdx.y];
@@ -177,6 +208,7 @@ void launch_dot(const cplx<T> *x, const cplx<T> *y, long n, double *out2, hipStr
 #define BH_INST(T)                                                                                         \
     template void launch_residual<T>(const ResidualWork<T> *, int, int, long, hipStream_t);                 \
     template void launch_rayleigh<T>(const RayleighWork<T> *, int, int, long, hipStream_t);                 \
+    template void launch_bnorm<T>(const NormWork<T> *, int, int, long, hipStream_t);                        \
     template void launch_random<T>(cplx<T> *, int, long, double, hipStream_t);                              \
     template void launch_fill<T>(cplx<T> *, cplx<T> *, const T *, double, double, long, hipStream_t);       \
     template void launch_scale<T>(cplx<T> *, long, double, double, hipStream_t);                            \

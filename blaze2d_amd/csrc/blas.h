@@ -119,6 +119,7 @@ struct DeflateWork {
     int nd, a, ld, pad;
 };
 constexpr int DEFL_MAXD = 24;  // most deflation vectors per item
+constexpr long DEFL_MAXROWS = 4096;  // longest column the one-launch deflation holds in registers
 template <typename T>
 void launch_deflate(const DeflateWork<T> *dwork, int nwork, long rows, int maxa, int maxd, hipStream_t st);
 template <typename T>

@@ -352,6 +352,19 @@ class Engine : public EngineBase {
         twy = dalloc<cplx<T>>(ny, st);
         BH_HIP_CHECK(hipMemcpyAsync(twx, tx.data(), sizeof(cplx<T>) * nx, hipMemcpyHostToDevice, st));
         BH_HIP_CHECK(hipMemcpyAsync(twy, ty.data(), sizeof(cplx<T>) * ny, hipMemcpyHostToDevice, st));
+        {
+            const std::vector<int> rx = digit_reversal(nx), ry = digit_reversal(ny);
+            revx = dalloc<int>(nx, st);
+            revy = dalloc<int>(ny, st);
+            BH_HIP_CHECK(hipMemcpyAsync(revx, rx.data(), sizeof(int) * nx, hipMemcpyHostToDevice, st));
+            BH_HIP_CHECK(hipMemcpyAsync(revy, ry.data(), sizeof(int) * ny, hipMemcpyHostToDevice, st));
+            BH_HIP_CHECK(hipStreamSynchronize(st));  // the host tables go out of scope
+        }
+        big = grid_multipass(nx, ny);
+        if (big) {  // an operator item applies to at most m0 columns of a slot
+            TMPC = m0max;
+            TMP = dalloc<cplx<T>>((size_t)S * TMPC * N, st);
+        }
         launch_random<T>(RND, NRMAX, N, 1.0, st);
         BH_HIP_CHECK(hipStreamSynchronize(st));
         slots.resize(S);
@@ -377,7 +390,7 @@ class Engine : public EngineBase {
             (void)hipEventDestroy(G.ev);
             if (G.own_st) (void)hipStreamDestroy(G.st);
         }
-        for (void *p : std::initializer_list<void *>{V, AV, H, RND, EPS, TEN, d_lam, d_Y, d_coef, d_cnt, d_rel, twx, twy})
+        for (void *p : std::initializer_list<void *>{V, AV, H, RND, EPS, TEN, d_lam, d_Y, d_coef, d_cnt, d_rel, twx, twy, revx, revy, TMP})
             if (p) (void)hipFree(p);
         (void)hipHostFree(h_lam);
         (void)hipHostFree(h_cnt);
@@ -402,6 +415,10 @@ class Engine : public EngineBase {
     double *h_lam{};
     int *h_cnt{};
     cplx<T> *twx{}, *twy{};
+    int *revx{}, *revy{};     // digit-reversal tables (multi-pass operators)
+    bool big = false;         // grid on the multi-pass operator path
+    int TMPC = 0;             // second-field columns per slot for multi-pass TE / TP
+    cplx<T> *TMP{};
     std::vector<Group> groups;
     int slots_per_group = 1;
     Group &group_of(int s) { return groups[s / slots_per_group]; }
@@ -421,6 +438,7 @@ class Engine : public EngineBase {
     cplx<T> *hcol(int s, int b, int j) { return H + ((size_t)s * 2 * nbmax + (size_t)b * nbmax + j) * N; }
     cplx<T> *rcol(int j) { return RND + (size_t)j * N; }
     T *eps(int s) { return EPS + (size_t)s * N; }
+    cplx<T> *tmpcol(int s) { return TMP ? TMP + (size_t)s * TMPC * N : nullptr; }
     T *ten(int s) { return TEN + (size_t)s * 4 * N; }
     double *coef(int s) { return d_coef + (size_t)s * LD * LD * 2; }
     // algorithmic work of one Theta column: field in + out (+ the 2x2 inverse-eps
@@ -1093,6 +1111,8 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
         o.eps = eps(s);
         o.tensor = ten(s);
         o.params = dparams + s;
+        o.tmp = tmpcol(s);
+        o.tmp_stride = N;
         o.in_stride = N;
         o.out_stride = N;
         o.count = sl.m;
@@ -1114,6 +1134,7 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
     std::vector<ResidualWork<T>> res;
     bool any_norms = false;  // residual B-norms on log iterations, read back after the step
     std::vector<DeflateWork<T>> defR, defP;
+    std::vector<int> defR_slot, defP_slot;  // slot of each deflation item
     std::vector<OpWork<T>> op_pre[2];  // [0] FD, [1] TP
     std::vector<OpWork<T>> op_p[2];    // Theta on the new directions [TM, TE]
     long theta_cols[2] = {0, 0};
@@ -1191,6 +1212,7 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
                 dw.rel = d_rel + (size_t)s * LD;
             }
             defR.push_back(dw);
+            defR_slot.push_back(s);
         } else if (log_it) {
             rw.rel = d_rel + (size_t)s * LD;
         }
@@ -1216,6 +1238,7 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
             dw.ld = LD;
             // soft part skipped for compacted P index c with soft[c]
             defP.push_back(dw);
+            defP_slot.push_back(s);
             DefMask dm{s, nd, a, std::vector<unsigned char>((size_t)LD * LD, 1)};
             for (int c = 0; c < a; ++c)
                 if (c < (int)sl.soft.size() && sl.soft[c])
@@ -1238,6 +1261,8 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
             o.eps = eps(s);
             o.tensor = ten(s);
             o.params = dparams + s;
+            o.tmp = tmpcol(s);
+            o.tmp_stride = N;
             o.in_stride = N;
             o.out_stride = N;
             o.count = a;
@@ -1251,6 +1276,8 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
             o.eps = eps(s);
             o.tensor = ten(s);
             o.params = dparams + s;
+            o.tmp = tmpcol(s);
+            o.tmp_stride = N;
             o.in_stride = N;
             o.out_stride = N;
             o.count = a;
@@ -1346,6 +1373,7 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
                 dw.lam = d_lam + (size_t)s * LD;
                 dw.rel = d_rel + (size_t)s * LD;
                 defR.push_back(dw);
+                defR_slot.push_back(s);
                 max_d = std::max(max_d, nd);
             } else {
                 rw.rel = d_rel + (size_t)s * LD;
@@ -1407,6 +1435,74 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
         for (size_t i = 0; i < masksP.size(); ++i) std::memcpy(mh + i * LD * LD, masksP[i].m.data(), LD * LD);
     }
     for (size_t i = 0; i < defP.size(); ++i) defP[i].mask = mask_d + i * LD * LD;
+    // Columns longer than the one-launch deflation holds in registers, or more
+    // deflation vectors than it takes: the same deflation in three launches
+    // (coefficients as a Gram over row chunks, their masked reduction into the
+    // slot's coefficient block, r -= D C) plus the residual norms it owes.
+    const bool defl_rows = N > DEFL_MAXROWS || max_d > DEFL_MAXD;
+    struct DeflRows {
+        std::vector<GramWork<T>> g;
+        std::vector<ReduceWork> r;
+        std::vector<LincombWork<T>> l;
+        std::vector<NormWork<T>> n;
+        int nch = 1;
+        GramWork<T> *dg{};
+        ReduceWork *dr{};
+        LincombWork<T> *dl{};
+        NormWork<T> *dn{};
+    } drR, drP;
+    auto to_rows = [&](const std::vector<DeflateWork<T>> &items, const std::vector<int> &islot, DeflRows &o) {
+        o.nch = nch_for(items.size());
+        for (size_t i = 0; i < items.size(); ++i) {
+            const DeflateWork<T> &d = items[i];
+            double *part = d_part + i * (size_t)o.nch * LD * LD * 2;
+            GramWork<T> g{};
+            g.a_ptrs = d.d_ptrs;  // <B d_i, r_c> = sum conj(d_i) w r_c
+            g.B = d.R;
+            g.b_stride = d.stride;
+            g.weight = d.weight;
+            g.na = d.nd;
+            g.nb = d.a;
+            g.ld = LD;
+            g.partial = part;
+            o.g.push_back(g);
+            o.r.push_back(ReduceWork{part, coef(islot[i]), d.mask, o.nch, 0});
+            LincombWork<T> l{};
+            l.a_ptrs = d.d_ptrs;
+            l.na = d.nd;
+            l.nq = d.a;
+            l.C = coef(islot[i]);
+            l.ldc = LD;
+            l.base = d.R;
+            l.base_stride = d.stride;
+            l.out = d.R;
+            l.out_stride = d.stride;
+            l.mode = 1;
+            o.l.push_back(l);
+            if (d.rel) o.n.push_back(NormWork<T>{d.R, d.weight, d.act, d.lam, d.rel, d.stride, d.a, 0});
+        }
+    };
+    if (defl_rows) {
+        to_rows(defR, defR_slot, drR);
+        to_rows(defP, defP_slot, drP);
+        for (DeflRows *o : {&drR, &drP}) {
+            o->dg = put(o->g);
+            o->dr = put(o->r);
+            o->dl = put(o->l);
+            o->dn = put(o->n);
+        }
+    }
+    auto deflate = [&](const DeflateWork<T> *dwork, const std::vector<DeflateWork<T>> &items, const DeflRows &o,
+                       int maxa) {
+        if (!defl_rows) {
+            launch_deflate<T>(dwork, (int)items.size(), N, maxa, max_d, st);
+            return;
+        }
+        launch_gram<T>(o.dg, (int)o.g.size(), o.nch, N, std::max(max_d, 1), maxa, st);
+        launch_reduce(o.dr, (int)o.r.size(), LD, st);
+        launch_lincomb<T>(o.dl, (int)o.l.size(), N, maxa, st);
+        launch_bnorm<T>(o.dn, (int)o.n.size(), maxa, N, st);
+    };
     auto *d_lc_hist = put(lc_hist);
     auto *d_lc_x = put(lc_x);
     OpWork<T> *d_op_x[2] = {put(op_x[0]), put(op_x[1])};
@@ -1458,7 +1554,7 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
         timed(TIMED_THETA, (double)cols * theta_bytes_per_col(pol), (double)cols * theta_flops_per_col(pol, n_log),
               [&] {
                   launch_op<T>(pol == 0 ? MODE_THETA_TM : MODE_THETA_TE, nx, ny, d_op_x[pol], (int)op_x[pol].size(),
-                               m0max, twx, twy, st);
+                               m0max, twx, twy, st, false, revx, revy);
               });
     }
     launch_rayleigh<T>(d_rq, (int)rq.size(), m0max, N, st);
@@ -1477,21 +1573,21 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
     };
     if (!defR.empty())
         timed(TIMED_DEFL, defl_bytes(defR), defl_flops(defR),
-              [&] { launch_deflate<T>(d_defR, (int)defR.size(), N, max_r, max_d, st); });
+              [&] { deflate(d_defR, defR, drR, max_r); });
     const bool prec_fp32 = eopt.precision == 2;
     launch_op<T>(MODE_PREC_FD, nx, ny, d_op_pre[0], (int)op_pre[0].size(), std::max(max_a, 1), twx, twy, st,
-                 prec_fp32);
+                 prec_fp32, revx, revy);
     launch_op<T>(MODE_PREC_TP, nx, ny, d_op_pre[1], (int)op_pre[1].size(), std::max(max_a, 1), twx, twy, st,
-                 prec_fp32);
+                 prec_fp32, revx, revy);
     if (!defP.empty())
         timed(TIMED_DEFL, defl_bytes(defP), defl_flops(defP),
-              [&] { launch_deflate<T>(d_defP, (int)defP.size(), N, max_a, max_d, st); });
+              [&] { deflate(d_defP, defP, drP, max_a); });
     for (int pol = 0; pol < 2; ++pol) {
         if (op_p[pol].empty()) continue;
         timed(TIMED_THETA, (double)theta_cols[pol] * theta_bytes_per_col(pol),
               (double)theta_cols[pol] * theta_flops_per_col(pol, n_log), [&] {
                   launch_op<T>(pol == 0 ? MODE_THETA_TM : MODE_THETA_TE, nx, ny, d_op_p[pol], (int)op_p[pol].size(),
-                               std::max(max_a, 1), twx, twy, st);
+                               std::max(max_a, 1), twx, twy, st, false, revx, revy);
               });
     }
     {

@@ -3,6 +3,8 @@
 
 #include "common.h"
 
+#include <vector>
+
 namespace bh {
 
 enum OpMode {
@@ -28,14 +30,32 @@ struct OpWork {
     long out_stride;
     int count;
     int pad;
+    cplx<T> *tmp;         // multi-pass TE / TP: a second field per column (tmp_stride apart)
+    long tmp_stride;
 };
 
+// Grids the operators run on: square 8..64 with the whole field in one
+// workgroup's LDS, and any nx x ny with both sides in {8, 16, 24, 32, 48, 64,
+// 96, 128, 192, 256, 384, 512} through the multi-pass kernels.
 bool grid_supported(int nx, int ny);
+// true when the grid takes the multi-pass path (its TE / TP applies need tmp)
+bool grid_multipass(int nx, int ny);
+// frequency index held at each digit-reversed position of an n-point line
+std::vector<int> digit_reversal(int n);
 
 // prec_fp32: the FD and TP preconditioners of an FP64 launch compute in FP32
 // (FP64 columns in and out)
 template <typename T>
 void launch_op(int mode, int nx, int ny, const OpWork<T> *dwork, int nwork, int maxcols, const cplx<T> *twx,
-               const cplx<T> *twy, hipStream_t st, bool prec_fp32 = false);
+               const cplx<T> *twy, hipStream_t st, bool prec_fp32 = false, const int *revx = nullptr,
+               const int *revy = nullptr);
+
+// Multi-pass operators: column passes (y transforms over tiles of 8 columns)
+// and row passes (x transforms of whole rows with the Fourier-space step), the
+// field passing through HBM (L2) between passes.  revx / revy: digit_reversal
+// tables of nx and ny on the device.
+template <typename T>
+void launch_op_big(int mode, int nx, int ny, const OpWork<T> *dwork, int nwork, int maxcols, const cplx<T> *twx,
+                   const cplx<T> *twy, const int *revx, const int *revy, hipStream_t st);
 
 }  // namespace bh

@@ -17,6 +17,7 @@
 // FFT convention (backend-cpu/src/lib.rs:94-177): forward unnormalised
 // e^{-2 pi i jk/n}, inverse e^{+...} scaled by 1/(nx ny).
 #include "common.h"
+#include "fft_core.h"
 #include "fft_ops.h"
 
 #include <type_traits>
@@ -35,272 +36,6 @@ namespace bh {
 #ifndef OP_NT_64_F2
 #define OP_NT_64_F2 1024
 #endif
-
-// ---------------------------------------------------------------------------
-// Radix plans: N = R0 * R1 * ... (largest radix first).  Radix 8 halves the
-// number of LDS round trips of a 64-point line compared with radix 4.
-// ---------------------------------------------------------------------------
-template <int N>
-struct Plan;
-#define BH_PLAN(NN, KK, A, B, C, D)                                                      \
-    template <>                                                                          \
-    struct Plan<NN> {                                                                    \
-        static constexpr int K = KK;                                                     \
-        __host__ __device__ static constexpr int r(int s) {                              \
-            return s == 0 ? A : s == 1 ? B : s == 2 ? C : D;                             \
-        }                                                                                \
-    };
-BH_PLAN(8, 2, 4, 2, 1, 1)
-BH_PLAN(16, 2, 4, 4, 1, 1)
-BH_PLAN(24, 3, 4, 3, 2, 1)
-BH_PLAN(32, 2, 8, 4, 1, 1)
-BH_PLAN(48, 3, 4, 4, 3, 1)
-BH_PLAN(64, 2, 8, 8, 1, 1)
-#undef BH_PLAN
-
-template <int N>
-__host__ __device__ constexpr int plan_len(int s) {  // transform length at pass s
-    int L = N;
-    for (int i = 0; i < s; ++i) L /= Plan<N>::r(i);
-    return L;
-}
-
-// frequency index held at digit-reversed storage position p
-template <int N>
-__device__ inline int rev_index(int p) {
-    int k = 0, mult = 1, rem = p, L = N;
-#pragma unroll
-    for (int s = 0; s < Plan<N>::K; ++s) {
-        const int R = Plan<N>::r(s);
-        L /= R;
-        int d = rem / L;
-        rem -= d * L;
-        k += d * mult;
-        mult *= R;
-    }
-    return k;
-}
-
-// ---------------------------------------------------------------------------
-// Small DFT butterflies (in registers).  INV selects e^{+2 pi i/R}.
-// ---------------------------------------------------------------------------
-template <typename T>
-__device__ inline cplx<T> mul_mi(cplx<T> a) { return {a.y, -a.x}; }  // * (-i)
-template <typename T>
-__device__ inline cplx<T> mul_pi(cplx<T> a) { return {-a.y, a.x}; }  // * (+i)
-
-template <typename T, int R, bool INV>
-__device__ inline void small_dft(cplx<T> *v) {
-    if constexpr (R == 2) {
-        cplx<T> a = v[0], b = v[1];
-        v[0] = cadd(a, b);
-        v[1] = csub(a, b);
-    } else if constexpr (R == 3) {
-        const T s = (T)0.86602540378443864676;
-        cplx<T> t1 = cadd(v[1], v[2]);
-        cplx<T> t2 = {v[0].x - (T)0.5 * t1.x, v[0].y - (T)0.5 * t1.y};
-        cplx<T> t3 = {s * (v[1].x - v[2].x), s * (v[1].y - v[2].y)};
-        v[0] = cadd(v[0], t1);
-        if (!INV) {
-            v[1] = cadd(t2, mul_mi(t3));
-            v[2] = cadd(t2, mul_pi(t3));
-        } else {
-            v[1] = cadd(t2, mul_pi(t3));
-            v[2] = cadd(t2, mul_mi(t3));
-        }
-    } else if constexpr (R == 4) {
-        cplx<T> s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
-        cplx<T> s13 = cadd(v[1], v[3]), d13 = csub(v[1], v[3]);
-        v[0] = cadd(s02, s13);
-        v[2] = csub(s02, s13);
-        if (!INV) {
-            v[1] = cadd(d02, mul_mi(d13));
-            v[3] = cadd(d02, mul_pi(d13));
-        } else {
-            v[1] = cadd(d02, mul_pi(d13));
-            v[3] = cadd(d02, mul_mi(d13));
-        }
-    } else if constexpr (R == 8) {
-        // radix 2 x 4: length-4 DFTs of the even and odd samples, odd half
-        // twiddled by W8^k (W8 = e^{-+ i pi/4}), then one radix-2 stage
-        cplx<T> e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
-        small_dft<T, 4, INV>(e);
-        small_dft<T, 4, INV>(o);
-        const T h = (T)0.70710678118654752440;
-        if (!INV) {
-            o[1] = {h * (o[1].x + o[1].y), h * (o[1].y - o[1].x)};   // * (1 - i)/sqrt2
-            o[2] = mul_mi(o[2]);                                      // * -i
-            o[3] = {h * (o[3].y - o[3].x), -h * (o[3].x + o[3].y)};  // * (-1 - i)/sqrt2
-        } else {
-            o[1] = {h * (o[1].x - o[1].y), h * (o[1].y + o[1].x)};   // * (1 + i)/sqrt2
-            o[2] = mul_pi(o[2]);                                      // * +i
-            o[3] = {-h * (o[3].x + o[3].y), h * (o[3].x - o[3].y)};  // * (-1 + i)/sqrt2
-        }
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            v[k] = cadd(e[k], o[k]);
-            v[k + 4] = csub(e[k], o[k]);
-        }
-    }
-}
-
-// LDS position of the field element with flat index i (row-major, rows RW
-// long).  For 64-wide rows the low three bits of the column are XORed with the
-// next three: a radix-8 pass over rows whose eight lanes read elements 8 apart
-// (128 bytes, one bank cycle) then hits eight different bank groups instead of
-// one, and the accesses that are contiguous across lanes stay within the same
-// 128-byte block.  It is a bijection on every 64-element row.
-template <int RW>
-__device__ inline int lds_pos(int i) {
-    if constexpr (RW == 64) return i ^ ((i >> 3) & 7);
-    else return i;
-}
-
-// Where a pass takes its input or puts its output: the LDS field, or (for
-// the first forward and the last inverse pass of a one-field operator) the
-// field in HBM, which saves one LDS round trip and one barrier at each end.
-struct PassIO {
-    const void *gin = nullptr;   // global input (natural order)
-    void *gout = nullptr;        // global output (natural order)
-    const void *geps = nullptr;  // optional: gbout = geps * output
-    void *gbout = nullptr;
-};
-
-// One radix pass over NLINES independent length-N transforms stored in LDS.
-// ROWS: elements contiguous (stride 1), lines NX apart; else elements NX
-// apart, lines contiguous.  NF fields are processed together.  GIN / GOUT:
-// read the input from / write the output to HBM (io) instead of LDS.
-template <typename T, int N, int NLINES, bool ROWS, int S, bool INV, int NF, int NT, int FSTRIDE, int RW,
-          bool GIN = false, bool GOUT = false, typename TS = T>
-__device__ inline void fft_pass(cplx<T> *buf, const cplx<T> *__restrict__ tw, const PassIO &io = PassIO()) {
-    constexpr int R = Plan<N>::r(S);
-    constexpr int L = plan_len<N>(S);
-    constexpr int M = L / R;
-    constexpr int NBF = N / R;
-    constexpr int TOTAL = NBF * NLINES * NF;
-    constexpr int TWS = N / L;
-    constexpr int ES = ROWS ? 1 : NLINES;  // element stride
-    constexpr int LS = ROWS ? N : 1;       // line stride
-#pragma unroll 1
-    for (int id = threadIdx.x; id < TOTAL; id += NT) {
-        int f = id / (NBF * NLINES);
-        int rem = id - f * (NBF * NLINES);
-        int line, bf;
-        if (ROWS) { line = rem / NBF; bf = rem - line * NBF; }
-        else { bf = rem / NLINES; line = rem - bf * NLINES; }
-        int b = bf / M, j2 = bf - b * M;
-        const int e0 = f * FSTRIDE + line * LS + (b * L + j2) * ES;
-        cplx<T> v[R];
-#pragma unroll
-        for (int j = 0; j < R; ++j) {
-            if constexpr (GIN) {  // HBM in storage precision TS
-                const cplx<TS> g = gload(static_cast<const cplx<TS> *>(io.gin), e0 + j * M * ES);
-                v[j] = {(T)g.x, (T)g.y};
-            }
-            else v[j] = buf[lds_pos<RW>(e0 + j * M * ES)];
-        }
-        if (!INV) {
-            small_dft<T, R, false>(v);
-            if (M > 1) {
-#pragma unroll
-                for (int k1 = 1; k1 < R; ++k1) v[k1] = cmul(v[k1], tw[j2 * k1 * TWS]);
-            }
-        } else {
-            if (M > 1) {
-#pragma unroll
-                for (int k1 = 1; k1 < R; ++k1) v[k1] = cmulc(v[k1], tw[j2 * k1 * TWS]);
-            }
-            small_dft<T, R, true>(v);
-        }
-#pragma unroll
-        for (int j = 0; j < R; ++j) {
-            if constexpr (GOUT) {
-                const int e = e0 + j * M * ES;
-                const cplx<TS> o = {(TS)v[j].x, (TS)v[j].y};
-                gstore(static_cast<cplx<TS> *>(io.gout), e, o);
-                if (io.gbout) {
-                    const TS w = gmem(static_cast<const TS *>(io.geps))[e];
-                    gstore(static_cast<cplx<TS> *>(io.gbout), e, cplx<TS>{o.x * w, o.y * w});
-                }
-            } else {
-                buf[lds_pos<RW>(e0 + j * M * ES)] = v[j];
-            }
-        }
-    }
-}
-
-template <typename T, int N, int NLINES, bool ROWS, bool INV, int NF, int NT, int FSTRIDE, int RW, int S>
-__device__ inline void fft_passes_fwd(cplx<T> *buf, const cplx<T> *tw) {
-    if constexpr (S < Plan<N>::K) {
-        fft_pass<T, N, NLINES, ROWS, S, INV, NF, NT, FSTRIDE, RW>(buf, tw);
-        __syncthreads();
-        fft_passes_fwd<T, N, NLINES, ROWS, INV, NF, NT, FSTRIDE, RW, S + 1>(buf, tw);
-    }
-}
-template <typename T, int N, int NLINES, bool ROWS, bool INV, int NF, int NT, int FSTRIDE, int RW, int S,
-          int SEND = 0>
-__device__ inline void fft_passes_inv(cplx<T> *buf, const cplx<T> *tw) {
-    if constexpr (S >= SEND) {
-        fft_pass<T, N, NLINES, ROWS, S, INV, NF, NT, FSTRIDE, RW>(buf, tw);
-        __syncthreads();
-        fft_passes_inv<T, N, NLINES, ROWS, INV, NF, NT, FSTRIDE, RW, S - 1, SEND>(buf, tw);
-    }
-}
-
-// natural -> digit-reversed (rows then columns)
-template <typename T, int NX, int NY, int NF, int NT>
-__device__ inline void fft2d_forward(cplx<T> *buf, const cplx<T> *twx, const cplx<T> *twy) {
-    fft_passes_fwd<T, NX, NY, true, false, NF, NT, NX * NY, NX, 0>(buf, twx);
-    fft_passes_fwd<T, NY, NX, false, false, NF, NT, NX * NY, NX, 0>(buf, twy);
-}
-// digit-reversed -> natural, unscaled
-template <typename T, int NX, int NY, int NF, int NT>
-__device__ inline void fft2d_inverse(cplx<T> *buf, const cplx<T> *twx, const cplx<T> *twy) {
-    fft_passes_inv<T, NY, NX, false, true, NF, NT, NX * NY, NX, Plan<NY>::K - 1>(buf, twy);
-    fft_passes_inv<T, NX, NY, true, true, NF, NT, NX * NY, NX, Plan<NX>::K - 1>(buf, twx);
-}
-// one field, natural order in HBM -> digit-reversed in LDS (the first row
-// pass reads HBM; twiddles must be visible in LDS before the call)
-template <typename T, int NX, int NY, int NT, typename TS = T>
-__device__ inline void fft2d_forward_from(cplx<T> *buf, const cplx<TS> *gin, const cplx<T> *twx,
-                                          const cplx<T> *twy) {
-    PassIO io;
-    io.gin = gin;
-    fft_pass<T, NX, NY, true, 0, false, 1, NT, NX * NY, NX, true, false, TS>(buf, twx, io);
-    __syncthreads();
-    fft_passes_fwd<T, NX, NY, true, false, 1, NT, NX * NY, NX, 1>(buf, twx);
-    fft_passes_fwd<T, NY, NX, false, false, 1, NT, NX * NY, NX, 0>(buf, twy);
-}
-// one field, digit-reversed in LDS -> natural order in HBM (the last row pass
-// writes gout, and gbout = geps * gout when gbout is set), unscaled
-template <typename T, int NX, int NY, int NT, typename TS = T>
-__device__ inline void fft2d_inverse_to(cplx<T> *buf, const PassIO &io, const cplx<T> *twx, const cplx<T> *twy) {
-    fft_passes_inv<T, NY, NX, false, true, 1, NT, NX * NY, NX, Plan<NY>::K - 1>(buf, twy);
-    fft_passes_inv<T, NX, NY, true, true, 1, NT, NX * NY, NX, Plan<NX>::K - 1, 1>(buf, twx);
-    fft_pass<T, NX, NY, true, 0, true, 1, NT, NX * NY, NX, false, true, TS>(buf, twx, io);
-}
-
-// k+G at the frequency stored at LDS position (px, py), with the reference's
-// |k+G|^2 <= 1e-9 clamp (maxwell.rs:1609-1690).
-template <int NX, int NY>
-__device__ inline void kplusg(const OpParams &P, int px, int py, double &kx, double &ky, double &ksq,
-                              bool &clamped) {
-    int kxi = rev_index<NX>(px), kyi = rev_index<NY>(py);
-    double n1 = kxi <= NX / 2 ? kxi : kxi - NX;
-    double n2 = kyi <= NY / 2 ? kyi : kyi - NY;
-    double gx = n1 * P.b1[0] + n2 * P.b2[0];
-    double gy = n1 * P.b1[1] + n2 * P.b2[1];
-    double rx = gx + P.bloch[0], ry = gy + P.bloch[1];
-    double raw = rx * rx + ry * ry;
-    clamped = raw <= KG_FLOOR;
-    if (raw >= KG_FLOOR) { kx = rx; ky = ry; }
-    else if (raw == 0.0) { kx = sqrt(KG_FLOOR); ky = 0.0; }
-    else { double s = sqrt(KG_FLOOR / raw); kx = rx * s; ky = ry * s; }
-    ksq = kx * kx + ky * ky;
-}
-
-template <typename T>
-__device__ inline cplx<T> scale_c(cplx<T> a, T s) { return {a.x * s, a.y * s}; }
 
 // ---------------------------------------------------------------------------
 // The fused operator kernel.  grid = (max columns, work items).  TS is the
@@ -556,15 +291,70 @@ static void launch_sized(int mode, const OpWork<T> *dwork, int nwork, int maxcol
     BH_HIP_CHECK(hipGetLastError());
 }
 
-bool grid_supported(int nx, int ny) {
+static bool lds_grid(int nx, int ny) {
     return nx == ny && (nx == 8 || nx == 16 || nx == 24 || nx == 32 || nx == 48 || nx == 64);
+}
+
+static bool line_supported(int n) {
+    switch (n) {
+        case 8: case 16: case 24: case 32: case 48: case 64: case 96: case 128: case 192: case 256: case 384:
+        case 512: return true;
+        default: return false;
+    }
+}
+
+bool grid_supported(int nx, int ny) { return lds_grid(nx, ny) || (line_supported(nx) && line_supported(ny)); }
+bool grid_multipass(int nx, int ny) { return grid_supported(nx, ny) && !lds_grid(nx, ny); }
+
+template <int N>
+static std::vector<int> digit_reversal_n() {
+    std::vector<int> r(N);
+    for (int p = 0; p < N; ++p) {  // rev_index on the host
+        int k = 0, mult = 1, rem = p, L = N;
+        for (int s = 0; s < Plan<N>::K; ++s) {
+            const int R = Plan<N>::r(s);
+            L /= R;
+            const int d = rem / L;
+            rem -= d * L;
+            k += d * mult;
+            mult *= R;
+        }
+        r[p] = k;
+    }
+    return r;
+}
+
+std::vector<int> digit_reversal(int n) {
+    switch (n) {
+        case 8: return digit_reversal_n<8>();
+        case 16: return digit_reversal_n<16>();
+        case 24: return digit_reversal_n<24>();
+        case 32: return digit_reversal_n<32>();
+        case 48: return digit_reversal_n<48>();
+        case 64: return digit_reversal_n<64>();
+        case 96: return digit_reversal_n<96>();
+        case 128: return digit_reversal_n<128>();
+        case 192: return digit_reversal_n<192>();
+        case 256: return digit_reversal_n<256>();
+        case 384: return digit_reversal_n<384>();
+        case 512: return digit_reversal_n<512>();
+        default: throw std::runtime_error("FFT length " + std::to_string(n) + " has no radix plan");
+    }
 }
 
 template <typename T>
 void launch_op(int mode, int nx, int ny, const OpWork<T> *dwork, int nwork, int maxcols, const cplx<T> *twx,
-               const cplx<T> *twy, hipStream_t st, bool prec_fp32) {
+               const cplx<T> *twy, hipStream_t st, bool prec_fp32, const int *revx, const int *revy) {
     if (nwork <= 0 || maxcols <= 0) return;
-    if (nx != ny) throw std::runtime_error("only square grids are instantiated");
+    if (!lds_grid(nx, ny)) {
+        if (!grid_supported(nx, ny))
+            throw std::runtime_error("grid " + std::to_string(nx) + " x " + std::to_string(ny) + " not supported");
+        if (prec_fp32)
+            throw std::runtime_error("FP32 preconditioners are instantiated for square grids up to 64 x 64");
+        if (!revx || !revy) throw std::runtime_error("multi-pass operator without digit-reversal tables");
+        launch_op_big<T>(mode, nx, ny, dwork, nwork, maxcols, twx, twy, revx, revy, st);
+        return;
+    }
     switch (nx) {
         case 8: launch_sized<T, 8, 8>(mode, dwork, nwork, maxcols, twx, twy, st, prec_fp32); break;
         case 16: launch_sized<T, 16, 16>(mode, dwork, nwork, maxcols, twx, twy, st, prec_fp32); break;
@@ -577,8 +367,8 @@ void launch_op(int mode, int nx, int ny, const OpWork<T> *dwork, int nwork, int 
 }
 
 template void launch_op<double>(int, int, int, const OpWork<double> *, int, int, const cplx<double> *,
-                                const cplx<double> *, hipStream_t, bool);
+                                const cplx<double> *, hipStream_t, bool, const int *, const int *);
 template void launch_op<float>(int, int, int, const OpWork<float> *, int, int, const cplx<float> *,
-                               const cplx<float> *, hipStream_t, bool);
+                               const cplx<float> *, hipStream_t, bool, const int *, const int *);
 
 }  // namespace bh
