@@ -492,15 +492,16 @@ int bh_hermitian_eigh(bh_context *ctx, int n, int count, const double *a, double
 int bh_rayleigh_ritz(bh_context *ctx, int n, int count, int nsplit, const double *g1, const double *g2, double *lam,
                      double *coef, int *info) {
     BH_API_TRY
-    if (!ctx || n <= 0 || n > GRAM_MAXC || count < 0 || nsplit <= 0 || !g1 || !g2 || !lam || !coef || !info)
-        return fail("bh_rayleigh_ritz: bad arguments");
+    if (!ctx || n <= 0 || n > 96 || count < 0 || nsplit <= 0 || !g1 || !g2 || !lam || !coef || !info)
+        return fail("bh_rayleigh_ritz: bad arguments (n must be 1..96)");
     if (count == 0) return 0;
     BH_HIP_CHECK(hipSetDevice(ctx->device));
     const size_t nn = (size_t)n * n;
     // device layout: [work items][G1 partials][G2 partials][lam][C][counts], each
     // partial row-major G(i, j) with one chunk
     const size_t wbytes = ((count * sizeof(RRWork) + 255) / 256) * 256;
-    const size_t bytes = wbytes + sizeof(double) * count * (2 * nn * 3 + n) + sizeof(int) * 8 * count;
+    const size_t wsb = n > GRAM_MAXC ? rr_wide_bytes(n) : 0;  // wide kernel workspace per problem
+    const size_t bytes = wbytes + sizeof(double) * count * (2 * nn * 3 + n) + sizeof(int) * 8 * count + 16 + wsb * count;
     char *base = (char *)ctx_scratch(ctx, bytes);
     RRWork *dw = (RRWork *)base;
     double *dp1 = (double *)(base + wbytes);
@@ -508,6 +509,7 @@ int bh_rayleigh_ritz(bh_context *ctx, int n, int count, int nsplit, const double
     double *dc = dp2 + 2 * nn * count;
     double *dl = dc + 2 * nn * count;
     int *dcnt = (int *)(dl + (size_t)n * count);
+    double *dws = (double *)(((uintptr_t)(dcnt + 8 * count) + 15) & ~(uintptr_t)15);
     std::vector<double> h1(2 * nn * count), h2(2 * nn * count);
     for (int c = 0; c < count; ++c)
         for (int i = 0; i < n; ++i)
@@ -531,6 +533,7 @@ int bh_rayleigh_ritz(bh_context *ctx, int n, int count, int nsplit, const double
         r.ld = n;
         r.ldc = n;
         r.nsplit = nsplit;
+        if (wsb) r.ws = dws + (wsb / sizeof(double)) * c;
         hw[c] = r;
     }
     BH_HIP_CHECK(hipMemsetAsync(dl, 0, sizeof(double) * n * count, ctx->st));

@@ -79,9 +79,11 @@ struct RRWork {
     int *cnt;               // [cnt_x, cnt_w, cnt_x + cnt_w, Jacobi sweeps run, SVQB path taken (0/1), rank]
     int n, nchunks, ld, ldc, nsplit, pad;
     int *sweeps_out;        // optional diagnostics (see rr_kernel)
+    double *ws;             // orders above 48: device workspace of rr_wide_bytes(n) (T, eigenvectors, G2 T)
 };
 
-constexpr int GRAM_MAXC = 48;  // maximum columns per Gram operand
+constexpr int GRAM_MAXC = 48;   // columns of the one-workgroup subspace kernels (up to 16 bands)
+constexpr int GRAM_WIDE = 128;  // columns of the wide (tiled / multi-workgroup) paths
 constexpr int EIGH_MAXN = 64;
 
 template <typename T>
@@ -122,13 +124,19 @@ constexpr int DEFL_MAXD = 24;  // most deflation vectors per item
 constexpr long DEFL_MAXROWS = 4096;  // longest column the one-launch deflation holds in registers
 template <typename T>
 void launch_deflate(const DeflateWork<T> *dwork, int nwork, long rows, int maxa, int maxd, hipStream_t st);
+// maxna: most inputs of an item (above 48 selects the wide variant)
 template <typename T>
-void launch_lincomb(const LincombWork<T> *dwork, int nwork, long rows, int maxq, hipStream_t st);
+void launch_lincomb(const LincombWork<T> *dwork, int nwork, long rows, int maxq, hipStream_t st,
+                    int maxna = GRAM_MAXC);
 void launch_eigh(const EighWork *dwork, int nwork, int maxn, hipStream_t st);
 // maxn: largest n in the launch (selects the 36- or 48-column kernel)
 template <typename T>
 void launch_gram2(const Gram2Work<T> *dwork, int nwork, int nchunks, long rows, int maxn, hipStream_t st);
+// orders up to 48: one 512-thread workgroup per problem, all in LDS; above
+// 48 (up to 128): one 1024-thread workgroup per problem with the projected
+// matrix in LDS and T, the eigenvectors and G2 T in the item's workspace
 void launch_rr(const RRWork *dwork, int nwork, int maxn, hipStream_t st);
+size_t rr_wide_bytes(int n);
 size_t eigh_shared_bytes(int maxn);
 
 }  // namespace bh

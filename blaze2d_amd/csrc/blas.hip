@@ -34,16 +34,24 @@ constexpr int GRAM_SR = 32;
 
 template <typename T, int BU>
 __global__ void __launch_bounds__(256) gram_kernel(const GramWork<T> *__restrict__ work, long rows_per_chunk,
-                                                   long nrows) {
+                                                   long nrows, int ntj) {
     constexpr int W = 16 * BU;
     const GramWork<T> &w = work[blockIdx.y];
-    const int na = w.dev_na ? *w.dev_na : w.na;
-    const int nb = w.dev_nb ? *w.dev_nb : w.nb;
-    if (na <= 0 || nb <= 0 || na > W || nb > W) return;  // the launch sizes W from the widest item
+    // output tile (ti, tj) of W x W: operands wider than W (up to GRAM_WIDE
+    // columns) are covered by several workgroups; a Hermitian product skips
+    // the tiles above the diagonal
+    const int ti = blockIdx.z / ntj, tj = blockIdx.z - ti * ntj;
+    const int i0 = ti * W, j0 = tj * W;
+    if (w.lower_only && tj > ti) return;
+    const int na_all = w.dev_na ? *w.dev_na : w.na;
+    const int nb_all = w.dev_nb ? *w.dev_nb : w.nb;
+    const int na = na_all - i0 < W ? na_all - i0 : W;
+    const int nb = nb_all - j0 < W ? nb_all - j0 : W;
+    if (na <= 0 || nb <= 0) return;
     __shared__ cplx<T> As[GRAM_SR][W + 1];
     __shared__ cplx<T> Bs[GRAM_SR][W + 1];
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-    const bool lower = w.lower_only != 0;
+    const bool lower = w.lower_only != 0 && ti == tj;  // only diagonal tiles have sub-blocks above the diagonal
     double ar[BU][BU] = {}, ai[BU][BU] = {};
     const long r0 = (long)blockIdx.x * rows_per_chunk;
     const long r1 = r0 + rows_per_chunk < nrows ? r0 + rows_per_chunk : nrows;
@@ -51,12 +59,12 @@ __global__ void __launch_bounds__(256) gram_kernel(const GramWork<T> *__restrict
         const int nr = (int)(r1 - rb < GRAM_SR ? r1 - rb : GRAM_SR);
         for (int e = threadIdx.x; e < GRAM_SR * na; e += 256) {
             int c = e / GRAM_SR, r = e - c * GRAM_SR;
-            const cplx<T> *col = w.a_ptrs ? w.a_ptrs[c] : w.A + (long)c * w.a_stride;
+            const cplx<T> *col = w.a_ptrs ? w.a_ptrs[i0 + c] : w.A + (long)(i0 + c) * w.a_stride;
             As[r][c] = r < nr ? gload(col, rb + r) : cplx<T>{(T)0, (T)0};
         }
         for (int e = threadIdx.x; e < GRAM_SR * nb; e += 256) {
             int c = e / GRAM_SR, r = e - c * GRAM_SR;
-            const cplx<T> *col = w.b_ptrs ? w.b_ptrs[c] : w.B + (long)c * w.b_stride;
+            const cplx<T> *col = w.b_ptrs ? w.b_ptrs[j0 + c] : w.B + (long)(j0 + c) * w.b_stride;
             cplx<T> v = r < nr ? gload(col, rb + r) : cplx<T>{(T)0, (T)0};
             if (w.weight && r < nr) {
                 T s = gmem(w.weight)[rb + r];
@@ -96,8 +104,8 @@ __global__ void __launch_bounds__(256) gram_kernel(const GramWork<T> *__restrict
         for (int v = 0; v < BU; ++v) {
             int i = ty + 16 * u, j = tx + 16 * v;
             if (i < na && j < nb) {
-                gmem(out)[2 * (i * w.ld + j)] = ar[u][v];
-                gmem(out)[2 * (i * w.ld + j) + 1] = ai[u][v];
+                gmem(out)[2 * ((i0 + i) * w.ld + j0 + j)] = ar[u][v];
+                gmem(out)[2 * ((i0 + i) * w.ld + j0 + j) + 1] = ai[u][v];
             }
         }
 }
@@ -105,16 +113,19 @@ __global__ void __launch_bounds__(256) gram_kernel(const GramWork<T> *__restrict
 template <typename T>
 void launch_gram(const GramWork<T> *dwork, int nwork, int nchunks, long rows, int maxa, int maxb, hipStream_t st) {
     if (nwork <= 0) return;
-    if (maxa > GRAM_MAXC || maxb > GRAM_MAXC) throw std::runtime_error("gram operand wider than 48 columns");
+    if (maxa > GRAM_WIDE || maxb > GRAM_WIDE) throw std::runtime_error("gram operand wider than 128 columns");
     long per = (rows + nchunks - 1) / nchunks;
     per = (per + GRAM_SR - 1) / GRAM_SR * GRAM_SR;
     const int wide = maxa > maxb ? maxa : maxb;
     if (wide <= 16)
-        hipLaunchKernelGGL((gram_kernel<T, 1>), dim3(nchunks, nwork), dim3(256), 0, st, dwork, per, rows);
+        hipLaunchKernelGGL((gram_kernel<T, 1>), dim3(nchunks, nwork, 1), dim3(256), 0, st, dwork, per, rows, 1);
     else if (wide <= 32)
-        hipLaunchKernelGGL((gram_kernel<T, 2>), dim3(nchunks, nwork), dim3(256), 0, st, dwork, per, rows);
-    else
-        hipLaunchKernelGGL((gram_kernel<T, 3>), dim3(nchunks, nwork), dim3(256), 0, st, dwork, per, rows);
+        hipLaunchKernelGGL((gram_kernel<T, 2>), dim3(nchunks, nwork, 1), dim3(256), 0, st, dwork, per, rows, 1);
+    else {  // 48-column tiles
+        const int nti = (maxa + GRAM_MAXC - 1) / GRAM_MAXC, ntj = (maxb + GRAM_MAXC - 1) / GRAM_MAXC;
+        hipLaunchKernelGGL((gram_kernel<T, 3>), dim3(nchunks, nwork, nti * ntj), dim3(256), 0, st, dwork, per, rows,
+                           ntj);
+    }
     BH_HIP_CHECK(hipGetLastError());
 }
 
@@ -397,7 +408,7 @@ __device__ __forceinline__ void lincomb_rows(const LincombWork<T> &w, const doub
     }
 }
 
-template <typename T, int LC_QG>
+template <typename T, int LC_QG, int NA>
 __global__ void __launch_bounds__(LC_NT) lincomb_kernel(const LincombWork<T> *__restrict__ work, long nrows) {
     const LincombWork<T> &w = work[blockIdx.y];
     const int na = w.dev_na ? *w.dev_na : w.na;
@@ -405,7 +416,8 @@ __global__ void __launch_bounds__(LC_NT) lincomb_kernel(const LincombWork<T> *__
     const int q0 = blockIdx.z * LC_QG;
     if (q0 >= nq) return;
     const int qn = nq - q0 < LC_QG ? nq - q0 : LC_QG;
-    __shared__ double2 Cs[GRAM_MAXC * LC_QG];  // [input i][output j], contiguous in j
+    if (na > NA) return;  // the launch sizes NA from the widest item
+    __shared__ double2 Cs[NA * LC_QG];  // [input i][output j], contiguous in j
     for (int e = threadIdx.x; e < na * LC_QG; e += LC_NT) {
         int i = e % na, j = e / na;
         double cr = 0, ci = 0;
@@ -428,25 +440,32 @@ __global__ void __launch_bounds__(LC_NT) lincomb_kernel(const LincombWork<T> *__
 }
 
 template <typename T>
-void launch_lincomb(const LincombWork<T> *dwork, int nwork, long rows, int maxq, hipStream_t st) {
+void launch_lincomb(const LincombWork<T> *dwork, int nwork, long rows, int maxq, hipStream_t st, int maxna) {
     if (nwork <= 0 || maxq <= 0) return;
     const unsigned rb = (unsigned)((rows + LC_NT - 1) / LC_NT);
+    if (maxna > GRAM_WIDE) throw std::runtime_error("linear combination of more than 128 columns");
+    if (maxna > GRAM_MAXC) {  // wide subspaces: 128-input coefficient block, 16 outputs per thread
+        hipLaunchKernelGGL((lincomb_kernel<T, 16, GRAM_WIDE>), dim3(rb, nwork, (maxq + 15) / 16), dim3(LC_NT), 0, st,
+                           dwork, rows);
+        BH_HIP_CHECK(hipGetLastError());
+        return;
+    }
     // Outputs per thread: 24 when every item has at most 24 (the Ritz update's
     // [X'|W'] = Z C with 2nb = 24 columns reads each input once), else 16.  A
     // 32-output variant measured slower: its register use leaves too few waves
     // to hide the column loads.
     if (maxq > 16 && maxq <= 24)
-        hipLaunchKernelGGL((lincomb_kernel<T, 24>), dim3(rb, nwork, 1), dim3(LC_NT), 0, st, dwork, rows);
+        hipLaunchKernelGGL((lincomb_kernel<T, 24, GRAM_MAXC>), dim3(rb, nwork, 1), dim3(LC_NT), 0, st, dwork, rows);
     else
-        hipLaunchKernelGGL((lincomb_kernel<T, 16>), dim3(rb, nwork, (maxq + 15) / 16), dim3(LC_NT), 0, st, dwork,
-                           rows);
+        hipLaunchKernelGGL((lincomb_kernel<T, 16, GRAM_MAXC>), dim3(rb, nwork, (maxq + 15) / 16), dim3(LC_NT), 0, st,
+                           dwork, rows);
     BH_HIP_CHECK(hipGetLastError());
 }
 
 template void launch_gram<double>(const GramWork<double> *, int, int, long, int, int, hipStream_t);
 template void launch_gram<float>(const GramWork<float> *, int, int, long, int, int, hipStream_t);
-template void launch_lincomb<double>(const LincombWork<double> *, int, long, int, hipStream_t);
-template void launch_lincomb<float>(const LincombWork<float> *, int, long, int, hipStream_t);
+template void launch_lincomb<double>(const LincombWork<double> *, int, long, int, hipStream_t, int);
+template void launch_lincomb<float>(const LincombWork<float> *, int, long, int, hipStream_t, int);
 
 // ---------------------------------------------------------------------------
 // Batched Hermitian eigensolver: one workgroup per matrix, parallel cyclic
@@ -1568,9 +1587,337 @@ __global__ void __launch_bounds__(NT) rr_kernel(const RRWork *__restrict__ work)
     }
 }
 
+// ---------------------------------------------------------------------------
+// Rayleigh-Ritz for wide subspaces (order 49..128: 32 bands give up to 96).
+// The same steps, thresholds and outputs as rr_kernel.  LDS holds one complex
+// matrix (149 KB at order 96, odd leading dimension): the scaled G1 and its
+// Cholesky factor in place, then G2, then the projected operator under
+// Jacobi.  The inverse factor's work matrix, T, the eigenvectors and X = G2 T
+// live in the item's device workspace (three matrices, L2-resident), which
+// the Jacobi rounds update as they update the eigenvectors.
+// ---------------------------------------------------------------------------
+constexpr int RRW_NT = 1024;
+
+size_t rr_wide_bytes(int n) {
+    const size_t ne = n + (n & 1), lda = ne + 1;
+    return sizeof(double2) * 3 * lda * ne;
+}
+
+static size_t rr_wide_shared_bytes(int maxn) {
+    const size_t ne = maxn + (maxn & 1), np = ne / 2, lda = ne + 1;
+    return sizeof(double2) * lda * ne + sizeof(JacobiRot) * 2 * np + sizeof(double) * 3 * ne + sizeof(int) * ne;
+}
+
+__global__ void __launch_bounds__(RRW_NT) rr_wide_kernel(const RRWork *__restrict__ work) {
+    constexpr int NT = RRW_NT;
+    const RRWork &w = work[blockIdx.x];
+    const int n = w.n;
+    const int tid = threadIdx.x;
+    if (n <= 0) {
+        if (tid == 0) { w.cnt[0] = 0; w.cnt[1] = 0; w.cnt[2] = 0; w.cnt[3] = 0; w.cnt[4] = 0; w.cnt[5] = 0; }
+        return;
+    }
+    const int ne = n + (n & 1);
+    const int np = ne / 2;
+    const int lda = ne + 1;
+    extern __shared__ __attribute__((aligned(16))) double rsm[];
+    double2 *A = reinterpret_cast<double2 *>(rsm);
+    JacobiRot *rot = reinterpret_cast<JacobiRot *>(A + lda * ne);
+    double *D = reinterpret_cast<double *>(rot + 2 * np);
+    double *wsh = D + ne;
+    double *piv = wsh + ne;
+    int *rank_of = reinterpret_cast<int *>(piv + ne);
+    __shared__ int rank, chol_ok;
+    __shared__ double msq, lmax;
+    __shared__ double red[1];
+    double2 *Tg = reinterpret_cast<double2 *>(w.ws);  // T (lda x ne)
+    double2 *Vg = Tg + lda * ne;                      // eigenvectors; the inverse's work matrix before
+    double2 *Xg = Vg + lda * ne;                      // G2 T
+    const int ld = w.ld;
+    int svqb_used = 0, fallback_sweeps = 0;
+    auto load = [&](const double *partial) {  // Hermitian n x n from lower partial sums, zero padding
+        for (int e = tid; e < ne * ne; e += NT) {
+            const int i = e % ne, j = e / ne;
+            double2 v = make_double2(0.0, 0.0);
+            if (i < n && j < n) {
+                const int a = i >= j ? i : j, b = i >= j ? j : i;
+                double re = 0.0, im = 0.0;
+                for (int c = 0; c < w.nchunks; ++c) {
+                    const auto *p = gmem(partial + 2 * ((size_t)c * ld * ld + (size_t)a * ld + b));
+                    re += p[0];
+                    im += p[1];
+                }
+                if (i == j) im = 0.0;
+                v = i >= j ? make_double2(re, im) : make_double2(re, -im);
+            }
+            A[i + j * lda] = v;
+        }
+    };
+    auto scale_g1 = [&] {
+        for (int e = tid; e < ne * ne; e += NT) {
+            const int i = e % ne, j = e / ne;
+            const double s = D[i] * D[j];
+            A[i + j * lda] = make_double2(A[i + j * lda].x * s, A[i + j * lda].y * s);
+        }
+    };
+    // 1. scaled G1
+    load(w.partial1);
+    __syncthreads();
+    for (int i = tid; i < ne; i += NT) {
+        const double nrm = i < n ? sqrt(fmax(A[i + i * lda].x, 0.0)) : 0.0;
+        D[i] = nrm > 1e-30 ? 1.0 / nrm : 1.0;
+    }
+    if (tid == 0) {
+        chol_ok = 1;
+        msq = 0.0;
+    }
+    __syncthreads();
+    scale_g1();
+    for (int e = tid; e < lda * ne; e += NT) Tg[e] = make_double2(0.0, 0.0);
+    __syncthreads();
+    // 2. right-looking Cholesky in place (L in the lower triangle of A), one
+    // barrier per column, as rr_kernel
+    constexpr int MAXE = (GRAM_WIDE * (GRAM_WIDE + 1) / 2 + NT - 1) / NT;
+    int own_i[MAXE], own_k[MAXE];
+    {
+        const int ntri = n * (n + 1) / 2;
+#pragma unroll
+        for (int u = 0; u < MAXE; ++u) {
+            const int e = tid + u * NT;
+            int i = -1, k = 0;
+            if (e < ntri) {
+                i = 0;
+                while ((i + 1) * (i + 2) / 2 <= e) ++i;
+                k = e - i * (i + 1) / 2;
+            }
+            own_i[u] = i;
+            own_k[u] = k;
+        }
+    }
+    for (int j = 0; j < n; ++j) {
+#pragma unroll
+        for (int u = 0; u < MAXE; ++u) {
+            const int i = own_i[u], k = own_k[u];
+            if (i < 0 || k < j) continue;
+            double2 a = A[i + k * lda];
+            if (j > 0) {
+                const double2 x = A[i + (j - 1) * lda], y = A[k + (j - 1) * lda];
+                cmacc(a.x, a.y, -y.x, -y.y, x.x, x.y);
+            }
+            if (k == j) {
+                double d = A[j + j * lda].x;
+                if (j > 0) {
+                    const double2 y = A[j + (j - 1) * lda];
+                    d -= y.x * y.x + y.y * y.y;
+                }
+                const double sd = d > 0.0 ? sqrt(d) : 1.0;
+                if (i == j) {
+                    piv[j] = sd;
+                    if (!(d > 1e-8)) chol_ok = 0;
+                } else {
+                    A[i + k * lda] = make_double2(a.x / sd, a.y / sd);
+                }
+            } else {
+                A[i + k * lda] = a;
+            }
+        }
+        __syncthreads();
+    }
+    if (chol_ok) {
+        // M = L^-1 right-looking with its work matrix B in Vg (starts as I);
+        // T[c][i] = D[c] conj(M[i][c]) as rows finish
+        for (int e = tid; e < lda * ne; e += NT) {
+            const int i = e % lda, j = e / lda;
+            Vg[e] = make_double2(i == j ? 1.0 : 0.0, 0.0);
+        }
+        __syncthreads();
+        double msq_own = 0.0;
+        for (int i = 0; i < n; ++i) {
+            const double inv = 1.0 / piv[i];
+#pragma unroll
+            for (int u = 0; u < MAXE; ++u) {
+                const int k = own_i[u], c = own_k[u];
+                if (k < i || c > i) continue;
+                const double2 b = Vg[i + c * lda];
+                const double2 m = make_double2(b.x * inv, b.y * inv);
+                if (k == i) {
+                    Tg[c + i * lda] = make_double2(D[c] * m.x, -D[c] * m.y);
+                    msq_own += m.x * m.x + m.y * m.y;
+                } else {
+                    const double2 l = A[k + i * lda];
+                    double2 bk = Vg[k + c * lda];
+                    cmac(bk.x, bk.y, -l.x, -l.y, m.x, m.y);
+                    Vg[k + c * lda] = bk;
+                }
+            }
+            __syncthreads();
+        }
+        // the gate of rr_kernel: ||L^-1||_F^2 n 1e-12 <= 1 proves SVQB keeps every direction
+        for (int o = 32; o > 0; o >>= 1) msq_own += __shfl_xor(msq_own, o);
+        if ((tid & 63) == 0) atomicAdd(&msq, msq_own);
+        __syncthreads();
+        if (tid == 0) {
+            if (msq * (double)n * 1e-12 > 1.0) chol_ok = 0;
+            else rank = n;
+            if (chol_ok && w.sweeps_out) atomicAdd(w.sweeps_out + 1, 1);
+        }
+        __syncthreads();
+    }
+    if (!chol_ok) {
+        // SVQB on the scaled G1, reloaded (the factorisation overwrote it)
+        svqb_used = 1;
+        load(w.partial1);
+        __syncthreads();
+        scale_g1();
+        for (int e = tid; e < lda * ne; e += NT) {
+            const int i = e % lda, j = e / lda;
+            Vg[e] = make_double2(i == j ? 1.0 : 0.0, 0.0);
+        }
+        __syncthreads();
+        const int sw = jacobi_lds<NT>(A, Vg, lda, ne, rot, red, RR_STOP_RATIO);
+        fallback_sweeps = sw;
+        for (int i = tid; i < n; i += NT) wsh[i] = A[i + i * lda].x;
+        __syncthreads();
+        for (int i = tid; i < n; i += NT) {
+            const double wi = wsh[i];
+            int rk = 0;
+            for (int j = 0; j < n; ++j) rk += (wsh[j] < wi || (wsh[j] == wi && j < i)) ? 1 : 0;
+            rank_of[i] = rk;
+        }
+        if (tid == 0) {
+            double m = 0.0;
+            for (int i = 0; i < n; ++i) m = fmax(m, wsh[i]);
+            lmax = m;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            const double thr = 1e-12 * lmax;
+            int r = 0;
+            for (int i = 0; i < n; ++i)
+                if (wsh[i] >= thr && wsh[i] > 0.0) ++r;
+            rank = r;
+            if (w.sweeps_out) {
+                atomicAdd(w.sweeps_out, sw);
+                atomicMax(w.sweeps_out + 2, sw);
+            }
+        }
+        __syncthreads();
+        for (int e = tid; e < n * n; e += NT) {
+            const int row = e % n, i = e / n;
+            const int pos = n - 1 - rank_of[i];
+            if (pos >= rank) continue;
+            const double sc = D[row] / sqrt(fmax(wsh[i], 1e-30));
+            const double2 v = Vg[row + i * lda];
+            Tg[row + pos * lda] = make_double2(v.x * sc, v.y * sc);
+        }
+    }
+    __syncthreads();
+    const int r = rank;
+    const int re = r + (r & 1);
+    // 3. X = G2 T (G2 in LDS), then M = T^H X into A, zero padded to re
+    load(w.partial2);
+    __syncthreads();
+    for (int e = tid; e < n * r; e += NT) {
+        const int i = e % n, c = e / n;
+        double xr = 0.0, xi = 0.0;
+        const int kmax = svqb_used ? n : (c + 1 < n ? c + 1 : n);  // T upper triangular on the Cholesky path
+        for (int k = 0; k < kmax; ++k) {
+            const double2 a = A[i + k * lda], t = Tg[k + c * lda];
+            cmac(xr, xi, a.x, a.y, t.x, t.y);
+        }
+        Xg[i + c * lda] = make_double2(xr, xi);
+    }
+    __syncthreads();
+    for (int e = tid; e < ne * ne; e += NT) {
+        const int a = e % ne, c = e / ne;
+        double mr = 0.0, mi = 0.0;
+        if (a < r && c < r) {
+            const int kmax = svqb_used ? n : (a + 1 < n ? a + 1 : n);
+            for (int k = 0; k < kmax; ++k) {  // conj(T[k][a]) X[k][c]
+                const double2 t = Tg[k + a * lda], x = Xg[k + c * lda];
+                cmacc(mr, mi, t.x, t.y, x.x, x.y);
+            }
+        }
+        A[a + c * lda] = make_double2(mr, mi);
+    }
+    __syncthreads();
+    for (int e = tid; e < r * r; e += NT) {  // Hermitian part
+        const int a = e % r, c = e / r;
+        if (a > c) continue;
+        if (a == c) {
+            A[a + a * lda].y = 0.0;
+            continue;
+        }
+        const double2 x = A[a + c * lda], y = A[c + a * lda];
+        const double2 h = make_double2(0.5 * (x.x + y.x), 0.5 * (x.y - y.y));
+        A[a + c * lda] = h;
+        A[c + a * lda] = make_double2(h.x, -h.y);
+    }
+    for (int e = tid; e < lda * ne; e += NT) {
+        const int i = e % lda, j = e / lda;
+        Vg[e] = make_double2(i == j && i < re ? 1.0 : 0.0, 0.0);
+    }
+    __syncthreads();
+    // 4. M = Y diag(theta) Y^H
+    const int sw = jacobi_lds<NT>(A, Vg, lda, re, rot, red, RR_STOP_RATIO);
+    if (tid == 0 && w.sweeps_out) {
+        atomicAdd(w.sweeps_out + 3, sw);
+        atomicMax(w.sweeps_out + 5, sw);
+    }
+    for (int i = tid; i < r; i += NT) wsh[i] = A[i + i * lda].x;
+    __syncthreads();
+    for (int i = tid; i < r; i += NT) {
+        const double wi = wsh[i];
+        int rk = 0;
+        for (int j = 0; j < r; ++j) rk += (wsh[j] < wi || (wsh[j] == wi && j < i)) ? 1 : 0;
+        rank_of[i] = rk;
+        w.lam[rk] = wi;
+    }
+    __syncthreads();
+    // 5. C = T Y for the first cx + cw Ritz vectors
+    const int cx = r < w.nsplit ? r : w.nsplit;
+    const int cw = (2 * w.nsplit < r ? 2 * w.nsplit : r) - cx;
+    for (int e = tid; e < n * r; e += NT) {
+        const int k = e % n, i = e / n;
+        const int col = rank_of[i];
+        if (col >= cx + cw) continue;
+        double cr = 0.0, ci = 0.0;
+        const int j0 = svqb_used ? 0 : k;  // T[k][j] = 0 for j < k on the Cholesky path
+        for (int j = j0; j < r; ++j) {
+            const double2 t = Tg[k + j * lda], y = Vg[j + i * lda];
+            cmac(cr, ci, t.x, t.y, y.x, y.y);
+        }
+        const size_t o = 2 * ((size_t)k + (size_t)col * w.ldc);
+        gmem(w.C)[o] = cr;
+        gmem(w.C)[o + 1] = ci;
+    }
+    if (tid == 0) {
+        w.cnt[0] = cx;
+        w.cnt[1] = cw;
+        w.cnt[2] = cx + cw;
+        w.cnt[3] = sw + fallback_sweeps;
+        w.cnt[4] = svqb_used;
+        w.cnt[5] = r;
+    }
+}
+
 void launch_rr(const RRWork *dwork, int nwork, int maxn, hipStream_t st) {
     if (nwork <= 0) return;
-    if (maxn > GRAM_MAXC) throw std::runtime_error("Rayleigh-Ritz dimension above 48");
+    if (maxn > GRAM_WIDE) throw std::runtime_error("Rayleigh-Ritz dimension above 128");
+    if (maxn > GRAM_MAXC) {
+        static const bool wattr = [] {
+            BH_HIP_CHECK(hipFuncSetAttribute((const void *)rr_wide_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                             (int)rr_wide_shared_bytes(GRAM_WIDE > 96 ? 96 : GRAM_WIDE)));
+            return true;
+        }();
+        (void)wattr;
+        const size_t sh = rr_wide_shared_bytes(maxn);
+        if (sh > rr_wide_shared_bytes(96))
+            throw std::runtime_error("Rayleigh-Ritz order above 96 exceeds the LDS of one workgroup");
+        hipLaunchKernelGGL(rr_wide_kernel, dim3(nwork), dim3(RRW_NT), sh, st, dwork);
+        BH_HIP_CHECK(hipGetLastError());
+        return;
+    }
     static const bool attr_set = [] {
         BH_HIP_CHECK(hipFuncSetAttribute((const void *)rr_kernel<RR_NT>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                          (int)std::max(rr_shared_bytes(GRAM_MAXC), RR_LDS_RESERVE)));

@@ -185,6 +185,7 @@ struct Group {
     Staging stg;
     double *part = nullptr, *gramrb = nullptr, *h_gramrb = nullptr;
     double *h_rel = nullptr;  // readback of the group's residual norms
+    double *rrws = nullptr;   // wide Rayleigh-Ritz workspace, one block per slot of the group
     int *sweeps = nullptr;
     int s0 = 0, s1 = 0;
     bool inflight = false;
@@ -296,12 +297,18 @@ class Engine : public EngineBase {
         N = (long)nx * ny;
         m0max = nbmax + 2;
         C = std::max(2 * m0max, 3 * nbmax);
-        if (C > GRAM_MAXC) throw std::runtime_error("n_bands too large for the 48-column subspace kernels");
+        // up to 16 bands (48 columns) the one-workgroup subspace kernels; up
+        // to 32 bands (96 columns) the tiled Grams and the wide Rayleigh-Ritz
+        if (C > 96) throw std::runtime_error("n_bands too large: at most 32 bands (96 subspace columns)");
+        wide = C > GRAM_MAXC;
         CV = C + 1;
         S = std::max(1, opt.slots);
-        LD = GRAM_MAXC;
         NRMAX = m0max + 2;
-        NCHMAX = (int)std::min<long>(64, std::max<long>(1, N / 32));
+        // leading dimension of the small matrices: the subspace width and the
+        // transition basis [E | H | R | y0] (2 nb + m0 - nb + 3 columns)
+        LD = wide ? ((std::max(C, 2 * nbmax + NRMAX + 1) + 15) / 16) * 16 : GRAM_MAXC;
+        NCHMAX = (int)std::min<long>(wide ? 16 : 64, std::max<long>(1, N / 32));
+        chunk_budget = wide ? 256 : 1024;
         if (opt.setup_stream) st = opt.setup_stream;
         else { BH_HIP_CHECK(hipStreamCreate(&st)); own_st = true; }
         V = dalloc<cplx<T>>((size_t)S * 2 * CV * N, st);
@@ -332,7 +339,8 @@ class Engine : public EngineBase {
             // per slot and step: dielectric tables of a new job plus descriptors / coefficients
             G.stg.init(((size_t)1 << 20) + (size_t)gs * (5 * (size_t)N * sizeof(T) + ((size_t)96 << 10)));
             // partial sums: two products (dual Gram) of up to 2048 chunk blocks
-            G.part = dalloc<double>((size_t)2 * (2048 + 2 * gs) * LD * LD * 2, st);
+            G.part = dalloc<double>((size_t)2 * (chunk_budget + 2 * gs) * LD * LD * 2, st);
+            if (wide) G.rrws = dalloc<double>((size_t)gs * rr_wide_bytes(C) / sizeof(double), st);
             G.gramrb = dalloc<double>((size_t)gs * LD * LD * 2, st);
             BH_HIP_CHECK(hipHostMalloc((void **)&G.h_gramrb, sizeof(double) * gs * LD * LD * 2, hipHostMallocDefault));
             BH_HIP_CHECK(hipHostMalloc((void **)&G.h_rel, sizeof(double) * gs * LD, hipHostMallocDefault));
@@ -378,6 +386,7 @@ class Engine : public EngineBase {
         for (auto &G : groups) {
             (void)hipStreamSynchronize(G.st);
             (void)hipFree(G.part);
+            if (G.rrws) (void)hipFree(G.rrws);
             (void)hipFree(G.gramrb);
             (void)hipFree(G.sweeps);
             (void)hipHostFree(G.h_gramrb);
@@ -403,6 +412,8 @@ class Engine : public EngineBase {
     int nx, ny;
     long N;
     int nbmax, m0max, C, CV, S, LD, NRMAX, NCHMAX;
+    bool wide = false;       // more than 48 subspace columns
+    int chunk_budget = 1024;  // Gram row chunks per launch (partial blocks)
     EngineOptions eopt;
     hipStream_t st{};
     bool own_st = false;
@@ -1139,6 +1150,7 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
     std::vector<OpWork<T>> op_p[2];    // Theta on the new directions [TM, TE]
     long theta_cols[2] = {0, 0};
     std::vector<Gram2Work<T>> g_dual;
+    std::vector<int> rr_slot;  // slot of each Rayleigh-Ritz item
     std::vector<RRWork> e_rr2;
     std::vector<LincombWork<T>> lc_ritz;
     int max_ritz = 0;
@@ -1303,6 +1315,7 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
         rw2.nsplit = sl.nb;
         if (eopt.profile) rw2.sweeps_out = d_sweeps;
         e_rr2.push_back(rw2);
+        rr_slot.push_back(s);
         sl.last_p = p;
         // Ritz update into the other copy: [X' | W'] = Z C, A[X' | W'] = (A Z) C,
         // B[X' | W'] = eps [X' | W'] (TM)
@@ -1412,7 +1425,7 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
     };
     // Gram partial buffers: item i of a launch uses partial block i
     auto nch_for = [&](size_t nwork) {
-        long c = (long)((1024 + nwork - 1) / std::max<size_t>(nwork, 1));
+        long c = (long)((chunk_budget + nwork - 1) / std::max<size_t>(nwork, 1));
         c = std::min<long>(std::max<long>(c, 1), NCHMAX);
         return (int)c;
     };
@@ -1427,6 +1440,28 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
         e_rr2[i].partial1 = g_dual[i].partial1;
         e_rr2[i].partial2 = g_dual[i].partial2;
         e_rr2[i].nchunks = nch_dual;
+        if (wide) e_rr2[i].ws = G.rrws + (size_t)(rr_slot[i] - G.s0) * (rr_wide_bytes(C) / sizeof(double));
+    }
+    // wide subspaces: the two Rayleigh-Ritz Grams as tiled Gram products
+    // (G1 = Z^H (B Z) with the B weight on the right operand, G2 = Z^H (A Z))
+    std::vector<GramWork<T>> g_wide;
+    if (wide) {
+        for (const Gram2Work<T> &d : g_dual) {
+            GramWork<T> g{};
+            g.A = d.Z;
+            g.B = d.Z;
+            g.a_stride = g.b_stride = d.stride;
+            g.weight = d.bweight;
+            g.na = g.nb = d.n;
+            g.ld = d.ld;
+            g.lower_only = 1;
+            g.partial = d.partial1;
+            g_wide.push_back(g);
+            g.B = d.AZ;
+            g.weight = nullptr;
+            g.partial = d.partial2;
+            g_wide.push_back(g);
+        }
     }
     // P-deflation masks
     unsigned char *mask_d = nullptr;
@@ -1500,7 +1535,7 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
         }
         launch_gram<T>(o.dg, (int)o.g.size(), o.nch, N, std::max(max_d, 1), maxa, st);
         launch_reduce(o.dr, (int)o.r.size(), LD, st);
-        launch_lincomb<T>(o.dl, (int)o.l.size(), N, maxa, st);
+        launch_lincomb<T>(o.dl, (int)o.l.size(), N, maxa, st, std::max(max_d, 1));
         launch_bnorm<T>(o.dn, (int)o.n.size(), maxa, N, st);
     };
     auto *d_lc_hist = put(lc_hist);
@@ -1513,6 +1548,7 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
     auto *d_defP = put(defP);
     OpWork<T> *d_op_p[2] = {put(op_p[0]), put(op_p[1])};
     auto *d_g_dual = put(g_dual);
+    auto *d_g_wide = put(g_wide);
     auto *d_e_rr2 = put(e_rr2);
     auto *d_lc_ritz = put(lc_ritz);
     auto *d_g_t1 = put(g_t1);
@@ -1546,7 +1582,7 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
     const double n_log = (double)N * std::log2((double)N);
     // T2
     launch_lincomb<T>(d_lc_hist, (int)lc_hist.size(), N, nbmax, st);
-    launch_lincomb<T>(d_lc_x, (int)lc_x.size(), N, std::max(maxq_x, 1), st);
+    launch_lincomb<T>(d_lc_x, (int)lc_x.size(), N, std::max(maxq_x, 1), st, LD);
     for (int pol = 0; pol < 2; ++pol) {
         if (op_x[pol].empty()) continue;
         long cols = 0;
@@ -1597,7 +1633,10 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
             flops += 2.0 * 8.0 * (double)N * g2.n * (g2.n + 1) / 2.0;
         }
         timed(TIMED_GRAM2, bytes, flops,
-              [&] { launch_gram2<T>(d_g_dual, (int)g_dual.size(), nch_dual, N, std::max(max_p, 1), st); });
+              [&] {
+                  if (wide) launch_gram<T>(d_g_wide, (int)g_wide.size(), nch_dual, N, max_p, max_p, st);
+                  else launch_gram2<T>(d_g_dual, (int)g_dual.size(), nch_dual, N, std::max(max_p, 1), st);
+              });
     }
     {
         double bytes = 0;  // the two n x n Grams in, the n x 2nb coefficients out
@@ -1616,7 +1655,7 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
             flops += 8.0 * N * l.na * nq;
         }
         timed(TIMED_RITZ, bytes, flops,
-              [&] { launch_lincomb<T>(d_lc_ritz, (int)lc_ritz.size(), N, std::max(max_ritz, 1), st); });
+              [&] { launch_lincomb<T>(d_lc_ritz, (int)lc_ritz.size(), N, std::max(max_ritz, 1), st, max_p); });
     }
     // T1 Grams -> compact readback
     if (!g_t1.empty()) {

@@ -85,7 +85,7 @@ int bh_lincomb(bh_context *ctx, const bh_buffer *q, int qi0, int r, const double
  * col-major eigenvectors.  dense::solve_hermitian_eigen (eigensolver/dense.rs:71-134). */
 int bh_hermitian_eigh(bh_context *ctx, int n, int count, const double *a, double *w, double *v);
 
-/* Rayleigh-Ritz of `count` subspaces Z of width n (n <= 48) from their Grams,
+/* Rayleigh-Ritz of `count` subspaces Z of width n (n <= 96) from their Grams,
  * FP64 on the device.  g1 = Z^H B Z and g2 = Z^H A Z, col-major n x n complex
  * per problem (lower triangles read).  The block is B-orthonormalised as
  * svqb_orthonormalize does (normalization.rs:281-544: unit B-norm columns,

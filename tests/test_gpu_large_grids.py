@@ -138,3 +138,39 @@ def test_cfg5_512_tm_16_bands_at_m(be):
           f"oracle {ref['iterations']}")
     assert res.converged.all()
     assert err <= 1e-6
+
+
+@pytest.mark.parametrize("rule", [0, 1])
+@pytest.mark.parametrize("pol", ["TM", "TE"])
+def test_bandstructure_32_bands(be, pol, rule):
+    # 32 bands: a 96-column subspace (tiled Grams, wide Rayleigh-Ritz, the
+    # row-chunked deflation against up to 33 vectors), on a 32 x 32 grid of the
+    # large case's geometry along a 7-point path, against the live oracle
+    kp = B.k_path("square", 2)
+    atoms = [(0.0, 0.0, 0.2, 1.0), (0.5, 0.5, 0.12, 1.0)]
+    geom = B.Geometry2D(B.Lattice2D.square(), 12.0, [B.BasisAtom((a[0], a[1]), a[2], a[3]) for a in atoms])
+    job = B.BandStructureJob(geom, 32, 32, pol, kp, n_bands=32, tol=1e-7)
+    (res,), st = B.run_batch(be, [job], B.RunOptions(exit_rule=rule), slots=1)
+    om, it, cv, tot = O.bandstructure(O.Job(lattice="square", eps_bg=12.0, atoms=atoms, nx=32, ny=32, pol=pol,
+                                            n_bands=32, tol=1e-7, kpath=kp, exit_rule=rule))
+    err = _rel(res.bands, om)
+    print(f"32x32 32 bands {pol} rule {rule}: max rel omega diff {err:.3e}, iterations gpu "
+          f"{res.iterations.sum()} oracle {it.sum()}; {st.rr_problems} Rayleigh-Ritz problems, "
+          f"{st.svqb_fallbacks} on the SVQB path")
+    assert res.converged.all() and cv.all()
+    assert err <= 1e-6
+
+
+def test_cfg5_256_32_bands_tm_te(be):
+    # the large combined case at 256 x 256, 32 bands, TM and TE at three
+    # k-points, batched over k-points in one run, against the oracle fixture
+    cases = cfg5_cases(256, 32)
+    assert len(cases) == 6
+    jobs = [c[0] for c in cases]
+    res, st = B.run_batch(be, jobs, B.RunOptions(exit_rule=1), slots=len(jobs))
+    for (job, ref), r in zip(cases, res):
+        err = _rel(r.bands, np.array([ref["omegas"]]))
+        print(f"256x256 32 bands {ref['pol']} k={ref['k']}: max rel omega diff {err:.3e}, iterations gpu "
+              f"{r.iterations[0]} oracle {ref['iterations']}")
+        assert r.converged.all()
+        assert err <= 1e-6

@@ -41,6 +41,13 @@ def make_case(kind, n, length, seed):
         # triangular with -1 above the diagonal (condition ~ 2^n): the scaled
         # Cholesky pivots are 1/(j+1) while lambda_min / lambda_max ~ 1e-15 at n = 24
         z = (q / np.sqrt(w)[:, None]) @ (np.eye(n) - np.triu(np.ones((n, n)), 1))
+    elif kind == "kahan_wide":
+        # the n = 24 Kahan block beside a well-conditioned block: the wide
+        # kernel (order above 48) must also take the SVQB path and drop one direction
+        rb = np.eye(n, dtype=complex)
+        rb[:24, :24] -= np.triu(np.ones((24, 24)), 1)
+        rb[24:, 24:] += 0.3 * (r.standard_normal((n - 24, n - 24)) + 1j * r.standard_normal((n - 24, n - 24)))
+        z = (q / np.sqrt(w)[:, None]) @ rb
     else:  # repeated column
         z = q @ (np.eye(n) + 0.3 * (r.standard_normal((n, n)) + 1j * r.standard_normal((n, n))))
         z[:, n - 1] = z[:, 3]
@@ -69,14 +76,19 @@ def scaled_gram_facts(g1):
     ("random", 36, 0, 36),
     ("kahan", 24, 1, 23),
     ("repeat", 30, 1, 29),
+    # orders above 48: the wide kernel (T, eigenvectors and G2 T in device workspace)
+    ("random", 96, 0, 96),
+    ("random", 61, 0, 61),
+    ("kahan_wide", 80, 1, 79),
+    ("repeat", 90, 1, 89),
 ])
 def test_rayleigh_ritz_matches_oracle_svqb(be, kind, n, expect_svqb, expect_rank):
-    nsplit = 12
+    nsplit = 12 if n <= 48 else 32
     z, w, d = make_case(kind, n, 600, 11 + n)
     g1 = z.conj().T @ (w[:, None] * z)
     g2 = z.conj().T @ (d[:, None] * z)
     pmin, ratio = scaled_gram_facts(g1)
-    if kind == "kahan":  # the case the pivot test alone gets wrong
+    if kind in ("kahan", "kahan_wide"):  # the case the pivot test alone gets wrong
         assert pmin > 1e-8 and ratio < 1e-12, (pmin, ratio)
     lam, coef, info = be.rayleigh_ritz(g1, g2, nsplit)
     rank, svqb, cx, cw = (int(v) for v in info[0])
