@@ -130,6 +130,7 @@ def main():
                     help="band structures timed on the CPU oracle, one per thread (16 threads at most)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip roofline / plain-GPU extra passes")
+    ap.add_argument("--no-large", action="store_true", help="skip the large combined case (256x256 x 32 bands, 512x512)")
     ap.add_argument("--timed-kernel", type=int, default=3, choices=(1, 2, 3, 4, 5),
                     help="kernel class timed for the roofline: 1 Theta operator, 2 dual Rayleigh-Ritz Gram, "
                          "3 Rayleigh-Ritz solve (the dominant kernel), 4 Ritz-update linear combinations")
@@ -347,6 +348,70 @@ def main():
                                      "lobpcg_iterations": single[2][2],
                                      "converged": int(single[2][0].converged.sum())},
             "max_rel_freq_diff_per_band": [float(f"{e:.3e}") for e in per_band]}
+        # the large combined case: 256 x 256, 32 bands, TE + TM, batched over the
+        # 63 distinct k-points of the 64-point Gamma-X-M-Gamma path (126 solves,
+        # each from the deterministic start), all bands converged; then the
+        # 512 x 512 TM 16-band solve at M.  Timed twice: the multi-pass FFT
+        # operator class (Theta) and the Rayleigh-Ritz Gram class, each with HIP
+        # events on its stream; frequencies at M checked against the oracle
+        # fixture.
+        if not args.no_large:
+            import json as _json
+            from blaze2d_amd.sweep import large_case_jobs, large_case_m_point
+            lj = large_case_jobs()
+            lopts = RunOptions(exit_rule=1)
+            run_batch(be, lj[:2], lopts, slots=2)  # first launches of the 256 x 256 kernels
+            large = {}
+            for cls in (1, 2):
+                t = time.perf_counter()
+                lres, lst = run_batch(be, lj, lopts, slots=min(64, len(lj)), timed_kernel=cls)
+                dt = time.perf_counter() - t
+                large[cls] = (lres, lst, dt)
+            lres, lst, dt = large[1]
+            conv_l = sum(int(r.converged.sum()) for r in lres)
+            ths = large[1][1]
+            grs = large[2][1]
+            th_avg = ths.timed_ms * 1e-3 / max(ths.timed_launches, 1)
+            gr_avg = grs.timed_ms * 1e-3 / max(grs.timed_launches, 1)
+            with open(os.path.join(ROOT, "tests", "golden", "cfg5_oracle.json")) as f:
+                fx = _json.load(f)
+            errs = []
+            for c in fx["cases"]:
+                if c["grid"] == 256 and c["k"] == [0.5, 0.5]:
+                    i = next(i for i, j in enumerate(lj) if j.pol == c["pol"] and
+                             abs(j.k_path[0][0] - 0.5) < 1e-12 and abs(j.k_path[0][1] - 0.5) < 1e-12)
+                    errs.append(rel_err(lres[i].bands, np.array([c["omegas"]])))
+            mj = large_case_m_point()
+            run_batch(be, [mj], lopts, slots=1)
+            t = time.perf_counter()
+            (mres,), mst = run_batch(be, [mj], lopts, slots=1, timed_kernel=1)
+            dtm5 = time.perf_counter() - t
+            mref = next(c for c in fx["cases"] if c["grid"] == 512)
+            out["large_case"] = {
+                "workload": "square eps_bg 12, eps 1 rods r 0.2 at (0,0) and r 0.12 at (1/2,1/2); 256x256, 32 bands, "
+                            "TE+TM, 63 distinct k-points of the 64-point Gamma-X-M-Gamma path batched over k, "
+                            "tol 1e-7, all bands converged, FP64",
+                "value": round(conv_l / dt, 3), "unit": "k-point solves/s", "solves": len(lj),
+                "converged": conv_l, "seconds": round(dt, 3),
+                "lobpcg_iterations_per_solve": round(lst.iterations / max(len(lj), 1), 3),
+                "max_rel_freq_err_vs_fp64_oracle_at_M": float(max(errs)) if errs else None,
+                "theta_multipass": {
+                    "avg_launch_ms": round(th_avg * 1e3, 4), "launches": ths.timed_launches,
+                    "algorithmic_bytes_per_launch": int(ths.timed_bytes / max(ths.timed_launches, 1)),
+                    "achieved_gbs": round(ths.timed_bytes / max(ths.timed_ms * 1e-3, 1e-12) / 1e9, 1),
+                    "frac_hbm": round(ths.timed_bytes / max(ths.timed_ms * 1e-3, 1e-12) / 1e9 / HBM_PEAK_GBS, 4)},
+                "rayleigh_ritz_gram": {
+                    "avg_launch_ms": round(gr_avg * 1e3, 4), "launches": grs.timed_launches,
+                    "algorithmic_flops_per_launch": int(grs.timed_flops / max(grs.timed_launches, 1)),
+                    "achieved_tflops_fp64": round(grs.timed_flops / max(grs.timed_ms * 1e-3, 1e-12) / 1e12, 3),
+                    "frac_fp64_peak": round(grs.timed_flops / max(grs.timed_ms * 1e-3, 1e-12) / 1e12
+                                            / FP64_PEAK_TFLOPS, 4)},
+                "m_point_512": {
+                    "workload": "512x512, TM, 16 bands, k = M, one solve",
+                    "seconds": round(dtm5, 3), "iterations": int(mres.iterations[0]),
+                    "converged": int(mres.converged[0]),
+                    "max_rel_freq_err_vs_fp64_oracle": rel_err(mres.bands, np.array([mref["omegas"]])),
+                    "theta_achieved_gbs": round(mst.timed_bytes / max(mst.timed_ms * 1e-3, 1e-12) / 1e9, 1)}}
         # the reference's own exit accounting on the same sweep (one pass)
         if args.exit_rule != 0:
             t = time.perf_counter()

@@ -80,3 +80,30 @@ def sweep_jobs(n_geometries: int, seed: int = DEFAULT_SEED, grid: Optional[int] 
 def res64_sweep(n_geometries: int, seed: int = DEFAULT_SEED, tol: float = 1e-7) -> List[BandStructureJob]:
     """The headline workload: the sweep generator fixed at 64 x 64 and 12 bands."""
     return sweep_jobs(n_geometries, seed, grid=64, n_bands=12, tol=tol)
+
+
+LARGE_ATOMS = [((0.0, 0.0), 0.2, 1.0), ((0.5, 0.5), 0.12, 1.0)]
+
+
+def large_case_geometry() -> Geometry2D:
+    """The large combined case: square lattice, eps_bg = 12, eps = 1 rods of
+    r = 0.2 a at (0, 0) and r = 0.12 a at (0.5, 0.5)."""
+    return Geometry2D(Lattice2D.square(), 12.0, [BasisAtom(p, r, e) for p, r, e in LARGE_ATOMS])
+
+
+def large_case_jobs(grid: int = 256, n_bands: int = 32, segments_per_leg: int = 21, tol: float = 1e-7,
+                    pols=("TE", "TM")) -> List[BandStructureJob]:
+    """The large combined case batched over k-points: one job per (polarisation,
+    k-point) of the Gamma-X-M-Gamma path (21 points per leg: 64 points, the
+    closing Gamma a copy of the first, so 63 solves per polarisation), each
+    solved from the deterministic start."""
+    geom = large_case_geometry()
+    kp = k_path("square", segments_per_leg)[:-1]
+    return [BandStructureJob(geom, grid, grid, pol, kp[i:i + 1], n_bands=n_bands, tol=tol, max_iter=400)
+            for pol in pols for i in range(kp.shape[0])]
+
+
+def large_case_m_point(grid: int = 512, n_bands: int = 16, tol: float = 1e-7) -> BandStructureJob:
+    """The 512 x 512 TM 16-band solve at M = (1/2, 1/2) of the large case."""
+    return BandStructureJob(large_case_geometry(), grid, grid, "TM", np.array([[0.5, 0.5]]), n_bands=n_bands,
+                            tol=tol, max_iter=400)
