@@ -60,6 +60,9 @@ class Job(ctypes.Structure):
         ("exit_rule", ctypes.c_int),
         ("nk", ctypes.c_int),
         ("kpath", c_dp),
+        ("warm_pw", c_dp),
+        ("warm_count", ctypes.c_int),
+        ("pw_half", ctypes.c_int),
     ]
 
 
@@ -87,6 +90,15 @@ class RunStats(ctypes.Structure):
         ("rr_problems", ctypes.c_long),
         ("svqb_fallbacks", ctypes.c_long),
         ("rank_drops", ctypes.c_long),
+    ]
+
+
+class RunOutputs(ctypes.Structure):
+    _fields_ = [
+        ("rel_residuals", c_dp),
+        ("rel_residual_iters", c_ip),
+        ("pw_half", ctypes.c_int),
+        ("pw_coefs", c_dp),
     ]
 
 
@@ -157,7 +169,7 @@ def lib() -> ctypes.CDLL:
     L.bh_bandstructure_run.argtypes = [vp, ctypes.POINTER(Job), ctypes.c_int, ctypes.POINTER(RunOptions), c_dp, c_ip,
                                        c_ip, ctypes.POINTER(RunStats)]
     L.bh_bandstructure_run_ex.argtypes = [vp, ctypes.POINTER(Job), ctypes.c_int, ctypes.POINTER(RunOptions), c_dp,
-                                          c_ip, c_ip, c_dp, c_ip, ctypes.POINTER(RunStats)]
+                                          c_ip, c_ip, ctypes.POINTER(RunOutputs), ctypes.POINTER(RunStats)]
     L.bh_operator_diagnostics.argtypes = [vp, ctypes.POINTER(Job), ctypes.c_int, ctypes.c_int,
                                           ctypes.POINTER(Diagnostics), c_dp]
     _lib = L

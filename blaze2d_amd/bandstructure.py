@@ -109,6 +109,10 @@ class BandStructureJob:
     max_iter: int = 200
     block_size: int = 0
     smoothing: bool = True
+    # learned start: warm vectors for the first k-point as plane-wave
+    # coefficients [count, (2 pw_half + 1)^2] complex (index (n2 + h)(2h + 1) + n1 + h)
+    warm_pw: Optional[np.ndarray] = None
+    pw_half: int = 0
 
     def to_c(self, opts: RunOptions) -> L.Job:
         j = L.Job()
@@ -131,6 +135,12 @@ class BandStructureJob:
             raise ValueError("k_path must be an (nk, 2) array")
         j.nk = self._kp.shape[0]
         j.kpath = self._kp.ctypes.data_as(L.c_dp)
+        if self.warm_pw is not None:
+            ng = (2 * self.pw_half + 1) ** 2
+            self._warm = np.ascontiguousarray(self.warm_pw, dtype=np.complex128).reshape(-1, ng)
+            j.warm_pw = self._warm.view(np.float64).ctypes.data_as(L.c_dp)
+            j.warm_count = self._warm.shape[0]
+            j.pw_half = self.pw_half
         return j
 
 
@@ -145,6 +155,9 @@ class BandStructureResult:
     # that iteration (1-based)
     relative_residuals: Optional[np.ndarray] = None   # [nk, n_bands]
     residual_iteration: Optional[np.ndarray] = None   # [nk]
+    # with run_batch(pw_half=h): the converged eigenvectors (ascending eigenvalue)
+    # on the plane waves |n1|, |n2| <= h: [nk, n_bands, (2h + 1)^2] complex
+    pw_coefs: Optional[np.ndarray] = None
 
 
 @dataclass
@@ -169,7 +182,8 @@ class BatchStats:
 
 
 def run_batch(backend: HipBackend, jobs: Sequence[BandStructureJob], options: Optional[RunOptions] = None,
-              slots: int = 64, precision: int = 0, profile: bool = False, groups: int = 4, timed_kernel: int = 0):
+              slots: int = 64, precision: int = 0, profile: bool = False, groups: int = 4, timed_kernel: int = 0,
+              pw_half: int = 0):
     """Runs many band structures concurrently on one GPU (all on one grid).
 
     `slots` band structures are resident at once; they are split into `groups`
@@ -192,12 +206,15 @@ def run_batch(backend: HipBackend, jobs: Sequence[BandStructureJob], options: Op
     cv = np.zeros(tot_k, dtype=np.int32)
     rel = np.zeros(tot_om)
     rit = np.zeros(tot_k, dtype=np.int32)
+    ng = (2 * pw_half + 1) ** 2
+    pw = np.zeros(tot_om * ng if pw_half > 0 else 1, dtype=np.complex128)
     ro = L.RunOptions(slots, precision, int(profile), groups, timed_kernel)
     st = L.RunStats()
+    extra = L.RunOutputs(rel.ctypes.data_as(L.c_dp), rit.ctypes.data_as(L.c_ip), pw_half,
+                         pw.view(np.float64).ctypes.data_as(L.c_dp) if pw_half > 0 else None)
     L.check(L.lib().bh_bandstructure_run_ex(backend.handle, arr, len(jobs), ctypes.byref(ro),
                                             om.ctypes.data_as(L.c_dp), it.ctypes.data_as(L.c_ip),
-                                            cv.ctypes.data_as(L.c_ip), rel.ctypes.data_as(L.c_dp),
-                                            rit.ctypes.data_as(L.c_ip), ctypes.byref(st)),
+                                            cv.ctypes.data_as(L.c_ip), ctypes.byref(extra), ctypes.byref(st)),
             "bh_bandstructure_run_ex")
     out = []
     o = k = 0
@@ -206,7 +223,9 @@ def run_batch(backend: HipBackend, jobs: Sequence[BandStructureJob], options: Op
         out.append(BandStructureResult(j.k_path, om[o:o + nk * j.n_bands].reshape(nk, j.n_bands).copy(),
                                        it[k:k + nk].copy(), cv[k:k + nk].copy(),
                                        rel[o:o + nk * j.n_bands].reshape(nk, j.n_bands).copy(),
-                                       rit[k:k + nk].copy()))
+                                       rit[k:k + nk].copy(),
+                                       pw[o * ng:(o + nk * j.n_bands) * ng].reshape(nk, j.n_bands, ng).copy()
+                                       if pw_half > 0 else None))
         o += nk * j.n_bands
         k += nk
     stats = BatchStats(st.steps, st.iterations, st.solves, st.seconds, st.timed_ms, st.timed_launches,

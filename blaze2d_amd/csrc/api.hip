@@ -1012,13 +1012,17 @@ int bh_operator_diagnostics(bh_context *ctx, const bh_job *jobs, int njobs, int 
 
 int bh_bandstructure_run(bh_context *ctx, const bh_job *jobs, int njobs, const bh_run_options *opts, double *omegas,
                          int *iters, int *converged, bh_run_stats *stats) {
-    return bh_bandstructure_run_ex(ctx, jobs, njobs, opts, omegas, iters, converged, nullptr, nullptr, stats);
+    return bh_bandstructure_run_ex(ctx, jobs, njobs, opts, omegas, iters, converged, nullptr, stats);
 }
 
 int bh_bandstructure_run_ex(bh_context *ctx, const bh_job *jobs, int njobs, const bh_run_options *opts,
-                            double *omegas, int *iters, int *converged, double *rel_residuals,
-                            int *rel_residual_iters, bh_run_stats *stats) {
+                            double *omegas, int *iters, int *converged, const bh_run_outputs *extra,
+                            bh_run_stats *stats) {
     BH_API_TRY
+    double *rel_residuals = extra ? extra->rel_residuals : nullptr;
+    int *rel_residual_iters = extra ? extra->rel_residual_iters : nullptr;
+    const int pw_half = extra && extra->pw_coefs ? extra->pw_half : 0;
+    if (pw_half < 0 || pw_half > 8) return fail("bh_bandstructure_run_ex: pw_half must be 0..8");
     if (!ctx || (!jobs && njobs > 0) || njobs < 0) return fail("bh_bandstructure_run: bad arguments");
     if (njobs == 0) return 0;
     std::vector<JobSpec> js(njobs);
@@ -1051,6 +1055,13 @@ int bh_bandstructure_run_ex(bh_context *ctx, const bh_job *jobs, int njobs, cons
         s.reuse_gamma = b.reuse_gamma;
         s.exit_rule = b.exit_rule;
         for (int k = 0; k < b.nk; ++k) s.kpath.push_back({b.kpath[2 * k], b.kpath[2 * k + 1]});
+        if (b.warm_pw && b.warm_count > 0) {
+            if (b.pw_half < 0 || b.pw_half > 8) return fail("bh_bandstructure_run: warm pw_half must be 0..8");
+            const size_t ngw = (size_t)(2 * b.pw_half + 1) * (2 * b.pw_half + 1);
+            s.warm_pw.assign(b.warm_pw, b.warm_pw + 2 * ngw * b.warm_count);
+            s.warm_count = b.warm_count;
+            s.pw_half = b.pw_half;
+        }
         nbmax = std::max(nbmax, std::max(b.n_bands, b.block_size > 0 ? b.block_size - 2 : 0));
     }
     if (!grid_supported(js[0].nx, js[0].ny)) return fail("bh_bandstructure_run: grid size not instantiated");
@@ -1062,6 +1073,7 @@ int bh_bandstructure_run_ex(bh_context *ctx, const bh_job *jobs, int njobs, cons
     eo.profile = opts ? opts->profile : 0;
     eo.groups = opts && opts->groups > 0 ? opts->groups : 4;
     eo.timed_kernel = opts ? opts->timed_kernel : 0;
+    eo.pw_half = pw_half;
     if (eo.timed_kernel < 0 || eo.timed_kernel > 5) return fail("bh_bandstructure_run: timed_kernel must be 0..5");
     eo.setup_stream = ctx->st;
     eo.group_streams[0] = ctx->st;
@@ -1079,6 +1091,10 @@ int bh_bandstructure_run_ex(bh_context *ctx, const bh_job *jobs, int njobs, cons
         if (omegas) std::memcpy(omegas + off, r.omegas.data(), sizeof(double) * r.omegas.size());
         if (rel_residuals)
             std::memcpy(rel_residuals + off, r.rel_residuals.data(), sizeof(double) * r.rel_residuals.size());
+        if (pw_half > 0) {
+            const size_t ng2 = (size_t)2 * (2 * pw_half + 1) * (2 * pw_half + 1);
+            std::memcpy(extra->pw_coefs + off * ng2, r.pw.data(), sizeof(double) * r.pw.size());
+        }
         for (size_t k = 0; k < r.iters.size(); ++k) {
             if (iters) iters[koff + k] = r.iters[k];
             if (converged) converged[koff + k] = r.converged[k];

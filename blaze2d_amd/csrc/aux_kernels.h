@@ -47,6 +47,24 @@ struct NormWork {
     int count, pad;
 };
 
+// Low plane waves G = n1 b1 + n2 b2, |n1|, |n2| <= h, index
+// g = (n2 + h)(2h + 1) + n1 + h, of fields on an nx x ny node grid.
+// Projection: c_g = (1/N) sum_r x(r) exp(-2 pi i (n1 ix / nx + n2 iy / ny));
+// synthesis:  x(r) = sum_g c_g exp(+2 pi i (n1 ix / nx + n2 iy / ny)).
+template <typename T>
+struct PwWork {
+    const cplx<T> *x;     // projection: count columns, stride apart
+    double *c;            // projection output: count x (2h+1)^2 complex
+    const double *cin;    // synthesis input: count x (2h+1)^2 complex
+    cplx<T> *out;         // synthesis output: count columns, stride apart
+    long stride;
+    int count, pad;
+};
+template <typename T>
+void launch_pw_project(const PwWork<T> *dwork, int nwork, int maxcount, int nx, int ny, int h, hipStream_t st);
+template <typename T>
+void launch_pw_synth(const PwWork<T> *dwork, int nwork, int maxcount, int nx, int ny, int h, hipStream_t st);
+
 template <typename T>
 void launch_residual(const ResidualWork<T> *dwork, int nwork, int maxcount, long rows, hipStream_t st);
 template <typename T>

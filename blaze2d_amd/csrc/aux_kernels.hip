@@ -45,6 +45,109 @@ void launch_residual(const ResidualWork<T> *dwork, int nwork, int maxcount, long
     BH_HIP_CHECK(hipGetLastError());
 }
 
+// grid = (columns, items, 2h + 1 values of n2); each thread accumulates the
+// 2h + 1 coefficients of its n2 over a stride of rows, then a workgroup reduction
+constexpr int PW_MAXH = 8;
+template <typename T>
+__global__ void __launch_bounds__(256) pw_project_kernel(const PwWork<T> *__restrict__ work, int nx, int ny,
+                                                         int h) {
+    const PwWork<T> &w = work[blockIdx.y];
+    const int c = blockIdx.x;
+    if (c >= w.count) return;
+    const int n2 = (int)blockIdx.z - h, nw = 2 * h + 1;
+    const cplx<T> *x = w.x + (size_t)c * w.stride;
+    double ar[2 * PW_MAXH + 1], ai[2 * PW_MAXH + 1];
+    for (int j = 0; j < nw; ++j) ar[j] = ai[j] = 0.0;
+    const long n = (long)nx * ny;
+    for (long r = threadIdx.x; r < n; r += 256) {
+        const int iy = (int)(r / nx), ix = (int)(r - (long)iy * nx);
+        const cplx<T> v = gload(x, r);
+        // v exp(-2 pi i n2 iy / ny), then times exp(-2 pi i n1 ix / nx) for n1 = -h..h
+        double sy, cy, s1, c1, s0, c0;
+        sincospi(-2.0 * n2 * iy / ny, &sy, &cy);
+        sincospi(-2.0 * ix / nx, &s1, &c1);       // step exp(-2 pi i ix / nx)
+        sincospi(2.0 * h * ix / nx, &s0, &c0);    // start exp(+2 pi i h ix / nx) (n1 = -h)
+        double br = (double)v.x * cy - (double)v.y * sy, bi = (double)v.x * sy + (double)v.y * cy;
+        double pr = c0, pi = s0;
+        for (int j = 0; j < nw; ++j) {
+            cmac(ar[j], ai[j], br, bi, pr, pi);
+            const double t = pr * c1 - pi * s1;
+            pi = pr * s1 + pi * c1;
+            pr = t;
+        }
+    }
+    __shared__ double red[4][2 * (2 * PW_MAXH + 1)];
+    for (int j = 0; j < nw; ++j) {
+        double vr = ar[j], vi = ai[j];
+        for (int o = 32; o > 0; o >>= 1) {
+            vr += __shfl_xor(vr, o);
+            vi += __shfl_xor(vi, o);
+        }
+        if ((threadIdx.x & 63) == 0) {
+            red[threadIdx.x >> 6][2 * j] = vr;
+            red[threadIdx.x >> 6][2 * j + 1] = vi;
+        }
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < 2 * nw) {
+        const double t = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+        const int j = threadIdx.x >> 1;
+        w.c[2 * ((size_t)c * nw * nw + (size_t)(n2 + h) * nw + j) + (threadIdx.x & 1)] = t / (double)n;
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) pw_synth_kernel(const PwWork<T> *__restrict__ work, int nx, int ny, int h) {
+    const PwWork<T> &w = work[blockIdx.z];
+    const int c = blockIdx.y;
+    if (c >= w.count) return;
+    const long n = (long)nx * ny;
+    const long r = (long)blockIdx.x * 256 + threadIdx.x;
+    if (r >= n) return;
+    const int iy = (int)(r / nx), ix = (int)(r - (long)iy * nx);
+    const int nw = 2 * h + 1;
+    const double *cf = w.cin + 2 * (size_t)c * nw * nw;
+    double sr = 0.0, si = 0.0;
+    double s1, c1, s2, c2;
+    sincospi(2.0 * ix / nx, &s1, &c1);
+    for (int b = 0; b < nw; ++b) {
+        const int n2 = b - h;
+        double sy, cy, s0, c0;
+        sincospi(2.0 * n2 * iy / ny, &sy, &cy);
+        sincospi(-2.0 * h * ix / nx, &s0, &c0);
+        // e = exp(+2 pi i (n1 ix / nx + n2 iy / ny)), n1 from -h
+        double er = c0 * cy - s0 * sy, ei = c0 * sy + s0 * cy;
+        for (int a = 0; a < nw; ++a) {
+            const double gr = cf[2 * (b * nw + a)], gi = cf[2 * (b * nw + a) + 1];
+            cmac(sr, si, gr, gi, er, ei);
+            const double t = er * c1 - ei * s1;
+            ei = er * s1 + ei * c1;
+            er = t;
+        }
+    }
+    (void)s2;
+    (void)c2;
+    gstore(w.out + (size_t)c * w.stride, r, cplx<T>{(T)sr, (T)si});
+}
+
+template <typename T>
+void launch_pw_project(const PwWork<T> *dwork, int nwork, int maxcount, int nx, int ny, int h, hipStream_t st) {
+    if (nwork <= 0 || maxcount <= 0) return;
+    if (h < 0 || h > PW_MAXH) throw std::runtime_error("plane-wave half width must be 0..8");
+    hipLaunchKernelGGL((pw_project_kernel<T>), dim3(maxcount, nwork, 2 * h + 1), dim3(256), 0, st, dwork, nx, ny, h);
+    BH_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+void launch_pw_synth(const PwWork<T> *dwork, int nwork, int maxcount, int nx, int ny, int h, hipStream_t st) {
+    if (nwork <= 0 || maxcount <= 0) return;
+    if (h < 0 || h > PW_MAXH) throw std::runtime_error("plane-wave half width must be 0..8");
+    const long n = (long)nx * ny;
+    hipLaunchKernelGGL((pw_synth_kernel<T>), dim3((unsigned)((n + 255) / 256), maxcount, nwork), dim3(256), 0, st,
+                       dwork, nx, ny, h);
+    BH_HIP_CHECK(hipGetLastError());
+}
+
 // one workgroup per (column, work item), the column's sum reduced in the workgroup
 template <typename T>
 __global__ void __launch_bounds__(256) bnorm_kernel(const NormWork<T> *__restrict__ work, long rows) {
@@ -209,6 +312,8 @@ void launch_dot(const cplx<T> *x, const cplx<T> *y, long n, double *out2, hipStr
     template void launch_residual<T>(const ResidualWork<T> *, int, int, long, hipStream_t);                 \
     template void launch_rayleigh<T>(const RayleighWork<T> *, int, int, long, hipStream_t);                 \
     template void launch_bnorm<T>(const NormWork<T> *, int, int, long, hipStream_t);                        \
+    template void launch_pw_project<T>(const PwWork<T> *, int, int, int, int, int, hipStream_t);             \
+    template void launch_pw_synth<T>(const PwWork<T> *, int, int, int, int, int, hipStream_t);               \
     template void launch_random<T>(cplx<T> *, int, long, double, hipStream_t);                              \
     template void launch_fill<T>(cplx<T> *, cplx<T> *, const T *, double, double, long, hipStream_t);       \
     template void launch_scale<T>(cplx<T> *, long, double, double, hipStream_t);                            \

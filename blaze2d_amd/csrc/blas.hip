@@ -8,22 +8,6 @@
 
 namespace bh {
 
-// Complex multiply-accumulate as four FMAs (written out: the plain expression
-// acc += a*b + c*d compiles to a multiply, an FMA and an add).
-// (re, im) += a * b
-__device__ inline void cmac(double &re, double &im, double ar, double ai, double br, double bi) {
-    re = fma(ar, br, re);
-    re = fma(-ai, bi, re);
-    im = fma(ar, bi, im);
-    im = fma(ai, br, im);
-}
-// (re, im) += conj(a) * b
-__device__ inline void cmacc(double &re, double &im, double ar, double ai, double br, double bi) {
-    re = fma(ar, br, re);
-    re = fma(ai, bi, re);
-    im = fma(ar, bi, im);
-    im = fma(-ai, br, im);
-}
 
 // ---------------------------------------------------------------------------
 // Gram: grid = (nchunks, nwork); 256 threads as 16 x 16, each owning a BU x BU

@@ -50,6 +50,23 @@ __host__ __device__ inline cplx<T> cadd(cplx<T> a, cplx<T> b) { return {a.x + b.
 template <typename T>
 __host__ __device__ inline cplx<T> csub(cplx<T> a, cplx<T> b) { return {a.x - b.x, a.y - b.y}; }
 
+// Complex multiply-accumulate as four FMAs (written out: the plain expression
+// acc += a*b + c*d compiles to a multiply, an FMA and an add).
+// (re, im) += a * b
+__device__ inline void cmac(double &re, double &im, double ar, double ai, double br, double bi) {
+    re = fma(ar, br, re);
+    re = fma(-ai, bi, re);
+    im = fma(ar, bi, im);
+    im = fma(ai, br, im);
+}
+// (re, im) += conj(a) * b
+__device__ inline void cmacc(double &re, double &im, double ar, double ai, double br, double bi) {
+    re = fma(ar, br, re);
+    re = fma(ai, bi, re);
+    im = fma(ar, bi, im);
+    im = fma(-ai, br, im);
+}
+
 struct hip_error : std::runtime_error {
     using std::runtime_error::runtime_error;
 };

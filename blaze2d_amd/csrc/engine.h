@@ -31,6 +31,11 @@ struct JobSpec {
     int reuse_gamma = 1;
     int exit_rule = 0;      // 0 solve() accounting, 1 soft-locked only
     std::vector<std::array<double, 2>> kpath;
+    // learned start: plane-wave coefficients of warm vectors for the first
+    // k-point, warm_count x (2 pw_half + 1)^2 complex (interleaved)
+    std::vector<double> warm_pw;
+    int warm_count = 0;
+    int pw_half = 0;
 };
 
 struct JobResult {
@@ -40,6 +45,10 @@ struct JobResult {
     // solve's last log iteration (nk * n_bands) and that iteration, 1-based (nk)
     std::vector<double> rel_residuals;
     std::vector<int> rel_iters;
+    // with EngineOptions::pw_half > 0: per k-point the n_bands converged
+    // eigenvectors (ascending eigenvalue) projected on the (2 h + 1)^2 plane
+    // waves G = n1 b1 + n2 b2, |n1|, |n2| <= h (nk * n_bands * (2h+1)^2 complex)
+    std::vector<double> pw;
 };
 
 struct RunStats {
@@ -71,6 +80,7 @@ struct EngineOptions {
     int groups = 4;         // slot groups pipelined on separate streams
     int host_threads = 0;   // host threads for per-slot planning (0: hardware threads, at most 16)
     int timed_kernel = 0;   // TIMED_* class whose launches are timed with events (0: none)
+    int pw_half = 0;        // > 0: project converged eigenvectors on the low plane waves (JobResult::pw)
     // Streams owned by the caller.  HIP gives each new stream its own hardware
     // queue until GPU_MAX_HW_QUEUES is reached and then shares queues, which
     // serialises the groups; a caller that creates the group streams early

@@ -186,6 +186,7 @@ struct Group {
     double *part = nullptr, *gramrb = nullptr, *h_gramrb = nullptr;
     double *h_rel = nullptr;  // readback of the group's residual norms
     double *rrws = nullptr;   // wide Rayleigh-Ritz workspace, one block per slot of the group
+    double *h_pw = nullptr;   // readback of the group's plane-wave projections
     int *sweeps = nullptr;
     int s0 = 0, s1 = 0;
     bool inflight = false;
@@ -276,6 +277,13 @@ struct Slot {
     std::vector<int> eperm;   // permuted eigenvector columns for the history copy
     int t2_kd = 0;
     int t2_hist_new = -1;
+    // learned start: warm vectors synthesised from the job's plane-wave
+    // coefficients into the second history buffer, used by the first k-point
+    int nWc = 0;
+    int t2_nwc = 0;
+    // plane-wave projections of a finished solve's eigenvectors (readback pending)
+    bool pw_pending = false;
+    std::vector<double> first_gamma_pw;
 };
 
 bool is_gamma_frac(const std::array<double, 2> &k) { return k[0] * k[0] + k[1] * k[1] < 1e-8; }
@@ -322,6 +330,8 @@ class Engine : public EngineBase {
         d_coef = dalloc<double>((size_t)S * LD * LD * 2, st);
         d_cnt = dalloc<int>(CNT_STRIDE * (size_t)S, st);
         d_rel = dalloc<double>((size_t)S * LD, st);
+        if (opt.pw_half < 0 || opt.pw_half > 8) throw std::runtime_error("plane-wave half width must be 0..8");
+        if (opt.pw_half > 0) d_pw = dalloc<double>((size_t)S * nbmax * pw_ng() * 2, st);
         BH_HIP_CHECK(hipHostMalloc((void **)&h_lam, sizeof(double) * S * LD, hipHostMallocDefault));
         BH_HIP_CHECK(hipHostMalloc((void **)&h_cnt, sizeof(int) * CNT_STRIDE * S, hipHostMallocDefault));
         int ng = opt.profile ? 1 : std::max(1, std::min(opt.groups, S));
@@ -344,6 +354,9 @@ class Engine : public EngineBase {
             G.gramrb = dalloc<double>((size_t)gs * LD * LD * 2, st);
             BH_HIP_CHECK(hipHostMalloc((void **)&G.h_gramrb, sizeof(double) * gs * LD * LD * 2, hipHostMallocDefault));
             BH_HIP_CHECK(hipHostMalloc((void **)&G.h_rel, sizeof(double) * gs * LD, hipHostMallocDefault));
+            if (d_pw)
+                BH_HIP_CHECK(hipHostMalloc((void **)&G.h_pw, sizeof(double) * gs * nbmax * pw_ng() * 2,
+                                           hipHostMallocDefault));
             G.sweeps = dalloc<int>(6, st);
         }
         // twiddle tables W_n^t = exp(-2 pi i t / n), computed in FP64 on the host
@@ -391,6 +404,7 @@ class Engine : public EngineBase {
             (void)hipFree(G.sweeps);
             (void)hipHostFree(G.h_gramrb);
             (void)hipHostFree(G.h_rel);
+            if (G.h_pw) (void)hipHostFree(G.h_pw);
             G.stg.release();
             for (auto &e : G.tev) {
                 (void)hipEventDestroy(e.first);
@@ -399,7 +413,7 @@ class Engine : public EngineBase {
             (void)hipEventDestroy(G.ev);
             if (G.own_st) (void)hipStreamDestroy(G.st);
         }
-        for (void *p : std::initializer_list<void *>{V, AV, H, RND, EPS, TEN, d_lam, d_Y, d_coef, d_cnt, d_rel, twx, twy, revx, revy, TMP})
+        for (void *p : std::initializer_list<void *>{V, AV, H, RND, EPS, TEN, d_lam, d_Y, d_coef, d_cnt, d_rel, d_pw, twx, twy, revx, revy, TMP})
             if (p) (void)hipFree(p);
         (void)hipHostFree(h_lam);
         (void)hipHostFree(h_cnt);
@@ -423,6 +437,7 @@ class Engine : public EngineBase {
     double *d_lam{}, *d_Y{}, *d_coef{};
     int *d_cnt{};
     double *d_rel{};  // relative residual B-norms, LD per slot
+    double *d_pw{};   // plane-wave projections of finished solves, nbmax x (2h+1)^2 per slot
     double *h_lam{};
     int *h_cnt{};
     cplx<T> *twx{}, *twy{};
@@ -486,7 +501,10 @@ class Engine : public EngineBase {
     void decide(int s);
     void finish_solve(int s, bool conv, int iters);
     void record_k(int s, int kidx, const std::vector<double> &om, int iters, bool conv,
-                  const std::vector<double> &rel, int rel_iter);
+                  const std::vector<double> &rel, int rel_iter, const double *pw = nullptr);
+    // plane waves per projected vector, (2h + 1)^2
+    int pw_ng() const { return (2 * eopt.pw_half + 1) * (2 * eopt.pw_half + 1); }
+    double *pwbuf(int s) { return d_pw + (size_t)s * nbmax * pw_ng() * 2; }
     void take_norms(int s);
     void set_kpoint(int s);
     void execute(StepPlan &plan, Group &G);
@@ -494,9 +512,13 @@ class Engine : public EngineBase {
 
 template <typename T>
 void Engine<T>::record_k(int s, int kidx, const std::vector<double> &om, int iters, bool conv,
-                         const std::vector<double> &rel, int rel_iter) {
+                         const std::vector<double> &rel, int rel_iter, const double *pw) {
     Slot &sl = slots[s];
     JobResult &r = (*out_)[sl.job];
+    if (pw && !r.pw.empty()) {
+        const size_t blk = (size_t)sl.nb * pw_ng() * 2;
+        std::memcpy(r.pw.data() + (size_t)kidx * blk, pw, sizeof(double) * blk);
+    }
     for (int b = 0; b < sl.nb; ++b) {
         r.omegas[(size_t)kidx * sl.nb + b] = b < (int)om.size() ? om[b] : NAN;
         r.rel_residuals[(size_t)kidx * sl.nb + b] = b < (int)rel.size() ? rel[b] : NAN;
@@ -566,6 +588,10 @@ void Engine<T>::start_job(int s, int job) {
     sl.tracking = J.tracking != 0;
     sl.reuse_gamma = J.reuse_gamma != 0;
     sl.diel = build_dielectric(J.geom, nx, ny, J.smoothing != 0);
+    if (J.warm_count > 0) {
+        if (J.pw_half < 0 || J.pw_half > 8) throw std::runtime_error("warm plane-wave half width must be 0..8");
+        sl.nWc = std::min(J.warm_count, nbmax);
+    }
     // create_gamma_mode (initialization.rs:166-224): u0 = 1, y0 = u0 / ||u0||_B
     double nsq = sl.pol == 0 ? sl.diel.eps_sum : (double)N;
     sl.y0val = 1.0 / (nsq > 0 ? std::sqrt(nsq) : 1.0);
@@ -582,7 +608,8 @@ void Engine<T>::schedule_t1(int s) {
     sl.nP = (sl.hist_cur >= 0) ? sl.nb : 0;
     sl.nR = sl.nE > 0 ? (sl.m0 - std::min(sl.nb, sl.m0) + 2) : sl.m0 + 2;
     sl.nR = std::min(sl.nR, NRMAX);
-    sl.kd = sl.nE + sl.nP + sl.nR + 1;
+    if (sl.nE > 0) sl.nWc = 0;  // the learned start serves the first k-point only
+    sl.kd = sl.nE + sl.nP + sl.nR + 1 + sl.nWc;
     if (sl.kd > LD) throw std::runtime_error("transition basis too wide");
     sl.state = GRAM_PENDING;
 }
@@ -762,19 +789,27 @@ void Engine<T>::transition_math(int s) {
         sl.prev_om = om;
         sl.have_prev = true;
         take_norms(s);
-        record_k(s, sl.k, om, sl.res_iters, sl.res_conv, sl.rel_last, sl.rel_last_iter);
+        // projections of the eigenvectors, in E order (ascending eigenvalue);
+        // tracking permutes the frequencies only
+        const double *pw = nullptr;
+        if (sl.pw_pending) {
+            pw = group_of(s).h_pw + (size_t)(s - group_of(s).s0) * nbmax * pw_ng() * 2;
+            sl.pw_pending = false;
+        }
+        record_k(s, sl.k, om, sl.res_iters, sl.res_conv, sl.rel_last, sl.rel_last_iter, pw);
         if (sl.reuse_gamma && sl.k == 0 && is_gamma_frac(J.kpath[0])) {
             sl.first_gamma = om;
             sl.first_gamma_iters = sl.res_iters;
             sl.first_gamma_rel = sl.rel_last;
             sl.first_gamma_rel_iter = sl.rel_last_iter;
+            if (pw) sl.first_gamma_pw.assign(pw, pw + (size_t)sl.nb * pw_ng() * 2);
             sl.have_first_gamma = true;
         }
         // closing Gamma reuse (bandstructure.rs:580-601)
         bool reuse = sl.reuse_gamma && sl.nk > 1 && is_gamma_frac(J.kpath[0]) && is_gamma_frac(J.kpath[sl.nk - 1]);
         if (knext == sl.nk - 1 && reuse && sl.have_first_gamma) {
             record_k(s, knext, sl.first_gamma, sl.first_gamma_iters, true, sl.first_gamma_rel,
-                     sl.first_gamma_rel_iter);
+                     sl.first_gamma_rel_iter, sl.first_gamma_pw.empty() ? nullptr : sl.first_gamma_pw.data());
             knext = sl.nk;
         }
         next_solve = knext < sl.nk;
@@ -782,6 +817,7 @@ void Engine<T>::transition_math(int s) {
         knext = sl.k;  // first k-point of the job
         next_solve = true;
     }
+    const int iW = iY + 1;  // learned-start columns follow y0
     if (!next_solve) {
         sl.state = FREE;
         sl.job = -1;
@@ -883,6 +919,13 @@ void Engine<T>::transition_math(int s) {
         ment.push_back(std::move(mv));
         return true;
     };
+    // initialize_block with warm vectors (initialization.rs:248-309): the learned
+    // start's vectors when the job brought them, before the random fill
+    for (int j = 0; j < sl.nWc && nE == 0; ++j) {
+        std::vector<zd> v(kd, 0.0);
+        v[iW + j] = 1.0;
+        warm.push_back(std::move(v));
+    }
     for (size_t i = 0; i < warm.size() && (int)ent.size() < sl.m0; ++i) try_add(warm[i]);
     int attempts = 0, r = 0;
     while ((int)ent.size() < sl.m0 && attempts < sl.m0 * 8) {
@@ -930,6 +973,7 @@ void Engine<T>::transition_math(int s) {
     sl.eperm.assign(nE, 0);
     for (int i = 0; i < nE; ++i) sl.eperm[i] = sl.E_cols[perm[i]];
     sl.t2_kd = kd;
+    sl.t2_nwc = nE == 0 ? sl.nWc : 0;
     sl.t2_hist_new = new_hist;
     sl.t2_pending = true;
     sl.it = 0;
@@ -960,6 +1004,7 @@ void Engine<T>::run(const std::vector<JobSpec> &jobs, std::vector<JobResult> &ou
         out[j].converged.assign(J.kpath.size(), 0);
         out[j].rel_residuals.assign(J.kpath.size() * J.n_bands, NAN);
         out[j].rel_iters.assign(J.kpath.size(), 0);
+        if (eopt.pw_half > 0) out[j].pw.assign(J.kpath.size() * J.n_bands * pw_ng() * 2, 0.0);
     }
     for (auto &sl : slots) sl = Slot();
     size_t next_job = 0;
@@ -1099,6 +1144,7 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
         for (int i = 0; i < nP; ++i) kp[q++] = hcol(s, sl.hist_cur, i);
         for (int i = 0; i < nR; ++i) kp[q++] = rcol(i);
         kp[q++] = vcol(s, C);
+        for (int i = 0; i < sl.t2_nwc; ++i) kp[q++] = hcol(s, 1, i);
         double *cd;
         double *ch = stg.alloc<double>((size_t)2 * kd * sl.m, &cd);
         for (int e = 0; e < sl.m; ++e)
@@ -1344,6 +1390,7 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
     // ---- T1: transition Grams ----
     std::vector<GramWork<T>> g_t1;
     std::vector<int> t1_slots;
+    std::vector<PwWork<T>> pw_proj, pw_syn;
     for (int s : plan.t1) {
         Slot &sl = slots[s];
         if (sl.rel_final) {
@@ -1393,6 +1440,17 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
             }
             res.push_back(rw);
         }
+        if (eopt.pw_half > 0 && sl.nE > 0) {  // projections of the finished solve's eigenvectors
+            for (int i = 0; i < sl.nE && i < sl.nb; ++i) {
+                PwWork<T> pw{};
+                pw.x = vcol(s, sl.E_cols[i]);
+                pw.c = pwbuf(s) + (size_t)i * pw_ng() * 2;
+                pw.stride = N;
+                pw.count = 1;
+                pw_proj.push_back(pw);
+            }
+            sl.pw_pending = true;
+        }
         const int kd = sl.kd;
         const cplx<T> **kp_d;
         const cplx<T> **kp = stg.alloc<const cplx<T> *>(kd, (const cplx<T> ***)&kp_d);
@@ -1401,6 +1459,7 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
         for (int i = 0; i < sl.nP; ++i) kp[q++] = hcol(s, sl.hist_cur, i);
         for (int i = 0; i < sl.nR; ++i) kp[q++] = rcol(i);
         kp[q++] = vcol(s, C);
+        for (int i = 0; i < (sl.nE == 0 ? sl.nWc : 0); ++i) kp[q++] = hcol(s, 1, i);
         GramWork<T> g{};
         g.a_ptrs = kp_d;
         g.b_ptrs = kp_d;
@@ -1414,6 +1473,27 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
         t1_slots.push_back(s);
     }
 
+    // learned start: the new jobs' warm vectors from their plane-wave coefficients
+    int max_wc = 0, wc_half = 0;
+    for (int s : plan.table_upload) {
+        Slot &sl = slots[s];
+        if (sl.nWc <= 0) continue;
+        const JobSpec &J = (*jobs_)[sl.job];
+        const size_t ngw = (size_t)(2 * J.pw_half + 1) * (2 * J.pw_half + 1);
+        if (!pw_syn.empty() && J.pw_half != wc_half)
+            throw std::runtime_error("jobs in one run must share the warm plane-wave half width");
+        wc_half = J.pw_half;
+        double *cd;
+        double *ch = stg.alloc<double>(2 * ngw * sl.nWc, &cd);
+        std::memcpy(ch, J.warm_pw.data(), sizeof(double) * 2 * ngw * sl.nWc);
+        PwWork<T> pw{};
+        pw.cin = cd;
+        pw.out = hcol(s, 1, 0);
+        pw.stride = N;
+        pw.count = sl.nWc;
+        pw_syn.push_back(pw);
+        max_wc = std::max(max_wc, sl.nWc);
+    }
     // ---- descriptor arrays into staging ----
     auto put = [&](auto &vec) {
         using W = typename std::remove_reference<decltype(vec)>::type::value_type;
@@ -1552,6 +1632,8 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
     auto *d_e_rr2 = put(e_rr2);
     auto *d_lc_ritz = put(lc_ritz);
     auto *d_g_t1 = put(g_t1);
+    auto *d_pw_proj = put(pw_proj);
+    auto *d_pw_syn = put(pw_syn);
     stg.upload(st);
 
     // ---- launches (single stream, in dependency order) ----
@@ -1562,6 +1644,7 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
         for (int b = 0; b < 2; ++b)  // the Gamma mode sits in column C of both copies
             launch_fill<T>(vbuf(s, b, C), nullptr, eps(s), sl.y0val, 0.0, N, st);
     }
+    launch_pw_synth<T>(d_pw_syn, (int)pw_syn.size(), max_wc, nx, ny, wc_half, st);
     // launches of the timed kernel class are bracketed by events on this stream
     auto timed = [&](int cls, double bytes, double flops, auto &&launch) {
         if (eopt.timed_kernel != cls) { launch(); return; }
@@ -1656,6 +1739,12 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
         }
         timed(TIMED_RITZ, bytes, flops,
               [&] { launch_lincomb<T>(d_lc_ritz, (int)lc_ritz.size(), N, std::max(max_ritz, 1), st, max_p); });
+    }
+    launch_pw_project<T>(d_pw_proj, (int)pw_proj.size(), 1, nx, ny, eopt.pw_half, st);
+    if (!pw_proj.empty()) {
+        const int gs = G.s1 - G.s0;
+        BH_HIP_CHECK(hipMemcpyAsync(G.h_pw, pwbuf(G.s0), sizeof(double) * gs * nbmax * pw_ng() * 2,
+                                    hipMemcpyDeviceToHost, st));
     }
     // T1 Grams -> compact readback
     if (!g_t1.empty()) {

@@ -166,6 +166,14 @@ typedef struct {
     int exit_rule;           /* 0: Eigensolver::solve() accounting; 1: soft-locked bands only */
     int nk;
     const double *kpath;     /* nk x 2 fractional k-points */
+    /* Learned start (optional; initialize_block's warm vectors, initialization.rs:248-309):
+     * warm_count vectors for the job's first k-point given by their coefficients on
+     * the plane waves G = n1 b1 + n2 b2, |n1|, |n2| <= pw_half (0..8), complex,
+     * index (n2 + pw_half)(2 pw_half + 1) + n1 + pw_half per vector; the field
+     * is sum_G c_G exp(2 pi i (n1 ix / nx + n2 iy / ny)).  NULL: default start. */
+    const double *warm_pw;
+    int warm_count;
+    int pw_half;
 } bh_job;
 
 typedef struct {
@@ -231,16 +239,32 @@ int bh_operator_diagnostics(bh_context *ctx, const bh_job *jobs, int njobs, int 
 int bh_bandstructure_run(bh_context *ctx, const bh_job *jobs, int njobs, const bh_run_options *opts,
                          double *omegas, int *iters, int *converged, bh_run_stats *stats);
 
-/* bh_bandstructure_run plus the residual diagnostics the reference's solve()
- * computes on its log iterations (1-5, 10, 20, 50, 100, then every 50;
- * eigensolver.rs:251-259,1729-1737): per k-point, the relative residual
- * B-norms ||r_i||_B / max(|lambda_i|, 1e-10) of the first n_bands block
- * entries (eigensolver.rs:744-779; block order, soft-locked bands 0) at the
- * solve's last log iteration, at the same offsets as omegas, and that
- * iteration (1-based) at the same offsets as iters.  Either may be NULL. */
+/* Optional outputs of bh_bandstructure_run_ex; any pointer may be NULL. */
+typedef struct {
+    /* The residual diagnostics the reference's solve() computes on its log
+     * iterations (1-5, 10, 20, 50, 100, then every 50; eigensolver.rs:251-259,
+     * 1729-1737): per k-point the relative residual B-norms
+     * ||r_i||_B / max(|lambda_i|, 1e-10) of the first n_bands block entries
+     * (eigensolver.rs:744-779; block order, soft-locked bands 0) at the solve's
+     * last log iteration, at the same offsets as omegas, and that iteration
+     * (1-based) at the same offsets as iters. */
+    double *rel_residuals;
+    int *rel_residual_iters;
+    /* pw_half > 0 (up to 8): per k-point the n_bands converged eigenvectors
+     * (all_eigenvectors order, eigensolver.rs:2404-2437: ascending eigenvalue,
+     * B-normalised) projected on the plane waves |n1|, |n2| <= pw_half,
+     * c_G = (1/N) sum_r x(r) exp(-2 pi i (n1 ix / nx + n2 iy / ny)), layout as
+     * bh_job::warm_pw, at offset (sum of nk * n_bands over earlier jobs + k
+     * n_bands + band) (2 pw_half + 1)^2 complex values.  The training data of
+     * the learned start. */
+    int pw_half;
+    double *pw_coefs;
+} bh_run_outputs;
+
+/* bh_bandstructure_run with the optional outputs above. */
 int bh_bandstructure_run_ex(bh_context *ctx, const bh_job *jobs, int njobs, const bh_run_options *opts,
-                            double *omegas, int *iters, int *converged, double *rel_residuals,
-                            int *rel_residual_iters, bh_run_stats *stats);
+                            double *omegas, int *iters, int *converged, const bh_run_outputs *extra,
+                            bh_run_stats *stats);
 
 #ifdef __cplusplus
 }
