@@ -99,6 +99,7 @@ class RunOutputs(ctypes.Structure):
         ("rel_residual_iters", c_ip),
         ("pw_half", ctypes.c_int),
         ("pw_coefs", c_dp),
+        ("prec_residuals", c_dp),
     ]
 
 
@@ -121,7 +122,9 @@ EXPORTS = [
     "bh_axpy", "bh_dot", "bh_gram", "bh_lincomb", "bh_theta_create", "bh_theta_destroy", "bh_theta_apply",
     "bh_mass_apply", "bh_precond_apply", "bh_dielectric", "bh_dielectric_batch", "bh_bandstructure_run",
     "bh_hermitian_eigh", "bh_operator_diagnostics", "bh_rayleigh_ritz",
-    "bh_bandstructure_run_ex", "bh_gram_noalloc",
+    "bh_bandstructure_run_ex", "bh_gram_noalloc", "bh_cnn_create", "bh_cnn_destroy", "bh_cnn_param_count",
+    "bh_cnn_get_params", "bh_cnn_set_params", "bh_cnn_get_grads", "bh_cnn_predict", "bh_cnn_train_step",
+    "bh_cnn_predict_geometries",
 ]
 
 _lib = None
@@ -172,6 +175,19 @@ def lib() -> ctypes.CDLL:
                                           c_ip, c_ip, ctypes.POINTER(RunOutputs), ctypes.POINTER(RunStats)]
     L.bh_operator_diagnostics.argtypes = [vp, ctypes.POINTER(Job), ctypes.c_int, ctypes.c_int,
                                           ctypes.POINTER(Diagnostics), c_dp]
+    c_fp = ctypes.POINTER(ctypes.c_float)
+    L.bh_cnn_create.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                ctypes.c_ulonglong, ctypes.POINTER(vp)]
+    L.bh_cnn_destroy.argtypes = [vp]
+    L.bh_cnn_destroy.restype = None
+    L.bh_cnn_param_count.argtypes = [vp]
+    L.bh_cnn_param_count.restype = ctypes.c_longlong
+    L.bh_cnn_get_params.argtypes = [vp, c_fp]
+    L.bh_cnn_set_params.argtypes = [vp, c_fp]
+    L.bh_cnn_get_grads.argtypes = [vp, c_fp]
+    L.bh_cnn_predict.argtypes = [vp, ctypes.c_int, c_fp, c_fp, c_dp]
+    L.bh_cnn_predict_geometries.argtypes = [vp, ctypes.c_int, ctypes.POINTER(Geometry), ctypes.c_int, c_fp, c_dp]
+    L.bh_cnn_train_step.argtypes = [vp, ctypes.c_int, c_fp, c_fp, c_dp, ctypes.c_float, c_dp]
     _lib = L
     return L
 

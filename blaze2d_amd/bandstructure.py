@@ -158,6 +158,9 @@ class BandStructureResult:
     # with run_batch(pw_half=h): the converged eigenvectors (ascending eigenvalue)
     # on the plane waves |n1|, |n2| <= h: [nk, n_bands, (2h + 1)^2] complex
     pw_coefs: Optional[np.ndarray] = None
+    # B-norms ||M^-1 r_i||_B of the preconditioned residuals of each solve's
+    # first iteration (the start block after its Rayleigh-Ritz): [nk, n_bands]
+    first_prec_residuals: Optional[np.ndarray] = None
 
 
 @dataclass
@@ -206,12 +209,14 @@ def run_batch(backend: HipBackend, jobs: Sequence[BandStructureJob], options: Op
     cv = np.zeros(tot_k, dtype=np.int32)
     rel = np.zeros(tot_om)
     rit = np.zeros(tot_k, dtype=np.int32)
+    prec = np.zeros(tot_om)
     ng = (2 * pw_half + 1) ** 2
     pw = np.zeros(tot_om * ng if pw_half > 0 else 1, dtype=np.complex128)
     ro = L.RunOptions(slots, precision, int(profile), groups, timed_kernel)
     st = L.RunStats()
     extra = L.RunOutputs(rel.ctypes.data_as(L.c_dp), rit.ctypes.data_as(L.c_ip), pw_half,
-                         pw.view(np.float64).ctypes.data_as(L.c_dp) if pw_half > 0 else None)
+                         pw.view(np.float64).ctypes.data_as(L.c_dp) if pw_half > 0 else None,
+                         prec.ctypes.data_as(L.c_dp))
     L.check(L.lib().bh_bandstructure_run_ex(backend.handle, arr, len(jobs), ctypes.byref(ro),
                                             om.ctypes.data_as(L.c_dp), it.ctypes.data_as(L.c_ip),
                                             cv.ctypes.data_as(L.c_ip), ctypes.byref(extra), ctypes.byref(st)),
@@ -225,7 +230,8 @@ def run_batch(backend: HipBackend, jobs: Sequence[BandStructureJob], options: Op
                                        rel[o:o + nk * j.n_bands].reshape(nk, j.n_bands).copy(),
                                        rit[k:k + nk].copy(),
                                        pw[o * ng:(o + nk * j.n_bands) * ng].reshape(nk, j.n_bands, ng).copy()
-                                       if pw_half > 0 else None))
+                                       if pw_half > 0 else None,
+                                       prec[o:o + nk * j.n_bands].reshape(nk, j.n_bands).copy()))
         o += nk * j.n_bands
         k += nk
     stats = BatchStats(st.steps, st.iterations, st.solves, st.seconds, st.timed_ms, st.timed_launches,

@@ -8,6 +8,7 @@
 
 #include "aux_kernels.h"
 #include "blas.h"
+#include "cnn.h"
 #include "diagnostics.h"
 #include "dielectric.h"
 #include "engine.h"
@@ -61,6 +62,11 @@ struct bh_buffer {
     size_t n() const { return (size_t)nx * ny; }
     size_t esize() const { return precision == 1 ? sizeof(cplx<float>) : sizeof(cplx<double>); }
     char *col(int i) const { return (char *)data + (size_t)i * n() * esize(); }
+};
+
+struct bh_cnn {
+    bh_context *ctx;
+    Cnn *net;
 };
 
 struct bh_theta {
@@ -1091,6 +1097,8 @@ int bh_bandstructure_run_ex(bh_context *ctx, const bh_job *jobs, int njobs, cons
         if (omegas) std::memcpy(omegas + off, r.omegas.data(), sizeof(double) * r.omegas.size());
         if (rel_residuals)
             std::memcpy(rel_residuals + off, r.rel_residuals.data(), sizeof(double) * r.rel_residuals.size());
+        if (extra && extra->prec_residuals)
+            std::memcpy(extra->prec_residuals + off, r.prec_first.data(), sizeof(double) * r.prec_first.size());
         if (pw_half > 0) {
             const size_t ng2 = (size_t)2 * (2 * pw_half + 1) * (2 * pw_half + 1);
             std::memcpy(extra->pw_coefs + off * ng2, r.pw.data(), sizeof(double) * r.pw.size());
@@ -1124,6 +1132,118 @@ int bh_bandstructure_run_ex(bh_context *ctx, const bh_job *jobs, int njobs, cons
         stats->svqb_fallbacks = rs.svqb_fallbacks;
         stats->rank_drops = rs.rank_drops;
     }
+    return 0;
+    BH_API_CATCH
+}
+
+// ---- learned start ---------------------------------------------------------
+
+int bh_cnn_create(bh_context *ctx, int height, int width, int n_vectors, int pw_half, int max_batch,
+                  unsigned long long seed, bh_cnn **out) {
+    BH_API_TRY
+    if (!ctx || !out) return fail("bh_cnn_create: null argument");
+    BH_HIP_CHECK(hipSetDevice(ctx->device));
+    auto *c = new bh_cnn{ctx, nullptr};
+    try {
+        c->net = new Cnn(height, width, n_vectors, pw_half, max_batch, seed, ctx->st);
+    } catch (...) {
+        delete c;
+        throw;
+    }
+    *out = c;
+    return 0;
+    BH_API_CATCH
+}
+
+void bh_cnn_destroy(bh_cnn *net) {
+    if (!net) return;
+    delete net->net;
+    delete net;
+}
+
+long long bh_cnn_param_count(const bh_cnn *net) { return net ? net->net->param_count() : -1; }
+
+int bh_cnn_get_params(bh_cnn *net, float *host) {
+    BH_API_TRY
+    if (!net || !host) return fail("bh_cnn_get_params: null argument");
+    net->net->get_params(host);
+    return 0;
+    BH_API_CATCH
+}
+
+int bh_cnn_set_params(bh_cnn *net, const float *host) {
+    BH_API_TRY
+    if (!net || !host) return fail("bh_cnn_set_params: null argument");
+    net->net->set_params(host);
+    return 0;
+    BH_API_CATCH
+}
+
+int bh_cnn_get_grads(bh_cnn *net, float *host) {
+    BH_API_TRY
+    if (!net || !host) return fail("bh_cnn_get_grads: null argument");
+    net->net->get_grads(host);
+    return 0;
+    BH_API_CATCH
+}
+
+static int cnn_check(const bh_cnn *net, int batch, const float *grid, const float *embed, const char *who) {
+    if (!net || !grid || !embed) return fail(std::string(who) + ": null argument");
+    if (batch < 1 || batch > net->net->max_batch_size())
+        return fail(std::string(who) + ": batch must be 1.." + std::to_string(net->net->max_batch_size()));
+    return 0;
+}
+
+int bh_cnn_predict(bh_cnn *net, int batch, const float *grid, const float *embed, double *coefs) {
+    BH_API_TRY
+    if (cnn_check(net, batch, grid, embed, "bh_cnn_predict")) return -1;
+    if (!coefs) return fail("bh_cnn_predict: null output");
+    net->net->upload_inputs(batch, grid, embed);
+    net->net->forward(batch);
+    net->net->get_output(batch, coefs);
+    return 0;
+    BH_API_CATCH
+}
+
+int bh_cnn_predict_geometries(bh_cnn *net, int batch, const bh_geometry *geoms, int smoothing, const float *embed,
+                              double *coefs) {
+    BH_API_TRY
+    if (!net || !geoms || !embed || !coefs) return fail("bh_cnn_predict_geometries: null argument");
+    if (batch < 1 || batch > net->net->max_batch_size())
+        return fail("bh_cnn_predict_geometries: batch must be 1.." + std::to_string(net->net->max_batch_size()));
+    std::vector<DielGeom> hg(batch);
+    for (int k = 0; k < batch; ++k) {
+        const bh_geometry &g = geoms[k];
+        if (g.n_atoms < 0 || g.n_atoms > BH_MAX_ATOMS) return fail("bh_cnn_predict_geometries: n_atoms out of range");
+        DielGeom &d = hg[k];
+        d = DielGeom{};
+        for (int c = 0; c < 2; ++c) {
+            d.a1[c] = g.a1[c];
+            d.a2[c] = g.a2[c];
+        }
+        d.eps_bg = g.eps_bg;
+        d.n_atoms = g.n_atoms;
+        for (int a = 0; a < g.n_atoms; ++a)
+            for (int c = 0; c < 4; ++c) d.atoms[a][c] = g.atoms[a][c];
+    }
+    net->net->inputs_from_geometries(batch, hg.data(), smoothing != 0, embed);
+    net->net->forward(batch);
+    net->net->get_output(batch, coefs);
+    return 0;
+    BH_API_CATCH
+}
+
+int bh_cnn_train_step(bh_cnn *net, int batch, const float *grid, const float *embed, const double *targets,
+                      float learning_rate, double *loss) {
+    BH_API_TRY
+    if (cnn_check(net, batch, grid, embed, "bh_cnn_train_step")) return -1;
+    if (!targets) return fail("bh_cnn_train_step: null targets");
+    if (!(learning_rate >= 0.0f)) return fail("bh_cnn_train_step: learning rate must be >= 0");
+    net->net->upload_inputs(batch, grid, embed);
+    net->net->forward(batch);
+    const double l = net->net->loss_and_backward(batch, targets);
+    if (learning_rate > 0.0f) net->net->adam(learning_rate);
+    if (loss) *loss = l;
     return 0;
     BH_API_CATCH
 }

@@ -41,7 +41,7 @@ struct NormWork {
     const cplx<T> *r;     // count columns, stride apart
     const T *bw;          // optional B weight (TM: eps); null: B = I
     const int *act;       // band index of each column
-    const double *lam;
+    const double *lam;    // null: the plain B-norm
     double *rel;          // indexed by band
     long stride;
     int count, pad;

@@ -168,7 +168,8 @@ __global__ void __launch_bounds__(256) bnorm_kernel(const NormWork<T> *__restric
     __syncthreads();
     if (threadIdx.x == 0) {
         const int b = w.act[c];
-        w.rel[b] = sqrt(fmax(part[0] + part[1] + part[2] + part[3], 0.0)) / fmax(fabs(w.lam[b]), 1e-10);
+        const double den = w.lam ? fmax(fabs(w.lam[b]), 1e-10) : 1.0;
+        w.rel[b] = sqrt(fmax(part[0] + part[1] + part[2] + part[3], 0.0)) / den;
     }
 }
 

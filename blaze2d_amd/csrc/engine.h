@@ -45,6 +45,9 @@ struct JobResult {
     // solve's last log iteration (nk * n_bands) and that iteration, 1-based (nk)
     std::vector<double> rel_residuals;
     std::vector<int> rel_iters;
+    // per k-point and band: B-norm of the preconditioned residual M^-1 r_i of
+    // the solve's first iteration (M = I without a preconditioner)
+    std::vector<double> prec_first;
     // with EngineOptions::pw_half > 0: per k-point the n_bands converged
     // eigenvectors (ascending eigenvalue) projected on the (2 h + 1)^2 plane
     // waves G = n1 b1 + n2 b2, |n1|, |n2| <= h (nk * n_bands * (2h+1)^2 complex)

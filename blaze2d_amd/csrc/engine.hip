@@ -185,6 +185,7 @@ struct Group {
     Staging stg;
     double *part = nullptr, *gramrb = nullptr, *h_gramrb = nullptr;
     double *h_rel = nullptr;  // readback of the group's residual norms
+    double *h_prec = nullptr; // readback of the first-iteration preconditioned residual norms
     double *rrws = nullptr;   // wide Rayleigh-Ritz workspace, one block per slot of the group
     double *h_pw = nullptr;   // readback of the group's plane-wave projections
     int *sweeps = nullptr;
@@ -261,6 +262,10 @@ struct Slot {
     int first_gamma_iters = 0;
     std::vector<double> first_gamma_rel;
     int first_gamma_rel_iter = 0;
+    std::vector<double> first_gamma_prec;
+    // B-norms of the preconditioned residuals M^-1 r_i of the first iteration
+    std::vector<double> prec_first;
+    bool prec_pending = false;
     // relative residual B-norms: the latest log iteration's values (block
     // order) and the norms of an iteration in flight (read back next step)
     std::vector<double> rel_last;
@@ -330,6 +335,7 @@ class Engine : public EngineBase {
         d_coef = dalloc<double>((size_t)S * LD * LD * 2, st);
         d_cnt = dalloc<int>(CNT_STRIDE * (size_t)S, st);
         d_rel = dalloc<double>((size_t)S * LD, st);
+        d_prec = dalloc<double>((size_t)S * LD, st);
         if (opt.pw_half < 0 || opt.pw_half > 8) throw std::runtime_error("plane-wave half width must be 0..8");
         if (opt.pw_half > 0) d_pw = dalloc<double>((size_t)S * nbmax * pw_ng() * 2, st);
         BH_HIP_CHECK(hipHostMalloc((void **)&h_lam, sizeof(double) * S * LD, hipHostMallocDefault));
@@ -354,6 +360,7 @@ class Engine : public EngineBase {
             G.gramrb = dalloc<double>((size_t)gs * LD * LD * 2, st);
             BH_HIP_CHECK(hipHostMalloc((void **)&G.h_gramrb, sizeof(double) * gs * LD * LD * 2, hipHostMallocDefault));
             BH_HIP_CHECK(hipHostMalloc((void **)&G.h_rel, sizeof(double) * gs * LD, hipHostMallocDefault));
+            BH_HIP_CHECK(hipHostMalloc((void **)&G.h_prec, sizeof(double) * gs * LD, hipHostMallocDefault));
             if (d_pw)
                 BH_HIP_CHECK(hipHostMalloc((void **)&G.h_pw, sizeof(double) * gs * nbmax * pw_ng() * 2,
                                            hipHostMallocDefault));
@@ -404,6 +411,7 @@ class Engine : public EngineBase {
             (void)hipFree(G.sweeps);
             (void)hipHostFree(G.h_gramrb);
             (void)hipHostFree(G.h_rel);
+            (void)hipHostFree(G.h_prec);
             if (G.h_pw) (void)hipHostFree(G.h_pw);
             G.stg.release();
             for (auto &e : G.tev) {
@@ -413,7 +421,7 @@ class Engine : public EngineBase {
             (void)hipEventDestroy(G.ev);
             if (G.own_st) (void)hipStreamDestroy(G.st);
         }
-        for (void *p : std::initializer_list<void *>{V, AV, H, RND, EPS, TEN, d_lam, d_Y, d_coef, d_cnt, d_rel, d_pw, twx, twy, revx, revy, TMP})
+        for (void *p : std::initializer_list<void *>{V, AV, H, RND, EPS, TEN, d_lam, d_Y, d_coef, d_cnt, d_rel, d_prec, d_pw, twx, twy, revx, revy, TMP})
             if (p) (void)hipFree(p);
         (void)hipHostFree(h_lam);
         (void)hipHostFree(h_cnt);
@@ -437,6 +445,7 @@ class Engine : public EngineBase {
     double *d_lam{}, *d_Y{}, *d_coef{};
     int *d_cnt{};
     double *d_rel{};  // relative residual B-norms, LD per slot
+    double *d_prec{}; // first-iteration preconditioned residual B-norms, LD per slot
     double *d_pw{};   // plane-wave projections of finished solves, nbmax x (2h+1)^2 per slot
     double *h_lam{};
     int *h_cnt{};
@@ -500,7 +509,7 @@ class Engine : public EngineBase {
     void absorb(int s);
     void decide(int s);
     void finish_solve(int s, bool conv, int iters);
-    void record_k(int s, int kidx, const std::vector<double> &om, int iters, bool conv,
+    void record_k(int s, int kidx, const std::vector<double> &om, int iters, bool conv, const std::vector<double> &prec,
                   const std::vector<double> &rel, int rel_iter, const double *pw = nullptr);
     // plane waves per projected vector, (2h + 1)^2
     int pw_ng() const { return (2 * eopt.pw_half + 1) * (2 * eopt.pw_half + 1); }
@@ -512,6 +521,7 @@ class Engine : public EngineBase {
 
 template <typename T>
 void Engine<T>::record_k(int s, int kidx, const std::vector<double> &om, int iters, bool conv,
+                         const std::vector<double> &prec,
                          const std::vector<double> &rel, int rel_iter, const double *pw) {
     Slot &sl = slots[s];
     JobResult &r = (*out_)[sl.job];
@@ -522,6 +532,7 @@ void Engine<T>::record_k(int s, int kidx, const std::vector<double> &om, int ite
     for (int b = 0; b < sl.nb; ++b) {
         r.omegas[(size_t)kidx * sl.nb + b] = b < (int)om.size() ? om[b] : NAN;
         r.rel_residuals[(size_t)kidx * sl.nb + b] = b < (int)rel.size() ? rel[b] : NAN;
+        r.prec_first[(size_t)kidx * sl.nb + b] = b < (int)prec.size() ? prec[b] : NAN;
     }
     r.iters[kidx] = iters;
     r.converged[kidx] = conv ? 1 : 0;
@@ -641,6 +652,11 @@ void Engine<T>::absorb(int s) {
     Slot &sl = slots[s];
     if (sl.it == 0) return;
     take_norms(s);
+    if (sl.prec_pending) {
+        const double *h = group_of(s).h_prec + (size_t)(s - group_of(s).s0) * LD;
+        sl.prec_first.assign(h, h + std::min(sl.m, sl.nb));
+        sl.prec_pending = false;
+    }
     const int *cnt = h_cnt + CNT_STRIDE * (size_t)s;
     int cx = cnt[0], cw = cnt[1];
     if (eopt.timed_kernel == TIMED_RR) stats_->timed_flops += rr_flops(sl.last_p, cx + cw, cnt[3]);
@@ -796,19 +812,20 @@ void Engine<T>::transition_math(int s) {
             pw = group_of(s).h_pw + (size_t)(s - group_of(s).s0) * nbmax * pw_ng() * 2;
             sl.pw_pending = false;
         }
-        record_k(s, sl.k, om, sl.res_iters, sl.res_conv, sl.rel_last, sl.rel_last_iter, pw);
+        record_k(s, sl.k, om, sl.res_iters, sl.res_conv, sl.prec_first, sl.rel_last, sl.rel_last_iter, pw);
         if (sl.reuse_gamma && sl.k == 0 && is_gamma_frac(J.kpath[0])) {
             sl.first_gamma = om;
             sl.first_gamma_iters = sl.res_iters;
             sl.first_gamma_rel = sl.rel_last;
             sl.first_gamma_rel_iter = sl.rel_last_iter;
+            sl.first_gamma_prec = sl.prec_first;
             if (pw) sl.first_gamma_pw.assign(pw, pw + (size_t)sl.nb * pw_ng() * 2);
             sl.have_first_gamma = true;
         }
         // closing Gamma reuse (bandstructure.rs:580-601)
         bool reuse = sl.reuse_gamma && sl.nk > 1 && is_gamma_frac(J.kpath[0]) && is_gamma_frac(J.kpath[sl.nk - 1]);
         if (knext == sl.nk - 1 && reuse && sl.have_first_gamma) {
-            record_k(s, knext, sl.first_gamma, sl.first_gamma_iters, true, sl.first_gamma_rel,
+            record_k(s, knext, sl.first_gamma, sl.first_gamma_iters, true, sl.first_gamma_prec, sl.first_gamma_rel,
                      sl.first_gamma_rel_iter, sl.first_gamma_pw.empty() ? nullptr : sl.first_gamma_pw.data());
             knext = sl.nk;
         }
@@ -981,6 +998,8 @@ void Engine<T>::transition_math(int s) {
     sl.rel_last.clear();
     sl.rel_last_iter = 0;
     sl.rel_pending = sl.rel_final = false;
+    sl.prec_first.clear();
+    sl.prec_pending = false;
     sl.nW = 0;
     sl.soft.assign(m0, 0);
     sl.soft_iter.assign(m0, 0);
@@ -1004,6 +1023,7 @@ void Engine<T>::run(const std::vector<JobSpec> &jobs, std::vector<JobResult> &ou
         out[j].converged.assign(J.kpath.size(), 0);
         out[j].rel_residuals.assign(J.kpath.size() * J.n_bands, NAN);
         out[j].rel_iters.assign(J.kpath.size(), 0);
+        out[j].prec_first.assign(J.kpath.size() * J.n_bands, NAN);
         if (eopt.pw_half > 0) out[j].pw.assign(J.kpath.size() * J.n_bands * pw_ng() * 2, 0.0);
     }
     for (auto &sl : slots) sl = Slot();
@@ -1193,6 +1213,7 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
     std::vector<DeflateWork<T>> defR, defP;
     std::vector<int> defR_slot, defP_slot;  // slot of each deflation item
     std::vector<OpWork<T>> op_pre[2];  // [0] FD, [1] TP
+    std::vector<NormWork<T>> prec_norms;  // first iterations: B-norms of M^-1 r
     std::vector<OpWork<T>> op_p[2];    // Theta on the new directions [TM, TE]
     long theta_cols[2] = {0, 0};
     std::vector<Gram2Work<T>> g_dual;
@@ -1310,6 +1331,21 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
             sl.rel_pending_iter = sl.it + 1;
             sl.rel_act = rcols;
             sl.rel_m = m;
+        }
+        // the first iteration: B-norms of the preconditioned residuals (how far
+        // one preconditioned step moves each start vector; M = I without a
+        // preconditioner), taken before the direction deflation
+        if (sl.it == 0 && a > 0) {
+            NormWork<T> nw{};
+            nw.r = vcol(s, m + w);
+            nw.bw = bweight(s);
+            nw.act = act_d;
+            nw.lam = nullptr;
+            nw.rel = d_prec + (size_t)s * LD;
+            nw.stride = N;
+            nw.count = a;
+            prec_norms.push_back(nw);
+            sl.prec_pending = true;
         }
         // preconditioner (P in place)
         if (sl.precond == 1 || sl.precond == 2) {
@@ -1623,6 +1659,7 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
     OpWork<T> *d_op_x[2] = {put(op_x[0]), put(op_x[1])};
     auto *d_rq = put(rq);
     auto *d_res = put(res);
+    auto *d_prec_norms = put(prec_norms);
     auto *d_defR = put(defR);
     OpWork<T> *d_op_pre[2] = {put(op_pre[0]), put(op_pre[1])};
     auto *d_defP = put(defP);
@@ -1698,6 +1735,7 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
                  prec_fp32, revx, revy);
     launch_op<T>(MODE_PREC_TP, nx, ny, d_op_pre[1], (int)op_pre[1].size(), std::max(max_a, 1), twx, twy, st,
                  prec_fp32, revx, revy);
+    launch_bnorm<T>(d_prec_norms, (int)prec_norms.size(), std::max(max_a, 1), N, st);
     if (!defP.empty())
         timed(TIMED_DEFL, defl_bytes(defP), defl_flops(defP),
               [&] { deflate(d_defP, defP, drP, max_a); });
@@ -1759,6 +1797,11 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
     if (any_norms) {
         const int gs = G.s1 - G.s0;
         BH_HIP_CHECK(hipMemcpyAsync(G.h_rel, d_rel + (size_t)G.s0 * LD, sizeof(double) * gs * LD,
+                                    hipMemcpyDeviceToHost, st));
+    }
+    if (!prec_norms.empty()) {
+        const int gs = G.s1 - G.s0;
+        BH_HIP_CHECK(hipMemcpyAsync(G.h_prec, d_prec + (size_t)G.s0 * LD, sizeof(double) * gs * LD,
                                     hipMemcpyDeviceToHost, st));
     }
     if (!plan.iter.empty()) {

@@ -28,6 +28,7 @@ extern "C" {
 typedef struct bh_context bh_context;
 typedef struct bh_buffer bh_buffer;
 typedef struct bh_theta bh_theta;
+typedef struct bh_cnn bh_cnn;
 
 /* Library identification. */
 const char *bh_version(void);
@@ -259,12 +260,62 @@ typedef struct {
      * the learned start. */
     int pw_half;
     double *pw_coefs;
+    /* Per k-point, at the same offsets as omegas: the B-norms ||M^-1 r_i||_B
+     * of the preconditioned residuals of the solve's first iteration (the
+     * start block after its Rayleigh-Ritz; M = I without a preconditioner).
+     * No counterpart in the reference: the learned start's fallback test
+     * compares them between a predicted and the default start. */
+    double *prec_residuals;
 } bh_run_outputs;
 
 /* bh_bandstructure_run with the optional outputs above. */
 int bh_bandstructure_run_ex(bh_context *ctx, const bh_job *jobs, int njobs, const bh_run_options *opts,
                             double *omegas, int *iters, int *converged, const bh_run_outputs *extra,
                             bh_run_stats *stats);
+
+/* ---- learned start ------------------------------------------------------
+ * The reference has no learned start: its warm vectors come from the
+ * previous k-point (subspace_prediction.rs:276-380) or none.  This network
+ * predicts warm vectors for a solve from the eps grid and the k-point; they
+ * enter the solver through bh_job::warm_pw, the warm-start ingestion of
+ * initialize_block (initialization.rs:248-309).
+ *
+ * Network on a height x width input (multiples of 8), C = 32 channels:
+ *   e   = W_e2 relu(W_e1 [k1, k2, pol, lattice] + b_e1) + b_e2   (4 C)
+ *   a_1 = relu(conv3x3(x, stride 1) + b_1 + e[0:C])
+ *   a_l = relu(conv3x3(a_{l-1}, stride 2) + b_l + e[(l-1) C : l C]), l = 2..4
+ *   y   = W_h a_4 + b_h   (n_vectors x (2 pw_half + 1)^2 complex, bh_job::warm_pw layout)
+ * with periodic 3 x 3 convolutions.  Parameters (FP32 master copy, used as
+ * BF16 with FP32 accumulation), in this order in the flat array:
+ *   per layer l: w_l [C][c_in][3][3] (c_in = 2 for l = 1, else C), b_l [C];
+ *   W_e1 [C][4], b_e1 [C]; W_e2 [4C][C], b_e2 [4C];
+ *   W_h [2 n_vectors (2 pw_half + 1)^2][C (height/8) (width/8)], b_h.
+ * Inputs per sample: grid [2][height][width] floats (eps / 16, 1 / eps),
+ * embed [4] floats (k1, k2 fractional; pol 0 TM / 1 TE; lattice 0 square /
+ * 1 hexagonal).  Training loss per sample, invariant to the basis of either
+ * block: L = 1 - tr(P_Y P_U) / n_vectors, P the orthogonal projector on the
+ * columns (predicted Y, target U), averaged over the batch. */
+int bh_cnn_create(bh_context *ctx, int height, int width, int n_vectors, int pw_half, int max_batch,
+                  unsigned long long seed, bh_cnn **out);
+void bh_cnn_destroy(bh_cnn *net);
+long long bh_cnn_param_count(const bh_cnn *net);
+int bh_cnn_get_params(bh_cnn *net, float *host);
+/* Replaces the master weights (and their BF16 copy); restarts Adam. */
+int bh_cnn_set_params(bh_cnn *net, const float *host);
+/* Gradients of the last bh_cnn_train_step's loss w.r.t. the parameters. */
+int bh_cnn_get_grads(bh_cnn *net, float *host);
+/* coefs: [batch][n_vectors][(2 pw_half + 1)^2] complex (interleaved doubles). */
+int bh_cnn_predict(bh_cnn *net, int batch, const float *grid, const float *embed, double *coefs);
+/* bh_cnn_predict with the grid input computed on the device from the
+ * geometries: the smoothed (smoothing = 1) or plain eps of each on the
+ * network's height x width grid (as bh_dielectric with nx = width, ny =
+ * height), then eps / 16 and 1 / eps. */
+int bh_cnn_predict_geometries(bh_cnn *net, int batch, const bh_geometry *geoms, int smoothing, const float *embed,
+                              double *coefs);
+/* Forward, loss, backward and (learning_rate > 0) one Adam step (beta 0.9,
+ * 0.999, eps 1e-8, bias-corrected).  targets as coefs.  loss may be NULL. */
+int bh_cnn_train_step(bh_cnn *net, int batch, const float *grid, const float *embed, const double *targets,
+                      float learning_rate, double *loss);
 
 #ifdef __cplusplus
 }
