@@ -274,11 +274,21 @@ class LearnedSolve:
     inference_seconds: float
 
 
+def keep_prediction(prec_predicted: np.ndarray, prec_default: np.ndarray, margin: float = 1.0) -> np.ndarray:
+    """True where the predicted start is kept: its first-iteration
+    preconditioned residual norm is not worse than margin times the default
+    start's (margin 1: not worse; NaN compares as worse)."""
+    pp = np.asarray(prec_predicted, dtype=float)
+    pd = np.asarray(prec_default, dtype=float)
+    return pp <= margin * pd
+
+
 def solve(backend: HipBackend, net: CnnNet, jobs: Sequence[BandStructureJob], options: Optional[RunOptions] = None,
-          slots: int = 256, fallback: bool = True) -> LearnedSolve:
+          slots: int = 512, groups: int = 2, fallback: bool = True, margin: float = 1.0,
+          precision: int = 0) -> LearnedSolve:
     """Solves one-k-point jobs from the network's predicted start, with the
     default start as the fallback wherever the predicted block's first
-    preconditioned residual is worse."""
+    preconditioned residual is worse (see keep_prediction)."""
     opts = options or RunOptions()
     jobs = list(jobs)
     for j in jobs:
@@ -292,7 +302,7 @@ def solve(backend: HipBackend, net: CnnNet, jobs: Sequence[BandStructureJob], op
     warm = [dataclasses.replace(j, warm_pw=c, pw_half=net.pw_half) for j, c in zip(jobs, coefs)]
     n = len(jobs)
     run = warm + ([dataclasses.replace(j, max_iter=1) for j in jobs] if fallback else [])
-    res, _ = run_batch(backend, run, opts, slots=min(slots, len(run)))
+    res, _ = run_batch(backend, run, opts, slots=min(slots, len(run)), groups=groups, precision=precision)
     spent = int(sum(int(r.iterations.sum()) for r in res))
     pp = np.array([np.linalg.norm(r.first_prec_residuals[0]) for r in res[:n]])
     pd = np.full(n, np.inf)
@@ -300,15 +310,27 @@ def solve(backend: HipBackend, net: CnnNet, jobs: Sequence[BandStructureJob], op
     res = list(res)
     if fallback:
         pd = np.array([np.linalg.norm(r.first_prec_residuals[0]) for r in res[n:]])
-        use = ~(pp > pd)
+        use = keep_prediction(pp, pd, margin)
         redo = [i for i in range(n) if not use[i]]
         if redo:
-            again, _ = run_batch(backend, [jobs[i] for i in redo], opts, slots=min(slots, len(redo)))
+            again, _ = run_batch(backend, [jobs[i] for i in redo], opts, slots=min(slots, len(redo)), groups=groups,
+                                 precision=precision)
             spent += int(sum(int(r.iterations.sum()) for r in again))
             for i, r in zip(redo, again):
                 res[i] = r
     t1 = time.perf_counter()
     return LearnedSolve(res[:n], use, pd, pp, spent, t1 - t0, ti)
+
+
+def mean_loss(net: CnnNet, grid: np.ndarray, embed: np.ndarray, targets: np.ndarray) -> float:
+    """Mean projector loss of the network's predictions (the GPU loss kernel;
+    no parameter update)."""
+    tot = 0.0
+    B = grid.shape[0]
+    for s in range(0, B, net.max_batch):
+        n = min(net.max_batch, B - s)
+        tot += net.train_step(grid[s:s + n], embed[s:s + n], targets[s:s + n], 0.0) * n
+    return tot / B
 
 
 def held_out_jobs(n_geometries: int, seed: int, grid: int = 64, n_bands: int = 12, tol: float = 1e-7,

@@ -1441,8 +1441,31 @@ Geometry geometry_from_job(const OracleJob *j) {
     return g;
 }
 
+// Fields x(r) = sum_G c_G exp(+2 pi i (n1 ix / nx + n2 iy / ny)) from
+// coefficients on |n1|, |n2| <= h, index (n2 + h)(2h + 1) + n1 + h (the
+// learned start's warm vectors; the HIP engine synthesises them the same way)
+std::vector<Field> fields_from_plane_waves(const double *pw, int count, int h, int nx, int ny) {
+    const int nw = 2 * h + 1;
+    std::vector<Field> out(count, Field((size_t)nx * ny));
+    for (int c = 0; c < count; ++c) {
+        const double *cf = pw + 2 * (size_t)c * nw * nw;
+        for (int iy = 0; iy < ny; ++iy)
+            for (int ix = 0; ix < nx; ++ix) {
+                cd acc(0, 0);
+                for (int n2 = -h; n2 <= h; ++n2)
+                    for (int n1 = -h; n1 <= h; ++n1) {
+                        const double a = 2.0 * M_PI * ((double)n1 * ix / nx + (double)n2 * iy / ny);
+                        const cd g(cf[2 * ((n2 + h) * nw + n1 + h)], cf[2 * ((n2 + h) * nw + n1 + h) + 1]);
+                        acc += g * cd(std::cos(a), std::sin(a));
+                    }
+                out[c][(size_t)iy * nx + ix] = acc;
+            }
+    }
+    return out;
+}
+
 int run_core(const OracleJob *job, double *omegas_out, int *iters_out, int *conv_out, double *rel_out,
-             int *rel_iter_out) {
+             int *rel_iter_out, const double *warm_pw = nullptr, int warm_count = 0, int pw_half = 0) {
     Geometry g = geometry_from_job(job);
     Dielectric d = build_dielectric(g, job->nx, job->ny, job->smoothing != 0);
     int nb = job->n_bands;
@@ -1478,7 +1501,10 @@ int run_core(const OracleJob *job, double *omegas_out, int *iters_out, int *conv
         double bloch[2] = {kf[0] * d.b1[0] + kf[1] * d.b2[0], kf[0] * d.b1[1] + kf[1] * d.b2[1]};
         std::vector<Field> warm;
         bool use_warm = false;
-        if (job->subspace_prediction) {
+        if (k == 0 && warm_pw && warm_count > 0) {
+            warm = fields_from_plane_waves(warm_pw, warm_count, pw_half, job->nx, job->ny);
+            use_warm = true;
+        } else if (job->subspace_prediction) {
             warm = hist.predict(kf, eps_track);
             use_warm = !warm.empty();
         } else if (have_prev) {
@@ -1554,6 +1580,12 @@ int oracle_bandstructure_residuals(const OracleJob *job, double *omegas, int *it
                                    double *rel_residuals, int *rel_residual_iters) {
     if (!job || job->nx <= 0 || job->ny <= 0 || job->n_bands <= 0) return -1;
     return run_core(job, omegas, iters, converged, rel_residuals, rel_residual_iters);
+}
+
+int oracle_bandstructure_warm(const OracleJob *job, const double *warm_pw, int warm_count, int pw_half,
+                              double *omegas, int *iters, int *converged) {
+    if (!job || job->nx <= 0 || job->ny <= 0 || job->n_bands <= 0 || pw_half < 0 || pw_half > 8) return -1;
+    return run_core(job, omegas, iters, converged, nullptr, nullptr, warm_pw, warm_count, pw_half);
 }
 
 int oracle_dielectric(const OracleJob *job, double *eps, double *inv_eps, double *tensor) {

@@ -51,6 +51,13 @@ int oracle_bandstructure(const OracleJob *job, double *omegas, int *iters, int *
  * ||r_i||_B / max(|lambda_i|, 1e-10) of the first n_bands block entries (in
  * the solver's block order) at the last log iteration of the solve
  * (eigensolver.rs:251-259,744-779,1729-1737) and that iteration (1-based). */
+/* oracle_bandstructure with warm vectors for the first k-point (ingested by
+ * initialize_block, initialization.rs:248-309): warm_count fields given by
+ * plane-wave coefficients on |n1|, |n2| <= pw_half (index (n2 + h)(2h + 1) +
+ * n1 + h, interleaved complex), x(r) = sum_G c_G exp(+2 pi i (n1 ix / nx +
+ * n2 iy / ny)). */
+int oracle_bandstructure_warm(const OracleJob *job, const double *warm_pw, int warm_count, int pw_half,
+                              double *omegas, int *iters, int *converged);
 int oracle_bandstructure_residuals(const OracleJob *job, double *omegas, int *iters, int *converged,
                                    double *rel_residuals, int *rel_residual_iters);
 

@@ -273,6 +273,45 @@ def test_bandstructure_uniform_medium(pol):
         assert np.allclose(np.sort(row)[:4], want, rtol=1e-7, atol=1e-7)
 
 
+@pytest.mark.parametrize("pol", ["TM", "TE"])
+def test_warm_start_from_exact_plane_waves(pol):
+    # homogeneous medium at a generic k: the eigenvectors are the plane waves
+    # exp(i G r) with the lowest |k + G|.  Warm vectors given as those plane
+    # waves' coefficients (unit coefficient at G) start the solve on the exact
+    # eigenspace: it stops within three iterations with
+    # omega = |k+G|/sqrt(eps) to 1e-12.  A sign or index slip in the synthesis
+    # (exp(-i G r), n1 and n2 swapped) starts it on other plane waves, which
+    # the change test accepts as fast: the frequencies show it.
+    eps, h, nb = 2.25, 2, 7
+    k = (0.137, 0.061)
+    g = [(n1, n2) for n2 in range(-h, h + 1) for n1 in range(-h, h + 1)]
+    q2 = np.array([((2 * np.pi) ** 2) * ((n1 + k[0]) ** 2 + (n2 + k[1]) ** 2) for n1, n2 in g])
+    order = np.argsort(q2)
+    warm = np.zeros((nb, len(g)), complex)
+    for b in range(nb):
+        warm[b, order[b]] = 1.0
+    job = O.Job(lattice="square", eps_bg=eps, atoms=[], nx=16, ny=16, pol=pol, n_bands=nb, tol=1e-10,
+                kpath=np.array([k]))
+    om, it, cv, _ = O.bandstructure_warm(job, warm, h)
+    cold, itc, _, _ = O.bandstructure(job)
+    want = np.sqrt(q2[order[:nb]] / eps)
+    assert cv.all() and it[0] <= 3 < itc[0]
+    assert np.allclose(np.sort(om[0]), want, rtol=1e-12, atol=0)
+    # with n1 and n2 swapped the warm vectors are the plane waves (n2, n1):
+    # eigenvectors too, so the change test stops as fast, but on a set that is
+    # not the lowest seven ((-1, 1) becomes (1, -1); k is off the diagonal)
+    swapped = warm.reshape(nb, 2 * h + 1, 2 * h + 1).transpose(0, 2, 1).reshape(nb, -1)
+    oms, _, _, _ = O.bandstructure_warm(job, swapped, h)
+    assert np.abs(np.sort(oms[0]) - want).max() > 1e-3 * want.max()
+    # exp(-i G r): the coefficient of G placed at -G
+    neg = np.zeros_like(warm)
+    for b in range(nb):
+        n1, n2 = g[order[b]]
+        neg[b, g.index((-n1, -n2))] = 1.0
+    omn, _, _, _ = O.bandstructure_warm(job, neg, h)
+    assert np.abs(np.sort(omn[0]) - want).max() > 1e-3 * want.max()
+
+
 # --- full path vs the reference's own FP64 output ---------------------------
 @pytest.mark.parametrize("pol", ["TM", "TE"])
 def test_series6_golden_frequencies(pol):
