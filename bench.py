@@ -115,6 +115,158 @@ def cpu_model():
     return "unknown"
 
 
+def large_case(be):
+    # the large combined case: 256 x 256, 32 bands, TE + TM, batched over the
+    # 63 distinct k-points of the 64-point Gamma-X-M-Gamma path (126 solves,
+    # each from the deterministic start), all bands converged; then the
+    # 512 x 512 TM 16-band solve at M.  Timed twice: the multi-pass FFT
+    # operator class (Theta) and the Rayleigh-Ritz Gram class, each with HIP
+    # events on its stream; frequencies at M checked against the oracle
+    # fixture.
+    import json as _json
+    from blaze2d_amd.bandstructure import RunOptions, run_batch
+    from blaze2d_amd.sweep import large_case_jobs, large_case_m_point
+    lj = large_case_jobs()
+    lopts = RunOptions(exit_rule=1)
+    run_batch(be, lj[:2], lopts, slots=2)  # first launches of the 256 x 256 kernels
+    large = {}
+    for cls in (1, 2):
+        t = time.perf_counter()
+        lres, lst = run_batch(be, lj, lopts, slots=min(64, len(lj)), timed_kernel=cls)
+        dt = time.perf_counter() - t
+        large[cls] = (lres, lst, dt)
+    lres, lst, dt = large[1]
+    conv_l = sum(int(r.converged.sum()) for r in lres)
+    ths = large[1][1]
+    grs = large[2][1]
+    th_avg = ths.timed_ms * 1e-3 / max(ths.timed_launches, 1)
+    gr_avg = grs.timed_ms * 1e-3 / max(grs.timed_launches, 1)
+    with open(os.path.join(ROOT, "tests", "golden", "cfg5_oracle.json")) as f:
+        fx = _json.load(f)
+    errs = []
+    for c in fx["cases"]:
+        if c["grid"] == 256 and c["k"] == [0.5, 0.5]:
+            i = next(i for i, j in enumerate(lj) if j.pol == c["pol"] and
+                     abs(j.k_path[0][0] - 0.5) < 1e-12 and abs(j.k_path[0][1] - 0.5) < 1e-12)
+            errs.append(rel_err(lres[i].bands, np.array([c["omegas"]])))
+    mj = large_case_m_point()
+    run_batch(be, [mj], lopts, slots=1)
+    t = time.perf_counter()
+    (mres,), mst = run_batch(be, [mj], lopts, slots=1, timed_kernel=1)
+    dtm5 = time.perf_counter() - t
+    mref = next(c for c in fx["cases"] if c["grid"] == 512)
+    result = {
+        "workload": "square eps_bg 12, eps 1 rods r 0.2 at (0,0) and r 0.12 at (1/2,1/2); 256x256, 32 bands, "
+                    "TE+TM, 63 distinct k-points of the 64-point Gamma-X-M-Gamma path batched over k, "
+                    "tol 1e-7, all bands converged, FP64",
+        "value": round(conv_l / dt, 3), "unit": "k-point solves/s", "solves": len(lj),
+        "converged": conv_l, "seconds": round(dt, 3),
+        "lobpcg_iterations_per_solve": round(lst.iterations / max(len(lj), 1), 3),
+        "max_rel_freq_err_vs_fp64_oracle_at_M": float(max(errs)) if errs else None,
+        "theta_multipass": {
+            "avg_launch_ms": round(th_avg * 1e3, 4), "launches": ths.timed_launches,
+            "algorithmic_bytes_per_launch": int(ths.timed_bytes / max(ths.timed_launches, 1)),
+            "achieved_gbs": round(ths.timed_bytes / max(ths.timed_ms * 1e-3, 1e-12) / 1e9, 1),
+            "frac_hbm": round(ths.timed_bytes / max(ths.timed_ms * 1e-3, 1e-12) / 1e9 / HBM_PEAK_GBS, 4)},
+        "rayleigh_ritz_gram": {
+            "avg_launch_ms": round(gr_avg * 1e3, 4), "launches": grs.timed_launches,
+            "algorithmic_flops_per_launch": int(grs.timed_flops / max(grs.timed_launches, 1)),
+            "achieved_tflops_fp64": round(grs.timed_flops / max(grs.timed_ms * 1e-3, 1e-12) / 1e12, 3),
+            "frac_fp64_peak": round(grs.timed_flops / max(grs.timed_ms * 1e-3, 1e-12) / 1e12
+                                    / FP64_PEAK_TFLOPS, 4)},
+        "m_point_512": {
+            "workload": "512x512, TM, 16 bands, k = M, one solve",
+            "seconds": round(dtm5, 3), "iterations": int(mres.iterations[0]),
+            "converged": int(mres.converged[0]),
+            "max_rel_freq_err_vs_fp64_oracle": rel_err(mres.bands, np.array([mref["omegas"]])),
+            "theta_achieved_gbs": round(mst.timed_bytes / max(mst.timed_ms * 1e-3, 1e-12) / 1e9, 1)}}
+    return result
+
+
+def learned_case(be, seed):
+    # the learned start: a network trained on the GPU on 16k converged
+    # subspaces of the sweep generator (64 x 64, 12 bands; 2670 geometries x
+    # TM+TE x 3 k-points), then 1024 held-out geometries x TM+TE, one
+    # k-point each, solved from the default start and from the predicted
+    # start with the fallback test (one probe iteration from the default
+    # start per solve; solves whose predicted start is worse are solved
+    # again from the default start); both timed end to end
+    import dataclasses as _dc
+    from blaze2d_amd import learned as LS
+    from blaze2d_amd.bandstructure import RunOptions, run_batch
+    t = time.perf_counter()
+    ldata = LS.training_set(be, 2670, seed=seed + 1, k_per_path=3)
+    t_data = time.perf_counter() - t
+    lnet = LS.CnnNet(be, 64, 64, 12, 4, max_batch=256, seed=1)
+    epochs = 20
+    t = time.perf_counter()
+    hist = LS.train(lnet, ldata, epochs, batch=64, lr=1e-3)
+    t_train = time.perf_counter() - t
+    held = LS.held_out_jobs(1024, seed=seed + 2)
+    lro = RunOptions(exit_rule=1)
+    LS.solve(be, lnet, held[:64], lro, slots=64)  # first launches
+    t = time.perf_counter()
+    hbase, hbst = run_batch(be, held, lro, slots=512, groups=2)
+    t_def = time.perf_counter() - t
+    lsol = LS.solve(be, lnet, held, lro, slots=512, groups=2)
+    lnop = LS.solve(be, lnet, held, lro, slots=512, groups=2, fallback=False)
+    # untimed: projections of the default-start solutions (held-out
+    # loss) and a tight solve (accuracy of both starts)
+    hproj, _ = run_batch(be, held, lro, slots=512, groups=2, pw_half=4)
+    hg, he = LS.network_inputs(be, held)
+    hloss = LS.mean_loss(lnet, hg, he, np.array([r.pw_coefs[0] for r in hproj]))
+    htight, _ = run_batch(be, [_dc.replace(j, tol=1e-10) for j in held], lro, slots=512, groups=2)
+
+    def _errs(rs):
+        out_e = []
+        for r, q in zip(rs, htight):
+            a, b = np.sort(r.bands[0]), np.sort(q.bands[0])
+            m = b > 1e-3  # the Gamma zero mode has no relative error
+            out_e.append(float(np.max(np.abs(a[m] - b[m]) / b[m])))
+        return np.array(out_e)
+
+    nh = len(held)
+    it_def = float(np.mean([r.iterations[0] for r in hbase]))
+    it_l = float(np.mean([r.iterations[0] for r in lsol.results]))
+    conv_def = sum(int(r.converged[0]) for r in hbase)
+    conv_l = sum(int(r.converged[0]) for r in lsol.results)
+    result = {
+        "workload": "sweep generator at 64x64, 12 bands, tol 1e-7; network: 4 periodic 3x3 conv layers x 32 "
+                    "channels (strides 1,2,2,2) + MLP embedding of (k1, k2, pol, lattice), linear head to "
+                    "12 x 81 complex plane-wave coefficients (|n1|,|n2| <= 4), BF16 weights, FP32 "
+                    "accumulation, projector loss, Adam; held out: 1024 geometries x TM+TE, one k-point "
+                    "each drawn from the closed path (2048 one-k-point solves), 512 slots",
+        "training": {"samples": int(ldata.embed.shape[0]), "geometries": 2670, "epochs": epochs,
+                     "data_seconds": round(t_data, 3), "train_seconds": round(t_train, 3),
+                     "train_samples_per_s": round(epochs * ldata.embed.shape[0] / t_train, 1),
+                     "loss_first_epoch": round(hist[0], 5), "loss_last_epoch": round(hist[-1], 5),
+                     "held_out_loss": round(hloss, 5)},
+        "default_start": {"value": round(conv_def / t_def, 3), "unit": "k-point solves/s",
+                          "lobpcg_iterations_per_solve": round(it_def, 3), "seconds": round(t_def, 4),
+                          "converged": conv_def},
+        "learned_start": {"value": round(conv_l / lsol.seconds, 3), "unit": "k-point solves/s",
+                          "lobpcg_iterations_per_solve": round(it_l, 3),
+                          "iterations_spent_per_solve_incl_probes": round(lsol.iterations_spent / nh, 3),
+                          "seconds": round(lsol.seconds, 4),
+                          "inference_seconds": round(lsol.inference_seconds, 4),
+                          "fallback_rate": round(1.0 - float(lsol.used_prediction.mean()), 5),
+                          "converged": conv_l},
+        "learned_start_without_fallback_probe": {
+            "value": round(sum(int(r.converged[0]) for r in lnop.results) / lnop.seconds, 3),
+            "unit": "k-point solves/s", "seconds": round(lnop.seconds, 4)},
+        "iteration_ratio": round(it_l / it_def, 4),
+        "solves_per_s_ratio": round((conv_l / lsol.seconds) / (conv_def / t_def), 4),
+        # against a tight solve: both starts stop on the reference's
+        # relative eigenvalue-change test at 1e-7, which a start close
+        # to the answer can pass early on a slowly moving top band
+        "rel_freq_err_vs_tol_1e-10": {
+            "default_start": {"max": float(_errs(hbase).max()),
+                              "solves_above_1e-6": int((_errs(hbase) > 1e-6).sum())},
+            "learned_start": {"max": float(_errs(lsol.results).max()),
+                              "solves_above_1e-6": int((_errs(lsol.results) > 1e-6).sum())}}}
+    lnet.close()
+    return result
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -133,6 +285,10 @@ def main():
     ap.add_argument("--no-large", action="store_true", help="skip the large combined case (256x256 x 32 bands, 512x512)")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned start (network training and the held-out evaluation)")
+    ap.add_argument("--large-only", action="store_true",
+                    help="only run the large combined case and print its figures (for profiling)")
+    ap.add_argument("--learned-only", action="store_true",
+                    help="only run the learned start (training and held-out evaluation) and print its figures")
     ap.add_argument("--timed-kernel", type=int, default=3, choices=(1, 2, 3, 4, 5),
                     help="kernel class timed for the roofline: 1 Theta operator, 2 dual Rayleigh-Ritz Gram, "
                          "3 Rayleigh-Ritz solve (the dominant kernel), 4 Ritz-update linear combinations")
@@ -159,6 +315,17 @@ def main():
     from blaze2d_amd import HipBackend
     from blaze2d_amd.bandstructure import RunOptions, run_batch
     from blaze2d_amd.sweep import res64_sweep
+
+    if args.large_only or args.learned_only:
+        be = HipBackend(0)
+        part = {}
+        if args.large_only:
+            part["large_case"] = large_case(be)
+        if args.learned_only:
+            part["learned_start"] = learned_case(be, args.seed)
+        print(json.dumps(part))
+        be.close()
+        return
 
     ropts = RunOptions(exit_rule=args.exit_rule)
 
@@ -350,151 +517,10 @@ def main():
                                      "lobpcg_iterations": single[2][2],
                                      "converged": int(single[2][0].converged.sum())},
             "max_rel_freq_diff_per_band": [float(f"{e:.3e}") for e in per_band]}
-        # the large combined case: 256 x 256, 32 bands, TE + TM, batched over the
-        # 63 distinct k-points of the 64-point Gamma-X-M-Gamma path (126 solves,
-        # each from the deterministic start), all bands converged; then the
-        # 512 x 512 TM 16-band solve at M.  Timed twice: the multi-pass FFT
-        # operator class (Theta) and the Rayleigh-Ritz Gram class, each with HIP
-        # events on its stream; frequencies at M checked against the oracle
-        # fixture.
         if not args.no_large:
-            import json as _json
-            from blaze2d_amd.sweep import large_case_jobs, large_case_m_point
-            lj = large_case_jobs()
-            lopts = RunOptions(exit_rule=1)
-            run_batch(be, lj[:2], lopts, slots=2)  # first launches of the 256 x 256 kernels
-            large = {}
-            for cls in (1, 2):
-                t = time.perf_counter()
-                lres, lst = run_batch(be, lj, lopts, slots=min(64, len(lj)), timed_kernel=cls)
-                dt = time.perf_counter() - t
-                large[cls] = (lres, lst, dt)
-            lres, lst, dt = large[1]
-            conv_l = sum(int(r.converged.sum()) for r in lres)
-            ths = large[1][1]
-            grs = large[2][1]
-            th_avg = ths.timed_ms * 1e-3 / max(ths.timed_launches, 1)
-            gr_avg = grs.timed_ms * 1e-3 / max(grs.timed_launches, 1)
-            with open(os.path.join(ROOT, "tests", "golden", "cfg5_oracle.json")) as f:
-                fx = _json.load(f)
-            errs = []
-            for c in fx["cases"]:
-                if c["grid"] == 256 and c["k"] == [0.5, 0.5]:
-                    i = next(i for i, j in enumerate(lj) if j.pol == c["pol"] and
-                             abs(j.k_path[0][0] - 0.5) < 1e-12 and abs(j.k_path[0][1] - 0.5) < 1e-12)
-                    errs.append(rel_err(lres[i].bands, np.array([c["omegas"]])))
-            mj = large_case_m_point()
-            run_batch(be, [mj], lopts, slots=1)
-            t = time.perf_counter()
-            (mres,), mst = run_batch(be, [mj], lopts, slots=1, timed_kernel=1)
-            dtm5 = time.perf_counter() - t
-            mref = next(c for c in fx["cases"] if c["grid"] == 512)
-            out["large_case"] = {
-                "workload": "square eps_bg 12, eps 1 rods r 0.2 at (0,0) and r 0.12 at (1/2,1/2); 256x256, 32 bands, "
-                            "TE+TM, 63 distinct k-points of the 64-point Gamma-X-M-Gamma path batched over k, "
-                            "tol 1e-7, all bands converged, FP64",
-                "value": round(conv_l / dt, 3), "unit": "k-point solves/s", "solves": len(lj),
-                "converged": conv_l, "seconds": round(dt, 3),
-                "lobpcg_iterations_per_solve": round(lst.iterations / max(len(lj), 1), 3),
-                "max_rel_freq_err_vs_fp64_oracle_at_M": float(max(errs)) if errs else None,
-                "theta_multipass": {
-                    "avg_launch_ms": round(th_avg * 1e3, 4), "launches": ths.timed_launches,
-                    "algorithmic_bytes_per_launch": int(ths.timed_bytes / max(ths.timed_launches, 1)),
-                    "achieved_gbs": round(ths.timed_bytes / max(ths.timed_ms * 1e-3, 1e-12) / 1e9, 1),
-                    "frac_hbm": round(ths.timed_bytes / max(ths.timed_ms * 1e-3, 1e-12) / 1e9 / HBM_PEAK_GBS, 4)},
-                "rayleigh_ritz_gram": {
-                    "avg_launch_ms": round(gr_avg * 1e3, 4), "launches": grs.timed_launches,
-                    "algorithmic_flops_per_launch": int(grs.timed_flops / max(grs.timed_launches, 1)),
-                    "achieved_tflops_fp64": round(grs.timed_flops / max(grs.timed_ms * 1e-3, 1e-12) / 1e12, 3),
-                    "frac_fp64_peak": round(grs.timed_flops / max(grs.timed_ms * 1e-3, 1e-12) / 1e12
-                                            / FP64_PEAK_TFLOPS, 4)},
-                "m_point_512": {
-                    "workload": "512x512, TM, 16 bands, k = M, one solve",
-                    "seconds": round(dtm5, 3), "iterations": int(mres.iterations[0]),
-                    "converged": int(mres.converged[0]),
-                    "max_rel_freq_err_vs_fp64_oracle": rel_err(mres.bands, np.array([mref["omegas"]])),
-                    "theta_achieved_gbs": round(mst.timed_bytes / max(mst.timed_ms * 1e-3, 1e-12) / 1e9, 1)}}
-        # the learned start: a network trained on the GPU on 16k converged
-        # subspaces of the sweep generator (64 x 64, 12 bands; 2670 geometries x
-        # TM+TE x 3 k-points), then 1024 held-out geometries x TM+TE, one
-        # k-point each, solved from the default start and from the predicted
-        # start with the fallback test (one probe iteration from the default
-        # start per solve; solves whose predicted start is worse are solved
-        # again from the default start); both timed end to end
+            out["large_case"] = large_case(be)
         if not args.no_learned:
-            from blaze2d_amd import learned as LS
-            import dataclasses as _dc
-            t = time.perf_counter()
-            ldata = LS.training_set(be, 2670, seed=args.seed + 1, k_per_path=3)
-            t_data = time.perf_counter() - t
-            lnet = LS.CnnNet(be, 64, 64, 12, 4, max_batch=256, seed=1)
-            epochs = 20
-            t = time.perf_counter()
-            hist = LS.train(lnet, ldata, epochs, batch=64, lr=1e-3)
-            t_train = time.perf_counter() - t
-            held = LS.held_out_jobs(1024, seed=args.seed + 2)
-            lro = RunOptions(exit_rule=1)
-            LS.solve(be, lnet, held[:64], lro, slots=64)  # first launches
-            t = time.perf_counter()
-            hbase, hbst = run_batch(be, held, lro, slots=512, groups=2)
-            t_def = time.perf_counter() - t
-            lsol = LS.solve(be, lnet, held, lro, slots=512, groups=2)
-            lnop = LS.solve(be, lnet, held, lro, slots=512, groups=2, fallback=False)
-            # untimed: projections of the default-start solutions (held-out
-            # loss) and a tight solve (accuracy of both starts)
-            hproj, _ = run_batch(be, held, lro, slots=512, groups=2, pw_half=4)
-            hg, he = LS.network_inputs(be, held)
-            hloss = LS.mean_loss(lnet, hg, he, np.array([r.pw_coefs[0] for r in hproj]))
-            htight, _ = run_batch(be, [_dc.replace(j, tol=1e-10) for j in held], lro, slots=512, groups=2)
-
-            def _errs(rs):
-                out_e = []
-                for r, q in zip(rs, htight):
-                    a, b = np.sort(r.bands[0]), np.sort(q.bands[0])
-                    m = b > 1e-3  # the Gamma zero mode has no relative error
-                    out_e.append(float(np.max(np.abs(a[m] - b[m]) / b[m])))
-                return np.array(out_e)
-
-            nh = len(held)
-            it_def = float(np.mean([r.iterations[0] for r in hbase]))
-            it_l = float(np.mean([r.iterations[0] for r in lsol.results]))
-            conv_def = sum(int(r.converged[0]) for r in hbase)
-            conv_l = sum(int(r.converged[0]) for r in lsol.results)
-            out["learned_start"] = {
-                "workload": "sweep generator at 64x64, 12 bands, tol 1e-7; network: 4 periodic 3x3 conv layers x 32 "
-                            "channels (strides 1,2,2,2) + MLP embedding of (k1, k2, pol, lattice), linear head to "
-                            "12 x 81 complex plane-wave coefficients (|n1|,|n2| <= 4), BF16 weights, FP32 "
-                            "accumulation, projector loss, Adam; held out: 1024 geometries x TM+TE, one k-point "
-                            "each drawn from the closed path (2048 one-k-point solves), 512 slots",
-                "training": {"samples": int(ldata.embed.shape[0]), "geometries": 2670, "epochs": epochs,
-                             "data_seconds": round(t_data, 3), "train_seconds": round(t_train, 3),
-                             "train_samples_per_s": round(epochs * ldata.embed.shape[0] / t_train, 1),
-                             "loss_first_epoch": round(hist[0], 5), "loss_last_epoch": round(hist[-1], 5),
-                             "held_out_loss": round(hloss, 5)},
-                "default_start": {"value": round(conv_def / t_def, 3), "unit": "k-point solves/s",
-                                  "lobpcg_iterations_per_solve": round(it_def, 3), "seconds": round(t_def, 4),
-                                  "converged": conv_def},
-                "learned_start": {"value": round(conv_l / lsol.seconds, 3), "unit": "k-point solves/s",
-                                  "lobpcg_iterations_per_solve": round(it_l, 3),
-                                  "iterations_spent_per_solve_incl_probes": round(lsol.iterations_spent / nh, 3),
-                                  "seconds": round(lsol.seconds, 4),
-                                  "inference_seconds": round(lsol.inference_seconds, 4),
-                                  "fallback_rate": round(1.0 - float(lsol.used_prediction.mean()), 5),
-                                  "converged": conv_l},
-                "learned_start_without_fallback_probe": {
-                    "value": round(sum(int(r.converged[0]) for r in lnop.results) / lnop.seconds, 3),
-                    "unit": "k-point solves/s", "seconds": round(lnop.seconds, 4)},
-                "iteration_ratio": round(it_l / it_def, 4),
-                "solves_per_s_ratio": round((conv_l / lsol.seconds) / (conv_def / t_def), 4),
-                # against a tight solve: both starts stop on the reference's
-                # relative eigenvalue-change test at 1e-7, which a start close
-                # to the answer can pass early on a slowly moving top band
-                "rel_freq_err_vs_tol_1e-10": {
-                    "default_start": {"max": float(_errs(hbase).max()),
-                                      "solves_above_1e-6": int((_errs(hbase) > 1e-6).sum())},
-                    "learned_start": {"max": float(_errs(lsol.results).max()),
-                                      "solves_above_1e-6": int((_errs(lsol.results) > 1e-6).sum())}}}
-            lnet.close()
+            out["learned_start"] = learned_case(be, args.seed)
         # the reference's own exit accounting on the same sweep (one pass)
         if args.exit_rule != 0:
             t = time.perf_counter()
