@@ -36,6 +36,7 @@ class Cnn {
     // targets [B][nb][ng] complex (interleaved doubles)
     double loss_and_backward(int B, const double *targets);
     void adam(float lr);
+    void dense_bwd_data(const float *dy, const __hip_bfloat16 *Wm, float *dx, int B, int K, int M);
     void get_output(int B, double *host) const;  // [B][nb][ng] complex
     void get_params(float *host) const;
     void set_params(const float *host);
@@ -61,6 +62,9 @@ class Cnn {
     float *act[LAYERS]{}, *gact[LAYERS]{};
     float *yout = nullptr, *gy = nullptr, *geh = nullptr, *gev = nullptr;
     double *tgt = nullptr, *lossb = nullptr;
+    float *wpart = nullptr;   // partial weight-gradient rows (one per workgroup)
+    float *dpart = nullptr;   // partial sums of the dense layers' input gradients
+    size_t dpart_floats = 0;
     double *d_eps = nullptr;  // dielectric scratch of inputs_from_geometries (allocated on first use)
     void *d_geo = nullptr;
     size_t loss_lds = 0;

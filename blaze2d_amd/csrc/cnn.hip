@@ -38,9 +38,10 @@ constexpr int CH = Cnn::CH;
 constexpr int EMB_IN = Cnn::EMB_IN;
 constexpr int EMB_OUT = Cnn::LAYERS * CH;
 constexpr int NT = 256;
-constexpr int LOSS_NT = 64;
+constexpr int LOSS_NT = 256;
 constexpr int LOSS_MAXNB = 16;
 constexpr size_t LOSS_MAX_LDS = 160 * 1024;
+constexpr int WGRAD_NT = 320;
 
 __device__ inline float bf(const __hip_bfloat16 &v) { return __bfloat162float(v); }
 __device__ inline int wrap(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
@@ -111,6 +112,104 @@ __global__ void conv_bwd_data_kernel(const float *__restrict__ gpre, const __hip
     }
 }
 
+// Weight and bias gradients as sums over (sample, output position):
+//   dw[co][ci][dy][dx] = sum gpre[b][co][oy][ox] in[b][ci][s oy + dy - 1][s ox + dx - 1],
+//   db[co] = sum gpre[b][co][oy][ox].
+// A chunk is R output rows of one sample.  Each workgroup stages a chunk's
+// gradient rows ([position][channel] in LDS) and the input rows they read
+// (s (R - 1) + 3 rows per input channel), and moves on to chunk + gridDim.x.
+// Thread t takes tap j = t / NGRP (j = 9 ci + 3 dy + dx, or j = 9 CIN for the
+// bias) and COPT = CH / NGRP output channels: per position one input read and
+// COPT gradient reads (the same for the wave's lanes of a group: broadcast)
+// feed COPT FMAs.  Its sums go to one row of `partial`, which
+// wgrad_reduce_kernel adds up in a fixed order.
+template <int CIN, int COPT>
+__global__ void __launch_bounds__(WGRAD_NT) wgrad_partial_kernel(const float *__restrict__ in,
+                                                                 const float *__restrict__ gpre,
+                                                                 float *__restrict__ partial, int B, int hin, int win,
+                                                                 int stride, int R) {
+    constexpr int PER = CIN * 9 + 1;
+    constexpr int NGRP = CH / COPT;
+    static_assert(PER * NGRP <= WGRAD_NT, "taps x channel groups exceed the workgroup");
+    static_assert(COPT % 4 == 0 && CH % COPT == 0, "channels per thread: a multiple of 4 dividing CH");
+    extern __shared__ __attribute__((aligned(16))) float wl[];
+    const int hout = hin / stride, wout = win / stride;
+    const int rows_in = stride * (R - 1) + 3;
+    const int npos = R * wout;
+    float *g_l = wl;                 // [npos][CH]
+    float *i_l = wl + npos * CH;     // [CIN][rows_in][win]
+    const int tid = threadIdx.x;
+    const int tap = tid / NGRP, cg = tid - tap * NGRP;
+    const bool active = tap < PER;
+    const bool bias = tap == PER - 1;
+    const int ci = active && !bias ? tap / 9 : 0, dy = active && !bias ? (tap % 9) / 3 : 0,
+              dx = active && !bias ? tap % 3 : 0;
+    float acc[COPT];
+#pragma unroll
+    for (int c = 0; c < COPT; ++c) acc[c] = 0.0f;
+    const int per_b = hout / R, nchunks = B * per_b;
+    for (int ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+        const int b = ch / per_b, oy0 = (ch - b * per_b) * R;
+        __syncthreads();  // the previous chunk's reads are done
+        for (int e = tid; e < npos * CH; e += WGRAD_NT) {
+            const int co = e / npos, p = e - co * npos;
+            g_l[p * CH + co] = gpre[((long)b * CH + co) * hout * wout + (long)oy0 * wout + p];
+        }
+        for (int e = tid; e < CIN * rows_in * win; e += WGRAD_NT) {
+            const int c = e / (rows_in * win), r = (e / win) % rows_in, x = e % win;
+            const int iy = wrap(stride * oy0 - 1 + r, hin);
+            i_l[e] = in[(((long)b * CIN + c) * hin + iy) * win + x];
+        }
+        __syncthreads();
+        if (active) {
+            const float *irow = i_l + (ci * rows_in + dy) * win;
+            for (int oy = 0; oy < R; ++oy) {
+                const float *ir = irow + stride * oy * win;
+                const float *gr = g_l + oy * wout * CH + cg * COPT;
+                for (int ox = 0; ox < wout; ++ox) {
+                    const float iv = bias ? 1.0f : ir[wrap(stride * ox + dx - 1, win)];
+                    const float *gp = gr + ox * CH;
+#pragma unroll
+                    for (int c = 0; c < COPT; c += 4) {
+                        const float4 g4 = *reinterpret_cast<const float4 *>(gp + c);
+                        acc[c] = fmaf(g4.x, iv, acc[c]);
+                        acc[c + 1] = fmaf(g4.y, iv, acc[c + 1]);
+                        acc[c + 2] = fmaf(g4.z, iv, acc[c + 2]);
+                        acc[c + 3] = fmaf(g4.w, iv, acc[c + 3]);
+                    }
+                }
+            }
+        }
+    }
+    if (active) {
+        // row layout e = co (9 CIN + 1) + tap
+#pragma unroll
+        for (int c = 0; c < COPT; ++c) partial[(long)blockIdx.x * CH * PER + (long)(cg * COPT + c) * PER + tap] = acc[c];
+    }
+}
+
+// gw / gb from the partial rows: e = co (9 cin + 1) + j, j = 9 cin the bias.
+// A workgroup takes 64 outputs; its four waves add every fourth row each
+// (consecutive lanes, consecutive outputs), then combine in a fixed order.
+__global__ void __launch_bounds__(NT) wgrad_reduce_kernel(const float *__restrict__ partial, int nrows, int cin,
+                                                          float *__restrict__ gw, float *__restrict__ gb) {
+    __shared__ float red[NT];
+    const int per = cin * 9 + 1, nout = CH * per;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int e = blockIdx.x * 64 + lane;
+    float t = 0.0f;
+    if (e < nout)
+        for (int r = wv; r < nrows; r += NT / 64) t += partial[(long)r * nout + e];
+    red[threadIdx.x] = t;
+    __syncthreads();
+    if (wv == 0 && e < nout) {
+        const float v = ((red[lane] + red[64 + lane]) + red[128 + lane]) + red[192 + lane];
+        const int co = e / per, j = e - co * per;
+        if (j == per - 1) gb[co] = v;
+        else gw[(long)co * (per - 1) + j] = v;
+    }
+}
+
 // block reduction of one float per thread (NT threads); result valid in thread 0
 __device__ inline float block_sum(float acc) {
     for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
@@ -121,39 +220,6 @@ __device__ inline float block_sum(float acc) {
     if (threadIdx.x == 0)
         for (int v = 0; v < NT / 64; ++v) t += part[v];
     return t;
-}
-
-// one workgroup per weight w[co][ci][dy][dx] (j < 9 cin) or per bias of co (j = 9 cin):
-// the sum over the batch and the output positions
-__global__ void __launch_bounds__(NT) conv_bwd_weight_kernel(const float *__restrict__ in,
-                                                             const float *__restrict__ gpre, float *__restrict__ gw,
-                                                             float *__restrict__ gb, int B, int cin, int hin, int win,
-                                                             int stride) {
-    const int hout = hin / stride, wout = win / stride;
-    const int per = cin * 9 + 1;
-    const int co = blockIdx.x / per, j = blockIdx.x - co * per;
-    const bool bias = j == cin * 9;
-    const int ci = bias ? 0 : j / 9, dy = bias ? 0 : (j % 9) / 3, dx = bias ? 0 : j % 3;
-    const long plane = (long)hout * wout;
-    const long npos = (long)B * plane;
-    float acc = 0.0f;
-    for (long p = threadIdx.x; p < npos; p += NT) {
-        const int b = (int)(p / plane);
-        const long q = p - (long)b * plane;
-        const int oy = (int)(q / wout), ox = (int)(q - (long)oy * wout);
-        const float g = gpre[((long)b * CH + co) * plane + q];
-        if (bias) {
-            acc += g;
-        } else {
-            const int iy = wrap(oy * stride + dy - 1, hin), ix = wrap(ox * stride + dx - 1, win);
-            acc = fmaf(g, in[(((long)b * cin + ci) * hin + iy) * win + ix], acc);
-        }
-    }
-    const float t = block_sum(acc);
-    if (threadIdx.x == 0) {
-        if (bias) gb[co] = t;
-        else gw[((long)co * cin + ci) * 9 + dy * 3 + dx] = t;
-    }
 }
 
 // out[b out_ld + c] = sum over the plane of g[b][c] (block b CH + c)
@@ -188,16 +254,61 @@ __global__ void __launch_bounds__(NT) dense_fwd_kernel(const float *__restrict__
     }
 }
 
-// dx[b][k] = sum_m dy[b][m] W[m][k]; consecutive threads take consecutive k
-__global__ void dense_bwd_data_kernel(const float *__restrict__ dy, const __hip_bfloat16 *__restrict__ W,
-                                      float *__restrict__ dx, int B, int K, int M) {
-    const long total = (long)B * K;
-    for (long e = (long)blockIdx.x * NT + threadIdx.x; e < total; e += (long)gridDim.x * NT) {
-        const int b = (int)(e / K), k = (int)(e - (long)b * K);
-        const float *dp = dy + (long)b * M;
-        float acc = 0.0f;
-        for (int m = 0; m < M; ++m) acc = fmaf(dp[m], bf(W[(long)m * K + k]), acc);
-        dx[e] = acc;
+// dx[b][k] = sum_m dy[b][m] W[m][k] as S partial sums over slices of m.
+// Workgroup (kt, s): k = NT kt + thread, m in slice s, samples in blocks of
+// DBB with that block's dy[:, slice] staged in LDS ([m][b], read as float4
+// broadcasts), so each weight read feeds DBB FMAs.  part[s][b][k].
+constexpr int DBB = 64;
+constexpr int MSUB = 128;  // m rows staged at a time (32 KB of LDS)
+__global__ void __launch_bounds__(NT) dense_bwd_data_partial_kernel(const float *__restrict__ dy,
+                                                                    const __hip_bfloat16 *__restrict__ W,
+                                                                    float *__restrict__ part, int B, int K, int M,
+                                                                    int ms) {
+    __shared__ __attribute__((aligned(16))) float dl[MSUB * DBB];  // [m][b]
+    const int k = blockIdx.x * NT + threadIdx.x;
+    const int s = blockIdx.y;
+    const int m0 = s * ms, m1 = min(M, m0 + ms);
+    for (int b0 = 0; b0 < B; b0 += DBB) {
+        const int nbb = min(DBB, B - b0);
+        float acc[DBB];
+#pragma unroll
+        for (int i = 0; i < DBB; ++i) acc[i] = 0.0f;
+        for (int mc = m0; mc < m1; mc += MSUB) {
+            const int nm = min(MSUB, m1 - mc);
+            __syncthreads();  // the previous sub-slice's reads are done
+            for (int e = threadIdx.x; e < nm * DBB; e += NT) {
+                const int mm = e / DBB, i = e - mm * DBB;
+                dl[e] = i < nbb ? dy[(long)(b0 + i) * M + mc + mm] : 0.0f;
+            }
+            __syncthreads();
+            if (k < K)
+                for (int mm = 0; mm < nm; ++mm) {
+                    const float w = bf(W[(long)(mc + mm) * K + k]);
+                    const float *dp = dl + mm * DBB;
+#pragma unroll
+                    for (int i = 0; i < DBB; i += 4) {
+                        const float4 d4 = *reinterpret_cast<const float4 *>(dp + i);
+                        acc[i] = fmaf(d4.x, w, acc[i]);
+                        acc[i + 1] = fmaf(d4.y, w, acc[i + 1]);
+                        acc[i + 2] = fmaf(d4.z, w, acc[i + 2]);
+                        acc[i + 3] = fmaf(d4.w, w, acc[i + 3]);
+                    }
+                }
+        }
+        if (k < K) {
+#pragma unroll
+            for (int i = 0; i < DBB; ++i)
+                if (i < nbb) part[((long)s * B + b0 + i) * K + k] = acc[i];
+        }
+    }
+}
+
+// dx[b][k] = sum_s part[s][b][k]
+__global__ void dense_bwd_data_reduce_kernel(const float *__restrict__ part, float *__restrict__ dx, int S, long n) {
+    for (long e = (long)blockIdx.x * NT + threadIdx.x; e < n; e += (long)gridDim.x * NT) {
+        float t = 0.0f;
+        for (int s = 0; s < S; ++s) t += part[(long)s * n + e];
+        dx[e] = t;
     }
 }
 
@@ -218,43 +329,52 @@ __global__ void dense_bwd_weight_kernel(const float *__restrict__ dy, const floa
     }
 }
 
-// In-place inverse of a Hermitian positive definite n x n matrix (row-major,
-// n <= LOSS_MAXNB) by Gauss-Jordan with partial pivoting, on thread 0.
-__device__ void small_inverse(double2 *A, double2 *I, int n) {
-    for (int i = 0; i < n * n; ++i) I[i] = make_double2((i / n) == (i % n) ? 1.0 : 0.0, 0.0);
+// In-place inverse of an n x n complex matrix (row-major, n <= LOSS_MAXNB)
+// by Gauss-Jordan with partial pivoting, the whole workgroup cooperating.
+// I: n x n workspace, F: n entries.
+__device__ void small_inverse(double2 *A, double2 *I, double2 *F, int n) {
+    const int tid = threadIdx.x;
+    for (int e = tid; e < n * n; e += LOSS_NT) I[e] = make_double2((e / n) == (e % n) ? 1.0 : 0.0, 0.0);
+    __syncthreads();
     for (int c = 0; c < n; ++c) {
-        int p = c;
-        double best = -1.0;
-        for (int r = c; r < n; ++r) {
-            const double m = A[r * n + c].x * A[r * n + c].x + A[r * n + c].y * A[r * n + c].y;
-            if (m > best) { best = m; p = r; }
-        }
-        if (p != c)
-            for (int j = 0; j < n; ++j) {
-                double2 t = A[c * n + j]; A[c * n + j] = A[p * n + j]; A[p * n + j] = t;
-                t = I[c * n + j]; I[c * n + j] = I[p * n + j]; I[p * n + j] = t;
+        if (tid == 0) {
+            int p = c;
+            double best = -1.0;
+            for (int r = c; r < n; ++r) {
+                const double m = A[r * n + c].x * A[r * n + c].x + A[r * n + c].y * A[r * n + c].y;
+                if (m > best) { best = m; p = r; }
             }
+            if (p != c)
+                for (int j = 0; j < n; ++j) {
+                    double2 t = A[c * n + j]; A[c * n + j] = A[p * n + j]; A[p * n + j] = t;
+                    t = I[c * n + j]; I[c * n + j] = I[p * n + j]; I[p * n + j] = t;
+                }
+        }
+        __syncthreads();
         const double2 d = A[c * n + c];
+        __syncthreads();  // every thread holds the pivot before row c changes
         const double dd = d.x * d.x + d.y * d.y;
         const double2 inv = make_double2(d.x / dd, -d.y / dd);
-        for (int j = 0; j < n; ++j) {
-            const double2 a = A[c * n + j], b = I[c * n + j];
-            A[c * n + j] = make_double2(a.x * inv.x - a.y * inv.y, a.x * inv.y + a.y * inv.x);
-            I[c * n + j] = make_double2(b.x * inv.x - b.y * inv.y, b.x * inv.y + b.y * inv.x);
+        for (int j = tid; j < 2 * n; j += LOSS_NT) {
+            double2 *x = j < n ? &A[c * n + j] : &I[c * n + j - n];
+            const double2 a = *x;
+            *x = make_double2(a.x * inv.x - a.y * inv.y, a.x * inv.y + a.y * inv.x);
         }
-        for (int r = 0; r < n; ++r) {
+        for (int r = tid; r < n; r += LOSS_NT) F[r] = A[r * n + c];
+        __syncthreads();
+        for (int e = tid; e < n * 2 * n; e += LOSS_NT) {
+            const int r = e / (2 * n), j = e - r * 2 * n;
             if (r == c) continue;
-            const double2 f = A[r * n + c];
-            for (int j = 0; j < n; ++j) {
-                const double2 a = A[c * n + j], b = I[c * n + j];
-                A[r * n + j].x -= f.x * a.x - f.y * a.y;
-                A[r * n + j].y -= f.x * a.y + f.y * a.x;
-                I[r * n + j].x -= f.x * b.x - f.y * b.y;
-                I[r * n + j].y -= f.x * b.y + f.y * b.x;
-            }
+            const double2 f = F[r];
+            double2 *x = j < n ? &A[r * n + j] : &I[r * n + j - n];
+            const double2 b = j < n ? A[c * n + j] : I[c * n + j - n];
+            x->x -= f.x * b.x - f.y * b.y;
+            x->y -= f.x * b.y + f.y * b.x;
         }
+        __syncthreads();
     }
-    for (int i = 0; i < n * n; ++i) A[i] = I[i];
+    for (int e = tid; e < n * n; e += LOSS_NT) A[e] = I[e];
+    __syncthreads();
 }
 
 // One workgroup per sample, FP64.  y: [B][nb][ng][2] float (the head output),
@@ -274,6 +394,7 @@ __global__ void __launch_bounds__(LOSS_NT) projector_loss_kernel(const float *__
     double2 *A = Gu + nb * nb;
     double2 *T = A + nb * nb;
     double2 *S = T + nb * nb;  // inversion workspace
+    double2 *Fv = S + nb * nb;
     const float *yb = y + (size_t)b * nb * ng * 2;
     const double *ub = u + (size_t)b * nb * ng * 2;
     for (int e = tid; e < ng * nb; e += LOSS_NT) {
@@ -308,10 +429,10 @@ __global__ void __launch_bounds__(LOSS_NT) projector_loss_kernel(const float *__
         double ty = 0.0, tu = 0.0;
         for (int i = 0; i < nb; ++i) { ty += Gy[i * nb + i].x; tu += Gu[i * nb + i].x; }
         for (int i = 0; i < nb; ++i) { Gy[i * nb + i].x += 1e-10 * ty / nb; Gu[i * nb + i].x += 1e-10 * tu / nb; }
-        small_inverse(Gy, S, nb);
-        small_inverse(Gu, S, nb);
     }
     __syncthreads();
+    small_inverse(Gy, S, Fv, nb);
+    small_inverse(Gu, S, Fv, nb);
     small_prod(Gu, A, T);  // (U^H U)^-1 U^H Y
     __syncthreads();
     for (int e = tid; e < ng * nb; e += LOSS_NT) {  // R = U T = P_U Y
@@ -384,9 +505,25 @@ __global__ void fill_kernel(float *__restrict__ p, long n) {
     for (long e = (long)blockIdx.x * NT + threadIdx.x; e < n; e += (long)gridDim.x * NT) p[e] = 0.0f;
 }
 
+constexpr int WGRAD_MAX_WG = 512;
+constexpr size_t WGRAD_MAX_LDS = 64 * 1024;
+
+size_t wgrad_lds_bytes(int cin, int R, int win, int wout, int stride) {
+    return sizeof(float) * ((size_t)R * wout * CH + (size_t)cin * (stride * (R - 1) + 3) * win);
+}
+
+// rows per chunk: the largest divisor of the output height whose staging fits
+// in 64 KB of LDS (two workgroups per CU)
+int wgrad_rows(int cin, int hin, int win, int stride) {
+    const int hout = hin / stride, wout = win / stride;
+    for (int R = hout; R >= 1; --R)
+        if (hout % R == 0 && wgrad_lds_bytes(cin, R, win, wout, stride) <= WGRAD_MAX_LDS) return R;
+    throw std::runtime_error("CNN layer too wide for the weight-gradient staging");
+}
+
 unsigned grid_for(long n) { return (unsigned)std::max<long>(1, std::min<long>((n + NT - 1) / NT, 16384)); }
 
-size_t loss_lds_bytes(int nb, int ng) { return sizeof(double2) * (3 * (size_t)ng * nb + 5 * (size_t)nb * nb); }
+size_t loss_lds_bytes(int nb, int ng) { return sizeof(double2) * (3 * (size_t)ng * nb + 5 * (size_t)nb * nb + nb); }
 
 }  // namespace
 
@@ -454,9 +591,21 @@ Cnn::Cnn(int h_, int w_, int nb_, int pw_half_, int max_batch_, unsigned long lo
     geh = (float *)dal(sizeof(float) * Bm * CH);
     gev = (float *)dal(sizeof(float) * Bm * EMB_OUT);
     tgt = (double *)dal(sizeof(double) * Bm * OUT);
+    wpart = (float *)dal(sizeof(float) * WGRAD_MAX_WG * CH * (CH * 9 + 1));
+    // dense backward partials: at least one slice per sample block for either layer
+    dpart_floats = std::max<size_t>((size_t)64 << 20 >> 2, (size_t)Bm * std::max(F, CH));
+    dpart = (float *)dal(sizeof(float) * dpart_floats);
+    for (int l = 0; l < LAYERS; ++l) {  // the staging of every layer must fit
+        const int cin = l == 0 ? 2 : CH, hin = l == 0 ? H : (H >> (l - 1)), win = l == 0 ? W : (W >> (l - 1));
+        (void)wgrad_rows(cin, hin, win, l == 0 ? 1 : 2);
+    }
     lossb = (double *)dal(sizeof(double) * Bm);
     BH_HIP_CHECK(hipFuncSetAttribute((const void *)projector_loss_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)LOSS_MAX_LDS));
+    BH_HIP_CHECK(hipFuncSetAttribute((const void *)wgrad_partial_kernel<2, 4>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)WGRAD_MAX_LDS));
+    BH_HIP_CHECK(hipFuncSetAttribute((const void *)wgrad_partial_kernel<CH, CH>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)WGRAD_MAX_LDS));
     set_params(hp.data());
 }
 
@@ -507,6 +656,23 @@ void Cnn::forward(int B) {
     BH_HIP_CHECK(hipGetLastError());
 }
 
+// dx = dy W (dy: B x M, W: M x K): partial sums over S slices of M, enough
+// workgroups to fill the device, within the partial buffer
+void Cnn::dense_bwd_data(const float *dy, const __hip_bfloat16 *Wm, float *dx, int B, int K, int M) {
+    const int kt = (K + NT - 1) / NT;
+    long S = std::max(1, 1024 / kt);
+    S = std::min<long>(S, M);
+    S = std::min<long>(S, (long)(dpart_floats / ((size_t)B * K)));
+    if (S < 1) throw std::runtime_error("CNN dense backward: partial buffer too small");
+    const int ms = (int)((M + S - 1) / S);
+    S = (M + ms - 1) / ms;
+    hipLaunchKernelGGL(dense_bwd_data_partial_kernel, dim3(kt, (unsigned)S), dim3(NT), 0, st, dy, Wm, dpart, B, K, M,
+                       ms);
+    const long n = (long)B * K;
+    hipLaunchKernelGGL(dense_bwd_data_reduce_kernel, dim3(grid_for(n)), dim3(NT), 0, st, dpart, dx, (int)S, n);
+    BH_HIP_CHECK(hipGetLastError());
+}
+
 double Cnn::loss_and_backward(int B, const double *targets) {
     if (B < 1 || B > max_batch) throw std::runtime_error("CNN batch outside 1..max_batch");
     BH_HIP_CHECK(hipMemcpyAsync(tgt, targets, sizeof(double) * B * OUT, hipMemcpyHostToDevice, st));
@@ -514,8 +680,7 @@ double Cnn::loss_and_backward(int B, const double *targets) {
     const __hip_bfloat16 *wb = pbf;
     hipLaunchKernelGGL(dense_bwd_weight_kernel, dim3(grid_for((long)OUT * (F + 1))), dim3(NT), 0, st, gy,
                        act[LAYERS - 1], grads + off_h, grads + off_hb, B, F, OUT);
-    hipLaunchKernelGGL(dense_bwd_data_kernel, dim3(grid_for((long)B * F)), dim3(NT), 0, st, gy, wb + off_h,
-                       gact[LAYERS - 1], B, F, OUT);
+    dense_bwd_data(gy, wb + off_h, gact[LAYERS - 1], B, F, OUT);
     for (int l = LAYERS - 1; l >= 0; --l) {
         const int cin = l == 0 ? 2 : CH, hin = l == 0 ? H : (H >> (l - 1)), win = l == 0 ? W : (W >> (l - 1));
         const int stride = l == 0 ? 1 : 2;
@@ -524,16 +689,29 @@ double Cnn::loss_and_backward(int B, const double *targets) {
         hipLaunchKernelGGL(relu_bwd_kernel, dim3(grid_for(n)), dim3(NT), 0, st, gact[l], act[l], n);
         hipLaunchKernelGGL(channel_sum_kernel, dim3(B * CH), dim3(NT), 0, st, gact[l], gev + l * CH, plane, EMB_OUT);
         const float *in = l == 0 ? x0 : act[l - 1];
-        hipLaunchKernelGGL(conv_bwd_weight_kernel, dim3(CH * (cin * 9 + 1)), dim3(NT), 0, st, in, gact[l],
-                           grads + off_c[l], grads + off_cb[l], B, cin, hin, win, stride);
+        {
+            const int hout = hin / stride, wout = win / stride;
+            const int R = wgrad_rows(cin, hin, win, stride);
+            const int nchunks = B * (hout / R);
+            const int nwg = std::min(nchunks, WGRAD_MAX_WG);
+            const size_t lds = wgrad_lds_bytes(cin, R, win, wout, stride);
+            if (cin == 2)
+                hipLaunchKernelGGL((wgrad_partial_kernel<2, 4>), dim3(nwg), dim3(WGRAD_NT), lds, st, in, gact[l],
+                                   wpart, B, hin, win, stride, R);
+            else
+                hipLaunchKernelGGL((wgrad_partial_kernel<CH, CH>), dim3(nwg), dim3(WGRAD_NT), lds, st, in, gact[l],
+                                   wpart, B, hin, win, stride, R);
+            const int nout = CH * (cin * 9 + 1);
+            hipLaunchKernelGGL(wgrad_reduce_kernel, dim3((nout + 63) / 64), dim3(NT), 0, st, wpart, nwg, cin,
+                               grads + off_c[l], grads + off_cb[l]);
+        }
         if (l > 0)
             hipLaunchKernelGGL(conv_bwd_data_kernel, dim3(grid_for((long)B * cin * hin * win)), dim3(NT), 0, st,
                                gact[l], wb + off_c[l], gact[l - 1], B, cin, hin, win, stride);
     }
     hipLaunchKernelGGL(dense_bwd_weight_kernel, dim3(grid_for((long)EMB_OUT * (CH + 1))), dim3(NT), 0, st, gev, eh,
                        grads + off_e2, grads + off_e2b, B, CH, EMB_OUT);
-    hipLaunchKernelGGL(dense_bwd_data_kernel, dim3(grid_for((long)B * CH)), dim3(NT), 0, st, gev, wb + off_e2, geh,
-                       B, CH, EMB_OUT);
+    dense_bwd_data(gev, wb + off_e2, geh, B, CH, EMB_OUT);
     hipLaunchKernelGGL(relu_bwd_kernel, dim3(grid_for((long)B * CH)), dim3(NT), 0, st, geh, eh, (long)B * CH);
     hipLaunchKernelGGL(dense_bwd_weight_kernel, dim3(grid_for((long)CH * (EMB_IN + 1))), dim3(NT), 0, st, geh, e0,
                        grads + off_e1, grads + off_e1b, B, EMB_IN, CH);
