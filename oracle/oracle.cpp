@@ -1583,9 +1583,10 @@ int oracle_bandstructure_residuals(const OracleJob *job, double *omegas, int *it
 }
 
 int oracle_bandstructure_warm(const OracleJob *job, const double *warm_pw, int warm_count, int pw_half,
-                              double *omegas, int *iters, int *converged) {
+                              double *omegas, int *iters, int *converged, double *rel_residuals,
+                              int *rel_residual_iters) {
     if (!job || job->nx <= 0 || job->ny <= 0 || job->n_bands <= 0 || pw_half < 0 || pw_half > 8) return -1;
-    return run_core(job, omegas, iters, converged, nullptr, nullptr, warm_pw, warm_count, pw_half);
+    return run_core(job, omegas, iters, converged, rel_residuals, rel_residual_iters, warm_pw, warm_count, pw_half);
 }
 
 int oracle_dielectric(const OracleJob *job, double *eps, double *inv_eps, double *tensor) {

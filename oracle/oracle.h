@@ -55,9 +55,11 @@ int oracle_bandstructure(const OracleJob *job, double *omegas, int *iters, int *
  * initialize_block, initialization.rs:248-309): warm_count fields given by
  * plane-wave coefficients on |n1|, |n2| <= pw_half (index (n2 + h)(2h + 1) +
  * n1 + h, interleaved complex), x(r) = sum_G c_G exp(+2 pi i (n1 ix / nx +
- * n2 iy / ny)). */
+ * n2 iy / ny)).  rel_residuals / rel_residual_iters (may be NULL) as in
+ * oracle_bandstructure_residuals. */
 int oracle_bandstructure_warm(const OracleJob *job, const double *warm_pw, int warm_count, int pw_half,
-                              double *omegas, int *iters, int *converged);
+                              double *omegas, int *iters, int *converged, double *rel_residuals,
+                              int *rel_residual_iters);
 int oracle_bandstructure_residuals(const OracleJob *job, double *omegas, int *iters, int *converged,
                                    double *rel_residuals, int *rel_residual_iters);
 

@@ -67,7 +67,7 @@ def lib() -> ctypes.CDLL:
         L.oracle_bandstructure.restype = ctypes.c_int
         L.oracle_bandstructure_residuals.argtypes = [ctypes.POINTER(_OracleJob), dp, ip, ip, dp, ip]
         L.oracle_bandstructure_warm.argtypes = [ctypes.POINTER(_OracleJob), dp, ctypes.c_int, ctypes.c_int, dp, ip,
-                                                ip]
+                                                ip, dp, ip]
         L.oracle_bandstructure_residuals.restype = ctypes.c_int
         L.oracle_dielectric.argtypes = [ctypes.POINTER(_OracleJob), dp, dp, dp]
         L.oracle_dielectric.restype = ctypes.c_int
@@ -189,20 +189,25 @@ def bandstructure_residuals(job: Job):
     return om, it, cv, tot, rel, rit
 
 
-def bandstructure_warm(job: Job, warm_pw: np.ndarray, pw_half: int):
+def bandstructure_warm(job: Job, warm_pw: np.ndarray, pw_half: int, residuals: bool = False):
     """bandstructure() with warm vectors for the first k-point given as
     plane-wave coefficients [count, (2 pw_half + 1)^2] complex (the learned
-    start's layout): (omegas, iterations, converged, total)."""
+    start's layout): (omegas, iterations, converged, total), plus
+    (rel[nk, nb], rel_iter[nk]) as in bandstructure_residuals when residuals."""
     cj, kp = job.to_c()
     nk = cj.nk
     w = np.ascontiguousarray(warm_pw, dtype=np.complex128).reshape(-1, (2 * pw_half + 1) ** 2)
     om = np.zeros((nk, job.n_bands))
     it = np.zeros(nk, dtype=np.int32)
     cv = np.zeros(nk, dtype=np.int32)
+    rel = np.zeros((nk, job.n_bands))
+    rit = np.zeros(nk, dtype=np.int32)
     tot = lib().oracle_bandstructure_warm(ctypes.byref(cj), _dp(w.view(np.float64)), w.shape[0], pw_half, _dp(om),
-                                          _ip(it), _ip(cv))
+                                          _ip(it), _ip(cv), _dp(rel), _ip(rit))
     if tot < 0:
         raise RuntimeError("oracle_bandstructure_warm failed")
+    if residuals:
+        return om, it, cv, tot, rel, rit
     return om, it, cv, tot
 
 

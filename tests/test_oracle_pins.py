@@ -312,6 +312,42 @@ def test_warm_start_from_exact_plane_waves(pol):
     assert np.abs(np.sort(omn[0]) - want).max() > 1e-3 * want.max()
 
 
+def _synth(coefs, h, nx, ny):
+    """x(r) = sum_G c_G exp(+2 pi i (n1 ix / nx + n2 iy / ny)), index (n2 + h)(2h + 1) + n1 + h."""
+    iy, ix = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+    x = np.zeros((ny, nx), complex)
+    for n2 in range(-h, h + 1):
+        for n1 in range(-h, h + 1):
+            x += coefs[(n2 + h) * (2 * h + 1) + n1 + h] * np.exp(2j * np.pi * (n1 * ix / nx + n2 * iy / ny))
+    return x.ravel()
+
+
+@pytest.mark.parametrize("pol", ["TM", "TE"])
+def test_relative_residual_one_vector_closed_form(pol):
+    # one warm vector, block of one, one iteration: the block is x / ||x||_B,
+    # theta = x^H A x / x^H B x, and the reported relative residual of the log
+    # iteration 1 is ||A x - theta B x||_B / |theta| (eigensolver.rs:744-779)
+    r = np.random.default_rng(11)
+    h = 2
+    coefs = (r.standard_normal(25) + 1j * r.standard_normal(25)) / (1 + np.arange(25))
+    job = O.Job(lattice="square", eps_bg=9.0, atoms=[(0.5, 0.5, 0.3, 1.5)], nx=16, ny=16, pol=pol, n_bands=1,
+                block_size=1, max_iter=1, tol=1e-7, kpath=np.array([(0.21, 0.07)]))
+    om, it, cv, _, rel, rit = O.bandstructure_warm(job, coefs[None, :], h, residuals=True)
+    x = _synth(coefs, h, 16, 16)[None, :]
+    ax = O.operator_apply(job, (0.21, 0.07), "A", x)[0]
+    bx = O.operator_apply(job, (0.21, 0.07), "B", x)[0]
+    x = x[0]
+    nb = np.vdot(x, bx).real
+    theta = np.vdot(x, ax).real / nb
+    res = (ax - theta * bx) / np.sqrt(nb)
+    bres = O.operator_apply(job, (0.21, 0.07), "B", res[None, :])[0]
+    want = np.sqrt(np.vdot(res, bres).real) / abs(theta)
+    assert rit[0] == 1 and it[0] == 1
+    assert abs(rel[0, 0] - want) <= 1e-12 * want
+    # the iteration's Rayleigh-Ritz over [x, W] can only lower the Ritz value
+    assert 0 < om[0, 0] < np.sqrt(theta)
+
+
 # --- full path vs the reference's own FP64 output ---------------------------
 @pytest.mark.parametrize("pol", ["TM", "TE"])
 def test_series6_golden_frequencies(pol):
