@@ -115,7 +115,7 @@ def cpu_model():
     return "unknown"
 
 
-def large_case(be):
+def large_case(be, seed=None):
     # the large combined case: 256 x 256, 32 bands, TE + TM, batched over the
     # 63 distinct k-points of the 64-point Gamma-X-M-Gamma path (126 solves,
     # each from the deterministic start), all bands converged; then the
@@ -180,7 +180,59 @@ def large_case(be):
             "converged": int(mres.converged[0]),
             "max_rel_freq_err_vs_fp64_oracle": rel_err(mres.bands, np.array([mref["omegas"]])),
             "theta_achieved_gbs": round(mst.timed_bytes / max(mst.timed_ms * 1e-3, 1e-12) / 1e9, 1)}}
+    if seed is not None:
+        result["learned_start"] = large_learned(be, seed, lj, lopts, fx)
     return result
+
+
+def large_learned(be, seed, lj, lopts, fx):
+    """The learned start on the large case: a 32-vector network trained at
+    64 x 64 (sweep generator, 32 bands, 16,020 converged subspaces), its input
+    eps evaluated on the network's 64 x 64 grid (the large grid's eps
+    downsampled), its prediction synthesised on the 256 x 256 grid (prolonged)
+    as the first 32 columns of each block; the 126 solves again, timed end to
+    end (inference, probe iteration, fallbacks)."""
+    from blaze2d_amd import learned as LS
+    t = time.perf_counter()
+    data = LS.training_set(be, 2670, seed=seed + 3, n_bands=32, k_per_path=3)
+    t_data = time.perf_counter() - t
+    net = LS.CnnNet(be, 64, 64, 32, 4, max_batch=128, seed=1)
+    epochs = 20
+    t = time.perf_counter()
+    hist = LS.train(net, data, epochs, batch=64, lr=1e-3)
+    t_train = time.perf_counter() - t
+    from blaze2d_amd.bandstructure import run_batch
+    t = time.perf_counter()
+    _, st_default = run_batch(be, lj, lopts, slots=min(64, len(lj)))  # default start, untimed kernels
+    t_default = time.perf_counter() - t
+    sol = LS.solve(be, net, lj, lopts, slots=min(64, len(lj)), groups=4)
+    nop = LS.solve(be, net, lj, lopts, slots=min(64, len(lj)), groups=4, fallback=False)
+    net.close()
+    errs = []
+    for c in fx["cases"]:
+        if c["grid"] == 256 and c["k"] == [0.5, 0.5]:
+            i = next(i for i, j in enumerate(lj) if j.pol == c["pol"] and
+                     abs(j.k_path[0][0] - 0.5) < 1e-12 and abs(j.k_path[0][1] - 0.5) < 1e-12)
+            errs.append(rel_err(sol.results[i].bands, np.array([c["omegas"]])))
+    n = len(lj)
+    conv = sum(int(r.converged[0]) for r in sol.results)
+    it_l = float(np.mean([r.iterations[0] for r in sol.results]))
+    it_d = st_default.iterations / max(n, 1)
+    return {
+        "training": {"samples": int(data.embed.shape[0]), "n_vectors": 32, "epochs": epochs,
+                     "data_seconds": round(t_data, 3), "train_seconds": round(t_train, 3),
+                     "loss_first_epoch": round(hist[0], 5), "loss_last_epoch": round(hist[-1], 5)},
+        "value": round(conv / sol.seconds, 3), "unit": "k-point solves/s", "converged": conv,
+        "seconds": round(sol.seconds, 4), "inference_seconds": round(sol.inference_seconds, 4),
+        "lobpcg_iterations_per_solve": round(it_l, 3),
+        "iterations_spent_per_solve_incl_probes": round(sol.iterations_spent / n, 3),
+        "fallback_rate": round(1.0 - float(sol.used_prediction.mean()), 5),
+        "without_fallback_probe": {"value": round(sum(int(r.converged[0]) for r in nop.results) / nop.seconds, 3),
+                                   "seconds": round(nop.seconds, 4)},
+        "default_start": {"value": round(n / t_default, 3), "lobpcg_iterations_per_solve": round(it_d, 3)},
+        "iteration_ratio": round(it_l / it_d, 4),
+        "solves_per_s_ratio": round((conv / sol.seconds) / (n / t_default), 4),
+        "max_rel_freq_err_vs_fp64_oracle_at_M": float(max(errs)) if errs else None}
 
 
 def learned_case(be, seed):
@@ -320,7 +372,7 @@ def main():
         be = HipBackend(0)
         part = {}
         if args.large_only:
-            part["large_case"] = large_case(be)
+            part["large_case"] = large_case(be, None if args.no_learned else args.seed)
         if args.learned_only:
             part["learned_start"] = learned_case(be, args.seed)
         print(json.dumps(part))
@@ -518,7 +570,7 @@ def main():
                                      "converged": int(single[2][0].converged.sum())},
             "max_rel_freq_diff_per_band": [float(f"{e:.3e}") for e in per_band]}
         if not args.no_large:
-            out["large_case"] = large_case(be)
+            out["large_case"] = large_case(be, None if args.no_learned else args.seed)
         if not args.no_learned:
             out["learned_start"] = learned_case(be, args.seed)
         # the reference's own exit accounting on the same sweep (one pass)

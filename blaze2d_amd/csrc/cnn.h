@@ -18,7 +18,7 @@ class Cnn {
     static constexpr int LAYERS = 4;   // conv layers (stride 1, then 2, 2, 2)
     static constexpr int EMB_IN = 4;   // k1, k2, polarisation (0 TM, 1 TE), lattice (0 square, 1 hex)
 
-    // h, w: input grid (multiples of 8); nb: predicted vectors (1..16);
+    // h, w: input grid (multiples of 8); nb: predicted vectors (1..32);
     // pw_half: plane waves |n1|, |n2| <= pw_half (0..8); max_batch: samples per call
     Cnn(int h, int w, int nb, int pw_half, int max_batch, unsigned long long seed, hipStream_t st);
     ~Cnn();

@@ -39,7 +39,7 @@ constexpr int EMB_IN = Cnn::EMB_IN;
 constexpr int EMB_OUT = Cnn::LAYERS * CH;
 constexpr int NT = 256;
 constexpr int LOSS_NT = 256;
-constexpr int LOSS_MAXNB = 16;
+constexpr int LOSS_MAXNB = 32;
 constexpr size_t LOSS_MAX_LDS = 160 * 1024;
 constexpr int WGRAD_NT = 320;
 
@@ -377,8 +377,9 @@ __device__ void small_inverse(double2 *A, double2 *I, double2 *F, int n) {
     __syncthreads();
 }
 
-// One workgroup per sample, FP64.  y: [B][nb][ng][2] float (the head output),
-// u: [B][nb][ng][2] double (targets); writes loss[b] and gy = d(mean loss)/dy.
+// One workgroup per sample.  y: [B][nb][ng][2] float (the head output), u:
+// [B][nb][ng][2] double (targets, held as FP32 in LDS like y); every product
+// accumulates in FP64.  Writes loss[b] and gy = d(mean loss)/dy.
 __global__ void __launch_bounds__(LOSS_NT) projector_loss_kernel(const float *__restrict__ y,
                                                                  const double *__restrict__ u,
                                                                  double *__restrict__ loss, float *__restrict__ gy,
@@ -386,24 +387,23 @@ __global__ void __launch_bounds__(LOSS_NT) projector_loss_kernel(const float *__
     const int b = blockIdx.x;
     const int tid = threadIdx.x;
     extern __shared__ __attribute__((aligned(16))) double lsm[];
-    double2 *Y = reinterpret_cast<double2 *>(lsm);  // [ng][nb]
-    double2 *U = Y + (size_t)ng * nb;               // [ng][nb]
-    double2 *R = U + (size_t)ng * nb;               // P_U Y, then (I - P_Y) P_U Y
+    double2 *R = reinterpret_cast<double2 *>(lsm);  // [ng][nb]: P_U Y, then (I - P_Y) P_U Y
     double2 *Gy = R + (size_t)ng * nb;              // Y^H Y, then its inverse
     double2 *Gu = Gy + nb * nb;                     // U^H U, then its inverse
     double2 *A = Gu + nb * nb;
-    double2 *T = A + nb * nb;
-    double2 *S = T + nb * nb;  // inversion workspace
-    double2 *Fv = S + nb * nb;
+    double2 *T = A + nb * nb;                       // also the inversions' workspace
+    double2 *Fv = T + nb * nb;
+    float2 *Y = reinterpret_cast<float2 *>(Fv + nb);  // [ng][nb]
+    float2 *U = Y + (size_t)ng * nb;                  // [ng][nb]
     const float *yb = y + (size_t)b * nb * ng * 2;
     const double *ub = u + (size_t)b * nb * ng * 2;
     for (int e = tid; e < ng * nb; e += LOSS_NT) {
         const int g = e / nb, j = e - g * nb;
-        Y[e] = make_double2(yb[2 * (j * ng + g)], yb[2 * (j * ng + g) + 1]);
-        U[e] = make_double2(ub[2 * (j * ng + g)], ub[2 * (j * ng + g) + 1]);
+        Y[e] = make_float2(yb[2 * (j * ng + g)], yb[2 * (j * ng + g) + 1]);
+        U[e] = make_float2((float)ub[2 * (j * ng + g)], (float)ub[2 * (j * ng + g) + 1]);
     }
     __syncthreads();
-    auto herm_prod = [&](const double2 *P, const double2 *Q, double2 *out) {  // P^H Q, nb x nb
+    auto herm_prod = [&](const auto *P, const auto *Q, double2 *out) {  // P^H Q, nb x nb
         for (int e = tid; e < nb * nb; e += LOSS_NT) {
             const int i = e / nb, j = e - i * nb;
             double re = 0.0, im = 0.0;
@@ -431,8 +431,8 @@ __global__ void __launch_bounds__(LOSS_NT) projector_loss_kernel(const float *__
         for (int i = 0; i < nb; ++i) { Gy[i * nb + i].x += 1e-10 * ty / nb; Gu[i * nb + i].x += 1e-10 * tu / nb; }
     }
     __syncthreads();
-    small_inverse(Gy, S, Fv, nb);
-    small_inverse(Gu, S, Fv, nb);
+    small_inverse(Gy, T, Fv, nb);
+    small_inverse(Gu, T, Fv, nb);
     small_prod(Gu, A, T);  // (U^H U)^-1 U^H Y
     __syncthreads();
     for (int e = tid; e < ng * nb; e += LOSS_NT) {  // R = U T = P_U Y
@@ -455,7 +455,8 @@ __global__ void __launch_bounds__(LOSS_NT) projector_loss_kernel(const float *__
     for (int e = tid; e < ng * nb; e += LOSS_NT) {
         const int g = e / nb, j = e - g * nb;
         double re = R[e].x, im = R[e].y;
-        for (int k = 0; k < nb; ++k) cmac(re, im, -Y[g * nb + k].x, -Y[g * nb + k].y, T[k * nb + j].x, T[k * nb + j].y);
+        for (int k = 0; k < nb; ++k)
+            cmac(re, im, -(double)Y[g * nb + k].x, -(double)Y[g * nb + k].y, T[k * nb + j].x, T[k * nb + j].y);
         R[e] = make_double2(re, im);
     }
     __syncthreads();
@@ -523,7 +524,9 @@ int wgrad_rows(int cin, int hin, int win, int stride) {
 
 unsigned grid_for(long n) { return (unsigned)std::max<long>(1, std::min<long>((n + NT - 1) / NT, 16384)); }
 
-size_t loss_lds_bytes(int nb, int ng) { return sizeof(double2) * (3 * (size_t)ng * nb + 5 * (size_t)nb * nb + nb); }
+size_t loss_lds_bytes(int nb, int ng) {
+    return sizeof(double2) * ((size_t)ng * nb + 4 * (size_t)nb * nb + nb) + sizeof(float2) * 2 * (size_t)ng * nb;
+}
 
 }  // namespace
 
@@ -531,7 +534,7 @@ Cnn::Cnn(int h_, int w_, int nb_, int pw_half_, int max_batch_, unsigned long lo
     : H(h_), W(w_), nb(nb_), pw_half(pw_half_), max_batch(max_batch_), st(st_) {
     if (H % 8 || W % 8 || H < 8 || W < 8 || H > 1024 || W > 1024)
         throw std::runtime_error("CNN input sides must be multiples of 8 in [8, 1024]");
-    if (nb < 1 || nb > LOSS_MAXNB) throw std::runtime_error("CNN predicts 1..16 vectors");
+    if (nb < 1 || nb > LOSS_MAXNB) throw std::runtime_error("CNN predicts 1..32 vectors");
     if (pw_half < 0 || pw_half > 8) throw std::runtime_error("CNN plane-wave half width must be 0..8");
     if (max_batch < 1 || max_batch > 4096) throw std::runtime_error("CNN batch must be 1..4096");
     ng = (2 * pw_half + 1) * (2 * pw_half + 1);

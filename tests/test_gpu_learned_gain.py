@@ -13,8 +13,15 @@ used on 48 held-out geometries (TM and TE, one k-point each):
     iteration count and frequencies to 1e-9, and lie within 1e-6 of the
     oracle's tight default-start solve;
   * the fallback path: with margin 0 every solve falls back and the results
-    are those of the default start (same iterations, frequencies to 1e-12).
+    are those of the default start (same iterations, frequencies to 1e-12);
+  * the large grid: a 32-vector network trained at 64 x 64 (600 geometries,
+    32 bands) starts the 256 x 256, 32-band solves of the large combined case
+    (eps evaluated on the network's 64 x 64 grid, the prediction synthesised
+    on 256 x 256): every solve converges within 1e-6 of the oracle fixture
+    (tests/golden/cfg5_oracle.json) in fewer iterations in total.
 """
+import json
+import os
 import dataclasses
 
 import numpy as np
@@ -108,3 +115,39 @@ def test_fallback_gives_default_start_results(trained):
         assert np.abs(r.bands - b.bands).max() <= 1e-12 * np.abs(b.bands).max()
     # the decision itself
     assert list(LS.keep_prediction([1.0, 2.0, np.nan], [1.5, 1.5, 1.0])) == [True, False, False]
+
+
+def test_learned_start_on_the_large_grid():
+    be = HipBackend(0)
+    net = LS.CnnNet(be, 64, 64, 32, 4, max_batch=128, seed=3)
+    try:
+        data = LS.training_set(be, 600, seed=4321, n_bands=32, k_per_path=3)
+        hist = LS.train(net, data, 12, batch=64, lr=1e-3, seed=1)
+        with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "cfg5_oracle.json")) as f:
+            fx = json.load(f)
+        geom = _large_geometry(fx)
+        cases = [c for c in fx["cases"] if c["grid"] == 256]
+        jobs = [BandStructureJob(geom, 256, 256, c["pol"], np.array([c["k"]]), n_bands=32, tol=fx["tol"],
+                                 max_iter=fx["max_iter"]) for c in cases]
+        ls = LS.solve(be, net, jobs, OPTS, slots=len(jobs))
+        tot_l = tot_d = 0
+        for c, r in zip(cases, ls.results):
+            err = _rel(r.bands[0], np.array(c["omegas"]))
+            print(f"256x256 32 bands {c['pol']} k={c['k']}: learned start {r.iterations[0]} iterations "
+                  f"(default start {c['iterations']}), max rel omega diff vs oracle {err:.2e}")
+            assert r.converged.all()
+            assert err <= 1e-6
+            tot_l += int(r.iterations[0])
+            tot_d += int(c["iterations"])
+        print(f"trained on {data.embed.shape[0]} samples (loss {hist[0]:.4f} -> {hist[-1]:.4f}); iterations "
+              f"{tot_l} learned vs {tot_d} default; fallback rate {1 - ls.used_prediction.mean():.2f}")
+        assert tot_l < tot_d
+    finally:
+        net.close()
+        be.close()
+
+
+def _large_geometry(fx):
+    from blaze2d_amd import bandstructure as B
+    atoms = [B.BasisAtom((a[0], a[1]), a[2], a[3]) for a in fx["atoms"]]
+    return B.Geometry2D(B.Lattice2D.square(), fx["eps_bg"], atoms)
