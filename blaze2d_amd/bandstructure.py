@@ -57,7 +57,7 @@ class Geometry2D:
         g.a2[0], g.a2[1] = self.lattice.a2
         g.eps_bg = self.eps_bg
         if len(self.atoms) > L.MAX_ATOMS:
-            raise ValueError("at most 8 atoms")
+            raise ValueError(f"at most {L.MAX_ATOMS} atoms")
         g.n_atoms = len(self.atoms)
         for i, a in enumerate(self.atoms):
             g.atoms[i][0], g.atoms[i][1] = a.pos

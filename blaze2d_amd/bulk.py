@@ -24,8 +24,9 @@ Reference behaviour restated here:
 
 Not restated: the operator-data solver, selective output mode, streaming and
 batched I/O, adaptive threading (the GPU engine replaces the per-job thread
-pool).  A job the GPU path does not support (a grid other than 8, 16, 24,
-32, 48 or 64 squared, more than 16 bands or 8 atoms) raises an error.
+pool).  A job the GPU path does not support (a grid side outside 8, 16, 24,
+32, 48, 64, 96, 128, 192, 256, 384 and 512, a block wider than 96 columns,
+more than 64 atoms) raises an error.
 """
 from __future__ import annotations
 

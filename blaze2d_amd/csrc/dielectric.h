@@ -5,7 +5,7 @@
 
 namespace bh {
 
-constexpr int DIEL_MAX_ATOMS = 8;
+constexpr int DIEL_MAX_ATOMS = 64;
 
 struct DielGeom {
     double a1[2], a2[2];

@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define BH_MAX_ATOMS 8
+#define BH_MAX_ATOMS 64
 
 typedef struct bh_context bh_context;
 typedef struct bh_buffer bh_buffer;

@@ -19,7 +19,7 @@
 extern "C" {
 #endif
 
-#define ORACLE_MAX_ATOMS 8
+#define ORACLE_MAX_ATOMS 64
 
 typedef struct {
     double a1[2], a2[2];          /* lattice vectors (crates/core/src/lattice.rs) */

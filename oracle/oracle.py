@@ -17,7 +17,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
-MAX_ATOMS = 8
+MAX_ATOMS = 64
 
 
 class _OracleJob(ctypes.Structure):

@@ -152,6 +152,26 @@ def test_bandstructure_cfg1_square_res24(be, pol):
     _compare_bands(res, om, it, f"cfg1 {pol}")
 
 
+# a 4 x 4 array of rods (16 atoms) with eps alternating 1 and 2.5 and two
+# radii, past the 8 atoms an earlier build allowed
+MANY_ATOMS = [((i + 0.5) / 4, (j + 0.5) / 4, 0.07 + 0.02 * ((i + j) % 2), 1.0 + 1.5 * ((i * 4 + j) % 2))
+              for i in range(4) for j in range(4)]
+
+
+@pytest.mark.parametrize("pol", ["TM", "TE"])
+def test_bandstructure_many_atoms(be, pol):
+    kp = B.k_path("square", 2)
+    geom = B.Geometry2D(B.Lattice2D.square(), 10.0, [B.BasisAtom((a[0], a[1]), a[2], a[3]) for a in MANY_ATOMS])
+    job = B.BandStructureJob(geom, 32, 32, pol, kp, n_bands=8, tol=1e-7)
+    res = B.run_with_options(be, job)
+    om, it, cv, tot = O.bandstructure(O.Job(lattice="square", eps_bg=10.0, atoms=MANY_ATOMS, nx=32, ny=32, pol=pol,
+                                            n_bands=8, tol=1e-7, kpath=kp))
+    _compare_bands(res, om, it, f"16 atoms {pol}")
+    eps, inv, ten, _ = B.dielectric_batch(be, [geom], 32, 32)
+    oe, oi, ot = O.dielectric(O.Job(lattice="square", eps_bg=10.0, atoms=MANY_ATOMS, nx=32, ny=32))
+    assert np.abs(eps[0] - oe).max() < 1e-12 and np.abs(ten[0] - ot).max() < 1e-12
+
+
 def test_bandstructure_batch_mixed(be):
     # several geometries and both polarisations in one batched run, each vs the oracle
     specs = [
@@ -246,7 +266,7 @@ def test_entry_points_reject_bad_arguments(be):
     good = B.BandStructureJob(B.Geometry2D(B.Lattice2D.square(), 13.0, [B.BasisAtom((0.5, 0.5), 0.3, 1.0)]),
                               n, n, "TM", kp, n_bands=4)
     for field_name, value, msg in (("pol", 2, "pol"), ("precond", 3, "precond"), ("exit_rule", 2, "exit_rule"),
-                                   ("n_atoms", 9, "n_atoms")):
+                                   ("n_atoms", L.MAX_ATOMS + 1, "n_atoms")):
         arr = (L.Job * 1)()
         arr[0] = good.to_c(B.RunOptions())
         if field_name == "n_atoms":
