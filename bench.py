@@ -47,8 +47,8 @@ FP64_PEAK_TFLOPS = 78.6
 # slots 2048, groups 4), FETCH_SIZE x2 + WRITE_SIZE x1 from two rocprofv3 --pmc
 # passes scaled by a plain copy (profiles/rr_kernel_traffic.md); reported only
 # for that configuration.
-RR_TRAFFIC_BYTES = 18.12e6
-RR_TRAFFIC_CONFIG = (2048, 4, 2048)  # slots, groups, geometries the counters were read on
+RR_TRAFFIC_BYTES = 15.36e6
+RR_TRAFFIC_CONFIG = (2048, 4, 2048, 2)  # slots, groups, geometries, engines the counters were read on
 TIMED_NAMES = {1: "op_kernel THETA_TM/TE (64x64 fused 2D FFT operator)",
                2: "gram2_kernel (dual Rayleigh-Ritz Gram Z^H BZ, Z^H AZ)",
                3: "rr_kernel (Cholesky/SVQB + projection + Jacobi Rayleigh-Ritz, FP64)",
@@ -327,6 +327,8 @@ def main():
     ap.add_argument("--geoms", type=int, default=2048, help="geometries in the sweep (x2 polarisations)")
     ap.add_argument("--slots", type=int, default=2048, help="concurrent band structures (0 = all)")
     ap.add_argument("--groups", type=int, default=4, help="slot groups pipelined on separate streams")
+    ap.add_argument("--engines", type=int, default=2,
+                    help="engines run side by side, each with its share of the jobs and slots")
     ap.add_argument("--precision", type=int, default=0, choices=(0, 1, 2),
                     help="0 FP64; 1 FP32 fields; 2 FP64 fields with FP32 preconditioners")
     ap.add_argument("--seed", type=int, default=0xB1A2E2D)
@@ -389,24 +391,26 @@ def main():
 
     for w in range(args.warmup):
         t = time.perf_counter()
-        res, st = run_batch(be, jobs, ropts, slots=slots, precision=args.precision, groups=args.groups)
+        res, st = run_batch(be, jobs, ropts, slots=slots, precision=args.precision, groups=args.groups,
+                            engines=args.engines)
         print(f"[bench] warmup {w}: {time.perf_counter() - t:.2f}s", file=sys.stderr, flush=True)
 
     t0 = time.perf_counter()
     conv = 0
     tot_iters = 0
     last = None
-    timed = {"ms": 0.0, "launches": 0, "bytes": 0.0, "flops": 0.0}
+    timed = {"ms": 0.0, "launches": 0, "bytes": 0.0, "flops": 0.0, "flops_executed": 0.0}
     rr_stats = {"problems": 0, "svqb_path": 0, "rank_drops": 0}
     for k in range(args.steps):
         res, st = run_batch(be, jobs, ropts, slots=slots, precision=args.precision, groups=args.groups,
-                            timed_kernel=args.timed_kernel)
+                            timed_kernel=args.timed_kernel, engines=args.engines)
         conv += sum(int(r.converged[:-1].sum()) for r in res)
         tot_iters += st.iterations
         timed["ms"] += st.timed_ms
         timed["launches"] += st.timed_launches
         timed["bytes"] += st.timed_bytes
         timed["flops"] += st.timed_flops
+        timed["flops_executed"] += st.timed_flops_executed
         rr_stats["problems"] += st.rr_problems
         rr_stats["svqb_path"] += st.svqb_fallbacks
         rr_stats["rank_drops"] += st.rank_drops
@@ -438,6 +442,7 @@ def main():
             "lobpcg_iterations_per_solve": round(tot_iters / max(args.steps * solves_of(jobs), 1), 3),
             "slots": slots,
             "groups": args.groups,
+            "engines": args.engines,
             "seed": args.seed,
             "exit_rule": "all bands soft-locked" if args.exit_rule == 1 else "reference solve() accounting",
             "rayleigh_ritz_problems": rr_stats["problems"],
@@ -455,15 +460,18 @@ def main():
         intensity = flops_per / max(bytes_per, 1.0)
         ridge = FP64_PEAK_TFLOPS * 1e12 / (HBM_PEAK_GBS * 1e9)
         hbm = intensity < ridge
+        # FP64 peak: MI355X's dense FP64 rate (matrix and vector alike); the
+        # timed kernels issue FP64 vector FMAs, no MFMA
         out["roofline"] = {
-            "bound": "hbm" if hbm else "mfma",
+            "bound": "hbm" if hbm else "fp64",
             "achieved": round(gbs if hbm else tfs, 3),
             "peak": HBM_PEAK_GBS if hbm else FP64_PEAK_TFLOPS,
             "unit": "GB/s" if hbm else "TFLOP/s",
             "frac": round(gbs / HBM_PEAK_GBS if hbm else tfs / FP64_PEAK_TFLOPS, 5),
             "traffic": (RR_TRAFFIC_BYTES if (args.timed_kernel == 3 and args.slots == RR_TRAFFIC_CONFIG[0]
                                               and args.groups == RR_TRAFFIC_CONFIG[1]
-                                              and args.geoms == RR_TRAFFIC_CONFIG[2] and args.precision == 0)
+                                              and args.geoms == RR_TRAFFIC_CONFIG[2]
+                                              and args.engines == RR_TRAFFIC_CONFIG[3] and args.precision == 0)
                         else None),
             "kernel": TIMED_NAMES[args.timed_kernel],
             "avg_launch_ms": round(avg_s * 1e3, 4),
@@ -472,6 +480,11 @@ def main():
             "algorithmic_flops_per_launch": int(flops_per),
             "achieved_tflops_fp64": round(tfs, 3),
             "frac_fp64_peak": round(tfs / FP64_PEAK_TFLOPS, 5),
+            "flop_model": ("method-independent: Cholesky, inverse and projection 4n^3/3 complex multiply-adds, "
+                           "Hermitian eigendecomposition with vectors 30n^3, back-transform 4n^2k"
+                           if args.timed_kernel == 3 else "algorithmic, per kernel class"),
+            "executed_flops_per_launch": int(timed["flops_executed"] / timed["launches"]),
+            "executed_tflops_fp64": round(timed["flops_executed"] / timed["launches"] / avg_s / 1e12, 3),
         }
     if not args.no_extra:
         # dielectric tables of the whole sweep on the GPU (one launch, event-timed
@@ -504,27 +517,27 @@ def main():
             "headline_with_diagnostics": round(value * step_s / (step_s + dwall), 3),
             "max_self_adjoint_err": max(d.self_adjoint_err for d in diags)}
         # the whole synthetic-sweep configuration: grid drawn from {32, 48, 64} and
-        # bands from {8, 12, 16} per geometry, one engine call per grid size
+        # bands from {8, 12, 16} per geometry, in one call: one engine per grid
+        # size (2 slot groups each) side by side, 3072 slots shared by work
         from blaze2d_amd.sweep import sweep_jobs
         mixed = sweep_jobs(args.geoms, seed=args.seed, tol=1e-7)
         by_grid = {}
-        for j in mixed:
-            by_grid.setdefault(j.nx, []).append(j)
-        for g, js in sorted(by_grid.items()):  # first launches of each grid size out of the way
-            run_batch(be, js[:2], ropts, slots=2, precision=args.precision, groups=args.groups)
+        for i, j in enumerate(mixed):
+            by_grid.setdefault(j.nx, []).append(i)
+        for g, idx in sorted(by_grid.items()):  # first launches of each grid size out of the way
+            run_batch(be, [mixed[i] for i in idx[:2]], ropts, slots=2, precision=args.precision)
         t = time.perf_counter()
-        conv_m = 0
-        mixed_res = {}
-        for g, js in sorted(by_grid.items()):
-            rm, _ = run_batch(be, js, ropts, slots=min(slots, len(js)), precision=args.precision, groups=args.groups)
-            conv_m += sum(int(x.converged[:-1].sum()) for x in rm)
-            mixed_res[g] = rm
+        rmix, _ = run_batch(be, mixed, ropts, slots=3072, precision=args.precision, groups=2, engines=len(by_grid))
         dtm = time.perf_counter() - t
+        conv_m = sum(int(x.converged[:-1].sum()) for x in rmix)
+        mixed_res = {g: [rmix[i] for i in idx] for g, idx in by_grid.items()}
+        by_grid = {g: [mixed[i] for i in idx] for g, idx in by_grid.items()}
         out["mixed_sweep"] = {
             "value": round(conv_m / dtm, 3), "unit": "k-point solves/s", "band_structures": len(mixed),
             "solves": solves_of(mixed), "converged_fraction": round(conv_m / max(solves_of(mixed), 1), 6),
             "band_structures_per_grid": {str(g): len(js) for g, js in sorted(by_grid.items())},
-            "bands": "8/12/16 drawn per geometry", "seconds": round(dtm, 3)}
+            "bands": "8/12/16 drawn per geometry", "seconds": round(dtm, 3),
+            "engines": "one per grid size, 2 slot groups each, 3072 slots shared by work"}
         if not args.no_cpu:
             # accuracy on a sample: the first four band structures of each grid
             # size (both polarisations of two geometries, their band counts as drawn)
@@ -577,7 +590,7 @@ def main():
         if args.exit_rule != 0:
             t = time.perf_counter()
             resf, stf = run_batch(be, jobs, RunOptions(exit_rule=0), slots=slots, precision=args.precision,
-                                  groups=args.groups)
+                                  groups=args.groups, engines=args.engines)
             dtf = time.perf_counter() - t
             out["solve_exit_mode"] = {
                 "value": round(sum(int(r.converged[:-1].sum()) for r in resf) / dtf, 3),
@@ -611,7 +624,8 @@ def main():
         # FP64 fields with the FD / TP preconditioners in FP32 (precision 2), one pass
         if args.precision == 0:
             t = time.perf_counter()
-            resm, stm = run_batch(be, jobs, ropts, slots=slots, precision=2, groups=args.groups)
+            resm, stm = run_batch(be, jobs, ropts, slots=slots, precision=2, groups=args.groups,
+                                  engines=args.engines)
             dtm = time.perf_counter() - t
             out["fp32_preconditioners"] = {
                 "value": round(sum(int(r.converged[:-1].sum()) for r in resm) / dtm, 3),

@@ -68,7 +68,7 @@ class Job(ctypes.Structure):
 
 class RunOptions(ctypes.Structure):
     _fields_ = [("slots", ctypes.c_int), ("precision", ctypes.c_int), ("profile", ctypes.c_int),
-                ("groups", ctypes.c_int), ("timed_kernel", ctypes.c_int)]
+                ("groups", ctypes.c_int), ("timed_kernel", ctypes.c_int), ("engines", ctypes.c_int)]
 
 
 class RunStats(ctypes.Structure):
@@ -90,6 +90,7 @@ class RunStats(ctypes.Structure):
         ("rr_problems", ctypes.c_long),
         ("svqb_fallbacks", ctypes.c_long),
         ("rank_drops", ctypes.c_long),
+        ("timed_flops_executed", ctypes.c_double),
     ]
 
 

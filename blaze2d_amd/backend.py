@@ -20,6 +20,7 @@ class HipBackend:
     """One device context (a HIP stream)."""
 
     def __init__(self, device: int = 0):
+        self.device = device
         self._ctx = ctypes.c_void_p()
         L.check(L.lib().bh_context_create(device, ctypes.byref(self._ctx)), "bh_context_create")
 

@@ -182,15 +182,19 @@ class BatchStats:
     rr_problems: int = 0            # Rayleigh-Ritz problems solved
     svqb_fallbacks: int = 0         # of them, B-orthonormalised on the SVQB path
     rank_drops: int = 0             # of them, with a direction dropped
+    timed_flops_executed: float = 0.0  # FLOPs the timed kernels ran (Rayleigh-Ritz: Jacobi sweeps as run)
 
 
 def run_batch(backend: HipBackend, jobs: Sequence[BandStructureJob], options: Optional[RunOptions] = None,
               slots: int = 64, precision: int = 0, profile: bool = False, groups: int = 4, timed_kernel: int = 0,
-              pw_half: int = 0):
-    """Runs many band structures concurrently on one GPU (all on one grid).
+              pw_half: int = 0, engines: int = 1):
+    """Runs many band structures concurrently on one GPU.
 
     `slots` band structures are resident at once; they are split into `groups`
-    pipelined on separate HIP streams.  `timed_kernel` (1 Theta operator,
+    pipelined on separate HIP streams.  Jobs may use different grids: each
+    grid size runs as an engine of its own, and `engines` > 1 splits the
+    largest ones further; the engines run side by side (own streams, own host
+    thread), sharing the slots in proportion to their work.  `timed_kernel` (1 Theta operator,
     2 dual Rayleigh-Ritz Gram, 3 Rayleigh-Ritz solve, 4 Ritz update, 5
     deflation) times every launch of that kernel class with HIP events on its
     stream during the run.
@@ -212,7 +216,7 @@ def run_batch(backend: HipBackend, jobs: Sequence[BandStructureJob], options: Op
     prec = np.zeros(tot_om)
     ng = (2 * pw_half + 1) ** 2
     pw = np.zeros(tot_om * ng if pw_half > 0 else 1, dtype=np.complex128)
-    ro = L.RunOptions(slots, precision, int(profile), groups, timed_kernel)
+    ro = L.RunOptions(slots, precision, int(profile), groups, timed_kernel, engines)
     st = L.RunStats()
     extra = L.RunOutputs(rel.ctypes.data_as(L.c_dp), rit.ctypes.data_as(L.c_ip), pw_half,
                          pw.view(np.float64).ctypes.data_as(L.c_dp) if pw_half > 0 else None,
@@ -237,7 +241,8 @@ def run_batch(backend: HipBackend, jobs: Sequence[BandStructureJob], options: Op
     stats = BatchStats(st.steps, st.iterations, st.solves, st.seconds, st.timed_ms, st.timed_launches,
                        st.timed_bytes, st.timed_flops, tuple(st.eigh_calls), tuple(st.eigh_sweeps),
                        tuple(st.eigh_max_sweeps), tuple(st.eigh_launch_max_sum), st.host_wait_seconds,
-                       st.host_plan_seconds, st.rr_problems, st.svqb_fallbacks, st.rank_drops)
+                       st.host_plan_seconds, st.rr_problems, st.svqb_fallbacks, st.rank_drops,
+                       st.timed_flops_executed)
     return out, stats
 
 

@@ -1,9 +1,13 @@
 // C ABI of the gfx950 band-solve backend (include/blaze2d_hip.h).
 #include "../../include/blaze2d_hip.h"
 
+#include <chrono>
 #include <cmath>
 #include <cstring>
+#include <exception>
+#include <map>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "aux_kernels.h"
@@ -1032,10 +1036,8 @@ int bh_bandstructure_run_ex(bh_context *ctx, const bh_job *jobs, int njobs, cons
     if (!ctx || (!jobs && njobs > 0) || njobs < 0) return fail("bh_bandstructure_run: bad arguments");
     if (njobs == 0) return 0;
     std::vector<JobSpec> js(njobs);
-    int nbmax = 0;
     for (int j = 0; j < njobs; ++j) {
         const bh_job &b = jobs[j];
-        if (b.nx != jobs[0].nx || b.ny != jobs[0].ny) return fail("bh_bandstructure_run: jobs must share the grid");
         if (b.nk <= 0 || !b.kpath) return fail("bh_bandstructure_run: empty k-path");
         if (b.n_bands <= 0) return fail("bh_bandstructure_run: n_bands must be positive");
         if (b.pol != 0 && b.pol != 1) return fail("bh_bandstructure_run: pol must be 0 (TM) or 1 (TE)");
@@ -1044,6 +1046,7 @@ int bh_bandstructure_run_ex(bh_context *ctx, const bh_job *jobs, int njobs, cons
         if (b.geom.n_atoms < 0 || b.geom.n_atoms > BH_MAX_ATOMS)
             return fail("bh_bandstructure_run: n_atoms out of range");
         if (b.block_size < 0) return fail("bh_bandstructure_run: block_size must be >= 0");
+        if (!grid_supported(b.nx, b.ny)) return fail("bh_bandstructure_run: grid size not instantiated");
         JobSpec &s = js[j];
         s.geom = to_geom(b.geom);
         s.nx = b.nx;
@@ -1068,12 +1071,9 @@ int bh_bandstructure_run_ex(bh_context *ctx, const bh_job *jobs, int njobs, cons
             s.warm_count = b.warm_count;
             s.pw_half = b.pw_half;
         }
-        nbmax = std::max(nbmax, std::max(b.n_bands, b.block_size > 0 ? b.block_size - 2 : 0));
     }
-    if (!grid_supported(js[0].nx, js[0].ny)) return fail("bh_bandstructure_run: grid size not instantiated");
     EngineOptions eo;
-    eo.slots = opts && opts->slots > 0 ? opts->slots : 64;
-    eo.slots = std::min(eo.slots, njobs);
+    const int slots_total = opts && opts->slots > 0 ? opts->slots : 64;
     eo.precision = opts ? opts->precision : 0;
     if (eo.precision < 0 || eo.precision > 2) return fail("bh_bandstructure_run: precision must be 0, 1 or 2");
     eo.profile = opts ? opts->profile : 0;
@@ -1081,19 +1081,87 @@ int bh_bandstructure_run_ex(bh_context *ctx, const bh_job *jobs, int njobs, cons
     eo.timed_kernel = opts ? opts->timed_kernel : 0;
     eo.pw_half = pw_half;
     if (eo.timed_kernel < 0 || eo.timed_kernel > 5) return fail("bh_bandstructure_run: timed_kernel must be 0..5");
-    eo.setup_stream = ctx->st;
-    eo.group_streams[0] = ctx->st;
-    eo.group_streams[1] = ctx->gst[0];
-    eo.group_streams[2] = ctx->gst[1];
-    eo.group_streams[3] = ctx->gst[2];
+    const int engines = opts && opts->engines > 0 ? opts->engines : 1;
+    if (engines > 16) return fail("bh_bandstructure_run: engines must be 1..16");
+    // partitions: the jobs of each grid, then the largest split in two (job
+    // order kept) until there are `engines` of them; each runs as one engine
+    // with its own slots, streams and host thread
+    struct Part {
+        int nx, ny;
+        std::vector<int> idx;
+        double work = 0;
+        int slots = 1, nbmax = 0;
+    };
+    std::vector<Part> parts;
+    {
+        std::map<std::pair<int, int>, std::vector<int>> by_grid;
+        for (int j = 0; j < njobs; ++j) by_grid[{js[j].nx, js[j].ny}].push_back(j);
+        for (auto &kv : by_grid) parts.push_back(Part{kv.first.first, kv.first.second, kv.second});
+    }
+    while ((int)parts.size() < engines) {
+        int big = -1;
+        for (int p = 0; p < (int)parts.size(); ++p)
+            if (parts[p].idx.size() >= 2 && (big < 0 || parts[p].idx.size() > parts[big].idx.size())) big = p;
+        if (big < 0) break;
+        Part a{parts[big].nx, parts[big].ny, {}}, b{parts[big].nx, parts[big].ny, {}};
+        for (size_t i = 0; i < parts[big].idx.size(); ++i) (i % 2 ? b : a).idx.push_back(parts[big].idx[i]);
+        parts[big] = a;
+        parts.push_back(b);
+    }
+    double work_total = 0;
+    for (auto &p : parts) {
+        for (int j : p.idx) {
+            const JobSpec &s = js[j];
+            p.work += (double)s.kpath.size() * s.nx * s.ny * s.n_bands;
+            p.nbmax = std::max(p.nbmax, std::max(s.n_bands, s.block_size > 0 ? s.block_size - 2 : 0));
+        }
+        work_total += p.work;
+    }
+    for (auto &p : parts) {
+        const double share = parts.size() == 1 ? slots_total : slots_total * p.work / std::max(work_total, 1.0);
+        p.slots = std::max(1, std::min((int)p.idx.size(), (int)std::lround(share)));
+    }
     BH_HIP_CHECK(hipSetDevice(ctx->device));
-    auto eng = make_engine(js[0].nx, js[0].ny, nbmax, eo);
-    std::vector<JobResult> res;
-    RunStats rs;
-    eng->run(js, res, rs);
+    std::vector<std::vector<JobResult>> pres(parts.size());
+    std::vector<RunStats> pstats(parts.size());
+    std::vector<std::exception_ptr> perr(parts.size());
+    auto run_part = [&](int p) {
+        try {
+            BH_HIP_CHECK(hipSetDevice(ctx->device));
+            EngineOptions o = eo;
+            o.slots = parts[p].slots;
+            if (p == 0) {  // the first engine on the context's streams
+                o.setup_stream = ctx->st;
+                o.group_streams[0] = ctx->st;
+                o.group_streams[1] = ctx->gst[0];
+                o.group_streams[2] = ctx->gst[1];
+                o.group_streams[3] = ctx->gst[2];
+            }
+            std::vector<JobSpec> pj;
+            pj.reserve(parts[p].idx.size());
+            for (int j : parts[p].idx) pj.push_back(js[j]);
+            auto eng = make_engine(parts[p].nx, parts[p].ny, parts[p].nbmax, o);
+            eng->run(pj, pres[p], pstats[p]);
+        } catch (...) {
+            perr[p] = std::current_exception();
+        }
+    };
+    const auto t0 = std::chrono::steady_clock::now();
+    {
+        std::vector<std::thread> th;
+        for (int p = 1; p < (int)parts.size(); ++p) th.emplace_back(run_part, p);
+        run_part(0);
+        for (auto &t : th) t.join();
+    }
+    const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    for (auto &e : perr)
+        if (e) std::rethrow_exception(e);
+    std::vector<const JobResult *> res(njobs);
+    for (size_t p = 0; p < parts.size(); ++p)
+        for (size_t i = 0; i < parts[p].idx.size(); ++i) res[parts[p].idx[i]] = &pres[p][i];
     size_t off = 0, koff = 0;
     for (int j = 0; j < njobs; ++j) {
-        const JobResult &r = res[j];
+        const JobResult &r = *res[j];
         if (omegas) std::memcpy(omegas + off, r.omegas.data(), sizeof(double) * r.omegas.size());
         if (rel_residuals)
             std::memcpy(rel_residuals + off, r.rel_residuals.data(), sizeof(double) * r.rel_residuals.size());
@@ -1112,25 +1180,29 @@ int bh_bandstructure_run_ex(bh_context *ctx, const bh_job *jobs, int njobs, cons
         koff += r.iters.size();
     }
     if (stats) {
-        stats->steps = rs.steps;
-        stats->iterations = rs.iterations;
-        stats->solves = rs.solves;
-        stats->seconds = rs.seconds;
-        for (int p = 0; p < 2; ++p) {
-            stats->eigh_calls[p] = rs.eigh_calls[p];
-            stats->eigh_sweeps[p] = rs.eigh_sweeps[p];
-            stats->eigh_max_sweeps[p] = rs.eigh_max_sweeps[p];
-            stats->eigh_launch_max_sum[p] = rs.eigh_launch_max_sum[p];
+        *stats = bh_run_stats{};
+        for (const RunStats &rs : pstats) {
+            stats->steps += rs.steps;
+            stats->iterations += rs.iterations;
+            stats->solves += rs.solves;
+            for (int p = 0; p < 2; ++p) {
+                stats->eigh_calls[p] += rs.eigh_calls[p];
+                stats->eigh_sweeps[p] += rs.eigh_sweeps[p];
+                stats->eigh_max_sweeps[p] = std::max(stats->eigh_max_sweeps[p], (long)rs.eigh_max_sweeps[p]);
+                stats->eigh_launch_max_sum[p] += rs.eigh_launch_max_sum[p];
+            }
+            stats->timed_ms += rs.timed_ms;
+            stats->timed_launches += rs.timed_launches;
+            stats->timed_bytes += rs.timed_bytes;
+            stats->timed_flops += rs.timed_flops;
+            stats->timed_flops_executed += rs.timed_flops_executed;
+            stats->host_wait_seconds += rs.host_wait_seconds;
+            stats->host_plan_seconds += rs.host_plan_seconds;
+            stats->rr_problems += rs.rr_problems;
+            stats->svqb_fallbacks += rs.svqb_fallbacks;
+            stats->rank_drops += rs.rank_drops;
         }
-        stats->timed_ms = rs.timed_ms;
-        stats->timed_launches = rs.timed_launches;
-        stats->timed_bytes = rs.timed_bytes;
-        stats->timed_flops = rs.timed_flops;
-        stats->host_wait_seconds = rs.host_wait_seconds;
-        stats->host_plan_seconds = rs.host_plan_seconds;
-        stats->rr_problems = rs.rr_problems;
-        stats->svqb_fallbacks = rs.svqb_fallbacks;
-        stats->rank_drops = rs.rank_drops;
+        stats->seconds = parts.size() == 1 ? pstats[0].seconds : wall;
     }
     return 0;
     BH_API_CATCH

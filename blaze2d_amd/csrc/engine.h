@@ -65,6 +65,7 @@ struct RunStats {
     long timed_launches = 0;
     double timed_bytes = 0;       // algorithmic HBM bytes summed over those launches
     double timed_flops = 0;       // algorithmic FLOPs summed over those launches
+    double timed_flops_executed = 0;  // the FLOPs the kernels ran (Rayleigh-Ritz: Jacobi sweeps as run)
     long eigh_calls[2] = {0, 0};     // [SVQB, Rayleigh-Ritz] problems solved (profile mode)
     long eigh_sweeps[2] = {0, 0};    // Jacobi sweeps summed over those problems
     long eigh_max_sweeps[2] = {0, 0};      // largest sweep count seen in one problem

@@ -188,6 +188,9 @@ typedef struct {
                                 3 Rayleigh-Ritz solve (Cholesky/SVQB + projection + Jacobi),
                                 4 Ritz-update linear combinations,
                                 5 residual and direction deflation */
+    int engines;             /* engines run side by side, each with its share of the slots, its own
+                                streams and host thread (0 -> 1): the jobs of each grid size form one
+                                engine's share, the largest shares are split further to reach this count */
 } bh_run_options;
 
 typedef struct {
@@ -208,6 +211,8 @@ typedef struct {
     long rr_problems;            /* Rayleigh-Ritz problems solved */
     long svqb_fallbacks;         /* of them, B-orthonormalised on the SVQB path */
     long rank_drops;             /* of them, with a direction dropped (rank below the subspace width) */
+    double timed_flops_executed; /* the FLOPs the timed kernels ran (differs from timed_flops for the
+                                    Rayleigh-Ritz class: its Jacobi sweeps as run, not a method-independent count) */
 } bh_run_stats;
 
 /* One-time operator diagnostics of a job (drivers/bandstructure.rs:437-498):

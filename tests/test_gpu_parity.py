@@ -172,6 +172,29 @@ def test_bandstructure_many_atoms(be, pol):
     assert np.abs(eps[0] - oe).max() < 1e-12 and np.abs(ten[0] - ot).max() < 1e-12
 
 
+def test_mixed_grids_in_one_call(be):
+    # the sweep generator's own draws (grids 32/48/64, 8/12/16 bands) in one
+    # call: one engine per grid size, the largest split further (engines=4),
+    # all running side by side; each band structure against the oracle, and
+    # against the same jobs run one grid at a time
+    from blaze2d_amd.sweep import sweep_jobs
+    jobs = sweep_jobs(4, seed=11, tol=1e-7, segments_per_leg=2)
+    assert len({j.nx for j in jobs}) > 1
+    res, st = B.run_batch(be, jobs, B.RunOptions(exit_rule=1), slots=8, engines=4)
+    for j, r in zip(jobs, res):
+        lat = "square" if abs(j.geom.lattice.a2[0]) < 1e-12 else "hex"
+        om, it, cv, tot = O.bandstructure(O.Job(
+            lattice=lat, eps_bg=j.geom.eps_bg, atoms=[(a.pos[0], a.pos[1], a.radius, a.eps_inside) for a in j.geom.atoms],
+            nx=j.nx, ny=j.ny, pol=j.pol, n_bands=j.n_bands, tol=1e-7, kpath=j.k_path, exit_rule=1))
+        _compare_bands(r, om, it, f"mixed {j.nx}x{j.ny} {j.n_bands} bands {j.pol}")
+    for g in sorted({j.nx for j in jobs}):
+        idx = [i for i, j in enumerate(jobs) if j.nx == g]
+        sep, _ = B.run_batch(be, [jobs[i] for i in idx], B.RunOptions(exit_rule=1), slots=len(idx))
+        for i, r in zip(idx, sep):
+            assert np.array_equal(r.iterations, res[i].iterations)
+            assert np.abs(np.sort(r.bands, axis=1) - np.sort(res[i].bands, axis=1)).max() <= 1e-9
+
+
 def test_bandstructure_batch_mixed(be):
     # several geometries and both polarisations in one batched run, each vs the oracle
     specs = [
