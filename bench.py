@@ -44,7 +44,7 @@ METRIC = "converged k-point solves/sec, res64 12 bands, ≤1e-6 rel freq err vs 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 # HBM bytes per rr_kernel launch on the default workload (2048 geometries,
-# slots 2048, groups 4), FETCH_SIZE x2 + WRITE_SIZE x1 from two rocprofv3 --pmc
+# slots 2048, groups 4, two engines), FETCH_SIZE x2 + WRITE_SIZE x1 from two rocprofv3 --pmc
 # passes scaled by a plain copy (profiles/rr_kernel_traffic.md); reported only
 # for that configuration.
 RR_TRAFFIC_BYTES = 15.36e6

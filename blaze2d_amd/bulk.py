@@ -406,24 +406,18 @@ def expand_jobs(config) -> List[ExpandedJob]:
 
 
 def run_jobs(backend, config, jobs: Optional[List[ExpandedJob]] = None, slots: int = 1024, groups: int = 4):
-    """Runs every expanded job on the GPU (one engine call per grid), in job order.
-    Returns (results in job order, engine seconds)."""
+    """Runs every expanded job on the GPU in one call (one engine per grid
+    size, side by side).  Returns (results in job order, engine seconds)."""
     cfg = config if isinstance(config, BulkConfig) else BulkConfig(config)
     jobs = expand_jobs(cfg) if jobs is None else jobs
+    if not jobs:
+        return [], 0.0
     opts = RunOptions(disable_band_tracking=bool(cfg.bulk.get("disable_band_tracking", False)))
     precision = 1 if cfg.precision == "f32" else 0
-    results: List[Optional[BandStructureResult]] = [None] * len(jobs)
-    seconds = 0.0
-    by_grid: Dict[Tuple[int, int], List[int]] = {}
-    for i, ej in enumerate(jobs):
-        by_grid.setdefault((ej.job.nx, ej.job.ny), []).append(i)
-    for idx in by_grid.values():
-        res, st = run_batch(backend, [jobs[i].job for i in idx], opts, slots=min(slots, max(len(idx), 1)),
-                            precision=precision, groups=groups)
-        seconds += st.seconds
-        for i, r in zip(idx, res):
-            results[i] = r
-    return results, seconds
+    grids = {(ej.job.nx, ej.job.ny) for ej in jobs}
+    res, st = run_batch(backend, [ej.job for ej in jobs], opts, slots=min(slots, len(jobs)), precision=precision,
+                        groups=groups, engines=len(grids))
+    return list(res), st.seconds
 
 
 def write_full_csv(directory: str, prefix: str, ej: ExpandedJob, result: BandStructureResult) -> str:
