@@ -239,7 +239,8 @@ typedef struct {
 int bh_operator_diagnostics(bh_context *ctx, const bh_job *jobs, int njobs, int power_iterations,
                             bh_diagnostics *out, double *gpu_ms);
 
-/* Runs every job to completion.  All jobs must share nx, ny.  Outputs per
+/* Runs every job to completion (jobs may use different grids; see
+ * bh_run_options.engines).  Outputs per
  * job j at offsets: omegas[(sum of nk*n_bands over jobs < j) ...] as
  * omega (not omega / 2 pi), iters / converged per k-point. */
 int bh_bandstructure_run(bh_context *ctx, const bh_job *jobs, int njobs, const bh_run_options *opts,
