@@ -6,6 +6,8 @@
 // for many independent instances in one launch.  Accumulation is always FP64.
 #include "blas.h"
 
+#include <type_traits>
+
 namespace bh {
 
 
@@ -94,6 +96,90 @@ __global__ void __launch_bounds__(256) gram_kernel(const GramWork<T> *__restrict
         }
 }
 
+// ---------------------------------------------------------------------------
+// Gram on the FP64 matrix cores, for operands wider than 32 columns (the
+// 96-column subspaces of 32 bands): grid = (nchunks, nwork, tiles) with
+// 32 x 32 complex output tiles; four waves, each owning one 16 x 16 block.
+// Rows are staged through LDS 32 at a time as in gram_kernel (zero beyond the
+// operand widths and the chunk), then each wave runs v_mfma_f64_16x16x4f64
+// over them four rows at a time: the complex product conj(a) b is four real
+// products, Re += ar br + ai bi and Im += ar bi - ai br.  Operand layouts of
+// the instruction: A[m][k] in lane m + 16 k, B[k][n] in lane n + 16 k, and
+// accumulator j of lane l holds D[l / 16 + 4 j][l % 16].
+// ---------------------------------------------------------------------------
+typedef double gram_v4d __attribute__((ext_vector_type(4)));
+constexpr int GMT = 32;
+
+__global__ void __launch_bounds__(256) gram_mfma_kernel(const GramWork<double> *__restrict__ work,
+                                                        long rows_per_chunk, long nrows, int ntj) {
+    const GramWork<double> &w = work[blockIdx.y];
+    const int ti = blockIdx.z / ntj, tj = blockIdx.z - ti * ntj;
+    const int i0 = ti * GMT, j0 = tj * GMT;
+    if (w.lower_only && tj > ti) return;
+    const int na_all = w.dev_na ? *w.dev_na : w.na;
+    const int nb_all = w.dev_nb ? *w.dev_nb : w.nb;
+    const int na = na_all - i0 < GMT ? na_all - i0 : GMT;
+    const int nb = nb_all - j0 < GMT ? nb_all - j0 : GMT;
+    if (na <= 0 || nb <= 0) return;
+    __shared__ double2 As[GRAM_SR][GMT + 1];
+    __shared__ double2 Bs[GRAM_SR][GMT + 1];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int wi = wv >> 1, wj = wv & 1;
+    // a diagonal tile of a Hermitian product skips its block above the diagonal
+    const bool skip = w.lower_only != 0 && ti == tj && wj > wi;
+    const int lm = lane & 15, lk = lane >> 4;
+    gram_v4d re = {0.0, 0.0, 0.0, 0.0}, im = {0.0, 0.0, 0.0, 0.0};
+    const long r0 = (long)blockIdx.x * rows_per_chunk;
+    const long r1 = r0 + rows_per_chunk < nrows ? r0 + rows_per_chunk : nrows;
+    for (long rb = r0; rb < r1; rb += GRAM_SR) {
+        const int nr = (int)(r1 - rb < GRAM_SR ? r1 - rb : GRAM_SR);
+        for (int e = threadIdx.x; e < GRAM_SR * GMT; e += 256) {
+            const int c = e / GRAM_SR, r = e - c * GRAM_SR;
+            double2 v = make_double2(0.0, 0.0);
+            if (c < na && r < nr) {
+                const cplx<double> *col = w.a_ptrs ? w.a_ptrs[i0 + c] : w.A + (long)(i0 + c) * w.a_stride;
+                const cplx<double> x = gload(col, rb + r);
+                v = make_double2(x.x, x.y);
+            }
+            As[r][c] = v;
+        }
+        for (int e = threadIdx.x; e < GRAM_SR * GMT; e += 256) {
+            const int c = e / GRAM_SR, r = e - c * GRAM_SR;
+            double2 v = make_double2(0.0, 0.0);
+            if (c < nb && r < nr) {
+                const cplx<double> *col = w.b_ptrs ? w.b_ptrs[j0 + c] : w.B + (long)(j0 + c) * w.b_stride;
+                const cplx<double> x = gload(col, rb + r);
+                const double s = w.weight ? gmem(w.weight)[rb + r] : 1.0;
+                v = make_double2(x.x * s, x.y * s);
+            }
+            Bs[r][c] = v;
+        }
+        __syncthreads();
+        if (!skip) {
+            const int ksteps = (nr + 3) / 4;
+            for (int q = 0; q < ksteps; ++q) {
+                const double2 a = As[4 * q + lk][16 * wi + lm];
+                const double2 b = Bs[4 * q + lk][16 * wj + lm];
+                re = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.x, re, 0, 0, 0);
+                re = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.y, re, 0, 0, 0);
+                im = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.y, im, 0, 0, 0);
+                im = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.y, b.x, im, 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    if (skip) return;
+    double *out = w.partial + 2 * (size_t)blockIdx.x * w.ld * w.ld;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int i = 16 * wi + lk + 4 * j, jj = 16 * wj + lm;
+        if (i < na && jj < nb) {
+            gmem(out)[2 * ((size_t)(i0 + i) * w.ld + j0 + jj)] = re[j];
+            gmem(out)[2 * ((size_t)(i0 + i) * w.ld + j0 + jj) + 1] = im[j];
+        }
+    }
+}
+
 template <typename T>
 void launch_gram(const GramWork<T> *dwork, int nwork, int nchunks, long rows, int maxa, int maxb, hipStream_t st) {
     if (nwork <= 0) return;
@@ -105,7 +191,11 @@ void launch_gram(const GramWork<T> *dwork, int nwork, int nchunks, long rows, in
         hipLaunchKernelGGL((gram_kernel<T, 1>), dim3(nchunks, nwork, 1), dim3(256), 0, st, dwork, per, rows, 1);
     else if (wide <= 32)
         hipLaunchKernelGGL((gram_kernel<T, 2>), dim3(nchunks, nwork, 1), dim3(256), 0, st, dwork, per, rows, 1);
-    else {  // 48-column tiles
+    else if (std::is_same<T, double>::value) {  // FP64 matrix cores, 32-column tiles
+        const int nti = (maxa + GMT - 1) / GMT, ntj = (maxb + GMT - 1) / GMT;
+        hipLaunchKernelGGL(gram_mfma_kernel, dim3(nchunks, nwork, nti * ntj), dim3(256), 0, st,
+                           (const GramWork<double> *)dwork, per, rows, ntj);
+    } else {  // 48-column tiles
         const int nti = (maxa + GRAM_MAXC - 1) / GRAM_MAXC, ntj = (maxb + GRAM_MAXC - 1) / GRAM_MAXC;
         hipLaunchKernelGGL((gram_kernel<T, 3>), dim3(nchunks, nwork, nti * ntj), dim3(256), 0, st, dwork, per, rows,
                            ntj);
