@@ -68,7 +68,8 @@ class Job(ctypes.Structure):
 
 class RunOptions(ctypes.Structure):
     _fields_ = [("slots", ctypes.c_int), ("precision", ctypes.c_int), ("profile", ctypes.c_int),
-                ("groups", ctypes.c_int), ("timed_kernel", ctypes.c_int), ("engines", ctypes.c_int)]
+                ("groups", ctypes.c_int), ("timed_kernel", ctypes.c_int), ("engines", ctypes.c_int),
+                ("gram_mfma_min", ctypes.c_int)]
 
 
 class RunStats(ctypes.Structure):

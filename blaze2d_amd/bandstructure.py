@@ -187,14 +187,16 @@ class BatchStats:
 
 def run_batch(backend: HipBackend, jobs: Sequence[BandStructureJob], options: Optional[RunOptions] = None,
               slots: int = 64, precision: int = 0, profile: bool = False, groups: int = 4, timed_kernel: int = 0,
-              pw_half: int = 0, engines: int = 1):
+              pw_half: int = 0, engines: int = 1, gram_mfma_min: int = 0):
     """Runs many band structures concurrently on one GPU.
 
     `slots` band structures are resident at once; they are split into `groups`
     pipelined on separate HIP streams.  Jobs may use different grids: each
     grid size runs as an engine of its own, and `engines` > 1 splits the
     largest ones further; the engines run side by side (own streams, own host
-    thread), sharing the slots in proportion to their work.  `timed_kernel` (1 Theta operator,
+    thread), sharing the slots in proportion to their work.  `gram_mfma_min`
+    (0: 33) is the width from which FP64 Gram products run on the matrix
+    cores.  `timed_kernel` (1 Theta operator,
     2 dual Rayleigh-Ritz Gram, 3 Rayleigh-Ritz solve, 4 Ritz update, 5
     deflation) times every launch of that kernel class with HIP events on its
     stream during the run.
@@ -216,7 +218,7 @@ def run_batch(backend: HipBackend, jobs: Sequence[BandStructureJob], options: Op
     prec = np.zeros(tot_om)
     ng = (2 * pw_half + 1) ** 2
     pw = np.zeros(tot_om * ng if pw_half > 0 else 1, dtype=np.complex128)
-    ro = L.RunOptions(slots, precision, int(profile), groups, timed_kernel, engines)
+    ro = L.RunOptions(slots, precision, int(profile), groups, timed_kernel, engines, gram_mfma_min)
     st = L.RunStats()
     extra = L.RunOutputs(rel.ctypes.data_as(L.c_dp), rit.ctypes.data_as(L.c_ip), pw_half,
                          pw.view(np.float64).ctypes.data_as(L.c_dp) if pw_half > 0 else None,

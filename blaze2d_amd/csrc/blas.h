@@ -86,9 +86,12 @@ constexpr int GRAM_MAXC = 48;   // columns of the one-workgroup subspace kernels
 constexpr int GRAM_WIDE = 128;  // columns of the wide (tiled / multi-workgroup) paths
 constexpr int EIGH_MAXN = 64;
 
+// Gram products of up to GRAM_WIDE columns; in FP64, products wider than
+// mfma_min - 1 columns run on the matrix cores (gram_mfma_kernel)
+constexpr int GRAM_MFMA_MIN_DEFAULT = 33;
 template <typename T>
 void launch_gram(const GramWork<T> *dwork, int nwork, int nchunks, long rows, int maxa, int maxb,
-                 hipStream_t st);
+                 hipStream_t st, int mfma_min = GRAM_MFMA_MIN_DEFAULT);
 void launch_gram_reduce(const double *partial, double *out, int nwork, int nchunks, int ld, long part_stride,
                         long out_stride, hipStream_t st);
 // Batched reduce with per-item partial/output pointers and an optional

@@ -181,21 +181,22 @@ __global__ void __launch_bounds__(256) gram_mfma_kernel(const GramWork<double> *
 }
 
 template <typename T>
-void launch_gram(const GramWork<T> *dwork, int nwork, int nchunks, long rows, int maxa, int maxb, hipStream_t st) {
+void launch_gram(const GramWork<T> *dwork, int nwork, int nchunks, long rows, int maxa, int maxb, hipStream_t st,
+                 int mfma_min) {
     if (nwork <= 0) return;
     if (maxa > GRAM_WIDE || maxb > GRAM_WIDE) throw std::runtime_error("gram operand wider than 128 columns");
     long per = (rows + nchunks - 1) / nchunks;
     per = (per + GRAM_SR - 1) / GRAM_SR * GRAM_SR;
     const int wide = maxa > maxb ? maxa : maxb;
-    if (wide <= 16)
-        hipLaunchKernelGGL((gram_kernel<T, 1>), dim3(nchunks, nwork, 1), dim3(256), 0, st, dwork, per, rows, 1);
-    else if (wide <= 32)
-        hipLaunchKernelGGL((gram_kernel<T, 2>), dim3(nchunks, nwork, 1), dim3(256), 0, st, dwork, per, rows, 1);
-    else if (std::is_same<T, double>::value) {  // FP64 matrix cores, 32-column tiles
+    if (std::is_same<T, double>::value && wide >= mfma_min) {  // FP64 matrix cores, 32-column tiles
         const int nti = (maxa + GMT - 1) / GMT, ntj = (maxb + GMT - 1) / GMT;
         hipLaunchKernelGGL(gram_mfma_kernel, dim3(nchunks, nwork, nti * ntj), dim3(256), 0, st,
                            (const GramWork<double> *)dwork, per, rows, ntj);
-    } else {  // 48-column tiles
+    } else if (wide <= 16)
+        hipLaunchKernelGGL((gram_kernel<T, 1>), dim3(nchunks, nwork, 1), dim3(256), 0, st, dwork, per, rows, 1);
+    else if (wide <= 32)
+        hipLaunchKernelGGL((gram_kernel<T, 2>), dim3(nchunks, nwork, 1), dim3(256), 0, st, dwork, per, rows, 1);
+    else {  // 48-column tiles
         const int nti = (maxa + GRAM_MAXC - 1) / GRAM_MAXC, ntj = (maxb + GRAM_MAXC - 1) / GRAM_MAXC;
         hipLaunchKernelGGL((gram_kernel<T, 3>), dim3(nchunks, nwork, nti * ntj), dim3(256), 0, st, dwork, per, rows,
                            ntj);
@@ -536,8 +537,8 @@ void launch_lincomb(const LincombWork<T> *dwork, int nwork, long rows, int maxq,
     BH_HIP_CHECK(hipGetLastError());
 }
 
-template void launch_gram<double>(const GramWork<double> *, int, int, long, int, int, hipStream_t);
-template void launch_gram<float>(const GramWork<float> *, int, int, long, int, int, hipStream_t);
+template void launch_gram<double>(const GramWork<double> *, int, int, long, int, int, hipStream_t, int);
+template void launch_gram<float>(const GramWork<float> *, int, int, long, int, int, hipStream_t, int);
 template void launch_lincomb<double>(const LincombWork<double> *, int, long, int, hipStream_t, int);
 template void launch_lincomb<float>(const LincombWork<float> *, int, long, int, hipStream_t, int);
 

@@ -9,6 +9,7 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include "blas.h"
 #include "hostmath.h"
 
 namespace bh {
@@ -85,6 +86,7 @@ struct EngineOptions {
     int host_threads = 0;   // host threads for per-slot planning (0: hardware threads, at most 16)
     int timed_kernel = 0;   // TIMED_* class whose launches are timed with events (0: none)
     int pw_half = 0;        // > 0: project converged eigenvectors on the low plane waves (JobResult::pw)
+    int gram_mfma_min = GRAM_MFMA_MIN_DEFAULT;  // FP64 Gram products at least this wide use the matrix cores
     // Streams owned by the caller.  HIP gives each new stream its own hardware
     // queue until GPU_MAX_HW_QUEUES is reached and then shares queues, which
     // serialises the groups; a caller that creates the group streams early

@@ -1662,7 +1662,7 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
             launch_deflate<T>(dwork, (int)items.size(), N, maxa, max_d, st);
             return;
         }
-        launch_gram<T>(o.dg, (int)o.g.size(), o.nch, N, std::max(max_d, 1), maxa, st);
+        launch_gram<T>(o.dg, (int)o.g.size(), o.nch, N, std::max(max_d, 1), maxa, st, eopt.gram_mfma_min);
         launch_reduce(o.dr, (int)o.r.size(), LD, st);
         launch_lincomb<T>(o.dl, (int)o.l.size(), N, maxa, st, std::max(max_d, 1));
         launch_bnorm<T>(o.dn, (int)o.n.size(), maxa, N, st);
@@ -1769,7 +1769,8 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
         }
         timed(TIMED_GRAM2, bytes, flops,
               [&] {
-                  if (wide) launch_gram<T>(d_g_wide, (int)g_wide.size(), nch_dual, N, max_p, max_p, st);
+                  if (wide)
+                      launch_gram<T>(d_g_wide, (int)g_wide.size(), nch_dual, N, max_p, max_p, st, eopt.gram_mfma_min);
                   else launch_gram2<T>(d_g_dual, (int)g_dual.size(), nch_dual, N, std::max(max_p, 1), st);
               });
     }
@@ -1802,7 +1803,7 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
     if (!g_t1.empty()) {
         int maxkd = 0;
         for (int s : t1_slots) maxkd = std::max(maxkd, slots[s].kd);
-        launch_gram<T>(d_g_t1, (int)g_t1.size(), nch_t1, N, maxkd, maxkd, st);
+        launch_gram<T>(d_g_t1, (int)g_t1.size(), nch_t1, N, maxkd, maxkd, st, eopt.gram_mfma_min);
         launch_gram_reduce(d_part, d_gramrb, (int)g_t1.size(), nch_t1, LD, (long)nch_t1 * LD * LD * 2,
                            (long)LD * LD * 2, st);
         BH_HIP_CHECK(hipMemcpyAsync(G.h_gramrb, d_gramrb, sizeof(double) * g_t1.size() * LD * LD * 2,

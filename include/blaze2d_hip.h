@@ -191,6 +191,8 @@ typedef struct {
     int engines;             /* engines run side by side, each with its share of the slots, its own
                                 streams and host thread (0 -> 1): the jobs of each grid size form one
                                 engine's share, the largest shares are split further to reach this count */
+    int gram_mfma_min;       /* FP64 Gram products at least this many columns wide run on the matrix
+                                cores (0 -> 33); a tuning knob (blaze2d_amd.autotune) */
 } bh_run_options;
 
 typedef struct {
