@@ -764,8 +764,12 @@ __device__ inline int rr_slot(int r, int i, int ne) {
     if (d < 0) d += ne - 1;
     return d <= ne / 2 - 1 ? d : ne - 1 - d;
 }
-// the rotation that annihilates a_pq (as jacobi_lds); big = 1 when its
-// relative size |a_pq|^2 / |a_pp a_qq| reaches stop2
+// The rotation that annihilates a_pq (the same rotation as jacobi_lds, the
+// angle |theta| <= pi/4), from the half-angle form: with d = a_qq - a_pp and
+// rho = sqrt(d^2 + 4|h|^2), cos 2theta = |d| / rho, c^2 = (1 + cos 2theta) / 2
+// and |s| = |h| / (rho c), so sigma = -sgn(d) conj(h) / (rho c).  Two
+// reciprocal square roots and no division; c^2 + |s|^2 = 1 to rounding.
+// big = 1 when the relative size |a_pq|^2 / |a_pp a_qq| reaches stop2.
 __device__ inline JacobiRot jacobi_rotation(int p, int q, double app, double aqq, double2 h, double stop2,
                                             double &big) {
     JacobiRot R;
@@ -777,14 +781,13 @@ __device__ inline JacobiRot jacobi_rotation(int p, int q, double app, double aqq
     int flag = 0;
     if (hh != 0.0 && hh > 4.84e-32 * papq) {  // |h| > 2.2e-16 sqrt(|a_pp a_qq|)
         flag = 1;
-        const double ri = rsqrt(hh);  // 1 / |h|
-        const double tau = (aqq - app) * 0.5 * ri;
-        const double at = fmin(fabs(tau), 1e100);
-        const double u = at + sqrt(1.0 + at * at);
-        const double wv = rsqrt(u * u + 1.0);
-        const double sr = (tau >= 0 ? -wv : wv) * ri;
-        R.c = u * wv;
-        R.s = make_double2(sr * h.x, -sr * h.y);  // sigma = s conj(h/|h|)
+        const double d = aqq - app;
+        const double rr = rsqrt(d * d + 4.0 * hh);  // 1 / rho
+        const double hc = 0.5 + 0.5 * fabs(d) * rr;  // c^2 in [1/2, 1]
+        const double ci = rsqrt(hc);                 // 1 / c
+        R.c = hc * ci;
+        const double sm = (d >= 0.0 ? -rr : rr) * ci;
+        R.s = make_double2(sm * h.x, -sm * h.y);
         if (hh >= stop2 * papq) big = 1.0;
     } else if (hh != 0.0) {
         flag = 2;
@@ -792,56 +795,115 @@ __device__ inline JacobiRot jacobi_rotation(int p, int q, double app, double aqq
     R.pk = p | (q << 8) | (flag << 16);
     return R;
 }
-// a 2x2 block (rows of pair R1, columns of pair R2) after the round's
-// rotations, as jacobi_lds applies them; diag: R1 and R2 are the same pair
-__device__ inline void jacobi_block(double2 &a11, double2 &a12, double2 &a21, double2 &a22, const JacobiRot &R1,
-                                    const JacobiRot &R2, bool diag) {
-    const int f1 = (R1.pk >> 16) & 3, f2 = (R2.pk >> 16) & 3;
-    if (f1 != 1 && f2 != 1) {
-        if (diag && f1 == 2) {
-            a12 = make_double2(0.0, 0.0);
-            a21 = make_double2(0.0, 0.0);
-        }
-        return;
-    }
-    if (f2 == 1) {  // columns: [:,p] = c [:,p] + s [:,q]; [:,q] = -conj(s) [:,p] + c [:,q]
+// A 2x2 block (rows of pair R1, columns of pair R2) after the round's
+// rotations, without branches on the flags: a rotation of flag 0 or 2 is
+// stored as c = 1, s = 0, so applying it leaves the block exactly as it is; a
+// diagonal block (diag: R1 and R2 are the same pair) of flag 1 or 2 has its
+// off-diagonal zeroed, and of flag 1 its diagonal made real, as jacobi_lds
+// does for the pair it rotates.
+// Every load of an item can then be issued before any of its arithmetic.
+__device__ inline void jacobi_block_nb(double2 &a11, double2 &a12, double2 &a21, double2 &a22, const JacobiRot &R1,
+                                       const JacobiRot &R2, bool diag) {
+    {  // columns: [:,p] = c [:,p] + s [:,q]; [:,q] = -conj(s) [:,p] + c [:,q]
         const double c = R2.c;
         const double2 s = R2.s;
-        double2 n11 = make_double2(c * a11.x + s.x * a12.x - s.y * a12.y, c * a11.y + s.x * a12.y + s.y * a12.x);
-        double2 n12 = make_double2(-(s.x * a11.x + s.y * a11.y) + c * a12.x, -(s.x * a11.y - s.y * a11.x) + c * a12.y);
-        double2 n21 = make_double2(c * a21.x + s.x * a22.x - s.y * a22.y, c * a21.y + s.x * a22.y + s.y * a22.x);
-        double2 n22 = make_double2(-(s.x * a21.x + s.y * a21.y) + c * a22.x, -(s.x * a21.y - s.y * a21.x) + c * a22.y);
+        const double2 n11 = make_double2(c * a11.x + s.x * a12.x - s.y * a12.y, c * a11.y + s.x * a12.y + s.y * a12.x);
+        const double2 n12 = make_double2(-(s.x * a11.x + s.y * a11.y) + c * a12.x, -(s.x * a11.y - s.y * a11.x) + c * a12.y);
+        const double2 n21 = make_double2(c * a21.x + s.x * a22.x - s.y * a22.y, c * a21.y + s.x * a22.y + s.y * a22.x);
+        const double2 n22 = make_double2(-(s.x * a21.x + s.y * a21.y) + c * a22.x, -(s.x * a21.y - s.y * a21.x) + c * a22.y);
         a11 = n11; a12 = n12; a21 = n21; a22 = n22;
     }
-    if (f1 == 1) {  // rows: [p,:] = c [p,:] + conj(s) [q,:]; [q,:] = -s [p,:] + c [q,:]
+    {  // rows: [p,:] = c [p,:] + conj(s) [q,:]; [q,:] = -s [p,:] + c [q,:]
         const double c = R1.c;
         const double2 s = R1.s;
-        double2 n11 = make_double2(c * a11.x + s.x * a21.x + s.y * a21.y, c * a11.y + s.x * a21.y - s.y * a21.x);
-        double2 n21 = make_double2(-(s.x * a11.x - s.y * a11.y) + c * a21.x, -(s.x * a11.y + s.y * a11.x) + c * a21.y);
-        double2 n12 = make_double2(c * a12.x + s.x * a22.x + s.y * a22.y, c * a12.y + s.x * a22.y - s.y * a22.x);
-        double2 n22 = make_double2(-(s.x * a12.x - s.y * a12.y) + c * a22.x, -(s.x * a12.y + s.y * a12.x) + c * a22.y);
+        const double2 n11 = make_double2(c * a11.x + s.x * a21.x + s.y * a21.y, c * a11.y + s.x * a21.y - s.y * a21.x);
+        const double2 n21 = make_double2(-(s.x * a11.x - s.y * a11.y) + c * a21.x, -(s.x * a11.y + s.y * a11.x) + c * a21.y);
+        const double2 n12 = make_double2(c * a12.x + s.x * a22.x + s.y * a22.y, c * a12.y + s.x * a22.y - s.y * a22.x);
+        const double2 n22 = make_double2(-(s.x * a12.x - s.y * a12.y) + c * a22.x, -(s.x * a12.y + s.y * a12.x) + c * a22.y);
         a11 = n11; a12 = n12; a21 = n21; a22 = n22;
     }
-    if (diag) {  // the rotated pair: exact zero off-diagonal, real diagonal
-        a12 = make_double2(0.0, 0.0);
-        a21 = make_double2(0.0, 0.0);
-        a11.y = 0.0;
-        a22.y = 0.0;
-    }
+    const int f = (R1.pk >> 16) & 3;
+    const bool zero_off = diag && f != 0, real_diag = diag && f == 1;
+    a12 = zero_off ? make_double2(0.0, 0.0) : a12;
+    a21 = zero_off ? make_double2(0.0, 0.0) : a21;
+    a11.y = real_diag ? 0.0 : a11.y;
+    a22.y = real_diag ? 0.0 : a22.y;
 }
-// Rotating threads: tid < ne/2 (wave 0); block and V threads: tid >= 64.
-// Needs NT >= 128 and ne <= 48; rot holds 2 * ne / 2 entries, red one double.
+
+// the three entries of round r+1's pair (p, q) after round r, from the
+// current buffer and round r's rotations: a_pp and a_qq are diagonal entries
+// of the rotated 2x2 diagonal blocks of p's and q's round-r pairs, a_pq one
+// entry of their rotated off-diagonal block.  Rotations of flag 0 or 2 are
+// stored as c = 1, s = 0, so the formulas leave those blocks as they are.
+__device__ inline void jacobi_next_entries(const double2 *Ac, const JacobiRot *Rr, int ld, int ne, int r, int p,
+                                           int q, double &app, double &aqq, double2 &h) {
+    const int sP = rr_slot(r, p, ne), sQ = rr_slot(r, q, ne);
+    int pP, qP, pQ, qQ;
+    rr_pair(r, sP, ne, pP, qP);
+    rr_pair(r, sQ, ne, pQ, qQ);
+    // every address is known from the schedule: one round of LDS loads
+    const JacobiRot RP = Rr[sP], RQ = Rr[sQ];
+    const double x11 = Ac[pP + pP * ld].x, x22 = Ac[qP + qP * ld].x;
+    const double2 x12 = Ac[pP + qP * ld];
+    const double y11 = Ac[pQ + pQ * ld].x, y22 = Ac[qQ + qQ * ld].x;
+    const double2 y12 = Ac[pQ + qQ * ld];
+    const double2 b11 = Ac[pP + pQ * ld], b12 = Ac[pP + qQ * ld];
+    const double2 b21 = Ac[qP + pQ * ld], b22 = Ac[qP + qQ * ld];
+    // rotated diagonal block [[a11, a12], [conj(a12), a22]] with (c, s):
+    //   a11' = c^2 a11 + |s|^2 a22 + 2c Re(s a12), a22' = |s|^2 a11 + c^2 a22 - 2c Re(s a12)
+    {
+        const double c = RP.c, c2 = c * c, s2 = RP.s.x * RP.s.x + RP.s.y * RP.s.y;
+        const double t = 2.0 * c * (RP.s.x * x12.x - RP.s.y * x12.y);
+        app = p == pP ? c2 * x11 + s2 * x22 + t : s2 * x11 + c2 * x22 - t;
+    }
+    {
+        const double c = RQ.c, c2 = c * c, s2 = RQ.s.x * RQ.s.x + RQ.s.y * RQ.s.y;
+        const double t = 2.0 * c * (RQ.s.x * y12.x - RQ.s.y * y12.y);
+        aqq = q == pQ ? c2 * y11 + s2 * y22 + t : s2 * y11 + c2 * y22 - t;
+    }
+    if (sP == sQ) {  // only at ne = 2: the same pair again, zeroed unless its rotation was the identity
+        h = ((RP.pk >> 16) & 3) == 0 ? x12 : make_double2(0.0, 0.0);
+        return;
+    }
+    // columns of the off-diagonal block by RQ:
+    //   [:, pQ] = c [:, pQ] + s [:, qQ], [:, qQ] = -conj(s) [:, pQ] + c [:, qQ]
+    const double cq = RQ.c;
+    const double2 sq = RQ.s;
+    double2 u1, u2;
+    if (q == pQ) {
+        u1 = make_double2(cq * b11.x + sq.x * b12.x - sq.y * b12.y, cq * b11.y + sq.x * b12.y + sq.y * b12.x);
+        u2 = make_double2(cq * b21.x + sq.x * b22.x - sq.y * b22.y, cq * b21.y + sq.x * b22.y + sq.y * b22.x);
+    } else {
+        u1 = make_double2(cq * b12.x - (sq.x * b11.x + sq.y * b11.y), cq * b12.y - (sq.x * b11.y - sq.y * b11.x));
+        u2 = make_double2(cq * b22.x - (sq.x * b21.x + sq.y * b21.y), cq * b22.y - (sq.x * b21.y - sq.y * b21.x));
+    }
+    // then rows by RP: [pP, :] = c [pP, :] + conj(s) [qP, :], [qP, :] = -s [pP, :] + c [qP, :]
+    const double cp = RP.c;
+    const double2 sp = RP.s;
+    if (p == pP)
+        h = make_double2(cp * u1.x + sp.x * u2.x + sp.y * u2.y, cp * u1.y + sp.x * u2.y - sp.y * u2.x);
+    else
+        h = make_double2(cp * u2.x - (sp.x * u1.x - sp.y * u1.y), cp * u2.y - (sp.x * u1.y + sp.y * u1.x));
+}
+
+// Parallel cyclic Jacobi with one barrier per round, eigenvectors accumulated
+// into V (nv rows: the caller seeds V with the matrix the eigenvectors should
+// multiply).  Rotating threads: tid < ne/2 (wave 0); block and V threads:
+// tid >= 64.  Every index of a round follows from the round-robin schedule, so
+// a thread issues all of its LDS loads for an item (rotations and entries) at
+// once.  Needs NT >= 128, ne <= 48, nv <= 48; rot holds 2 * ne / 2 entries,
+// red one double.
 template <int NT>
-__device__ int jacobi_lds_db(double2 *A0, double2 *A1, double2 *V, int ld, int ne, JacobiRot *rot, double *red,
-                             double stop_ratio, int &final_buf) {
+__device__ int jacobi_lds_db(double2 *A0, double2 *A1, double2 *V, int ld, int ne, int nv, JacobiRot *rot,
+                             double *red, double stop_ratio, int &final_buf) {
     const int tid = threadIdx.x;
     const int np = ne / 2;
     const int ntri = np * (np + 1) / 2;  // block pairs bt1 >= bt2: the lower block triangle
-    const int nwork = ntri + np * ne;    // block items, then V items (pair, row)
+    const int nwork = ntri + np * nv;    // block items, then V items (pair, row)
     const double stop2 = stop_ratio * stop_ratio;
     // A worker's items are the same in every round: decode them once.  Item
     // it = tid - 64 + j (NT - 64); a block item packs bt1 | bt2 << 8, a V item
-    // t | k << 8 | 1 << 16, and -1 marks no item.  ne <= 48 gives at most
+    // t | k << 8 | 1 << 16, and -1 marks no item.  ne, nv <= 48 give at most
     // 300 + 1152 items.
     constexpr int JW = (300 + 1152 + (NT - 64) - 1) / (NT - 64);
     int item[JW];
@@ -857,7 +919,7 @@ __device__ int jacobi_lds_db(double2 *A0, double2 *A1, double2 *V, int ld, int n
             item[j] = bt1 | (it - bt1 * (bt1 + 1) / 2) << 8;
         } else {
             const int e = it - ntri;
-            item[j] = e / ne | (e % ne) << 8 | 1 << 16;
+            item[j] = e / nv | (e % nv) << 8 | 1 << 16;
         }
     }
     double big_cur = 0.0, big_next = 0.0;  // rotating threads: this sweep's and the next sweep's
@@ -883,12 +945,13 @@ __device__ int jacobi_lds_db(double2 *A0, double2 *A1, double2 *V, int ld, int n
                         // lower block (bt1 >= bt2); the matching upper block is
                         // its conjugate transpose, written by the same thread
                         const int bt1 = d & 255, bt2 = (d >> 8) & 255;
+                        int p1, q1, p2, q2;
+                        rr_pair(r, bt1, ne, p1, q1);
+                        rr_pair(r, bt2, ne, p2, q2);
                         const JacobiRot R1 = Rr[bt1], R2 = Rr[bt2];  // row pair, column pair
-                        const int p1 = R1.pk & 255, q1 = (R1.pk >> 8) & 255;
-                        const int p2 = R2.pk & 255, q2 = (R2.pk >> 8) & 255;
                         double2 a11 = Ac[p1 + p2 * ld], a12 = Ac[p1 + q2 * ld];
                         double2 a21 = Ac[q1 + p2 * ld], a22 = Ac[q1 + q2 * ld];
-                        jacobi_block(a11, a12, a21, a22, R1, R2, bt1 == bt2);
+                        jacobi_block_nb(a11, a12, a21, a22, R1, R2, bt1 == bt2);
                         An[p1 + p2 * ld] = a11;
                         An[p1 + q2 * ld] = a12;
                         An[q1 + p2 * ld] = a21;
@@ -901,41 +964,29 @@ __device__ int jacobi_lds_db(double2 *A0, double2 *A1, double2 *V, int ld, int n
                         }
                     } else {
                         const int t = d & 255, k = (d >> 8) & 255;
+                        int p, q;
+                        rr_pair(r, t, ne, p, q);
                         const JacobiRot R = Rr[t];
-                        if (((R.pk >> 16) & 3) == 1) {
-                            const int p = R.pk & 255, q = (R.pk >> 8) & 255;
-                            const double c = R.c;
-                            const double2 sg = R.s;
-                            const double2 vkp = V[k + p * ld], vkq = V[k + q * ld];
-                            V[k + p * ld] = make_double2(c * vkp.x + sg.x * vkq.x - sg.y * vkq.y,
-                                                         c * vkp.y + sg.x * vkq.y + sg.y * vkq.x);
-                            V[k + q * ld] = make_double2(-(sg.x * vkp.x + sg.y * vkp.y) + c * vkq.x,
-                                                         -(sg.x * vkp.y - sg.y * vkp.x) + c * vkq.y);
-                        }
+                        const double2 vkp = V[k + p * ld], vkq = V[k + q * ld];
+                        // unconditional: an identity rotation (c = 1, s = 0) rewrites the same values
+                        const double c = R.c;
+                        const double2 sg = R.s;
+                        V[k + p * ld] = make_double2(c * vkp.x + sg.x * vkq.x - sg.y * vkq.y,
+                                                     c * vkp.y + sg.x * vkq.y + sg.y * vkq.x);
+                        V[k + q * ld] = make_double2(-(sg.x * vkp.x + sg.y * vkp.y) + c * vkq.x,
+                                                     -(sg.x * vkp.y - sg.y * vkp.x) + c * vkq.y);
                     }
                 }
             } else if (tid < np) {
-                // round r+1 (round 0 of the next sweep after the last round):
-                // a_pp, a_qq from the diagonal blocks of p's and q's round-r
-                // pairs, a_pq from their off-diagonal block
+                // round r+1 (round 0 of the next sweep after the last round)
                 int rn = r + 1;
                 const bool wraps = rn == ne - 1;
                 if (wraps) rn = 0;
                 int p, q;
                 rr_pair(rn, tid, ne, p, q);
-                const int sP = rr_slot(r, p, ne), sQ = rr_slot(r, q, ne);  // equal only when ne = 2
-                const JacobiRot RP = Rr[sP], RQ = Rr[sQ];
-                const int pP = RP.pk & 255, qP = (RP.pk >> 8) & 255;
-                const int pQ = RQ.pk & 255, qQ = (RQ.pk >> 8) & 255;
-                double2 b11 = Ac[pP + pP * ld], b12 = Ac[pP + qP * ld], b21 = Ac[qP + pP * ld], b22 = Ac[qP + qP * ld];
-                jacobi_block(b11, b12, b21, b22, RP, RP, true);
-                const double app = p == pP ? b11.x : b22.x;
-                double2 d11 = Ac[pQ + pQ * ld], d12 = Ac[pQ + qQ * ld], d21 = Ac[qQ + pQ * ld], d22 = Ac[qQ + qQ * ld];
-                jacobi_block(d11, d12, d21, d22, RQ, RQ, true);
-                const double aqq = q == pQ ? d11.x : d22.x;
-                double2 c11 = Ac[pP + pQ * ld], c12 = Ac[pP + qQ * ld], c21 = Ac[qP + pQ * ld], c22 = Ac[qP + qQ * ld];
-                jacobi_block(c11, c12, c21, c22, RP, RQ, sP == sQ);
-                const double2 h = p == pP ? (q == pQ ? c11 : c12) : (q == pQ ? c21 : c22);
+                double app, aqq;
+                double2 h;
+                jacobi_next_entries(Ac, Rr, ld, ne, r, p, q, app, aqq, h);
                 rot[(1 - par) * np + tid] = jacobi_rotation(p, q, app, aqq, h, stop2, wraps ? big_next : big_cur);
             }
             __syncthreads();
@@ -1599,34 +1650,22 @@ __global__ void __launch_bounds__(NT) rr_kernel(const RRWork *__restrict__ work)
         A[a + c * lda] = h;
         A[c + a * lda] = make_double2(h.x, -h.y);
     }
-    for (int e = tid; e < ne * ne; e += NT) {
-        const int i = e % ne, j = e / ne;
-        V[i + j * lda] = make_double2(i == j && i < re ? 1.0 : 0.0, 0.0);
-    }
-    // T waits in this slot's coefficient array (the output, written only at
-    // the end) while its LDS holds the Jacobi double buffer
-    for (int e = tid; e < n * r; e += NT) {
+    // 4. M = Y diag(theta) Y^H, with the rotations applied to a copy of T
+    // (n rows, re columns), which leaves V = T Y: the Ritz coefficients
+    // without a back-transform, and T never leaves LDS
+    for (int e = tid; e < n * re; e += NT) {
         const int k = e % n, c = e / n;
-        const double2 t = Tm[k + c * lda];
-        gmem(w.C)[2 * ((size_t)k + (size_t)c * w.ldc)] = t.x;
-        gmem(w.C)[2 * ((size_t)k + (size_t)c * w.ldc) + 1] = t.y;
+        V[k + c * lda] = c < r ? Tm[k + c * lda] : make_double2(0.0, 0.0);
     }
     __syncthreads();
-    // 4. M = Y diag(theta) Y^H
     int fb = 0;
-    const int sw = jacobi_lds_db<NT>(A, Tm, V, lda, re, rot, red, RR_STOP_RATIO, fb);
+    const int sw = jacobi_lds_db<NT>(A, Tm, V, lda, re, n, rot, red, RR_STOP_RATIO, fb);
     const double2 *Af = fb ? Tm : A;  // the diagonalised matrix
-    double2 *Tl = fb ? A : Tm;        // free again: T comes back here
     if (tid == 0 && w.sweeps_out) {
         atomicAdd(w.sweeps_out + 3, sw);
         atomicMax(w.sweeps_out + 5, sw);
     }
     for (int i = tid; i < r; i += NT) wsh[i] = Af[i + i * lda].x;
-    for (int e = tid; e < n * r; e += NT) {
-        const int k = e % n, c = e / n;
-        Tl[k + c * lda] = make_double2(gmem(w.C)[2 * ((size_t)k + (size_t)c * w.ldc)],
-                                       gmem(w.C)[2 * ((size_t)k + (size_t)c * w.ldc) + 1]);
-    }
     __syncthreads();
     for (int i = tid; i < r; i += NT) {
         const double wi = wsh[i];
@@ -1636,21 +1675,17 @@ __global__ void __launch_bounds__(NT) rr_kernel(const RRWork *__restrict__ work)
         w.lam[rk] = wi;
     }
     __syncthreads();
-    // 5. C = T Y for the first cx + cw Ritz vectors
+    // 5. C = T Y for the first cx + cw Ritz vectors, ascending
     const int cx = r < w.nsplit ? r : w.nsplit;
     const int cw = (2 * w.nsplit < r ? 2 * w.nsplit : r) - cx;
     for (int e = tid; e < n * r; e += NT) {
         const int k = e % n, i = e / n;
         const int col = rank_of[i];
         if (col >= cx + cw) continue;
-        double cr = 0.0, ci = 0.0;
-        for (int j = 0; j < r; ++j) {
-            const double2 t = Tl[k + j * lda], y = V[j + i * lda];
-            cmac(cr, ci, t.x, t.y, y.x, y.y);
-        }
+        const double2 v = V[k + i * lda];
         const size_t o = 2 * ((size_t)k + (size_t)col * w.ldc);
-        gmem(w.C)[o] = cr;
-        gmem(w.C)[o + 1] = ci;
+        gmem(w.C)[o] = v.x;
+        gmem(w.C)[o + 1] = v.y;
     }
     if (tid == 0) {
         w.cnt[0] = cx;
