@@ -126,7 +126,7 @@ EXPORTS = [
     "bh_hermitian_eigh", "bh_operator_diagnostics", "bh_rayleigh_ritz",
     "bh_bandstructure_run_ex", "bh_gram_noalloc", "bh_cnn_create", "bh_cnn_destroy", "bh_cnn_param_count",
     "bh_cnn_get_params", "bh_cnn_set_params", "bh_cnn_get_grads", "bh_cnn_predict", "bh_cnn_train_step",
-    "bh_cnn_predict_geometries",
+    "bh_cnn_predict_geometries", "bh_cnn_set_graphs",
 ]
 
 _lib = None
@@ -190,6 +190,7 @@ def lib() -> ctypes.CDLL:
     L.bh_cnn_predict.argtypes = [vp, ctypes.c_int, c_fp, c_fp, c_dp]
     L.bh_cnn_predict_geometries.argtypes = [vp, ctypes.c_int, ctypes.POINTER(Geometry), ctypes.c_int, c_fp, c_dp]
     L.bh_cnn_train_step.argtypes = [vp, ctypes.c_int, c_fp, c_fp, c_dp, ctypes.c_float, c_dp]
+    L.bh_cnn_set_graphs.argtypes = [vp, ctypes.c_int]
     _lib = L
     return L
 

@@ -1313,10 +1313,17 @@ int bh_cnn_train_step(bh_cnn *net, int batch, const float *grid, const float *em
     if (!targets) return fail("bh_cnn_train_step: null targets");
     if (!(learning_rate >= 0.0f)) return fail("bh_cnn_train_step: learning rate must be >= 0");
     net->net->upload_inputs(batch, grid, embed);
-    net->net->forward(batch);
-    const double l = net->net->loss_and_backward(batch, targets);
-    if (learning_rate > 0.0f) net->net->adam(learning_rate);
+    const double l = net->net->train_step(batch, targets, learning_rate);
     if (loss) *loss = l;
+    return 0;
+    BH_API_CATCH
+}
+
+int bh_cnn_set_graphs(bh_cnn *net, int enable) {
+    BH_API_TRY
+    if (!net || !net->net) return fail("bh_cnn_set_graphs: null network");
+    if (enable != 0 && enable != 1) return fail("bh_cnn_set_graphs: enable must be 0 or 1");
+    net->net->set_graphs(enable == 1);
     return 0;
     BH_API_CATCH
 }

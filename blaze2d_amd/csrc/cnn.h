@@ -5,6 +5,7 @@
 
 #include <hip/hip_bf16.h>
 
+#include <map>
 #include <vector>
 
 #include "common.h"
@@ -32,10 +33,15 @@ class Cnn {
     // eps / 16 and 1 / eps
     void inputs_from_geometries(int B, const DielGeom *geoms, bool smoothing, const float *emb);
     void forward(int B);
-    // after forward(B): loss (batch mean) and the parameter gradients;
-    // targets [B][nb][ng] complex (interleaved doubles)
-    double loss_and_backward(int B, const double *targets);
-    void adam(float lr);
+    // One training step on inputs already uploaded: forward, projector loss and
+    // its gradient, backward, and (lr > 0) an Adam update; returns the batch
+    // mean loss.  targets [B][nb][ng] complex (interleaved doubles).  With
+    // graphs on, the step's device work after the uploads is replayed from a
+    // hipGraph captured once per (batch size, update or not); the step's
+    // learning rate and bias corrections are read from device memory, so one
+    // graph serves every step, and the eager path reads them the same way.
+    double train_step(int B, const double *targets, float lr);
+    void set_graphs(bool on) { use_graphs = on; }
     void dense_bwd_data(const float *dy, const __hip_bfloat16 *Wm, float *dx, int B, int K, int M);
     void get_output(int B, double *host) const;  // [B][nb][ng] complex
     void get_params(float *host) const;
@@ -66,6 +72,10 @@ class Cnn {
     float *dpart = nullptr;   // partial sums of the dense layers' input gradients
     size_t dpart_floats = 0;
     double *d_eps = nullptr;  // dielectric scratch of inputs_from_geometries (allocated on first use)
+    float *d_hyper = nullptr, *h_hyper = nullptr;  // [lr, 1 - b1^t, 1 - b2^t] of the step (device, pinned host)
+    bool use_graphs = true;
+    std::map<int, hipGraphExec_t> graphs;  // key 2 B + (Adam update)
+    void step_launches(int B, bool update);  // the device work of one step (kernels only: capturable)
     void *d_geo = nullptr;
     size_t loss_lds = 0;
 };

@@ -472,9 +472,11 @@ __global__ void __launch_bounds__(LOSS_NT) projector_loss_kernel(const float *__
 }
 
 // Adam with bias correction on the FP32 master weights, then their BF16 copy
+// hyper = [learning rate, 1 - b1^t, 1 - b2^t] of step t, from device memory
 __global__ void adam_kernel(float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m,
-                            float *__restrict__ v, __hip_bfloat16 *__restrict__ pb, long n, float lr, float b1,
-                            float b2, float c1, float c2, float eps) {
+                            float *__restrict__ v, __hip_bfloat16 *__restrict__ pb, long n,
+                            const float *__restrict__ hyper, float b1, float b2, float eps) {
+    const float lr = hyper[0], c1 = hyper[1], c2 = hyper[2];
     for (long e = (long)blockIdx.x * NT + threadIdx.x; e < n; e += (long)gridDim.x * NT) {
         const float ge = g[e];
         const float me = b1 * m[e] + (1.0f - b1) * ge;
@@ -603,6 +605,8 @@ Cnn::Cnn(int h_, int w_, int nb_, int pw_half_, int max_batch_, unsigned long lo
         (void)wgrad_rows(cin, hin, win, l == 0 ? 1 : 2);
     }
     lossb = (double *)dal(sizeof(double) * Bm);
+    d_hyper = (float *)dal(sizeof(float) * 4);
+    BH_HIP_CHECK(hipHostMalloc((void **)&h_hyper, sizeof(float) * 4, hipHostMallocDefault));
     BH_HIP_CHECK(hipFuncSetAttribute((const void *)projector_loss_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)LOSS_MAX_LDS));
     BH_HIP_CHECK(hipFuncSetAttribute((const void *)wgrad_partial_kernel<2, 4>,
@@ -614,6 +618,8 @@ Cnn::Cnn(int h_, int w_, int nb_, int pw_half_, int max_batch_, unsigned long lo
 
 Cnn::~Cnn() {
     (void)hipStreamSynchronize(st);
+    for (auto &g : graphs) (void)hipGraphExecDestroy(g.second);
+    if (h_hyper) (void)hipHostFree(h_hyper);
     for (void *p : bufs) (void)hipFree(p);
 }
 
@@ -676,9 +682,8 @@ void Cnn::dense_bwd_data(const float *dy, const __hip_bfloat16 *Wm, float *dx, i
     BH_HIP_CHECK(hipGetLastError());
 }
 
-double Cnn::loss_and_backward(int B, const double *targets) {
-    if (B < 1 || B > max_batch) throw std::runtime_error("CNN batch outside 1..max_batch");
-    BH_HIP_CHECK(hipMemcpyAsync(tgt, targets, sizeof(double) * B * OUT, hipMemcpyHostToDevice, st));
+void Cnn::step_launches(int B, bool update) {
+    forward(B);
     hipLaunchKernelGGL(projector_loss_kernel, dim3(B), dim3(LOSS_NT), loss_lds, st, yout, tgt, lossb, gy, B, nb, ng);
     const __hip_bfloat16 *wb = pbf;
     hipLaunchKernelGGL(dense_bwd_weight_kernel, dim3(grid_for((long)OUT * (F + 1))), dim3(NT), 0, st, gy,
@@ -719,21 +724,54 @@ double Cnn::loss_and_backward(int B, const double *targets) {
     hipLaunchKernelGGL(dense_bwd_weight_kernel, dim3(grid_for((long)CH * (EMB_IN + 1))), dim3(NT), 0, st, geh, e0,
                        grads + off_e1, grads + off_e1b, B, EMB_IN, CH);
     BH_HIP_CHECK(hipGetLastError());
+    if (update) {
+        hipLaunchKernelGGL(adam_kernel, dim3(grid_for(P)), dim3(NT), 0, st, params, grads, adam_m, adam_v, pbf, P,
+                           d_hyper, 0.9f, 0.999f, 1e-8f);
+        BH_HIP_CHECK(hipGetLastError());
+    }
+}
+
+double Cnn::train_step(int B, const double *targets, float lr) {
+    if (B < 1 || B > max_batch) throw std::runtime_error("CNN batch outside 1..max_batch");
+    BH_HIP_CHECK(hipMemcpyAsync(tgt, targets, sizeof(double) * B * OUT, hipMemcpyHostToDevice, st));
+    const bool update = lr > 0.0f;
+    if (update) {
+        ++step;
+        h_hyper[0] = lr;
+        h_hyper[1] = 1.0f - std::pow(0.9f, (float)step);
+        h_hyper[2] = 1.0f - std::pow(0.999f, (float)step);
+        BH_HIP_CHECK(hipMemcpyAsync(d_hyper, h_hyper, sizeof(float) * 3, hipMemcpyHostToDevice, st));
+    }
+    if (use_graphs) {
+        const int key = 2 * B + (update ? 1 : 0);
+        auto it = graphs.find(key);
+        if (it == graphs.end()) {
+            BH_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            hipGraph_t g = nullptr;
+            try {
+                step_launches(B, update);
+            } catch (...) {
+                (void)hipStreamEndCapture(st, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            BH_HIP_CHECK(hipStreamEndCapture(st, &g));
+            hipGraphExec_t ge = nullptr;
+            const hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            BH_HIP_CHECK(ie);
+            it = graphs.emplace(key, ge).first;
+        }
+        BH_HIP_CHECK(hipGraphLaunch(it->second, st));
+    } else {
+        step_launches(B, update);
+    }
     std::vector<double> lb(B);
     BH_HIP_CHECK(hipMemcpyAsync(lb.data(), lossb, sizeof(double) * B, hipMemcpyDeviceToHost, st));
     BH_HIP_CHECK(hipStreamSynchronize(st));
     double s = 0.0;
     for (double v : lb) s += v;
     return s / B;
-}
-
-void Cnn::adam(float lr) {
-    ++step;
-    const float b1 = 0.9f, b2 = 0.999f;
-    const float c1 = 1.0f - std::pow(b1, (float)step), c2 = 1.0f - std::pow(b2, (float)step);
-    hipLaunchKernelGGL(adam_kernel, dim3(grid_for(P)), dim3(NT), 0, st, params, grads, adam_m, adam_v, pbf, P, lr, b1,
-                       b2, c1, c2, 1e-8f);
-    BH_HIP_CHECK(hipGetLastError());
 }
 
 void Cnn::get_output(int B, double *host) const {

@@ -136,6 +136,11 @@ class CnnNet:
             out[s:s + n] = o
         return out
 
+    def use_graphs(self, enable: bool) -> None:
+        """Replay each training step's device work from a HIP graph (the
+        default) or launch its kernels one by one; the results are identical."""
+        L.check(L.lib().bh_cnn_set_graphs(self.handle, 1 if enable else 0), "bh_cnn_set_graphs")
+
     def train_step(self, grid: np.ndarray, embed: np.ndarray, targets: np.ndarray, lr: float) -> float:
         """Forward, projector loss, backward and (lr > 0) one Adam step; returns the batch loss."""
         g, e = self._inputs(grid, embed)

@@ -324,6 +324,13 @@ int bh_cnn_predict_geometries(bh_cnn *net, int batch, const bh_geometry *geoms, 
  * 0.999, eps 1e-8, bias-corrected).  targets as coefs.  loss may be NULL. */
 int bh_cnn_train_step(bh_cnn *net, int batch, const float *grid, const float *embed, const double *targets,
                       float learning_rate, double *loss);
+/* No reference counterpart (the reference has no network).  enable = 1 (the
+ * default): bh_cnn_train_step replays its device work (forward, loss,
+ * backward, Adam; about thirty kernels) from a HIP graph captured on the
+ * first step of each batch size, the inputs, targets and the step's learning
+ * rate and bias corrections copied in before each replay; enable = 0: the
+ * same kernels launched one by one.  Both give identical results. */
+int bh_cnn_set_graphs(bh_cnn *net, int enable);
 
 #ifdef __cplusplus
 }

@@ -8,7 +8,10 @@
   GPU accumulates in FP32 where the oracle uses FP64;
 * Adam: two steps against the oracle's update applied to the GPU's gradients
   (1e-6 absolute);
-* training fits a small fixed set (the loss falls by more than half).
+* training fits a small fixed set (the loss falls by more than half);
+* training steps replayed from HIP graphs equal the same steps launched
+  kernel by kernel, bit for bit (losses, gradients, weights), across batch
+  sizes, learning rates and evaluation-only steps.
 """
 import numpy as np
 import pytest
@@ -119,3 +122,28 @@ def test_bad_arguments_raise(be):
             net.train_step(grid, embed, targets, 1e-3)  # batch 3 > max_batch 2
     finally:
         net.close()
+
+
+def test_graph_replays_equal_eager_steps(be):
+    # two networks from the same weights: one replays each step from a HIP
+    # graph (captured on the first step of each batch size and update kind),
+    # the other launches the same kernels one by one; every replay is followed
+    # by the step's own synchronisation (the loss readback) and compared
+    nets = [CnnNet(be, 16, 16, 4, 2, max_batch=8, seed=3) for _ in range(2)]
+    try:
+        nets[0].use_graphs(True)
+        nets[1].use_graphs(False)
+        p0 = nets[0].params()
+        nets[1].set_params(p0)
+        nets[0].set_params(p0)
+        plan = [(5, 1e-3), (5, 8e-4), (3, 1e-3), (5, 0.0), (5, 5e-4), (3, 2e-4), (8, 1e-3), (5, 1e-4)]
+        for step, (B, lr) in enumerate(plan):
+            grid, embed, targets = random_case(100 + step, B, 16, 16, 4, 2)
+            la = nets[0].train_step(grid, embed, targets, lr)
+            lb = nets[1].train_step(grid, embed, targets, lr)
+            assert la == lb, (step, la, lb)
+            assert np.array_equal(nets[0].grads(), nets[1].grads()), step
+            assert np.array_equal(nets[0].params(), nets[1].params()), step
+    finally:
+        for n in nets:
+            n.close()
