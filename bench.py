@@ -401,6 +401,7 @@ def main():
     last = None
     timed = {"ms": 0.0, "launches": 0, "bytes": 0.0, "flops": 0.0, "flops_executed": 0.0}
     rr_stats = {"problems": 0, "svqb_path": 0, "rank_drops": 0}
+    alg_bytes = 0.0
     for k in range(args.steps):
         res, st = run_batch(be, jobs, ropts, slots=slots, precision=args.precision, groups=args.groups,
                             timed_kernel=args.timed_kernel, engines=args.engines)
@@ -414,6 +415,7 @@ def main():
         rr_stats["problems"] += st.rr_problems
         rr_stats["svqb_path"] += st.svqb_fallbacks
         rr_stats["rank_drops"] += st.rank_drops
+        alg_bytes += st.alg_bytes
         last = res
         print(f"[bench] step {k}: engine {st.seconds:.2f}s, {st.steps} engine steps, host plan "
               f"{st.host_plan_seconds:.2f}s, host wait {st.host_wait_seconds:.2f}s", file=sys.stderr, flush=True)
@@ -485,6 +487,17 @@ def main():
                            if args.timed_kernel == 3 else "algorithmic, per kernel class"),
             "executed_flops_per_launch": int(timed["flops_executed"] / timed["launches"]),
             "executed_tflops_fp64": round(timed["flops_executed"] / timed["launches"] / avg_s / 1e12, 3),
+        }
+    if alg_bytes > 0:
+        # the whole solve path against HBM: algorithmic bytes of every launch of
+        # the timed steps (each column read or written once per launch) over
+        # their wall time
+        out["engine_hbm"] = {
+            "algorithmic_bytes_per_step": int(alg_bytes / max(args.steps, 1)),
+            "achieved_gbs": round(alg_bytes / elapsed / 1e9, 1),
+            "peak_gbs": HBM_PEAK_GBS,
+            "frac": round(alg_bytes / elapsed / 1e9 / HBM_PEAK_GBS, 4),
+            "bytes_per_lobpcg_iteration": int(alg_bytes / max(tot_iters, 1)),
         }
     if not args.no_extra:
         # dielectric tables of the whole sweep on the GPU (one launch, event-timed

@@ -92,6 +92,7 @@ class RunStats(ctypes.Structure):
         ("svqb_fallbacks", ctypes.c_long),
         ("rank_drops", ctypes.c_long),
         ("timed_flops_executed", ctypes.c_double),
+        ("alg_bytes", ctypes.c_double),
     ]
 
 

@@ -183,6 +183,7 @@ class BatchStats:
     svqb_fallbacks: int = 0         # of them, B-orthonormalised on the SVQB path
     rank_drops: int = 0             # of them, with a direction dropped
     timed_flops_executed: float = 0.0  # FLOPs the timed kernels ran (Rayleigh-Ritz: Jacobi sweeps as run)
+    alg_bytes: float = 0.0          # algorithmic HBM bytes of every launch of the run
 
 
 def run_batch(backend: HipBackend, jobs: Sequence[BandStructureJob], options: Optional[RunOptions] = None,
@@ -244,7 +245,7 @@ def run_batch(backend: HipBackend, jobs: Sequence[BandStructureJob], options: Op
                        st.timed_bytes, st.timed_flops, tuple(st.eigh_calls), tuple(st.eigh_sweeps),
                        tuple(st.eigh_max_sweeps), tuple(st.eigh_launch_max_sum), st.host_wait_seconds,
                        st.host_plan_seconds, st.rr_problems, st.svqb_fallbacks, st.rank_drops,
-                       st.timed_flops_executed)
+                       st.timed_flops_executed, st.alg_bytes)
     return out, stats
 
 

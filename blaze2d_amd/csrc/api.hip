@@ -1202,6 +1202,7 @@ int bh_bandstructure_run_ex(bh_context *ctx, const bh_job *jobs, int njobs, cons
             stats->rr_problems += rs.rr_problems;
             stats->svqb_fallbacks += rs.svqb_fallbacks;
             stats->rank_drops += rs.rank_drops;
+            stats->alg_bytes += rs.alg_bytes;
         }
         stats->seconds = parts.size() == 1 ? pstats[0].seconds : wall;
     }

@@ -76,6 +76,7 @@ struct RunStats {
     long rr_problems = 0;                  // Rayleigh-Ritz problems solved
     long svqb_fallbacks = 0;               // of them, solved on the SVQB path (Cholesky QR not provably equivalent)
     long rank_drops = 0;                   // of them, with a direction dropped (rank < subspace width)
+    double alg_bytes = 0;                  // algorithmic HBM bytes of every launch (each column read or written once)
 };
 
 struct EngineOptions {

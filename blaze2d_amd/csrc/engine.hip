@@ -1696,7 +1696,26 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
     }
     launch_pw_synth<T>(d_pw_syn, (int)pw_syn.size(), max_wc, nx, ny, wc_half, st);
     // launches of the timed kernel class are bracketed by events on this stream
+    // algorithmic HBM bytes of every launch: each column read or written once, tables and weights once
+    const double colb = (double)N * sizeof(cplx<T>), wb = (double)N * sizeof(T);
+    auto lc_bytes = [&](const std::vector<LincombWork<T>> &v) {
+        double b = 0;
+        for (auto &l : v) b += colb * (l.na + l.nq * (l.out2 ? 2.0 : 1.0));
+        return b;
+    };
+    for (auto &tc : tcopies) stats_->alg_bytes += 2.0 * sizeof(T) * tc.n;
+    stats_->alg_bytes += 2.0 * colb * plan.fill_y0.size();
+    stats_->alg_bytes += lc_bytes(lc_hist) + lc_bytes(lc_x);
+    for (auto &r : rq) stats_->alg_bytes += r.count * 2.0 * colb + (r.bw ? wb : 0.0);
+    for (auto &r : res) stats_->alg_bytes += r.count * 3.0 * colb + (r.bw ? wb : 0.0);
+    for (int k = 0; k < 2; ++k)
+        for (auto &o : op_pre[k])
+            stats_->alg_bytes += o.count * (2.0 + (o.bout ? 1.0 : 0.0)) * colb + (k == 1 || o.bout ? wb : 0.0);
+    for (auto &n : prec_norms) stats_->alg_bytes += n.count * colb + (n.bw ? wb : 0.0);
+    for (auto &g : g_t1) stats_->alg_bytes += (g.na + g.nb) * colb + (g.weight ? wb : 0.0);
+    stats_->alg_bytes += pw_proj.size() * nbmax * colb;
     auto timed = [&](int cls, double bytes, double flops, auto &&launch) {
+        stats_->alg_bytes += bytes;
         if (eopt.timed_kernel != cls) { launch(); return; }
         if (G.tev_used == G.tev.size()) {
             hipEvent_t a, b;

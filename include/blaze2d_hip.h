@@ -215,6 +215,8 @@ typedef struct {
     long rank_drops;             /* of them, with a direction dropped (rank below the subspace width) */
     double timed_flops_executed; /* the FLOPs the timed kernels ran (differs from timed_flops for the
                                     Rayleigh-Ritz class: its Jacobi sweeps as run, not a method-independent count) */
+    double alg_bytes;            /* algorithmic HBM bytes of every launch of the run (each column read or
+                                    written once per launch, tables once): the whole path's HBM traffic floor */
 } bh_run_stats;
 
 /* One-time operator diagnostics of a job (drivers/bandstructure.rs:437-498):
