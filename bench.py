@@ -53,7 +53,8 @@ TIMED_NAMES = {1: "op_kernel THETA_TM/TE (64x64 fused 2D FFT operator)",
                2: "gram2_kernel (dual Rayleigh-Ritz Gram Z^H BZ, Z^H AZ)",
                3: "rr_kernel (Cholesky/SVQB + projection + Jacobi Rayleigh-Ritz, FP64)",
                4: "lincomb_kernel (Ritz update [X'|W'] = Z C, A Z C)",
-               5: "deflate_kernel (residual / direction deflation against locked bands)"}
+               5: "deflate_kernel (residual / direction deflation against locked bands)",
+               6: "transition_kernel (k-point transition: tracking, prediction, start block)"}
 
 
 def solves_of(jobs):
@@ -343,7 +344,7 @@ def main():
                     help="only run the large combined case and print its figures (for profiling)")
     ap.add_argument("--learned-only", action="store_true",
                     help="only run the learned start (training and held-out evaluation) and print its figures")
-    ap.add_argument("--timed-kernel", type=int, default=3, choices=(1, 2, 3, 4, 5),
+    ap.add_argument("--timed-kernel", type=int, default=3, choices=(1, 2, 3, 4, 5, 6),
                     help="kernel class timed for the roofline: 1 Theta operator, 2 dual Rayleigh-Ritz Gram, "
                          "3 Rayleigh-Ritz solve (the dominant kernel), 4 Ritz-update linear combinations")
     ap.add_argument("--calibrate-copy", action="store_true",

@@ -17,7 +17,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libblaze2d_hip.so")
 ARCH = os.environ.get("BLAZE2D_HIP_ARCH", "gfx950")
 
-SOURCES = ["fft_ops.hip", "fft_big.hip", "blas.hip", "aux_kernels.hip", "dielectric.hip", "diagnostics.hip", "engine.hip", "api.hip", "cnn.hip", "hostmath.cpp"]
+SOURCES = ["fft_ops.hip", "fft_big.hip", "blas.hip", "aux_kernels.hip", "dielectric.hip", "diagnostics.hip", "transition.hip", "engine.hip", "api.hip", "cnn.hip", "hostmath.cpp"]
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 

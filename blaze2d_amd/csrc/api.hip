@@ -1080,7 +1080,7 @@ int bh_bandstructure_run_ex(bh_context *ctx, const bh_job *jobs, int njobs, cons
     eo.groups = opts && opts->groups > 0 ? opts->groups : 4;
     eo.timed_kernel = opts ? opts->timed_kernel : 0;
     eo.pw_half = pw_half;
-    if (eo.timed_kernel < 0 || eo.timed_kernel > 5) return fail("bh_bandstructure_run: timed_kernel must be 0..5");
+    if (eo.timed_kernel < 0 || eo.timed_kernel > 6) return fail("bh_bandstructure_run: timed_kernel must be 0..6");
     eo.gram_mfma_min = opts && opts->gram_mfma_min > 0 ? opts->gram_mfma_min : GRAM_MFMA_MIN_DEFAULT;
     const int engines = opts && opts->engines > 0 ? opts->engines : 1;
     if (engines > 16) return fail("bh_bandstructure_run: engines must be 1..16");

@@ -14,7 +14,8 @@
 
 namespace bh {
 
-enum TimedKernel { TIMED_NONE = 0, TIMED_THETA = 1, TIMED_GRAM2 = 2, TIMED_RR = 3, TIMED_RITZ = 4, TIMED_DEFL = 5 };
+enum TimedKernel { TIMED_NONE = 0, TIMED_THETA = 1, TIMED_GRAM2 = 2, TIMED_RR = 3, TIMED_RITZ = 4, TIMED_DEFL = 5,
+                   TIMED_TRANS = 6 };
 
 struct JobSpec {
     GeometrySpec geom;

@@ -5,11 +5,12 @@
 // advance together in steps; one step runs one LOBPCG iteration
 // (eigensolver.rs:1684-2192) for every iterating slot with batched kernels,
 // plus the k-point transitions (band tracking, subspace prediction, block
-// initialisation) of slots that just converged.  The host keeps the
+// initialisation) of slots that just converged, which run on the device in
+// the same step as the new k-point's first iteration.  The host keeps the
 // reference's per-instance control (convergence by relative eigenvalue
-// change, soft locking with its degeneracy guard, the solve() exit rule)
-// and the O(nb^3) transition algebra on small Gram matrices; every O(N)
-// operation runs on the GPU.  One host synchronisation per step.
+// change, soft locking with its degeneracy guard, the solve() exit rule) and
+// the decisions that depend only on the k-path; every O(N) operation and the
+// transition algebra run on the GPU.  One host synchronisation per step.
 //
 // Per-slot device layout (columns of N complex values, storage precision T),
 // two copies b = 0, 1 of V and AV; the slot's parity selects the current one
@@ -37,6 +38,7 @@
 #include "blas.h"
 #include "engine.h"
 #include "fft_ops.h"
+#include "transition.h"
 
 namespace bh {
 
@@ -183,7 +185,7 @@ struct Group {
     hipStream_t st{};
     hipEvent_t ev{};
     Staging stg;
-    double *part = nullptr, *gramrb = nullptr, *h_gramrb = nullptr;
+    double *part = nullptr, *gramrb = nullptr;  // Gram partial sums; reduced transition Grams
     double *h_rel = nullptr;  // readback of the group's residual norms
     double *h_prec = nullptr; // readback of the first-iteration preconditioned residual norms
     double *rrws = nullptr;   // wide Rayleigh-Ritz workspace, one block per slot of the group
@@ -217,7 +219,9 @@ T *dalloc(size_t n, hipStream_t st) {
     return p;
 }
 
-enum SlotState { FREE = 0, GRAM_PENDING = 1, ITER = 2 };
+// FREE: no job; ITER: solving at its k-point; DRAIN: the job's last solve has
+// finished and its record waits for the step's results
+enum SlotState { FREE = 0, ITER = 2, DRAIN = 3 };
 
 // ints per slot in the Rayleigh-Ritz counter array (RRWork::cnt)
 constexpr int CNT_STRIDE = 8;
@@ -243,13 +247,34 @@ struct Slot {
     bool lam_prev_valid = false;
     bool gamma_defl = false;
     OpParams params{};
-    // finished solve awaiting transition
+    // finished solve
     std::vector<double> res_eigs;
     int res_iters = 0;
     bool res_conv = false;
     std::vector<int> E_cols;  // column id in V (C == y0)
-    // K layout of the pending transition Gram
-    int kd = 0, nE = 0, nP = 0, nR = 0, rb_index = -1;
+    // the k-point transition launched in this step: K = [E | P | R | y0 | Wc]
+    // (E the finished eigenvectors, P the current history, R start vectors,
+    // Wc learned-start vectors), the device builds the tracking permutation,
+    // the history copy and the next block from K's Gram
+    bool t_pending = false;    // planned, launched by the next execute()
+    bool t_launched = false;   // launched; its status comes back with the step
+    int kd = 0, nE = 0, nP = 0, nR = 0, t_nwc = 0, t_flags = 0, t_m0 = 0;
+    int t_hist_new = -1, t_hist_old = -1;
+    double t_alpha = 0;
+    std::vector<double> t_om;  // finished frequencies (ascending) for the tracking
+    bool fin_pending = false;  // finishing work of the solve (exit norms, plane-wave projections) this step
+    // the finished k-point's record, completed when the step's results are back
+    // (the tracking permutation orders its frequencies)
+    struct Record {
+        int kidx = 0, iters = 0, rel_iter = 0, rel_m = 0, closing_k = 0;
+        bool conv = false, rel_fin = false, pw = false, track = false, save_first_gamma = false, closing = false;
+        bool gamma = false;  // the finished k-point deflated y0
+        std::vector<double> om, prec, rel;
+        std::vector<int> rel_act;
+        std::vector<char> rel_soft;  // soft-locked bands at the exit iteration
+    };
+    bool rec_pending = false;
+    Record rec;
     // tracking / history
     bool have_prev = false;
     std::vector<double> prev_om;
@@ -276,16 +301,9 @@ struct Slot {
     std::vector<int> rel_act;     // its active (non-soft-locked) bands
     std::vector<char> rel_soft;   // soft-locked bands at the exit iteration (rel_final)
     int rel_m = 0;                // block size of that iteration
-    // planned transition coefficients (set in decide, consumed in plan)
-    bool t2_pending = false;
-    std::vector<zd> cx;       // kd x m0 coefficients over the physical transition basis
-    std::vector<int> eperm;   // permuted eigenvector columns for the history copy
-    int t2_kd = 0;
-    int t2_hist_new = -1;
     // learned start: warm vectors synthesised from the job's plane-wave
     // coefficients into the second history buffer, used by the first k-point
     int nWc = 0;
-    int t2_nwc = 0;
     // plane-wave projections of a finished solve's eigenvectors (readback pending)
     bool pw_pending = false;
     std::vector<double> first_gamma_pw;
@@ -338,8 +356,21 @@ class Engine : public EngineBase {
         d_prec = dalloc<double>((size_t)S * LD, st);
         if (opt.pw_half < 0 || opt.pw_half > 8) throw std::runtime_error("plane-wave half width must be 0..8");
         if (opt.pw_half > 0) d_pw = dalloc<double>((size_t)S * nbmax * pw_ng() * 2, st);
+        // k-point transitions: block and history coefficients, status and
+        // permutation, the exit iteration's residual norms
+        d_tcx = dalloc<double>((size_t)S * LD * LD * 2, st);
+        d_thist = dalloc<double>((size_t)S * nbmax * nbmax * 2, st);
+        TOUT = 2 + LD;
+        d_tout = dalloc<int>((size_t)S * TOUT, st);
+        d_relf = dalloc<double>((size_t)S * LD, st);
+        if (wide) {  // the transition kernel's workspace does not fit in LDS
+            tbytes = (transition_bytes(LD, nbmax, m0max, nbmax) + 255) & ~size_t(255);
+            d_tscr = dalloc<char>((size_t)S * tbytes, st);
+        }
         BH_HIP_CHECK(hipHostMalloc((void **)&h_lam, sizeof(double) * S * LD, hipHostMallocDefault));
         BH_HIP_CHECK(hipHostMalloc((void **)&h_cnt, sizeof(int) * CNT_STRIDE * S, hipHostMallocDefault));
+        BH_HIP_CHECK(hipHostMalloc((void **)&h_tout, sizeof(int) * TOUT * S, hipHostMallocDefault));
+        BH_HIP_CHECK(hipHostMalloc((void **)&h_relf, sizeof(double) * LD * S, hipHostMallocDefault));
         int ng = opt.profile ? 1 : std::max(1, std::min(opt.groups, S));
         slots_per_group = (S + ng - 1) / ng;
         ng = (S + slots_per_group - 1) / slots_per_group;
@@ -358,7 +389,6 @@ class Engine : public EngineBase {
             G.part = dalloc<double>((size_t)2 * (chunk_budget + 2 * gs) * LD * LD * 2, st);
             if (wide) G.rrws = dalloc<double>((size_t)gs * rr_wide_bytes(C) / sizeof(double), st);
             G.gramrb = dalloc<double>((size_t)gs * LD * LD * 2, st);
-            BH_HIP_CHECK(hipHostMalloc((void **)&G.h_gramrb, sizeof(double) * gs * LD * LD * 2, hipHostMallocDefault));
             BH_HIP_CHECK(hipHostMalloc((void **)&G.h_rel, sizeof(double) * gs * LD, hipHostMallocDefault));
             BH_HIP_CHECK(hipHostMalloc((void **)&G.h_prec, sizeof(double) * gs * LD, hipHostMallocDefault));
             if (d_pw)
@@ -409,7 +439,6 @@ class Engine : public EngineBase {
             if (G.rrws) (void)hipFree(G.rrws);
             (void)hipFree(G.gramrb);
             (void)hipFree(G.sweeps);
-            (void)hipHostFree(G.h_gramrb);
             (void)hipHostFree(G.h_rel);
             (void)hipHostFree(G.h_prec);
             if (G.h_pw) (void)hipHostFree(G.h_pw);
@@ -421,10 +450,13 @@ class Engine : public EngineBase {
             (void)hipEventDestroy(G.ev);
             if (G.own_st) (void)hipStreamDestroy(G.st);
         }
-        for (void *p : std::initializer_list<void *>{V, AV, H, RND, EPS, TEN, d_lam, d_Y, d_coef, d_cnt, d_rel, d_prec, d_pw, twx, twy, revx, revy, TMP})
+        for (void *p : std::initializer_list<void *>{V, AV, H, RND, EPS, TEN, d_lam, d_Y, d_coef, d_cnt, d_rel, d_prec, d_pw, twx, twy, revx, revy, TMP,
+                                                     d_tcx, d_thist, d_tout, d_relf, d_tscr})
             if (p) (void)hipFree(p);
         (void)hipHostFree(h_lam);
         (void)hipHostFree(h_cnt);
+        (void)hipHostFree(h_tout);
+        (void)hipHostFree(h_relf);
         if (own_st) (void)hipStreamDestroy(st);
     }
 
@@ -449,6 +481,14 @@ class Engine : public EngineBase {
     double *d_pw{};   // plane-wave projections of finished solves, nbmax x (2h+1)^2 per slot
     double *h_lam{};
     int *h_cnt{};
+    // k-point transitions (see Slot::t_pending)
+    double *d_tcx{};    // LD x LD complex per slot: the new block over K
+    double *d_thist{};  // nbmax x nbmax complex per slot: the history copy
+    int *d_tout{}, *h_tout{};  // TOUT ints per slot: status, full rank, permutation
+    int TOUT = 0;
+    double *d_relf{}, *h_relf{};  // LD per slot: relative residual norms of a solve's exit iteration
+    char *d_tscr{};     // wide subspaces: tbytes of transition workspace per slot
+    size_t tbytes = 0;
     cplx<T> *twx{}, *twy{};
     int *revx{}, *revy{};     // digit-reversal tables (multi-pass operators)
     bool big = false;         // grid on the multi-pass operator path
@@ -508,14 +548,15 @@ class Engine : public EngineBase {
     struct StepPlan {
         std::vector<int> fill_y0;            // new jobs: set the Gamma mode column
         std::vector<int> table_upload;       // new jobs: upload eps / tensor tables
-        std::vector<int> t2;                 // transitions to apply (then iterate)
+        std::vector<int> fin;                // finished solves: exit-iteration norms, plane-wave projections
+        std::vector<int> trans;              // k-point transitions (Gram, device transition, new block)
         std::vector<int> iter;               // slots iterating this step
-        std::vector<int> t1;                 // transition Grams to compute
     };
 
     void start_job(int s, int job);
-    void schedule_t1(int s);
-    void transition_math(int s);
+    void begin_transition(int s);
+    void begin_block(int s, bool first_k);
+    void complete_step_results(int s);
     void absorb(int s);
     void decide(int s);
     void finish_solve(int s, bool conv, int iters);
@@ -617,22 +658,7 @@ void Engine<T>::start_job(int s, int job) {
     double nsq = sl.pol == 0 ? sl.diel.eps_sum : (double)N;
     sl.y0val = 1.0 / (nsq > 0 ? std::sqrt(nsq) : 1.0);
     set_kpoint(s);
-    sl.state = GRAM_PENDING;
-}
-
-// Transition Gram over K = [E (converged eigenvectors) | P (history current)
-// | R (start vectors) | y0], B-weighted.
-template <typename T>
-void Engine<T>::schedule_t1(int s) {
-    Slot &sl = slots[s];
-    sl.nE = (int)sl.E_cols.size();
-    sl.nP = (sl.hist_cur >= 0) ? sl.nb : 0;
-    sl.nR = sl.nE > 0 ? (sl.m0 - std::min(sl.nb, sl.m0) + 2) : sl.m0 + 2;
-    sl.nR = std::min(sl.nR, NRMAX);
-    if (sl.nE > 0) sl.nWc = 0;  // the learned start serves the first k-point only
-    sl.kd = sl.nE + sl.nP + sl.nR + 1 + sl.nWc;
-    if (sl.kd > LD) throw std::runtime_error("transition basis too wide");
-    sl.state = GRAM_PENDING;
+    sl.E_cols.clear();
 }
 
 template <typename T>
@@ -653,7 +679,7 @@ void Engine<T>::finish_solve(int s, bool conv, int iters) {
     sl.res_conv = conv;
     stats_->iterations += iters;
     stats_->solves += 1;
-    schedule_t1(s);
+    begin_transition(s);
 }
 
 // Results of the previous step's Rayleigh-Ritz for an iterating slot.
@@ -767,259 +793,196 @@ void Engine<T>::decide(int s) {
     // continue iterating: the plan for this slot is built in execute()
 }
 
-// Band tracking, history update, subspace prediction and block initialisation
-// for the next k-point, all in coefficient space over the transition basis.
+// A solve has finished (finish_solve): plan the move to the next k-point.
+// The reference's end-of-solve work (bandstructure.rs:755-806 tracking and
+// recording, subspace_prediction.rs:248-380 history and prediction,
+// initialization.rs:248-367 the next block) runs on the device in this step
+// (transition_kernel over the Gram of K = [E | P | R | y0]); the host keeps
+// the decisions that depend only on the k-path and the solve's own results,
+// and records the k-point once the tracking permutation is back.
 template <typename T>
-void Engine<T>::transition_math(int s) {
+void Engine<T>::begin_transition(int s) {
     Slot &sl = slots[s];
     const JobSpec &J = (*jobs_)[sl.job];
-    const int kd = sl.kd, nE = sl.nE, nP = sl.nP, nR = sl.nR;
-    const int iE = 0, iP = nE, iR = nE + nP, iY = nE + nP + nR;
-    std::vector<zd> M((size_t)kd * kd);
-    const double *g = group_of(s).h_gramrb + (size_t)sl.rb_index * LD * LD * 2;
-    for (int j = 0; j < kd; ++j)
-        for (int i = 0; i < kd; ++i) M[i + (size_t)j * kd] = zd(g[2 * (i + (size_t)j * LD)], g[2 * (i + (size_t)j * LD) + 1]);
-    // Hermitian symmetrisation from the lower triangle
-    for (int j = 0; j < kd; ++j)
-        for (int i = 0; i < j; ++i) M[i + (size_t)j * kd] = std::conj(M[j + (size_t)i * kd]);
-    for (int i = 0; i < kd; ++i) M[i + (size_t)i * kd] = M[i + (size_t)i * kd].real();
-    CoefSpace cs{kd, &M};
-
-    std::vector<int> perm(nE);
-    std::iota(perm.begin(), perm.end(), 0);
-    // the tracking overlap's polar factor, kept for the prediction below
-    std::vector<zd> track_u;
-    double track_smin = 0.0;
-    bool track_full = false;
-    int track_m = 0;
-    bool next_solve = false;
-    int knext = sl.k + 1;
-    if (nE > 0) {
-        // finished solve at sl.k: omegas, tracking (bandstructure.rs:755-806)
-        std::vector<double> om(sl.res_eigs.size());
-        for (size_t i = 0; i < om.size(); ++i) om[i] = sl.res_eigs[i] > 0 ? std::sqrt(sl.res_eigs[i]) : 0.0;
-        if (sl.tracking && sl.have_prev && nP > 0) {
-            int m = std::min(nE, nP);
-            std::vector<zd> O((size_t)m * m);
-            for (int j = 0; j < m; ++j)
-                for (int i = 0; i < m; ++i) O[i + (size_t)j * m] = M[(iP + i) + (size_t)(iE + j) * kd];
-            std::vector<int> p = track_bands(m, O, sl.prev_om, om, &track_u, &track_smin, &track_full);
-            track_m = m;
-            bool swaps = false;
-            for (int i = 0; i < (int)p.size(); ++i) swaps |= p[i] != i;
-            if (swaps) {
-                std::vector<double> om0 = om;
-                for (int i = 0; i < (int)p.size() && i < (int)om.size(); ++i) {
-                    om[i] = om0[p[i]];
-                    perm[i] = p[i];
-                }
-            }
-        }
-        sl.prev_om = om;
-        sl.have_prev = true;
-        take_norms(s);
-        // projections of the eigenvectors, in E order (ascending eigenvalue);
-        // tracking permutes the frequencies only
-        const double *pw = nullptr;
-        if (sl.pw_pending) {
-            pw = group_of(s).h_pw + (size_t)(s - group_of(s).s0) * nbmax * pw_ng() * 2;
-            sl.pw_pending = false;
-        }
-        record_k(s, sl.k, om, sl.res_iters, sl.res_conv, sl.prec_first, sl.rel_last, sl.rel_last_iter, pw);
-        if (sl.reuse_gamma && sl.k == 0 && is_gamma_frac(J.kpath[0])) {
-            sl.first_gamma = om;
-            sl.first_gamma_iters = sl.res_iters;
-            sl.first_gamma_rel = sl.rel_last;
-            sl.first_gamma_rel_iter = sl.rel_last_iter;
-            sl.first_gamma_prec = sl.prec_first;
-            if (pw) sl.first_gamma_pw.assign(pw, pw + (size_t)sl.nb * pw_ng() * 2);
-            sl.have_first_gamma = true;
-        }
-        // closing Gamma reuse (bandstructure.rs:580-601)
-        bool reuse = sl.reuse_gamma && sl.nk > 1 && is_gamma_frac(J.kpath[0]) && is_gamma_frac(J.kpath[sl.nk - 1]);
-        if (knext == sl.nk - 1 && reuse && sl.have_first_gamma) {
-            record_k(s, knext, sl.first_gamma, sl.first_gamma_iters, true, sl.first_gamma_prec, sl.first_gamma_rel,
-                     sl.first_gamma_rel_iter, sl.first_gamma_pw.empty() ? nullptr : sl.first_gamma_pw.data());
-            knext = sl.nk;
-        }
-        next_solve = knext < sl.nk;
+    const int nE = (int)sl.E_cols.size();
+    const int nP = sl.hist_cur >= 0 ? sl.nb : 0;
+    sl.nWc = 0;  // the learned start serves the first k-point only
+    // the record of the finished k-point
+    Slot::Record &R = sl.rec;
+    R = Slot::Record();
+    R.kidx = sl.k;
+    R.om.resize(sl.res_eigs.size());
+    for (size_t i = 0; i < R.om.size(); ++i) R.om[i] = sl.res_eigs[i] > 0 ? std::sqrt(sl.res_eigs[i]) : 0.0;
+    R.iters = sl.res_iters;
+    R.conv = sl.res_conv;
+    R.prec = sl.prec_first;
+    R.rel_fin = sl.rel_final;  // the exit iteration's norms are computed in this step
+    R.gamma = sl.gamma_defl;
+    if (R.rel_fin) {
+        R.rel_act = sl.rel_act;
+        R.rel_soft = sl.rel_soft;
+        R.rel_m = sl.rel_m;
+        R.rel_iter = sl.rel_pending_iter;
     } else {
-        knext = sl.k;  // first k-point of the job
-        next_solve = true;
+        R.rel = sl.rel_last;
+        R.rel_iter = sl.rel_last_iter;
     }
-    const int iW = iY + 1;  // learned-start columns follow y0
+    R.pw = eopt.pw_half > 0 && nE > 0;
+    R.track = sl.tracking && sl.have_prev && nP > 0 && nE > 0;
+    R.save_first_gamma = sl.reuse_gamma && sl.k == 0 && is_gamma_frac(J.kpath[0]);
+    int knext = sl.k + 1;
+    // closing Gamma reuse (bandstructure.rs:580-601)
+    const bool reuse = sl.reuse_gamma && sl.nk > 1 && is_gamma_frac(J.kpath[0]) && is_gamma_frac(J.kpath[sl.nk - 1]);
+    if (knext == sl.nk - 1 && reuse && (sl.have_first_gamma || R.save_first_gamma)) {
+        R.closing = true;
+        R.closing_k = knext;
+        knext = sl.nk;
+    }
+    const bool next_solve = knext < sl.nk;
+    sl.rec_pending = true;
+    sl.fin_pending = R.rel_fin || R.pw;
+    sl.rel_pending = sl.rel_final = false;
+    // K = [E | P | R | y0]
+    sl.nE = nE;
+    sl.nP = nP;
+    sl.nR = std::min(nE > 0 ? (sl.m0 - std::min(sl.nb, sl.m0) + 2) : sl.m0 + 2, NRMAX);
+    sl.kd = nE + nP + sl.nR + 1;
+    if (sl.kd > LD) throw std::runtime_error("transition basis too wide");
+    sl.t_nwc = 0;
+    sl.t_flags = R.track ? TR_TRACK : 0;
+    sl.t_om = R.om;
+    sl.t_alpha = 0.0;
+    sl.t_hist_old = sl.hist_cur;
+    sl.t_hist_new = -1;
     if (!next_solve) {
-        sl.state = FREE;
-        sl.job = -1;
+        // the job ends: the device is needed only for the tracking permutation
+        sl.state = DRAIN;
+        sl.t_m0 = 0;
+        sl.t_pending = R.track;
+        if (!sl.t_pending && !sl.fin_pending) complete_step_results(s);
         return;
     }
-    // --- history update (subspace_prediction.rs:248-262) and prediction ---
-    std::vector<std::vector<zd>> warm;  // coefficient vectors over K (kd)
+    // history update (subspace_prediction.rs:248-262)
     const bool had_cur = sl.hist_cur >= 0;
-    int new_hist = -1;
     if (nE > 0) {
-        new_hist = had_cur ? 1 - sl.hist_cur : 0;
+        sl.t_hist_new = had_cur ? 1 - sl.hist_cur : 0;
         if (had_cur) { sl.prev_k[0] = sl.cur_k[0]; sl.prev_k[1] = sl.cur_k[1]; }
         sl.hist_prev_valid = had_cur;
         const auto &kc = J.kpath[sl.k];
         sl.cur_k[0] = kc[0];
         sl.cur_k[1] = kc[1];
-        const int nb = std::min(nE, sl.nb);
-        auto ecoef = [&](int i) {  // coefficient vector of the permuted eigenvector i
-            std::vector<zd> c(kd, 0.0);
-            c[iE + perm[i]] = 1.0;
-            return c;
-        };
-        if (!sl.prediction || !had_cur) {
-            for (int i = 0; i < nb; ++i) warm.push_back(ecoef(i));
-        } else {
-            // predict_with_rotation: O = prev^H B curr (curr = permuted E)
-            int m = std::min(nb, nP);
-            std::vector<zd> O((size_t)m * m), U;
-            for (int j = 0; j < m; ++j)
-                for (int i = 0; i < m; ++i) O[i + (size_t)j * m] = M[(iP + i) + (size_t)(iE + perm[j]) * kd];
-            // O here is the tracking overlap with its columns permuted by perm
-            // (O P); when that overlap had full rank its polar factor is
-            // unique, and the factor of O P is U P: reuse it
-            double smin;
-            if (track_full && track_m == m) {
-                U.resize((size_t)m * m);
-                for (int j = 0; j < m; ++j)
-                    for (int i = 0; i < m; ++i) U[i + (size_t)j * m] = track_u[i + (size_t)perm[j] * m];
-                smin = track_smin;
-            } else {
-                smin = polar(m, O, U);
-            }
+        if (sl.prediction && had_cur) {
+            sl.t_flags |= TR_PREDICT;
+            // extrapolation unless the path turns a corner (CORNER_DOT_THRESHOLD);
+            // the kernel also drops it when the overlap is degenerate
             const auto &kn = J.kpath[knext];
-            double dkp[2] = {sl.cur_k[0] - sl.prev_k[0], sl.cur_k[1] - sl.prev_k[1]};
-            double dkn[2] = {kn[0] - sl.cur_k[0], kn[1] - sl.cur_k[1]};
-            bool degenerate = smin < 0.1;                           // DEGENERACY_THRESHOLD
-            bool corner = dkp[0] * dkn[0] + dkp[1] * dkn[1] < 0.0;  // CORNER_DOT_THRESHOLD
-            bool extrap = sl.extrapolation && !degenerate && !corner;
-            double alpha = 0.0, ca = 1.0;
-            if (extrap) {
-                double np_ = std::sqrt(dkp[0] * dkp[0] + dkp[1] * dkp[1]);
-                double nn_ = std::sqrt(dkn[0] * dkn[0] + dkn[1] * dkn[1]);
-                alpha = np_ > 1e-12 ? nn_ / np_ : 1.0;
-                ca = 1.0 + alpha;
-            }
-            // pred_j = sum_i curr_i (ca conj(U(j,i))) - alpha prev_j
-            for (int j = 0; j < m; ++j) {
-                std::vector<zd> c(kd, 0.0);
-                for (int i = 0; i < m; ++i) {
-                    zd u = U[j + (size_t)i * m];
-                    c[iE + perm[i]] += zd(u.real() * ca, -u.imag() * ca);
-                }
-                if (extrap) c[iP + j] += zd(-alpha, 0.0);
-                warm.push_back(c);
-            }
-            // orthonormalize_fields: modified Gram-Schmidt in the B inner
-            // product; with M w_i at hand, w_i^H M w_j = (M w_i)^H w_j
-            std::vector<zd> mw(kd);
-            for (int i = 0; i < m; ++i) {
-                cs.apply(warm[i].data(), mw.data());
-                double nrm = std::sqrt(std::max(cdot(warm[i].data(), mw.data(), kd).real(), 0.0));
-                if (nrm > 1e-14) {
-                    for (auto &v : warm[i]) v /= nrm;
-                    for (auto &v : mw) v /= nrm;
-                }
-                for (int j = i + 1; j < m; ++j) {
-                    zd c = cdot(mw.data(), warm[j].data(), kd);
-                    for (int q = 0; q < kd; ++q) warm[j][q] -= c * warm[i][q];
-                }
+            const double dkp[2] = {sl.cur_k[0] - sl.prev_k[0], sl.cur_k[1] - sl.prev_k[1]};
+            const double dkn[2] = {kn[0] - sl.cur_k[0], kn[1] - sl.cur_k[1]};
+            const bool corner = dkp[0] * dkn[0] + dkp[1] * dkn[1] < 0.0;
+            if (sl.extrapolation && !corner) {
+                sl.t_flags |= TR_EXTRAP;
+                const double np_ = std::sqrt(dkp[0] * dkp[0] + dkp[1] * dkp[1]);
+                const double nn_ = std::sqrt(dkn[0] * dkn[0] + dkn[1] * dkn[1]);
+                sl.t_alpha = np_ > 1e-12 ? nn_ / np_ : 1.0;
             }
         }
+        sl.hist_cur = sl.t_hist_new;
     }
-    // --- initialize_block (initialization.rs:248-367) with Gamma deflation ---
     sl.k = knext;
     set_kpoint(s);
-    std::vector<std::vector<zd>> ent, ment;  // accepted entries and their images M e
-    auto try_add = [&](std::vector<zd> v) {
-        for (size_t k = 0; k < ent.size(); ++k) {  // project_out: v -= <e, v>_B e
-            zd c = cdot(ment[k].data(), v.data(), kd);
-            for (int q = 0; q < kd; ++q) v[q] -= c * ent[k][q];
-        }
-        std::vector<zd> mv(kd);
-        cs.apply(v.data(), mv.data());
-        double nrm = std::sqrt(std::max(cdot(v.data(), mv.data(), kd).real(), 0.0));
-        if (!(nrm > 1e-12)) return false;
-        for (auto &x : v) x /= nrm;
-        for (auto &x : mv) x /= nrm;
-        ent.push_back(std::move(v));
-        ment.push_back(std::move(mv));
-        return true;
-    };
-    // initialize_block with warm vectors (initialization.rs:248-309): the learned
-    // start's vectors when the job brought them, before the random fill
-    for (int j = 0; j < sl.nWc && nE == 0; ++j) {
-        std::vector<zd> v(kd, 0.0);
-        v[iW + j] = 1.0;
-        warm.push_back(std::move(v));
+    begin_block(s, false);
+}
+
+// The start block of a solve (initialize_block, initialization.rs:248-367)
+// built on the device this step: a new job's first k-point (K = [R | y0 | Wc])
+// or the next k-point of a transition planned above.  The slot iterates in
+// the same step.
+template <typename T>
+void Engine<T>::begin_block(int s, bool first_k) {
+    Slot &sl = slots[s];
+    if (first_k) {
+        sl.nE = sl.nP = 0;
+        sl.nR = std::min(sl.m0 + 2, NRMAX);
+        sl.t_nwc = sl.nWc;
+        sl.kd = sl.nR + 1 + sl.t_nwc;
+        if (sl.kd > LD) throw std::runtime_error("transition basis too wide");
+        sl.t_flags = 0;
+        sl.t_om.clear();
+        sl.t_alpha = 0.0;
+        sl.t_hist_old = sl.t_hist_new = -1;
     }
-    for (size_t i = 0; i < warm.size() && (int)ent.size() < sl.m0; ++i) try_add(warm[i]);
-    int attempts = 0, r = 0;
-    while ((int)ent.size() < sl.m0 && attempts < sl.m0 * 8) {
-        if (r >= nR) throw std::runtime_error("ran out of start vectors during block initialisation");
-        std::vector<zd> v(kd, 0.0);
-        v[iR + r] = 1.0;
-        ++r;
-        if (!try_add(v)) ++attempts;
-    }
-    if (sl.gamma_defl) {
-        // deflation add_vector(y0) then project_single + renormalise each entry
-        std::vector<zd> y(kd, 0.0), my(kd);
-        y[iY] = 1.0;
-        cs.apply(y.data(), my.data());
-        double yn = std::sqrt(std::max(cdot(y.data(), my.data(), kd).real(), 0.0));
-        if (yn > 1e-12 && yn >= 1e-10) {
-            for (auto &v : y) v /= yn;
-            for (auto &v : my) v /= yn;
-        }
-        for (size_t k = 0; k < ent.size(); ++k) {
-            auto &e = ent[k], &me = ment[k];
-            zd c = cdot(my.data(), e.data(), kd);
-            for (int q = 0; q < kd; ++q) {
-                e[q] -= c * y[q];
-                me[q] -= c * my[q];
-            }
-            double nrm = std::sqrt(std::max(cdot(e.data(), me.data(), kd).real(), 0.0));
-            if (nrm > 1e-12)
-                for (auto &v : e) v /= nrm;
-        }
-    }
-    // physical basis for T2: [H_new (permuted E) | H_cur (P) | R | y0]
-    const int m0 = (int)ent.size();
-    sl.cx.assign((size_t)kd * std::max(m0, 1), 0.0);
-    for (int e = 0; e < m0; ++e)
-        for (int q = 0; q < kd; ++q) {
-            int pq = q;
-            if (q < nE) {
-                // E index q -> position i in the permuted order (perm[i] == q)
-                for (int i = 0; i < nE; ++i)
-                    if (perm[i] == q) { pq = i; break; }
-            }
-            sl.cx[pq + (size_t)e * kd] = ent[e][q];
-        }
-    sl.eperm.assign(nE, 0);
-    for (int i = 0; i < nE; ++i) sl.eperm[i] = sl.E_cols[perm[i]];
-    sl.t2_kd = kd;
-    sl.t2_nwc = nE == 0 ? sl.nWc : 0;
-    sl.t2_hist_new = new_hist;
-    sl.t2_pending = true;
+    if (sl.gamma_defl) sl.t_flags |= TR_GAMMA;
+    sl.t_m0 = sl.m0;
+    sl.t_pending = true;
     sl.it = 0;
-    sl.m = m0;
+    sl.m = sl.m0;
     sl.rel_last.clear();
     sl.rel_last_iter = 0;
     sl.rel_pending = sl.rel_final = false;
     sl.prec_first.clear();
     sl.prec_pending = false;
     sl.nW = 0;
-    sl.soft.assign(m0, 0);
-    sl.soft_iter.assign(m0, 0);
+    sl.soft.assign(sl.m0, 0);
+    sl.soft_iter.assign(sl.m0, 0);
     sl.lam.clear();
     sl.lam_prev.clear();
     sl.lam_prev_valid = false;
     sl.state = ITER;
+}
+
+// Results of the slot's transition and finished solve from the step that
+// launched them: the block's status, the tracking permutation, and the
+// record of the finished k-point (tracked frequencies, exit norms, plane-wave
+// projections).
+template <typename T>
+void Engine<T>::complete_step_results(int s) {
+    Slot &sl = slots[s];
+    const int *to = h_tout + (size_t)TOUT * s;
+    std::vector<int> perm;
+    if (sl.t_launched) {
+        sl.t_launched = false;
+        if (to[0] == -1) throw std::runtime_error("ran out of start vectors during block initialisation");
+        if (to[0] == -2) throw std::runtime_error("k-point transition outside the kernel's capacity");
+        if (to[0] != sl.t_m0) throw std::runtime_error("k-point transition built a short block");
+        if (sl.rec_pending && sl.rec.track) perm.assign(to + 2, to + 2 + sl.rec.om.size());
+    }
+    if (!sl.rec_pending) return;
+    sl.rec_pending = false;
+    Slot::Record &R = sl.rec;
+    std::vector<double> om = R.om;
+    for (size_t i = 0; i < perm.size(); ++i) {
+        if (perm[i] < 0 || perm[i] >= (int)R.om.size()) throw std::runtime_error("invalid tracking permutation");
+        om[i] = R.om[perm[i]];
+    }
+    sl.prev_om = om;
+    sl.have_prev = true;
+    std::vector<double> rel = R.rel;
+    if (R.rel_fin) {  // soft-locked bands: zero residual
+        rel.assign(std::min(R.rel_m, sl.nb), 0.0);
+        const double *h = h_relf + (size_t)LD * s;
+        for (int b : R.rel_act)
+            if (b < (int)rel.size()) rel[b] = h[b];
+    }
+    // projections of the eigenvectors, in E order (ascending eigenvalue);
+    // tracking permutes the frequencies only
+    const double *pw = R.pw ? group_of(s).h_pw + (size_t)(s - group_of(s).s0) * nbmax * pw_ng() * 2 : nullptr;
+    record_k(s, R.kidx, om, R.iters, R.conv, R.prec, rel, R.rel_iter, pw);
+    if (R.save_first_gamma) {
+        sl.first_gamma = om;
+        sl.first_gamma_iters = R.iters;
+        sl.first_gamma_rel = rel;
+        sl.first_gamma_rel_iter = R.rel_iter;
+        sl.first_gamma_prec = R.prec;
+        if (pw) sl.first_gamma_pw.assign(pw, pw + (size_t)sl.nb * pw_ng() * 2);
+        sl.have_first_gamma = true;
+    }
+    if (R.closing)
+        record_k(s, R.closing_k, sl.first_gamma, sl.first_gamma_iters, true, sl.first_gamma_prec, sl.first_gamma_rel,
+                 sl.first_gamma_rel_iter, sl.first_gamma_pw.empty() ? nullptr : sl.first_gamma_pw.data());
+    if (sl.state == DRAIN) {
+        sl.state = FREE;
+        sl.job = -1;
+    }
 }
 
 template <typename T>
@@ -1056,23 +1019,15 @@ void Engine<T>::run(const std::vector<JobSpec> &jobs, std::vector<JobResult> &ou
             }
             auto p0 = std::chrono::steady_clock::now();
             StepPlan plan;
-            // k-point transitions are independent across slots: run them on the pool
-            std::vector<int> trans;
+            // results of the group's previous step: transition status, tracking
+            // permutations, records of finished k-points
             for (int s = G.s0; s < G.s1; ++s)
-                if (slots[s].state == GRAM_PENDING && slots[s].rb_index >= 0) trans.push_back(s);
-            pool->parallel_for((int)trans.size(), [&](int i) { transition_math(trans[i]); });
-            std::vector<char> in_trans(G.s1 - G.s0, 0);
-            for (int s : trans) in_trans[s - G.s0] = 1;
-            for (int s = G.s0; s < G.s1; ++s) {
-                Slot &sl = slots[s];
-                if (in_trans[s - G.s0]) {
-                    sl.rb_index = -1;
-                    if (sl.state == ITER) plan.t2.push_back(s);
-                } else if (sl.state == ITER) {
+                if (slots[s].t_launched || slots[s].rec_pending) complete_step_results(s);
+            for (int s = G.s0; s < G.s1; ++s)
+                if (slots[s].state == ITER) {
                     absorb(s);
                     decide(s);
                 }
-            }
             std::vector<std::pair<int, int>> starts;  // (slot, job)
             for (int s = G.s0; s < G.s1; ++s)
                 if (slots[s].state == FREE && next_job < jobs.size()) starts.push_back({s, (int)next_job++});
@@ -1081,15 +1036,15 @@ void Engine<T>::run(const std::vector<JobSpec> &jobs, std::vector<JobResult> &ou
                 const int s = sj.first;
                 plan.table_upload.push_back(s);
                 plan.fill_y0.push_back(s);
-                slots[s].E_cols.clear();
-                schedule_t1(s);
+                begin_block(s, true);
             }
-            bool any = false;
             for (int s = G.s0; s < G.s1; ++s) {
                 Slot &sl = slots[s];
-                if (sl.state == ITER) { plan.iter.push_back(s); any = true; }
-                else if (sl.state == GRAM_PENDING && sl.rb_index < 0) { plan.t1.push_back(s); any = true; }
+                if (sl.fin_pending) plan.fin.push_back(s);
+                if (sl.t_pending) plan.trans.push_back(s);
+                if (sl.state == ITER) plan.iter.push_back(s);
             }
+            const bool any = !plan.iter.empty() || !plan.trans.empty() || !plan.fin.empty();
             if (!any) {
                 stats.host_plan_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - p0).count();
                 continue;
@@ -1140,61 +1095,176 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
         tcopies.push_back({ten(s), dt, (size_t)4 * N});
     }
 
-    // ---- T2: apply transitions (history copy, X = K C, mass, Theta, RQ) ----
+    // Finished solves and k-point transitions are planned before the
+    // iteration section, which moves each iterating slot to its other V / AV
+    // copy: they address the copy that holds the finished solve's X and AX.
+
+    // ---- finished solves: the exit iteration's residual norms, plane-wave projections ----
+    std::vector<ResidualWork<T>> res_fin;
+    std::vector<DeflateWork<T>> def_fin;
+    std::vector<int> def_fin_slot;
+    std::vector<PwWork<T>> pw_proj;
+    int max_r_fin = 0, max_d_fin = 0;
+    for (int s : plan.fin) {
+        Slot &sl = slots[s];
+        sl.fin_pending = false;
+        const Slot::Record &R = sl.rec;
+        if (R.rel_fin) {
+            // R = A X - B X lambda of the bands not soft-locked, into the free
+            // columns [m, m + a), deflated against y0 and the soft-locked
+            // bands, then their relative B-norms (into d_relf)
+            const int m = R.rel_m, a = (int)R.rel_act.size();
+            int *act_d;
+            int *act_h = stg.alloc<int>(std::max(a, 1), &act_d);
+            for (int i = 0; i < a; ++i) act_h[i] = R.rel_act[i];
+            ResidualWork<T> rw{};
+            rw.ax = avcol(s, 0);
+            rw.bx = vcol(s, 0);
+            rw.bw = bweight(s);
+            rw.out = vcol(s, m);
+            rw.act = act_d;
+            rw.lam = d_lam + (size_t)s * LD;
+            rw.stride = N;
+            rw.count = a;
+            max_r_fin = std::max(max_r_fin, a);
+            std::vector<int> dcols;
+            if (R.gamma) dcols.push_back(C);
+            for (int j = 0; j < (int)R.rel_soft.size() && j < m; ++j)
+                if (R.rel_soft[j]) dcols.push_back(j);
+            const int nd = (int)dcols.size();
+            if (nd > 0 && a > 0) {
+                const cplx<T> **dd_d;
+                const cplx<T> **dd = stg.alloc<const cplx<T> *>(nd, (const cplx<T> ***)&dd_d);
+                for (int i = 0; i < nd; ++i) dd[i] = vcol(s, dcols[i]);
+                DeflateWork<T> dw{};
+                dw.d_ptrs = dd_d;
+                dw.weight = bweight(s);
+                dw.R = vcol(s, m);
+                dw.stride = N;
+                dw.nd = nd;
+                dw.a = a;
+                dw.ld = LD;
+                dw.act = act_d;
+                dw.lam = d_lam + (size_t)s * LD;
+                dw.rel = d_relf + (size_t)s * LD;
+                def_fin.push_back(dw);
+                def_fin_slot.push_back(s);
+                max_d_fin = std::max(max_d_fin, nd);
+            } else {
+                rw.rel = d_relf + (size_t)s * LD;
+            }
+            res_fin.push_back(rw);
+        }
+        if (R.pw) {  // projections of the finished solve's eigenvectors
+            for (int i = 0; i < (int)sl.E_cols.size() && i < sl.nb; ++i) {
+                PwWork<T> pw{};
+                pw.x = vcol(s, sl.E_cols[i]);
+                pw.c = pwbuf(s) + (size_t)i * pw_ng() * 2;
+                pw.stride = N;
+                pw.count = 1;
+                pw_proj.push_back(pw);
+            }
+        }
+    }
+
+    // ---- k-point transitions: Gram of K, the device transition, history copy,
+    // the new block X = K cx, its A X and Rayleigh quotients ----
+    std::vector<GramWork<T>> g_t1;
+    std::vector<TransWork> tw;
     std::vector<LincombWork<T>> lc_hist, lc_x;
     std::vector<OpWork<T>> op_x[2];
     std::vector<RayleighWork<T>> rq;
-    int maxq_x = 0;
-    for (int s : plan.t2) {
+    int maxq_x = 0, maxkd = 0;
+    for (int s : plan.trans) {
         Slot &sl = slots[s];
-        const int kd = sl.t2_kd, nE = sl.nE, nP = sl.nP, nR = sl.nR;
-        const int hn = sl.t2_hist_new;
-        if (nE > 0) {
-            // history copy: H[hn][i] = E[perm[i]]
+        sl.t_pending = false;
+        sl.t_launched = true;
+        const int kd = sl.kd, nE = sl.nE, nP = sl.nP, nR = sl.nR, m0 = sl.t_m0;
+        maxkd = std::max(maxkd, kd);
+        // K = [E | P | R | y0 | Wc], B-weighted Gram (lower triangle)
+        const cplx<T> **kp_d;
+        const cplx<T> **kp = stg.alloc<const cplx<T> *>(kd, (const cplx<T> ***)&kp_d);
+        int q = 0;
+        for (int i = 0; i < nE; ++i) kp[q++] = vcol(s, sl.E_cols[i]);
+        for (int i = 0; i < nP; ++i) kp[q++] = hcol(s, sl.t_hist_old, i);
+        for (int i = 0; i < nR; ++i) kp[q++] = rcol(i);
+        kp[q++] = vcol(s, C);
+        for (int i = 0; i < sl.t_nwc; ++i) kp[q++] = hcol(s, 1, i);
+        GramWork<T> g{};
+        g.a_ptrs = kp_d;
+        g.b_ptrs = kp_d;
+        g.weight = bweight(s);
+        g.na = kd;
+        g.nb = kd;
+        g.ld = LD;
+        g.lower_only = 1;
+        TransWork t{};
+        t.gram = d_gramrb + g_t1.size() * (size_t)LD * LD * 2;
+        g_t1.push_back(g);
+        double *om_d = nullptr, *pv_d = nullptr;
+        if (!sl.t_om.empty()) {
+            double *h = stg.alloc<double>(sl.t_om.size(), &om_d);
+            std::memcpy(h, sl.t_om.data(), sizeof(double) * sl.t_om.size());
+        }
+        if (!sl.prev_om.empty()) {
+            double *h = stg.alloc<double>(sl.prev_om.size(), &pv_d);
+            std::memcpy(h, sl.prev_om.data(), sizeof(double) * sl.prev_om.size());
+        }
+        t.om = om_d;
+        t.prev_om = pv_d;
+        t.nprev = (int)sl.prev_om.size();
+        t.cx = d_tcx + (size_t)s * LD * LD * 2;
+        t.hist = d_thist + (size_t)s * nbmax * nbmax * 2;
+        t.out = d_tout + (size_t)s * TOUT;
+        t.scratch = d_tscr ? d_tscr + (size_t)s * tbytes : nullptr;
+        t.alpha = sl.t_alpha;
+        t.ld = LD;
+        t.kd = kd;
+        t.nE = nE;
+        t.nP = nP;
+        t.nR = nR;
+        t.nWc = sl.t_nwc;
+        t.nb = sl.nb;
+        t.m0 = m0;
+        t.flags = sl.t_flags;
+        tw.push_back(t);
+        if (m0 == 0) continue;  // the job ends: the tracking permutation only
+        const int hn = sl.t_hist_new;
+        if (nE > 0 && hn >= 0) {
+            // history copy: H[hn][i] = E[perm[i]] through permutation coefficients
             const cplx<T> **pp_d;
             const cplx<T> **pp = stg.alloc<const cplx<T> *>(nE, (const cplx<T> ***)&pp_d);
-            for (int i = 0; i < nE; ++i) pp[i] = vcol(s, sl.eperm[i]);
-            double *cd;
-            double *ch = stg.alloc<double>((size_t)2 * nE * nE, &cd);
-            std::memset(ch, 0, sizeof(double) * 2 * nE * nE);
-            for (int i = 0; i < nE; ++i) ch[2 * (i + (size_t)i * nE)] = 1.0;
+            for (int i = 0; i < nE; ++i) pp[i] = vcol(s, sl.E_cols[i]);
             LincombWork<T> w{};
             w.a_ptrs = pp_d;
             w.na = nE;
             w.nq = nE;
-            w.C = cd;
+            w.C = t.hist;
             w.ldc = nE;
             w.out = hcol(s, hn, 0);
             w.out_stride = N;
             w.mode = 0;
             lc_hist.push_back(w);
         }
-        // X = [H_new | H_cur | R | y0] cx
-        const cplx<T> **kp_d;
-        const cplx<T> **kp = stg.alloc<const cplx<T> *>(kd, (const cplx<T> ***)&kp_d);
-        int q = 0;
-        for (int i = 0; i < nE; ++i) kp[q++] = hcol(s, hn, i);
-        for (int i = 0; i < nP; ++i) kp[q++] = hcol(s, sl.hist_cur, i);
-        for (int i = 0; i < nR; ++i) kp[q++] = rcol(i);
-        kp[q++] = vcol(s, C);
-        for (int i = 0; i < sl.t2_nwc; ++i) kp[q++] = hcol(s, 1, i);
-        double *cd;
-        double *ch = stg.alloc<double>((size_t)2 * kd * sl.m, &cd);
-        for (int e = 0; e < sl.m; ++e)
-            for (int i = 0; i < kd; ++i) {
-                ch[2 * (i + (size_t)e * kd)] = sl.cx[i + (size_t)e * kd].real();
-                ch[2 * (i + (size_t)e * kd) + 1] = sl.cx[i + (size_t)e * kd].imag();
-            }
+        // X = [H_new | P | R | y0 | Wc] cx
+        const cplx<T> **xp_d;
+        const cplx<T> **xp = stg.alloc<const cplx<T> *>(kd, (const cplx<T> ***)&xp_d);
+        q = 0;
+        for (int i = 0; i < nE; ++i) xp[q++] = hcol(s, hn, i);
+        for (int i = 0; i < nP; ++i) xp[q++] = hcol(s, sl.t_hist_old, i);
+        for (int i = 0; i < nR; ++i) xp[q++] = rcol(i);
+        xp[q++] = vcol(s, C);
+        for (int i = 0; i < sl.t_nwc; ++i) xp[q++] = hcol(s, 1, i);
         LincombWork<T> w{};
-        w.a_ptrs = kp_d;
+        w.a_ptrs = xp_d;
         w.na = kd;
-        w.nq = sl.m;
-        w.C = cd;
+        w.nq = m0;
+        w.C = t.cx;
         w.ldc = kd;
         w.out = vcol(s, 0);
         w.out_stride = N;
         lc_x.push_back(w);
-        maxq_x = std::max(maxq_x, sl.m);
+        maxq_x = std::max(maxq_x, m0);
         OpWork<T> o{};
         o.in = vcol(s, 0);
         o.out = avcol(s, 0);
@@ -1205,7 +1275,7 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
         o.tmp_stride = N;
         o.in_stride = N;
         o.out_stride = N;
-        o.count = sl.m;
+        o.count = m0;
         op_x[sl.pol].push_back(o);
         RayleighWork<T> rw{};
         rw.x = vcol(s, 0);
@@ -1214,10 +1284,8 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
         rw.bw = bweight(s);
         rw.lam = d_lam + (size_t)s * LD;
         rw.stride = N;
-        rw.count = sl.m;
+        rw.count = m0;
         rq.push_back(rw);
-        if (hn >= 0) sl.hist_cur = hn;
-        sl.t2_pending = false;
     }
 
     // ---- iteration for ITER slots ----
@@ -1436,92 +1504,7 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
         sl.it += 1;
     }
 
-    // ---- T1: transition Grams ----
-    std::vector<GramWork<T>> g_t1;
-    std::vector<int> t1_slots;
-    std::vector<PwWork<T>> pw_proj, pw_syn;
-    for (int s : plan.t1) {
-        Slot &sl = slots[s];
-        if (sl.rel_final) {
-            // the exit iteration's residuals R = A X - B X lambda of the
-            // non-soft-locked bands into the free columns [m, m + a), deflated
-            // against y0 and the soft-locked bands, then their B-norms
-            const int m = sl.rel_m, a = (int)sl.rel_act.size();
-            int *act_d;
-            int *act_h = stg.alloc<int>(std::max(a, 1), &act_d);
-            for (int i = 0; i < a; ++i) act_h[i] = sl.rel_act[i];
-            ResidualWork<T> rw{};
-            rw.ax = avcol(s, 0);
-            rw.bx = vcol(s, 0);
-            rw.bw = bweight(s);
-            rw.out = vcol(s, m);
-            rw.act = act_d;
-            rw.lam = d_lam + (size_t)s * LD;
-            rw.stride = N;
-            rw.count = a;
-            max_r = std::max(max_r, a);
-            any_norms = true;
-            std::vector<int> dcols;
-            if (sl.gamma_defl) dcols.push_back(C);
-            for (int j = 0; j < (int)sl.rel_soft.size() && j < m; ++j)
-                if (sl.rel_soft[j]) dcols.push_back(j);
-            const int nd = (int)dcols.size();
-            if (nd > 0 && a > 0) {
-                const cplx<T> **dd_d;
-                const cplx<T> **dd = stg.alloc<const cplx<T> *>(nd, (const cplx<T> ***)&dd_d);
-                for (int i = 0; i < nd; ++i) dd[i] = vcol(s, dcols[i]);
-                DeflateWork<T> dw{};
-                dw.d_ptrs = dd_d;
-                dw.weight = bweight(s);
-                dw.R = vcol(s, m);
-                dw.stride = N;
-                dw.nd = nd;
-                dw.a = a;
-                dw.ld = LD;
-                dw.act = act_d;
-                dw.lam = d_lam + (size_t)s * LD;
-                dw.rel = d_rel + (size_t)s * LD;
-                defR.push_back(dw);
-                defR_slot.push_back(s);
-                max_d = std::max(max_d, nd);
-            } else {
-                rw.rel = d_rel + (size_t)s * LD;
-            }
-            res.push_back(rw);
-        }
-        if (eopt.pw_half > 0 && sl.nE > 0) {  // projections of the finished solve's eigenvectors
-            for (int i = 0; i < sl.nE && i < sl.nb; ++i) {
-                PwWork<T> pw{};
-                pw.x = vcol(s, sl.E_cols[i]);
-                pw.c = pwbuf(s) + (size_t)i * pw_ng() * 2;
-                pw.stride = N;
-                pw.count = 1;
-                pw_proj.push_back(pw);
-            }
-            sl.pw_pending = true;
-        }
-        const int kd = sl.kd;
-        const cplx<T> **kp_d;
-        const cplx<T> **kp = stg.alloc<const cplx<T> *>(kd, (const cplx<T> ***)&kp_d);
-        int q = 0;
-        for (int i = 0; i < sl.nE; ++i) kp[q++] = vcol(s, sl.E_cols[i]);
-        for (int i = 0; i < sl.nP; ++i) kp[q++] = hcol(s, sl.hist_cur, i);
-        for (int i = 0; i < sl.nR; ++i) kp[q++] = rcol(i);
-        kp[q++] = vcol(s, C);
-        for (int i = 0; i < (sl.nE == 0 ? sl.nWc : 0); ++i) kp[q++] = hcol(s, 1, i);
-        GramWork<T> g{};
-        g.a_ptrs = kp_d;
-        g.b_ptrs = kp_d;
-        g.weight = sl.pol == 0 ? eps(s) : nullptr;
-        g.na = kd;
-        g.nb = kd;
-        g.ld = LD;
-        g.lower_only = 1;
-        g_t1.push_back(g);
-        sl.rb_index = (int)t1_slots.size();
-        t1_slots.push_back(s);
-    }
-
+    std::vector<PwWork<T>> pw_syn;
     // learned start: the new jobs' warm vectors from their plane-wave coefficients
     int max_wc = 0, wc_half = 0;
     for (int s : plan.table_upload) {
@@ -1614,7 +1597,7 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
         ReduceWork *dr{};
         LincombWork<T> *dl{};
         NormWork<T> *dn{};
-    } drR, drP;
+    } drR, drP, drF;
     auto to_rows = [&](const std::vector<DeflateWork<T>> &items, const std::vector<int> &islot, DeflRows &o) {
         o.nch = nch_for(items.size());
         for (size_t i = 0; i < items.size(); ++i) {
@@ -1646,27 +1629,33 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
             if (d.rel) o.n.push_back(NormWork<T>{d.R, d.weight, d.act, d.lam, d.rel, d.stride, d.a, 0});
         }
     };
+    // the finished solves' exit deflation, run before the transitions
+    const bool defl_rows_fin = N > DEFL_MAXROWS || max_d_fin > DEFL_MAXD;
     if (defl_rows) {
         to_rows(defR, defR_slot, drR);
         to_rows(defP, defP_slot, drP);
-        for (DeflRows *o : {&drR, &drP}) {
-            o->dg = put(o->g);
-            o->dr = put(o->r);
-            o->dl = put(o->l);
-            o->dn = put(o->n);
-        }
+    }
+    if (defl_rows_fin) to_rows(def_fin, def_fin_slot, drF);
+    for (DeflRows *o : {&drR, &drP, &drF}) {
+        o->dg = put(o->g);
+        o->dr = put(o->r);
+        o->dl = put(o->l);
+        o->dn = put(o->n);
     }
     auto deflate = [&](const DeflateWork<T> *dwork, const std::vector<DeflateWork<T>> &items, const DeflRows &o,
-                       int maxa) {
-        if (!defl_rows) {
-            launch_deflate<T>(dwork, (int)items.size(), N, maxa, max_d, st);
+                       int maxa, int maxd, bool rows) {
+        if (!rows) {
+            launch_deflate<T>(dwork, (int)items.size(), N, maxa, maxd, st);
             return;
         }
-        launch_gram<T>(o.dg, (int)o.g.size(), o.nch, N, std::max(max_d, 1), maxa, st, eopt.gram_mfma_min);
+        launch_gram<T>(o.dg, (int)o.g.size(), o.nch, N, std::max(maxd, 1), maxa, st, eopt.gram_mfma_min);
         launch_reduce(o.dr, (int)o.r.size(), LD, st);
-        launch_lincomb<T>(o.dl, (int)o.l.size(), N, maxa, st, std::max(max_d, 1));
+        launch_lincomb<T>(o.dl, (int)o.l.size(), N, maxa, st, std::max(maxd, 1));
         launch_bnorm<T>(o.dn, (int)o.n.size(), maxa, N, st);
     };
+    auto *d_res_fin = put(res_fin);
+    auto *d_def_fin = put(def_fin);
+    auto *d_tw = put(tw);
     auto *d_lc_hist = put(lc_hist);
     auto *d_lc_x = put(lc_x);
     OpWork<T> *d_op_x[2] = {put(op_x[0]), put(op_x[1])};
@@ -1708,12 +1697,14 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
     stats_->alg_bytes += lc_bytes(lc_hist) + lc_bytes(lc_x);
     for (auto &r : rq) stats_->alg_bytes += r.count * 2.0 * colb + (r.bw ? wb : 0.0);
     for (auto &r : res) stats_->alg_bytes += r.count * 3.0 * colb + (r.bw ? wb : 0.0);
+    for (auto &r : res_fin) stats_->alg_bytes += r.count * 3.0 * colb + (r.bw ? wb : 0.0);
+    for (auto &d : def_fin) stats_->alg_bytes += (double)N * ((2.0 * d.a + d.nd) * sizeof(cplx<T>) + (d.weight ? sizeof(T) : 0));
     for (int k = 0; k < 2; ++k)
         for (auto &o : op_pre[k])
             stats_->alg_bytes += o.count * (2.0 + (o.bout ? 1.0 : 0.0)) * colb + (k == 1 || o.bout ? wb : 0.0);
     for (auto &n : prec_norms) stats_->alg_bytes += n.count * colb + (n.bw ? wb : 0.0);
     for (auto &g : g_t1) stats_->alg_bytes += (g.na + g.nb) * colb + (g.weight ? wb : 0.0);
-    stats_->alg_bytes += pw_proj.size() * nbmax * colb;
+    stats_->alg_bytes += pw_proj.size() * colb;
     auto timed = [&](int cls, double bytes, double flops, auto &&launch) {
         stats_->alg_bytes += bytes;
         if (eopt.timed_kernel != cls) { launch(); return; }
@@ -1733,7 +1724,29 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
         stats_->timed_flops_executed += flops;
     };
     const double n_log = (double)N * std::log2((double)N);
-    // T2
+    // finished solves: exit-iteration norms and plane-wave projections (read X)
+    launch_residual<T>(d_res_fin, (int)res_fin.size(), std::max(max_r_fin, 1), N, st);
+    if (!def_fin.empty()) deflate(d_def_fin, def_fin, drF, max_r_fin, max_d_fin, defl_rows_fin);
+    launch_pw_project<T>(d_pw_proj, (int)pw_proj.size(), 1, nx, ny, eopt.pw_half, st);
+    // transitions: Gram of K, tracking / prediction / block on the device,
+    // history copy (reads E), X = K cx (overwrites E's copy)
+    if (!g_t1.empty()) {
+        launch_gram<T>(d_g_t1, (int)g_t1.size(), nch_t1, N, maxkd, maxkd, st, eopt.gram_mfma_min);
+        launch_gram_reduce(d_part, d_gramrb, (int)g_t1.size(), nch_t1, LD, (long)nch_t1 * LD * LD * 2,
+                           (long)LD * LD * 2, st);
+        // the workspace is laid out for this launch's largest item
+        double bytes = 0;  // the Gram in, block and history coefficients out
+        int KDl = 1, NBWl = 1, M0l = 1, MMl = 1;
+        for (const TransWork &t : tw) {
+            bytes += 16.0 * ((double)t.kd * t.kd + (double)t.kd * t.m0 + (double)t.nE * t.nE);
+            KDl = std::max(KDl, t.kd);
+            NBWl = std::max(NBWl, std::max(std::min(t.nE, t.nb), t.nWc));
+            M0l = std::max(M0l, t.m0);
+            MMl = std::max(MMl, std::min(t.nE, t.nP));
+        }
+        timed(TIMED_TRANS, bytes, 0.0,
+              [&] { launch_transition(d_tw, (int)tw.size(), KDl, NBWl, M0l, MMl, d_tscr != nullptr, st); });
+    }
     launch_lincomb<T>(d_lc_hist, (int)lc_hist.size(), N, nbmax, st);
     launch_lincomb<T>(d_lc_x, (int)lc_x.size(), N, std::max(maxq_x, 1), st, LD);
     for (int pol = 0; pol < 2; ++pol) {
@@ -1762,7 +1775,7 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
     };
     if (!defR.empty())
         timed(TIMED_DEFL, defl_bytes(defR), defl_flops(defR),
-              [&] { deflate(d_defR, defR, drR, max_r); });
+              [&] { deflate(d_defR, defR, drR, max_r, max_d, defl_rows); });
     const bool prec_fp32 = eopt.precision == 2;
     launch_op<T>(MODE_PREC_FD, nx, ny, d_op_pre[0], (int)op_pre[0].size(), std::max(max_a, 1), twx, twy, st,
                  prec_fp32, revx, revy);
@@ -1771,7 +1784,7 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
     launch_bnorm<T>(d_prec_norms, (int)prec_norms.size(), std::max(max_a, 1), N, st);
     if (!defP.empty())
         timed(TIMED_DEFL, defl_bytes(defP), defl_flops(defP),
-              [&] { deflate(d_defP, defP, drP, max_a); });
+              [&] { deflate(d_defP, defP, drP, max_a, max_d, defl_rows); });
     for (int pol = 0; pol < 2; ++pol) {
         if (op_p[pol].empty()) continue;
         timed(TIMED_THETA, (double)theta_cols[pol] * theta_bytes_per_col(pol),
@@ -1812,20 +1825,19 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
         timed(TIMED_RITZ, bytes, flops,
               [&] { launch_lincomb<T>(d_lc_ritz, (int)lc_ritz.size(), N, std::max(max_ritz, 1), st, max_p); });
     }
-    launch_pw_project<T>(d_pw_proj, (int)pw_proj.size(), 1, nx, ny, eopt.pw_half, st);
     if (!pw_proj.empty()) {
         const int gs = G.s1 - G.s0;
         BH_HIP_CHECK(hipMemcpyAsync(G.h_pw, pwbuf(G.s0), sizeof(double) * gs * nbmax * pw_ng() * 2,
                                     hipMemcpyDeviceToHost, st));
     }
-    // T1 Grams -> compact readback
-    if (!g_t1.empty()) {
-        int maxkd = 0;
-        for (int s : t1_slots) maxkd = std::max(maxkd, slots[s].kd);
-        launch_gram<T>(d_g_t1, (int)g_t1.size(), nch_t1, N, maxkd, maxkd, st, eopt.gram_mfma_min);
-        launch_gram_reduce(d_part, d_gramrb, (int)g_t1.size(), nch_t1, LD, (long)nch_t1 * LD * LD * 2,
-                           (long)LD * LD * 2, st);
-        BH_HIP_CHECK(hipMemcpyAsync(G.h_gramrb, d_gramrb, sizeof(double) * g_t1.size() * LD * LD * 2,
+    if (!tw.empty()) {
+        const int gs = G.s1 - G.s0;
+        BH_HIP_CHECK(hipMemcpyAsync(h_tout + (size_t)TOUT * G.s0, d_tout + (size_t)TOUT * G.s0,
+                                    sizeof(int) * TOUT * gs, hipMemcpyDeviceToHost, st));
+    }
+    if (!res_fin.empty()) {
+        const int gs = G.s1 - G.s0;
+        BH_HIP_CHECK(hipMemcpyAsync(h_relf + (size_t)LD * G.s0, d_relf + (size_t)LD * G.s0, sizeof(double) * gs * LD,
                                     hipMemcpyDeviceToHost, st));
     }
     if (any_norms) {

@@ -187,7 +187,8 @@ typedef struct {
                                 0 none, 1 Theta operator, 2 dual Rayleigh-Ritz Gram,
                                 3 Rayleigh-Ritz solve (Cholesky/SVQB + projection + Jacobi),
                                 4 Ritz-update linear combinations,
-                                5 residual and direction deflation */
+                                5 residual and direction deflation,
+                                6 k-point transitions (tracking, prediction, start block) */
     int engines;             /* engines run side by side, each with its share of the slots, its own
                                 streams and host thread (0 -> 1): the jobs of each grid size form one
                                 engine's share, the largest shares are split further to reach this count */
