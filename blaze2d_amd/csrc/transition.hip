@@ -37,20 +37,29 @@ inline __host__ __device__ size_t tr_take(size_t &o, size_t bytes) {
 
 // Byte offsets of the kernel's arrays in its workspace (LDS or per-item memory).
 struct TransLayout {
-    size_t mp, warm, ent, ment, tv, tmv, col, cb, A, V, rot, red, Wt, Ut, Up, fw, mag, hu, sig, ord, perm, iperm,
-        tperm, ci, lp, amb, blk, total;
+    size_t mp, warm, mc, ent, ment, gr, tv, tmv, col, cb, A, V, rot, red, Wt, Ut, Up, fw, mag, hu, sig, ord, perm,
+        iperm, tperm, ci, lp, amb, blk, rr, tk, dk, total;
+    int NC;  // candidates of the block: warm vectors, then start vectors
     __host__ __device__ TransLayout(int KD, int NBW, int M0, int MM) {
         size_t o = 0;
         const size_t z = 16;  // complex FP64
         const size_t ne = MM + (MM & 1);
+        NC = NBW + M0 + 2;
+        const int NE = NBW > M0 ? NBW : M0;  // entries of one Gram-Schmidt
         mp = tr_take(o, z * KD * KD);
-        warm = tr_take(o, z * NBW * KD);
-        ent = tr_take(o, z * M0 * KD);
-        ment = tr_take(o, z * M0 * KD);
+        warm = tr_take(o, z * NC * KD);
+        mc = tr_take(o, z * NC * KD);
+        // the candidates' Gram during the factorisation, then the block's entries and their images
+        const size_t gsz = z * NC * NC, esz = z * M0 * KD;
+        gr = tr_take(o, gsz > 2 * esz ? gsz : 2 * esz);
+        ent = gr;
+        ment = gr + esz;
         tv = tr_take(o, z * KD);
         tmv = tr_take(o, z * KD);
         col = tr_take(o, z * MM);
-        cb = tr_take(o, z * (M0 + MM));
+        cb = tr_take(o, z * NC);
+        // the polar and tracking arrays, free once the warm vectors are built
+        const size_t p0 = o;
         A = tr_take(o, z * ne * ne);
         V = tr_take(o, z * ne * ne);
         rot = tr_take(o, sizeof(JacobiRot) * ne);  // two rounds of ne / 2 rotations
@@ -60,6 +69,9 @@ struct TransLayout {
         Up = tr_take(o, z * MM * MM);
         fw = tr_take(o, 8 * MM * MM);
         mag = tr_take(o, 8 * MM * MM);
+        // the Gram-Schmidt coefficients reuse them when they fit
+        const size_t rsz = z * NE * NC;
+        rr = rsz <= o - p0 ? p0 : tr_take(o, rsz);
         hu = tr_take(o, 8 * (MM + 1));
         sig = tr_take(o, 8 * MM);
         ord = tr_take(o, 4 * MM);
@@ -70,6 +82,8 @@ struct TransLayout {
         lp = tr_take(o, 4 * MM);
         amb = tr_take(o, MM);
         blk = tr_take(o, 4 * 8 * (MM + 1));
+        tk = tr_take(o, 4 * NE);
+        dk = tr_take(o, 8 * NE);  // 1 / d_k of the entries
         total = o;
     }
 };
@@ -117,6 +131,32 @@ __device__ __forceinline__ double wave_min(double v) {
     return lane63_d(v);
 }
 __device__ __forceinline__ double wave_max(double v) { return -wave_min(-v); }
+// sums over groups of S adjacent lanes (S = 1, 2, 4, 8 or 16, uniform), in every lane of the group
+__device__ __forceinline__ void group_sum2(double &a, double &b, int S) {
+    if (S >= 2) {
+        a += dpp_d<0xb1>(0.0, a);   // quad_perm [1,0,3,2]: pairs
+        b += dpp_d<0xb1>(0.0, b);
+    }
+    if (S >= 4) {
+        a += dpp_d<0x4e>(0.0, a);   // quad_perm [2,3,0,1]: quads
+        b += dpp_d<0x4e>(0.0, b);
+    }
+    if (S >= 8) {
+        a += dpp_d<0x141>(0.0, a);  // row_half_mirror: eights
+        b += dpp_d<0x141>(0.0, b);
+    }
+    if (S >= 16) {
+        a += dpp_d<0x140>(0.0, a);  // row_mirror: sixteens
+        b += dpp_d<0x140>(0.0, b);
+    }
+}
+// lanes per item for n items of length len over one wave: the smallest power of
+// two S <= 16 with n * S >= 64 spread, so that short items use the whole wave
+__device__ __forceinline__ int lanes_per_item(int n, int len) {
+    int S = 1;
+    while (S < 16 && n * S * 2 <= TR_NT && len >= 2 * S) S *= 2;
+    return S;
+}
 
 // Maximum-weight assignment of an n x n weight matrix (w[i + j n] = weight of
 // row i to column j, n < 64) by the shortest-augmenting-path Hungarian method
@@ -330,6 +370,12 @@ __global__ void __launch_bounds__(TR_NT) transition_kernel(const TransWork *__re
     const TransLayout L(KD, NBW, M0, MM);
     double2 *const Mp = reinterpret_cast<double2 *>(base + L.mp);
     double2 *const warm = reinterpret_cast<double2 *>(base + L.warm);
+    double2 *const MC = reinterpret_cast<double2 *>(base + L.mc);
+    double2 *const Gc = reinterpret_cast<double2 *>(base + L.gr);
+    double2 *const Rr = reinterpret_cast<double2 *>(base + L.rr);
+    int *const tk = reinterpret_cast<int *>(base + L.tk);
+    double *const dk = reinterpret_cast<double *>(base + L.dk);
+    const int NC = L.NC;
     double2 *const ent = reinterpret_cast<double2 *>(base + L.ent);
     double2 *const ment = reinterpret_cast<double2 *>(base + L.ment);
     double2 *const tv = reinterpret_cast<double2 *>(base + L.tv);
@@ -359,7 +405,8 @@ __global__ void __launch_bounds__(TR_NT) transition_kernel(const TransWork *__re
     const int mt = min(nE, nP);        // order of the tracking overlap
     const int nbp = min(nE, w.nb);     // eigenvectors carried into the history
     const int mp2 = min(nbp, nP);      // order of the prediction overlap
-    const bool shape_ok = kd >= 1 && kd <= KD && kd <= 2 * TR_NT && kd <= w.ld && kd == iW + (nE == 0 ? w.nWc : 0) && mt <= MM &&
+    const bool shape_ok = kd >= 1 && kd <= KD && kd <= w.ld && kd == iW + (nE == 0 ? w.nWc : 0) && mt <= MM &&
+                          (w.m0 == 0 || nR <= w.m0 + 2) &&
                           mp2 <= MM && w.m0 >= 0 && w.m0 <= M0 && nbp <= NBW && w.nWc <= NBW && nE <= KD &&
                           nE >= 0 && nP >= 0 && nR >= 0 && w.nprev >= 0;
     if (!shape_ok) {
@@ -390,37 +437,206 @@ __global__ void __launch_bounds__(TR_NT) transition_kernel(const TransWork *__re
         wave_sum2(re, im);
         return make_double2(re, im);
     };
-    // out = M v, a row per lane (v complete in memory: a barrier precedes every call)
+    // out = M v (v complete in memory: a barrier precedes every call); a row
+    // per group of S adjacent lanes, each lane a segment of the row
     auto matvec = [&](const double2 *v, double2 *out) __attribute__((always_inline)) {
-        for (int i = tid; i < kd; i += TR_NT) {
+        const int S = lanes_per_item(kd, kd), R = TR_NT / S, L = (kd + S - 1) / S;
+        const int sg = tid % S, j0 = sg * L, j1 = min(kd, j0 + L);
+        for (int i0 = 0; i0 < kd; i0 += R) {
+            const int i = i0 + tid / S;
             double re = 0.0, im = 0.0;
-#pragma unroll 8
-            for (int j = 0; j < kd; ++j) {
-                const double2 g = Mp[i + j * KD], x = v[j];
-                re = fma(g.x, x.x, re);
-                re = fma(-g.y, x.y, re);
-                im = fma(g.x, x.y, im);
-                im = fma(g.y, x.x, im);
+            if (i < kd) {
+#pragma unroll 4
+                for (int j = j0; j < j1; ++j) {
+                    const double2 g = Mp[i + j * KD], x = v[j];
+                    re = fma(g.x, x.x, re);
+                    re = fma(-g.y, x.y, re);
+                    im = fma(g.x, x.y, im);
+                    im = fma(g.y, x.x, im);
+                }
             }
-            out[i] = make_double2(re, im);
+            group_sum2(re, im, S);
+            if (i < kd && sg == 0) out[i] = make_double2(re, im);
         }
+        __syncthreads();  // out is read by other lanes
     };
-    // cb[j] = a^H B_j for the nb vectors B_j = B + j KD, one per lane (a and B complete in memory)
+    // cb[j] = a^H B_j for the nb vectors B_j = B + j KD (a and B complete in
+    // memory): a vector per group of S adjacent lanes, each lane a segment
     auto dots = [&](const double2 *a, const double2 *B, int nb) __attribute__((always_inline)) {
-        for (int j = tid; j < nb; j += TR_NT) {
-            const double2 *bj = B + (size_t)j * KD;
+        if (nb <= 0) return;
+        const int S = lanes_per_item(nb, kd), R = TR_NT / S, L = (kd + S - 1) / S;
+        const int sg = tid % S, q0 = sg * L, q1 = min(kd, q0 + L);
+        for (int jb = 0; jb < nb; jb += R) {
+            const int j = jb + tid / S;
             double re = 0.0, im = 0.0;
-#pragma unroll 8
-            for (int q = 0; q < kd; ++q) {
-                const double2 x = a[q], y = bj[q];
-                re += x.x * y.x + x.y * y.y;
-                im += x.x * y.y - x.y * y.x;
+            if (j < nb) {
+                const double2 *bj = B + (size_t)j * KD;
+#pragma unroll 4
+                for (int q = q0; q < q1; ++q) {
+                    const double2 x = a[q], y = bj[q];
+                    re += x.x * y.x + x.y * y.y;
+                    im += x.x * y.y - x.y * y.x;
+                }
             }
-            cb[j] = make_double2(re, im);
+            group_sum2(re, im, S);
+            if (j < nb && sg == 0) cb[j] = make_double2(re, im);
         }
+        __syncthreads();  // cb is read by every lane
     };
     auto unit = [&](double2 *v, int idx) __attribute__((always_inline)) {
         for (int i = tid; i < kd; i += TR_NT) v[i] = make_double2(i == idx ? 1.0 : 0.0, 0.0);
+    };
+
+    // Gram-Schmidt in the B inner product of the nc candidates Cc (stride KD),
+    // in order, as the right-looking Cholesky factorisation of their Gram
+    // G = Cc^H M Cc: with the earlier entries projected out, candidate t has
+    // squared B-norm G_tt, and projecting entry k out of a later candidate u
+    // takes the coefficient R_ku = G_tu / d_k; G's trailing block then loses
+    // conj(R_ku) R_ku'.  In exact arithmetic this is the modified Gram-Schmidt
+    // of orthonormalize_fields and of initialize_block's project_out, with the
+    // same norms and thresholds; each step is a few parallel passes over small
+    // matrices instead of a chain of inner products.  A candidate whose norm
+    // is not above tol is rejected (reject), counting as an attempt from
+    // index nfree on (at most cap * 8), or kept unnormalised (!reject, the
+    // 1e-14 rule of orthonormalize_fields).  MC holds M c_t for the candidates
+    // (computed here unless mc_ready).  Entry k goes to E + k KD and M e_k to
+    // ME + k KD; E and ME may be Cc and MC themselves when nothing is
+    // rejected.  Returns the number of entries (at most cap).
+    auto orth = [&](double2 *Cc, int nc, int cap, double tol, bool reject, int nfree, bool mc_ready, double2 *E,
+                    double2 *ME) __attribute__((always_inline)) -> int {
+        if (!mc_ready) {
+            for (int e = tid; e < kd * nc; e += TR_NT) {  // MC = M Cc
+                const int i = e % kd, t = e / kd;
+                const double2 *v = Cc + (size_t)t * KD;
+                double re = 0.0, im = 0.0;
+                int j = 0;
+                for (; j + 4 <= kd; j += 4) {
+                    double2 g[4], x[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        g[u] = Mp[i + (j + u) * KD];
+                        x[u] = v[j + u];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        re = fma(g[u].x, x[u].x, re);
+                        re = fma(-g[u].y, x[u].y, re);
+                        im = fma(g[u].x, x[u].y, im);
+                        im = fma(g[u].y, x[u].x, im);
+                    }
+                }
+                for (; j < kd; ++j) {
+                    const double2 g = Mp[i + j * KD], x = v[j];
+                    re = fma(g.x, x.x, re);
+                    re = fma(-g.y, x.y, re);
+                    im = fma(g.x, x.y, im);
+                    im = fma(g.y, x.x, im);
+                }
+                MC[(size_t)t * KD + i] = make_double2(re, im);
+            }
+            __syncthreads();
+        }
+        for (int e = tid; e < nc * nc; e += TR_NT) {  // G = Cc^H (M Cc), Hermitian
+            const int t = e % nc, u = e / nc;
+            if (t > u) continue;
+            const double2 *a = Cc + (size_t)t * KD, *b = MC + (size_t)u * KD;
+            double re = 0.0, im = 0.0;
+            int q = 0;
+            for (; q + 4 <= kd; q += 4) {
+                double2 x[4], y[4];
+#pragma unroll
+                for (int w4 = 0; w4 < 4; ++w4) {
+                    x[w4] = a[q + w4];
+                    y[w4] = b[q + w4];
+                }
+#pragma unroll
+                for (int w4 = 0; w4 < 4; ++w4) {
+                    re += x[w4].x * y[w4].x + x[w4].y * y[w4].y;
+                    im += x[w4].x * y[w4].y - x[w4].y * y[w4].x;
+                }
+            }
+            for (; q < kd; ++q) {
+                const double2 x = a[q], y = b[q];
+                re += x.x * y.x + x.y * y.y;
+                im += x.x * y.y - x.y * y.x;
+            }
+            Gc[t + u * nc] = make_double2(re, im);
+            if (t != u) Gc[u + t * nc] = make_double2(re, -im);
+        }
+        __syncthreads();
+        int n = 0, attempts = 0;
+        for (int t = 0; t < nc && n < cap; ++t) {
+            if (t >= nfree && attempts >= cap * 8) break;
+            const double piv = Gc[t + t * nc].x;
+            const double d = sqrt(fmax(piv, 0.0));
+            double dd = d;
+            if (!(d > tol)) {
+                if (reject) {
+                    if (t >= nfree) ++attempts;
+                    continue;
+                }
+                dd = 1.0;
+            }
+            const double rd = 1.0 / dd;
+            const double f = 2.0 - piv * rd * rd;  // 1 for a normalised entry
+            // the coefficients R_nu = G_tu / d (kept for the entries) and the
+            // trailing update, each lane forming the ones it needs
+            double2 *rn = Rr + (size_t)n * NC;
+            const int nt = nc - t - 1;
+            for (int u = t + 1 + tid; u < nc; u += TR_NT) {
+                const double2 g = Gc[t + u * nc];
+                rn[u] = make_double2(g.x * rd, g.y * rd);
+            }
+            for (int e = tid; e < nt * nt; e += TR_NT) {
+                const int u = t + 1 + e % nt, v = t + 1 + e / nt;
+                const double2 gu = Gc[t + u * nc], gv = Gc[t + v * nc];
+                const double fr = f * rd * rd;
+                double2 &g = Gc[u + v * nc];
+                g.x -= fr * (gu.x * gv.x + gu.y * gv.y);
+                g.y -= fr * (gu.x * gv.y - gu.y * gv.x);
+            }
+            if (tid == 0) {
+                tk[n] = t;
+                dk[n] = rd;
+            }
+            __syncthreads();
+            ++n;
+        }
+        // e_k = (c_{t_k} - sum_{l<k} R_{l t_k} e_l) / d_k (and the same for M e_k),
+        // each lane its own elements; loads four terms ahead
+        auto solve = [&](const double2 *Src, double2 *Dst) __attribute__((always_inline)) {
+            for (int k = 0; k < n; ++k) {
+                const int t = tk[k];
+                const double rd = dk[k];  // 1 / d_k
+                for (int i = tid; i < kd; i += TR_NT) {
+                    double2 x = Src[(size_t)t * KD + i];
+                    int l = 0;
+                    for (; l + 4 <= k; l += 4) {
+                        double2 c[4], el[4];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            c[u] = Rr[(size_t)(l + u) * NC + t];
+                            el[u] = Dst[(size_t)(l + u) * KD + i];
+                        }
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            x.x -= c[u].x * el[u].x - c[u].y * el[u].y;
+                            x.y -= c[u].x * el[u].y + c[u].y * el[u].x;
+                        }
+                    }
+                    for (; l < k; ++l) {
+                        const double2 c = Rr[(size_t)l * NC + t], el = Dst[(size_t)l * KD + i];
+                        x.x -= c.x * el.x - c.y * el.y;
+                        x.y -= c.x * el.y + c.y * el.x;
+                    }
+                    Dst[(size_t)k * KD + i] = make_double2(x.x * rd, x.y * rd);
+                }
+            }
+        };
+        solve(MC, ME);
+        solve(Cc, E);
+        __syncthreads();
+        return n;
     };
 
     // Polar factor U of the m x m overlap O(i, j) = M(ro + i, cmap[j]) from
@@ -546,8 +762,7 @@ __global__ void __launch_bounds__(TR_NT) transition_kernel(const TransWork *__re
         // ---- warm vectors: the tracked eigenvectors or their prediction ----
         int nw = 0;
         if (nE > 0 && !(w.flags & TR_PREDICT)) {
-            for (int v = 0; v < nbp; ++v) unit(warm + (size_t)v * KD, perm[v]);
-            nw = nbp;
+            nw = nbp;  // the unit vectors e_perm[v], set with the start vectors below
         } else if (nE > 0) {
             // predict_with_rotation: O = prev^H B curr with curr the tracked E;
             // a full-rank tracking overlap's polar factor permuted by the
@@ -579,121 +794,30 @@ __global__ void __launch_bounds__(TR_NT) transition_kernel(const TransWork *__re
             if (extrap)
                 for (int j = tid; j < m; j += TR_NT) warm[(size_t)j * KD + iP + j] = make_double2(-alpha, 0.0);
             nw = m;
-            // modified Gram-Schmidt in the B inner product (orthonormalize_fields):
-            // with M w_i at hand, w_i^H M w_j = (M w_i)^H w_j
-            for (int i = 0; i < m; ++i) {
-                __syncthreads();
-                double2 *wi = warm + (size_t)i * KD;
-                matvec(wi, tmv);
-                const double nrm = sqrt(fmax(cdot(wi, tmv).x, 0.0));
-                if (nrm > 1e-14)
-                    for (int q = tid; q < kd; q += TR_NT) {
-                        wi[q] = make_double2(wi[q].x / nrm, wi[q].y / nrm);
-                        tmv[q] = make_double2(tmv[q].x / nrm, tmv[q].y / nrm);
-                    }
-                // the later vectors' coefficients are independent: one per lane
-                __syncthreads();
-                dots(tmv, wi + KD, m - 1 - i);
-                __syncthreads();
-                for (int q = tid; q < kd; q += TR_NT) {
-                    const double2 x = wi[q];
-#pragma unroll 4
-                    for (int j = i + 1; j < m; ++j) {
-                        const double2 c = cb[j - i - 1];
-                        double2 &y = warm[(size_t)j * KD + q];
-                        y.x -= c.x * x.x - c.y * x.y;
-                        y.y -= c.x * x.y + c.y * x.x;
-                    }
-                }
-            }
+            // orthonormalize_fields: Gram-Schmidt in the B inner product, in
+            // place, keeping M w beside each vector for the block below
+            __syncthreads();
+            orth(warm, m, m, 1e-14, false, m, false, warm, MC);
         }
         // the learned start's vectors at a job's first k-point
         if (nE == 0)
-            for (int j = 0; j < w.nWc; ++j) unit(warm + (size_t)(nw++) * KD, iW + j);
+            nw += w.nWc;  // the unit vectors e_{iW + j}, set with the start vectors below
 
-        // ---- initialize_block: warm vectors, then start vectors, B-orthonormal ----
-        // the candidate lives in registers (element tid + 64 r), so each
-        // projection is two loads and one wave reduction
-        constexpr int EPL = 2;  // elements per lane: kd <= 128
-        double2 tvr[EPL];
-        int nent = 0;
-        auto try_add = [&]() __attribute__((always_inline)) -> bool {
-            for (int k = 0; k < nent; ++k) {  // project_out: v -= <e, v>_B e
-                const double2 *me = ment + (size_t)k * KD, *e = ent + (size_t)k * KD;
-                double re = 0.0, im = 0.0;
-                double2 ev[EPL];
-#pragma unroll
-                for (int r = 0; r < EPL; ++r) {
-                    const int i = tid + TR_NT * r;
-                    ev[r] = make_double2(0.0, 0.0);
-                    if (i < kd) {
-                        const double2 x = me[i], y = tvr[r];
-                        ev[r] = e[i];
-                        re += x.x * y.x + x.y * y.y;
-                        im += x.x * y.y - x.y * y.x;
-                    }
-                }
-                wave_sum2(re, im);
-#pragma unroll
-                for (int r = 0; r < EPL; ++r) {
-                    tvr[r].x -= re * ev[r].x - im * ev[r].y;
-                    tvr[r].y -= re * ev[r].y + im * ev[r].x;
-                }
-            }
-            __syncthreads();  // every lane has read tv for the previous candidate
-#pragma unroll
-            for (int r = 0; r < EPL; ++r)
-                if (tid + TR_NT * r < kd) tv[tid + TR_NT * r] = tvr[r];
-            __syncthreads();
-            matvec(tv, tmv);
-            double re = 0.0, im = 0.0;
-#pragma unroll
-            for (int r = 0; r < EPL; ++r) {
-                const int i = tid + TR_NT * r;
-                if (i < kd) {
-                    const double2 x = tvr[r], y = tmv[i];
-                    re += x.x * y.x + x.y * y.y;
-                }
-            }
-            wave_sum2(re, im);
-            const double nrm = sqrt(fmax(re, 0.0));
-            if (!(nrm > 1e-12)) return false;
-            double2 *e = ent + (size_t)nent * KD, *me = ment + (size_t)nent * KD;
-#pragma unroll
-            for (int r = 0; r < EPL; ++r) {
-                const int i = tid + TR_NT * r;
-                if (i < kd) {
-                    e[i] = make_double2(tvr[r].x / nrm, tvr[r].y / nrm);
-                    me[i] = make_double2(tmv[i].x / nrm, tmv[i].y / nrm);
-                }
-            }
-            ++nent;
-            return true;
-        };
-        auto set_unit = [&](int idx) __attribute__((always_inline)) {
-#pragma unroll
-            for (int r = 0; r < EPL; ++r) tvr[r] = make_double2(tid + TR_NT * r == idx ? 1.0 : 0.0, 0.0);
-        };
+        // ---- initialize_block: the warm vectors, then the start vectors
+        // e_{iR + r} while the block is short, each B-orthonormalised against
+        // the entries before it and rejected when its norm is not above 1e-12 ----
+        // M c for the candidates: from the Gram-Schmidt above for the
+        // predicted vectors, a column of M for a unit vector
+        const int nu0 = (nE > 0 && (w.flags & TR_PREDICT)) ? nw : 0;
+        for (int e = tid; e < kd * (nw + nR - nu0); e += TR_NT) {
+            const int i = e % kd, t = nu0 + e / kd;
+            const int idx = t < nw ? (nE > 0 ? perm[t] : iW + t) : iR + (t - nw);
+            warm[(size_t)t * KD + i] = make_double2(i == idx ? 1.0 : 0.0, 0.0);
+            MC[(size_t)t * KD + i] = Mp[i + idx * KD];
+        }
         __syncthreads();
-        for (int v = 0; v < nw && nent < w.m0; ++v) {
-#pragma unroll
-            for (int r = 0; r < EPL; ++r) {
-                const int i = tid + TR_NT * r;
-                tvr[r] = i < kd ? warm[(size_t)v * KD + i] : make_double2(0.0, 0.0);
-            }
-            try_add();
-        }
-        int attempts = 0, r = 0;
-        bool ran_out = false;
-        while (nent < w.m0 && attempts < w.m0 * 8) {
-            if (r >= nR) {
-                ran_out = true;
-                break;
-            }
-            set_unit(iR + r);
-            ++r;
-            if (!try_add()) ++attempts;
-        }
+        const int nent = orth(warm, nw + nR, w.m0, 1e-12, true, nw, true, ent, ment);
+        const bool ran_out = nent < w.m0;
         if (w.flags & TR_GAMMA) {
             // add_vector(y0), then project each entry against it and renormalise
             __syncthreads();
