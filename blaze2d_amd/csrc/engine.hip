@@ -364,7 +364,7 @@ class Engine : public EngineBase {
         d_tout = dalloc<int>((size_t)S * TOUT, st);
         d_relf = dalloc<double>((size_t)S * LD, st);
         if (wide) {  // the transition kernel's workspace does not fit in LDS
-            tbytes = (transition_bytes(LD, nbmax, m0max, nbmax) + 255) & ~size_t(255);
+            tbytes = (transition_bytes(LD, nbmax, m0max, nbmax, nbmax + m0max + 2) + 255) & ~size_t(255);
             d_tscr = dalloc<char>((size_t)S * tbytes, st);
         }
         BH_HIP_CHECK(hipHostMalloc((void **)&h_lam, sizeof(double) * S * LD, hipHostMallocDefault));
@@ -1736,16 +1736,19 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
                            (long)LD * LD * 2, st);
         // the workspace is laid out for this launch's largest item
         double bytes = 0;  // the Gram in, block and history coefficients out
-        int KDl = 1, NBWl = 1, M0l = 1, MMl = 1;
+        int KDl = 1, NBWl = 1, M0l = 1, MMl = 1, NCl = 1;
         for (const TransWork &t : tw) {
             bytes += 16.0 * ((double)t.kd * t.kd + (double)t.kd * t.m0 + (double)t.nE * t.nE);
+            const int nw = std::max(std::min(t.nE, t.nb), t.nWc);
             KDl = std::max(KDl, t.kd);
-            NBWl = std::max(NBWl, std::max(std::min(t.nE, t.nb), t.nWc));
+            NBWl = std::max(NBWl, nw);
             M0l = std::max(M0l, t.m0);
             MMl = std::max(MMl, std::min(t.nE, t.nP));
+            NCl = std::max(NCl, std::max(nw + t.nR, t.m0));
         }
-        timed(TIMED_TRANS, bytes, 0.0,
-              [&] { launch_transition(d_tw, (int)tw.size(), KDl, NBWl, M0l, MMl, d_tscr != nullptr, st); });
+        timed(TIMED_TRANS, bytes, 0.0, [&] {
+            launch_transition(d_tw, (int)tw.size(), KDl, NBWl, M0l, MMl, NCl, d_tscr != nullptr, st);
+        });
     }
     launch_lincomb<T>(d_lc_hist, (int)lc_hist.size(), N, nbmax, st);
     launch_lincomb<T>(d_lc_x, (int)lc_x.size(), N, std::max(maxq_x, 1), st, LD);

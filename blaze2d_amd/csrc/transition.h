@@ -34,10 +34,11 @@ struct TransWork {
 };
 
 // Capacities of one launch: KD coefficient rows, NBW warm vectors, M0 block
-// vectors, MM order of the polar problems (at most 32).
-size_t transition_bytes(int KD, int NBW, int M0, int MM);
+// vectors, MM order of the polar problems (at most 32), NC candidates of the
+// block (warm vectors and start vectors).
+size_t transition_bytes(int KD, int NBW, int M0, int MM, int NC);
 // scratch: items bring their own workspace (else dynamic LDS of transition_bytes)
-void launch_transition(const TransWork *dwork, int nwork, int KD, int NBW, int M0, int MM, bool scratch,
+void launch_transition(const TransWork *dwork, int nwork, int KD, int NBW, int M0, int MM, int NC, bool scratch,
                        hipStream_t st);
 
 }  // namespace bh

@@ -37,23 +37,19 @@ inline __host__ __device__ size_t tr_take(size_t &o, size_t bytes) {
 
 // Byte offsets of the kernel's arrays in its workspace (LDS or per-item memory).
 struct TransLayout {
-    size_t mp, warm, mc, ent, ment, gr, tv, tmv, col, cb, A, V, rot, red, Wt, Ut, Up, fw, mag, hu, sig, ord, perm,
-        iperm, tperm, ci, lp, amb, blk, rr, tk, dk, total;
-    int NC;  // candidates of the block: warm vectors, then start vectors
-    __host__ __device__ TransLayout(int KD, int NBW, int M0, int MM) {
+    size_t mp, warm, mc, gr, tv, tmv, col, cb, A, V, rot, red, Wt, Ut, Up, fw, mag, hu, sig, ord, perm, iperm,
+        tperm, ci, lp, amb, blk, rr, tk, dk, total;
+    __host__ __device__ TransLayout(int KD, int NBW, int M0, int MM, int NC) {
         size_t o = 0;
         const size_t z = 16;  // complex FP64
         const size_t ne = MM + (MM & 1);
-        NC = NBW + M0 + 2;
         const int NE = NBW > M0 ? NBW : M0;  // entries of one Gram-Schmidt
         mp = tr_take(o, z * KD * KD);
+        // candidates and their images M c; the block's entries and their
+        // images replace them in place
         warm = tr_take(o, z * NC * KD);
         mc = tr_take(o, z * NC * KD);
-        // the candidates' Gram during the factorisation, then the block's entries and their images
-        const size_t gsz = z * NC * NC, esz = z * M0 * KD;
-        gr = tr_take(o, gsz > 2 * esz ? gsz : 2 * esz);
-        ent = gr;
-        ment = gr + esz;
+        gr = tr_take(o, z * NC * NC);  // the candidates' Gram
         tv = tr_take(o, z * KD);
         tmv = tr_take(o, z * KD);
         col = tr_take(o, z * MM);
@@ -363,11 +359,11 @@ __device__ __forceinline__ void track_assign(int m, const double2 *U, const doub
 // a template parameter, so that LDS accesses compile to ds_* instructions
 template <bool SCR>
 __global__ void __launch_bounds__(TR_NT) transition_kernel(const TransWork *__restrict__ work, int KD, int NBW,
-                                                           int M0, int MM) {
+                                                           int M0, int MM, int NC) {
     const TransWork &w = work[blockIdx.x];
     extern __shared__ __attribute__((aligned(16))) double tsm[];
     char *const base = SCR ? w.scratch : reinterpret_cast<char *>(tsm);
-    const TransLayout L(KD, NBW, M0, MM);
+    const TransLayout L(KD, NBW, M0, MM, NC);
     double2 *const Mp = reinterpret_cast<double2 *>(base + L.mp);
     double2 *const warm = reinterpret_cast<double2 *>(base + L.warm);
     double2 *const MC = reinterpret_cast<double2 *>(base + L.mc);
@@ -375,9 +371,8 @@ __global__ void __launch_bounds__(TR_NT) transition_kernel(const TransWork *__re
     double2 *const Rr = reinterpret_cast<double2 *>(base + L.rr);
     int *const tk = reinterpret_cast<int *>(base + L.tk);
     double *const dk = reinterpret_cast<double *>(base + L.dk);
-    const int NC = L.NC;
-    double2 *const ent = reinterpret_cast<double2 *>(base + L.ent);
-    double2 *const ment = reinterpret_cast<double2 *>(base + L.ment);
+    double2 *const ent = warm;  // the block's entries replace the candidates
+    double2 *const ment = MC;
     double2 *const tv = reinterpret_cast<double2 *>(base + L.tv);
     double2 *const tmv = reinterpret_cast<double2 *>(base + L.tmv);
     double2 *const col = reinterpret_cast<double2 *>(base + L.col);
@@ -406,7 +401,7 @@ __global__ void __launch_bounds__(TR_NT) transition_kernel(const TransWork *__re
     const int nbp = min(nE, w.nb);     // eigenvectors carried into the history
     const int mp2 = min(nbp, nP);      // order of the prediction overlap
     const bool shape_ok = kd >= 1 && kd <= KD && kd <= w.ld && kd == iW + (nE == 0 ? w.nWc : 0) && mt <= MM &&
-                          (w.m0 == 0 || nR <= w.m0 + 2) &&
+                          (w.m0 == 0 || max(nbp, w.nWc) + nR <= NC) &&
                           mp2 <= MM && w.m0 >= 0 && w.m0 <= M0 && nbp <= NBW && w.nWc <= NBW && nE <= KD &&
                           nE >= 0 && nP >= 0 && nR >= 0 && w.nprev >= 0;
     if (!shape_ok) {
@@ -895,15 +890,17 @@ __global__ void __launch_bounds__(TR_NT) transition_kernel(const TransWork *__re
 
 }  // namespace
 
-size_t transition_bytes(int KD, int NBW, int M0, int MM) { return TransLayout(KD, NBW, M0, MM).total; }
+size_t transition_bytes(int KD, int NBW, int M0, int MM, int NC) {
+    return TransLayout(KD, NBW, M0, MM, NC).total;
+}
 
-void launch_transition(const TransWork *dwork, int nwork, int KD, int NBW, int M0, int MM, bool scratch,
+void launch_transition(const TransWork *dwork, int nwork, int KD, int NBW, int M0, int MM, int NC, bool scratch,
                        hipStream_t st) {
     if (nwork <= 0) return;
     if (MM > 32) throw std::runtime_error("transition: polar order above 32");
     // in LDS: up to 48 coefficient rows, 16 bands (the one-workgroup subspace path)
-    static const size_t lds_max = transition_bytes(48, 16, 18, 16);
-    const size_t sh = scratch ? 0 : transition_bytes(KD, NBW, M0, MM);
+    static const size_t lds_max = transition_bytes(48, 16, 18, 16, 16 + 18 + 2);
+    const size_t sh = scratch ? 0 : transition_bytes(KD, NBW, M0, MM, NC);
     if (sh > lds_max) throw std::runtime_error("transition: workspace exceeds the LDS allowance");
     static const bool attr_set = [] {
         BH_HIP_CHECK(hipFuncSetAttribute((const void *)transition_kernel<false>,
@@ -912,9 +909,9 @@ void launch_transition(const TransWork *dwork, int nwork, int KD, int NBW, int M
     }();
     (void)attr_set;
     if (scratch)
-        hipLaunchKernelGGL(transition_kernel<true>, dim3(nwork), dim3(TR_NT), 0, st, dwork, KD, NBW, M0, MM);
+        hipLaunchKernelGGL(transition_kernel<true>, dim3(nwork), dim3(TR_NT), 0, st, dwork, KD, NBW, M0, MM, NC);
     else
-        hipLaunchKernelGGL(transition_kernel<false>, dim3(nwork), dim3(TR_NT), sh, st, dwork, KD, NBW, M0, MM);
+        hipLaunchKernelGGL(transition_kernel<false>, dim3(nwork), dim3(TR_NT), sh, st, dwork, KD, NBW, M0, MM, NC);
     BH_HIP_CHECK(hipGetLastError());
 }
 
