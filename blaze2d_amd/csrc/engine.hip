@@ -185,7 +185,7 @@ struct Group {
     hipStream_t st{};
     hipEvent_t ev{};
     Staging stg;
-    double *part = nullptr, *gramrb = nullptr;  // Gram partial sums; reduced transition Grams
+    double *part = nullptr;  // Gram partial sums
     double *h_rel = nullptr;  // readback of the group's residual norms
     double *h_prec = nullptr; // readback of the first-iteration preconditioned residual norms
     double *rrws = nullptr;   // wide Rayleigh-Ritz workspace, one block per slot of the group
@@ -388,7 +388,6 @@ class Engine : public EngineBase {
             // partial sums: two products (dual Gram) of up to 2048 chunk blocks
             G.part = dalloc<double>((size_t)2 * (chunk_budget + 2 * gs) * LD * LD * 2, st);
             if (wide) G.rrws = dalloc<double>((size_t)gs * rr_wide_bytes(C) / sizeof(double), st);
-            G.gramrb = dalloc<double>((size_t)gs * LD * LD * 2, st);
             BH_HIP_CHECK(hipHostMalloc((void **)&G.h_rel, sizeof(double) * gs * LD, hipHostMallocDefault));
             BH_HIP_CHECK(hipHostMalloc((void **)&G.h_prec, sizeof(double) * gs * LD, hipHostMallocDefault));
             if (d_pw)
@@ -437,7 +436,6 @@ class Engine : public EngineBase {
             (void)hipStreamSynchronize(G.st);
             (void)hipFree(G.part);
             if (G.rrws) (void)hipFree(G.rrws);
-            (void)hipFree(G.gramrb);
             (void)hipFree(G.sweeps);
             (void)hipHostFree(G.h_rel);
             (void)hipHostFree(G.h_prec);
@@ -1071,7 +1069,6 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
     const hipStream_t st = G.st;
     G.harvest(&stats_->timed_ms);  // this group's previous step has completed
     double *const d_part = G.part;
-    double *const d_gramrb = G.gramrb;
     int *const d_sweeps = G.sweeps;
     stg.off = 0;
     const bool need_b_eps = true;
@@ -1171,7 +1168,7 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
     // the new block X = K cx, its A X and Rayleigh quotients ----
     std::vector<GramWork<T>> g_t1;
     std::vector<TransWork> tw;
-    std::vector<LincombWork<T>> lc_hist, lc_x;
+    std::vector<LincombWork<T>> lc_x;  // history copies and new blocks, one launch
     std::vector<OpWork<T>> op_x[2];
     std::vector<RayleighWork<T>> rq;
     int maxq_x = 0, maxkd = 0;
@@ -1198,8 +1195,7 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
         g.nb = kd;
         g.ld = LD;
         g.lower_only = 1;
-        TransWork t{};
-        t.gram = d_gramrb + g_t1.size() * (size_t)LD * LD * 2;
+        TransWork t{};  // t.gram and t.nchunks: the Gram's partial sums, assigned below
         g_t1.push_back(g);
         double *om_d = nullptr, *pv_d = nullptr;
         if (!sl.t_om.empty()) {
@@ -1244,27 +1240,22 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
             w.out = hcol(s, hn, 0);
             w.out_stride = N;
             w.mode = 0;
-            lc_hist.push_back(w);
+            lc_x.push_back(w);
         }
-        // X = [H_new | P | R | y0 | Wc] cx
-        const cplx<T> **xp_d;
-        const cplx<T> **xp = stg.alloc<const cplx<T> *>(kd, (const cplx<T> ***)&xp_d);
-        q = 0;
-        for (int i = 0; i < nE; ++i) xp[q++] = hcol(s, hn, i);
-        for (int i = 0; i < nP; ++i) xp[q++] = hcol(s, sl.t_hist_old, i);
-        for (int i = 0; i < nR; ++i) xp[q++] = rcol(i);
-        xp[q++] = vcol(s, C);
-        for (int i = 0; i < sl.t_nwc; ++i) xp[q++] = hcol(s, 1, i);
+        // X = K cx into the slot's other V copy (K's columns stay untouched
+        // while the history copy reads E in the same launch)
+        const int nxt = 1 - sl.par;
         LincombWork<T> w{};
-        w.a_ptrs = xp_d;
+        w.a_ptrs = kp_d;
         w.na = kd;
         w.nq = m0;
         w.C = t.cx;
         w.ldc = kd;
-        w.out = vcol(s, 0);
+        w.out = vbuf(s, nxt, 0);
         w.out_stride = N;
         lc_x.push_back(w);
-        maxq_x = std::max(maxq_x, m0);
+        maxq_x = std::max(maxq_x, std::max(m0, nE));
+        sl.par = nxt;
         OpWork<T> o{};
         o.in = vcol(s, 0);
         o.out = avcol(s, 0);
@@ -1544,8 +1535,13 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
     auto assign_partials = [&](std::vector<GramWork<T>> &v, int nch) {
         for (size_t i = 0; i < v.size(); ++i) v[i].partial = d_part + i * (size_t)nch * LD * LD * 2;
     };
-    const int nch_dual = nch_for(g_dual.size()), nch_t1 = nch_for(g_t1.size());
+    // the transition kernel sums the transition Gram's partials itself: few chunks
+    const int nch_dual = nch_for(g_dual.size()), nch_t1 = std::min(8, nch_for(g_t1.size()));
     assign_partials(g_t1, nch_t1);
+    for (size_t i = 0; i < g_t1.size(); ++i) {  // the transition kernel sums the Gram's partials itself
+        tw[i].gram = g_t1[i].partial;
+        tw[i].nchunks = nch_t1;
+    }
     for (size_t i = 0; i < g_dual.size(); ++i) {  // two products per item
         g_dual[i].partial1 = d_part + 2 * i * (size_t)nch_dual * LD * LD * 2;
         g_dual[i].partial2 = g_dual[i].partial1 + (size_t)nch_dual * LD * LD * 2;
@@ -1656,7 +1652,6 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
     auto *d_res_fin = put(res_fin);
     auto *d_def_fin = put(def_fin);
     auto *d_tw = put(tw);
-    auto *d_lc_hist = put(lc_hist);
     auto *d_lc_x = put(lc_x);
     OpWork<T> *d_op_x[2] = {put(op_x[0]), put(op_x[1])};
     auto *d_rq = put(rq);
@@ -1694,7 +1689,7 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
     };
     for (auto &tc : tcopies) stats_->alg_bytes += 2.0 * sizeof(T) * tc.n;
     stats_->alg_bytes += 2.0 * colb * plan.fill_y0.size();
-    stats_->alg_bytes += lc_bytes(lc_hist) + lc_bytes(lc_x);
+    stats_->alg_bytes += lc_bytes(lc_x);
     for (auto &r : rq) stats_->alg_bytes += r.count * 2.0 * colb + (r.bw ? wb : 0.0);
     for (auto &r : res) stats_->alg_bytes += r.count * 3.0 * colb + (r.bw ? wb : 0.0);
     for (auto &r : res_fin) stats_->alg_bytes += r.count * 3.0 * colb + (r.bw ? wb : 0.0);
@@ -1732,8 +1727,6 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
     // history copy (reads E), X = K cx (overwrites E's copy)
     if (!g_t1.empty()) {
         launch_gram<T>(d_g_t1, (int)g_t1.size(), nch_t1, N, maxkd, maxkd, st, eopt.gram_mfma_min);
-        launch_gram_reduce(d_part, d_gramrb, (int)g_t1.size(), nch_t1, LD, (long)nch_t1 * LD * LD * 2,
-                           (long)LD * LD * 2, st);
         // the workspace is laid out for this launch's largest item
         double bytes = 0;  // the Gram in, block and history coefficients out
         int KDl = 1, NBWl = 1, M0l = 1, MMl = 1, NCl = 1;
@@ -1750,7 +1743,6 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
             launch_transition(d_tw, (int)tw.size(), KDl, NBWl, M0l, MMl, NCl, d_tscr != nullptr, st);
         });
     }
-    launch_lincomb<T>(d_lc_hist, (int)lc_hist.size(), N, nbmax, st);
     launch_lincomb<T>(d_lc_x, (int)lc_x.size(), N, std::max(maxq_x, 1), st, LD);
     for (int pol = 0; pol < 2; ++pol) {
         if (op_x[pol].empty()) continue;

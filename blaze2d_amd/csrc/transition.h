@@ -19,16 +19,17 @@ enum TransFlags {
 // eigenvalue), P (nP vectors of the current history), R (nR start vectors),
 // y0 (the Gamma mode), Wc (nWc learned-start vectors, first k-point only).
 struct TransWork {
-    const double *gram;     // M(i, j) for i >= j at 2 (i + j ld) (complex interleaved)
+    const double *gram;     // Gram partial sums: M(i, j), i >= j, is the sum over c < nchunks of the
+                            // complex at 2 (c ld ld + i ld + j)
     const double *om;       // nE frequencies of the finished solve (ascending)
     const double *prev_om;  // nprev tracked frequencies of the previous k-point
-    double *cx;             // out: kd x m0 block coefficients over [H_new | P | R | y0 | Wc], ld kd
+    double *cx;             // out: kd x m0 block coefficients over K, ld kd
     double *hist;           // out: nE x nE history-copy coefficients (H_new_i = E_perm[i]), ld nE
     int *out;               // out: [0] block vectors built (-1 start vectors ran out, -2 shape error),
                             //      [1] 1 when the tracking overlap had full rank, [2, 2 + nE) tracking permutation
     char *scratch;          // per-item workspace of transition_bytes() (null: dynamic LDS)
     double alpha;           // extrapolation factor |dk_next| / |dk_prev|
-    int ld, kd, nE, nP, nR, nWc, nb;
+    int ld, nchunks, kd, nE, nP, nR, nWc, nb;
     int m0;                 // block vectors to build (0: tracking only)
     int nprev, flags;
 };

@@ -37,8 +37,8 @@ inline __host__ __device__ size_t tr_take(size_t &o, size_t bytes) {
 
 // Byte offsets of the kernel's arrays in its workspace (LDS or per-item memory).
 struct TransLayout {
-    size_t mp, warm, mc, gr, tv, tmv, col, cb, A, V, rot, red, Wt, Ut, Up, fw, mag, hu, sig, ord, perm, iperm,
-        tperm, ci, lp, amb, blk, rr, tk, dk, total;
+    size_t mp, warm, mc, gr, tv, tmv, col, cb, A, V, rot, red, Wt, Ut, Up, fw, mag, hu, sig, ord, perm, tperm, ci,
+        lp, amb, blk, rr, tk, dk, total;
     __host__ __device__ TransLayout(int KD, int NBW, int M0, int MM, int NC) {
         size_t o = 0;
         const size_t z = 16;  // complex FP64
@@ -72,7 +72,6 @@ struct TransLayout {
         sig = tr_take(o, 8 * MM);
         ord = tr_take(o, 4 * MM);
         perm = tr_take(o, 4 * KD);
-        iperm = tr_take(o, 4 * KD);
         tperm = tr_take(o, 4 * MM);
         ci = tr_take(o, 4 * MM);
         lp = tr_take(o, 4 * MM);
@@ -243,7 +242,8 @@ struct TrackScratch {
 __device__ __forceinline__ void track_assign(int m, const double2 *U, const double *pw, int npw, const double *cw, int ncw, int *perm,
                              const TrackScratch &s) {
     const int lane = threadIdx.x;
-    for (int e = lane; e < m * m; e += TR_NT) s.mag[e] = hypot(U[e].x, U[e].y);
+    // |U| <= 1: no scaling needed
+    for (int e = lane; e < m * m; e += TR_NT) s.mag[e] = sqrt(U[e].x * U[e].x + U[e].y * U[e].y);
     __syncthreads();
     // detect_degenerate_bands_from_rotation: the ambiguous rows
     if (m >= 2 && lane < m) {
@@ -386,7 +386,6 @@ __global__ void __launch_bounds__(TR_NT) transition_kernel(const TransWork *__re
     double *const sig = reinterpret_cast<double *>(base + L.sig);
     int *const ord = reinterpret_cast<int *>(base + L.ord);
     int *const perm = reinterpret_cast<int *>(base + L.perm);
-    int *const iperm = reinterpret_cast<int *>(base + L.iperm);
     int *const tperm = reinterpret_cast<int *>(base + L.tperm);
     const TrackScratch ts{reinterpret_cast<double *>(base + L.mag), reinterpret_cast<double *>(base + L.fw),
                           reinterpret_cast<double *>(base + L.hu),  reinterpret_cast<int *>(base + L.ci),
@@ -409,12 +408,18 @@ __global__ void __launch_bounds__(TR_NT) transition_kernel(const TransWork *__re
         return;
     }
 
-    // M (col-major, leading dimension KD) from the lower triangle, diagonal real
+    // M (col-major, leading dimension KD) from the lower triangle of the
+    // Gram's row-chunk partial sums (summed in chunk order), diagonal real
     for (int e = tid; e < kd * kd; e += TR_NT) {
         const int i = e % kd, j = e / kd;
         if (i < j) continue;
-        const __attribute__((address_space(1))) double *g = gmem(w.gram) + 2 * ((size_t)i + (size_t)j * w.ld);
-        const double gx = g[0], gy = i == j ? 0.0 : g[1];
+        const __attribute__((address_space(1))) double *g = gmem(w.gram) + 2 * ((size_t)i * w.ld + j);
+        double gx = 0.0, gy = 0.0;
+        for (int c = 0; c < w.nchunks; ++c) {
+            gx += g[2 * (size_t)c * w.ld * w.ld];
+            gy += g[2 * (size_t)c * w.ld * w.ld + 1];
+        }
+        if (i == j) gy = 0.0;
         Mp[i + j * KD] = make_double2(gx, gy);
         Mp[j + i * KD] = make_double2(gx, -gy);
     }
@@ -494,13 +499,13 @@ __global__ void __launch_bounds__(TR_NT) transition_kernel(const TransWork *__re
     // is not above tol is rejected (reject), counting as an attempt from
     // index nfree on (at most cap * 8), or kept unnormalised (!reject, the
     // 1e-14 rule of orthonormalize_fields).  MC holds M c_t for the candidates
-    // (computed here unless mc_ready).  Entry k goes to E + k KD and M e_k to
+    // (computed here for the first ndense).  Entry k goes to E + k KD and M e_k to
     // ME + k KD; E and ME may be Cc and MC themselves when nothing is
     // rejected.  Returns the number of entries (at most cap).
-    auto orth = [&](double2 *Cc, int nc, int cap, double tol, bool reject, int nfree, bool mc_ready, double2 *E,
+    auto orth = [&](double2 *Cc, int nc, int ndense, int cap, double tol, bool reject, int nfree, double2 *E,
                     double2 *ME) __attribute__((always_inline)) -> int {
-        if (!mc_ready) {
-            for (int e = tid; e < kd * nc; e += TR_NT) {  // MC = M Cc
+        if (ndense > 0) {
+            for (int e = tid; e < kd * ndense; e += TR_NT) {  // MC = M Cc for the leading dense candidates
                 const int i = e % kd, t = e / kd;
                 const double2 *v = Cc + (size_t)t * KD;
                 double re = 0.0, im = 0.0;
@@ -597,39 +602,41 @@ __global__ void __launch_bounds__(TR_NT) transition_kernel(const TransWork *__re
             __syncthreads();
             ++n;
         }
-        // e_k = (c_{t_k} - sum_{l<k} R_{l t_k} e_l) / d_k (and the same for M e_k),
-        // each lane its own elements; loads four terms ahead
-        auto solve = [&](const double2 *Src, double2 *Dst) __attribute__((always_inline)) {
-            for (int k = 0; k < n; ++k) {
-                const int t = tk[k];
-                const double rd = dk[k];  // 1 / d_k
-                for (int i = tid; i < kd; i += TR_NT) {
-                    double2 x = Src[(size_t)t * KD + i];
-                    int l = 0;
-                    for (; l + 4 <= k; l += 4) {
-                        double2 c[4], el[4];
+        // e_k = (c_{t_k} - sum_{l<k} R_{l t_k} e_l) / d_k and the same for M e_k,
+        // each lane its own elements, both in one pass
+        for (int k = 0; k < n; ++k) {
+            const int t = tk[k];
+            const double rd = dk[k];  // 1 / d_k
+            for (int i = tid; i < kd; i += TR_NT) {
+                double2 x = Cc[(size_t)t * KD + i], mx = MC[(size_t)t * KD + i];
+                int l = 0;
+                for (; l + 4 <= k; l += 4) {
+                    double2 c[4], el[4], ml[4];
 #pragma unroll
-                        for (int u = 0; u < 4; ++u) {
-                            c[u] = Rr[(size_t)(l + u) * NC + t];
-                            el[u] = Dst[(size_t)(l + u) * KD + i];
-                        }
+                    for (int u = 0; u < 4; ++u) {
+                        c[u] = Rr[(size_t)(l + u) * NC + t];
+                        el[u] = E[(size_t)(l + u) * KD + i];
+                        ml[u] = ME[(size_t)(l + u) * KD + i];
+                    }
 #pragma unroll
-                        for (int u = 0; u < 4; ++u) {
-                            x.x -= c[u].x * el[u].x - c[u].y * el[u].y;
-                            x.y -= c[u].x * el[u].y + c[u].y * el[u].x;
-                        }
+                    for (int u = 0; u < 4; ++u) {
+                        x.x -= c[u].x * el[u].x - c[u].y * el[u].y;
+                        x.y -= c[u].x * el[u].y + c[u].y * el[u].x;
+                        mx.x -= c[u].x * ml[u].x - c[u].y * ml[u].y;
+                        mx.y -= c[u].x * ml[u].y + c[u].y * ml[u].x;
                     }
-                    for (; l < k; ++l) {
-                        const double2 c = Rr[(size_t)l * NC + t], el = Dst[(size_t)l * KD + i];
-                        x.x -= c.x * el.x - c.y * el.y;
-                        x.y -= c.x * el.y + c.y * el.x;
-                    }
-                    Dst[(size_t)k * KD + i] = make_double2(x.x * rd, x.y * rd);
                 }
+                for (; l < k; ++l) {
+                    const double2 c = Rr[(size_t)l * NC + t], el = E[(size_t)l * KD + i], ml = ME[(size_t)l * KD + i];
+                    x.x -= c.x * el.x - c.y * el.y;
+                    x.y -= c.x * el.y + c.y * el.x;
+                    mx.x -= c.x * ml.x - c.y * ml.y;
+                    mx.y -= c.x * ml.y + c.y * ml.x;
+                }
+                E[(size_t)k * KD + i] = make_double2(x.x * rd, x.y * rd);
+                ME[(size_t)k * KD + i] = make_double2(mx.x * rd, mx.y * rd);
             }
-        };
-        solve(MC, ME);
-        solve(Cc, E);
+        }
         __syncthreads();
         return n;
     };
@@ -789,10 +796,6 @@ __global__ void __launch_bounds__(TR_NT) transition_kernel(const TransWork *__re
             if (extrap)
                 for (int j = tid; j < m; j += TR_NT) warm[(size_t)j * KD + iP + j] = make_double2(-alpha, 0.0);
             nw = m;
-            // orthonormalize_fields: Gram-Schmidt in the B inner product, in
-            // place, keeping M w beside each vector for the block below
-            __syncthreads();
-            orth(warm, m, m, 1e-14, false, m, false, warm, MC);
         }
         // the learned start's vectors at a job's first k-point
         if (nE == 0)
@@ -801,8 +804,13 @@ __global__ void __launch_bounds__(TR_NT) transition_kernel(const TransWork *__re
         // ---- initialize_block: the warm vectors, then the start vectors
         // e_{iR + r} while the block is short, each B-orthonormalised against
         // the entries before it and rejected when its norm is not above 1e-12 ----
-        // M c for the candidates: from the Gram-Schmidt above for the
-        // predicted vectors, a column of M for a unit vector
+        // orthonormalize_fields of the predicted vectors followed by
+        // initialize_block over them is one Gram-Schmidt pass over the
+        // predicted vectors and then the start vectors: the first pass's
+        // output is already B-orthonormal, so in exact arithmetic the second
+        // keeps it and projects the start vectors against the same span.
+        // M c for the candidates: computed for the predicted vectors, a
+        // column of M for a unit vector
         const int nu0 = (nE > 0 && (w.flags & TR_PREDICT)) ? nw : 0;
         for (int e = tid; e < kd * (nw + nR - nu0); e += TR_NT) {
             const int i = e % kd, t = nu0 + e / kd;
@@ -811,7 +819,7 @@ __global__ void __launch_bounds__(TR_NT) transition_kernel(const TransWork *__re
             MC[(size_t)t * KD + i] = Mp[i + idx * KD];
         }
         __syncthreads();
-        const int nent = orth(warm, nw + nR, w.m0, 1e-12, true, nw, true, ent, ment);
+        const int nent = orth(warm, nw + nR, nu0, w.m0, 1e-12, true, nw, ent, ment);
         const bool ran_out = nent < w.m0;
         if (w.flags & TR_GAMMA) {
             // add_vector(y0), then project each entry against it and renormalise
@@ -860,17 +868,12 @@ __global__ void __launch_bounds__(TR_NT) transition_kernel(const TransWork *__re
                     }
                 }
         }
-        // coefficients over the physical basis [H_new (tracked E) | P | R | y0 | Wc]
-        for (int i = tid; i < nE; i += TR_NT) iperm[i] = i;
-        __syncthreads();
-        for (int i = tid; i < nE; i += TR_NT) iperm[perm[i]] = i;
-        __syncthreads();
+        // the block's coefficients over K
         for (int e = 0; e < nent; ++e)
             for (int q = tid; q < kd; q += TR_NT) {
-                const int pq = q < nE ? min(max(iperm[q], 0), nE - 1) : q;
                 const double2 x = ent[(size_t)e * KD + q];
-                w.cx[2 * (pq + (size_t)e * kd)] = x.x;
-                w.cx[2 * (pq + (size_t)e * kd) + 1] = x.y;
+                w.cx[2 * (q + (size_t)e * kd)] = x.x;
+                w.cx[2 * (q + (size_t)e * kd) + 1] = x.y;
             }
         for (int e = nent; e < w.m0; ++e)  // a short block is an error the host reports; keep it finite
             for (int q = tid; q < 2 * kd; q += TR_NT) w.cx[2 * (size_t)e * kd + q] = 0.0;
