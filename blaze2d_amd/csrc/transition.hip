@@ -245,7 +245,10 @@ __device__ __forceinline__ void track_assign(int m, const double2 *U, const doub
     // |U| <= 1: no scaling needed
     for (int e = lane; e < m * m; e += TR_NT) s.mag[e] = sqrt(U[e].x * U[e].x + U[e].y * U[e].y);
     __syncthreads();
-    // detect_degenerate_bands_from_rotation: the ambiguous rows
+    // detect_degenerate_bands_from_rotation: the ambiguous rows; each row's
+    // largest entry's column is kept for the shortcut below
+    int jmax = lane;
+    bool clear = true;  // this lane's row is not ambiguous
     if (m >= 2 && lane < m) {
         double r0 = -INFINITY, r1 = -INFINITY;
         for (int j = 0; j < m; ++j) {
@@ -253,13 +256,26 @@ __device__ __forceinline__ void track_assign(int m, const double2 *U, const doub
             if (x > r0) {
                 r1 = r0;
                 r0 = x;
+                jmax = j;
             } else if (x > r1) {
                 r1 = x;
             }
         }
         const double ratio = r0 > 1e-10 ? r1 / r0 : 0.0;
-        s.amb[lane] = (r0 < 0.7 || ratio > 0.5) ? 1 : 0;
+        clear = !(r0 < 0.7 || ratio > 0.5);
+        s.amb[lane] = clear ? 0 : 1;
+        s.ci[lane] = jmax;
     }
+    __syncthreads();
+    // When no row is ambiguous (each row's largest |U| above 0.7, the next at
+    // most half of it) and the rows' largest entries lie in distinct columns,
+    // that assignment is the unique maximum-weight one: any other permutation
+    // takes at most half of the maximum in every row where it differs.  The
+    // assignment problem and the ambiguous-block refinement are then skipped.
+    bool distinct = true;
+    if (m >= 2 && lane < m)
+        for (int i = 0; i < lane; ++i) distinct = distinct && s.ci[i] != jmax;
+    const bool shortcut = m >= 2 && !__any(!(lane >= m || (clear && distinct)));
     __syncthreads();
     // blk: [0] rb count, [1] db count, [2] mb count, then (start, size) pairs
     int *const cnt = s.blk;
@@ -304,7 +320,13 @@ __device__ __forceinline__ void track_assign(int m, const double2 *U, const doub
         cnt[0] = nrb;
         cnt[1] = ndb;
     }
-    hungarian_wave(m, s.mag, perm, s.hu);  // begins with a barrier
+    if (shortcut) {
+        if (lane < m) perm[lane] = jmax;
+        if (lane == 0) cnt[1] = 0;
+        __syncthreads();
+    } else {
+        hungarian_wave(m, s.mag, perm, s.hu);  // begins with a barrier
+    }
     // reassign a block of consecutive previous bands by frequency distance
     auto refine = [&](int st, int sz) {
         const int en = st + sz;
