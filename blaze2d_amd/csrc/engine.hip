@@ -1535,8 +1535,8 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
     auto assign_partials = [&](std::vector<GramWork<T>> &v, int nch) {
         for (size_t i = 0; i < v.size(); ++i) v[i].partial = d_part + i * (size_t)nch * LD * LD * 2;
     };
-    // the transition kernel sums the transition Gram's partials itself: few chunks
-    const int nch_dual = nch_for(g_dual.size()), nch_t1 = std::min(8, nch_for(g_t1.size()));
+    // the transition kernel sums the transition Gram's partials as it loads M
+    const int nch_dual = nch_for(g_dual.size()), nch_t1 = std::min(16, nch_for(g_t1.size()));
     assign_partials(g_t1, nch_t1);
     for (size_t i = 0; i < g_t1.size(); ++i) {  // the transition kernel sums the Gram's partials itself
         tw[i].gram = g_t1[i].partial;

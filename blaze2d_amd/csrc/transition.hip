@@ -437,9 +437,24 @@ __global__ void __launch_bounds__(TR_NT) transition_kernel(const TransWork *__re
         if (i < j) continue;
         const __attribute__((address_space(1))) double *g = gmem(w.gram) + 2 * ((size_t)i * w.ld + j);
         double gx = 0.0, gy = 0.0;
-        for (int c = 0; c < w.nchunks; ++c) {
-            gx += g[2 * (size_t)c * w.ld * w.ld];
-            gy += g[2 * (size_t)c * w.ld * w.ld + 1];
+        const size_t cs = 2 * (size_t)w.ld * w.ld;
+        int c = 0;
+        for (; c + 8 <= w.nchunks; c += 8) {  // eight partials in flight, summed in chunk order
+            double px[8], py[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                px[u] = g[(c + u) * cs];
+                py[u] = g[(c + u) * cs + 1];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                gx += px[u];
+                gy += py[u];
+            }
+        }
+        for (; c < w.nchunks; ++c) {
+            gx += g[c * cs];
+            gy += g[c * cs + 1];
         }
         if (i == j) gy = 0.0;
         Mp[i + j * KD] = make_double2(gx, gy);
