@@ -47,7 +47,7 @@ FP64_PEAK_TFLOPS = 78.6
 # slots 2048, groups 4, two engines), FETCH_SIZE x2 + WRITE_SIZE x1 from two rocprofv3 --pmc
 # passes scaled by a plain copy (profiles/rr_kernel_traffic.md); reported only
 # for that configuration.
-RR_TRAFFIC_BYTES = 14.34e6
+RR_TRAFFIC_BYTES = 4.14e6
 RR_TRAFFIC_CONFIG = (2048, 4, 2048, 2)  # slots, groups, geometries, engines the counters were read on
 TIMED_NAMES = {1: "op_kernel THETA_TM/TE (64x64 fused 2D FFT operator)",
                2: "gram2_kernel (dual Rayleigh-Ritz Gram Z^H BZ, Z^H AZ)",

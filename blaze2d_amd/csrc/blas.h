@@ -135,6 +135,10 @@ void launch_eigh(const EighWork *dwork, int nwork, int maxn, hipStream_t st);
 // maxn: largest n in the launch (selects the 36- or 48-column kernel)
 template <typename T>
 void launch_gram2(const Gram2Work<T> *dwork, int nwork, int nchunks, long rows, int maxn, hipStream_t st);
+// Sums the dual Gram's row-chunk partials (lower triangles) into chunk 0 of
+// each product, in chunk order from 0.0, so a reader loads one block.
+template <typename T>
+void launch_gram2_reduce(const Gram2Work<T> *dwork, int nwork, int nchunks, int maxn, hipStream_t st);
 // orders up to 48: one 512-thread workgroup per problem, all in LDS; above
 // 48 (up to 128): one 1024-thread workgroup per problem with the projected
 // matrix in LDS and T, the eigenvectors and G2 T in the item's workspace

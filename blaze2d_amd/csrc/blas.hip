@@ -1134,22 +1134,24 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) g
         __syncthreads();
         cnt = h;
     }
-    if (!active || g != 0) return;
-    double *p1 = w.partial1 + 2 * (size_t)blockIdx.x * w.ld * w.ld;
-    double *p2 = w.partial2 + 2 * (size_t)blockIdx.x * w.ld * w.ld;
+    if (active && g == 0) {
+        const size_t blk = 2 * (size_t)w.ld * w.ld;
+        double *p1 = w.partial1 + blockIdx.x * blk;
+        double *p2 = w.partial2 + blockIdx.x * blk;
 #pragma unroll
-    for (int u = 0; u < 3; ++u)
+        for (int u = 0; u < 3; ++u)
 #pragma unroll
-        for (int v = 0; v < 3; ++v) {
-            const int i = 3 * bi + u, j = 3 * bj + v;
-            if (i < n && j < n) {
-                const size_t o = 2 * ((size_t)i * w.ld + j);
-                gmem(p1)[o] = a1r[u][v];
-                gmem(p1)[o + 1] = a1i[u][v];
-                gmem(p2)[o] = a2r[u][v];
-                gmem(p2)[o + 1] = a2i[u][v];
+            for (int v = 0; v < 3; ++v) {
+                const int i = 3 * bi + u, j = 3 * bj + v;
+                if (i < n && j < n) {
+                    const size_t o = 2 * ((size_t)i * w.ld + j);
+                    gmem(p1)[o] = a1r[u][v];
+                    gmem(p1)[o + 1] = a1i[u][v];
+                    gmem(p2)[o] = a2r[u][v];
+                    gmem(p2)[o + 1] = a2i[u][v];
+                }
             }
-        }
+    }
 }
 
 template <typename T, int NBK>
@@ -1178,6 +1180,55 @@ void launch_gram2(const Gram2Work<T> *dwork, int nwork, int nchunks, long rows, 
 
 template void launch_gram2<double>(const Gram2Work<double> *, int, int, long, int, hipStream_t);
 template void launch_gram2<float>(const Gram2Work<float> *, int, int, long, int, hipStream_t);
+
+// One thread per lower-triangle entry of one product (x: entries of both
+// products of an item, y: items).  The chunk loads are issued eight at a time
+// and added in chunk order, so the sum is the one a reader adding the
+// partials one by one from 0.0 would form.
+template <typename T>
+__global__ void __launch_bounds__(256) gram2_reduce_kernel(const Gram2Work<T> *__restrict__ work, int nchunks) {
+    const Gram2Work<T> &w = work[blockIdx.y];
+    const int n = w.n;
+    const int ntri = n * (n + 1) / 2;
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= 2 * ntri) return;
+    const int prod = e >= ntri, t = e - prod * ntri;
+    int i = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);  // row of entry t of the row-order triangle
+    while (i * (i + 1) / 2 > t) --i;
+    while ((i + 1) * (i + 2) / 2 <= t) ++i;
+    const int j = t - i * (i + 1) / 2;
+    double *p = prod ? w.partial2 : w.partial1;
+    const size_t blk = 2 * (size_t)w.ld * w.ld, o = 2 * ((size_t)i * w.ld + j);
+    double re = 0.0, im = 0.0;
+    for (int c0 = 0; c0 < nchunks; c0 += 8) {
+        double2 v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (c0 + u < nchunks) {
+                const auto *q = gmem(p + (c0 + u) * blk + o);
+                v[u] = make_double2(q[0], q[1]);
+            }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (c0 + u < nchunks) {
+                re += v[u].x;
+                im += v[u].y;
+            }
+    }
+    gmem(p)[o] = re;
+    gmem(p)[o + 1] = im;
+}
+
+template <typename T>
+void launch_gram2_reduce(const Gram2Work<T> *dwork, int nwork, int nchunks, int maxn, hipStream_t st) {
+    if (nwork <= 0 || maxn <= 0 || nchunks <= 1) return;
+    const int nent = maxn * (maxn + 1);  // both products' lower triangles
+    hipLaunchKernelGGL((gram2_reduce_kernel<T>), dim3((nent + 255) / 256, nwork), dim3(256), 0, st, dwork, nchunks);
+    BH_HIP_CHECK(hipGetLastError());
+}
+
+template void launch_gram2_reduce<double>(const Gram2Work<double> *, int, int, int, hipStream_t);
+template void launch_gram2_reduce<float>(const Gram2Work<float> *, int, int, int, hipStream_t);
 
 // ---------------------------------------------------------------------------
 // Rayleigh-Ritz on the dual Gram, one 1024-thread workgroup per instance, FP64.

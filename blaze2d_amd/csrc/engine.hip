@@ -1547,7 +1547,8 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
         g_dual[i].partial2 = g_dual[i].partial1 + (size_t)nch_dual * LD * LD * 2;
         e_rr2[i].partial1 = g_dual[i].partial1;
         e_rr2[i].partial2 = g_dual[i].partial2;
-        e_rr2[i].nchunks = nch_dual;
+        // narrow: the dual Gram's partials are summed into chunk 0 before the Rayleigh-Ritz launch
+        e_rr2[i].nchunks = wide ? nch_dual : 1;
         if (wide) e_rr2[i].ws = G.rrws + (size_t)(rr_slot[i] - G.s0) * (rr_wide_bytes(C) / sizeof(double));
     }
     // wide subspaces: the two Rayleigh-Ritz Grams as tiled Gram products
@@ -1798,7 +1799,10 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
               [&] {
                   if (wide)
                       launch_gram<T>(d_g_wide, (int)g_wide.size(), nch_dual, N, max_p, max_p, st, eopt.gram_mfma_min);
-                  else launch_gram2<T>(d_g_dual, (int)g_dual.size(), nch_dual, N, std::max(max_p, 1), st);
+                  else {
+                      launch_gram2<T>(d_g_dual, (int)g_dual.size(), nch_dual, N, std::max(max_p, 1), st);
+                      launch_gram2_reduce<T>(d_g_dual, (int)g_dual.size(), nch_dual, max_p, st);
+                  }
               });
     }
     {
