@@ -208,12 +208,60 @@ __device__ inline void fft_pass(cplx<T> *buf, const cplx<T> *__restrict__ tw, co
     }
 }
 
-template <typename T, int N, int NLINES, bool ROWS, bool INV, int NF, int NT, int FSTRIDE, int RW, int S>
+// One radix pass like fft_pass, with the element loads and stores done by the
+// caller: ld(f, i) returns element i (flat row-major position) of field f and
+// st(f, i, v) stores it.  This folds a pointwise step into a pass that reads or
+// writes the data anyway.  Each element of a field is loaded and stored by one
+// thread, so ld and st may also touch other fields at position i, as long as
+// no thread of the same pass writes what they read.
+template <typename T, int N, int NLINES, bool ROWS, int S, bool INV, int NF, int NT, class LD, class ST>
+__device__ inline void fft_pass_fn(const cplx<T> *__restrict__ tw, LD ld, ST st) {
+    constexpr int R = Plan<N>::r(S);
+    constexpr int L = plan_len<N>(S);
+    constexpr int M = L / R;
+    constexpr int NBF = N / R;
+    constexpr int TOTAL = NBF * NLINES * NF;
+    constexpr int TWS = N / L;
+    constexpr int ES = ROWS ? 1 : NLINES;
+    constexpr int LS = ROWS ? N : 1;
+#pragma unroll 1
+    for (int id = threadIdx.x; id < TOTAL; id += NT) {
+        const int f = id / (NBF * NLINES);
+        const int rem = id - f * (NBF * NLINES);
+        int line, bf;
+        if (ROWS) { line = rem / NBF; bf = rem - line * NBF; }
+        else { bf = rem / NLINES; line = rem - bf * NLINES; }
+        const int b = bf / M, j2 = bf - b * M;
+        const int e0 = line * LS + (b * L + j2) * ES;
+        cplx<T> v[R];
+#pragma unroll
+        for (int j = 0; j < R; ++j) v[j] = ld(f, e0 + j * M * ES);
+        if (!INV) {
+            small_dft<T, R, false>(v);
+            if (M > 1) {
+#pragma unroll
+                for (int k1 = 1; k1 < R; ++k1) v[k1] = cmul(v[k1], tw[j2 * k1 * TWS]);
+            }
+        } else {
+            if (M > 1) {
+#pragma unroll
+                for (int k1 = 1; k1 < R; ++k1) v[k1] = cmulc(v[k1], tw[j2 * k1 * TWS]);
+            }
+            small_dft<T, R, true>(v);
+        }
+#pragma unroll
+        for (int j = 0; j < R; ++j) st(f, e0 + j * M * ES, v[j]);
+    }
+}
+
+// passes S .. SEND - 1 of a forward transform, a barrier after each
+template <typename T, int N, int NLINES, bool ROWS, bool INV, int NF, int NT, int FSTRIDE, int RW, int S,
+          int SEND = Plan<N>::K>
 __device__ inline void fft_passes_fwd(cplx<T> *buf, const cplx<T> *tw) {
-    if constexpr (S < Plan<N>::K) {
+    if constexpr (S < SEND) {
         fft_pass<T, N, NLINES, ROWS, S, INV, NF, NT, FSTRIDE, RW>(buf, tw);
         __syncthreads();
-        fft_passes_fwd<T, N, NLINES, ROWS, INV, NF, NT, FSTRIDE, RW, S + 1>(buf, tw);
+        fft_passes_fwd<T, N, NLINES, ROWS, INV, NF, NT, FSTRIDE, RW, S + 1, SEND>(buf, tw);
     }
 }
 template <typename T, int N, int NLINES, bool ROWS, bool INV, int NF, int NT, int FSTRIDE, int RW, int S,

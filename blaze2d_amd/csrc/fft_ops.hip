@@ -64,9 +64,9 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) op
     for (int i = threadIdx.x; i < NY; i += NT) tly[i] = {(T)gtwy[i].x, (T)gtwy[i].y};
     const cplx<T> *twx = tlx, *twy = tly;
     __syncthreads();
-    fft2d_forward_from<T, NX, NY, NT, TS>(F, in, twx, twy);
 
     if constexpr (MODE == MODE_THETA_TM || MODE == MODE_PREC_FD) {
+        fft2d_forward_from<T, NX, NY, NT, TS>(F, in, twx, twy);
 #pragma unroll 2
         for (int i = threadIdx.x; i < N; i += NT) {
             int py = i / NX, px = i - py * NX;
@@ -97,69 +97,102 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) op
         fft2d_inverse_to<T, NX, NY, NT, TS>(F, io, twx, twy);
     } else {
         cplx<T> *G2 = F + N;
-        // gradient step: THETA_TE  g = i(k+G) u_hat ; PREC_TP  g = -i(k+G) u_hat / (|k+G|^2 + s)
-#pragma unroll 2
-        for (int i = threadIdx.x; i < N; i += NT) {
-            int py = i / NX, px = i - py * NX;
-            double kx, ky, ksq;
-            bool cl;
-            kplusg<NX, NY>(P, px, py, kx, ky, ksq, cl);
-            cplx<T> u = F[lds_pos<NX>(i)];
-            if (MODE == MODE_THETA_TE) {
-                T skx = (T)kx * invN, sky = (T)ky * invN;
-                F[lds_pos<NX>(i)] = {-u.y * skx, u.x * skx};   // u * (0 + i kx)
-                G2[lds_pos<NX>(i)] = {-u.y * sky, u.x * sky};
-            } else {
-                double ik = 0.0;
-                if (!cl) { double den = ksq + P.shift; ik = den > 1e-15 ? 1.0 / den : 0.0; }
-                T fx = (T)(kx * ik) * invN, fy = (T)(ky * ik) * invN;
-                F[lds_pos<NX>(i)] = {u.y * fx, -u.x * fx};     // u * (0 - i kx ik)
-                G2[lds_pos<NX>(i)] = {u.y * fy, -u.x * fy};
-            }
+        constexpr int KX = Plan<NX>::K, KY = Plan<NY>::K;
+        auto at = [&](int f, int i) -> cplx<T> & { return F[f * N + lds_pos<NX>(i)]; };
+        // forward transform of the field; its last column pass applies the
+        // gradient step on the way out: THETA_TE g = i(k+G) u_hat, PREC_TP
+        // g = -i(k+G) u_hat / (|k+G|^2 + s)
+        {
+            PassIO io;
+            io.gin = in;
+            fft_pass<T, NX, NY, true, 0, false, 1, NT, N, NX, true, false, TS>(F, twx, io);
+            __syncthreads();
+            fft_passes_fwd<T, NX, NY, true, false, 1, NT, N, NX, 1>(F, twx);
+            fft_passes_fwd<T, NY, NX, false, false, 1, NT, N, NX, 0, KY - 1>(F, twy);
+            auto grad = [&](int, int i, cplx<T> u) {
+                const int py = i / NX, px = i - py * NX;
+                double kx, ky, ksq;
+                bool cl;
+                kplusg<NX, NY>(P, px, py, kx, ky, ksq, cl);
+                if (MODE == MODE_THETA_TE) {
+                    const T skx = (T)kx * invN, sky = (T)ky * invN;
+                    at(0, i) = {-u.y * skx, u.x * skx};  // u * (0 + i kx)
+                    at(1, i) = {-u.y * sky, u.x * sky};
+                } else {
+                    double ik = 0.0;
+                    if (!cl) { double den = ksq + P.shift; ik = den > 1e-15 ? 1.0 / den : 0.0; }
+                    const T fx = (T)(kx * ik) * invN, fy = (T)(ky * ik) * invN;
+                    at(0, i) = {u.y * fx, -u.x * fx};  // u * (0 - i kx ik)
+                    at(1, i) = {u.y * fy, -u.x * fy};
+                }
+            };
+            fft_pass_fn<T, NY, NX, false, KY - 1, false, 1, NT>(twy, at, grad);
+            __syncthreads();
         }
-        __syncthreads();
-        fft2d_inverse<T, NX, NY, 2, NT>(F, twx, twy);
-        // real-space step: THETA_TE eta tensor [xx, xy, yx, yy]; PREC_TP eps
+        // inverse transform of both gradient components, then the real-space
+        // step: THETA_TE eta tensor [xx, xy, yx, yy] (mixes the two fields, so
+        // it runs as its own step); PREC_TP eps, applied by the last row pass
+        fft_passes_inv<T, NY, NX, false, true, 2, NT, N, NX, KY - 1>(F, twy);
+        if constexpr (MODE == MODE_THETA_TE) {
+            fft_passes_inv<T, NX, NY, true, true, 2, NT, N, NX, KX - 1>(F, twx);
 #pragma unroll 2
-        for (int i = threadIdx.x; i < N; i += NT) {
-            cplx<T> gx = F[lds_pos<NX>(i)], gy = G2[lds_pos<NX>(i)];
-            if (MODE == MODE_THETA_TE) {
+            for (int i = threadIdx.x; i < N; i += NT) {
+                cplx<T> gx = F[lds_pos<NX>(i)], gy = G2[lds_pos<NX>(i)];
                 const auto *t = gmem(w.tensor + 4 * (size_t)i);
                 T t0 = (T)t[0], t1 = (T)t[1], t2 = (T)t[2], t3 = (T)t[3];
                 F[lds_pos<NX>(i)] = {gx.x * t0 + gy.x * t1, gx.y * t0 + gy.y * t1};
                 G2[lds_pos<NX>(i)] = {gx.x * t2 + gy.x * t3, gx.y * t2 + gy.y * t3};
-            } else {
-                T e = (T)gmem(w.eps)[i];
-                F[lds_pos<NX>(i)] = {gx.x * e, gx.y * e};
-                G2[lds_pos<NX>(i)] = {gy.x * e, gy.y * e};
             }
+        } else {
+            fft_passes_inv<T, NX, NY, true, true, 2, NT, N, NX, KX - 1, 1>(F, twx);
+            auto scale_eps = [&](int f, int i, cplx<T> g) {
+                const T e = (T)gmem(w.eps)[i];
+                at(f, i) = {g.x * e, g.y * e};
+            };
+            fft_pass_fn<T, NX, NY, true, 0, true, 2, NT>(twx, at, scale_eps);
         }
         __syncthreads();
         fft2d_forward<T, NX, NY, 2, NT>(F, twx, twy);
-        // divergence: out_hat = -(i kx gx + i ky gy) [ * 1/(|k+G|^2+s) for TP ]
+        // inverse transform of the divergence; its first column pass forms
+        // out_hat = -(i kx gx + i ky gy) [ * 1/(|k+G|^2+s) for TP ] as it loads
+        // (it writes only field 0 at the positions it reads)
+        {
+            auto div = [&](int, int i) -> cplx<T> {
+                const int py = i / NX, px = i - py * NX;
+                double kx, ky, ksq;
+                bool cl;
+                kplusg<NX, NY>(P, px, py, kx, ky, ksq, cl);
+                double sc = 1.0;
+                if (MODE == MODE_PREC_TP) {
+                    double ik = 0.0;
+                    if (!cl) { double den = ksq + P.shift; ik = den > 1e-15 ? 1.0 / den : 0.0; }
+                    sc = ik;
+                }
+                const T skx = (T)kx, sky = (T)ky, s = (T)sc * invN;
+                const cplx<T> gx = at(0, i), gy = at(1, i);
+                // div = i kx gx + i ky gy ; out = -div
+                const T dre = -(skx * gx.y + sky * gy.y);
+                const T dim = skx * gx.x + sky * gy.x;
+                return cplx<T>{-dre * s, -dim * s};
+            };
+            auto put = [&](int, int i, cplx<T> v) { at(0, i) = v; };
+            if constexpr (MODE == MODE_THETA_TE) {
+                fft_pass_fn<T, NY, NX, false, KY - 1, true, 1, NT>(twy, div, put);
+            } else {
+                // TP: the division would spill registers inside the pass; the
+                // divergence runs as its own step before the first pass
 #pragma unroll 2
-        for (int i = threadIdx.x; i < N; i += NT) {
-            int py = i / NX, px = i - py * NX;
-            double kx, ky, ksq;
-            bool cl;
-            kplusg<NX, NY>(P, px, py, kx, ky, ksq, cl);
-            double sc = 1.0;
-            if (MODE == MODE_PREC_TP) {
-                double ik = 0.0;
-                if (!cl) { double den = ksq + P.shift; ik = den > 1e-15 ? 1.0 / den : 0.0; }
-                sc = ik;
+                for (int i = threadIdx.x; i < N; i += NT) at(0, i) = div(0, i);
+                __syncthreads();
+                fft_pass<T, NY, NX, false, KY - 1, true, 1, NT, N, NX>(F, twy);
             }
-            T skx = (T)kx, sky = (T)ky, s = (T)sc * invN;
-            cplx<T> gx = F[lds_pos<NX>(i)], gy = G2[lds_pos<NX>(i)];
-            // div = i kx gx + i ky gy ; out = -div
-            T dre = -(skx * gx.y + sky * gy.y);
-            T dim = skx * gx.x + sky * gy.x;
-            F[lds_pos<NX>(i)] = {-dre * s, -dim * s};
+            __syncthreads();
+            fft_passes_inv<T, NY, NX, false, true, 1, NT, N, NX, KY - 2>(F, twy);
+            fft_passes_inv<T, NX, NY, true, true, 1, NT, N, NX, KX - 1, 1>(F, twx);
+            PassIO io;
+            io.gout = out;
+            fft_pass<T, NX, NY, true, 0, true, 1, NT, N, NX, false, true, TS>(F, twx, io);
         }
-        __syncthreads();
-        PassIO io;
-        io.gout = out;
-        fft2d_inverse_to<T, NX, NY, NT, TS>(F, io, twx, twy);
     }
 }
 
