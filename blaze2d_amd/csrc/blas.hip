@@ -1300,21 +1300,21 @@ __global__ void __launch_bounds__(NT) rr_kernel(const RRWork *__restrict__ work)
     __shared__ double red[1];
     const int ld = w.ld;
     auto load = [&](const double *partial) {  // Hermitian n x n from lower partial sums, zero padding
+        // each lower entry (i >= j) is loaded once and also written mirrored
         for (int e = tid; e < ne * ne; e += NT) {
             const int i = e % ne, j = e / ne;
-            double2 v = make_double2(0.0, 0.0);
-            if (i < n && j < n) {
-                const int a = i >= j ? i : j, b = i >= j ? j : i;
-                double re = 0.0, im = 0.0;
+            if (i < j) continue;
+            double re = 0.0, im = 0.0;
+            if (i < n) {
                 for (int c = 0; c < w.nchunks; ++c) {
-                    const auto *p = gmem(partial + 2 * ((size_t)c * ld * ld + (size_t)a * ld + b));
+                    const auto *p = gmem(partial + 2 * ((size_t)c * ld * ld + (size_t)i * ld + j));
                     re += p[0];
                     im += p[1];
                 }
                 if (i == j) im = 0.0;
-                v = i >= j ? make_double2(re, im) : make_double2(re, -im);
             }
-            A[i + j * lda] = v;
+            A[i + j * lda] = make_double2(re, im);
+            if (i != j) A[j + i * lda] = make_double2(re, i < n ? -im : 0.0);  // padding stays +0
         }
     };
     // 1. scaled G1
