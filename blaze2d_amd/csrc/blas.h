@@ -139,6 +139,9 @@ void launch_gram2(const Gram2Work<T> *dwork, int nwork, int nchunks, long rows, 
 // each product, in chunk order from 0.0, so a reader loads one block.
 template <typename T>
 void launch_gram2_reduce(const Gram2Work<T> *dwork, int nwork, int nchunks, int maxn, hipStream_t st);
+// The same for a Hermitian Gram of launch_gram (na = nb, lower triangle).
+template <typename T>
+void launch_gram_lower_reduce(const GramWork<T> *dwork, int nwork, int nchunks, int maxn, hipStream_t st);
 // orders up to 48: one 512-thread workgroup per problem, all in LDS; above
 // 48 (up to 128): one 1024-thread workgroup per problem with the projected
 // matrix in LDS and T, the eigenvectors and G2 T in the item's workspace

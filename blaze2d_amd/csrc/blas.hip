@@ -1228,6 +1228,51 @@ void launch_gram2_reduce(const Gram2Work<T> *dwork, int nwork, int nchunks, int 
 }
 
 template void launch_gram2_reduce<double>(const Gram2Work<double> *, int, int, int, hipStream_t);
+
+// One thread per lower-triangle entry (x) of one item (y), summed as above.
+template <typename T>
+__global__ void __launch_bounds__(256) gram_lower_reduce_kernel(const GramWork<T> *__restrict__ work, int nchunks) {
+    const GramWork<T> &w = work[blockIdx.y];
+    const int n = w.dev_na ? *w.dev_na : w.na;
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= n * (n + 1) / 2) return;
+    int i = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);  // row of entry t of the row-order triangle
+    while (i * (i + 1) / 2 > t) --i;
+    while ((i + 1) * (i + 2) / 2 <= t) ++i;
+    const int j = t - i * (i + 1) / 2;
+    double *p = w.partial;
+    const size_t blk = 2 * (size_t)w.ld * w.ld, o = 2 * ((size_t)i * w.ld + j);
+    double re = 0.0, im = 0.0;
+    for (int c0 = 0; c0 < nchunks; c0 += 8) {
+        double2 v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (c0 + u < nchunks) {
+                const auto *q = gmem(p + (c0 + u) * blk + o);
+                v[u] = make_double2(q[0], q[1]);
+            }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (c0 + u < nchunks) {
+                re += v[u].x;
+                im += v[u].y;
+            }
+    }
+    gmem(p)[o] = re;
+    gmem(p)[o + 1] = im;
+}
+
+template <typename T>
+void launch_gram_lower_reduce(const GramWork<T> *dwork, int nwork, int nchunks, int maxn, hipStream_t st) {
+    if (nwork <= 0 || maxn <= 0 || nchunks <= 1) return;
+    const int nent = maxn * (maxn + 1) / 2;
+    hipLaunchKernelGGL((gram_lower_reduce_kernel<T>), dim3((nent + 255) / 256, nwork), dim3(256), 0, st, dwork,
+                       nchunks);
+    BH_HIP_CHECK(hipGetLastError());
+}
+
+template void launch_gram_lower_reduce<double>(const GramWork<double> *, int, int, int, hipStream_t);
+template void launch_gram_lower_reduce<float>(const GramWork<float> *, int, int, int, hipStream_t);
 template void launch_gram2_reduce<float>(const Gram2Work<float> *, int, int, int, hipStream_t);
 
 // ---------------------------------------------------------------------------

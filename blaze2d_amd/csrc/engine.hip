@@ -1535,12 +1535,12 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
     auto assign_partials = [&](std::vector<GramWork<T>> &v, int nch) {
         for (size_t i = 0; i < v.size(); ++i) v[i].partial = d_part + i * (size_t)nch * LD * LD * 2;
     };
-    // the transition kernel sums the transition Gram's partials as it loads M
-    const int nch_dual = nch_for(g_dual.size()), nch_t1 = std::min(16, nch_for(g_t1.size()));
+    // the transition Gram's partials are summed into chunk 0 before the transition kernel loads M
+    const int nch_dual = nch_for(g_dual.size()), nch_t1 = nch_for(g_t1.size());
     assign_partials(g_t1, nch_t1);
-    for (size_t i = 0; i < g_t1.size(); ++i) {  // the transition kernel sums the Gram's partials itself
+    for (size_t i = 0; i < g_t1.size(); ++i) {
         tw[i].gram = g_t1[i].partial;
-        tw[i].nchunks = nch_t1;
+        tw[i].nchunks = 1;
     }
     for (size_t i = 0; i < g_dual.size(); ++i) {  // two products per item
         g_dual[i].partial1 = d_part + 2 * i * (size_t)nch_dual * LD * LD * 2;
@@ -1728,6 +1728,7 @@ void Engine<T>::execute(StepPlan &plan, Group &G) {
     // history copy (reads E), X = K cx (overwrites E's copy)
     if (!g_t1.empty()) {
         launch_gram<T>(d_g_t1, (int)g_t1.size(), nch_t1, N, maxkd, maxkd, st, eopt.gram_mfma_min);
+        launch_gram_lower_reduce<T>(d_g_t1, (int)g_t1.size(), nch_t1, maxkd, st);
         // the workspace is laid out for this launch's largest item
         double bytes = 0;  // the Gram in, block and history coefficients out
         int KDl = 1, NBWl = 1, M0l = 1, MMl = 1, NCl = 1;
